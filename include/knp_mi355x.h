@@ -1,0 +1,238 @@
+/*
+ * knp_mi355x.h - C ABI of the MI355X (gfx950) backend for the KNP spiking-network step.
+ *
+ * This is the drop-in boundary between a KNP-style C++ backend object (knp::core::Backend,
+ * reference knp/core-library/include/knp/core/backend.h:55-457) and the HIP kernels that run the
+ * per-step update of the reference's single-threaded CPU backend
+ * (knp/backends/cpu/cpu-single-threaded-backend/impl/backend.cpp:89-141).
+ *
+ * Conventions
+ *   - Opaque handle, plain pointers and sizes, no C++ or torch types.
+ *   - Every function returns 0 (KNP_OK) on success or a negative KNP_ERR_* code; the message of the
+ *     last failure on the calling thread is returned by knp_mi355x_last_error().
+ *   - Host buffers passed in are copied before the call returns; the caller keeps ownership.
+ *   - One host thread drives one handle at a time; handles are independent.
+ *
+ * Every entry point below names the reference interface it replaces.
+ */
+#ifndef KNP_MI355X_H
+#define KNP_MI355X_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- error codes ------------------------------------------------------------------------------ */
+#define KNP_OK 0
+#define KNP_ERR_INVALID_ARGUMENT (-1) /* bad index, size or parameter */
+#define KNP_ERR_UNSUPPORTED (-2)      /* "Type is not supported." (knp/meta-library variant_helpers.h:67) */
+#define KNP_ERR_HIP (-3)              /* a HIP runtime call failed */
+#define KNP_ERR_OUT_OF_MEMORY (-4)    /* device or host allocation failed, or the allocation cap was hit */
+#define KNP_ERR_STATE (-5)            /* call not allowed in the current state (e.g. load after start) */
+#define KNP_ERR_RANGE (-6)            /* exact accumulator or buffer capacity exceeded */
+
+#define KNP_NO_POPULATION 0xFFFFFFFFu /* presynaptic side of a channel-fed projection (null UID) */
+
+/* OutputType, knp/synapse-traits-library/include/knp/synapse-traits/output_types.h:36-62 */
+enum knp_output_type {
+    KNP_EXCITATORY = 0,
+    KNP_INHIBITORY_CURRENT = 1,
+    KNP_INHIBITORY_CONDUCTANCE = 2,
+    KNP_DOPAMINE = 3,
+    KNP_BLOCKING = 4
+};
+
+/* ISIPeriodType, knp/neuron-traits-library/include/knp/neuron-traits/stdp_synaptic_resource_rule.h */
+enum knp_isi_status {
+    KNP_ISI_IS_FORCED = 0,
+    KNP_ISI_PERIOD_STARTED = 1,
+    KNP_ISI_PERIOD_CONTINUED = 2,
+    KNP_ISI_NOT_IN_PERIOD = 3
+};
+
+/* Projection::SharedSynapseParameters::ProcessingType, knp/core-library/include/knp/core/projection.h:139-158 */
+enum knp_stdp_processing { KNP_STDP_ONLY = 0, KNP_STDP_AND_SPIKE = 1 };
+
+/* neuron_parameters<BLIFATNeuron>, knp/neuron-traits-library/include/knp/neuron-traits/blifat.h:207-343.
+ * Same fields, types and meaning; knp_blifat_defaults() returns default_values<BLIFATNeuron>. */
+typedef struct knp_blifat_neuron {
+    uint64_t n_time_steps_since_last_firing;
+    double activation_threshold;
+    double additional_threshold;
+    double dynamic_threshold;
+    double threshold_decay;
+    double threshold_increment;
+    double postsynaptic_trace;
+    double postsynaptic_trace_decay;
+    double postsynaptic_trace_increment;
+    double inhibitory_conductance;
+    double inhibitory_conductance_decay;
+    double potential;
+    double pre_impact_potential;
+    double potential_decay;
+    uint32_t bursting_phase;
+    uint32_t bursting_period;
+    double reflexive_weight;
+    double reversal_inhibitory_potential;
+    uint32_t absolute_refractory_period;
+    double potential_reset_value;
+    double min_potential;
+    int64_t total_blocking_period;
+    double dopamine_value;
+    double stochastic_stimulation;
+    uint32_t random_number_generator_state;
+} knp_blifat_neuron;
+
+/* neuron_parameters<SynapticResourceSTDPNeuron<BLIFATNeuron>>,
+ * knp/neuron-traits-library/include/knp/neuron-traits/stdp_synaptic_resource_rule.h:91-192 */
+typedef struct knp_resource_blifat_neuron {
+    knp_blifat_neuron base;
+    float synapse_sum_threshold_coefficient;
+    uint32_t dopamine_plasticity_time;
+    float free_synaptic_resource;
+    float synaptic_resource_threshold;
+    uint32_t resource_drain_coefficient;
+    float stability;
+    float stability_change_parameter;
+    float stability_change_at_isi;
+    uint32_t isi_max;
+    float d_h;
+    int32_t isi_status; /* enum knp_isi_status */
+    uint64_t last_step;
+    uint64_t last_spike_step;
+    uint64_t first_isi_spike;
+    uint8_t is_being_forced;
+} knp_resource_blifat_neuron;
+
+/* STDPSynapticResourceRule<DeltaSynapse>,
+ * knp/synapse-traits-library/include/knp/synapse-traits/stdp_synaptic_resource_rule.h:43-100 */
+typedef struct knp_resource_rule {
+    float synaptic_resource;
+    float w_min;
+    float w_max;
+    float d_u;
+    uint32_t dopamine_plasticity_period;
+    uint64_t last_spike_step;
+    uint8_t had_hebbian_update;
+    uint8_t has_contributed;
+} knp_resource_rule;
+
+/* Defaults of the reference trait structs. */
+void knp_blifat_defaults(knp_blifat_neuron *out);
+void knp_resource_blifat_defaults(knp_resource_blifat_neuron *out);
+void knp_resource_rule_defaults(knp_resource_rule *out);
+
+/* ---- library / devices ------------------------------------------------------------------------ */
+typedef struct knp_net knp_net;
+
+const char *knp_mi355x_version(void);
+/* Message of the last failed call on this thread ("" if none). */
+const char *knp_mi355x_last_error(void);
+/* Backend::get_devices(), knp/core-library/include/knp/core/backend.h:172 (HIP device list). */
+int knp_mi355x_device_count(int *count);
+int knp_mi355x_device_name(int device, char *buf, size_t cap);
+
+/* Backend construction: SingleThreadedCPUBackend::create(),
+ * knp/backends/cpu/cpu-single-threaded-backend/impl/backend.cpp:48-52. */
+int knp_mi355x_create(int device, knp_net **out);
+int knp_mi355x_destroy(knp_net *net);
+/* Caps the device memory the handle may allocate (0 = no cap); exceeding it gives KNP_ERR_OUT_OF_MEMORY. */
+int knp_mi355x_set_memory_cap(knp_net *net, uint64_t bytes);
+
+/* ---- loading (Backend::load_all_populations / load_all_projections, backend.h:137,146;
+ *      SingleThreadedCPUBackend::load_populations / load_projections, backend.cpp:144-170) ---------- */
+/* Populations are numbered in load order (the order the reference updates them in). */
+int knp_mi355x_add_blifat_population(knp_net *net, const knp_blifat_neuron *neurons, uint32_t n, uint32_t *out_pop);
+int knp_mi355x_add_resource_blifat_population(
+    knp_net *net, const knp_resource_blifat_neuron *neurons, uint32_t n, uint32_t *out_pop);
+
+/* Projection<DeltaSynapse>. Synapse i of the reference container is (pre[i], post[i], weight[i], delay[i],
+ * output_type[i]); output_type may be NULL (all KNP_EXCITATORY). pre_pop is a population index or
+ * KNP_NO_POPULATION for a projection fed only by channels (knp_mi355x_push_spikes / sources); n_pre is the
+ * size of the presynaptic index range. Projections are numbered in load order. */
+int knp_mi355x_add_delta_projection(
+    knp_net *net, uint32_t pre_pop, uint32_t n_pre, uint32_t post_pop, const uint32_t *pre, const uint32_t *post,
+    const float *weight, const uint32_t *delay, const uint8_t *output_type, uint64_t n_syn, uint32_t *out_proj);
+
+/* Projection<SynapticResourceSTDPDeltaSynapse>: as above plus one rule per synapse. Locked by default
+ * (knp/core-library/include/knp/core/projection.h:432); see knp_mi355x_set_locked. */
+int knp_mi355x_add_resource_stdp_projection(
+    knp_net *net, uint32_t pre_pop, uint32_t n_pre, uint32_t post_pop, const uint32_t *pre, const uint32_t *post,
+    const float *weight, const uint32_t *delay, const uint8_t *output_type, const knp_resource_rule *rules,
+    uint64_t n_syn, uint32_t *out_proj);
+
+/* Projection<AdditiveSTDPDeltaSynapse> with rule {tau_plus, tau_minus} per synapse and the shared
+ * stdp_populations_ map (population index -> enum knp_stdp_processing). */
+int knp_mi355x_add_additive_stdp_projection(
+    knp_net *net, uint32_t pre_pop, uint32_t n_pre, uint32_t post_pop, const uint32_t *pre, const uint32_t *post,
+    const float *weight, const uint32_t *delay, const uint8_t *output_type, const float *tau_plus,
+    const float *tau_minus, uint64_t n_syn, const uint32_t *stdp_pops, const uint8_t *stdp_modes,
+    uint32_t n_stdp_pops, uint32_t *out_proj);
+
+/* Projection::lock_weights / unlock_weights (projection.h:415-426). */
+int knp_mi355x_set_locked(knp_net *net, uint32_t proj, int locked);
+/* Backend::start_learning / stop_learning (SingleThreadedCPUBackend, backend.h:324-332): unlock/lock all. */
+int knp_mi355x_start_learning(knp_net *net);
+int knp_mi355x_stop_learning(knp_net *net);
+
+/* Backend::_init (backend.cpp:207-214 + cpu-library init.h:93-110): uploads the network and builds the
+ * device routing. Called implicitly by the first step; loading afterwards is KNP_ERR_STATE. */
+int knp_mi355x_init(knp_net *net);
+
+/* ---- inputs (InputChannel::send -> SpikeMessage to a projection, base-framework io/input_channel.h:99-118) */
+/* One SpikeMessage with these presynaptic indexes, delivered to projection `proj` in the next step. */
+int knp_mi355x_push_spikes(knp_net *net, uint32_t proj, const uint32_t *idx, uint32_t n);
+/* A device-side Bernoulli channel group feeding `proj` every step: channel c spikes at step t iff
+ * knp_bernoulli_hash(seed, t, c) < thresholds[pattern * n_channels + c], where
+ * pattern = (t / steps_per_pattern) % n_patterns, and only when lo <= t % steps_per_pattern <= hi. */
+int knp_mi355x_add_bernoulli_source(
+    knp_net *net, uint32_t proj, uint32_t n_channels, const uint32_t *thresholds, uint32_t n_patterns,
+    uint32_t steps_per_pattern, uint32_t active_lo, uint32_t active_hi, uint64_t seed);
+
+/* ---- stepping (Backend::_step, backend.cpp:89-141) ------------------------------------------------ */
+int knp_mi355x_step(knp_net *net, uint64_t n_steps);
+/* Backend::get_step (backend.h:290). */
+int knp_mi355x_get_step(knp_net *net, uint64_t *step);
+int knp_mi355x_synchronize(knp_net *net);
+
+/* ---- outputs ------------------------------------------------------------------------------------- */
+/* Observe a population: its spikes of every following step are kept on device as bitmaps (ceil(n/32)
+ * words per step, bit i = neuron i spiked) for up to capacity_steps steps. */
+int knp_mi355x_record_spikes(knp_net *net, uint32_t pop, uint64_t capacity_steps);
+/* Copies the recorded bitmaps (oldest first) and the number of recorded steps, then clears the record. */
+int knp_mi355x_read_spike_record(knp_net *net, uint32_t pop, uint32_t *words, uint64_t n_words, uint64_t *n_steps);
+/* SpikeMessage of the last step for `pop`: ascending neuron indexes (populations_old.h:110-118). */
+int knp_mi355x_read_last_spikes(knp_net *net, uint32_t pop, uint32_t *idx, uint32_t cap, uint32_t *n);
+/* get_network_data (backend.h:420; ST get_network.cpp:28-93): state of every neuron / synapse,
+ * in load order and synapse-index order. */
+int knp_mi355x_read_potentials(knp_net *net, uint32_t pop, float *out, uint32_t n);
+int knp_mi355x_read_blifat_neurons(knp_net *net, uint32_t pop, knp_blifat_neuron *out, uint32_t n);
+int knp_mi355x_read_resource_neurons(knp_net *net, uint32_t pop, knp_resource_blifat_neuron *out, uint32_t n);
+int knp_mi355x_read_weights(knp_net *net, uint32_t proj, float *out, uint64_t n_syn);
+int knp_mi355x_read_resource_rules(knp_net *net, uint32_t proj, knp_resource_rule *out, uint64_t n_syn);
+/* Number of SynapticImpacts created so far (sum over steps of spikes x out-degree over all projections). */
+int knp_mi355x_get_event_count(knp_net *net, uint64_t *events);
+/* Fixed-point fraction bits of the exact impact accumulator of a population (chosen at init). */
+int knp_mi355x_get_accumulator_bits(knp_net *net, uint32_t pop, int32_t *frac_bits);
+
+/* ---- kernel timing (HIP events on the library's own stream) -------------------------------------- */
+enum knp_kernel_id {
+    KNP_KERNEL_NEURON = 0, /* fused population step (+ spike compaction) */
+    KNP_KERNEL_SCATTER = 1, /* spike -> impact scatter into the delay ring */
+    KNP_KERNEL_TRAIN = 2,   /* synaptic-resource STDP */
+    KNP_KERNEL_COUNT = 3
+};
+int knp_mi355x_enable_timing(knp_net *net, int enable);
+/* Average duration (ms) and launch count of a kernel class since timing was enabled. */
+int knp_mi355x_kernel_time(knp_net *net, int kernel_id, double *avg_ms, uint64_t *launches);
+
+/* ---- shared helper: the counter-based hash behind Bernoulli sources (identical on host and device) */
+uint32_t knp_bernoulli_hash(uint64_t seed, uint64_t step, uint32_t channel);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* KNP_MI355X_H */

@@ -1,0 +1,195 @@
+"""ctypes mirror of include/knp_mi355x.h (the C ABI of the MI355X backend).
+
+The structs follow the reference trait structs field by field:
+  knp_blifat_neuron           neuron_parameters<BLIFATNeuron>      knp/neuron-traits-library/include/knp/neuron-traits/blifat.h:207-343
+  knp_resource_blifat_neuron  neuron_parameters<SynapticResourceSTDPNeuron<BLIFATNeuron>>
+                                                                   knp/neuron-traits-library/include/knp/neuron-traits/stdp_synaptic_resource_rule.h:91-192
+  knp_resource_rule           STDPSynapticResourceRule<DeltaSynapse>
+                                                                   knp/synapse-traits-library/include/knp/synapse-traits/stdp_synaptic_resource_rule.h:43-100
+"""
+import ctypes as C
+
+import numpy as np
+
+KNP_OK = 0
+KNP_NO_POPULATION = 0xFFFFFFFF
+
+# OutputType, knp/synapse-traits-library/include/knp/synapse-traits/output_types.h:36-62
+EXCITATORY = 0
+INHIBITORY_CURRENT = 1
+INHIBITORY_CONDUCTANCE = 2
+DOPAMINE = 3
+BLOCKING = 4
+
+# ISIPeriodType
+ISI_IS_FORCED = 0
+ISI_PERIOD_STARTED = 1
+ISI_PERIOD_CONTINUED = 2
+ISI_NOT_IN_PERIOD = 3
+
+# ProcessingType of additive STDP projections
+STDP_ONLY = 0
+STDP_AND_SPIKE = 1
+
+KERNEL_NEURON = 0
+KERNEL_SCATTER = 1
+KERNEL_TRAIN = 2
+
+SIZE_MAX = 0xFFFFFFFFFFFFFFFF
+INT64_MAX = 0x7FFFFFFFFFFFFFFF
+FLT_MAX = float(np.finfo(np.float32).max)
+
+
+class BLIFATNeuron(C.Structure):
+    _fields_ = [
+        ("n_time_steps_since_last_firing", C.c_uint64),
+        ("activation_threshold", C.c_double),
+        ("additional_threshold", C.c_double),
+        ("dynamic_threshold", C.c_double),
+        ("threshold_decay", C.c_double),
+        ("threshold_increment", C.c_double),
+        ("postsynaptic_trace", C.c_double),
+        ("postsynaptic_trace_decay", C.c_double),
+        ("postsynaptic_trace_increment", C.c_double),
+        ("inhibitory_conductance", C.c_double),
+        ("inhibitory_conductance_decay", C.c_double),
+        ("potential", C.c_double),
+        ("pre_impact_potential", C.c_double),
+        ("potential_decay", C.c_double),
+        ("bursting_phase", C.c_uint32),
+        ("bursting_period", C.c_uint32),
+        ("reflexive_weight", C.c_double),
+        ("reversal_inhibitory_potential", C.c_double),
+        ("absolute_refractory_period", C.c_uint32),
+        ("potential_reset_value", C.c_double),
+        ("min_potential", C.c_double),
+        ("total_blocking_period", C.c_int64),
+        ("dopamine_value", C.c_double),
+        ("stochastic_stimulation", C.c_double),
+        ("random_number_generator_state", C.c_uint32),
+    ]
+
+
+class ResourceBLIFATNeuron(C.Structure):
+    _fields_ = [
+        ("base", BLIFATNeuron),
+        ("synapse_sum_threshold_coefficient", C.c_float),
+        ("dopamine_plasticity_time", C.c_uint32),
+        ("free_synaptic_resource", C.c_float),
+        ("synaptic_resource_threshold", C.c_float),
+        ("resource_drain_coefficient", C.c_uint32),
+        ("stability", C.c_float),
+        ("stability_change_parameter", C.c_float),
+        ("stability_change_at_isi", C.c_float),
+        ("isi_max", C.c_uint32),
+        ("d_h", C.c_float),
+        ("isi_status", C.c_int32),
+        ("last_step", C.c_uint64),
+        ("last_spike_step", C.c_uint64),
+        ("first_isi_spike", C.c_uint64),
+        ("is_being_forced", C.c_uint8),
+    ]
+
+
+class ResourceRule(C.Structure):
+    _fields_ = [
+        ("synaptic_resource", C.c_float),
+        ("w_min", C.c_float),
+        ("w_max", C.c_float),
+        ("d_u", C.c_float),
+        ("dopamine_plasticity_period", C.c_uint32),
+        ("last_spike_step", C.c_uint64),
+        ("had_hebbian_update", C.c_uint8),
+        ("has_contributed", C.c_uint8),
+    ]
+
+
+def blifat_defaults(**overrides):
+    """default_values<BLIFATNeuron> (blifat.h:47-199) as a BLIFATNeuron struct."""
+    n = BLIFATNeuron()
+    n.n_time_steps_since_last_firing = SIZE_MAX
+    n.activation_threshold = 1.0
+    n.reversal_inhibitory_potential = -0.3
+    n.min_potential = -1.0e9
+    n.total_blocking_period = INT64_MAX
+    for k, v in overrides.items():
+        setattr(n, k, v)
+    return n
+
+
+def resource_blifat_defaults(base=None, **overrides):
+    """neuron_parameters<SynapticResourceSTDPNeuron<BLIFATNeuron>> defaults
+    (stdp_synaptic_resource_rule.h:91-192)."""
+    r = ResourceBLIFATNeuron()
+    r.base = base if base is not None else blifat_defaults()
+    r.synapse_sum_threshold_coefficient = 0.0
+    r.dopamine_plasticity_time = 1
+    r.free_synaptic_resource = 1.0
+    r.synaptic_resource_threshold = FLT_MAX
+    r.resource_drain_coefficient = 0
+    r.stability = 0.0
+    r.stability_change_parameter = 0.0
+    r.stability_change_at_isi = 0.0
+    r.isi_max = 1
+    r.d_h = 1.0
+    r.isi_status = ISI_NOT_IN_PERIOD
+    r.last_step = 0
+    r.last_spike_step = 0
+    r.first_isi_spike = 0
+    r.is_being_forced = 0
+    for k, v in overrides.items():
+        setattr(r, k, v)
+    return r
+
+
+def resource_rule_defaults(**overrides):
+    """STDPSynapticResourceRule defaults (synapse-traits stdp_synaptic_resource_rule.h:43-100)."""
+    r = ResourceRule()
+    r.synaptic_resource = 0.0
+    r.w_min = 0.0
+    r.w_max = 1.0
+    r.d_u = 0.0
+    r.dopamine_plasticity_period = 0
+    r.last_spike_step = 0
+    r.had_hebbian_update = 0
+    r.has_contributed = 0
+    for k, v in overrides.items():
+        setattr(r, k, v)
+    return r
+
+
+def neuron_array(proto, n):
+    """A ctypes array of n copies of `proto` (a BLIFATNeuron or ResourceBLIFATNeuron)."""
+    arr = (type(proto) * n)()
+    raw = bytes(proto)
+    buf = (C.c_char * (C.sizeof(proto) * n)).from_buffer(arr)
+    buf[:] = raw * n
+    return arr
+
+
+def rule_array(proto, n):
+    return neuron_array(proto, n)
+
+
+def u32p(a):
+    return a.ctypes.data_as(C.POINTER(C.c_uint32)) if a is not None else None
+
+
+def f32p(a):
+    return a.ctypes.data_as(C.POINTER(C.c_float)) if a is not None else None
+
+
+def f64p(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double)) if a is not None else None
+
+
+def u8p(a):
+    return a.ctypes.data_as(C.POINTER(C.c_uint8)) if a is not None else None
+
+
+def unpack_bitmaps(words, n_steps, n_neurons):
+    """Spike bitmaps (ceil(n/32) uint32 words per step, bit i = neuron i) -> bool array [steps, n]."""
+    w = (n_neurons + 31) // 32
+    a = np.asarray(words[: n_steps * w], dtype=np.uint32).reshape(n_steps, w)
+    bits = np.unpackbits(a.view(np.uint8).reshape(n_steps, w * 4), axis=1, bitorder="little")
+    return bits[:, :n_neurons].astype(bool)

@@ -1,0 +1,304 @@
+"""Python binding of the MI355X backend library (knp_amd/libknp_mi355x.so, C ABI in include/knp_mi355x.h).
+
+`MI355XBackend` mirrors the loading / stepping surface of KNP's backends
+(knp/core-library/include/knp/core/backend.h:55-457 and SingleThreadedCPUBackend,
+knp/backends/cpu/cpu-single-threaded-backend/include/knp/backends/cpu-single-threaded/backend.h):
+populations and projections are added in load order, `start_learning` / `stop_learning` unlock / lock
+projections, `_step` advances one network step and `get_step` returns the step counter. Errors raised by the
+library surface as `BackendError` (a RuntimeError) carrying the library's message.
+
+There is no fallback: if the library is missing or no HIP device is present, construction raises.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+from . import abi
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libknp_mi355x.so")
+_LIB = None
+
+
+class BackendError(RuntimeError):
+    pass
+
+
+def _declare(lib):
+    P = C.c_void_p
+    u32, u64, i32 = C.c_uint32, C.c_uint64, C.c_int
+    pu32, pf32, pu8 = C.POINTER(u32), C.POINTER(C.c_float), C.POINTER(C.c_uint8)
+    sig = {
+        "knp_mi355x_version": (C.c_char_p, []),
+        "knp_mi355x_last_error": (C.c_char_p, []),
+        "knp_mi355x_device_count": (i32, [C.POINTER(i32)]),
+        "knp_mi355x_device_name": (i32, [i32, C.c_char_p, C.c_size_t]),
+        "knp_mi355x_create": (i32, [i32, C.POINTER(P)]),
+        "knp_mi355x_destroy": (i32, [P]),
+        "knp_mi355x_set_memory_cap": (i32, [P, u64]),
+        "knp_mi355x_add_blifat_population": (i32, [P, C.POINTER(abi.BLIFATNeuron), u32, pu32]),
+        "knp_mi355x_add_resource_blifat_population": (i32, [P, C.POINTER(abi.ResourceBLIFATNeuron), u32, pu32]),
+        "knp_mi355x_add_delta_projection": (i32, [P, u32, u32, u32, pu32, pu32, pf32, pu32, pu8, u64, pu32]),
+        "knp_mi355x_add_resource_stdp_projection": (
+            i32, [P, u32, u32, u32, pu32, pu32, pf32, pu32, pu8, C.POINTER(abi.ResourceRule), u64, pu32]),
+        "knp_mi355x_add_additive_stdp_projection": (
+            i32, [P, u32, u32, u32, pu32, pu32, pf32, pu32, pu8, pf32, pf32, u64, pu32, pu8, u32, pu32]),
+        "knp_mi355x_set_locked": (i32, [P, u32, i32]),
+        "knp_mi355x_start_learning": (i32, [P]),
+        "knp_mi355x_stop_learning": (i32, [P]),
+        "knp_mi355x_init": (i32, [P]),
+        "knp_mi355x_push_spikes": (i32, [P, u32, pu32, u32]),
+        "knp_mi355x_add_bernoulli_source": (i32, [P, u32, u32, pu32, u32, u32, u32, u32, u64]),
+        "knp_mi355x_step": (i32, [P, u64]),
+        "knp_mi355x_get_step": (i32, [P, C.POINTER(u64)]),
+        "knp_mi355x_synchronize": (i32, [P]),
+        "knp_mi355x_record_spikes": (i32, [P, u32, u64]),
+        "knp_mi355x_read_spike_record": (i32, [P, u32, pu32, u64, C.POINTER(u64)]),
+        "knp_mi355x_read_last_spikes": (i32, [P, u32, pu32, u32, C.POINTER(u32)]),
+        "knp_mi355x_read_potentials": (i32, [P, u32, pf32, u32]),
+        "knp_mi355x_read_blifat_neurons": (i32, [P, u32, C.POINTER(abi.BLIFATNeuron), u32]),
+        "knp_mi355x_read_resource_neurons": (i32, [P, u32, C.POINTER(abi.ResourceBLIFATNeuron), u32]),
+        "knp_mi355x_read_weights": (i32, [P, u32, pf32, u64]),
+        "knp_mi355x_read_resource_rules": (i32, [P, u32, C.POINTER(abi.ResourceRule), u64]),
+        "knp_mi355x_get_event_count": (i32, [P, C.POINTER(u64)]),
+        "knp_mi355x_get_accumulator_bits": (i32, [P, u32, C.POINTER(C.c_int32)]),
+        "knp_mi355x_enable_timing": (i32, [P, i32]),
+        "knp_mi355x_kernel_time": (i32, [P, i32, C.POINTER(C.c_double), C.POINTER(u64)]),
+        "knp_blifat_defaults": (None, [C.POINTER(abi.BLIFATNeuron)]),
+        "knp_resource_blifat_defaults": (None, [C.POINTER(abi.ResourceBLIFATNeuron)]),
+        "knp_resource_rule_defaults": (None, [C.POINTER(abi.ResourceRule)]),
+        "knp_bernoulli_hash": (u32, [u64, u64, u32]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(lib, name)
+        f.restype = res
+        f.argtypes = args
+    return lib
+
+
+def load_library(path=LIB_PATH):
+    """Load the built library (raises if it has not been built)."""
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(path):
+            raise BackendError(f"{path} is missing: build it first (python -c 'import __graft_entry__ as g; g.build()')")
+        _LIB = _declare(C.CDLL(path))
+    return _LIB
+
+
+def device_count():
+    lib = load_library()
+    n = C.c_int(0)
+    if lib.knp_mi355x_device_count(C.byref(n)) != 0:
+        raise BackendError(lib.knp_mi355x_last_error().decode())
+    return n.value
+
+
+def device_name(dev=0):
+    lib = load_library()
+    buf = C.create_string_buffer(256)
+    if lib.knp_mi355x_device_name(dev, buf, 256) != 0:
+        raise BackendError(lib.knp_mi355x_last_error().decode())
+    return buf.value.decode()
+
+
+class MI355XBackend:
+    """A network instance on one MI355X (gfx950) device."""
+
+    def __init__(self, device=0):
+        self.lib = load_library()
+        self.net = C.c_void_p()
+        self._chk(self.lib.knp_mi355x_create(device, C.byref(self.net)))
+        self.pop_sizes = []
+        self.proj_sizes = []
+        self._rec_cap = {}
+
+    # ---- lifetime -----------------------------------------------------------------------------
+    def close(self):
+        if self.net:
+            self.lib.knp_mi355x_destroy(self.net)
+            self.net = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _chk(self, rc):
+        if rc != 0:
+            raise BackendError(f"knp_mi355x error {rc}: {self.lib.knp_mi355x_last_error().decode()}")
+        return rc
+
+    def set_memory_cap(self, nbytes):
+        self._chk(self.lib.knp_mi355x_set_memory_cap(self.net, nbytes))
+
+    # ---- loading ------------------------------------------------------------------------------
+    def add_blifat_population(self, neurons):
+        out = C.c_uint32()
+        self._chk(self.lib.knp_mi355x_add_blifat_population(self.net, neurons, len(neurons), C.byref(out)))
+        self.pop_sizes.append(len(neurons))
+        return out.value
+
+    def add_resource_blifat_population(self, neurons):
+        out = C.c_uint32()
+        self._chk(self.lib.knp_mi355x_add_resource_blifat_population(self.net, neurons, len(neurons), C.byref(out)))
+        self.pop_sizes.append(len(neurons))
+        return out.value
+
+    @staticmethod
+    def _syn(pre, post, weight, delay, otype):
+        pre = np.ascontiguousarray(pre, dtype=np.uint32)
+        post = np.ascontiguousarray(post, dtype=np.uint32)
+        weight = np.ascontiguousarray(weight, dtype=np.float32)
+        delay = np.ascontiguousarray(delay, dtype=np.uint32)
+        otype = None if otype is None else np.ascontiguousarray(otype, dtype=np.uint8)
+        return pre, post, weight, delay, otype
+
+    def add_delta_projection(self, pre_pop, n_pre, post_pop, pre, post, weight, delay, otype=None):
+        pre, post, weight, delay, otype = self._syn(pre, post, weight, delay, otype)
+        out = C.c_uint32()
+        self._chk(self.lib.knp_mi355x_add_delta_projection(
+            self.net, pre_pop, n_pre, post_pop, abi.u32p(pre), abi.u32p(post), abi.f32p(weight), abi.u32p(delay),
+            abi.u8p(otype), len(pre), C.byref(out)))
+        self.proj_sizes.append(len(pre))
+        return out.value
+
+    def add_resource_stdp_projection(self, pre_pop, n_pre, post_pop, pre, post, weight, delay, rules, otype=None):
+        pre, post, weight, delay, otype = self._syn(pre, post, weight, delay, otype)
+        out = C.c_uint32()
+        self._chk(self.lib.knp_mi355x_add_resource_stdp_projection(
+            self.net, pre_pop, n_pre, post_pop, abi.u32p(pre), abi.u32p(post), abi.f32p(weight), abi.u32p(delay),
+            abi.u8p(otype), rules, len(pre), C.byref(out)))
+        self.proj_sizes.append(len(pre))
+        return out.value
+
+    def add_additive_stdp_projection(self, pre_pop, n_pre, post_pop, pre, post, weight, delay, tau_plus, tau_minus,
+                                     stdp_pops=(), stdp_modes=(), otype=None):
+        pre, post, weight, delay, otype = self._syn(pre, post, weight, delay, otype)
+        tp = np.ascontiguousarray(tau_plus, dtype=np.float32)
+        tm = np.ascontiguousarray(tau_minus, dtype=np.float32)
+        sp = np.ascontiguousarray(stdp_pops, dtype=np.uint32)
+        sm = np.ascontiguousarray(stdp_modes, dtype=np.uint8)
+        out = C.c_uint32()
+        self._chk(self.lib.knp_mi355x_add_additive_stdp_projection(
+            self.net, pre_pop, n_pre, post_pop, abi.u32p(pre), abi.u32p(post), abi.f32p(weight), abi.u32p(delay),
+            abi.u8p(otype), abi.f32p(tp), abi.f32p(tm), len(pre), abi.u32p(sp), abi.u8p(sm), len(sp), C.byref(out)))
+        self.proj_sizes.append(len(pre))
+        return out.value
+
+    def set_locked(self, proj, locked):
+        self._chk(self.lib.knp_mi355x_set_locked(self.net, proj, 1 if locked else 0))
+
+    def start_learning(self):
+        self._chk(self.lib.knp_mi355x_start_learning(self.net))
+
+    def stop_learning(self):
+        self._chk(self.lib.knp_mi355x_stop_learning(self.net))
+
+    def _init(self):
+        self._chk(self.lib.knp_mi355x_init(self.net))
+
+    # ---- inputs / stepping --------------------------------------------------------------------
+    def push_spikes(self, proj, idx):
+        idx = np.ascontiguousarray(idx, dtype=np.uint32)
+        self._chk(self.lib.knp_mi355x_push_spikes(self.net, proj, abi.u32p(idx), len(idx)))
+
+    def add_bernoulli_source(self, proj, thresholds, steps_per_pattern=1, active_lo=0, active_hi=None, seed=0):
+        thr = np.ascontiguousarray(np.atleast_2d(thresholds), dtype=np.uint32)
+        n_patterns, n_ch = thr.shape
+        hi = steps_per_pattern - 1 if active_hi is None else active_hi
+        self._chk(self.lib.knp_mi355x_add_bernoulli_source(self.net, proj, n_ch, abi.u32p(thr), n_patterns,
+                                                           steps_per_pattern, active_lo, hi, seed))
+
+    def step(self, n=1):
+        self._chk(self.lib.knp_mi355x_step(self.net, n))
+
+    _step = step
+
+    def synchronize(self):
+        self._chk(self.lib.knp_mi355x_synchronize(self.net))
+
+    def get_step(self):
+        s = C.c_uint64()
+        self._chk(self.lib.knp_mi355x_get_step(self.net, C.byref(s)))
+        return s.value
+
+    current_step = property(get_step)
+
+    @property
+    def events(self):
+        e = C.c_uint64()
+        self._chk(self.lib.knp_mi355x_get_event_count(self.net, C.byref(e)))
+        return e.value
+
+    def accumulator_bits(self, pop):
+        b = C.c_int32()
+        self._chk(self.lib.knp_mi355x_get_accumulator_bits(self.net, pop, C.byref(b)))
+        return b.value
+
+    # ---- outputs ------------------------------------------------------------------------------
+    def record_spikes(self, pop, capacity_steps):
+        self._chk(self.lib.knp_mi355x_record_spikes(self.net, pop, capacity_steps))
+        self._rec_cap[pop] = capacity_steps
+
+    def read_spike_record(self, pop):
+        n = self.pop_sizes[pop]
+        w = (n + 31) // 32
+        cap = self._rec_cap.get(pop, 0)
+        buf = np.zeros(max(w * cap, 1), dtype=np.uint32)
+        steps = C.c_uint64()
+        self._chk(self.lib.knp_mi355x_read_spike_record(self.net, pop, abi.u32p(buf), buf.size, C.byref(steps)))
+        return abi.unpack_bitmaps(buf, steps.value, n)
+
+    def last_spikes(self, pop):
+        n = self.pop_sizes[pop]
+        buf = np.zeros(max(n, 1), dtype=np.uint32)
+        cnt = C.c_uint32()
+        self._chk(self.lib.knp_mi355x_read_last_spikes(self.net, pop, abi.u32p(buf), n, C.byref(cnt)))
+        return buf[: cnt.value].copy()
+
+    def potentials(self, pop):
+        n = self.pop_sizes[pop]
+        out = np.zeros(n, dtype=np.float32)
+        self._chk(self.lib.knp_mi355x_read_potentials(self.net, pop, abi.f32p(out), n))
+        return out
+
+    def blifat_neurons(self, pop):
+        n = self.pop_sizes[pop]
+        out = (abi.BLIFATNeuron * n)()
+        self._chk(self.lib.knp_mi355x_read_blifat_neurons(self.net, pop, out, n))
+        return out
+
+    def resource_neurons(self, pop):
+        n = self.pop_sizes[pop]
+        out = (abi.ResourceBLIFATNeuron * n)()
+        self._chk(self.lib.knp_mi355x_read_resource_neurons(self.net, pop, out, n))
+        return out
+
+    def weights(self, proj):
+        n = self.proj_sizes[proj]
+        out = np.zeros(n, dtype=np.float32)
+        self._chk(self.lib.knp_mi355x_read_weights(self.net, proj, abi.f32p(out), n))
+        return out
+
+    def resource_rules(self, proj):
+        n = self.proj_sizes[proj]
+        out = (abi.ResourceRule * n)()
+        self._chk(self.lib.knp_mi355x_read_resource_rules(self.net, proj, out, n))
+        return out
+
+    def resources(self, proj):
+        r = self.resource_rules(proj)
+        return np.array([x.synaptic_resource for x in r], dtype=np.float32)
+
+    # ---- kernel timing ------------------------------------------------------------------------
+    def enable_timing(self, on=True):
+        self._chk(self.lib.knp_mi355x_enable_timing(self.net, 1 if on else 0))
+
+    def kernel_time(self, kernel_id):
+        avg = C.c_double()
+        n = C.c_uint64()
+        self._chk(self.lib.knp_mi355x_kernel_time(self.net, kernel_id, C.byref(avg), C.byref(n)))
+        return avg.value, n.value

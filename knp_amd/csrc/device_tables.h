@@ -1,0 +1,151 @@
+// device_tables.h - POD tables describing a loaded network on the device.
+//
+// A network is a set of populations (BLIFAT or synaptic-resource BLIFAT) and projections (Delta,
+// synaptic-resource STDP Delta, additive STDP Delta), in load order, exactly as a KNP backend holds them
+// (knp/backends/cpu/cpu-single-threaded-backend/include/.../backend.h:107-122). The host builds these
+// tables once at init and re-uploads them only when learning locks or observation settings change.
+#pragma once
+#include <stdint.h>
+
+namespace knp_mi355x {
+
+constexpr int kMaxPops = 32;
+constexpr int kMaxProjs = 64;
+constexpr int kMaxSources = 32;
+constexpr int kMaxRowSources = 128;
+constexpr int kMaxTasks = kMaxPops + kMaxSources;
+constexpr int kMaxWorking = 8;
+constexpr int kBlock = 256;  // 4 wave64 per workgroup
+
+enum PopKind : uint32_t { POP_LEAN = 0, POP_FULL = 1, POP_RESOURCE = 2 };
+enum ProjKind : uint32_t { PROJ_DELTA = 0, PROJ_RESOURCE = 1, PROJ_ADDITIVE = 2 };
+
+// A parameter that is either one value for the whole population or one value per neuron.
+struct ParamF {
+    float v;
+    const float *a;
+};
+struct ParamU {
+    uint32_t v;
+    const uint32_t *a;
+};
+
+// Synaptic-resource neuron state and parameters (neuron_parameters<SynapticResourceSTDPNeuron<...>>).
+struct ResDev {
+    float *add_thr;      // additional_threshold_ (state written by training)
+    float *free_res;     // free_synaptic_resource_
+    float *stability;    // stability_
+    uint8_t *isi_status; // ISIPeriodType
+    uint64_t *last_step, *last_spike_step, *first_isi_spike;
+    float *dop_now;      // dopamine_value_ of this step (written by the population kernel)
+    uint8_t *forced_now; // is_being_forced_ of this step
+    ParamF coeff, res_thr, change_param, change_at_isi, d_h;
+    ParamU dpt, drain, isi_max;
+    uint32_t *cand;      // training candidates of this step (neuron indexes)
+    uint32_t n_work;     // working projections: unlocked resource-STDP projections targeting the population
+    uint32_t work[kMaxWorking];
+};
+
+struct PopDev {
+    uint32_t n, kind, D, words;  // D = ring depth (max delay of incoming projections), words = ceil(n/32)
+    int32_t frac_bits;            // fixed-point fraction bits of the exact accumulators
+    float scale, inv_scale;       // 2^frac_bits, 2^-frac_bits
+    int64_t *acc_v;               // [D][n] EXCITATORY - INHIBITORY_CURRENT
+    int64_t *acc_g;               // [D][n] INHIBITORY_CONDUCTANCE (null if no such projection)
+    int64_t *acc_d;               // [D][n] DOPAMINE (null if none)
+    uint32_t *acc_f;              // [D][n] forcing flags (resource populations only)
+    uint64_t *acc_b;              // [D][n] BLOCKING "last impact" keys (null if none)
+    // state (FULL / RESOURCE)
+    float *V, *Vpre, *dyn, *trace, *g, *dop;
+    uint32_t *n_ts, *bphase;
+    int64_t *tbp;
+    // state (LEAN): step of the last spike, UINT32_MAX = none since load
+    uint32_t *last_spike;
+    ParamF act, thr_decay, thr_inc, trace_decay, trace_inc, g_decay, V_decay, reflexive, E_inh, V_reset, V_min,
+        add_thr;
+    ParamU bperiod, refr;
+    uint32_t lean_ts_max;  // LEAN: every neuron started with n_time_steps_since_last_firing = SIZE_MAX
+    uint32_t track_tbp;    // FULL: total_blocking_period_ is streamed
+    uint64_t t_first;      // first step this population runs
+    // outputs
+    uint32_t *spike_list;  // compacted spikes of the current step (unordered)
+    uint32_t *bitmap;      // spikes of the current step, bit i = neuron i
+    uint32_t *rec;         // recorded bitmaps [rec_cap][words]
+    uint64_t rec_t0, rec_cap;
+    ResDev res;
+};
+
+// One source of spike rows for the scatter: the channel input list of a projection or the spike list of
+// a population, with its message rank in the reference delivery order.
+struct RowSource {
+    uint32_t proj;
+    uint32_t from_pop;  // population index, or UINT32_MAX for the projection's channel input list
+    uint32_t rank;      // 0 = channel messages, 1 + population index for population messages
+    uint32_t pad;
+};
+
+struct ProjDev {
+    uint32_t kind, pre_pop, post_pop, n_pre;
+    uint64_t n_syn;
+    const uint64_t *row_ptr;     // [n_pre + 1], CSR by presynaptic neuron, ascending synapse index
+    const uint32_t *post_type;   // [n_syn] CSR order: post | (output_type << 29)
+    const uint8_t *delay;        // [n_syn] CSR order
+    const float *w_csr;          // [n_syn] CSR order (plain Delta)
+    const uint32_t *csr_syn;     // [n_syn] CSR position -> synapse index (plastic projections)
+    float *W;                    // [n_syn] weights in synapse-index order (plastic projections)
+    // synaptic-resource rule state, synapse-index order
+    float *r, *w_min, *w_max, *d_u;
+    uint32_t *period;
+    uint64_t *last_spike_step;
+    uint8_t *flags;              // bit0 had_hebbian_update_, bit1 has_contributed_
+    const uint8_t *delay_syn;    // [n_syn] delay in synapse-index order
+    const uint64_t *col_ptr;     // [n_post + 1] CSC by postsynaptic neuron
+    const uint32_t *col_syn;     // [n_syn] synapse indexes, ascending within a column
+    // additive STDP
+    float *tau_plus, *tau_minus;
+    uint32_t *lim;               // spike-history cap per synapse
+    uint64_t *pre_times, *post_times;  // [n_syn][hist]
+    uint32_t *n_pre_t, *n_post_t;
+    uint32_t hist;
+    // channel input list
+    uint32_t *in_list;
+    uint32_t in_cap;
+    uint32_t forcing;            // messages of this projection are forcing (plain Delta)
+    uint32_t blocking_id;        // rank of this projection among projections with BLOCKING synapses
+    uint32_t n_stdp;
+    uint32_t stdp_pop[4];
+    uint32_t stdp_mode[4];
+};
+
+struct SourceDev {
+    uint32_t proj, n_ch, n_patterns, steps_per_pattern, lo, hi;
+    uint64_t seed;
+    const uint32_t *thr;
+};
+
+// Phase-1 task: a population update or a Bernoulli source, covering `blocks` workgroups.
+struct TaskDev {
+    uint32_t kind;  // 0 population, 1 source
+    uint32_t index;
+    uint32_t block_begin, blocks;
+};
+
+struct Ctl {
+    uint32_t pop_count[kMaxPops];
+    uint32_t in_count[kMaxProjs];
+    uint32_t cand_count[kMaxPops];
+};
+
+struct NetDev {
+    uint32_t n_pops, n_projs, n_srcs, n_tasks, n_row_sources, total_blocks;
+    PopDev pops[kMaxPops];
+    ProjDev projs[kMaxProjs];
+    SourceDev srcs[kMaxSources];
+    TaskDev tasks[kMaxTasks];
+    RowSource rows[kMaxRowSources];
+    Ctl *ctl;                  // [2], indexed by step parity
+    uint32_t blk_proj[64];     // projection index of each projection with BLOCKING synapses
+    unsigned long long *events;
+};
+
+}  // namespace knp_mi355x

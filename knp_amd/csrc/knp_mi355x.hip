@@ -1,0 +1,1316 @@
+// knp_mi355x.hip - host runtime and C ABI of the MI355X backend (see include/knp_mi355x.h).
+//
+// The runtime owns one HIP stream per network handle. Loading copies the caller's reference-layout
+// (AoS) populations and projections, _init (knp_mi355x_init) converts them into device structure-of-arrays
+// state, CSR/CSC synapse tables and per-population delay rings, and every step then runs on the device
+// with no host synchronization (reference step: knp/backends/cpu/cpu-single-threaded-backend/impl/
+// backend.cpp:89-141).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <limits>
+#include <string>
+#include <vector>
+
+#include "../../include/knp_mi355x.h"
+#include "device_tables.h"
+#include "step_kernels.h"
+
+using namespace knp_mi355x;
+
+namespace {
+
+thread_local std::string g_last_error;
+
+int fail(int code, const std::string &msg) {
+    g_last_error = msg;
+    return code;
+}
+
+#define HIP_TRY(expr)                                                                                     \
+    do {                                                                                                  \
+        hipError_t e_ = (expr);                                                                           \
+        if (e_ != hipSuccess) return fail(KNP_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+struct HostPop {
+    bool resource = false;
+    uint32_t n = 0;
+    std::vector<knp_blifat_neuron> nb;           // initial state as loaded
+    std::vector<knp_resource_blifat_neuron> nr;  // resource populations
+    uint32_t kind = POP_FULL;
+    uint32_t D = 1;
+    bool has_cond = false, has_dop = false, has_block = false, has_forcing = false;
+    int32_t frac_bits = 48;
+    uint64_t t_first = 0;
+    bool ran = false;
+    uint64_t rec_cap = 0, rec_t0 = 0;
+};
+
+struct HostProj {
+    uint32_t kind = PROJ_DELTA;
+    uint32_t pre_pop = KNP_NO_POPULATION, n_pre = 0, post_pop = 0;
+    uint64_t n_syn = 0;
+    std::vector<uint32_t> pre, post, delay;
+    std::vector<float> w;
+    std::vector<uint8_t> type;
+    std::vector<knp_resource_rule> rules;
+    std::vector<float> tau_p, tau_m;
+    std::vector<uint32_t> stdp_pops;
+    std::vector<uint8_t> stdp_modes;
+    bool locked = true;
+    uint32_t max_delay = 1;
+    bool has_blocking = false;
+    std::vector<std::vector<uint32_t>> pending;  // channel messages for the next step
+    uint32_t gen_channels = 0;                   // channels of device sources feeding it
+};
+
+struct HostSource {
+    uint32_t proj, n_ch, n_patterns, spp, lo, hi;
+    uint64_t seed;
+    std::vector<uint32_t> thr;
+};
+
+}  // namespace
+
+struct knp_net {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::vector<HostPop> pops;
+    std::vector<HostProj> projs;
+    std::vector<HostSource> srcs;
+    bool initialized = false;
+    uint64_t step = 0;
+    NetDev table{};
+    NetDev *dev_table = nullptr;
+    std::vector<void *> allocations;
+    uint64_t allocated = 0, cap = 0;
+    uint32_t scatter_grid = 1, train_grid = 1;
+    std::vector<uint32_t> staging;
+    bool timing = false;
+    std::vector<hipEvent_t> ev_pool;
+    size_t ev_used = 0;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> timed[KNP_KERNEL_COUNT];
+    double time_ms[KNP_KERNEL_COUNT] = {0, 0, 0};
+    uint64_t launches[KNP_KERNEL_COUNT] = {0, 0, 0};
+};
+
+namespace {
+
+int dev_alloc(knp_net *net, void **out, size_t bytes, bool zero = true) {
+    *out = nullptr;
+    if (bytes == 0) bytes = 4;
+    if (net->cap && net->allocated + bytes > net->cap)
+        return fail(KNP_ERR_OUT_OF_MEMORY, "device memory cap of " + std::to_string(net->cap) + " bytes exceeded");
+    hipError_t e = hipMalloc(out, bytes);
+    if (e != hipSuccess) {
+        *out = nullptr;
+        return fail(KNP_ERR_OUT_OF_MEMORY, std::string("hipMalloc: ") + hipGetErrorString(e));
+    }
+    net->allocations.push_back(*out);
+    net->allocated += bytes;
+    if (zero) HIP_TRY(hipMemsetAsync(*out, 0, bytes, net->stream));
+    return KNP_OK;
+}
+
+template <typename T>
+int upload(knp_net *net, T **out, const T *src, size_t count) {
+    int rc = dev_alloc(net, (void **)out, count * sizeof(T), false);
+    if (rc) return rc;
+    if (count) HIP_TRY(hipMemcpyAsync(*out, src, count * sizeof(T), hipMemcpyHostToDevice, net->stream));
+    // the source is host memory owned by the caller's frame: finish before it goes away
+    HIP_TRY(hipStreamSynchronize(net->stream));
+    return KNP_OK;
+}
+
+// A parameter field: one value if uniform over the population, else a device array.
+template <typename Get>
+int make_param_f(knp_net *net, uint32_t n, Get get, ParamF *out) {
+    out->a = nullptr;
+    out->v = n ? get(0) : 0.f;
+    bool uniform = true;
+    for (uint32_t i = 1; i < n && uniform; ++i) {
+        float a = get(i), b = out->v;
+        uniform = (std::memcmp(&a, &b, sizeof(float)) == 0);
+    }
+    if (uniform) return KNP_OK;
+    std::vector<float> v(n);
+    for (uint32_t i = 0; i < n; ++i) v[i] = get(i);
+    float *d = nullptr;
+    int rc = upload(net, &d, v.data(), n);
+    out->a = d;
+    return rc;
+}
+
+template <typename Get>
+int make_param_u(knp_net *net, uint32_t n, Get get, ParamU *out) {
+    out->a = nullptr;
+    out->v = n ? get(0) : 0u;
+    bool uniform = true;
+    for (uint32_t i = 1; i < n && uniform; ++i) uniform = get(i) == out->v;
+    if (uniform) return KNP_OK;
+    std::vector<uint32_t> v(n);
+    for (uint32_t i = 0; i < n; ++i) v[i] = get(i);
+    uint32_t *d = nullptr;
+    int rc = upload(net, &d, v.data(), n);
+    out->a = d;
+    return rc;
+}
+
+uint32_t ts_to_device(uint64_t v) {
+    if (v == std::numeric_limits<uint64_t>::max()) return 0xFFFFFFFFu;
+    if (v >= 0xFFFFFFFEull) return 0xFFFFFFFEu;
+    return (uint32_t)v;
+}
+
+uint64_t ts_from_device(uint32_t v) {
+    if (v == 0xFFFFFFFFu) return std::numeric_limits<uint64_t>::max();
+    return v;
+}
+
+bool lean_eligible(const HostPop &P) {
+    if (P.resource || P.has_cond || P.has_dop || P.has_block) return false;
+    bool any_max = false, any_other = false;
+    for (const auto &n : P.nb) {
+        if (n.n_time_steps_since_last_firing == std::numeric_limits<uint64_t>::max()) any_max = true;
+        else any_other = true;
+        if (n.dynamic_threshold != 0.0 || n.threshold_increment != 0.0) return false;
+        if (!std::isfinite(n.threshold_decay) || !std::isfinite(n.postsynaptic_trace_decay)) return false;
+        if (n.postsynaptic_trace != 0.0 || n.postsynaptic_trace_increment != 0.0) return false;
+        if (n.inhibitory_conductance != 0.0 || !std::isfinite(n.inhibitory_conductance_decay)) return false;
+        if (n.potential_decay != 0.0 || !std::isfinite(n.potential)) return false;
+        if (n.bursting_phase != 0 || n.bursting_period != 0) return false;
+        if (n.absolute_refractory_period != 0) return false;
+        if (n.total_blocking_period < (int64_t(1) << 62)) return false;
+    }
+    return !(any_max && any_other);
+}
+
+// Fraction bits of a population's exact accumulators: the largest sum of |weight| a neuron can receive in
+// one step must stay below 2^(62 - frac_bits) with a factor-4 margin; at most 52 bits.
+int32_t choose_frac_bits(double bound) {
+    int e = 0;
+    std::frexp(bound + 1.0, &e);  // bound + 1 < 2^e
+    int f = 61 - e - 2;
+    return std::min(52, f);
+}
+
+int upload_working_sets(knp_net *net);
+
+int init_network(knp_net *net) {
+    HIP_TRY(hipSetDevice(net->device));
+    const uint32_t n_pops = (uint32_t)net->pops.size(), n_projs = (uint32_t)net->projs.size();
+    if (n_pops > (uint32_t)kMaxPops) return fail(KNP_ERR_UNSUPPORTED, "too many populations");
+    if (n_projs > (uint32_t)kMaxProjs) return fail(KNP_ERR_UNSUPPORTED, "too many projections");
+    if (net->srcs.size() > (size_t)kMaxSources) return fail(KNP_ERR_UNSUPPORTED, "too many sources");
+    NetDev &T = net->table;
+    std::memset(&T, 0, sizeof(T));
+    T.n_pops = n_pops;
+    T.n_projs = n_projs;
+    T.n_srcs = (uint32_t)net->srcs.size();
+
+    // ---- incoming-projection facts per population
+    std::vector<std::vector<double>> bound(n_pops);
+    for (uint32_t p = 0; p < n_pops; ++p) bound[p].assign(net->pops[p].n, 0.0);
+    uint32_t n_blocking = 0;
+    for (uint32_t q = 0; q < n_projs; ++q) {
+        HostProj &Q = net->projs[q];
+        HostPop &P = net->pops[Q.post_pop];
+        P.D = std::max(P.D, Q.max_delay);
+        for (uint64_t s = 0; s < Q.n_syn; ++s) {
+            const uint8_t ty = Q.type[s];
+            if (ty == KNP_INHIBITORY_CONDUCTANCE) P.has_cond = true;
+            if (ty == KNP_DOPAMINE) P.has_dop = true;
+            if (ty == KNP_BLOCKING) P.has_block = true;
+            if (ty == KNP_EXCITATORY && Q.kind == PROJ_DELTA && P.resource) P.has_forcing = true;
+            double mag = std::fabs((double)Q.w[s]);
+            if (Q.kind == PROJ_RESOURCE)
+                mag = std::max({mag, std::fabs((double)Q.rules[s].w_min), std::fabs((double)Q.rules[s].w_max)});
+            if (Q.kind == PROJ_ADDITIVE) mag = 2.0 * mag + 1.0;
+            bound[Q.post_pop][Q.post[s]] += mag;
+        }
+        if (Q.has_blocking) {
+            if (Q.kind != PROJ_DELTA)
+                return fail(KNP_ERR_UNSUPPORTED, "BLOCKING synapses are supported on Delta projections only");
+            if (Q.max_delay > 64) return fail(KNP_ERR_UNSUPPORTED, "BLOCKING projection with delay > 64");
+            if (n_blocking >= 64) return fail(KNP_ERR_UNSUPPORTED, "more than 64 projections with BLOCKING synapses");
+            T.blk_proj[n_blocking] = q;
+            T.projs[q].blocking_id = n_blocking++;
+        }
+    }
+    if (n_pops > 31 && n_blocking) return fail(KNP_ERR_UNSUPPORTED, "BLOCKING ranks support up to 31 populations");
+
+    // ---- populations
+    for (uint32_t p = 0; p < n_pops; ++p) {
+        HostPop &H = net->pops[p];
+        PopDev &P = T.pops[p];
+        const uint32_t n = H.n;
+        double maxb = 0;
+        for (double b : bound[p]) maxb = std::max(maxb, b);
+        H.frac_bits = choose_frac_bits(maxb);
+        if (H.frac_bits < 16)
+            return fail(KNP_ERR_RANGE, "population " + std::to_string(p) +
+                                           ": incoming |weight| sums too large for the exact accumulator");
+        H.kind = H.resource ? POP_RESOURCE : (lean_eligible(H) ? POP_LEAN : POP_FULL);
+        P.n = n;
+        P.kind = H.kind;
+        P.D = H.D;
+        P.words = (n + 31) / 32;
+        P.frac_bits = H.frac_bits;
+        P.scale = std::ldexp(1.0f, H.frac_bits);
+        P.inv_scale = std::ldexp(1.0f, -H.frac_bits);
+        int rc;
+        const size_t ring = (size_t)H.D * n;
+        if ((rc = dev_alloc(net, (void **)&P.acc_v, ring * 8))) return rc;
+        if (H.has_cond && (rc = dev_alloc(net, (void **)&P.acc_g, ring * 8))) return rc;
+        if (H.has_dop && (rc = dev_alloc(net, (void **)&P.acc_d, ring * 8))) return rc;
+        if (H.has_forcing && (rc = dev_alloc(net, (void **)&P.acc_f, ring * 4))) return rc;
+        if (H.has_block && (rc = dev_alloc(net, (void **)&P.acc_b, ring * 8))) return rc;
+        const auto &nb = H.nb;
+        std::vector<float> Vh(n);
+        for (uint32_t i = 0; i < n; ++i) Vh[i] = (float)nb[i].potential;
+        if ((rc = upload(net, &P.V, Vh.data(), n))) return rc;
+        if ((rc = make_param_f(net, n, [&](uint32_t i) { return (float)nb[i].activation_threshold; }, &P.act))) return rc;
+        if ((rc = make_param_f(net, n, [&](uint32_t i) { return (float)nb[i].potential_reset_value; }, &P.V_reset))) return rc;
+        if ((rc = make_param_f(net, n, [&](uint32_t i) { return (float)nb[i].min_potential; }, &P.V_min))) return rc;
+        if (H.kind != POP_RESOURCE)
+            if ((rc = make_param_f(net, n, [&](uint32_t i) { return (float)nb[i].additional_threshold; }, &P.add_thr))) return rc;
+        if (H.kind == POP_LEAN) {
+            P.lean_ts_max = nb.empty() ? 0 : (nb[0].n_time_steps_since_last_firing == std::numeric_limits<uint64_t>::max());
+            std::vector<uint32_t> ls(n, 0xFFFFFFFFu);
+            if ((rc = upload(net, &P.last_spike, ls.data(), n))) return rc;
+        } else {
+            std::vector<float> f(n);
+            std::vector<uint32_t> u(n);
+            auto upf = [&](float **dst, auto get) {
+                for (uint32_t i = 0; i < n; ++i) f[i] = get(i);
+                return upload(net, dst, f.data(), n);
+            };
+            auto upu = [&](uint32_t **dst, auto get) {
+                for (uint32_t i = 0; i < n; ++i) u[i] = get(i);
+                return upload(net, dst, u.data(), n);
+            };
+            if ((rc = upf(&P.Vpre, [&](uint32_t i) { return (float)nb[i].pre_impact_potential; }))) return rc;
+            if ((rc = upf(&P.dyn, [&](uint32_t i) { return (float)nb[i].dynamic_threshold; }))) return rc;
+            if ((rc = upf(&P.trace, [&](uint32_t i) { return (float)nb[i].postsynaptic_trace; }))) return rc;
+            if ((rc = upf(&P.g, [&](uint32_t i) { return (float)nb[i].inhibitory_conductance; }))) return rc;
+            if ((rc = upf(&P.dop, [&](uint32_t i) { return (float)nb[i].dopamine_value; }))) return rc;
+            if ((rc = upu(&P.n_ts, [&](uint32_t i) { return ts_to_device(nb[i].n_time_steps_since_last_firing); }))) return rc;
+            if ((rc = upu(&P.bphase, [&](uint32_t i) { return nb[i].bursting_phase; }))) return rc;
+            bool track = H.has_block;
+            for (const auto &x : nb) track |= x.total_blocking_period < (int64_t(1) << 62);
+            P.track_tbp = track ? 1 : 0;
+            if (track) {
+                std::vector<int64_t> tb(n);
+                for (uint32_t i = 0; i < n; ++i) tb[i] = nb[i].total_blocking_period;
+                if ((rc = upload(net, &P.tbp, tb.data(), n))) return rc;
+            }
+            if ((rc = make_param_f(net, n, [&](uint32_t i) { return (float)nb[i].threshold_decay; }, &P.thr_decay))) return rc;
+            if ((rc = make_param_f(net, n, [&](uint32_t i) { return (float)nb[i].threshold_increment; }, &P.thr_inc))) return rc;
+            if ((rc = make_param_f(net, n, [&](uint32_t i) { return (float)nb[i].postsynaptic_trace_decay; }, &P.trace_decay))) return rc;
+            if ((rc = make_param_f(net, n, [&](uint32_t i) { return (float)nb[i].postsynaptic_trace_increment; }, &P.trace_inc))) return rc;
+            if ((rc = make_param_f(net, n, [&](uint32_t i) { return (float)nb[i].inhibitory_conductance_decay; }, &P.g_decay))) return rc;
+            if ((rc = make_param_f(net, n, [&](uint32_t i) { return (float)nb[i].potential_decay; }, &P.V_decay))) return rc;
+            if ((rc = make_param_f(net, n, [&](uint32_t i) { return (float)nb[i].reflexive_weight; }, &P.reflexive))) return rc;
+            if ((rc = make_param_f(net, n, [&](uint32_t i) { return (float)nb[i].reversal_inhibitory_potential; }, &P.E_inh))) return rc;
+            if ((rc = make_param_u(net, n, [&](uint32_t i) { return nb[i].bursting_period; }, &P.bperiod))) return rc;
+            if ((rc = make_param_u(net, n, [&](uint32_t i) { return nb[i].absolute_refractory_period; }, &P.refr))) return rc;
+        }
+        if (H.kind == POP_RESOURCE) {
+            ResDev &R = P.res;
+            const auto &nr = H.nr;
+            std::vector<float> f(n);
+            std::vector<uint64_t> u64(n);
+            std::vector<uint8_t> u8(n);
+            for (uint32_t i = 0; i < n; ++i) f[i] = (float)nb[i].additional_threshold;
+            if ((rc = upload(net, &R.add_thr, f.data(), n))) return rc;
+            for (uint32_t i = 0; i < n; ++i) f[i] = nr[i].free_synaptic_resource;
+            if ((rc = upload(net, &R.free_res, f.data(), n))) return rc;
+            for (uint32_t i = 0; i < n; ++i) f[i] = nr[i].stability;
+            if ((rc = upload(net, &R.stability, f.data(), n))) return rc;
+            for (uint32_t i = 0; i < n; ++i) u8[i] = (uint8_t)nr[i].isi_status;
+            if ((rc = upload(net, &R.isi_status, u8.data(), n))) return rc;
+            for (uint32_t i = 0; i < n; ++i) u64[i] = nr[i].last_step;
+            if ((rc = upload(net, &R.last_step, u64.data(), n))) return rc;
+            for (uint32_t i = 0; i < n; ++i) u64[i] = nr[i].last_spike_step;
+            if ((rc = upload(net, &R.last_spike_step, u64.data(), n))) return rc;
+            for (uint32_t i = 0; i < n; ++i) u64[i] = nr[i].first_isi_spike;
+            if ((rc = upload(net, &R.first_isi_spike, u64.data(), n))) return rc;
+            R.dop_now = P.dop;
+            if ((rc = dev_alloc(net, (void **)&R.forced_now, n))) return rc;
+            if ((rc = dev_alloc(net, (void **)&R.cand, (size_t)n * 4))) return rc;
+            if ((rc = make_param_f(net, n, [&](uint32_t i) { return nr[i].synapse_sum_threshold_coefficient; }, &R.coeff))) return rc;
+            if ((rc = make_param_f(net, n, [&](uint32_t i) { return nr[i].synaptic_resource_threshold; }, &R.res_thr))) return rc;
+            if ((rc = make_param_f(net, n, [&](uint32_t i) { return nr[i].stability_change_parameter; }, &R.change_param))) return rc;
+            if ((rc = make_param_f(net, n, [&](uint32_t i) { return nr[i].stability_change_at_isi; }, &R.change_at_isi))) return rc;
+            if ((rc = make_param_f(net, n, [&](uint32_t i) { return nr[i].d_h; }, &R.d_h))) return rc;
+            if ((rc = make_param_u(net, n, [&](uint32_t i) { return nr[i].dopamine_plasticity_time; }, &R.dpt))) return rc;
+            if ((rc = make_param_u(net, n, [&](uint32_t i) { return nr[i].resource_drain_coefficient; }, &R.drain))) return rc;
+            if ((rc = make_param_u(net, n, [&](uint32_t i) { return nr[i].isi_max; }, &R.isi_max))) return rc;
+        }
+        if ((rc = dev_alloc(net, (void **)&P.spike_list, (size_t)std::max(n, 1u) * 4))) return rc;
+        if ((rc = dev_alloc(net, (void **)&P.bitmap, (size_t)std::max(P.words, 1u) * 4))) return rc;
+        if (H.rec_cap) {
+            if ((rc = dev_alloc(net, (void **)&P.rec, (size_t)H.rec_cap * P.words * 4))) return rc;
+            P.rec_cap = H.rec_cap;
+            P.rec_t0 = net->step;
+            H.rec_t0 = net->step;
+        }
+        P.t_first = net->step;
+        H.t_first = net->step;
+    }
+
+    // ---- projections
+    for (uint32_t q = 0; q < n_projs; ++q) {
+        HostProj &H = net->projs[q];
+        ProjDev &Q = T.projs[q];
+        const uint32_t n_post = net->pops[H.post_pop].n;
+        Q.kind = H.kind;
+        Q.pre_pop = H.pre_pop;
+        Q.post_pop = H.post_pop;
+        Q.n_pre = H.n_pre;
+        Q.n_syn = H.n_syn;
+        Q.forcing = (H.kind == PROJ_DELTA) ? 1u : 0u;  // stdp_dispatcher.h:54-58, synaptic_resource_stdp.h:58-62
+        const uint64_t S = H.n_syn;
+        if (S >= 0xFFFFFFFFull) return fail(KNP_ERR_UNSUPPORTED, "projection with 2^32 or more synapses");
+        // CSR by presynaptic neuron (stable: ascending synapse index within a row)
+        std::vector<uint64_t> row_ptr((size_t)H.n_pre + 1, 0);
+        for (uint64_t s = 0; s < S; ++s) row_ptr[H.pre[s] + 1]++;
+        for (uint32_t i = 0; i < H.n_pre; ++i) row_ptr[i + 1] += row_ptr[i];
+        std::vector<uint32_t> csr_syn(S);
+        {
+            std::vector<uint64_t> fill(row_ptr.begin(), row_ptr.end() - 1);
+            for (uint64_t s = 0; s < S; ++s) csr_syn[fill[H.pre[s]]++] = (uint32_t)s;
+        }
+        if (H.has_blocking)
+            for (uint32_t i = 0; i < H.n_pre; ++i)
+                if (row_ptr[i + 1] - row_ptr[i] >= (1u << 17))
+                    return fail(KNP_ERR_UNSUPPORTED, "BLOCKING projection row longer than 131071 synapses");
+        std::vector<uint32_t> post_type(S);
+        std::vector<uint8_t> dl(S);
+        for (uint64_t k = 0; k < S; ++k) {
+            const uint32_t s = csr_syn[k];
+            post_type[k] = H.post[s] | ((uint32_t)H.type[s] << 29);
+            dl[k] = (uint8_t)H.delay[s];
+        }
+        int rc;
+        uint64_t *rp;
+        if ((rc = upload(net, &rp, row_ptr.data(), row_ptr.size()))) return rc;
+        Q.row_ptr = rp;
+        uint32_t *pt;
+        if ((rc = upload(net, &pt, post_type.data(), S))) return rc;
+        Q.post_type = pt;
+        uint8_t *dd;
+        if ((rc = upload(net, &dd, dl.data(), S))) return rc;
+        Q.delay = dd;
+        uint32_t *cs;
+        if ((rc = upload(net, &cs, csr_syn.data(), S))) return rc;
+        Q.csr_syn = cs;
+        if (H.kind == PROJ_DELTA) {
+            std::vector<float> wc(S);
+            for (uint64_t k = 0; k < S; ++k) wc[k] = H.w[csr_syn[k]];
+            float *wd;
+            if ((rc = upload(net, &wd, wc.data(), S))) return rc;
+            Q.w_csr = wd;
+        } else {
+            if ((rc = upload(net, &Q.W, H.w.data(), S))) return rc;
+            // CSC by postsynaptic neuron (stable)
+            std::vector<uint64_t> col_ptr((size_t)n_post + 1, 0);
+            for (uint64_t s = 0; s < S; ++s) col_ptr[H.post[s] + 1]++;
+            for (uint32_t i = 0; i < n_post; ++i) col_ptr[i + 1] += col_ptr[i];
+            std::vector<uint32_t> col_syn(S);
+            {
+                std::vector<uint64_t> fill(col_ptr.begin(), col_ptr.end() - 1);
+                for (uint64_t s = 0; s < S; ++s) col_syn[fill[H.post[s]]++] = (uint32_t)s;
+            }
+            uint64_t *cp;
+            if ((rc = upload(net, &cp, col_ptr.data(), col_ptr.size()))) return rc;
+            Q.col_ptr = cp;
+            uint32_t *cy;
+            if ((rc = upload(net, &cy, col_syn.data(), S))) return rc;
+            Q.col_syn = cy;
+            std::vector<uint8_t> ds(S);
+            for (uint64_t s = 0; s < S; ++s) ds[s] = (uint8_t)H.delay[s];
+            uint8_t *dsd;
+            if ((rc = upload(net, &dsd, ds.data(), S))) return rc;
+            Q.delay_syn = dsd;
+        }
+        if (H.kind == PROJ_RESOURCE) {
+            std::vector<float> f(S);
+            std::vector<uint32_t> u(S);
+            std::vector<uint64_t> u64(S);
+            std::vector<uint8_t> fl(S);
+            for (uint64_t s = 0; s < S; ++s) f[s] = H.rules[s].synaptic_resource;
+            if ((rc = upload(net, &Q.r, f.data(), S))) return rc;
+            for (uint64_t s = 0; s < S; ++s) f[s] = H.rules[s].w_min;
+            if ((rc = upload(net, &Q.w_min, f.data(), S))) return rc;
+            for (uint64_t s = 0; s < S; ++s) f[s] = H.rules[s].w_max;
+            if ((rc = upload(net, &Q.w_max, f.data(), S))) return rc;
+            for (uint64_t s = 0; s < S; ++s) f[s] = H.rules[s].d_u;
+            if ((rc = upload(net, &Q.d_u, f.data(), S))) return rc;
+            for (uint64_t s = 0; s < S; ++s) u[s] = H.rules[s].dopamine_plasticity_period;
+            if ((rc = upload(net, &Q.period, u.data(), S))) return rc;
+            for (uint64_t s = 0; s < S; ++s) u64[s] = H.rules[s].last_spike_step;
+            if ((rc = upload(net, &Q.last_spike_step, u64.data(), S))) return rc;
+            for (uint64_t s = 0; s < S; ++s)
+                fl[s] = (uint8_t)((H.rules[s].had_hebbian_update ? 1 : 0) | (H.rules[s].has_contributed ? 2 : 0));
+            if ((rc = upload(net, &Q.flags, fl.data(), S))) return rc;
+        }
+        if (H.kind == PROJ_ADDITIVE) {
+            if ((rc = upload(net, &Q.tau_plus, H.tau_p.data(), S))) return rc;
+            if ((rc = upload(net, &Q.tau_minus, H.tau_m.data(), S))) return rc;
+            std::vector<uint32_t> lim(S);
+            uint32_t hist = 1;
+            for (uint64_t s = 0; s < S; ++s) {
+                // push while size() < tau_minus_ + tau_plus_ (float comparison, additive_stdp.h:108)
+                const float sum = H.tau_m[s] + H.tau_p[s];
+                uint32_t l = 0;
+                while ((float)l < sum && l < 256) ++l;
+                lim[s] = l;
+                hist = std::max(hist, l);
+            }
+            if ((rc = upload(net, &Q.lim, lim.data(), S))) return rc;
+            Q.hist = hist;
+            if ((rc = dev_alloc(net, (void **)&Q.pre_times, (size_t)S * hist * 8))) return rc;
+            if ((rc = dev_alloc(net, (void **)&Q.post_times, (size_t)S * hist * 8))) return rc;
+            if ((rc = dev_alloc(net, (void **)&Q.n_pre_t, (size_t)S * 4))) return rc;
+            if ((rc = dev_alloc(net, (void **)&Q.n_post_t, (size_t)S * 4))) return rc;
+            if (H.stdp_pops.size() > 4) return fail(KNP_ERR_UNSUPPORTED, "more than 4 stdp populations");
+            Q.n_stdp = (uint32_t)H.stdp_pops.size();
+            for (uint32_t m = 0; m < Q.n_stdp; ++m) {
+                Q.stdp_pop[m] = H.stdp_pops[m];
+                Q.stdp_mode[m] = H.stdp_modes[m];
+            }
+        }
+        Q.in_cap = std::max(H.n_pre, 1u) + H.gen_channels;
+        if ((rc = dev_alloc(net, (void **)&Q.in_list, (size_t)Q.in_cap * 4))) return rc;
+    }
+
+    // ---- scatter row sources: channel messages first, then population messages in population order
+    uint32_t nrs = 0;
+    for (uint32_t q = 0; q < n_projs; ++q) {
+        const HostProj &H = net->projs[q];
+        if (nrs >= (uint32_t)kMaxRowSources) return fail(KNP_ERR_UNSUPPORTED, "too many row sources");
+        T.rows[nrs++] = RowSource{q, 0xFFFFFFFFu, 0u, 0u};
+        for (uint32_t p = 0; p < n_pops; ++p) {
+            bool sub = (H.pre_pop == p);
+            int mode = -1;
+            for (size_t m = 0; m < H.stdp_pops.size(); ++m)
+                if (H.stdp_pops[m] == p) {
+                    sub = true;
+                    mode = H.stdp_modes[m];
+                }
+            if (!sub || mode == KNP_STDP_ONLY) continue;  // STDPOnly messages are emptied (additive_stdp.h:162-166)
+            if (nrs >= (uint32_t)kMaxRowSources) return fail(KNP_ERR_UNSUPPORTED, "too many row sources");
+            T.rows[nrs++] = RowSource{q, p, 1u + p, 0u};
+        }
+    }
+    T.n_row_sources = nrs;
+
+    // ---- sources
+    for (uint32_t s = 0; s < T.n_srcs; ++s) {
+        const HostSource &H = net->srcs[s];
+        SourceDev &S = T.srcs[s];
+        S.proj = H.proj;
+        S.n_ch = H.n_ch;
+        S.n_patterns = H.n_patterns;
+        S.steps_per_pattern = H.spp;
+        S.lo = H.lo;
+        S.hi = H.hi;
+        S.seed = H.seed;
+        uint32_t *th;
+        int rc;
+        if ((rc = upload(net, &th, H.thr.data(), H.thr.size()))) return rc;
+        S.thr = th;
+    }
+
+    // ---- phase-1 tasks
+    uint32_t blocks = 0, nt = 0;
+    for (uint32_t p = 0; p < n_pops; ++p) {
+        const uint32_t b = std::max(1u, (net->pops[p].n + kBlock - 1) / kBlock);
+        T.tasks[nt++] = TaskDev{0u, p, blocks, b};
+        blocks += b;
+    }
+    for (uint32_t s = 0; s < T.n_srcs; ++s) {
+        const uint32_t b = std::max(1u, (net->srcs[s].n_ch + kBlock - 1) / kBlock);
+        T.tasks[nt++] = TaskDev{1u, s, blocks, b};
+        blocks += b;
+    }
+    T.n_tasks = nt;
+    T.total_blocks = blocks;
+
+    // ---- control blocks
+    int rc;
+    if ((rc = dev_alloc(net, (void **)&T.ctl, 2 * sizeof(Ctl)))) return rc;
+    if ((rc = dev_alloc(net, (void **)&T.events, 8))) return rc;
+
+    // launch shapes: enough waves for the largest step, capped near 8 workgroups per CU
+    uint64_t max_rows = 0;
+    for (uint32_t r = 0; r < nrs; ++r) {
+        const RowSource &R = T.rows[r];
+        max_rows += (R.from_pop == 0xFFFFFFFFu) ? T.projs[R.proj].in_cap : net->pops[R.from_pop].n;
+    }
+    net->scatter_grid = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(2048, (max_rows + 3) / 4));
+    uint32_t max_res = 1;
+    for (const auto &P : net->pops)
+        if (P.resource) max_res = std::max(max_res, P.n);
+    net->train_grid = std::max(1u, std::min(1024u, (max_res + 3) / 4));
+
+    if ((rc = upload_working_sets(net))) return rc;
+    net->initialized = true;
+    return KNP_OK;
+}
+
+
+int upload_table(knp_net *net) {
+    if (!net->dev_table) {
+        int rc = dev_alloc(net, (void **)&net->dev_table, sizeof(NetDev), false);
+        if (rc) return rc;
+    }
+    HIP_TRY(hipMemcpyAsync(net->dev_table, &net->table, sizeof(NetDev), hipMemcpyHostToDevice, net->stream));
+    HIP_TRY(hipStreamSynchronize(net->stream));
+    return KNP_OK;
+}
+
+// Working projections of a resource population: unlocked synaptic-resource STDP projections that target it,
+// in load order (find_projection_by_type_and_postsynaptic, populations_old.h:57-83). Refreshed whenever a
+// lock changes.
+int upload_working_sets(knp_net *net) {
+    for (uint32_t p = 0; p < net->pops.size(); ++p) {
+        if (!net->pops[p].resource) continue;
+        ResDev &R = net->table.pops[p].res;
+        R.n_work = 0;
+        for (uint32_t q = 0; q < net->projs.size(); ++q) {
+            const HostProj &H = net->projs[q];
+            if (H.kind != PROJ_RESOURCE || H.post_pop != p || H.locked) continue;
+            if (R.n_work >= (uint32_t)kMaxWorking)
+                return fail(KNP_ERR_UNSUPPORTED, "more than 8 trainable projections into one population");
+            R.work[R.n_work++] = q;
+        }
+    }
+    return upload_table(net);
+}
+
+void dev_free(knp_net *net, void *p) {
+    if (!p) return;
+    auto it = std::find(net->allocations.begin(), net->allocations.end(), p);
+    if (it != net->allocations.end()) net->allocations.erase(it);
+    (void)hipFree(p);
+}
+
+hipEvent_t take_event(knp_net *net) {
+    if (net->ev_used < net->ev_pool.size()) return net->ev_pool[net->ev_used++];
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    net->ev_pool.push_back(e);
+    net->ev_used++;
+    return e;
+}
+
+template <typename Launch>
+int timed_launch(knp_net *net, int kid, Launch launch) {
+    hipEvent_t a = nullptr, b = nullptr;
+    if (net->timing) {
+        a = take_event(net);
+        if (a) HIP_TRY(hipEventRecord(a, net->stream));
+    }
+    launch();
+    HIP_TRY(hipGetLastError());
+    if (net->timing && a) {
+        b = take_event(net);
+        if (b) {
+            HIP_TRY(hipEventRecord(b, net->stream));
+            net->timed[kid].emplace_back(a, b);
+        }
+    }
+    return KNP_OK;
+}
+
+int collect_timing(knp_net *net) {
+    HIP_TRY(hipStreamSynchronize(net->stream));
+    for (int k = 0; k < KNP_KERNEL_COUNT; ++k) {
+        for (auto &pr : net->timed[k]) {
+            float ms = 0.f;
+            HIP_TRY(hipEventElapsedTime(&ms, pr.first, pr.second));
+            net->time_ms[k] += ms;
+            net->launches[k]++;
+        }
+        net->timed[k].clear();
+    }
+    net->ev_used = 0;
+    return KNP_OK;
+}
+
+int deliver_pushes(knp_net *net) {
+    const uint64_t t = net->step;
+    bool any = false;
+    for (const auto &Q : net->projs) any |= !Q.pending.empty();
+    if (!any) return KNP_OK;
+    std::vector<std::vector<uint32_t>> lists(net->projs.size());
+    for (uint32_t q = 0; q < net->projs.size(); ++q) {
+        HostProj &H = net->projs[q];
+        if (H.pending.empty()) continue;
+        auto &L = lists[q];
+        for (auto &m : H.pending) L.insert(L.end(), m.begin(), m.end());
+        H.pending.clear();
+        ProjDev &Q = net->table.projs[q];
+        if (L.size() + H.gen_channels > Q.in_cap) {
+            dev_free(net, Q.in_list);
+            Q.in_cap = (uint32_t)(L.size() + H.gen_channels);
+            int rc = dev_alloc(net, (void **)&Q.in_list, (size_t)Q.in_cap * 4);
+            if (rc) return rc;
+            if ((rc = upload_table(net))) return rc;
+        }
+        if (!L.empty())
+            HIP_TRY(hipMemcpyAsync(Q.in_list, L.data(), L.size() * 4, hipMemcpyHostToDevice, net->stream));
+        const uint32_t cnt = (uint32_t)L.size();
+        Ctl *ctl = net->table.ctl + (t & 1);
+        HIP_TRY(hipMemcpyAsync(&ctl->in_count[q], &cnt, 4, hipMemcpyHostToDevice, net->stream));
+        HIP_TRY(hipStreamSynchronize(net->stream));
+    }
+    return KNP_OK;
+}
+
+int run_steps(knp_net *net, uint64_t n_steps) {
+    if (!net->initialized) {
+        int rc = init_network(net);
+        if (rc) return rc;
+    }
+    HIP_TRY(hipSetDevice(net->device));
+    NetDev *D = net->dev_table;
+    for (uint64_t it = 0; it < n_steps; ++it) {
+        const uint64_t t = net->step;
+        int rc = deliver_pushes(net);
+        if (rc) return rc;
+        const uint32_t blocks = net->table.total_blocks;
+        if ((rc = timed_launch(net, KNP_KERNEL_NEURON, [&] {
+                 k_phase1<<<blocks, kBlock, 0, net->stream>>>(D, t);
+             })))
+            return rc;
+        for (uint32_t p = 0; p < net->pops.size(); ++p) {
+            if (!net->pops[p].resource) continue;
+            const uint32_t g = net->train_grid;
+            if ((rc = timed_launch(net, KNP_KERNEL_TRAIN, [&] {
+                     k_train<<<g, kBlock, 0, net->stream>>>(D, p, t);
+                 })))
+                return rc;
+        }
+        for (uint32_t q = 0; q < net->projs.size(); ++q) {
+            if (net->projs[q].kind != PROJ_ADDITIVE || net->projs[q].stdp_pops.empty()) continue;
+            k_additive_append<<<net->scatter_grid, kBlock, 0, net->stream>>>(D, q, t);
+            HIP_TRY(hipGetLastError());
+        }
+        const uint32_t sg = net->scatter_grid;
+        if ((rc = timed_launch(net, KNP_KERNEL_SCATTER, [&] {
+                 k_scatter<<<sg, kBlock, 0, net->stream>>>(D, t);
+             })))
+            return rc;
+        for (uint32_t q = 0; q < net->projs.size(); ++q) {
+            if (net->projs[q].kind != PROJ_ADDITIVE) continue;
+            const uint32_t g = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(4096, (net->projs[q].n_syn + kBlock - 1) / kBlock));
+            k_additive_modify<<<g, kBlock, 0, net->stream>>>(D, q);
+            HIP_TRY(hipGetLastError());
+        }
+        net->step++;
+        for (auto &P : net->pops) P.ran = true;
+        if (net->timing && net->ev_used > 8192) {
+            if ((rc = collect_timing(net))) return rc;
+        }
+    }
+    return KNP_OK;
+}
+
+int check_pop(knp_net *net, uint32_t pop) {
+    if (!net) return fail(KNP_ERR_INVALID_ARGUMENT, "null handle");
+    if (pop >= net->pops.size()) return fail(KNP_ERR_INVALID_ARGUMENT, "population index out of range");
+    return KNP_OK;
+}
+
+int check_proj(knp_net *net, uint32_t proj) {
+    if (!net) return fail(KNP_ERR_INVALID_ARGUMENT, "null handle");
+    if (proj >= net->projs.size()) return fail(KNP_ERR_INVALID_ARGUMENT, "projection index out of range");
+    return KNP_OK;
+}
+
+int ensure_init(knp_net *net) {
+    if (!net->initialized) return init_network(net);
+    return KNP_OK;
+}
+
+template <typename T>
+int download(knp_net *net, T *dst, const T *src, size_t count) {
+    if (count == 0) return KNP_OK;
+    HIP_TRY(hipMemcpyAsync(dst, src, count * sizeof(T), hipMemcpyDeviceToHost, net->stream));
+    HIP_TRY(hipStreamSynchronize(net->stream));
+    return KNP_OK;
+}
+
+int add_projection_common(knp_net *net, uint32_t kind, uint32_t pre_pop, uint32_t n_pre, uint32_t post_pop,
+                          const uint32_t *pre, const uint32_t *post, const float *weight, const uint32_t *delay,
+                          const uint8_t *output_type, uint64_t n_syn, HostProj &H) {
+    if (!net) return fail(KNP_ERR_INVALID_ARGUMENT, "null handle");
+    if (net->initialized) return fail(KNP_ERR_STATE, "projections must be loaded before the first step");
+    if (post_pop >= net->pops.size()) return fail(KNP_ERR_INVALID_ARGUMENT, "postsynaptic population out of range");
+    if (pre_pop != KNP_NO_POPULATION) {
+        if (pre_pop >= net->pops.size()) return fail(KNP_ERR_INVALID_ARGUMENT, "presynaptic population out of range");
+        n_pre = net->pops[pre_pop].n;
+    }
+    if (n_syn && (!pre || !post || !weight || !delay)) return fail(KNP_ERR_INVALID_ARGUMENT, "null synapse array");
+    const uint32_t n_post = net->pops[post_pop].n;
+    if (n_post >= (1u << 29)) return fail(KNP_ERR_UNSUPPORTED, "postsynaptic population larger than 2^29");
+    H.kind = kind;
+    H.pre_pop = pre_pop;
+    H.n_pre = n_pre;
+    H.post_pop = post_pop;
+    H.n_syn = n_syn;
+    H.pre.assign(pre, pre + n_syn);
+    H.post.assign(post, post + n_syn);
+    H.w.assign(weight, weight + n_syn);
+    H.delay.assign(delay, delay + n_syn);
+    H.type.assign(n_syn, (uint8_t)KNP_EXCITATORY);
+    if (output_type) H.type.assign(output_type, output_type + n_syn);
+    for (uint64_t s = 0; s < n_syn; ++s) {
+        if (H.pre[s] >= n_pre) return fail(KNP_ERR_INVALID_ARGUMENT, "presynaptic index out of range");
+        if (H.post[s] >= n_post) return fail(KNP_ERR_INVALID_ARGUMENT, "postsynaptic index out of range");
+        if (H.delay[s] == 0 || H.delay[s] > 255)
+            return fail(KNP_ERR_UNSUPPORTED, "synaptic delay must be in 1..255 (delay 0 is lost by the reference)");
+        if (H.type[s] > KNP_BLOCKING) return fail(KNP_ERR_UNSUPPORTED, "Unhandled synapse type.");
+        if (!std::isfinite(H.w[s])) return fail(KNP_ERR_INVALID_ARGUMENT, "non-finite weight");
+        H.max_delay = std::max(H.max_delay, H.delay[s]);
+        if (H.type[s] == KNP_BLOCKING) H.has_blocking = true;
+    }
+    return KNP_OK;
+}
+
+}  // namespace
+
+// =====================================================================================================
+// C ABI
+// =====================================================================================================
+extern "C" {
+
+void knp_blifat_defaults(knp_blifat_neuron *o) {
+    std::memset(o, 0, sizeof(*o));
+    o->n_time_steps_since_last_firing = std::numeric_limits<uint64_t>::max();
+    o->activation_threshold = 1.0;
+    o->reversal_inhibitory_potential = -0.3;
+    o->min_potential = -1.0e9;
+    o->total_blocking_period = std::numeric_limits<int64_t>::max();
+}
+
+void knp_resource_blifat_defaults(knp_resource_blifat_neuron *o) {
+    std::memset(o, 0, sizeof(*o));
+    knp_blifat_defaults(&o->base);
+    o->dopamine_plasticity_time = 1;
+    o->free_synaptic_resource = 1.f;
+    o->synaptic_resource_threshold = std::numeric_limits<float>::max();
+    o->isi_max = 1;
+    o->d_h = 1.f;
+    o->isi_status = KNP_ISI_NOT_IN_PERIOD;
+}
+
+void knp_resource_rule_defaults(knp_resource_rule *o) {
+    std::memset(o, 0, sizeof(*o));
+    o->w_max = 1.f;
+}
+
+uint32_t knp_bernoulli_hash(uint64_t seed, uint64_t step, uint32_t channel) {
+    return knp_mi355x::bernoulli_hash(seed, step, channel);
+}
+
+const char *knp_mi355x_version(void) { return "knp-mi355x 0.1 (gfx950, wave64)"; }
+
+const char *knp_mi355x_last_error(void) { return g_last_error.c_str(); }
+
+int knp_mi355x_device_count(int *count) {
+    if (!count) return fail(KNP_ERR_INVALID_ARGUMENT, "null pointer");
+    *count = 0;
+    hipError_t e = hipGetDeviceCount(count);
+    if (e != hipSuccess) {
+        *count = 0;
+        if (e == hipErrorNoDevice) return KNP_OK;
+        return fail(KNP_ERR_HIP, std::string("hipGetDeviceCount: ") + hipGetErrorString(e));
+    }
+    return KNP_OK;
+}
+
+int knp_mi355x_device_name(int device, char *buf, size_t cap) {
+    hipDeviceProp_t prop;
+    HIP_TRY(hipGetDeviceProperties(&prop, device));
+    if (buf && cap) std::snprintf(buf, cap, "%s [%s, %d CUs, %.0f GiB]", prop.name, prop.gcnArchName,
+                                  prop.multiProcessorCount, (double)prop.totalGlobalMem / (1024.0 * 1024 * 1024));
+    return KNP_OK;
+}
+
+int knp_mi355x_create(int device, knp_net **out) {
+    if (!out) return fail(KNP_ERR_INVALID_ARGUMENT, "null output pointer");
+    *out = nullptr;
+    int count = 0;
+    int rc = knp_mi355x_device_count(&count);
+    if (rc) return rc;
+    if (device < 0 || device >= count)
+        return fail(KNP_ERR_INVALID_ARGUMENT, "device " + std::to_string(device) + " not present (" +
+                                                  std::to_string(count) + " HIP devices)");
+    auto *net = new knp_net();
+    net->device = device;
+    hipError_t e = hipSetDevice(device);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&net->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+        delete net;
+        return fail(KNP_ERR_HIP, std::string("stream creation: ") + hipGetErrorString(e));
+    }
+    *out = net;
+    return KNP_OK;
+}
+
+int knp_mi355x_destroy(knp_net *net) {
+    if (!net) return KNP_OK;
+    (void)hipSetDevice(net->device);
+    if (net->stream) (void)hipStreamSynchronize(net->stream);
+    for (void *p : net->allocations) (void)hipFree(p);
+    for (hipEvent_t e : net->ev_pool) (void)hipEventDestroy(e);
+    if (net->stream) (void)hipStreamDestroy(net->stream);
+    delete net;
+    return KNP_OK;
+}
+
+int knp_mi355x_set_memory_cap(knp_net *net, uint64_t bytes) {
+    if (!net) return fail(KNP_ERR_INVALID_ARGUMENT, "null handle");
+    net->cap = bytes;
+    return KNP_OK;
+}
+
+int knp_mi355x_add_blifat_population(knp_net *net, const knp_blifat_neuron *neurons, uint32_t n, uint32_t *out_pop) {
+    if (!net) return fail(KNP_ERR_INVALID_ARGUMENT, "null handle");
+    if (net->initialized) return fail(KNP_ERR_STATE, "populations must be loaded before the first step");
+    if (n && !neurons) return fail(KNP_ERR_INVALID_ARGUMENT, "null neuron array");
+    if (net->pops.size() >= (size_t)kMaxPops) return fail(KNP_ERR_UNSUPPORTED, "too many populations");
+    HostPop P;
+    P.n = n;
+    P.nb.assign(neurons, neurons + n);
+    net->pops.push_back(std::move(P));
+    if (out_pop) *out_pop = (uint32_t)net->pops.size() - 1;
+    return KNP_OK;
+}
+
+int knp_mi355x_add_resource_blifat_population(knp_net *net, const knp_resource_blifat_neuron *neurons, uint32_t n,
+                                              uint32_t *out_pop) {
+    if (!net) return fail(KNP_ERR_INVALID_ARGUMENT, "null handle");
+    if (net->initialized) return fail(KNP_ERR_STATE, "populations must be loaded before the first step");
+    if (n && !neurons) return fail(KNP_ERR_INVALID_ARGUMENT, "null neuron array");
+    if (net->pops.size() >= (size_t)kMaxPops) return fail(KNP_ERR_UNSUPPORTED, "too many populations");
+    HostPop P;
+    P.resource = true;
+    P.n = n;
+    P.nr.assign(neurons, neurons + n);
+    P.nb.resize(n);
+    for (uint32_t i = 0; i < n; ++i) P.nb[i] = neurons[i].base;
+    net->pops.push_back(std::move(P));
+    if (out_pop) *out_pop = (uint32_t)net->pops.size() - 1;
+    return KNP_OK;
+}
+
+int knp_mi355x_add_delta_projection(knp_net *net, uint32_t pre_pop, uint32_t n_pre, uint32_t post_pop,
+                                    const uint32_t *pre, const uint32_t *post, const float *weight,
+                                    const uint32_t *delay, const uint8_t *output_type, uint64_t n_syn,
+                                    uint32_t *out_proj) {
+    HostProj H;
+    int rc = add_projection_common(net, PROJ_DELTA, pre_pop, n_pre, post_pop, pre, post, weight, delay, output_type,
+                                   n_syn, H);
+    if (rc) return rc;
+    if (net->projs.size() >= (size_t)kMaxProjs) return fail(KNP_ERR_UNSUPPORTED, "too many projections");
+    net->projs.push_back(std::move(H));
+    if (out_proj) *out_proj = (uint32_t)net->projs.size() - 1;
+    return KNP_OK;
+}
+
+int knp_mi355x_add_resource_stdp_projection(knp_net *net, uint32_t pre_pop, uint32_t n_pre, uint32_t post_pop,
+                                            const uint32_t *pre, const uint32_t *post, const float *weight,
+                                            const uint32_t *delay, const uint8_t *output_type,
+                                            const knp_resource_rule *rules, uint64_t n_syn, uint32_t *out_proj) {
+    HostProj H;
+    int rc = add_projection_common(net, PROJ_RESOURCE, pre_pop, n_pre, post_pop, pre, post, weight, delay,
+                                   output_type, n_syn, H);
+    if (rc) return rc;
+    if (!net->pops[post_pop].resource)
+        return fail(KNP_ERR_UNSUPPORTED, "a synaptic-resource STDP projection must target a resource population");
+    if (n_syn && !rules) return fail(KNP_ERR_INVALID_ARGUMENT, "null rule array");
+    if (net->projs.size() >= (size_t)kMaxProjs) return fail(KNP_ERR_UNSUPPORTED, "too many projections");
+    H.rules.assign(rules, rules + n_syn);
+    net->projs.push_back(std::move(H));
+    if (out_proj) *out_proj = (uint32_t)net->projs.size() - 1;
+    return KNP_OK;
+}
+
+int knp_mi355x_add_additive_stdp_projection(knp_net *net, uint32_t pre_pop, uint32_t n_pre, uint32_t post_pop,
+                                            const uint32_t *pre, const uint32_t *post, const float *weight,
+                                            const uint32_t *delay, const uint8_t *output_type,
+                                            const float *tau_plus, const float *tau_minus, uint64_t n_syn,
+                                            const uint32_t *stdp_pops, const uint8_t *stdp_modes,
+                                            uint32_t n_stdp_pops, uint32_t *out_proj) {
+    HostProj H;
+    int rc = add_projection_common(net, PROJ_ADDITIVE, pre_pop, n_pre, post_pop, pre, post, weight, delay,
+                                   output_type, n_syn, H);
+    if (rc) return rc;
+    if (n_syn && (!tau_plus || !tau_minus)) return fail(KNP_ERR_INVALID_ARGUMENT, "null tau array");
+    if (n_stdp_pops > 4) return fail(KNP_ERR_UNSUPPORTED, "more than 4 stdp populations");
+    for (uint32_t m = 0; m < n_stdp_pops; ++m) {
+        if (stdp_pops[m] >= net->pops.size()) return fail(KNP_ERR_INVALID_ARGUMENT, "stdp population out of range");
+        if (stdp_modes[m] > KNP_STDP_AND_SPIKE) return fail(KNP_ERR_INVALID_ARGUMENT, "bad stdp processing type");
+    }
+    if (net->projs.size() >= (size_t)kMaxProjs) return fail(KNP_ERR_UNSUPPORTED, "too many projections");
+    H.tau_p.assign(tau_plus, tau_plus + n_syn);
+    H.tau_m.assign(tau_minus, tau_minus + n_syn);
+    H.stdp_pops.assign(stdp_pops, stdp_pops + n_stdp_pops);
+    H.stdp_modes.assign(stdp_modes, stdp_modes + n_stdp_pops);
+    net->projs.push_back(std::move(H));
+    if (out_proj) *out_proj = (uint32_t)net->projs.size() - 1;
+    return KNP_OK;
+}
+
+int knp_mi355x_set_locked(knp_net *net, uint32_t proj, int locked) {
+    int rc = check_proj(net, proj);
+    if (rc) return rc;
+    net->projs[proj].locked = locked != 0;
+    if (net->initialized) return upload_working_sets(net);
+    return KNP_OK;
+}
+
+int knp_mi355x_start_learning(knp_net *net) {
+    if (!net) return fail(KNP_ERR_INVALID_ARGUMENT, "null handle");
+    for (auto &Q : net->projs) Q.locked = false;
+    if (net->initialized) return upload_working_sets(net);
+    return KNP_OK;
+}
+
+int knp_mi355x_stop_learning(knp_net *net) {
+    if (!net) return fail(KNP_ERR_INVALID_ARGUMENT, "null handle");
+    for (auto &Q : net->projs) Q.locked = true;
+    if (net->initialized) return upload_working_sets(net);
+    return KNP_OK;
+}
+
+int knp_mi355x_init(knp_net *net) {
+    if (!net) return fail(KNP_ERR_INVALID_ARGUMENT, "null handle");
+    return ensure_init(net);
+}
+
+int knp_mi355x_push_spikes(knp_net *net, uint32_t proj, const uint32_t *idx, uint32_t n) {
+    int rc = check_proj(net, proj);
+    if (rc) return rc;
+    if (n && !idx) return fail(KNP_ERR_INVALID_ARGUMENT, "null index array");
+    HostProj &H = net->projs[proj];
+    for (uint32_t i = 0; i < n; ++i)
+        if (idx[i] >= H.n_pre) return fail(KNP_ERR_INVALID_ARGUMENT, "spike index out of the presynaptic range");
+    H.pending.emplace_back(idx, idx + n);
+    return KNP_OK;
+}
+
+int knp_mi355x_add_bernoulli_source(knp_net *net, uint32_t proj, uint32_t n_channels, const uint32_t *thresholds,
+                                    uint32_t n_patterns, uint32_t steps_per_pattern, uint32_t active_lo,
+                                    uint32_t active_hi, uint64_t seed) {
+    int rc = check_proj(net, proj);
+    if (rc) return rc;
+    if (net->initialized) return fail(KNP_ERR_STATE, "sources must be added before the first step");
+    if (n_patterns == 0 || steps_per_pattern == 0 || !thresholds)
+        return fail(KNP_ERR_INVALID_ARGUMENT, "bad source shape");
+    if (n_channels > net->projs[proj].n_pre)
+        return fail(KNP_ERR_INVALID_ARGUMENT, "more channels than presynaptic indexes");
+    if (net->srcs.size() >= (size_t)kMaxSources) return fail(KNP_ERR_UNSUPPORTED, "too many sources");
+    HostSource S;
+    S.proj = proj;
+    S.n_ch = n_channels;
+    S.n_patterns = n_patterns;
+    S.spp = steps_per_pattern;
+    S.lo = active_lo;
+    S.hi = active_hi;
+    S.seed = seed;
+    S.thr.assign(thresholds, thresholds + (size_t)n_channels * n_patterns);
+    net->srcs.push_back(std::move(S));
+    net->projs[proj].gen_channels += n_channels;
+    return KNP_OK;
+}
+
+int knp_mi355x_step(knp_net *net, uint64_t n_steps) {
+    if (!net) return fail(KNP_ERR_INVALID_ARGUMENT, "null handle");
+    return run_steps(net, n_steps);
+}
+
+int knp_mi355x_get_step(knp_net *net, uint64_t *step) {
+    if (!net || !step) return fail(KNP_ERR_INVALID_ARGUMENT, "null pointer");
+    *step = net->step;
+    return KNP_OK;
+}
+
+int knp_mi355x_synchronize(knp_net *net) {
+    if (!net) return fail(KNP_ERR_INVALID_ARGUMENT, "null handle");
+    HIP_TRY(hipStreamSynchronize(net->stream));
+    return KNP_OK;
+}
+
+int knp_mi355x_record_spikes(knp_net *net, uint32_t pop, uint64_t capacity_steps) {
+    int rc = check_pop(net, pop);
+    if (rc) return rc;
+    HostPop &H = net->pops[pop];
+    H.rec_cap = capacity_steps;
+    H.rec_t0 = net->step;
+    if (!net->initialized) return KNP_OK;
+    PopDev &P = net->table.pops[pop];
+    HIP_TRY(hipStreamSynchronize(net->stream));
+    dev_free(net, P.rec);
+    P.rec = nullptr;
+    if (capacity_steps) {
+        if ((rc = dev_alloc(net, (void **)&P.rec, (size_t)capacity_steps * P.words * 4))) return rc;
+    }
+    P.rec_cap = capacity_steps;
+    P.rec_t0 = net->step;
+    return upload_table(net);
+}
+
+int knp_mi355x_read_spike_record(knp_net *net, uint32_t pop, uint32_t *words, uint64_t n_words, uint64_t *n_steps) {
+    int rc = check_pop(net, pop);
+    if (rc) return rc;
+    if (!n_steps) return fail(KNP_ERR_INVALID_ARGUMENT, "null n_steps");
+    *n_steps = 0;
+    if (!net->initialized) return KNP_OK;
+    PopDev &P = net->table.pops[pop];
+    if (!P.rec) return fail(KNP_ERR_STATE, "population is not recorded");
+    const uint64_t steps = std::min<uint64_t>(net->step - P.rec_t0, P.rec_cap);
+    const uint64_t need = steps * P.words;
+    if (n_words < need) return fail(KNP_ERR_RANGE, "spike record buffer too small");
+    if ((rc = download(net, words, P.rec, need))) return rc;
+    *n_steps = steps;
+    P.rec_t0 = net->step;
+    net->pops[pop].rec_t0 = net->step;
+    return upload_table(net);
+}
+
+int knp_mi355x_read_last_spikes(knp_net *net, uint32_t pop, uint32_t *idx, uint32_t cap, uint32_t *n) {
+    int rc = check_pop(net, pop);
+    if (rc) return rc;
+    if (!n) return fail(KNP_ERR_INVALID_ARGUMENT, "null count pointer");
+    *n = 0;
+    if (!net->initialized || net->step == 0) return KNP_OK;
+    const PopDev &P = net->table.pops[pop];
+    std::vector<uint32_t> bm(P.words);
+    if ((rc = download(net, bm.data(), P.bitmap, P.words))) return rc;
+    uint32_t k = 0;
+    for (uint32_t w = 0; w < P.words; ++w) {
+        uint32_t x = bm[w];
+        while (x) {
+            const uint32_t b = __builtin_ctz(x);
+            x &= x - 1;
+            if (idx && k < cap) idx[k] = w * 32 + b;
+            ++k;
+        }
+    }
+    *n = k;
+    return KNP_OK;
+}
+
+int knp_mi355x_read_potentials(knp_net *net, uint32_t pop, float *out, uint32_t n) {
+    int rc = check_pop(net, pop);
+    if (rc) return rc;
+    if (n > net->pops[pop].n || (n && !out)) return fail(KNP_ERR_INVALID_ARGUMENT, "bad output buffer");
+    if ((rc = ensure_init(net))) return rc;
+    return download(net, out, net->table.pops[pop].V, n);
+}
+
+static int export_blifat(knp_net *net, uint32_t pop, std::vector<knp_blifat_neuron> &out) {
+    const HostPop &H = net->pops[pop];
+    const PopDev &P = net->table.pops[pop];
+    const uint32_t n = H.n;
+    out = H.nb;
+    int rc;
+    std::vector<float> V(n);
+    if ((rc = download(net, V.data(), P.V, n))) return rc;
+    const uint64_t steps_run = net->step - H.t_first;
+    if (H.kind == POP_LEAN) {
+        std::vector<uint32_t> ls(n);
+        if ((rc = download(net, ls.data(), P.last_spike, n))) return rc;
+        for (uint32_t i = 0; i < n; ++i) {
+            auto &o = out[i];
+            o.potential = V[i];
+            if (steps_run) {
+                o.pre_impact_potential = 0.0;
+                if (ls[i] == 0xFFFFFFFFu) o.n_time_steps_since_last_firing += steps_run;  // wraps like size_t
+                else o.n_time_steps_since_last_firing = (net->step - 1) - ls[i];
+                o.total_blocking_period -= (int64_t)steps_run;
+                o.dynamic_threshold *= 0.0;
+                o.postsynaptic_trace *= 0.0;
+                o.inhibitory_conductance *= 0.0;
+            }
+        }
+        return KNP_OK;
+    }
+    std::vector<float> Vp(n), dyn(n), tr(n), g(n), dop(n);
+    std::vector<uint32_t> ts(n), bp(n);
+    std::vector<int64_t> tbp;
+    if ((rc = download(net, Vp.data(), P.Vpre, n))) return rc;
+    if ((rc = download(net, dyn.data(), P.dyn, n))) return rc;
+    if ((rc = download(net, tr.data(), P.trace, n))) return rc;
+    if ((rc = download(net, g.data(), P.g, n))) return rc;
+    if ((rc = download(net, dop.data(), P.dop, n))) return rc;
+    if ((rc = download(net, ts.data(), P.n_ts, n))) return rc;
+    if ((rc = download(net, bp.data(), P.bphase, n))) return rc;
+    if (P.track_tbp) {
+        tbp.resize(n);
+        if ((rc = download(net, tbp.data(), P.tbp, n))) return rc;
+    }
+    for (uint32_t i = 0; i < n; ++i) {
+        auto &o = out[i];
+        o.potential = V[i];
+        o.pre_impact_potential = Vp[i];
+        o.dynamic_threshold = dyn[i];
+        o.postsynaptic_trace = tr[i];
+        o.inhibitory_conductance = g[i];
+        o.dopamine_value = dop[i];
+        o.n_time_steps_since_last_firing = ts_from_device(ts[i]);
+        o.bursting_phase = bp[i];
+        o.total_blocking_period = P.track_tbp ? tbp[i] : o.total_blocking_period - (int64_t)steps_run;
+    }
+    return KNP_OK;
+}
+
+int knp_mi355x_read_blifat_neurons(knp_net *net, uint32_t pop, knp_blifat_neuron *out, uint32_t n) {
+    int rc = check_pop(net, pop);
+    if (rc) return rc;
+    if (n > net->pops[pop].n || (n && !out)) return fail(KNP_ERR_INVALID_ARGUMENT, "bad output buffer");
+    if ((rc = ensure_init(net))) return rc;
+    std::vector<knp_blifat_neuron> v;
+    if ((rc = export_blifat(net, pop, v))) return rc;
+    if (net->pops[pop].resource) {
+        std::vector<float> add(net->pops[pop].n);
+        if ((rc = download(net, add.data(), net->table.pops[pop].res.add_thr, add.size()))) return rc;
+        for (uint32_t i = 0; i < v.size(); ++i) v[i].additional_threshold = add[i];
+    }
+    std::copy(v.begin(), v.begin() + n, out);
+    return KNP_OK;
+}
+
+int knp_mi355x_read_resource_neurons(knp_net *net, uint32_t pop, knp_resource_blifat_neuron *out, uint32_t n) {
+    int rc = check_pop(net, pop);
+    if (rc) return rc;
+    if (!net->pops[pop].resource) return fail(KNP_ERR_INVALID_ARGUMENT, "not a resource population");
+    if (n > net->pops[pop].n || (n && !out)) return fail(KNP_ERR_INVALID_ARGUMENT, "bad output buffer");
+    if ((rc = ensure_init(net))) return rc;
+    const uint32_t N = net->pops[pop].n;
+    std::vector<knp_blifat_neuron> base(N);
+    if ((rc = knp_mi355x_read_blifat_neurons(net, pop, base.data(), N))) return rc;
+    const ResDev &R = net->table.pops[pop].res;
+    std::vector<float> fr(N), st(N);
+    std::vector<uint8_t> is(N), fo(N);
+    std::vector<uint64_t> ls(N), lss(N), fis(N);
+    if ((rc = download(net, fr.data(), R.free_res, N))) return rc;
+    if ((rc = download(net, st.data(), R.stability, N))) return rc;
+    if ((rc = download(net, is.data(), R.isi_status, N))) return rc;
+    if ((rc = download(net, fo.data(), R.forced_now, N))) return rc;
+    if ((rc = download(net, ls.data(), R.last_step, N))) return rc;
+    if ((rc = download(net, lss.data(), R.last_spike_step, N))) return rc;
+    if ((rc = download(net, fis.data(), R.first_isi_spike, N))) return rc;
+    for (uint32_t i = 0; i < n; ++i) {
+        out[i] = net->pops[pop].nr[i];
+        out[i].base = base[i];
+        out[i].free_synaptic_resource = fr[i];
+        out[i].stability = st[i];
+        out[i].isi_status = is[i];
+        out[i].is_being_forced = fo[i];
+        out[i].last_step = ls[i];
+        out[i].last_spike_step = lss[i];
+        out[i].first_isi_spike = fis[i];
+    }
+    return KNP_OK;
+}
+
+int knp_mi355x_read_weights(knp_net *net, uint32_t proj, float *out, uint64_t n_syn) {
+    int rc = check_proj(net, proj);
+    if (rc) return rc;
+    const HostProj &H = net->projs[proj];
+    if (n_syn != H.n_syn || (n_syn && !out)) return fail(KNP_ERR_INVALID_ARGUMENT, "bad output buffer size");
+    if ((rc = ensure_init(net))) return rc;
+    const ProjDev &Q = net->table.projs[proj];
+    if (H.kind != PROJ_DELTA) return download(net, out, Q.W, n_syn);
+    std::vector<float> wc(n_syn);
+    std::vector<uint32_t> cs(n_syn);
+    if ((rc = download(net, wc.data(), Q.w_csr, n_syn))) return rc;
+    if ((rc = download(net, cs.data(), Q.csr_syn, n_syn))) return rc;
+    for (uint64_t k = 0; k < n_syn; ++k) out[cs[k]] = wc[k];
+    return KNP_OK;
+}
+
+int knp_mi355x_read_resource_rules(knp_net *net, uint32_t proj, knp_resource_rule *out, uint64_t n_syn) {
+    int rc = check_proj(net, proj);
+    if (rc) return rc;
+    const HostProj &H = net->projs[proj];
+    if (H.kind != PROJ_RESOURCE) return fail(KNP_ERR_INVALID_ARGUMENT, "not a synaptic-resource projection");
+    if (n_syn != H.n_syn || (n_syn && !out)) return fail(KNP_ERR_INVALID_ARGUMENT, "bad output buffer size");
+    if ((rc = ensure_init(net))) return rc;
+    const ProjDev &Q = net->table.projs[proj];
+    std::vector<float> r(n_syn), wmin(n_syn), wmax(n_syn), du(n_syn);
+    std::vector<uint32_t> per(n_syn);
+    std::vector<uint64_t> lss(n_syn);
+    std::vector<uint8_t> fl(n_syn);
+    if ((rc = download(net, r.data(), Q.r, n_syn))) return rc;
+    if ((rc = download(net, wmin.data(), Q.w_min, n_syn))) return rc;
+    if ((rc = download(net, wmax.data(), Q.w_max, n_syn))) return rc;
+    if ((rc = download(net, du.data(), Q.d_u, n_syn))) return rc;
+    if ((rc = download(net, per.data(), Q.period, n_syn))) return rc;
+    if ((rc = download(net, lss.data(), Q.last_spike_step, n_syn))) return rc;
+    if ((rc = download(net, fl.data(), Q.flags, n_syn))) return rc;
+    for (uint64_t s = 0; s < n_syn; ++s) {
+        out[s].synaptic_resource = r[s];
+        out[s].w_min = wmin[s];
+        out[s].w_max = wmax[s];
+        out[s].d_u = du[s];
+        out[s].dopamine_plasticity_period = per[s];
+        out[s].last_spike_step = lss[s];
+        out[s].had_hebbian_update = fl[s] & 1;
+        out[s].has_contributed = (fl[s] >> 1) & 1;
+    }
+    return KNP_OK;
+}
+
+int knp_mi355x_get_event_count(knp_net *net, uint64_t *events) {
+    if (!net || !events) return fail(KNP_ERR_INVALID_ARGUMENT, "null pointer");
+    *events = 0;
+    if (!net->initialized) return KNP_OK;
+    unsigned long long e = 0;
+    int rc = download(net, &e, net->table.events, 1);
+    *events = e;
+    return rc;
+}
+
+int knp_mi355x_get_accumulator_bits(knp_net *net, uint32_t pop, int32_t *frac_bits) {
+    int rc = check_pop(net, pop);
+    if (rc) return rc;
+    if (!frac_bits) return fail(KNP_ERR_INVALID_ARGUMENT, "null pointer");
+    if ((rc = ensure_init(net))) return rc;
+    *frac_bits = net->pops[pop].frac_bits;
+    return KNP_OK;
+}
+
+int knp_mi355x_enable_timing(knp_net *net, int enable) {
+    if (!net) return fail(KNP_ERR_INVALID_ARGUMENT, "null handle");
+    int rc = collect_timing(net);
+    if (rc) return rc;
+    net->timing = enable != 0;
+    for (int k = 0; k < KNP_KERNEL_COUNT; ++k) {
+        net->time_ms[k] = 0;
+        net->launches[k] = 0;
+    }
+    return KNP_OK;
+}
+
+int knp_mi355x_kernel_time(knp_net *net, int kernel_id, double *avg_ms, uint64_t *launches) {
+    if (!net || kernel_id < 0 || kernel_id >= KNP_KERNEL_COUNT) return fail(KNP_ERR_INVALID_ARGUMENT, "bad kernel id");
+    int rc = collect_timing(net);
+    if (rc) return rc;
+    if (launches) *launches = net->launches[kernel_id];
+    if (avg_ms) *avg_ms = net->launches[kernel_id] ? net->time_ms[kernel_id] / net->launches[kernel_id] : 0.0;
+    return KNP_OK;
+}
+
+}  // extern "C"

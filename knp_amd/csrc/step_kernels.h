@@ -1,0 +1,600 @@
+// step_kernels.h - HIP kernels (gfx950, wave64) of one KNP network step.
+//
+// One step of the reference single-threaded backend (knp/backends/cpu/cpu-single-threaded-backend/
+// impl/backend.cpp:89-141) becomes, on the device:
+//   k_phase1   every population's fused BLIFAT update (pre-impact, impact, post-impact, threshold) for all
+//              populations at once, spike compaction (wave ballot + one atomic per wave), spike bitmaps,
+//              training candidates, plus the device Bernoulli channel sources;
+//   k_train    synaptic-resource STDP of one population (only for its candidate neurons);
+//   k_additive_append / k_additive_modify   additive STDP histories and weight updates;
+//   k_scatter  every projection's spike -> impact expansion: one wave per spiking presynaptic row, impacts
+//              added into per-population delay rings of exact int64 fixed-point accumulators.
+// Integer accumulation makes the result independent of the order in which impacts arrive, so the
+// spike trains do not depend on thread scheduling.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "device_tables.h"
+
+namespace knp_mi355x {
+
+__host__ __device__ inline uint64_t mix64(uint64_t z) {
+    // splitmix64 finalizer (Steele, Lea, Flood 2014)
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+    return z ^ (z >> 31);
+}
+
+__host__ __device__ inline uint32_t bernoulli_hash(uint64_t seed, uint64_t step, uint32_t ch) {
+    uint64_t a = mix64(seed + step * 0x9E3779B97F4A7C15ULL);
+    uint64_t b = mix64(a ^ ((uint64_t)ch * 0xD6E8FEB86659FD93ULL + 0x632BE59BD9B4E019ULL));
+    return (uint32_t)(b >> 32);
+}
+
+__device__ __forceinline__ float pf(const ParamF &p, uint32_t i) { return p.a ? p.a[i] : p.v; }
+__device__ __forceinline__ uint32_t pu(const ParamU &p, uint32_t i) { return p.a ? p.a[i] : p.v; }
+
+// n_time_steps_since_last_firing_ kept in 32 bits: 0xFFFFFFFF stands for SIZE_MAX (wraps to 0 on the next
+// increment, blifat_impl.h:34 with the default of blifat.h:53); counts saturate at 0xFFFFFFFE.
+__device__ __forceinline__ uint32_t ts_inc(uint32_t x) {
+    return x == 0xFFFFFFFFu ? 0u : (x >= 0xFFFFFFFDu ? 0xFFFFFFFEu : x + 1u);
+}
+
+__device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
+
+// Exact "sum >= threshold" when the potential before impacts is 0: the accumulator holds the exact sum
+// on a 2^-frac grid, so comparing against the smallest grid point >= thr decides exactly.
+__device__ __forceinline__ bool fixed_ge(int64_t a, float thr, float scale, float v_float) {
+    float ts = thr * scale;
+    if (fabsf(ts) < 9.0e18f) return a >= (long long)ceilf(ts);
+    return v_float >= thr;
+}
+
+__device__ __forceinline__ float fixed_to_float(int64_t a, float inv_scale) {
+    return __ll2float_rn((long long)a) * inv_scale;
+}
+
+// Wave-level compaction: lanes with `pred` append `value` to list[*counter ...] with one atomic per wave.
+__device__ __forceinline__ void wave_append(bool pred, uint32_t value, uint32_t *counter, uint32_t *list) {
+    uint64_t mask = __ballot(pred);
+    if (mask == 0) return;
+    uint32_t lane = lane_id();
+    int leader = __ffsll((unsigned long long)mask) - 1;
+    uint32_t base = 0;
+    if ((int)lane == leader) base = atomicAdd(counter, (uint32_t)__popcll(mask));
+    base = __shfl(base, leader);
+    if (pred) list[base + __popcll(mask & ((1ull << lane) - 1ull))] = value;
+}
+
+// Write the wave's 64-neuron spike mask into the bitmap words (and the record), bit i = neuron i.
+__device__ __forceinline__ void wave_bitmap(uint64_t mask, uint32_t first_neuron, const PopDev &P, uint64_t t) {
+    uint32_t lane = lane_id();
+    uint32_t w0 = first_neuron >> 5;
+    if (lane < 2 && w0 + lane < P.words) {
+        uint32_t word = lane == 0 ? (uint32_t)mask : (uint32_t)(mask >> 32);
+        P.bitmap[w0 + lane] = word;
+        if (P.rec && t >= P.rec_t0 && t - P.rec_t0 < P.rec_cap) P.rec[(t - P.rec_t0) * P.words + w0 + lane] = word;
+    }
+}
+
+// Decode a BLOCKING key (see k_scatter) into the blocking period the reference assigns:
+// total_blocking_period_ = static_cast<int64_t>(impact_value_) (blifat_impl.h:86-88).
+__device__ __forceinline__ int64_t blocking_value(const NetDev *net, uint64_t key) {
+    uint64_t k = key - 1;
+    uint32_t bid = (uint32_t)(k >> 57);
+    uint32_t pre = (uint32_t)((k >> 17) & 0x1FFFFFFFull);
+    uint32_t pos = (uint32_t)(k & 0x1FFFFull);
+    const ProjDev &Q = net->projs[net->blk_proj[bid]];
+    float w = Q.w_csr[Q.row_ptr[pre] + pos];
+    return (int64_t)w;
+}
+
+// ---- phase 1: populations + sources ----------------------------------------------------------------
+template <uint32_t KIND>
+__device__ __forceinline__ void population_step(const NetDev *__restrict__ net, uint32_t p, uint32_t local_block, uint64_t t, Ctl *ctl) {
+    const PopDev &P = net->pops[p];
+    const uint32_t i = local_block * kBlock + threadIdx.x;
+    const bool active = i < P.n;
+    const uint32_t slot = (uint32_t)(t % P.D);
+    const size_t ai = (size_t)slot * P.n + i;
+    bool spike = false;
+    bool cand = false;
+    if (active) {
+        int64_t a = 0;
+        if (P.acc_v) {
+            a = P.acc_v[ai];
+            P.acc_v[ai] = 0;
+        }
+        if (KIND == POP_LEAN) {
+            // default BLIFAT (blifat.h defaults): V_pre = 0, no decays, no refractory period, no blocking.
+            const bool can = !(P.lean_ts_max && t == P.t_first);  // step 0 rule (wrap of SIZE_MAX)
+            const float thr = pf(P.act, i) + pf(P.add_thr, i);
+            float V = fixed_to_float(a, P.inv_scale);
+            spike = can && fixed_ge(a, thr, P.scale, V);
+            if (spike) {
+                V = pf(P.V_reset, i);
+                P.last_spike[i] = (uint32_t)t;
+            }
+            const float vmin = pf(P.V_min, i);
+            if (V < vmin) V = vmin;
+            P.V[i] = V;
+        } else {
+            // pre-impact (blifat_impl.h:31-65)
+            uint32_t n_ts = ts_inc(P.n_ts[i]);
+            float dyn = P.dyn[i] * pf(P.thr_decay, i);
+            float trace = P.trace[i] * pf(P.trace_decay, i);
+            float g = P.g[i] * pf(P.g_decay, i);
+            float V = P.V[i] * pf(P.V_decay, i);
+            uint32_t bphase = P.bphase[i];
+            if (bphase == 1) {
+                if (KIND != POP_RESOURCE) bphase = 0;  // resource neurons keep the phase (blifat_impl.h:62)
+                V += pf(P.reflexive, i);
+            }
+            const float V_pre = V;
+            P.Vpre[i] = V_pre;
+            // impacts (blifat_impl.h:68-107): exact sums of this step's impacts
+            int64_t tbp = P.track_tbp ? P.tbp[i] : INT64_MAX;
+            if (P.acc_g) {
+                int64_t ag = P.acc_g[ai];
+                P.acc_g[ai] = 0;
+                if (ag) g += fixed_to_float(ag, P.inv_scale);
+            }
+            float dop_now = 0.f;
+            if (P.acc_d) {
+                int64_t ad = P.acc_d[ai];
+                P.acc_d[ai] = 0;
+                dop_now = fixed_to_float(ad, P.inv_scale);
+            }
+            uint32_t forced = 0;
+            if (P.acc_f) {
+                forced = P.acc_f[ai];
+                P.acc_f[ai] = 0;
+            }
+            if (P.acc_b) {
+                uint64_t key = P.acc_b[ai];
+                P.acc_b[ai] = 0;
+                if (key) tbp = blocking_value(net, key);
+            }
+            V = (V_pre == 0.f) ? fixed_to_float(a, P.inv_scale) : V_pre + fixed_to_float(a, P.inv_scale);
+            bool exact = (V_pre == 0.f);
+            // post-impact (blifat_impl.h:119-168)
+            if (tbp <= 0) {
+                V = V_pre;
+                exact = false;
+                a = 0;
+            }
+            if (tbp < 0) {
+                ++tbp;
+                if (tbp == 0) tbp = INT64_MAX;
+            } else if (tbp > 0) {
+                tbp -= 1;
+            }
+            if (g < 1.f) {
+                if (g != 0.f) exact = false;
+                V -= (V - pf(P.E_inh, i)) * g;
+            } else {
+                V = pf(P.E_inh, i);
+                exact = false;
+            }
+            const float add = (KIND == POP_RESOURCE) ? P.res.add_thr[i] : pf(P.add_thr, i);
+            const float thr = pf(P.act, i) + dyn + add;
+            const bool above = exact ? fixed_ge(a, thr, P.scale, V) : (V >= thr);
+            spike = (n_ts > pu(P.refr, i)) && above;
+            if (spike) {
+                dyn += pf(P.thr_inc, i);
+                trace += pf(P.trace_inc, i);
+                V = pf(P.V_reset, i);
+                bphase = pu(P.bperiod, i);
+                n_ts = 0;
+            }
+            const float vmin = pf(P.V_min, i);
+            if (V < vmin) V = vmin;
+            P.V[i] = V;
+            P.n_ts[i] = n_ts;
+            P.dyn[i] = dyn;
+            P.trace[i] = trace;
+            P.g[i] = g;
+            P.bphase[i] = bphase;
+            if (P.track_tbp) P.tbp[i] = tbp;
+            if (KIND == POP_RESOURCE) {
+                const ResDev &R = P.res;
+                R.dop_now[i] = dop_now;  // dopamine_value_ is reset every step for resource neurons
+                R.forced_now[i] = (uint8_t)(forced != 0);
+                // training candidate: spiked, has dopamine, or due for resource renormalization
+                // (the renormalization test only depends on state training leaves unchanged for others)
+                bool renorm = false;
+                const float fr = R.free_res[i];
+                if (fabsf(fr) >= pf(R.res_thr, i)) {
+                    const uint8_t st = R.isi_status[i];
+                    const bool in_isi = (t - R.last_step[i] <= (uint64_t)pu(R.isi_max, i)) && st != 0;
+                    renorm = !in_isi;
+                }
+                cand = spike || dop_now != 0.f || renorm;
+            } else if (P.acc_d) {
+                P.dop[i] += dop_now;  // plain BLIFAT keeps summing dopamine (never reset for it)
+            }
+        }
+    }
+    // spike compaction and bitmaps (whole wave participates)
+    wave_append(spike, i, &ctl->pop_count[p], P.spike_list);
+    const uint64_t mask = __ballot(spike);
+    wave_bitmap(mask, local_block * kBlock + (threadIdx.x & ~63u), P, t);
+    if (KIND == POP_RESOURCE) wave_append(cand, i, &ctl->cand_count[p], P.res.cand);
+}
+
+__device__ __forceinline__ void source_step(const NetDev *__restrict__ net, uint32_t s, uint32_t local_block,
+                                            uint64_t t, Ctl *ctl) {
+    const SourceDev &S = net->srcs[s];
+    const uint32_t c = local_block * kBlock + threadIdx.x;
+    const uint32_t phase = (uint32_t)(t % S.steps_per_pattern);
+    bool spike = false;
+    if (c < S.n_ch && phase >= S.lo && phase <= S.hi) {
+        const uint32_t pattern = (uint32_t)((t / S.steps_per_pattern) % S.n_patterns);
+        spike = bernoulli_hash(S.seed, t, c) < S.thr[(size_t)pattern * S.n_ch + c];
+    }
+    const ProjDev &Q = net->projs[S.proj];
+    wave_append(spike, c, &ctl->in_count[S.proj], Q.in_list);
+}
+
+__global__ __launch_bounds__(kBlock) void k_phase1(const NetDev *__restrict__ net, uint64_t t) {
+    Ctl *ctl = &net->ctl[t & 1];
+    if (blockIdx.x == 0) {
+        uint32_t *nxt = (uint32_t *)&net->ctl[(t + 1) & 1];
+        for (uint32_t k = threadIdx.x; k < sizeof(Ctl) / 4; k += blockDim.x) nxt[k] = 0;
+    }
+    const uint32_t b = blockIdx.x;
+    uint32_t task = 0;
+    for (uint32_t k = 0; k < net->n_tasks; ++k)
+        if (b >= net->tasks[k].block_begin) task = k;
+    const TaskDev &T = net->tasks[task];
+    const uint32_t lb = b - T.block_begin;
+    if (T.kind == 1) {
+        source_step(net, T.index, lb, t, ctl);
+        return;
+    }
+    switch (net->pops[T.index].kind) {
+        case POP_LEAN: population_step<POP_LEAN>(net, T.index, lb, t, ctl); break;
+        case POP_FULL: population_step<POP_FULL>(net, T.index, lb, t, ctl); break;
+        default: population_step<POP_RESOURCE>(net, T.index, lb, t, ctl); break;
+    }
+}
+
+// ---- scatter: spike rows -> impacts in the delay rings (delta_impl.h:54-114) -------------------------
+// BLOCKING keys order impacts as the reference delivers them (the "last" impact wins, blifat_impl.h:86-88):
+// projection load order, then creation step (later = shorter delay), then message, then presynaptic index,
+// then position in the row.  key = 1 + [blocking projection:6][64-delay:6][message rank:5][pre:29][pos:17].
+__global__ __launch_bounds__(kBlock) void k_scatter(const NetDev *__restrict__ net, uint64_t t) {
+    __shared__ uint32_t prefix[kMaxRowSources + 1];
+    const Ctl *ctl = &net->ctl[t & 1];
+    const uint32_t n_rs = net->n_row_sources;
+    if (threadIdx.x == 0) {
+        uint32_t acc = 0;
+        for (uint32_t r = 0; r < n_rs; ++r) {
+            prefix[r] = acc;
+            const RowSource &R = net->rows[r];
+            acc += (R.from_pop == 0xFFFFFFFFu) ? ctl->in_count[R.proj] : ctl->pop_count[R.from_pop];
+        }
+        prefix[n_rs] = acc;
+    }
+    __syncthreads();
+    const uint32_t total = prefix[n_rs];
+    const uint32_t lane = lane_id();
+    const uint32_t waves = gridDim.x * (kBlock / 64);
+    for (uint32_t row = blockIdx.x * (kBlock / 64) + threadIdx.x / 64; row < total; row += waves) {
+        uint32_t rs = 0;
+        while (rs + 1 < n_rs && prefix[rs + 1] <= row) ++rs;
+        const RowSource &R = net->rows[rs];
+        const uint32_t idx = row - prefix[rs];
+        const ProjDev &Q = net->projs[R.proj];
+        const uint32_t pre = (R.from_pop == 0xFFFFFFFFu) ? Q.in_list[idx] : net->pops[R.from_pop].spike_list[idx];
+        if (pre >= Q.n_pre) continue;
+        const uint64_t b = Q.row_ptr[pre], e = Q.row_ptr[pre + 1];
+        if (lane == 0 && e > b) atomicAdd(net->events, (unsigned long long)(e - b));
+        const PopDev &P = net->pops[Q.post_pop];
+        const uint32_t slot0 = (uint32_t)(t % P.D);
+        const float scale = P.scale;
+        for (uint64_t k = b + lane; k < e; k += 64) {
+            const uint32_t pt = Q.post_type[k];
+            const uint32_t post = pt & 0x1FFFFFFFu;
+            const uint32_t type = pt >> 29;
+            const uint32_t d = Q.delay[k];
+            float w;
+            if (Q.kind == PROJ_DELTA) {
+                w = Q.w_csr[k];
+            } else {
+                const uint32_t s = Q.csr_syn[k];
+                w = Q.W[s];
+                if (Q.kind == PROJ_RESOURCE) Q.last_spike_step[s] = t;  // init_synapse (synaptic_resource_stdp.h:36-41)
+            }
+            uint32_t slot = slot0 + d;
+            if (slot >= P.D) slot -= P.D;
+            const size_t ai = (size_t)slot * P.n + post;
+            const long long v = __float2ll_rn(w * scale);
+            switch (type) {
+                case 0:  // EXCITATORY
+                    atomicAdd((unsigned long long *)&P.acc_v[ai], (unsigned long long)v);
+                    if (Q.forcing && P.acc_f) atomicOr(&P.acc_f[ai], 1u);
+                    break;
+                case 1:  // INHIBITORY_CURRENT
+                    atomicAdd((unsigned long long *)&P.acc_v[ai], (unsigned long long)(-v));
+                    break;
+                case 2:  // INHIBITORY_CONDUCTANCE
+                    atomicAdd((unsigned long long *)&P.acc_g[ai], (unsigned long long)v);
+                    break;
+                case 3:  // DOPAMINE
+                    atomicAdd((unsigned long long *)&P.acc_d[ai], (unsigned long long)v);
+                    break;
+                default: {  // BLOCKING
+                    const uint64_t key = 1ull + (((uint64_t)Q.blocking_id << 57) | ((uint64_t)(64u - d) << 51) |
+                                                 ((uint64_t)R.rank << 46) | ((uint64_t)pre << 17) | (uint64_t)(k - b));
+                    atomicMax((unsigned long long *)&P.acc_b[ai], (unsigned long long)key);
+                    break;
+                }
+            }
+        }
+    }
+}
+
+// ---- synaptic-resource STDP (blifat_stdp.h:35-172, training/stdp.h:40-186) ---------------------------
+__device__ __forceinline__ float recalc_weight(float r, float w_min, float w_max) {
+    // recalculate_synapse_weights (training/stdp.h:40-53)
+    const float syn_w = r > 0.f ? r : 0.f;
+    const float weight_diff = w_max - w_min;
+    return w_min + weight_diff * syn_w / (weight_diff + syn_w);
+}
+
+__device__ __forceinline__ float stability_factor(float stability) {
+    // std::min(static_cast<float>(std::pow(2, -stability)), 1.F)
+    const float v = exp2f(-stability);
+    return v < 1.f ? v : 1.f;
+}
+
+__device__ __forceinline__ bool in_interval(uint64_t b, uint64_t e, uint64_t p) {
+    if (b > e) {
+        uint64_t tmp = b;
+        b = e;
+        e = tmp;
+    }
+    return b <= p && p <= e;
+}
+
+__device__ __forceinline__ long long wave_sum_i64(long long v) {
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+    return v;
+}
+
+// The reference updates free_synaptic_resource_ once per synapse in synapse order; the float result
+// depends on that order, so one lane replays it: free += d_u[l]; if contributing: free += delta.
+__device__ __forceinline__ float serial_free(float freer, uint64_t active, uint64_t contrib, const float *du_lds,
+                                             float delta, bool use_du) {
+    if (lane_id() == 0) {
+        uint64_t m = use_du ? active : contrib;
+        while (m) {
+            const int l = __ffsll((unsigned long long)m) - 1;
+            m &= m - 1;
+            if (use_du) freer += du_lds[l];
+            if ((contrib >> l) & 1ull) freer += delta;
+        }
+    }
+    return __shfl(freer, 0);
+}
+
+__global__ __launch_bounds__(kBlock) void k_train(const NetDev *__restrict__ net, uint32_t p, uint64_t t) {
+    __shared__ float du_lds[kBlock / 64][64];
+    const PopDev &P = net->pops[p];
+    const ResDev &R = P.res;
+    const Ctl *ctl = &net->ctl[t & 1];
+    const uint32_t count = ctl->cand_count[p];
+    const uint32_t lane = lane_id();
+    const uint32_t wv = threadIdx.x / 64;
+    float *du_w = du_lds[wv];
+    const uint32_t waves = gridDim.x * (kBlock / 64);
+    for (uint32_t c = blockIdx.x * (kBlock / 64) + wv; c < count; c += waves) {
+        const uint32_t n = R.cand[c];
+        const bool spiked = (P.bitmap[n >> 5] >> (n & 31)) & 1u;
+        const float dop = R.dop_now[n];
+        const bool forced = R.forced_now[n] != 0;
+        uint32_t status = R.isi_status[n];
+        uint64_t last_step = R.last_step[n], lss = R.last_spike_step[n], fis = R.first_isi_spike[n];
+        float stab = R.stability[n];
+        float freer = R.free_res[n];
+        const uint32_t isi_max = pu(R.isi_max, n);
+        // ---- process_spiking_neurons_impl
+        if (spiked) {
+            lss = t;
+            if (forced) {
+                status = 0;  // is_forced
+            } else {
+                switch (status) {
+                    case 3:  // not_in_period
+                    case 0:  // is_forced
+                        status = 1;
+                        fis = t;
+                        break;
+                    case 1:  // period_started
+                        if (last_step - t < (uint64_t)isi_max) status = 2;
+                        break;
+                    case 2:  // period_continued
+                        if (last_step - t >= (uint64_t)isi_max || dop != 0.f) {
+                            status = 1;
+                            fis = t;
+                        }
+                        break;
+                    default: break;
+                }
+                last_step = t;
+            }
+            if (status == 1) stab -= pf(R.change_at_isi, n);
+            const float d_h = pf(R.d_h, n) * stability_factor(stab);
+            long long sum = 0;
+            for (uint32_t j = 0; j < R.n_work; ++j) {
+                const ProjDev &Q = net->projs[R.work[j]];
+                const uint64_t cb = Q.col_ptr[n], ce = Q.col_ptr[n + 1];
+                for (uint64_t base = cb; base < ce; base += 64) {
+                    const uint64_t k = base + lane;
+                    const bool act = k < ce;
+                    bool contributing = false;
+                    float du = 0.f;
+                    if (act) {
+                        const uint32_t s = Q.col_syn[k];
+                        const float w = Q.W[s];
+                        if (w > 0.f) sum += __float2ll_rn(w * P.scale);
+                        uint8_t fl = Q.flags[s];
+                        bool hc = fl & 2, hh = fl & 1;
+                        const uint64_t per = Q.period[s];
+                        const bool had_spike = in_interval(t - per, t, Q.last_spike_step[s] + Q.delay_syn[s] - 1);
+                        if (status != 2 || had_spike) hc = had_spike;
+                        if (status != 2) hh = false;
+                        float r = Q.r[s];
+                        if (status != 0) {
+                            du = Q.d_u[s];
+                            r -= du;
+                            if (hc && !hh) {
+                                r += d_h;
+                                hh = true;
+                                contributing = true;
+                            }
+                            Q.r[s] = r;
+                        }
+                        Q.W[s] = recalc_weight(r, Q.w_min[s], Q.w_max[s]);
+                        Q.flags[s] = (uint8_t)((hc ? 2 : 0) | (hh ? 1 : 0));
+                    }
+                    if (status != 0) {
+                        du_w[lane] = du;
+                        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+                        __builtin_amdgcn_wave_barrier();
+                        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+                        const uint64_t am = __ballot(act), cm = __ballot(contributing);
+                        freer = serial_free(freer, am, cm, du_w, -d_h, true);
+                        __builtin_amdgcn_wave_barrier();
+                    }
+                }
+            }
+            sum = wave_sum_i64(sum);
+            if (lane == 0) R.add_thr[n] = fixed_to_float(sum, P.inv_scale) * pf(R.coeff, n);
+        }
+        // ---- do_dopamine_plasticity_impl
+        if (dop > 0.f || (dop < 0.f && status != 0)) {
+            const bool time_ok = (t - lss) <= (uint64_t)pu(R.dpt, n);
+            const float rc = dop * stability_factor(stab);
+            for (uint32_t j = 0; j < R.n_work; ++j) {
+                const ProjDev &Q = net->projs[R.work[j]];
+                const uint64_t cb = Q.col_ptr[n], ce = Q.col_ptr[n + 1];
+                for (uint64_t base = cb; base < ce; base += 64) {
+                    const uint64_t k = base + lane;
+                    bool contributing = false;
+                    if (k < ce) {
+                        const uint32_t s = Q.col_syn[k];
+                        float r = Q.r[s];
+                        if (time_ok && (Q.flags[s] & 2)) {
+                            r += rc;
+                            Q.r[s] = r;
+                            contributing = true;
+                        }
+                        Q.W[s] = recalc_weight(r, Q.w_min[s], Q.w_max[s]);
+                    }
+                    const uint64_t cm = __ballot(contributing);
+                    freer = serial_free(freer, cm, cm, du_w, -rc, false);
+                }
+            }
+            if (forced || dop < 0.f) {
+                stab = fmaf(-dop, pf(R.change_param, n), stab);
+                stab = stab > 0.f ? stab : 0.f;
+            } else {
+                const float difference = (float)(t - fis - (uint64_t)isi_max);
+                float f = 2.f - difference / (float)isi_max;
+                f = f > -1.f ? f : -1.f;
+                stab = stab + pf(R.change_param, n) * dop * f;
+            }
+        }
+        // ---- renormalize_resource
+        const bool in_isi = (t - last_step <= (uint64_t)isi_max) && status != 0;
+        if (!in_isi && fabsf(freer) >= pf(R.res_thr, n)) {
+            uint64_t k_total = 0;
+            for (uint32_t j = 0; j < R.n_work; ++j) {
+                const ProjDev &Q = net->projs[R.work[j]];
+                k_total += Q.col_ptr[n + 1] - Q.col_ptr[n];
+            }
+            const float add = freer / (float)(k_total + (uint64_t)pu(R.drain, n));
+            for (uint32_t j = 0; j < R.n_work; ++j) {
+                const ProjDev &Q = net->projs[R.work[j]];
+                const uint64_t cb = Q.col_ptr[n], ce = Q.col_ptr[n + 1];
+                for (uint64_t k = cb + lane; k < ce; k += 64) {
+                    const uint32_t s = Q.col_syn[k];
+                    const float r = Q.r[s] + add;
+                    Q.r[s] = r;
+                    Q.W[s] = recalc_weight(r, Q.w_min[s], Q.w_max[s]);
+                }
+            }
+            freer = 0.f;
+        }
+        if (lane == 0) {
+            R.isi_status[n] = (uint8_t)status;
+            R.last_step[n] = last_step;
+            R.last_spike_step[n] = lss;
+            R.first_isi_spike[n] = fis;
+            R.stability[n] = stab;
+            R.free_res[n] = freer;
+        }
+    }
+}
+
+// ---- additive STDP (additive_stdp.h:93-196) ----------------------------------------------------------
+// init_projection: spikes of the projection's stdp_populations_ append the step to the spike histories of
+// every synapse whose POSTsynaptic index equals the spiking neuron's index (additive_stdp.h:149-160).
+__global__ __launch_bounds__(kBlock) void k_additive_append(const NetDev *__restrict__ net, uint32_t q, uint64_t t) {
+    const ProjDev &Q = net->projs[q];
+    const Ctl *ctl = &net->ctl[t & 1];
+    const uint32_t lane = lane_id();
+    const uint32_t waves = gridDim.x * (kBlock / 64);
+    const uint32_t wave = blockIdx.x * (kBlock / 64) + threadIdx.x / 64;
+    const uint32_t n_post = net->pops[Q.post_pop].n;
+    for (uint32_t m = 0; m < Q.n_stdp; ++m) {
+        const uint32_t sp = Q.stdp_pop[m];
+        const uint32_t cnt = ctl->pop_count[sp];
+        const uint32_t *list = net->pops[sp].spike_list;
+        for (uint32_t r = wave; r < cnt; r += waves) {
+            const uint32_t nidx = list[r];
+            if (nidx >= n_post) continue;
+            const uint64_t cb = Q.col_ptr[nidx], ce = Q.col_ptr[nidx + 1];
+            for (uint64_t k = cb + lane; k < ce; k += 64) {
+                const uint32_t s = Q.col_syn[k];
+                const uint32_t lim = Q.lim[s];
+                uint32_t old = atomicAdd(&Q.n_post_t[s], 1u);
+                if (old < lim) Q.post_times[(size_t)s * Q.hist + old] = t;
+                else atomicSub(&Q.n_post_t[s], 1u);
+                if (Q.stdp_mode[m] == KNP_STDP_AND_SPIKE) {
+                    old = atomicAdd(&Q.n_pre_t[s], 1u);
+                    if (old < lim) Q.pre_times[(size_t)s * Q.hist + old] = t;
+                    else atomicSub(&Q.n_pre_t[s], 1u);
+                }
+            }
+        }
+    }
+}
+
+// modify_weights: synapses whose two histories reached tau_plus_ + tau_minus_ get
+// w += sum_pre sum_post stdp_w(t_post - t_pre) and their histories cleared (additive_stdp.h:173-196).
+__global__ __launch_bounds__(kBlock) void k_additive_modify(const NetDev *__restrict__ net, uint32_t q) {
+    const ProjDev &Q = net->projs[q];
+    for (uint64_t s = blockIdx.x * (uint64_t)kBlock + threadIdx.x; s < Q.n_syn; s += (uint64_t)gridDim.x * kBlock) {
+        const float tp = Q.tau_plus[s], tm = Q.tau_minus[s];
+        const float period = tp + tm;
+        const uint32_t np = Q.n_pre_t[s], nq = Q.n_post_t[s];
+        if ((float)np >= period && (float)nq >= period) {
+            float w_j = 0.f;
+            for (uint32_t a = 0; a < np; ++a) {
+                const uint64_t tf = Q.pre_times[s * Q.hist + a];
+                for (uint32_t b2 = 0; b2 < nq; ++b2) {
+                    const float diff = (float)(Q.post_times[s * Q.hist + b2] - tf);
+                    w_j += diff > 0.f ? expf(-diff / tp) : expf(diff / tm);
+                }
+            }
+            Q.W[s] += w_j;
+            Q.n_pre_t[s] = 0;
+            Q.n_post_t[s] = 0;
+        }
+    }
+}
+
+}  // namespace knp_mi355x

@@ -1,0 +1,207 @@
+"""Seeded synthetic networks for the benchmark configurations and tests.
+
+A `NetworkSpec` is an ordered recipe (populations, projections, channel sources, learning switch) that can be
+loaded into any target exposing the backend loading methods (`MI355XBackend`, or the CPU oracle in tests), so
+both sides see the same network, in the same load order, with the same synapse-index order.
+
+Synapse index orders follow the reference connectors:
+  all_to_all      index -> (pre = index % n_pre, post = index / n_pre)
+                  (knp/base-framework/include/knp/framework/projection/synapse_generators.h:58-73)
+  FixedNumberPre  index -> (pre random, post = index % n_post)   (synapse_generators.h:409-459, seeded here)
+  one_to_one      index -> (index, index)
+The reference connectors draw from std::random_device; these use fixed numpy seeds instead.
+
+Bernoulli channels: channel c fires at step t iff hash(seed, t, c) < floor(p * 2^32) (counter-based, the same
+definition on host and device), so the GPU and the CPU oracle receive identical input spikes.
+"""
+import ctypes
+
+import numpy as np
+
+from . import abi
+
+
+def bernoulli_threshold(p):
+    """uint32 threshold for probability p (p <= 0 -> never, p >= 1 -> always except hash 0xFFFFFFFF)."""
+    p = np.clip(np.asarray(p, dtype=np.float64), 0.0, 1.0)
+    return np.minimum(np.floor(p * 4294967296.0), 4294967295.0).astype(np.uint32)
+
+
+class NetworkSpec:
+    def __init__(self, name):
+        self.name = name
+        self.ops = []
+        self.pop_sizes = []
+        self.proj_sizes = []
+        self.learning = False
+        self.meta = {}
+
+    # recipe construction (returns indexes in load order, like the targets do)
+    def add_blifat_population(self, neurons):
+        self.ops.append(("add_blifat_population", (neurons,), {}))
+        self.pop_sizes.append(len(neurons))
+        return len(self.pop_sizes) - 1
+
+    def add_resource_blifat_population(self, neurons):
+        self.ops.append(("add_resource_blifat_population", (neurons,), {}))
+        self.pop_sizes.append(len(neurons))
+        return len(self.pop_sizes) - 1
+
+    def add_delta_projection(self, *args, **kw):
+        self.ops.append(("add_delta_projection", args, kw))
+        self.proj_sizes.append(len(args[3]))
+        return len(self.proj_sizes) - 1
+
+    def add_resource_stdp_projection(self, *args, **kw):
+        self.ops.append(("add_resource_stdp_projection", args, kw))
+        self.proj_sizes.append(len(args[3]))
+        return len(self.proj_sizes) - 1
+
+    def add_additive_stdp_projection(self, *args, **kw):
+        self.ops.append(("add_additive_stdp_projection", args, kw))
+        self.proj_sizes.append(len(args[3]))
+        return len(self.proj_sizes) - 1
+
+    def add_bernoulli_source(self, *args, **kw):
+        self.ops.append(("add_bernoulli_source", args, kw))
+
+    def start_learning(self):
+        self.learning = True
+
+    def load_into(self, target):
+        for name, args, kw in self.ops:
+            getattr(target, name)(*args, **kw)
+        if self.learning:
+            target.start_learning()
+        return target
+
+    @property
+    def n_synapses(self):
+        return int(sum(self.proj_sizes))
+
+
+def _uniform_f32(rng, n, hi):
+    """U(0, hi) float32: 24-bit uniform mantissa grid times float32(hi)."""
+    return rng.random(n, dtype=np.float32) * np.float32(hi)
+
+
+def plain_small(seed=1, n=1000, n_syn=100_000, w_max=0.1, max_delay=5, p_in=0.02):
+    """Config 0: 1,000 default BLIFAT neurons, 100,000 random recurrent DeltaSynapses (pre/post i.i.d.
+    uniform, w ~ U(0, 0.1), delay ~ U{1..5}, excitatory), 1,000 Bernoulli(0.02) channels wired one-to-one
+    with weight 1.0 and delay 1."""
+    rng = np.random.default_rng(seed)
+    spec = NetworkSpec("plain_small")
+    pop = spec.add_blifat_population(abi.neuron_array(abi.blifat_defaults(), n))
+    pre = rng.integers(0, n, n_syn, dtype=np.uint32)
+    post = rng.integers(0, n, n_syn, dtype=np.uint32)
+    w = _uniform_f32(rng, n_syn, w_max)
+    d = rng.integers(1, max_delay + 1, n_syn, dtype=np.uint32)
+    idx = np.arange(n, dtype=np.uint32)
+    inp = spec.add_delta_projection(abi.KNP_NO_POPULATION, n, pop, idx, idx, np.ones(n, np.float32),
+                                    np.ones(n, np.uint32))
+    rec = spec.add_delta_projection(pop, n, pop, pre, post, w, d)
+    spec.add_bernoulli_source(inp, bernoulli_threshold(np.full(n, p_in)), seed=seed * 7919 + 17)
+    spec.meta.update(pop=pop, input_proj=inp, recurrent_proj=rec, steps=1000, events_per_step_hint=2000)
+    return spec
+
+
+def plain_gpu_scale(seed=2, n=100_000, in_degree=1000, exc_frac=0.8, w_exc=0.02, w_inh=0.08, max_delay=20,
+                    p_in=0.005):
+    """Config 1: 100,000 BLIFAT neurons (first 80% excitatory, last 20% inhibitory, the type carried by
+    the outgoing synapses), fixed in-degree 1,000 (FixedNumberPre order, sources with replacement),
+    exc w ~ U(0, 0.02) EXCITATORY, inh w ~ U(0, 0.08) INHIBITORY_CURRENT, delay ~ U{1..20}; external
+    Bernoulli(0.005) drive one-to-one with weight 1.0, delay 1. 10,000 steps."""
+    rng = np.random.default_rng(seed)
+    spec = NetworkSpec("plain_gpu_scale")
+    pop = spec.add_blifat_population(abi.neuron_array(abi.blifat_defaults(), n))
+    S = n * in_degree
+    post = (np.arange(S, dtype=np.uint64) % n).astype(np.uint32)
+    pre = rng.integers(0, n, S, dtype=np.uint32)
+    n_exc = int(round(exc_frac * n))
+    inh = pre >= n_exc
+    otype = inh.astype(np.uint8) * np.uint8(abi.INHIBITORY_CURRENT)
+    w = _uniform_f32(rng, S, 1.0)
+    w *= np.where(inh, np.float32(w_inh), np.float32(w_exc)).astype(np.float32)
+    d = rng.integers(1, max_delay + 1, S, dtype=np.uint32)
+    idx = np.arange(n, dtype=np.uint32)
+    inp = spec.add_delta_projection(abi.KNP_NO_POPULATION, n, pop, idx, idx, np.ones(n, np.float32),
+                                    np.ones(n, np.uint32))
+    rec = spec.add_delta_projection(pop, n, pop, pre, post, w, d, otype)
+    spec.add_bernoulli_source(inp, bernoulli_threshold(np.full(n, p_in)), seed=seed * 7919 + 17)
+    spec.meta.update(pop=pop, input_proj=inp, recurrent_proj=rec, steps=10_000)
+    return spec
+
+
+def blob_images(rng, n_images, size=28):
+    """Synthetic uint8 images: a Gaussian blob (random centre, sigma in [2, 5], peak 255) plus U{0..30}
+    noise, clipped to 255."""
+    yy, xx = np.mgrid[0:size, 0:size].astype(np.float64)
+    cy = rng.uniform(0, size - 1, n_images)
+    cx = rng.uniform(0, size - 1, n_images)
+    sig = rng.uniform(2.0, 5.0, n_images)
+    blob = 255.0 * np.exp(-((yy[None] - cy[:, None, None]) ** 2 + (xx[None] - cx[:, None, None]) ** 2)
+                          / (2.0 * sig[:, None, None] ** 2))
+    noise = rng.integers(0, 31, (n_images, size, size))
+    return np.clip(np.rint(blob) + noise, 0, 255).astype(np.uint8)
+
+
+def resource_from_weight(weight, w_min, w_max):
+    """Inverse of recalculate_synapse_weights (as examples/mnist-learn/models/resource_from_weight.h:50-61):
+    the synaptic resource whose weight is `weight`."""
+    diff = np.float64(w_max) - np.float64(w_min)
+    over = np.asarray(weight, dtype=np.float64) - np.float64(w_min)
+    return (over * diff / (diff - over)).astype(np.float32)
+
+
+def stdp_learning(seed=3, n_images=2000, steps_per_image=20, n_in=784, n_out=1024, w0=0.05, p_max=0.1,
+                  lateral_w=0.5, dopamine_p=0.1, dopamine_w=1.0, potential_decay=2.0 / 3.0, synapse_period=10):
+    """Config 2: 784 image-rate Bernoulli channels (p = 0.1 * pixel / 255, 20 steps per image) -> dense
+    all-to-all SynapticResourceSTDPDeltaSynapse 784 -> 1,024 (initial w ~ U(0, 0.05) encoded as resources,
+    delay 1, unlocked) onto SynapticResourceSTDP BLIFAT neurons; lateral all-to-all 1,024 x 1,024
+    INHIBITORY_CURRENT Delta w = 0.5, delay 1; one dopamine channel firing with p = 0.1 on the last step of
+    every image, broadcast 1 -> 1,024 as DOPAMINE Delta (w = 1.0, delay 1).
+    Neurons use the reference defaults except potential_decay_ (2/3, so that inputs integrate over steps);
+    synapses use the rule defaults except dopamine_plasticity_period_ (10, as the mnist-learn example:
+    with the default 0 no synapse can ever count as contributing, blifat_stdp.h:58-60)."""
+    rng = np.random.default_rng(seed)
+    spec = NetworkSpec("stdp_learning")
+    base = abi.blifat_defaults(potential_decay=potential_decay)
+    pop = spec.add_resource_blifat_population(abi.neuron_array(abi.resource_blifat_defaults(base), n_out))
+    # dense input projection, all_to_all index order
+    S = n_in * n_out
+    s = np.arange(S, dtype=np.uint64)
+    pre = (s % n_in).astype(np.uint32)
+    post = (s // n_in).astype(np.uint32)
+    w = _uniform_f32(rng, S, w0)
+    rules = abi.rule_array(abi.resource_rule_defaults(dopamine_plasticity_period=synapse_period), S)
+    res = resource_from_weight(w, 0.0, 1.0)
+    # write the synaptic_resource_ field (offset 0, float32) of every rule
+    raw = (ctypes.c_uint8 * (S * ctypes.sizeof(abi.ResourceRule))).from_buffer(rules)
+    np.frombuffer(raw, dtype=np.uint8).reshape(S, ctypes.sizeof(abi.ResourceRule))[:, 0:4] = \
+        res.view(np.uint8).reshape(S, 4)
+    # weights as the reference recalculates them from the resources
+    w_rec = (np.float32(0) + np.float32(1.0) * np.maximum(res, np.float32(0)) /
+             (np.float32(1.0) + np.maximum(res, np.float32(0)))).astype(np.float32)
+    inp = spec.add_resource_stdp_projection(abi.KNP_NO_POPULATION, n_in, pop, pre, post, w_rec,
+                                            np.ones(S, np.uint32), rules)
+    # lateral inhibition, all_to_all order
+    L = n_out * n_out
+    s = np.arange(L, dtype=np.uint64)
+    lpre = (s % n_out).astype(np.uint32)
+    lpost = (s // n_out).astype(np.uint32)
+    lat = spec.add_delta_projection(pop, n_out, pop, lpre, lpost, np.full(L, lateral_w, np.float32),
+                                    np.ones(L, np.uint32), np.full(L, abi.INHIBITORY_CURRENT, np.uint8))
+    # dopamine broadcast 1 -> n_out
+    dop = spec.add_delta_projection(abi.KNP_NO_POPULATION, 1, pop, np.zeros(n_out, np.uint32),
+                                    np.arange(n_out, dtype=np.uint32), np.full(n_out, dopamine_w, np.float32),
+                                    np.ones(n_out, np.uint32), np.full(n_out, abi.DOPAMINE, np.uint8))
+    images = blob_images(rng, n_images)
+    thr = bernoulli_threshold(p_max * images.reshape(n_images, n_in).astype(np.float64) / 255.0)
+    spec.add_bernoulli_source(inp, thr, steps_per_pattern=steps_per_image, seed=seed * 7919 + 17)
+    spec.add_bernoulli_source(dop, bernoulli_threshold([dopamine_p]), steps_per_pattern=steps_per_image,
+                              active_lo=steps_per_image - 1, active_hi=steps_per_image - 1, seed=seed * 7919 + 29)
+    spec.start_learning()
+    spec.meta.update(pop=pop, input_proj=inp, lateral_proj=lat, dopamine_proj=dop,
+                     steps=n_images * steps_per_image)
+    return spec
+

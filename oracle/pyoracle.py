@@ -1,0 +1,241 @@
+"""ctypes binding of the CPU oracle (oracle/knp_oracle.c). TEST INFRASTRUCTURE ONLY.
+
+Only tests, the smoke check and the benchmark's CPU-baseline leg import this module; the product path
+(knp_amd) never does. The oracle restates the semantics of the reference single-threaded CPU backend
+(knp/backends/cpu/cpu-single-threaded-backend/impl/backend.cpp:89-141); see knp_oracle.c for the
+file-by-file citations.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+from knp_amd import abi
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIBS = {}
+
+
+def build():
+    import subprocess
+
+    subprocess.run(["make", "-C", _HERE, "-s"], check=True)
+
+
+def _load(policy):
+    if policy in _LIBS:
+        return _LIBS[policy]
+    name = {"exact": "liboracle_exact.so", "f64": "liboracle_f64.so"}[policy]
+    path = os.path.join(_HERE, "lib", name)
+    if not os.path.exists(path):
+        build()
+    lib = C.CDLL(path)
+    P = C.c_void_p
+    u32, u64, i32 = C.c_uint32, C.c_uint64, C.c_int
+    pu32, pf32, pu8, pf64 = C.POINTER(u32), C.POINTER(C.c_float), C.POINTER(C.c_uint8), C.POINTER(C.c_double)
+    sig = {
+        "ko_create": (P, []),
+        "ko_destroy": (None, [P]),
+        "ko_last_error": (C.c_char_p, [P]),
+        "ko_add_blifat_population": (i32, [P, C.POINTER(abi.BLIFATNeuron), u32]),
+        "ko_add_resource_blifat_population": (i32, [P, C.POINTER(abi.ResourceBLIFATNeuron), u32]),
+        "ko_add_delta_projection": (i32, [P, u32, u32, u32, pu32, pu32, pf32, pu32, pu8, u64]),
+        "ko_add_resource_stdp_projection": (
+            i32, [P, u32, u32, u32, pu32, pu32, pf32, pu32, pu8, C.POINTER(abi.ResourceRule), u64]),
+        "ko_add_additive_stdp_projection": (
+            i32, [P, u32, u32, u32, pu32, pu32, pf32, pu32, pu8, pf32, pf32, u64, pu32, pu8, u32]),
+        "ko_set_locked": (i32, [P, u32, i32]),
+        "ko_record_spikes": (i32, [P, u32, u64]),
+        "ko_read_spike_record": (i32, [P, u32, pu32, u64, C.POINTER(u64)]),
+        "ko_push_spikes": (i32, [P, u32, pu32, u32]),
+        "ko_add_bernoulli_source": (i32, [P, u32, u32, pu32, u32, u32, u32, u32, u64]),
+        "ko_step": (i32, [P, u64]),
+        "ko_get_step": (u64, [P]),
+        "ko_get_event_count": (u64, [P]),
+        "ko_read_potentials": (i32, [P, u32, pf64, u32]),
+        "ko_read_last_spikes": (i32, [P, u32, pu32, u32, C.POINTER(u32)]),
+        "ko_read_blifat_neurons": (i32, [P, u32, C.POINTER(abi.BLIFATNeuron), u32]),
+        "ko_read_resource_neurons": (i32, [P, u32, C.POINTER(abi.ResourceBLIFATNeuron), u32]),
+        "ko_read_weights": (i32, [P, u32, pf64, u64]),
+        "ko_read_resources": (i32, [P, u32, pf64, u64]),
+        "ko_read_resource_rules": (i32, [P, u32, C.POINTER(abi.ResourceRule), u64]),
+        "ko_bernoulli_hash": (u32, [u64, u64, u32]),
+        "ko_wreal_bytes": (i32, []),
+    }
+    for name_, (res, args) in sig.items():
+        f = getattr(lib, name_)
+        f.restype = res
+        f.argtypes = args
+    _LIBS[policy] = lib
+    return lib
+
+
+def bernoulli_hash(seed, step, channel):
+    return _load("exact").ko_bernoulli_hash(seed, step, channel)
+
+
+class Oracle:
+    """One network instance of the restated single-threaded CPU backend."""
+
+    def __init__(self, policy="exact"):
+        self.lib = _load(policy)
+        self.net = self.lib.ko_create()
+        self.pop_sizes = []
+        self.proj_sizes = []
+        self._rec_cap = {}
+
+    def close(self):
+        if self.net:
+            self.lib.ko_destroy(self.net)
+            self.net = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _chk(self, rc):
+        if rc < 0:
+            raise RuntimeError("oracle: " + self.lib.ko_last_error(self.net).decode())
+        return rc
+
+    # ---- loading ------------------------------------------------------------------------------
+    def add_blifat_population(self, neurons):
+        n = len(neurons)
+        pid = self._chk(self.lib.ko_add_blifat_population(self.net, neurons, n))
+        self.pop_sizes.append(n)
+        return pid
+
+    def add_resource_blifat_population(self, neurons):
+        n = len(neurons)
+        pid = self._chk(self.lib.ko_add_resource_blifat_population(self.net, neurons, n))
+        self.pop_sizes.append(n)
+        return pid
+
+    @staticmethod
+    def _syn(pre, post, weight, delay, otype):
+        pre = np.ascontiguousarray(pre, dtype=np.uint32)
+        post = np.ascontiguousarray(post, dtype=np.uint32)
+        weight = np.ascontiguousarray(weight, dtype=np.float32)
+        delay = np.ascontiguousarray(delay, dtype=np.uint32)
+        otype = None if otype is None else np.ascontiguousarray(otype, dtype=np.uint8)
+        return pre, post, weight, delay, otype
+
+    def add_delta_projection(self, pre_pop, n_pre, post_pop, pre, post, weight, delay, otype=None):
+        pre, post, weight, delay, otype = self._syn(pre, post, weight, delay, otype)
+        q = self._chk(self.lib.ko_add_delta_projection(
+            self.net, pre_pop, n_pre, post_pop, abi.u32p(pre), abi.u32p(post), abi.f32p(weight), abi.u32p(delay),
+            abi.u8p(otype), len(pre)))
+        self.proj_sizes.append(len(pre))
+        return q
+
+    def add_resource_stdp_projection(self, pre_pop, n_pre, post_pop, pre, post, weight, delay, rules, otype=None):
+        pre, post, weight, delay, otype = self._syn(pre, post, weight, delay, otype)
+        q = self._chk(self.lib.ko_add_resource_stdp_projection(
+            self.net, pre_pop, n_pre, post_pop, abi.u32p(pre), abi.u32p(post), abi.f32p(weight), abi.u32p(delay),
+            abi.u8p(otype), rules, len(pre)))
+        self.proj_sizes.append(len(pre))
+        return q
+
+    def add_additive_stdp_projection(self, pre_pop, n_pre, post_pop, pre, post, weight, delay, tau_plus, tau_minus,
+                                     stdp_pops=(), stdp_modes=(), otype=None):
+        pre, post, weight, delay, otype = self._syn(pre, post, weight, delay, otype)
+        tp = np.ascontiguousarray(tau_plus, dtype=np.float32)
+        tm = np.ascontiguousarray(tau_minus, dtype=np.float32)
+        sp = np.ascontiguousarray(stdp_pops, dtype=np.uint32)
+        sm = np.ascontiguousarray(stdp_modes, dtype=np.uint8)
+        q = self._chk(self.lib.ko_add_additive_stdp_projection(
+            self.net, pre_pop, n_pre, post_pop, abi.u32p(pre), abi.u32p(post), abi.f32p(weight), abi.u32p(delay),
+            abi.u8p(otype), abi.f32p(tp), abi.f32p(tm), len(pre), abi.u32p(sp), abi.u8p(sm), len(sp)))
+        self.proj_sizes.append(len(pre))
+        return q
+
+    def set_locked(self, proj, locked):
+        self._chk(self.lib.ko_set_locked(self.net, proj, 1 if locked else 0))
+
+    def start_learning(self):
+        for q in range(len(self.proj_sizes)):
+            self.set_locked(q, False)
+
+    # ---- inputs / stepping --------------------------------------------------------------------
+    def push_spikes(self, proj, idx):
+        idx = np.ascontiguousarray(idx, dtype=np.uint32)
+        self._chk(self.lib.ko_push_spikes(self.net, proj, abi.u32p(idx), len(idx)))
+
+    def add_bernoulli_source(self, proj, thresholds, steps_per_pattern=1, active_lo=0, active_hi=None, seed=0):
+        thr = np.ascontiguousarray(np.atleast_2d(thresholds), dtype=np.uint32)
+        n_patterns, n_ch = thr.shape
+        hi = steps_per_pattern - 1 if active_hi is None else active_hi
+        self._chk(self.lib.ko_add_bernoulli_source(self.net, proj, n_ch, abi.u32p(thr), n_patterns,
+                                                   steps_per_pattern, active_lo, hi, seed))
+
+    def step(self, n=1):
+        self._chk(self.lib.ko_step(self.net, n))
+
+    @property
+    def current_step(self):
+        return self.lib.ko_get_step(self.net)
+
+    @property
+    def events(self):
+        return self.lib.ko_get_event_count(self.net)
+
+    # ---- outputs ------------------------------------------------------------------------------
+    def record_spikes(self, pop, capacity_steps):
+        self._chk(self.lib.ko_record_spikes(self.net, pop, capacity_steps))
+        self._rec_cap[pop] = capacity_steps
+
+    def read_spike_record(self, pop):
+        n = self.pop_sizes[pop]
+        words_per = (n + 31) // 32
+        cap = self._rec_cap.get(pop, 0)
+        buf = np.zeros(words_per * max(cap, 1), dtype=np.uint32)
+        steps = C.c_uint64(0)
+        rc = self.lib.ko_read_spike_record(self.net, pop, abi.u32p(buf), buf.size, C.byref(steps))
+        if rc < 0:
+            raise RuntimeError("oracle: spike record larger than the read buffer")
+        return abi.unpack_bitmaps(buf, steps.value, n)
+
+    def last_spikes(self, pop):
+        n = self.pop_sizes[pop]
+        buf = np.zeros(max(n, 1), dtype=np.uint32)
+        cnt = C.c_uint32(0)
+        self._chk(self.lib.ko_read_last_spikes(self.net, pop, abi.u32p(buf), n, C.byref(cnt)))
+        return buf[: cnt.value].copy()
+
+    def potentials(self, pop):
+        n = self.pop_sizes[pop]
+        out = np.zeros(n, dtype=np.float64)
+        self._chk(self.lib.ko_read_potentials(self.net, pop, abi.f64p(out), n))
+        return out
+
+    def blifat_neurons(self, pop):
+        n = self.pop_sizes[pop]
+        out = (abi.BLIFATNeuron * n)()
+        self._chk(self.lib.ko_read_blifat_neurons(self.net, pop, out, n))
+        return out
+
+    def resource_neurons(self, pop):
+        n = self.pop_sizes[pop]
+        out = (abi.ResourceBLIFATNeuron * n)()
+        self._chk(self.lib.ko_read_resource_neurons(self.net, pop, out, n))
+        return out
+
+    def weights(self, proj):
+        n = self.proj_sizes[proj]
+        out = np.zeros(n, dtype=np.float64)
+        self._chk(self.lib.ko_read_weights(self.net, proj, abi.f64p(out), n))
+        return out
+
+    def resources(self, proj):
+        n = self.proj_sizes[proj]
+        out = np.zeros(n, dtype=np.float64)
+        self._chk(self.lib.ko_read_resources(self.net, proj, abi.f64p(out), n))
+        return out
+
+    def resource_rules(self, proj):
+        n = self.proj_sizes[proj]
+        out = (abi.ResourceRule * n)()
+        self._chk(self.lib.ko_read_resource_rules(self.net, proj, out, n))
+        return out

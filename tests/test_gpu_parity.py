@@ -1,0 +1,307 @@
+"""GPU parity tests: the HIP path (through the C ABI) against the CPU oracle on the same seeded inputs.
+
+Bar (integer/index work and default BLIFAT): spike trains bit-exact, SynapticImpact counts equal.
+Floating point: FP32 device state against the oracle's double state within the tolerances written in each
+test (north-star bar for potentials: 1e-6 relative).
+"""
+import sys
+
+import numpy as np
+import pytest
+
+from knp_amd import abi, workloads
+from knp_amd.backend import BackendError, MI355XBackend, device_count
+from oracle.pyoracle import Oracle
+
+pytestmark = pytest.mark.gpu
+
+POT_RTOL = 1e-6
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module", autouse=True)
+def need_device():
+    assert device_count() >= 1, "no HIP device visible"
+
+
+def both(spec):
+    return spec.load_into(MI355XBackend(0)), spec.load_into(Oracle("exact"))
+
+
+def rel_err(a, b):
+    a = np.asarray(a, dtype=np.float64)
+    b = np.asarray(b, dtype=np.float64)
+    return float(np.max(np.abs(a - b) / np.maximum(np.abs(b), 1e-30))) if a.size else 0.0
+
+
+def smallest(o, proj_kind="delta"):
+    pop = o.add_blifat_population(abi.neuron_array(abi.blifat_defaults(), 1))
+    inp = o.add_delta_projection(abi.KNP_NO_POPULATION, 1, pop, [0], [0], [1.0], [1])
+    o.add_delta_projection(pop, 1, pop, [0], [0], [1.1], [6])
+    return pop, inp
+
+
+def run_smallest(o, pop, inp, steps=20):
+    res = []
+    for step in range(steps):
+        if step % 5 == 0:
+            o.push_spikes(inp, [0])
+        o.step()
+        if len(o.last_spikes(pop)):
+            res.append(step)
+    return res
+
+
+EXPECTED = [1, 6, 7, 11, 12, 13, 16, 17, 18, 19]
+
+
+def test_smallest_network():
+    say("case: SmallestNetwork")
+    g = MI355XBackend(0)
+    pop, inp = smallest(g)
+    assert run_smallest(g, pop, inp) == EXPECTED
+    assert g.get_step() == 20
+
+
+def test_additive_stdp_network():
+    say("case: AdditiveSTDPBLIFATNetwork")
+    res = {}
+    for name, o in (("gpu", MI355XBackend(0)), ("oracle", Oracle("exact"))):
+        pop = o.add_blifat_population(abi.neuron_array(abi.blifat_defaults(), 1))
+        inp = o.add_additive_stdp_projection(abi.KNP_NO_POPULATION, 1, pop, [0], [0], [1.0], [1], [2.0], [2.0])
+        loop = o.add_additive_stdp_projection(pop, 1, pop, [0], [0], [1.0], [6], [1.0], [1.0],
+                                              stdp_pops=[pop], stdp_modes=[abi.STDP_AND_SPIKE])
+        res[name] = (run_smallest(o, pop, inp), o.weights(loop)[0], o.weights(inp)[0])
+    assert res["gpu"][0] == EXPECTED
+    assert res["gpu"][2] == 1.0
+    assert res["gpu"][1] == pytest.approx(res["oracle"][1], rel=1e-6)
+
+
+def resource_smallest(o):
+    neuron = abi.resource_blifat_defaults(synaptic_resource_threshold=1.0, free_synaptic_resource=2.0, isi_max=0)
+    pop = o.add_resource_blifat_population(abi.neuron_array(neuron, 1))
+    r_in = abi.rule_array(abi.resource_rule_defaults(synaptic_resource=0.0, w_min=1.0, w_max=2.0, d_u=0.1), 1)
+    r_loop = abi.rule_array(abi.resource_rule_defaults(synaptic_resource=0.0, w_min=1.0, w_max=2.0), 1)
+    inp = o.add_resource_stdp_projection(abi.KNP_NO_POPULATION, 1, pop, [0], [0], [1.0], [1], r_in)
+    loop = o.add_resource_stdp_projection(pop, 1, pop, [0], [0], [1.0], [6], r_loop)
+    o.start_learning()
+    return pop, inp, loop
+
+
+def test_resource_stdp_network():
+    say("case: ResourceSTDPBLIFATNetwork")
+    g, o = MI355XBackend(0), Oracle("exact")
+    gp, gi, gl = resource_smallest(g)
+    op, oi, ol = resource_smallest(o)
+    assert run_smallest(g, gp, gi) == EXPECTED
+    assert run_smallest(o, op, oi) == EXPECTED
+    for proj_g, proj_o in ((gi, oi), (gl, ol)):
+        assert np.array_equal(g.weights(proj_g), o.weights(proj_o).astype(np.float32))
+        rg, ro = g.resource_rules(proj_g)[0], o.resource_rules(proj_o)[0]
+        assert (rg.synaptic_resource, rg.last_spike_step, rg.had_hebbian_update, rg.has_contributed) == \
+            (ro.synaptic_resource, ro.last_spike_step, ro.had_hebbian_update, ro.has_contributed)
+    ng, no = g.resource_neurons(gp)[0], o.resource_neurons(op)[0]
+    for f in ("free_synaptic_resource", "stability", "isi_status", "last_step", "last_spike_step",
+              "first_isi_spike"):
+        assert getattr(ng, f) == getattr(no, f), f
+    assert g.weights(gl)[0] != 1.0
+
+
+def test_plain_small_bit_exact():
+    """Config 0 over its full 1,000 steps: spikes bit-exact, potentials within 1e-6 relative."""
+    say("case: plain_small 1000 steps")
+    spec = workloads.plain_small()
+    g, o = both(spec)
+    steps = spec.meta["steps"]
+    pop = spec.meta["pop"]
+    g.record_spikes(pop, steps)
+    o.record_spikes(pop, steps)
+    g.step(steps)
+    o.step(steps)
+    sg, so = g.read_spike_record(pop), o.read_spike_record(pop)
+    assert sg.shape == (steps, 1000)
+    assert so.sum() > 10_000
+    assert np.array_equal(sg, so), np.argwhere(sg != so)[:10]
+    assert rel_err(g.potentials(pop), o.potentials(pop)) <= POT_RTOL
+    assert g.events == o.events
+    ng, no = g.blifat_neurons(pop), o.blifat_neurons(pop)
+    assert [x.n_time_steps_since_last_firing for x in ng] == [x.n_time_steps_since_last_firing for x in no]
+    assert [x.total_blocking_period for x in ng] == [x.total_blocking_period for x in no]
+
+
+def test_determinism():
+    say("case: determinism (two identical runs)")
+    outs = []
+    for _ in range(2):
+        spec = workloads.plain_small(seed=9, n=500, n_syn=40_000)
+        g = spec.load_into(MI355XBackend(0))
+        g.record_spikes(0, 300)
+        g.step(300)
+        outs.append((g.read_spike_record(0), g.potentials(0).tobytes(), g.events))
+        g.close()
+    assert np.array_equal(outs[0][0], outs[1][0]) and outs[0][1:] == outs[1][1:]
+
+
+def random_network(o, rng, n, S, steps, full=False, blocking=False):
+    """A random recurrent network; full=True adds non-default BLIFAT parameters whose FP32 arithmetic is
+    exact (powers of two, weights on a 2^-8 grid)."""
+    if full:
+        base = abi.blifat_defaults(potential_decay=0.5, threshold_decay=0.5, threshold_increment=0.5,
+                                   inhibitory_conductance_decay=0.5, reversal_inhibitory_potential=-0.25,
+                                   absolute_refractory_period=1, bursting_period=1, reflexive_weight=0.25,
+                                   min_potential=-4.0)
+    else:
+        base = abi.blifat_defaults()
+    pop = o.add_blifat_population(abi.neuron_array(base, n))
+    ident = np.arange(n, dtype=np.uint32)
+    inp = o.add_delta_projection(abi.KNP_NO_POPULATION, n, pop, ident, ident, np.full(n, 1.25, np.float32),
+                                 np.ones(n, np.uint32))
+    pre = rng.integers(0, n, S).astype(np.uint32)
+    post = rng.integers(0, n, S).astype(np.uint32)
+    w = (rng.integers(1, 64, S) / 256.0).astype(np.float32)
+    d = rng.integers(1, 8, S).astype(np.uint32)
+    types = [abi.EXCITATORY, abi.INHIBITORY_CURRENT]
+    if full:
+        types += [abi.INHIBITORY_CONDUCTANCE, abi.DOPAMINE]
+    otype = rng.choice(types, S).astype(np.uint8)
+    o.add_delta_projection(pop, n, pop, pre, post, w, d, otype)
+    if blocking:
+        # two blocking projections racing for the same targets with different delays
+        for k in range(2):
+            m = 30
+            bp = rng.integers(0, n, m).astype(np.uint32)
+            bq = rng.integers(0, n, m).astype(np.uint32)
+            bw = (-(rng.integers(1, 4, m))).astype(np.float32)
+            bd = rng.integers(1, 4, m).astype(np.uint32)
+            o.add_delta_projection(pop, n, pop, bp, bq, bw, bd, np.full(m, abi.BLOCKING, np.uint8))
+    return pop, inp
+
+
+def drive_both(g, o, pop, inp, rng, n, steps, rate):
+    g.record_spikes(pop, steps)
+    o.record_spikes(pop, steps)
+    for t in range(steps):
+        idx = np.nonzero(rng.random(n) < rate)[0].astype(np.uint32)
+        if len(idx):
+            g.push_spikes(inp, idx)
+            o.push_spikes(inp, idx)
+        g.step()
+        o.step()
+    return g.read_spike_record(pop), o.read_spike_record(pop)
+
+
+@pytest.mark.parametrize("seed", [0, 1])
+def test_random_default_network(seed):
+    say(f"case: random default network seed={seed}")
+    rng = np.random.default_rng(100 + seed)
+    g, o = MI355XBackend(0), Oracle("exact")
+    n, S, steps = 300, 6000, 150
+    state = rng.bit_generator.state
+    pg, ig = random_network(g, rng, n, S, steps)
+    rng.bit_generator.state = state
+    po, io = random_network(o, rng, n, S, steps)
+    sg, so = drive_both(g, o, pg, ig, np.random.default_rng(seed), n, steps, 0.05)
+    assert so.sum() > 100
+    assert np.array_equal(sg, so)
+    assert np.array_equal(g.potentials(pg).astype(np.float64), o.potentials(po))
+    assert g.events == o.events
+
+
+@pytest.mark.parametrize("blocking", [False, True])
+def test_random_full_network(blocking):
+    """Non-default BLIFAT features (decays, conductance, dopamine, refractory, bursting, threshold
+    increments, blocking) on exactly representable values."""
+    say(f"case: random full-feature network blocking={blocking}")
+    rng = np.random.default_rng(7)
+    g, o = MI355XBackend(0), Oracle("exact")
+    n, S, steps = 200, 3000, 120
+    state = rng.bit_generator.state
+    pg, ig = random_network(g, rng, n, S, steps, full=True, blocking=blocking)
+    rng.bit_generator.state = state
+    po, io = random_network(o, rng, n, S, steps, full=True, blocking=blocking)
+    sg, so = drive_both(g, o, pg, ig, np.random.default_rng(3), n, steps, 0.08)
+    assert so.sum() > 50
+    assert np.array_equal(sg, so), np.argwhere(sg != so)[:10]
+    ng, no = g.blifat_neurons(pg), o.blifat_neurons(po)
+    for f in ("potential", "dynamic_threshold", "inhibitory_conductance", "dopamine_value"):
+        assert rel_err([getattr(x, f) for x in ng], [getattr(x, f) for x in no]) <= POT_RTOL, f
+    for f in ("n_time_steps_since_last_firing", "bursting_phase", "total_blocking_period"):
+        assert [getattr(x, f) for x in ng] == [getattr(x, f) for x in no], f
+    assert g.events == o.events
+
+
+def agreement(a, b):
+    inter = np.logical_and(a, b).sum()
+    union = np.logical_or(a, b).sum()
+    return 1.0 - (union - inter) / max(1, union)
+
+
+def test_stdp_learning_short():
+    """Config 2 shape, 100 images (2,000 steps): FP32 device training against the KNP-exact oracle."""
+    say("case: stdp_learning 100 images")
+    spec = workloads.stdp_learning(n_images=100)
+    g, o = both(spec)
+    steps = 2000
+    g.record_spikes(0, steps)
+    o.record_spikes(0, steps)
+    g.step(steps)
+    o.step(steps)
+    sg, so = g.read_spike_record(0), o.read_spike_record(0)
+    agr = agreement(sg, so)
+    say(f"  spikes gpu={sg.sum()} oracle={so.sum()} agreement={agr:.6f}")
+    assert so.sum() > 1000
+    assert agr >= 0.99
+    wg, wo = g.weights(0), o.weights(0)
+    say(f"  weight max abs diff {np.max(np.abs(wg - wo)):.3e}")
+    if np.array_equal(sg, so):
+        assert np.max(np.abs(wg - wo)) <= 1e-6
+
+
+def test_plain_gpu_scale_prefix():
+    """Config 1 at full size (100,000 neurons, 1e8 synapses), first 40 steps: spikes bit-exact."""
+    say("case: plain_gpu_scale 40 steps (full size)")
+    spec = workloads.plain_gpu_scale()
+    g = spec.load_into(MI355XBackend(0))
+    steps = 40
+    g.record_spikes(0, steps)
+    g.step(steps)
+    sg = g.read_spike_record(0)
+    ev_g = g.events
+    vg = g.potentials(0)
+    g.close()
+    o = spec.load_into(Oracle("exact"))
+    o.record_spikes(0, steps)
+    o.step(steps)
+    so = o.read_spike_record(0)
+    say(f"  spikes {sg.sum()} events {ev_g}")
+    assert so.sum() > 5000
+    assert np.array_equal(sg, so)
+    assert ev_g == o.events
+    assert rel_err(vg, o.potentials(0)) <= POT_RTOL
+
+
+def test_errors_are_reported():
+    say("case: error paths")
+    g = MI355XBackend(0)
+    pop = g.add_blifat_population(abi.neuron_array(abi.blifat_defaults(), 4))
+    with pytest.raises(BackendError, match="delay"):
+        g.add_delta_projection(pop, 4, pop, [0], [1], [0.5], [0])
+    with pytest.raises(BackendError, match="out of range"):
+        g.add_delta_projection(pop, 4, pop, [9], [1], [0.5], [1])
+    with pytest.raises(BackendError, match="Unhandled synapse type"):
+        g.add_delta_projection(pop, 4, pop, [0], [1], [0.5], [1], [7])
+    q = g.add_delta_projection(pop, 4, pop, [0], [1], [0.5], [1])
+    with pytest.raises(BackendError, match="out of the presynaptic range"):
+        g.push_spikes(q, [4])
+    g.step(2)
+    with pytest.raises(BackendError, match="before the first step"):
+        g.add_blifat_population(abi.neuron_array(abi.blifat_defaults(), 4))
+    h = MI355XBackend(0)
+    h.add_blifat_population(abi.neuron_array(abi.blifat_defaults(), 1000))
+    h.set_memory_cap(1024)
+    with pytest.raises(BackendError, match="cap"):
+        h.step(1)

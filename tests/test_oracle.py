@@ -1,0 +1,356 @@
+"""Pins the CPU oracle (oracle/knp_oracle.c) before it is trusted as the parity reference.
+
+Checks, in order:
+  * the reference's own known-answer tests for this path
+    (knp/tests/backend/single_thread_cpu_test.cpp:54-264): spike steps of SmallestNetwork,
+    AdditiveSTDPBLIFATNetwork and ResourceSTDPBLIFATNetwork, and that their weights change;
+  * an exact hand trace of the additive-STDP weight (additive_stdp.h:50-76, 173-196, including the
+    unsigned-wrap quirk that makes negative spike-time differences contribute 0);
+  * total-resource conservation of synaptic-resource STDP (every transfer in blifat_stdp.h:35-155 moves
+    resource between a synapse and the neuron's free pool; renormalization with drain 0 spreads all of it);
+  * the closed form of default (memoryless) BLIFAT networks against an independent numpy simulation;
+  * hand-computed single-neuron traces for decay, refractory period, conductance, blocking, threshold
+    increments and bursting (blifat_impl.h:31-168);
+  * the Bernoulli channel hash against an independent numpy implementation.
+"""
+import numpy as np
+import pytest
+
+from knp_amd import abi, workloads
+from oracle.pyoracle import Oracle, bernoulli_hash
+
+POLICIES = ["exact", "f64"]
+
+
+def smallest_network(o, neuron=None, loop_w=1.1):
+    """SmallestNetwork (single_thread_cpu_test.cpp:54-92; generators.h:48-57)."""
+    pop = o.add_blifat_population(abi.neuron_array(neuron or abi.blifat_defaults(), 1))
+    inp = o.add_delta_projection(abi.KNP_NO_POPULATION, 1, pop, [0], [0], [1.0], [1])
+    o.add_delta_projection(pop, 1, pop, [0], [0], [loop_w], [6])
+    return pop, inp
+
+
+def run_smallest(o, pop, inp, steps=20):
+    res = []
+    for step in range(steps):
+        if step % 5 == 0:
+            o.push_spikes(inp, [0])
+        o.step()
+        if len(o.last_spikes(pop)):
+            res.append(step)
+    return res
+
+
+EXPECTED = [1, 6, 7, 11, 12, 13, 16, 17, 18, 19]
+
+
+@pytest.mark.parametrize("policy", POLICIES)
+def test_smallest_network_kat(oracle_built, policy):
+    o = Oracle(policy)
+    pop, inp = smallest_network(o)
+    assert run_smallest(o, pop, inp) == EXPECTED
+
+
+@pytest.mark.parametrize("policy", POLICIES)
+def test_additive_stdp_kat_and_hand_trace(oracle_built, policy):
+    """AdditiveSTDPBLIFATNetwork (single_thread_cpu_test.cpp:95-176)."""
+    o = Oracle(policy)
+    pop = o.add_blifat_population(abi.neuron_array(abi.blifat_defaults(), 1))
+    inp = o.add_additive_stdp_projection(abi.KNP_NO_POPULATION, 1, pop, [0], [0], [1.0], [1], [2.0], [2.0])
+    loop = o.add_additive_stdp_projection(pop, 1, pop, [0], [0], [1.0], [6], [1.0], [1.0],
+                                          stdp_pops=[pop], stdp_modes=[abi.STDP_AND_SPIKE])
+    assert run_smallest(o, pop, inp) == EXPECTED
+    w = o.weights(loop)[0]
+    assert w != 1.0
+    assert o.weights(inp)[0] == 1.0
+    # Hand trace: both histories of the loop synapse receive every population spike (cap tau+ + tau- = 2);
+    # when both hold 2 times the weight grows by sum_pre sum_post stdp_w(t_post - t_pre), where
+    # t_post < t_pre wraps to a huge positive float and contributes exp(-huge) = 0.
+    f32 = np.float32
+    dt = f32 if policy == "exact" else np.float64
+    expw = dt(1.0)
+    for a, b in [(1, 6), (7, 11), (12, 13), (16, 17), (18, 19)]:
+        wj = dt(0)
+        for tf in (a, b):
+            for tn in (a, b):
+                diff = dt(float((tn - tf) % 2**64))
+                wj = dt(wj + (dt(np.exp(-diff / dt(1))) if diff > 0 else dt(np.exp(diff / dt(1)))))
+        expw = dt(expw + wj)
+    assert w == pytest.approx(float(expw), rel=1e-6)
+    assert abs(float(expw) - (11 + np.exp(-5) + np.exp(-4) + 3 * np.exp(-1))) < 1e-5
+
+
+def resource_smallest(o):
+    """ResourceSTDPBLIFATNetwork (single_thread_cpu_test.cpp:179-264)."""
+    neuron = abi.resource_blifat_defaults(synaptic_resource_threshold=1.0, free_synaptic_resource=2.0, isi_max=0)
+    pop = o.add_resource_blifat_population(abi.neuron_array(neuron, 1))
+    r_in = abi.rule_array(abi.resource_rule_defaults(synaptic_resource=0.0, w_min=1.0, w_max=2.0, d_u=0.1), 1)
+    r_loop = abi.rule_array(abi.resource_rule_defaults(synaptic_resource=0.0, w_min=1.0, w_max=2.0), 1)
+    inp = o.add_resource_stdp_projection(abi.KNP_NO_POPULATION, 1, pop, [0], [0], [1.0], [1], r_in)
+    loop = o.add_resource_stdp_projection(pop, 1, pop, [0], [0], [1.0], [6], r_loop)
+    o.start_learning()
+    return pop, inp, loop
+
+
+@pytest.mark.parametrize("policy", POLICIES)
+def test_resource_stdp_kat(oracle_built, policy):
+    o = Oracle(policy)
+    pop, inp, loop = resource_smallest(o)
+    assert run_smallest(o, pop, inp) == EXPECTED
+    assert o.weights(loop)[0] != 1.0
+
+
+def test_resource_conservation(oracle_built):
+    """free + sum of synaptic resources is invariant under resource-STDP training with drain 0."""
+    o = Oracle("f64")
+    pop, inp, loop = resource_smallest(o)
+    total0 = 2.0
+    for step in range(40):
+        if step % 5 == 0:
+            o.push_spikes(inp, [0])
+        o.step()
+        n = o.resource_neurons(pop)[0]
+        total = n.free_synaptic_resource + o.resources(inp)[0] + o.resources(loop)[0]
+        assert total == pytest.approx(total0, abs=1e-6)
+
+
+def test_locked_projection_does_not_train(oracle_built):
+    """Projections are locked by default (projection.h:432): without start_learning nothing trains."""
+    o = Oracle("exact")
+    neuron = abi.resource_blifat_defaults(free_synaptic_resource=2.0)
+    pop = o.add_resource_blifat_population(abi.neuron_array(neuron, 1))
+    r = abi.rule_array(abi.resource_rule_defaults(synaptic_resource=0.5, w_min=0.0, w_max=2.0), 1)
+    inp = o.add_resource_stdp_projection(abi.KNP_NO_POPULATION, 1, pop, [0], [0], [1.5], [1], r)
+    for step in range(10):
+        o.push_spikes(inp, [0])
+        o.step()
+    assert o.weights(inp)[0] == np.float32(1.5)
+    assert o.resources(inp)[0] == np.float32(0.5)
+
+
+def memoryless_reference(n, steps, syn, inputs):
+    """Independent simulation of default BLIFAT networks: V_t = sum of impacts arriving at t, spike iff
+    t > 0 and V_t >= 1 (activation threshold), V reset to 0, clamp at -1e9. Impacts from a spike at step s
+    through a synapse of delay d arrive at step s + d."""
+    pre, post, w, d, sign = syn
+    in_idx, in_w = inputs  # per step: list of channel indexes (one-to-one weight in_w, delay 1)
+    spikes = np.zeros((steps, n), dtype=bool)
+    V = np.zeros((steps, n))
+    for t in range(steps):
+        acc = np.zeros(n)
+        if t >= 1:
+            for c in in_idx[t - 1]:
+                acc[c] += in_w
+        for dd in np.unique(d):
+            if t - dd < 0:
+                continue
+            m = (d == dd) & spikes[t - dd][pre]
+            np.add.at(acc, post[m], sign[m] * w[m])
+        s = (acc >= 1.0) & (t > 0)
+        spikes[t] = s
+        V[t] = np.where(s, 0.0, np.maximum(acc, -1e9))
+    return spikes, V
+
+
+@pytest.mark.parametrize("seed", [0, 1, 2])
+def test_memoryless_closed_form(oracle_built, seed):
+    rng = np.random.default_rng(seed)
+    n, S, steps = 40, 600, 60
+    pre = rng.integers(0, n, S)
+    post = rng.integers(0, n, S)
+    # weights on a 2^-12 grid keep every double sum exact, so the decision is order-free on both sides
+    w = (rng.integers(1, 400, S) / 4096.0).astype(np.float32)
+    d = rng.integers(1, 6, S)
+    otype = (rng.random(S) < 0.3).astype(np.uint8)  # 30% INHIBITORY_CURRENT
+    sign = np.where(otype == 1, -1.0, 1.0)
+    in_idx = [np.unique(rng.integers(0, n, rng.integers(0, 6))).astype(np.uint32) for _ in range(steps)]
+    o = Oracle("exact")
+    pop = o.add_blifat_population(abi.neuron_array(abi.blifat_defaults(), n))
+    ident = np.arange(n, dtype=np.uint32)
+    inp = o.add_delta_projection(abi.KNP_NO_POPULATION, n, pop, ident, ident, np.ones(n, np.float32),
+                                 np.ones(n, np.uint32))
+    o.add_delta_projection(pop, n, pop, pre, post, w, d, otype)
+    o.record_spikes(pop, steps)
+    Vs = []
+    for t in range(steps):
+        if len(in_idx[t]):
+            o.push_spikes(inp, in_idx[t])
+        o.step()
+        Vs.append(o.potentials(pop))
+    got = o.read_spike_record(pop)
+    ref, Vref = memoryless_reference(n, steps, (pre, post, w.astype(np.float64), d, sign), (in_idx, 1.0))
+    assert got.sum() > 0
+    assert np.array_equal(got, ref)
+    assert np.array_equal(np.array(Vs), Vref)
+    expected_events = sum(len(x) for x in in_idx) + sum(
+        np.bincount(pre, minlength=n)[np.nonzero(ref[t])[0]].sum() for t in range(steps))
+    assert o.events == expected_events
+
+
+def single_neuron(o, neuron, extra_types=()):
+    """One neuron with one channel-fed projection per impact type (weights given per push)."""
+    pop = o.add_blifat_population(abi.neuron_array(neuron, 1))
+    return pop
+
+
+def drive(o, pop, plan, steps, types=(abi.EXCITATORY,)):
+    """plan: {step: [(type, weight), ...]} delivered with delay 1 (applied at step + 1)."""
+    projs = {}
+    # one projection per (type, weight) pair, loaded in plan order
+    keys = []
+    for t in sorted(plan):
+        for k in plan[t]:
+            if k not in keys:
+                keys.append(k)
+    for k in keys:
+        projs[k] = o.add_delta_projection(abi.KNP_NO_POPULATION, 1, pop, [0], [0], [k[1]], [1], [k[0]])
+    out = []
+    for t in range(steps):
+        for k in plan.get(t, []):
+            o.push_spikes(projs[k], [0])
+        o.step()
+        n = o.blifat_neurons(pop)[0]
+        out.append((len(o.last_spikes(pop)) > 0, n.potential, n))
+    return out
+
+
+def test_hand_potential_decay(oracle_built):
+    o = Oracle("exact")
+    pop = single_neuron(o, abi.blifat_defaults(potential_decay=0.5))
+    tr = drive(o, pop, {0: [(0, 0.625)], 1: [(0, 0.625)], 2: [(0, 0.625)]}, 5)
+    assert [x[0] for x in tr] == [False, False, False, True, False]
+    assert [x[1] for x in tr] == [0.0, 0.625, 0.9375, 0.0, 0.0]  # 0.3125 + 0.625 = 0.9375; then >= 1 -> reset
+
+
+def test_hand_refractory(oracle_built):
+    o = Oracle("exact")
+    pop = single_neuron(o, abi.blifat_defaults(absolute_refractory_period=2))
+    tr = drive(o, pop, {t: [(0, 1.0)] for t in range(12)}, 12)
+    assert [t for t, x in enumerate(tr) if x[0]] == [3, 6, 9]
+
+
+def test_hand_conductance(oracle_built):
+    o = Oracle("exact")
+    pop = single_neuron(o, abi.blifat_defaults(inhibitory_conductance_decay=0.5))
+    tr = drive(o, pop, {0: [(0, 1.25), (abi.INHIBITORY_CONDUCTANCE, 0.5)], 1: [(0, 1.5)]}, 4)
+    # step 1: V = 1.25, g = 0.5 -> V -= (V + 0.3) * g = 0.475 (no spike);
+    # step 2: g = 0.25 (decay 0.5), V = 1.5 -> 1.5 - 1.8 * 0.25 = 1.05 -> spike
+    v1 = 1.25
+    v1 -= (v1 - (-0.3)) * 0.5
+    assert [x[0] for x in tr] == [False, False, True, False]
+    assert tr[1][1] == v1
+    assert tr[1][2].inhibitory_conductance == 0.5
+    # conductance >= 1 pins the potential to the reversal potential
+    o2 = Oracle("exact")
+    pop2 = single_neuron(o2, abi.blifat_defaults())
+    tr2 = drive(o2, pop2, {0: [(0, 5.0), (abi.INHIBITORY_CONDUCTANCE, 1.5)]}, 3)
+    assert tr2[1][1] == -0.3 and not tr2[1][0]
+
+
+def test_hand_blocking(oracle_built):
+    o = Oracle("exact")
+    pop = single_neuron(o, abi.blifat_defaults())
+    # a negative blocking value blocks impacts for |value| steps, then resets to INT64_MAX
+    tr = drive(o, pop, {0: [(abi.BLOCKING, -2.0)], 1: [(0, 1.5)], 2: [(0, 1.5)], 3: [(0, 1.5)]}, 5)
+    assert [x[0] for x in tr] == [False, False, False, True, True]
+    assert tr[1][2].total_blocking_period == -1
+    assert tr[2][2].total_blocking_period == abi.INT64_MAX
+    assert tr[1][1] == 0.0 and tr[2][1] == 0.0  # impacts of steps 1 and 2 are undone
+
+
+def test_hand_blocking_positive_and_restore(oracle_built):
+    o = Oracle("exact")
+    pop = single_neuron(o, abi.blifat_defaults())
+    tr = drive(o, pop, {0: [(abi.BLOCKING, -1.0)], 1: [(0, 1.5)], 3: [(0, 1.5)]}, 5)
+    # step 1: tbp = -1 -> potential restored to V_pre = 0 (impacts ignored), then tbp -> 0 -> INT64_MAX
+    assert tr[1][2].total_blocking_period == abi.INT64_MAX and tr[1][1] == 0.0
+    # step 2: impact pushed at 1 arrives; no block -> spike
+    assert [x[0] for x in tr] == [False, False, True, False, True]
+
+
+def test_hand_threshold_increment(oracle_built):
+    o = Oracle("exact")
+    pop = single_neuron(o, abi.blifat_defaults(threshold_increment=0.5, threshold_decay=0.5))
+    tr = drive(o, pop, {t: [(0, 1.125)] for t in range(6)}, 6)
+    # spike at 1 -> dyn 0.5; step 2 dyn 0.25 -> thr 1.25 > 1.125 no; step 3 dyn 0.125 -> 1.125 >= 1.125 spike
+    # -> dyn 0.625; step 4 thr 1.3125, step 5 thr 1.15625: no
+    assert [x[0] for x in tr] == [False, True, False, True, False, False]
+    assert tr[1][2].dynamic_threshold == 0.5
+
+
+def test_hand_bursting(oracle_built):
+    o = Oracle("exact")
+    pop = single_neuron(o, abi.blifat_defaults(bursting_period=1, reflexive_weight=0.75, potential_decay=1.0))
+    tr = drive(o, pop, {0: [(0, 1.0)], 1: [(0, 0.25)]}, 4)
+    # spike at 1 (V reset 0, phase 1); step 2: phase 1 -> 0, V += 0.75, + 0.25 impact = 1.0 -> spike again
+    assert [x[0] for x in tr] == [False, True, True, False]
+    assert tr[3][1] == 0.75  # step 3: phase reset to 1 by the step-2 spike -> V = 0 + 0.75
+
+
+def test_resource_neuron_keeps_bursting_phase(oracle_built):
+    """Resource neurons add the reflexive weight while bursting_phase_ == 1 without decrementing it
+    (blifat_impl.h:62)."""
+    o = Oracle("exact")
+    base = abi.blifat_defaults(bursting_phase=1, reflexive_weight=0.25, potential_decay=1.0)
+    pop = o.add_resource_blifat_population(abi.neuron_array(abi.resource_blifat_defaults(base), 1))
+    o.step(3)
+    n = o.resource_neurons(pop)[0]
+    assert n.base.bursting_phase == 1
+    assert n.base.potential == 0.75
+
+
+def _np_hash(seed, step, ch):
+    M = (1 << 64) - 1
+
+    def mix(z):
+        z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & M
+        z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & M
+        return z ^ (z >> 31)
+
+    a = mix((seed + step * 0x9E3779B97F4A7C15) & M)
+    b = mix(a ^ ((ch * 0xD6E8FEB86659FD93 + 0x632BE59BD9B4E019) & M))
+    return b >> 32
+
+
+def test_bernoulli_hash_and_rate(oracle_built):
+    rng = np.random.default_rng(5)
+    for _ in range(200):
+        s, t, c = int(rng.integers(0, 2**63)), int(rng.integers(0, 2**40)), int(rng.integers(0, 2**32))
+        assert bernoulli_hash(s, t, c) == _np_hash(s, t, c)
+    thr = int(workloads.bernoulli_threshold(0.02))
+    hits = sum(bernoulli_hash(99, t, c) < thr for t in range(50) for c in range(400))
+    assert abs(hits / 20000 - 0.02) < 0.004
+
+
+def test_source_matches_pushes(oracle_built):
+    """A device-style Bernoulli source is the same as pushing its spikes from the host."""
+    spec = workloads.plain_small(seed=4, n=200, n_syn=5000)
+    a = spec.load_into(Oracle("exact"))
+    b = Oracle("exact")
+    pop = b.add_blifat_population(abi.neuron_array(abi.blifat_defaults(), 200))
+    for name, args, kw in spec.ops[1:]:
+        if name == "add_delta_projection":
+            getattr(b, name)(*args, **kw)
+    thr = int(workloads.bernoulli_threshold(0.02))
+    seed = 4 * 7919 + 17
+    a.record_spikes(0, 40)
+    b.record_spikes(pop, 40)
+    for t in range(40):
+        idx = [c for c in range(200) if bernoulli_hash(seed, t, c) < thr]
+        if idx:
+            b.push_spikes(0, idx)
+        b.step()
+    a.step(40)
+    assert np.array_equal(a.read_spike_record(0), b.read_spike_record(pop))
+    assert a.events == b.events
+
+
+def test_exact_and_f64_agree_without_plasticity(oracle_built):
+    spec = workloads.plain_small(seed=3, n=300, n_syn=20000)
+    a = spec.load_into(Oracle("exact"))
+    b = spec.load_into(Oracle("f64"))
+    for x in (a, b):
+        x.record_spikes(0, 100)
+        x.step(100)
+    assert np.array_equal(a.read_spike_record(0), b.read_spike_record(0))
+    assert np.array_equal(a.potentials(0), b.potentials(0))
