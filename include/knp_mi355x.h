@@ -215,6 +215,9 @@ int knp_mi355x_read_weights(knp_net *net, uint32_t proj, float *out, uint64_t n_
 int knp_mi355x_read_resource_rules(knp_net *net, uint32_t proj, knp_resource_rule *out, uint64_t n_syn);
 /* Number of SynapticImpacts created so far (sum over steps of spikes x out-degree over all projections). */
 int knp_mi355x_get_event_count(knp_net *net, uint64_t *events);
+/* Counters of the spike -> impact expansion so far: impacts (as above) and presynaptic rows expanded
+ * (one per spike per projection that receives it). */
+int knp_mi355x_get_counters(knp_net *net, uint64_t *events, uint64_t *rows);
 /* Fixed-point fraction bits of the exact impact accumulator of a population (chosen at init). */
 int knp_mi355x_get_accumulator_bits(knp_net *net, uint32_t pop, int32_t *frac_bits);
 

@@ -62,6 +62,7 @@ def _declare(lib):
         "knp_mi355x_read_weights": (i32, [P, u32, pf32, u64]),
         "knp_mi355x_read_resource_rules": (i32, [P, u32, C.POINTER(abi.ResourceRule), u64]),
         "knp_mi355x_get_event_count": (i32, [P, C.POINTER(u64)]),
+        "knp_mi355x_get_counters": (i32, [P, C.POINTER(u64), C.POINTER(u64)]),
         "knp_mi355x_get_accumulator_bits": (i32, [P, u32, C.POINTER(C.c_int32)]),
         "knp_mi355x_enable_timing": (i32, [P, i32]),
         "knp_mi355x_kernel_time": (i32, [P, i32, C.POINTER(C.c_double), C.POINTER(u64)]),
@@ -233,6 +234,11 @@ class MI355XBackend:
         self._chk(self.lib.knp_mi355x_get_event_count(self.net, C.byref(e)))
         return e.value
 
+    def counters(self):
+        e, r = C.c_uint64(), C.c_uint64()
+        self._chk(self.lib.knp_mi355x_get_counters(self.net, C.byref(e), C.byref(r)))
+        return e.value, r.value
+
     def accumulator_bits(self, pop):
         b = C.c_int32()
         self._chk(self.lib.knp_mi355x_get_accumulator_bits(self.net, pop, C.byref(b)))
@@ -302,3 +308,6 @@ class MI355XBackend:
         n = C.c_uint64()
         self._chk(self.lib.knp_mi355x_kernel_time(self.net, kernel_id, C.byref(avg), C.byref(n)))
         return avg.value, n.value
+
+
+KERNEL_IDS = {"neuron": abi.KERNEL_NEURON, "scatter": abi.KERNEL_SCATTER, "train": abi.KERNEL_TRAIN}

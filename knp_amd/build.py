@@ -39,8 +39,33 @@ def build_library():
     return out
 
 
+CXX = os.environ.get("CXX", "g++")
+CXX_FLAGS = ["-std=c++17", "-O2", "-Wall", "-I" + os.path.join(HERE, "host", "include"),
+             "-I" + os.path.join(ROOT, "include")]
+
+
+def build_backend_plugin():
+    """libknp-mi355x-backend.so: the knp::core::Backend implementation exporting create_knp_backend."""
+    out = os.path.join(HERE, "libknp-mi355x-backend.so")
+    _run([CXX, *CXX_FLAGS, "-fPIC", "-shared", os.path.join(HERE, "host", "src", "mi355x_backend.cpp"),
+          "-L" + HERE, "-lknp_mi355x", "-Wl,-rpath,$ORIGIN", "-o", out + ".tmp"])
+    os.replace(out + ".tmp", out)
+    return out
+
+
+def build_cpp_tests():
+    """bin/mi355x_backend_test: the reference's backend tests written against the knp::core API."""
+    os.makedirs(os.path.join(HERE, "bin"), exist_ok=True)
+    out = os.path.join(HERE, "bin", "mi355x_backend_test")
+    _run([CXX, *CXX_FLAGS, os.path.join(ROOT, "tests", "cpp", "mi355x_backend_test.cpp"),
+          os.path.join(HERE, "host", "src", "mi355x_backend.cpp"), "-L" + HERE, "-lknp_mi355x", "-ldl",
+          "-Wl,-rpath,$ORIGIN/..", "-o", out + ".tmp"])
+    os.replace(out + ".tmp", out)
+    return out
+
+
 def build_all():
-    return [build_library()]
+    return [build_library(), build_backend_plugin(), build_cpp_tests()]
 
 
 if __name__ == "__main__":

@@ -145,7 +145,7 @@ struct NetDev {
     RowSource rows[kMaxRowSources];
     Ctl *ctl;                  // [2], indexed by step parity
     uint32_t blk_proj[64];     // projection index of each projection with BLOCKING synapses
-    unsigned long long *events;
+    unsigned long long *events;  // [0] impacts created, [1] presynaptic rows expanded
 };
 
 }  // namespace knp_mi355x

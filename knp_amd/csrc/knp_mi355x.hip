@@ -545,7 +545,7 @@ int init_network(knp_net *net) {
     // ---- control blocks
     int rc;
     if ((rc = dev_alloc(net, (void **)&T.ctl, 2 * sizeof(Ctl)))) return rc;
-    if ((rc = dev_alloc(net, (void **)&T.events, 8))) return rc;
+    if ((rc = dev_alloc(net, (void **)&T.events, 16))) return rc;
 
     // launch shapes: enough waves for the largest step, capped near 8 workgroups per CU
     uint64_t max_rows = 0;
@@ -1281,6 +1281,18 @@ int knp_mi355x_get_event_count(knp_net *net, uint64_t *events) {
     int rc = download(net, &e, net->table.events, 1);
     *events = e;
     return rc;
+}
+
+int knp_mi355x_get_counters(knp_net *net, uint64_t *events, uint64_t *rows) {
+    if (!net) return fail(KNP_ERR_INVALID_ARGUMENT, "null handle");
+    unsigned long long c[2] = {0, 0};
+    if (net->initialized) {
+        int rc = download(net, c, net->table.events, 2);
+        if (rc) return rc;
+    }
+    if (events) *events = c[0];
+    if (rows) *rows = c[1];
+    return KNP_OK;
 }
 
 int knp_mi355x_get_accumulator_bits(knp_net *net, uint32_t pop, int32_t *frac_bits) {

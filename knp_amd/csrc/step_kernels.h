@@ -289,7 +289,10 @@ __global__ __launch_bounds__(kBlock) void k_scatter(const NetDev *__restrict__ n
         const uint32_t pre = (R.from_pop == 0xFFFFFFFFu) ? Q.in_list[idx] : net->pops[R.from_pop].spike_list[idx];
         if (pre >= Q.n_pre) continue;
         const uint64_t b = Q.row_ptr[pre], e = Q.row_ptr[pre + 1];
-        if (lane == 0 && e > b) atomicAdd(net->events, (unsigned long long)(e - b));
+        if (lane == 0) {
+            if (e > b) atomicAdd(&net->events[0], (unsigned long long)(e - b));
+            atomicAdd(&net->events[1], 1ull);
+        }
         const PopDev &P = net->pops[Q.post_pop];
         const uint32_t slot0 = (uint32_t)(t % P.D);
         const float scale = P.scale;

@@ -1,0 +1,64 @@
+// knp/backends/gpu-mi355x/backend.h - the MI355X backend, a drop-in sibling of SingleThreadedCPUBackend
+// (knp/backends/cpu/cpu-single-threaded-backend/include/knp/backends/cpu-single-threaded/backend.h:52-453):
+// same create(), load_populations/load_projections, start/stop_learning, _init and _step, same
+// SpikeMessage traffic on the message bus. The network state lives on the GPU; population spikes are sent
+// on the bus only for populations some other endpoint subscribes to. The library exports the
+// create_knp_backend alias that BackendLoader resolves (backend.cpp:340 of the reference ST backend).
+#pragma once
+#include <memory>
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "../../core/backend.h"
+
+struct knp_net;
+
+namespace knp::backends::gpu_mi355x {
+
+class MI355XBackend : public core::Backend {
+public:
+    using PopulationVariants = core::AllPopulationsVariant;
+    using ProjectionVariants = core::AllProjectionsVariant;
+
+    static std::shared_ptr<MI355XBackend> create();
+    explicit MI355XBackend(int device = 0);
+    ~MI355XBackend() override;
+    MI355XBackend(const MI355XBackend &) = delete;
+    MI355XBackend &operator=(const MI355XBackend &) = delete;
+
+    bool plasticity_supported() const override { return true; }
+    std::vector<std::string> get_supported_neurons() const override;
+    std::vector<std::string> get_supported_synapses() const override;
+
+    void load_populations(const std::vector<PopulationVariants> &populations);
+    void load_projections(const std::vector<ProjectionVariants> &projections);
+    void load_all_populations(const std::vector<core::AllPopulationsVariant> &p) override { load_populations(p); }
+    void load_all_projections(const std::vector<core::AllProjectionsVariant> &p) override { load_projections(p); }
+    void remove_projections(const std::vector<core::UID> &uids) override;
+    void remove_populations(const std::vector<core::UID> &uids) override;
+
+    std::vector<std::unique_ptr<core::Device>> get_devices() const override;
+
+    void stop_learning() override;
+    void start_learning() override;
+    void _init() override;
+    void _step() override;
+
+    // Backend::get_network_data (knp/core-library/include/knp/core/backend.h:396-420): copies of the loaded
+    // populations and projections carrying the current device state.
+    std::pair<std::vector<PopulationVariants>, std::vector<ProjectionVariants>> get_network_data();
+
+    knp_net *native_handle() { return net_; }
+
+private:
+    void check(int rc, const char *what) const;
+    int device_ = 0;
+    knp_net *net_ = nullptr;
+    bool learning_ = false;
+    std::vector<PopulationVariants> populations_;
+    std::vector<ProjectionVariants> projections_;
+    std::vector<core::UID> pop_uids_, proj_uids_;
+};
+
+}  // namespace knp::backends::gpu_mi355x

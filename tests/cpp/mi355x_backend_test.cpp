@@ -1,0 +1,213 @@
+// mi355x_backend_test.cpp - the reference's backend integration tests, run against the MI355X backend
+// through the knp::core API: SmallestNetwork, AdditiveSTDPBLIFATNetwork and ResourceSTDPBLIFATNetwork
+// (knp/tests/backend/single_thread_cpu_test.cpp:54-264, generators knp/tests/common/generators.h:48-100),
+// the supported-type names (:362-381), BackendLoader (knp/tests/framework/backend_loader_test.cpp:29-70)
+// and the Backend::start run loop with a stop predicate.
+// Usage: mi355x_backend_test <path to libknp-mi355x-backend.so>. Prints one PASS/FAIL line per case.
+#include <algorithm>
+#include <cstdio>
+#include <filesystem>
+#include <functional>
+#include <string>
+#include <vector>
+
+#include "knp/backends/gpu-mi355x/backend.h"
+#include "knp/framework/backend_loader.h"
+
+using knp::core::UID;
+using knp::core::messaging::SpikeMessage;
+using DeltaProjection = knp::core::Projection<knp::synapse_traits::DeltaSynapse>;
+using BLIFATPopulation = knp::core::Population<knp::neuron_traits::BLIFATNeuron>;
+using Backend = knp::backends::gpu_mi355x::MI355XBackend;
+
+static int g_failures = 0;
+#define CHECK(cond)                                                              \
+    do {                                                                         \
+        if (!(cond)) {                                                           \
+            std::printf("  check failed at line %d: %s\n", __LINE__, #cond);     \
+            throw std::runtime_error("check failed");                            \
+        }                                                                        \
+    } while (0)
+
+static void run_case(const char *name, const std::function<void()> &f) {
+    try {
+        f();
+        std::printf("PASS %s\n", name);
+    } catch (const std::exception &e) {
+        std::printf("FAIL %s: %s\n", name, e.what());
+        ++g_failures;
+    }
+    std::fflush(stdout);
+}
+
+static const std::vector<knp::core::Step> kExpected = {1, 6, 7, 11, 12, 13, 16, 17, 18, 19};
+
+// Drive a network for 20 steps: input spikes on steps 0, 5, 10, 15; record steps with output spikes.
+static std::vector<knp::core::Step> run_smallest(knp::core::Backend &backend, const UID &input_proj,
+                                                 const UID &population) {
+    auto endpoint = backend.get_message_bus().create_endpoint();
+    const UID in_channel_uid, out_channel_uid;
+    backend.subscribe<SpikeMessage>(input_proj, {in_channel_uid});
+    endpoint.subscribe<SpikeMessage>(out_channel_uid, {population});
+    std::vector<knp::core::Step> results;
+    for (knp::core::Step step = 0; step < 20; ++step) {
+        if (step % 5 == 0) endpoint.send_message(SpikeMessage{{in_channel_uid, 0}, {0}});
+        backend._step();
+        endpoint.receive_all_messages();
+        auto output = endpoint.unload_messages<SpikeMessage>(out_channel_uid);
+        if (!output.empty()) results.push_back(step);
+    }
+    return results;
+}
+
+int main(int argc, char **argv) {
+    if (argc < 2) {
+        std::fprintf(stderr, "usage: %s <libknp-mi355x-backend.so>\n", argv[0]);
+        return 2;
+    }
+    const std::filesystem::path lib = argv[1];
+    // --cpu-only: the cases that need no HIP device (type names, loader resolution)
+    const bool cpu_only = argc > 2 && std::string(argv[2]) == "--cpu-only";
+    if (cpu_only) {
+        run_case("NeuronsAndSynapsesGetting", [] {
+            Backend backend;
+            CHECK(backend.get_supported_neurons()[0] == "BLIFATNeuron");
+            CHECK(backend.get_supported_synapses()[0] == "DeltaSynapse");
+        });
+        run_case("BackendLoaderResolves", [&] {
+            knp::framework::BackendLoader loader;
+            CHECK(knp::framework::BackendLoader::is_backend(lib));
+            CHECK(!knp::framework::BackendLoader::is_backend(lib.parent_path() / "no-such-backend"));
+            auto backend = loader.load(lib);
+            CHECK(backend->get_supported_synapses().size() == 3);
+            bool threw = false;
+            try {
+                loader.load(lib.parent_path() / "no-such-backend");
+            } catch (const std::runtime_error &) {
+                threw = true;
+            }
+            CHECK(threw);
+        });
+        return g_failures ? 1 : 0;
+    }
+
+    run_case("SmallestNetwork", [] {
+        Backend backend;
+        BLIFATPopulation population{[](size_t) { return knp::neuron_traits::neuron_parameters<knp::neuron_traits::BLIFATNeuron>{}; }, 1};
+        DeltaProjection loop{population.get_uid(), population.get_uid(),
+                             [](size_t) { return DeltaProjection::Synapse{{1.1, 6, knp::synapse_traits::OutputType::EXCITATORY}, 0, 0}; },
+                             1};
+        DeltaProjection input{UID{false}, population.get_uid(),
+                              [](size_t) { return DeltaProjection::Synapse{{1.0, 1, knp::synapse_traits::OutputType::EXCITATORY}, 0, 0}; },
+                              1};
+        const UID input_uid = input.get_uid();
+        backend.load_populations({population});
+        backend.load_projections({input, loop});
+        backend._init();
+        CHECK(run_smallest(backend, input_uid, population.get_uid()) == kExpected);
+        CHECK(backend.get_step() == 20);
+    });
+
+    run_case("AdditiveSTDPBLIFATNetwork", [] {
+        using STDPDeltaProjection = knp::core::Projection<knp::synapse_traits::AdditiveSTDPDeltaSynapse>;
+        Backend backend;
+        BLIFATPopulation population{UID(), [](size_t) { return knp::neuron_traits::neuron_parameters<knp::neuron_traits::BLIFATNeuron>{}; }, 1};
+        auto loop = STDPDeltaProjection{
+            population.get_uid(), population.get_uid(),
+            [](size_t) { return STDPDeltaProjection::Synapse{{{1.0, 6, knp::synapse_traits::OutputType::EXCITATORY}, {1, 1}}, 0, 0}; },
+            1};
+        STDPDeltaProjection input{
+            UID{false}, population.get_uid(),
+            [](size_t) { return STDPDeltaProjection::Synapse{{{1.0, 1, knp::synapse_traits::OutputType::EXCITATORY}, {2, 2}}, 0, 0}; },
+            1};
+        loop.get_shared_parameters().stdp_populations_[population.get_uid()] =
+            STDPDeltaProjection::SharedSynapseParameters::ProcessingType::STDPAndSpike;
+        const UID input_uid = input.get_uid();
+        backend.load_populations({population});
+        backend.load_projections({input, loop});
+        backend._init();
+        CHECK(run_smallest(backend, input_uid, population.get_uid()) == kExpected);
+        auto data = backend.get_network_data();
+        const auto &loop_after = std::get<STDPDeltaProjection>(data.second[1]);
+        CHECK(std::get<knp::core::synapse_data>(loop_after[0]).weight_ != 1.0f);
+    });
+
+    run_case("ResourceSTDPBLIFATNetwork", [] {
+        using STDPDeltaProjection = knp::core::Projection<knp::synapse_traits::SynapticResourceSTDPDeltaSynapse>;
+        using BlifatStdpPopulation = knp::core::Population<knp::neuron_traits::SynapticResourceSTDPBLIFATNeuron>;
+        Backend backend;
+        BlifatStdpPopulation population{
+            UID(),
+            [](uint64_t) -> std::optional<BlifatStdpPopulation::NeuronParameters> {
+                BlifatStdpPopulation::NeuronParameters neuron{{}};
+                neuron.synaptic_resource_threshold_ = 1;
+                neuron.free_synaptic_resource_ = 2;
+                neuron.isi_max_ = 0;
+                return neuron;
+            },
+            1};
+        auto loop = STDPDeltaProjection{
+            population.get_uid(), population.get_uid(),
+            [](size_t) { return STDPDeltaProjection::Synapse{{{1.0, 6, knp::synapse_traits::OutputType::EXCITATORY}, {0, 1, 2}}, 0, 0}; },
+            1};
+        STDPDeltaProjection input{
+            UID{false}, population.get_uid(),
+            [](size_t) { return STDPDeltaProjection::Synapse{{{1.0, 1, knp::synapse_traits::OutputType::EXCITATORY}, {0, 1, 2, 0.1F}}, 0, 0}; },
+            1};
+        const UID input_uid = input.get_uid();
+        backend.load_populations({population});
+        backend.load_projections({input, loop});
+        backend._init();
+        backend.start_learning();
+        CHECK(run_smallest(backend, input_uid, population.get_uid()) == kExpected);
+        auto data = backend.get_network_data();
+        const auto &loop_after = std::get<STDPDeltaProjection>(data.second[1]);
+        CHECK(std::get<knp::core::synapse_data>(loop_after[0]).weight_ != 1.0f);
+    });
+
+    run_case("NeuronsAndSynapsesGetting", [] {
+        Backend backend;
+        CHECK(backend.get_supported_neurons()[0] == "BLIFATNeuron");
+        CHECK(backend.get_supported_synapses()[0] == "DeltaSynapse");
+        CHECK(backend.plasticity_supported());
+    });
+
+    run_case("BackendLoader", [&] {
+        knp::framework::BackendLoader loader;
+        CHECK(knp::framework::BackendLoader::is_backend(lib));
+        auto backend = loader.load(lib);
+        CHECK(backend != nullptr);
+        CHECK(backend->get_supported_neurons()[0] == "BLIFATNeuron");
+        CHECK(!backend->get_devices().empty());
+        bool threw = false;
+        try {
+            loader.load(lib.parent_path() / "no-such-backend");
+        } catch (const std::runtime_error &) {
+            threw = true;
+        }
+        CHECK(threw);
+        threw = false;
+        try {
+            backend->select_devices({UID()});
+        } catch (const std::logic_error &) {
+            threw = true;
+        }
+        CHECK(threw);
+    });
+
+    run_case("StartWithStopPredicate", [&] {
+        knp::framework::BackendLoader loader;
+        auto backend = loader.load(lib);
+        BLIFATPopulation population{[](size_t) { return knp::neuron_traits::neuron_parameters<knp::neuron_traits::BLIFATNeuron>{}; }, 1};
+        DeltaProjection loop{population.get_uid(), population.get_uid(),
+                             [](size_t) { return DeltaProjection::Synapse{{1.1, 6, knp::synapse_traits::OutputType::EXCITATORY}, 0, 0}; },
+                             1};
+        backend->load_all_populations({population});
+        backend->load_all_projections({loop});
+        backend->start([](knp::core::Step s) { return s < 7; });
+        CHECK(backend->get_step() == 7);
+        CHECK(!backend->running());
+    });
+
+    return g_failures ? 1 : 0;
+}
