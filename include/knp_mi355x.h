@@ -185,12 +185,15 @@ int knp_mi355x_init(knp_net *net);
 /* ---- inputs (InputChannel::send -> SpikeMessage to a projection, base-framework io/input_channel.h:99-118) */
 /* One SpikeMessage with these presynaptic indexes, delivered to projection `proj` in the next step. */
 int knp_mi355x_push_spikes(knp_net *net, uint32_t proj, const uint32_t *idx, uint32_t n);
-/* A device-side Bernoulli channel group feeding `proj` every step: channel c spikes at step t iff
- * knp_bernoulli_hash(seed, t, c) < thresholds[pattern * n_channels + c], where
- * pattern = (t / steps_per_pattern) % n_patterns, and only when lo <= t % steps_per_pattern <= hi. */
+/* A device-side Bernoulli channel group feeding presynaptic indexes first_index .. first_index+n_channels-1
+ * of `proj` every step: channel c spikes at step t iff
+ * knp_bernoulli_hash(seed, t, hash_offset + c) < thresholds[pattern * n_channels + c], where
+ * pattern = (t / steps_per_pattern) % n_patterns, and only when lo <= t % steps_per_pattern <= hi.
+ * (Stands in for an InputChannel whose generator the host would otherwise run every step.) */
 int knp_mi355x_add_bernoulli_source(
-    knp_net *net, uint32_t proj, uint32_t n_channels, const uint32_t *thresholds, uint32_t n_patterns,
-    uint32_t steps_per_pattern, uint32_t active_lo, uint32_t active_hi, uint64_t seed);
+    knp_net *net, uint32_t proj, uint32_t first_index, uint32_t n_channels, const uint32_t *thresholds,
+    uint32_t n_patterns, uint32_t steps_per_pattern, uint32_t active_lo, uint32_t active_hi, uint64_t seed,
+    uint32_t hash_offset);
 
 /* ---- stepping (Backend::_step, backend.cpp:89-141) ------------------------------------------------ */
 int knp_mi355x_step(knp_net *net, uint64_t n_steps);

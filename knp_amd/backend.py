@@ -49,7 +49,7 @@ def _declare(lib):
         "knp_mi355x_stop_learning": (i32, [P]),
         "knp_mi355x_init": (i32, [P]),
         "knp_mi355x_push_spikes": (i32, [P, u32, pu32, u32]),
-        "knp_mi355x_add_bernoulli_source": (i32, [P, u32, u32, pu32, u32, u32, u32, u32, u64]),
+        "knp_mi355x_add_bernoulli_source": (i32, [P, u32, u32, u32, pu32, u32, u32, u32, u32, u64, u32]),
         "knp_mi355x_step": (i32, [P, u64]),
         "knp_mi355x_get_step": (i32, [P, C.POINTER(u64)]),
         "knp_mi355x_synchronize": (i32, [P]),
@@ -206,12 +206,14 @@ class MI355XBackend:
         idx = np.ascontiguousarray(idx, dtype=np.uint32)
         self._chk(self.lib.knp_mi355x_push_spikes(self.net, proj, abi.u32p(idx), len(idx)))
 
-    def add_bernoulli_source(self, proj, thresholds, steps_per_pattern=1, active_lo=0, active_hi=None, seed=0):
+    def add_bernoulli_source(self, proj, thresholds, steps_per_pattern=1, active_lo=0, active_hi=None, seed=0,
+                             first_index=0, hash_offset=0):
         thr = np.ascontiguousarray(np.atleast_2d(thresholds), dtype=np.uint32)
         n_patterns, n_ch = thr.shape
         hi = steps_per_pattern - 1 if active_hi is None else active_hi
-        self._chk(self.lib.knp_mi355x_add_bernoulli_source(self.net, proj, n_ch, abi.u32p(thr), n_patterns,
-                                                           steps_per_pattern, active_lo, hi, seed))
+        self._chk(self.lib.knp_mi355x_add_bernoulli_source(self.net, proj, first_index, n_ch, abi.u32p(thr),
+                                                           n_patterns, steps_per_pattern, active_lo, hi, seed,
+                                                           hash_offset))
 
     def step(self, n=1):
         self._chk(self.lib.knp_mi355x_step(self.net, n))

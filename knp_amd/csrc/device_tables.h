@@ -118,7 +118,7 @@ struct ProjDev {
 };
 
 struct SourceDev {
-    uint32_t proj, n_ch, n_patterns, steps_per_pattern, lo, hi;
+    uint32_t proj, first, n_ch, n_patterns, steps_per_pattern, lo, hi, hash_offset;
     uint64_t seed;
     const uint32_t *thr;
 };

@@ -69,7 +69,7 @@ struct HostProj {
 };
 
 struct HostSource {
-    uint32_t proj, n_ch, n_patterns, spp, lo, hi;
+    uint32_t proj, first, n_ch, n_patterns, spp, lo, hi, hash_offset;
     uint64_t seed;
     std::vector<uint32_t> thr;
 };
@@ -515,6 +515,8 @@ int init_network(knp_net *net) {
         const HostSource &H = net->srcs[s];
         SourceDev &S = T.srcs[s];
         S.proj = H.proj;
+        S.first = H.first;
+        S.hash_offset = H.hash_offset;
         S.n_ch = H.n_ch;
         S.n_patterns = H.n_patterns;
         S.steps_per_pattern = H.spp;
@@ -1009,25 +1011,27 @@ int knp_mi355x_push_spikes(knp_net *net, uint32_t proj, const uint32_t *idx, uin
     return KNP_OK;
 }
 
-int knp_mi355x_add_bernoulli_source(knp_net *net, uint32_t proj, uint32_t n_channels, const uint32_t *thresholds,
-                                    uint32_t n_patterns, uint32_t steps_per_pattern, uint32_t active_lo,
-                                    uint32_t active_hi, uint64_t seed) {
+int knp_mi355x_add_bernoulli_source(knp_net *net, uint32_t proj, uint32_t first_index, uint32_t n_channels,
+                                    const uint32_t *thresholds, uint32_t n_patterns, uint32_t steps_per_pattern,
+                                    uint32_t active_lo, uint32_t active_hi, uint64_t seed, uint32_t hash_offset) {
     int rc = check_proj(net, proj);
     if (rc) return rc;
     if (net->initialized) return fail(KNP_ERR_STATE, "sources must be added before the first step");
     if (n_patterns == 0 || steps_per_pattern == 0 || !thresholds)
         return fail(KNP_ERR_INVALID_ARGUMENT, "bad source shape");
-    if (n_channels > net->projs[proj].n_pre)
-        return fail(KNP_ERR_INVALID_ARGUMENT, "more channels than presynaptic indexes");
+    if ((uint64_t)first_index + n_channels > net->projs[proj].n_pre)
+        return fail(KNP_ERR_INVALID_ARGUMENT, "source channels exceed the presynaptic index range");
     if (net->srcs.size() >= (size_t)kMaxSources) return fail(KNP_ERR_UNSUPPORTED, "too many sources");
     HostSource S;
     S.proj = proj;
+    S.first = first_index;
     S.n_ch = n_channels;
     S.n_patterns = n_patterns;
     S.spp = steps_per_pattern;
     S.lo = active_lo;
     S.hi = active_hi;
     S.seed = seed;
+    S.hash_offset = hash_offset;
     S.thr.assign(thresholds, thresholds + (size_t)n_channels * n_patterns);
     net->srcs.push_back(std::move(S));
     net->projs[proj].gen_channels += n_channels;

@@ -230,10 +230,10 @@ __device__ __forceinline__ void source_step(const NetDev *__restrict__ net, uint
     bool spike = false;
     if (c < S.n_ch && phase >= S.lo && phase <= S.hi) {
         const uint32_t pattern = (uint32_t)((t / S.steps_per_pattern) % S.n_patterns);
-        spike = bernoulli_hash(S.seed, t, c) < S.thr[(size_t)pattern * S.n_ch + c];
+        spike = bernoulli_hash(S.seed, t, S.hash_offset + c) < S.thr[(size_t)pattern * S.n_ch + c];
     }
     const ProjDev &Q = net->projs[S.proj];
-    wave_append(spike, c, &ctl->in_count[S.proj], Q.in_list);
+    wave_append(spike, S.first + c, &ctl->in_count[S.proj], Q.in_list);
 }
 
 __global__ __launch_bounds__(kBlock) void k_phase1(const NetDev *__restrict__ net, uint64_t t) {

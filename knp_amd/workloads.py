@@ -154,7 +154,8 @@ def resource_from_weight(weight, w_min, w_max):
 
 
 def stdp_learning(seed=3, n_images=2000, steps_per_image=20, n_in=784, n_out=1024, w0=0.05, p_max=0.1,
-                  lateral_w=0.5, dopamine_p=0.1, dopamine_w=1.0, potential_decay=2.0 / 3.0, synapse_period=10):
+                  lateral_w=0.5, dopamine_p=0.1, dopamine_w=1.0, potential_decay=2.0 / 3.0, synapse_period=10,
+                  source_seed=None, instance=0):
     """Config 2: 784 image-rate Bernoulli channels (p = 0.1 * pixel / 255, 20 steps per image) -> dense
     all-to-all SynapticResourceSTDPDeltaSynapse 784 -> 1,024 (initial w ~ U(0, 0.05) encoded as resources,
     delay 1, unlocked) onto SynapticResourceSTDP BLIFAT neurons; lateral all-to-all 1,024 x 1,024
@@ -162,8 +163,10 @@ def stdp_learning(seed=3, n_images=2000, steps_per_image=20, n_in=784, n_out=102
     every image, broadcast 1 -> 1,024 as DOPAMINE Delta (w = 1.0, delay 1).
     Neurons use the reference defaults except potential_decay_ (2/3, so that inputs integrate over steps);
     synapses use the rule defaults except dopamine_plasticity_period_ (10, as the mnist-learn example:
-    with the default 0 no synapse can ever count as contributing, blifat_stdp.h:58-60)."""
+    with the default 0 no synapse can ever count as contributing, blifat_stdp.h:58-60).
+    `instance` selects this network's Bernoulli streams inside a batch (see batched_stdp)."""
     rng = np.random.default_rng(seed)
+    src_seed = seed * 7919 + 17 if source_seed is None else source_seed
     spec = NetworkSpec("stdp_learning")
     base = abi.blifat_defaults(potential_decay=potential_decay)
     pop = spec.add_resource_blifat_population(abi.neuron_array(abi.resource_blifat_defaults(base), n_out))
@@ -197,11 +200,108 @@ def stdp_learning(seed=3, n_images=2000, steps_per_image=20, n_in=784, n_out=102
                                     np.ones(n_out, np.uint32), np.full(n_out, abi.DOPAMINE, np.uint8))
     images = blob_images(rng, n_images)
     thr = bernoulli_threshold(p_max * images.reshape(n_images, n_in).astype(np.float64) / 255.0)
-    spec.add_bernoulli_source(inp, thr, steps_per_pattern=steps_per_image, seed=seed * 7919 + 17)
+    spec.add_bernoulli_source(inp, thr, steps_per_pattern=steps_per_image, seed=src_seed,
+                              hash_offset=instance * n_in)
     spec.add_bernoulli_source(dop, bernoulli_threshold([dopamine_p]), steps_per_pattern=steps_per_image,
-                              active_lo=steps_per_image - 1, active_hi=steps_per_image - 1, seed=seed * 7919 + 29)
+                              active_lo=steps_per_image - 1, active_hi=steps_per_image - 1, seed=src_seed + 12,
+                              hash_offset=instance)
     spec.start_learning()
     spec.meta.update(pop=pop, input_proj=inp, lateral_proj=lat, dopamine_proj=dop,
-                     steps=n_images * steps_per_image)
+                     steps=n_images * steps_per_image, images=thr)
     return spec
 
+
+
+def _op(spec, name):
+    return [(a, kw) for n, a, kw in spec.ops if n == name]
+
+
+def batched_stdp(batch=256, n_images=250, steps_per_image=20, seed=30, **kw):
+    """Config 3: `batch` independent copies of the config-2 network (own weights from seed + b, own image
+    streams), composed into one device network: one population of batch x 1,024 neurons and block-diagonal
+    projections, so that every instance advances in the same kernel launches. Instance b alone is
+    stdp_learning(seed=seed + b, source_seed=..., instance=b) - the same synapse values and input streams."""
+    src_seed = seed * 7919 + 17
+    parts = [stdp_learning(seed=seed + b, n_images=n_images, steps_per_image=steps_per_image, source_seed=src_seed,
+                           instance=b, **kw) for b in range(batch)]
+    return compose(parts, "batched_stdp", per_instance_steps=n_images * steps_per_image)
+
+
+def compose(parts, name, per_instance_steps):
+    """Block-diagonal composition of networks that share one layout (same populations and projections in the
+    same order): population k of the result holds population k of every part, back to back; projection q
+    holds every part's projection q with indexes shifted; sources are merged into one channel range."""
+    spec = NetworkSpec(name)
+    p0 = parts[0]
+    pop_ops = [(n, a, kw) for n, a, kw in p0.ops if n.endswith("population")]
+    proj_ops = [(n, a, kw) for n, a, kw in p0.ops if n.endswith("projection")]
+    src_ops = [(n, a, kw) for n, a, kw in p0.ops if n == "add_bernoulli_source"]
+    pop_offsets = []  # [part][pop] -> neuron offset
+    for k, (name_k, a, kw) in enumerate(pop_ops):
+        neurons = [next(x for i, x in enumerate(y for y in pt.ops if y[0].endswith("population")) if i == k)[1][0]
+                   for pt in parts]
+        sizes = [len(x) for x in neurons]
+        allv = (type(neurons[0][0]) * sum(sizes))()
+        raw = memoryview(allv).cast("B")
+        off = 0
+        for x in neurons:
+            nb = ctypes.sizeof(x)
+            raw[off:off + nb] = memoryview(x).cast("B")
+            off += nb
+        getattr(spec, name_k)(allv)
+        pop_offsets.append(np.concatenate([[0], np.cumsum(sizes)[:-1]]).astype(np.uint64))
+    channel_offsets = []
+    for q, (name_q, a, kw) in enumerate(proj_ops):
+        per = [[y for y in pt.ops if y[0].endswith("projection")][q] for pt in parts]
+        pre_pop, n_pre, post_pop = a[0], a[1], a[2]
+        pre_l, post_l, w_l, d_l, extra_l, ty_l = [], [], [], [], [], []
+        ch_off = []
+        total_pre = 0
+        for b, (_, ab, kwb) in enumerate(per):
+            if pre_pop == abi.KNP_NO_POPULATION:
+                pre_shift = total_pre
+                ch_off.append(total_pre)
+                total_pre += ab[1]
+            else:
+                pre_shift = int(pop_offsets[pre_pop][b])
+            pre_l.append(np.asarray(ab[3], np.uint64) + pre_shift)
+            post_l.append(np.asarray(ab[4], np.uint64) + int(pop_offsets[post_pop][b]))
+            w_l.append(np.asarray(ab[5], np.float32))
+            d_l.append(np.asarray(ab[6], np.uint32))
+            if name_q == "add_resource_stdp_projection":
+                extra_l.append(np.frombuffer(memoryview(ab[7]).cast("B"), dtype=np.uint8))
+            otype = kwb.get("otype", ab[8] if name_q == "add_resource_stdp_projection" and len(ab) > 8 else
+                            (ab[7] if name_q == "add_delta_projection" and len(ab) > 7 else None))
+            ty_l.append(np.zeros(len(ab[3]), np.uint8) if otype is None else np.asarray(otype, np.uint8))
+        channel_offsets.append(ch_off)
+        new_n_pre = total_pre if pre_pop == abi.KNP_NO_POPULATION else n_pre * len(parts)
+        pre = np.concatenate(pre_l).astype(np.uint32)
+        post = np.concatenate(post_l).astype(np.uint32)
+        w = np.concatenate(w_l)
+        d = np.concatenate(d_l)
+        ty = np.concatenate(ty_l)
+        if name_q == "add_resource_stdp_projection":
+            raw = np.concatenate(extra_l)
+            rules = (abi.ResourceRule * len(pre)).from_buffer(bytearray(raw.tobytes()))
+            spec.add_resource_stdp_projection(pre_pop, new_n_pre, post_pop, pre, post, w, d, rules, otype=ty)
+        elif name_q == "add_delta_projection":
+            spec.add_delta_projection(pre_pop, new_n_pre, post_pop, pre, post, w, d, ty)
+        else:
+            raise ValueError("additive STDP projections are not composed")
+    # sources: part b's source k feeds its projection's channel block; thresholds of all parts side by side
+    for k, (_, a, kw) in enumerate(src_ops):
+        per = [[y for y in pt.ops if y[0] == "add_bernoulli_source"][k] for pt in parts]
+        proj = a[0]
+        thr = np.concatenate([np.atleast_2d(np.asarray(x[1][1], np.uint32)) for x in per], axis=1)
+        kw0 = dict(per[0][2])
+        offs = [x[2].get("hash_offset", 0) for x in per]
+        assert offs == list(range(offs[0], offs[0] + thr.shape[1], thr.shape[1] // len(parts))), \
+            "parts must use consecutive hash offsets"
+        kw0["hash_offset"] = offs[0]
+        kw0["first_index"] = channel_offsets[proj][0] if channel_offsets[proj] else 0
+        spec.add_bernoulli_source(proj, thr, **kw0)
+    if p0.learning:
+        spec.start_learning()
+    spec.meta.update(parts=len(parts), steps=per_instance_steps, pop_offsets=pop_offsets,
+                     part_sizes=[pt.pop_sizes for pt in parts])
+    return spec

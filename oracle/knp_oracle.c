@@ -98,7 +98,7 @@ typedef struct {
 } ko_pop;
 
 typedef struct {
-    uint32_t n_channels, n_patterns, steps_per_pattern, lo, hi;
+    uint32_t first, n_channels, n_patterns, steps_per_pattern, lo, hi, hash_offset;
     uint32_t *thr;
     uint64_t seed;
     uint32_t proj;
@@ -414,18 +414,21 @@ int ko_push_spikes(ko_net *net, uint32_t proj, const uint32_t *idx, uint32_t n) 
     return 0;
 }
 
-int ko_add_bernoulli_source(ko_net *net, uint32_t proj, uint32_t n_channels, const uint32_t *thr, uint32_t n_patterns,
-                            uint32_t steps_per_pattern, uint32_t lo, uint32_t hi, uint64_t seed) {
+int ko_add_bernoulli_source(ko_net *net, uint32_t proj, uint32_t first, uint32_t n_channels, const uint32_t *thr,
+                            uint32_t n_patterns, uint32_t steps_per_pattern, uint32_t lo, uint32_t hi, uint64_t seed,
+                            uint32_t hash_offset) {
     if (proj >= net->n_projs || n_patterns == 0 || steps_per_pattern == 0) return -1;
-    if (n_channels > net->projs[proj].n_pre) return -1;
+    if ((uint64_t)first + n_channels > net->projs[proj].n_pre) return -1;
     net->srcs = (ko_source *)realloc(net->srcs, (net->n_srcs + 1) * sizeof(ko_source));
     ko_source *s = &net->srcs[net->n_srcs++];
+    s->first = first;
     s->n_channels = n_channels;
     s->n_patterns = n_patterns;
     s->steps_per_pattern = steps_per_pattern;
     s->lo = lo;
     s->hi = hi;
     s->seed = seed;
+    s->hash_offset = hash_offset;
     s->proj = proj;
     s->thr = (uint32_t *)xcalloc((size_t)n_channels * n_patterns, 4);
     memcpy(s->thr, thr, (size_t)n_channels * n_patterns * 4);
@@ -773,7 +776,7 @@ static void generate_sources(ko_net *net, uint64_t step) {
         uint32_t *idx = (uint32_t *)xcalloc(src->n_channels, 4);
         uint32_t n = 0;
         for (uint32_t c = 0; c < src->n_channels; ++c)
-            if (ko_bernoulli_hash(src->seed, step, c) < thr[c]) idx[n++] = c;
+            if (ko_bernoulli_hash(src->seed, step, src->hash_offset + c) < thr[c]) idx[n++] = src->first + c;
         if (n) ko_push_spikes(net, src->proj, idx, n);
         free(idx);
     }

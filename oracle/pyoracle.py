@@ -48,7 +48,7 @@ def _load(policy):
         "ko_record_spikes": (i32, [P, u32, u64]),
         "ko_read_spike_record": (i32, [P, u32, pu32, u64, C.POINTER(u64)]),
         "ko_push_spikes": (i32, [P, u32, pu32, u32]),
-        "ko_add_bernoulli_source": (i32, [P, u32, u32, pu32, u32, u32, u32, u32, u64]),
+        "ko_add_bernoulli_source": (i32, [P, u32, u32, u32, pu32, u32, u32, u32, u32, u64, u32]),
         "ko_step": (i32, [P, u64]),
         "ko_get_step": (u64, [P]),
         "ko_get_event_count": (u64, [P]),
@@ -163,12 +163,13 @@ class Oracle:
         idx = np.ascontiguousarray(idx, dtype=np.uint32)
         self._chk(self.lib.ko_push_spikes(self.net, proj, abi.u32p(idx), len(idx)))
 
-    def add_bernoulli_source(self, proj, thresholds, steps_per_pattern=1, active_lo=0, active_hi=None, seed=0):
+    def add_bernoulli_source(self, proj, thresholds, steps_per_pattern=1, active_lo=0, active_hi=None, seed=0,
+                             first_index=0, hash_offset=0):
         thr = np.ascontiguousarray(np.atleast_2d(thresholds), dtype=np.uint32)
         n_patterns, n_ch = thr.shape
         hi = steps_per_pattern - 1 if active_hi is None else active_hi
-        self._chk(self.lib.ko_add_bernoulli_source(self.net, proj, n_ch, abi.u32p(thr), n_patterns,
-                                                   steps_per_pattern, active_lo, hi, seed))
+        self._chk(self.lib.ko_add_bernoulli_source(self.net, proj, first_index, n_ch, abi.u32p(thr), n_patterns,
+                                                   steps_per_pattern, active_lo, hi, seed, hash_offset))
 
     def step(self, n=1):
         self._chk(self.lib.ko_step(self.net, n))
