@@ -195,6 +195,25 @@ int knp_mi355x_add_bernoulli_source(
     uint32_t n_patterns, uint32_t steps_per_pattern, uint32_t active_lo, uint32_t active_hi, uint64_t seed,
     uint32_t hash_offset);
 
+/* A dense feed-forward Projection<DeltaSynapse> from per-instance blocks of device Bernoulli channels into
+ * many independent network instances at once (inference: the weights never change). Instance k owns
+ * postsynaptic neurons post_begin[k] .. post_begin[k] + post_count[k] - 1 of `post_pop` and n_in channels;
+ * channel i of instance k spikes at step t iff knp_bernoulli_hash(seed, t, hash_offset + k * n_in + i) <
+ * threshold. Each spike is a SynapticImpact of weight row_scale[j] * w(j, i) on every neuron j of its
+ * instance, EXCITATORY, delay 1 - the same as the equivalent sparse projection - but the impacts are
+ * computed time_block steps at a time as (binary spikes) x (weights) on MFMA with FP32 accumulation.
+ * weights: row-major [sum(post_count)][n_in] in `format`; row_scale may be NULL (all 1). Power-of-two
+ * scales keep the dequantized weights exact. time_block: a multiple of 32, or 0 to let the autotuner pick
+ * (cached in $KNP_MI355X_TUNE_CACHE). One dense projection per population; not for resource populations.
+ * (No counterpart class in the reference: it is the batched form of its DeltaSynapse projection.) */
+enum knp_dense_format { KNP_DENSE_FP8_E4M3 = 0, KNP_DENSE_BF16 = 1 };
+int knp_mi355x_add_dense_input_projection(
+    knp_net *net, uint32_t post_pop, uint32_t n_instances, const uint32_t *post_begin, const uint32_t *post_count,
+    uint32_t n_in, int format, const void *weights, const float *row_scale, uint32_t threshold, uint64_t seed,
+    uint32_t hash_offset, uint32_t time_block, uint32_t *out_dense);
+/* The time block the dense projection runs with (after init). */
+int knp_mi355x_get_dense_time_block(knp_net *net, uint32_t dense, uint32_t *time_block);
+
 /* ---- stepping (Backend::_step, backend.cpp:89-141) ------------------------------------------------ */
 int knp_mi355x_step(knp_net *net, uint64_t n_steps);
 /* Backend::get_step (backend.h:290). */
@@ -229,7 +248,8 @@ enum knp_kernel_id {
     KNP_KERNEL_NEURON = 0, /* fused population step (+ spike compaction) */
     KNP_KERNEL_SCATTER = 1, /* spike -> impact scatter into the delay ring */
     KNP_KERNEL_TRAIN = 2,   /* synaptic-resource STDP */
-    KNP_KERNEL_COUNT = 3
+    KNP_KERNEL_DENSE = 3,   /* dense input projection on MFMA (spike bits + product), per time block */
+    KNP_KERNEL_COUNT = 4
 };
 int knp_mi355x_enable_timing(knp_net *net, int enable);
 /* Average duration (ms) and launch count of a kernel class since timing was enabled. */

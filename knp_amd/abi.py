@@ -34,6 +34,11 @@ STDP_AND_SPIKE = 1
 KERNEL_NEURON = 0
 KERNEL_SCATTER = 1
 KERNEL_TRAIN = 2
+KERNEL_DENSE = 3
+
+# knp_dense_format
+DENSE_FP8_E4M3 = 0
+DENSE_BF16 = 1
 
 SIZE_MAX = 0xFFFFFFFFFFFFFFFF
 INT64_MAX = 0x7FFFFFFFFFFFFFFF
@@ -193,3 +198,47 @@ def unpack_bitmaps(words, n_steps, n_neurons):
     a = np.asarray(words[: n_steps * w], dtype=np.uint32).reshape(n_steps, w)
     bits = np.unpackbits(a.view(np.uint8).reshape(n_steps, w * 4), axis=1, bitorder="little")
     return bits[:, :n_neurons].astype(bool)
+
+
+# ---- FP8 E4M3 (OCP "E4M3FN": bias 7, no infinities, max 448, NaN = S.1111.111) and BF16 codecs ---------
+def _e4m3_table():
+    vals = np.zeros(128, dtype=np.float64)
+    for b in range(128):
+        e, m = (b >> 3) & 15, b & 7
+        vals[b] = np.ldexp(m / 8.0, -6) if e == 0 else np.ldexp(1.0 + m / 8.0, e - 7)
+    vals[127] = np.nan
+    return vals
+
+
+E4M3_VALUES = _e4m3_table()  # code -> value for the 127 non-negative finite codes (index 127 is NaN)
+
+
+def e4m3_decode(codes):
+    codes = np.asarray(codes, dtype=np.uint8)
+    mag = E4M3_VALUES[codes & 0x7F]
+    return np.where(codes & 0x80, -mag, mag)
+
+
+def e4m3_encode(x):
+    """Round to nearest e4m3 value (ties to even code), saturating at +-448."""
+    x = np.asarray(x, dtype=np.float64)
+    a = np.minimum(np.abs(x), 448.0)
+    grid = E4M3_VALUES[:127]
+    hi = np.clip(np.searchsorted(grid, a, side="left"), 0, 126)
+    lo = np.maximum(hi - 1, 0)
+    d_lo = a - grid[lo]
+    d_hi = grid[hi] - a
+    pick_hi = (d_hi < d_lo) | ((d_hi == d_lo) & (hi % 2 == 0))
+    code = np.where(pick_hi, hi, lo).astype(np.uint8)
+    return np.where(x < 0, code | 0x80, code).astype(np.uint8)
+
+
+def bf16_encode(x):
+    """float32 -> bf16 bits, round to nearest even."""
+    f = np.asarray(x, dtype=np.float32).view(np.uint32).astype(np.uint64)
+    r = ((f + 0x7FFF + ((f >> 16) & 1)) >> 16).astype(np.uint16)
+    return r
+
+
+def bf16_decode(bits):
+    return (np.asarray(bits, dtype=np.uint32) << 16).view(np.float32)

@@ -50,6 +50,9 @@ def _declare(lib):
         "knp_mi355x_init": (i32, [P]),
         "knp_mi355x_push_spikes": (i32, [P, u32, pu32, u32]),
         "knp_mi355x_add_bernoulli_source": (i32, [P, u32, u32, u32, pu32, u32, u32, u32, u32, u64, u32]),
+        "knp_mi355x_add_dense_input_projection": (
+            i32, [P, u32, u32, pu32, pu32, u32, i32, C.c_void_p, pf32, u32, u64, u32, u32, pu32]),
+        "knp_mi355x_get_dense_time_block": (i32, [P, u32, pu32]),
         "knp_mi355x_step": (i32, [P, u64]),
         "knp_mi355x_get_step": (i32, [P, C.POINTER(u64)]),
         "knp_mi355x_synchronize": (i32, [P]),
@@ -215,6 +218,25 @@ class MI355XBackend:
                                                            n_patterns, steps_per_pattern, active_lo, hi, seed,
                                                            hash_offset))
 
+    def add_dense_input_projection(self, post_pop, post_begin, post_count, n_in, fmt, weights, row_scale,
+                                   threshold, seed, hash_offset=0, time_block=0):
+        """Dense feed-forward projection from per-instance Bernoulli channel blocks (see the C ABI)."""
+        pb = np.ascontiguousarray(post_begin, dtype=np.uint32)
+        pc = np.ascontiguousarray(post_count, dtype=np.uint32)
+        w = np.ascontiguousarray(weights)
+        assert w.dtype == (np.uint8 if fmt == abi.DENSE_FP8_E4M3 else np.uint16)
+        sc = None if row_scale is None else np.ascontiguousarray(row_scale, dtype=np.float32)
+        out = C.c_uint32()
+        self._chk(self.lib.knp_mi355x_add_dense_input_projection(
+            self.net, post_pop, len(pb), abi.u32p(pb), abi.u32p(pc), n_in, fmt, w.ctypes.data_as(C.c_void_p),
+            abi.f32p(sc), int(threshold), seed, hash_offset, time_block, C.byref(out)))
+        return out.value
+
+    def dense_time_block(self, dense=0):
+        t = C.c_uint32()
+        self._chk(self.lib.knp_mi355x_get_dense_time_block(self.net, dense, C.byref(t)))
+        return t.value
+
     def step(self, n=1):
         self._chk(self.lib.knp_mi355x_step(self.net, n))
 
@@ -312,4 +334,5 @@ class MI355XBackend:
         return avg.value, n.value
 
 
-KERNEL_IDS = {"neuron": abi.KERNEL_NEURON, "scatter": abi.KERNEL_SCATTER, "train": abi.KERNEL_TRAIN}
+KERNEL_IDS = {"neuron": abi.KERNEL_NEURON, "scatter": abi.KERNEL_SCATTER, "train": abi.KERNEL_TRAIN,
+              "dense": abi.KERNEL_DENSE}

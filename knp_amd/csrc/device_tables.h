@@ -13,8 +13,10 @@ constexpr int kMaxPops = 32;
 constexpr int kMaxProjs = 64;
 constexpr int kMaxSources = 32;
 constexpr int kMaxRowSources = 128;
+constexpr int kMaxVSeg = 16384;   // virtual segments the scatter scans in LDS
 constexpr int kMaxTasks = kMaxPops + kMaxSources;
 constexpr int kMaxWorking = 8;
+constexpr int kMaxDense = 4;
 constexpr int kBlock = 256;  // 4 wave64 per workgroup
 
 enum PopKind : uint32_t { POP_LEAN = 0, POP_FULL = 1, POP_RESOURCE = 2 };
@@ -65,23 +67,27 @@ struct PopDev {
         add_thr;
     ParamU bperiod, refr;
     uint32_t lean_ts_max;  // LEAN: every neuron started with n_time_steps_since_last_firing = SIZE_MAX
+    uint32_t seg_base;     // first spike segment (one per 256 neurons)
+    const float *ff;       // dense input impacts [2 ff_T][n] by arrival step (null if none)
+    uint32_t ff_T;
     uint32_t track_tbp;    // FULL: total_blocking_period_ is streamed
     uint64_t t_first;      // first step this population runs
     // outputs
-    uint32_t *spike_list;  // compacted spikes of the current step (unordered)
+    uint32_t *spike_list;  // spikes of the current step: segment c holds neurons 256c.. (ascending), see seg_count
     uint32_t *bitmap;      // spikes of the current step, bit i = neuron i
     uint32_t *rec;         // recorded bitmaps [rec_cap][words]
     uint64_t rec_t0, rec_cap;
     ResDev res;
 };
 
-// One source of spike rows for the scatter: the channel input list of a projection or the spike list of
-// a population, with its message rank in the reference delivery order.
+// Spike lists are kept in segments: every phase-1 workgroup writes the spikes of its 256 neurons (or
+// channels) in ascending order into its own fixed segment and stores the segment's count, so compaction needs
+// no global atomics. Host-pushed channel messages of a projection form one more segment. The scatter walks a
+// "virtual" segment list: for every projection (load order), its channel segments, then the segments of every
+// population whose spike messages it receives, with the message rank of each.
 struct RowSource {
     uint32_t proj;
-    uint32_t from_pop;  // population index, or UINT32_MAX for the projection's channel input list
     uint32_t rank;      // 0 = channel messages, 1 + population index for population messages
-    uint32_t pad;
 };
 
 struct ProjDev {
@@ -107,9 +113,10 @@ struct ProjDev {
     uint64_t *pre_times, *post_times;  // [n_syn][hist]
     uint32_t *n_pre_t, *n_post_t;
     uint32_t hist;
-    // channel input list
+    // host-pushed channel messages of the current step (one segment)
     uint32_t *in_list;
     uint32_t in_cap;
+    uint32_t push_seg;
     uint32_t forcing;            // messages of this projection are forcing (plain Delta)
     uint32_t blocking_id;        // rank of this projection among projections with BLOCKING synapses
     uint32_t n_stdp;
@@ -121,6 +128,8 @@ struct SourceDev {
     uint32_t proj, first, n_ch, n_patterns, steps_per_pattern, lo, hi, hash_offset;
     uint64_t seed;
     const uint32_t *thr;
+    uint32_t *list;        // spiking channels (presynaptic indexes), segment c = channels 256c..
+    uint32_t seg_base;
 };
 
 // Phase-1 task: a population update or a Bernoulli source, covering `blocks` workgroups.
@@ -130,9 +139,29 @@ struct TaskDev {
     uint32_t block_begin, blocks;
 };
 
+// Dense input projection from per-instance Bernoulli channel blocks, evaluated on MFMA one time block at a time.
+// Weights are stored in MFMA fragment order: for tile c (32 postsynaptic neurons of one instance) and K-step
+// kk (16 inputs), 64 lanes x (8 fp8 | 8 bf16) values, lane l holding rows j0 + (l % 32), inputs
+// 16 kk + 8 (l / 32) + 0..7.
+struct DenseDev {
+    uint32_t format;       // 0 fp8 e4m3, 1 bf16
+    uint32_t post_pop;
+    uint32_t n_in, ksteps;  // inputs, ceil(n_in / 16)
+    uint32_t n_inst, n_tiles, T;  // instances, 32-column tiles, steps per time block
+    uint32_t words;        // spike-bit words per (step, instance): 2 * ksteps (16 bits per K-step)
+    uint32_t threshold, hash_offset;
+    uint64_t seed;
+    const uint8_t *frag;   // [n_tiles][ksteps][64][8 or 16 bytes]
+    const uint32_t *tile_inst;  // [n_tiles] instance of the tile
+    const uint32_t *tile_j0;    // [n_tiles] first postsynaptic neuron (population index)
+    const uint32_t *tile_ncol;  // [n_tiles] valid columns (<= 32)
+    const float *row_scale;     // [population size] (1 outside instances)
+    const uint32_t *inst_post_count;  // [n_inst]
+    uint8_t *bits;         // [T][n_inst][2 ksteps] spike bits of the current block (bit b of byte = input 8 byte + b)
+    float *ff;             // [2T][population size] impacts by arrival step
+};
+
 struct Ctl {
-    uint32_t pop_count[kMaxPops];
-    uint32_t in_count[kMaxProjs];
     uint32_t cand_count[kMaxPops];
 };
 
@@ -144,8 +173,16 @@ struct NetDev {
     TaskDev tasks[kMaxTasks];
     RowSource rows[kMaxRowSources];
     Ctl *ctl;                  // [2], indexed by step parity
+    uint32_t n_seg, n_vseg;
+    uint32_t *seg_count;       // [2][n_seg] spikes per segment, indexed by step parity
+    uint32_t *const *seg_list; // [n_seg] first element of every segment
+    const uint32_t *vseg;      // [n_vseg] segment of every virtual segment
+    const uint32_t *vrs;       // [n_vseg] row source of every virtual segment
+    unsigned long long *part;  // [2 * scatter grid] per-workgroup impact and row counters
     uint32_t blk_proj[64];     // projection index of each projection with BLOCKING synapses
-    unsigned long long *events;  // [0] impacts created, [1] presynaptic rows expanded
+    uint32_t n_dense;
+    DenseDev dense[kMaxDense];
+    unsigned long long *events;  // [0] impacts, [1] rows expanded by the dense path (scatter: part)
 };
 
 }  // namespace knp_mi355x

@@ -10,7 +10,9 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
+#include <fstream>
 #include <limits>
 #include <string>
 #include <vector>
@@ -68,6 +70,14 @@ struct HostProj {
     uint32_t gen_channels = 0;                   // channels of device sources feeding it
 };
 
+struct HostDense {
+    uint32_t post_pop = 0, n_inst = 0, n_in = 0, format = 0, threshold = 0, hash_offset = 0, T = 0;
+    uint64_t seed = 0;
+    std::vector<uint32_t> post_begin, post_count;
+    std::vector<uint8_t> weights;  // row-major [sum post_count][n_in], 1 or 2 bytes per value
+    std::vector<float> row_scale;  // per row of `weights`
+};
+
 struct HostSource {
     uint32_t proj, first, n_ch, n_patterns, spp, lo, hi, hash_offset;
     uint64_t seed;
@@ -82,6 +92,7 @@ struct knp_net {
     std::vector<HostPop> pops;
     std::vector<HostProj> projs;
     std::vector<HostSource> srcs;
+    std::vector<HostDense> dense;
     bool initialized = false;
     uint64_t step = 0;
     NetDev table{};
@@ -89,13 +100,14 @@ struct knp_net {
     std::vector<void *> allocations;
     uint64_t allocated = 0, cap = 0;
     uint32_t scatter_grid = 1, train_grid = 1;
+    std::vector<uint32_t *> seg_list_host;
     std::vector<uint32_t> staging;
     bool timing = false;
     std::vector<hipEvent_t> ev_pool;
     size_t ev_used = 0;
     std::vector<std::pair<hipEvent_t, hipEvent_t>> timed[KNP_KERNEL_COUNT];
-    double time_ms[KNP_KERNEL_COUNT] = {0, 0, 0};
-    uint64_t launches[KNP_KERNEL_COUNT] = {0, 0, 0};
+    double time_ms[KNP_KERNEL_COUNT] = {};
+    uint64_t launches[KNP_KERNEL_COUNT] = {};
 };
 
 namespace {
@@ -199,6 +211,190 @@ int32_t choose_frac_bits(double bound) {
 }
 
 int upload_working_sets(knp_net *net);
+int upload_table(knp_net *net);
+void dev_free(knp_net *net, void *p);
+
+// Value of a dense weight as a float (fp8 e4m3 decoded exactly, bf16 widened).
+double dense_value(const HostDense &D, uint64_t row, uint32_t i) {
+    const size_t idx = (size_t)row * D.n_in + i;
+    if (D.format == KNP_DENSE_BF16) {
+        uint32_t bits = (uint32_t)(D.weights[2 * idx] | (D.weights[2 * idx + 1] << 8)) << 16;
+        float f;
+        std::memcpy(&f, &bits, 4);
+        return f;
+    }
+    const uint8_t b = D.weights[idx];
+    const int e = (b >> 3) & 15, m = b & 7;
+    if ((b & 0x7F) == 0x7F) return std::numeric_limits<double>::quiet_NaN();
+    const double mag = e == 0 ? std::ldexp(m / 8.0, -6) : std::ldexp(1.0 + m / 8.0, e - 7);
+    return (b & 0x80) ? -mag : mag;
+}
+
+uint32_t dense_ksteps(const HostDense &D) { return (D.n_in + 15) / 16; }
+
+int alloc_dense_buffers(knp_net *net, uint32_t d, uint32_t T) {
+    DenseDev &Dn = net->table.dense[d];
+    const HostDense &Dh = net->dense[d];
+    const uint32_t n = net->pops[Dh.post_pop].n;
+    dev_free(net, Dn.bits);
+    dev_free(net, Dn.ff);
+    Dn.bits = nullptr;
+    Dn.ff = nullptr;
+    Dn.T = T;
+    int rc;
+    if ((rc = dev_alloc(net, (void **)&Dn.bits, (size_t)T * Dh.n_inst * 2 * dense_ksteps(Dh)))) return rc;
+    if ((rc = dev_alloc(net, (void **)&Dn.ff, (size_t)2 * T * n * sizeof(float)))) return rc;
+    PopDev &P = net->table.pops[Dh.post_pop];
+    P.ff = Dn.ff;
+    P.ff_T = T;
+    return KNP_OK;
+}
+
+void launch_dense(knp_net *net, uint32_t d, uint64_t t0) {
+    const DenseDev &Dn = net->table.dense[d];
+    const uint64_t bytes = (uint64_t)Dn.T * Dn.n_inst * 2 * Dn.ksteps;
+    const uint32_t g1 = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(8192, (bytes + kBlock - 1) / kBlock));
+    k_dense_spikes<<<g1, kBlock, 0, net->stream>>>(net->dev_table, d, t0);
+    const uint32_t g2 = std::max(1u, std::min(4096u, (Dn.n_tiles + 3) / 4));
+    if (Dn.format == 0) k_dense_mfma<0><<<g2, kBlock, 0, net->stream>>>(net->dev_table, d, t0);
+    else k_dense_mfma<1><<<g2, kBlock, 0, net->stream>>>(net->dev_table, d, t0);
+}
+
+std::string tune_cache_path() {
+    if (const char *p = std::getenv("KNP_MI355X_TUNE_CACHE")) return p;
+    if (const char *h = std::getenv("HOME")) return std::string(h) + "/.knp_mi355x_tune.txt";
+    return "";
+}
+
+// Picks the time block of a dense projection by timing the candidates on the loaded weights (cost per step
+// = block time / block length) and caches the choice per shape.
+int autotune_dense(knp_net *net, uint32_t d, uint32_t *best_T) {
+    const HostDense &Dh = net->dense[d];
+    DenseDev &Dn = net->table.dense[d];
+    char key[160];
+    std::snprintf(key, sizeof key, "dense fmt=%u n_in=%u tiles=%u inst=%u", Dh.format, Dh.n_in, Dn.n_tiles, Dh.n_inst);
+    const std::string path = tune_cache_path();
+    if (!path.empty()) {
+        std::ifstream in(path);
+        std::string line;
+        while (std::getline(in, line)) {
+            const size_t sp = line.rfind(' ');
+            if (sp != std::string::npos && line.substr(0, sp) == key) {
+                *best_T = (uint32_t)std::strtoul(line.c_str() + sp + 1, nullptr, 10);
+                if (*best_T >= 32 && *best_T % 32 == 0) return KNP_OK;
+            }
+        }
+    }
+    const uint32_t n = net->pops[Dh.post_pop].n;
+    double best = 1e30;
+    *best_T = 64;
+    hipEvent_t a, b;
+    HIP_TRY(hipEventCreate(&a));
+    HIP_TRY(hipEventCreate(&b));
+    for (uint32_t T : {32u, 64u, 128u, 256u, 512u}) {
+        if ((uint64_t)2 * T * n * 4 > (uint64_t)8 << 30) break;  // keep the impact ring under 8 GiB
+        int rc = alloc_dense_buffers(net, d, T);
+        if (rc) return rc;
+        if ((rc = upload_table(net))) return rc;
+        launch_dense(net, d, 0);  // warm
+        HIP_TRY(hipEventRecord(a, net->stream));
+        for (int r = 0; r < 3; ++r) launch_dense(net, d, (uint64_t)r * T);
+        HIP_TRY(hipEventRecord(b, net->stream));
+        HIP_TRY(hipEventSynchronize(b));
+        float ms = 0;
+        HIP_TRY(hipEventElapsedTime(&ms, a, b));
+        const double per_step = ms / (3.0 * T);
+        if (per_step < best) {
+            best = per_step;
+            *best_T = T;
+        }
+    }
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    HIP_TRY(hipMemsetAsync(&net->table.events[0], 0, 16, net->stream));  // tuning spikes are not events
+    if (!path.empty()) {
+        std::ofstream out(path, std::ios::app);
+        if (out) out << key << " " << *best_T << "\n";
+    }
+    return KNP_OK;
+}
+
+int init_dense(knp_net *net, uint32_t d) {
+    const HostDense &Dh = net->dense[d];
+    DenseDev &Dn = net->table.dense[d];
+    const HostPop &P = net->pops[Dh.post_pop];
+    if (P.resource) return fail(KNP_ERR_UNSUPPORTED, "dense input projections target BLIFAT populations only");
+    for (uint32_t e = 0; e < d; ++e)
+        if (net->dense[e].post_pop == Dh.post_pop)
+            return fail(KNP_ERR_UNSUPPORTED, "one dense input projection per population");
+    const uint32_t ks = dense_ksteps(Dh);
+    const uint32_t FB = Dh.format == KNP_DENSE_FP8_E4M3 ? 8 : 16;
+    const uint32_t vb = Dh.format == KNP_DENSE_FP8_E4M3 ? 1 : 2;  // bytes per value
+    std::vector<uint32_t> tinst, tj0, tncol;
+    std::vector<uint64_t> trow;  // first weight row of each tile
+    uint64_t row = 0;
+    for (uint32_t k = 0; k < Dh.n_inst; ++k) {
+        for (uint32_t c = 0; c < Dh.post_count[k]; c += 32) {
+            tinst.push_back(k);
+            tj0.push_back(Dh.post_begin[k] + c);
+            tncol.push_back(std::min(32u, Dh.post_count[k] - c));
+            trow.push_back(row + c);
+        }
+        row += Dh.post_count[k];
+    }
+    const uint32_t n_tiles = (uint32_t)tinst.size();
+    std::vector<uint8_t> frag((size_t)n_tiles * ks * 64 * FB, 0);
+    for (uint32_t tl = 0; tl < n_tiles; ++tl)
+        for (uint32_t kk = 0; kk < ks; ++kk)
+            for (uint32_t lane = 0; lane < 64; ++lane) {
+                const uint32_t col = lane & 31;
+                if (col >= tncol[tl]) continue;
+                uint8_t *dst = &frag[(((size_t)tl * ks + kk) * 64 + lane) * FB];
+                const uint64_t r = trow[tl] + col;
+                for (uint32_t e = 0; e < 8; ++e) {
+                    const uint32_t i = 16 * kk + 8 * (lane >> 5) + e;
+                    if (i >= Dh.n_in) continue;
+                    std::memcpy(dst + e * vb, &Dh.weights[((size_t)r * Dh.n_in + i) * vb], vb);
+                }
+            }
+    std::vector<float> scale(P.n, 1.f);
+    row = 0;
+    for (uint32_t k = 0; k < Dh.n_inst; ++k)
+        for (uint32_t j = 0; j < Dh.post_count[k]; ++j, ++row) scale[Dh.post_begin[k] + j] = Dh.row_scale[row];
+    int rc;
+    uint8_t *fr;
+    uint32_t *a1, *a2, *a3, *a4;
+    float *sc;
+    if ((rc = upload(net, &fr, frag.data(), frag.size()))) return rc;
+    if ((rc = upload(net, &a1, tinst.data(), n_tiles))) return rc;
+    if ((rc = upload(net, &a2, tj0.data(), n_tiles))) return rc;
+    if ((rc = upload(net, &a3, tncol.data(), n_tiles))) return rc;
+    if ((rc = upload(net, &a4, Dh.post_count.data(), Dh.n_inst))) return rc;
+    if ((rc = upload(net, &sc, scale.data(), scale.size()))) return rc;
+    Dn.format = Dh.format;
+    Dn.post_pop = Dh.post_pop;
+    Dn.n_in = Dh.n_in;
+    Dn.ksteps = ks;
+    Dn.n_inst = Dh.n_inst;
+    Dn.n_tiles = n_tiles;
+    Dn.words = 2 * ks;
+    Dn.threshold = Dh.threshold;
+    Dn.hash_offset = Dh.hash_offset;
+    Dn.seed = Dh.seed;
+    Dn.frag = fr;
+    Dn.tile_inst = a1;
+    Dn.tile_j0 = a2;
+    Dn.tile_ncol = a3;
+    Dn.inst_post_count = a4;
+    Dn.row_scale = sc;
+    uint32_t T = Dh.T;
+    if (T == 0) {
+        if ((rc = autotune_dense(net, d, &T))) return rc;
+    }
+    net->dense[d].T = T;
+    if ((rc = alloc_dense_buffers(net, d, T))) return rc;
+    return KNP_OK;
+}
 
 int init_network(knp_net *net) {
     HIP_TRY(hipSetDevice(net->device));
@@ -242,6 +438,15 @@ int init_network(knp_net *net) {
         }
     }
     if (n_pops > 31 && n_blocking) return fail(KNP_ERR_UNSUPPORTED, "BLOCKING ranks support up to 31 populations");
+    for (const auto &Dh : net->dense) {
+        uint64_t row = 0;
+        for (uint32_t k = 0; k < Dh.n_inst; ++k)
+            for (uint32_t j = 0; j < Dh.post_count[k]; ++j, ++row) {
+                double sum = 0;
+                for (uint32_t i = 0; i < Dh.n_in; ++i) sum += std::fabs(dense_value(Dh, row, i));
+                bound[Dh.post_pop][Dh.post_begin[k] + j] += sum * std::fabs((double)Dh.row_scale[row]);
+            }
+    }
 
     // ---- populations
     for (uint32_t p = 0; p < n_pops; ++p) {
@@ -351,7 +556,8 @@ int init_network(knp_net *net) {
             if ((rc = make_param_u(net, n, [&](uint32_t i) { return nr[i].resource_drain_coefficient; }, &R.drain))) return rc;
             if ((rc = make_param_u(net, n, [&](uint32_t i) { return nr[i].isi_max; }, &R.isi_max))) return rc;
         }
-        if ((rc = dev_alloc(net, (void **)&P.spike_list, (size_t)std::max(n, 1u) * 4))) return rc;
+        const uint32_t chunks = std::max(1u, (n + kBlock - 1) / kBlock);
+        if ((rc = dev_alloc(net, (void **)&P.spike_list, (size_t)chunks * kBlock * 4))) return rc;
         if ((rc = dev_alloc(net, (void **)&P.bitmap, (size_t)std::max(P.words, 1u) * 4))) return rc;
         if (H.rec_cap) {
             if ((rc = dev_alloc(net, (void **)&P.rec, (size_t)H.rec_cap * P.words * 4))) return rc;
@@ -485,30 +691,9 @@ int init_network(knp_net *net) {
                 Q.stdp_mode[m] = H.stdp_modes[m];
             }
         }
-        Q.in_cap = std::max(H.n_pre, 1u) + H.gen_channels;
+        Q.in_cap = std::max(H.n_pre, 1u);
         if ((rc = dev_alloc(net, (void **)&Q.in_list, (size_t)Q.in_cap * 4))) return rc;
     }
-
-    // ---- scatter row sources: channel messages first, then population messages in population order
-    uint32_t nrs = 0;
-    for (uint32_t q = 0; q < n_projs; ++q) {
-        const HostProj &H = net->projs[q];
-        if (nrs >= (uint32_t)kMaxRowSources) return fail(KNP_ERR_UNSUPPORTED, "too many row sources");
-        T.rows[nrs++] = RowSource{q, 0xFFFFFFFFu, 0u, 0u};
-        for (uint32_t p = 0; p < n_pops; ++p) {
-            bool sub = (H.pre_pop == p);
-            int mode = -1;
-            for (size_t m = 0; m < H.stdp_pops.size(); ++m)
-                if (H.stdp_pops[m] == p) {
-                    sub = true;
-                    mode = H.stdp_modes[m];
-                }
-            if (!sub || mode == KNP_STDP_ONLY) continue;  // STDPOnly messages are emptied (additive_stdp.h:162-166)
-            if (nrs >= (uint32_t)kMaxRowSources) return fail(KNP_ERR_UNSUPPORTED, "too many row sources");
-            T.rows[nrs++] = RowSource{q, p, 1u + p, 0u};
-        }
-    }
-    T.n_row_sources = nrs;
 
     // ---- sources
     for (uint32_t s = 0; s < T.n_srcs; ++s) {
@@ -527,6 +712,84 @@ int init_network(knp_net *net) {
         int rc;
         if ((rc = upload(net, &th, H.thr.data(), H.thr.size()))) return rc;
         S.thr = th;
+        const uint32_t chunks = std::max(1u, (H.n_ch + kBlock - 1) / kBlock);
+        if ((rc = dev_alloc(net, (void **)&S.list, (size_t)chunks * kBlock * 4))) return rc;
+    }
+
+    // ---- spike segments and the scatter's virtual segment list
+    {
+        int rc;
+        std::vector<uint32_t *> seg_list;
+        std::vector<uint32_t> pop_chunks(n_pops);
+        for (uint32_t p = 0; p < n_pops; ++p) {
+            PopDev &P = T.pops[p];
+            pop_chunks[p] = std::max(1u, (P.n + kBlock - 1) / kBlock);
+            P.seg_base = (uint32_t)seg_list.size();
+            for (uint32_t c = 0; c < pop_chunks[p]; ++c) seg_list.push_back(P.spike_list + (size_t)c * kBlock);
+        }
+        std::vector<uint32_t> vseg, vrs;
+        uint32_t nrs = 0;
+        for (uint32_t q = 0; q < n_projs; ++q) {
+            const HostProj &H = net->projs[q];
+            ProjDev &Q = T.projs[q];
+            // channel messages: host pushes, then every source feeding q
+            Q.push_seg = (uint32_t)seg_list.size();
+            seg_list.push_back(Q.in_list);
+            if (nrs >= (uint32_t)kMaxRowSources) return fail(KNP_ERR_UNSUPPORTED, "too many row sources");
+            T.rows[nrs] = RowSource{q, 0u};
+            vseg.push_back(Q.push_seg);
+            vrs.push_back(nrs);
+            for (uint32_t si = 0; si < T.n_srcs; ++si) {
+                SourceDev &S = T.srcs[si];
+                if (S.proj != q) continue;
+                const uint32_t chunks = std::max(1u, (S.n_ch + kBlock - 1) / kBlock);
+                S.seg_base = (uint32_t)seg_list.size();
+                for (uint32_t c = 0; c < chunks; ++c) {
+                    vseg.push_back((uint32_t)seg_list.size());
+                    vrs.push_back(nrs);
+                    seg_list.push_back(S.list + (size_t)c * kBlock);
+                }
+            }
+            ++nrs;
+            // population messages in population order: the presynaptic population and, for additive STDP,
+            // every stdp population whose messages also produce impacts (STDPOnly messages are emptied,
+            // additive_stdp.h:162-166)
+            for (uint32_t p = 0; p < n_pops; ++p) {
+                bool sub = (H.pre_pop == p);
+                int mode = -1;
+                for (size_t m = 0; m < H.stdp_pops.size(); ++m)
+                    if (H.stdp_pops[m] == p) {
+                        sub = true;
+                        mode = H.stdp_modes[m];
+                    }
+                if (!sub || mode == KNP_STDP_ONLY) continue;
+                if (nrs >= (uint32_t)kMaxRowSources) return fail(KNP_ERR_UNSUPPORTED, "too many row sources");
+                T.rows[nrs] = RowSource{q, 1u + p};
+                for (uint32_t c = 0; c < pop_chunks[p]; ++c) {
+                    vseg.push_back(T.pops[p].seg_base + c);
+                    vrs.push_back(nrs);
+                }
+                ++nrs;
+            }
+        }
+        if (vseg.size() > (size_t)kMaxVSeg) return fail(KNP_ERR_UNSUPPORTED, "too many spike segments for one step");
+        T.n_row_sources = nrs;
+        T.n_seg = (uint32_t)seg_list.size();
+        T.n_vseg = (uint32_t)vseg.size();
+        uint32_t **sl;
+        uint32_t *vs, *vr;
+        if ((rc = upload(net, &sl, seg_list.data(), seg_list.size()))) return rc;
+        T.seg_list = sl;
+        net->seg_list_host = seg_list;
+        if ((rc = upload(net, &vs, vseg.data(), vseg.size()))) return rc;
+        if ((rc = upload(net, &vr, vrs.data(), vrs.size()))) return rc;
+        T.vseg = vs;
+        T.vrs = vr;
+        if ((rc = dev_alloc(net, (void **)&T.seg_count, (size_t)2 * T.n_seg * 4))) return rc;
+        uint64_t max_rows = 0;
+        for (uint32_t v = 0; v < T.n_vseg; ++v) max_rows += kBlock;  // segments hold at most 256 rows, pushes grow
+        net->scatter_grid = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(2048, max_rows));
+        if ((rc = dev_alloc(net, (void **)&T.part, (size_t)2 * net->scatter_grid * 8))) return rc;
     }
 
     // ---- phase-1 tasks
@@ -549,13 +812,12 @@ int init_network(knp_net *net) {
     if ((rc = dev_alloc(net, (void **)&T.ctl, 2 * sizeof(Ctl)))) return rc;
     if ((rc = dev_alloc(net, (void **)&T.events, 16))) return rc;
 
-    // launch shapes: enough waves for the largest step, capped near 8 workgroups per CU
-    uint64_t max_rows = 0;
-    for (uint32_t r = 0; r < nrs; ++r) {
-        const RowSource &R = T.rows[r];
-        max_rows += (R.from_pop == 0xFFFFFFFFu) ? T.projs[R.proj].in_cap : net->pops[R.from_pop].n;
-    }
-    net->scatter_grid = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(2048, (max_rows + 3) / 4));
+    // ---- dense input projections
+    T.n_dense = (uint32_t)net->dense.size();
+    for (uint32_t d = 0; d < T.n_dense; ++d)
+        if ((rc = init_dense(net, d))) return rc;
+
+    // launch shapes
     uint32_t max_res = 1;
     for (const auto &P : net->pops)
         if (P.resource) max_res = std::max(max_res, P.n);
@@ -659,18 +921,22 @@ int deliver_pushes(knp_net *net) {
         for (auto &m : H.pending) L.insert(L.end(), m.begin(), m.end());
         H.pending.clear();
         ProjDev &Q = net->table.projs[q];
-        if (L.size() + H.gen_channels > Q.in_cap) {
+        if (L.size() > Q.in_cap) {
             dev_free(net, Q.in_list);
-            Q.in_cap = (uint32_t)(L.size() + H.gen_channels);
+            Q.in_cap = (uint32_t)L.size();
             int rc = dev_alloc(net, (void **)&Q.in_list, (size_t)Q.in_cap * 4);
             if (rc) return rc;
+            net->seg_list_host[Q.push_seg] = Q.in_list;
+            HIP_TRY(hipMemcpyAsync((void *)(net->table.seg_list + Q.push_seg), &Q.in_list, sizeof(uint32_t *),
+                                   hipMemcpyHostToDevice, net->stream));
+            HIP_TRY(hipStreamSynchronize(net->stream));
             if ((rc = upload_table(net))) return rc;
         }
         if (!L.empty())
             HIP_TRY(hipMemcpyAsync(Q.in_list, L.data(), L.size() * 4, hipMemcpyHostToDevice, net->stream));
         const uint32_t cnt = (uint32_t)L.size();
-        Ctl *ctl = net->table.ctl + (t & 1);
-        HIP_TRY(hipMemcpyAsync(&ctl->in_count[q], &cnt, 4, hipMemcpyHostToDevice, net->stream));
+        uint32_t *slot = net->table.seg_count + (size_t)(t & 1) * net->table.n_seg + Q.push_seg;
+        HIP_TRY(hipMemcpyAsync(slot, &cnt, 4, hipMemcpyHostToDevice, net->stream));
         HIP_TRY(hipStreamSynchronize(net->stream));
     }
     return KNP_OK;
@@ -687,6 +953,11 @@ int run_steps(knp_net *net, uint64_t n_steps) {
         const uint64_t t = net->step;
         int rc = deliver_pushes(net);
         if (rc) return rc;
+        for (uint32_t d = 0; d < net->dense.size(); ++d) {
+            const uint64_t t_first = net->pops[net->dense[d].post_pop].t_first;
+            if ((t - t_first) % net->dense[d].T != 0) continue;
+            if ((rc = timed_launch(net, KNP_KERNEL_DENSE, [&] { launch_dense(net, d, t); }))) return rc;
+        }
         const uint32_t blocks = net->table.total_blocks;
         if ((rc = timed_launch(net, KNP_KERNEL_NEURON, [&] {
                  k_phase1<<<blocks, kBlock, 0, net->stream>>>(D, t);
@@ -707,7 +978,7 @@ int run_steps(knp_net *net, uint64_t n_steps) {
         }
         const uint32_t sg = net->scatter_grid;
         if ((rc = timed_launch(net, KNP_KERNEL_SCATTER, [&] {
-                 k_scatter<<<sg, kBlock, 0, net->stream>>>(D, t);
+                 k_scatter<<<sg, kBlock, (net->table.n_vseg + 1) * 4, net->stream>>>(D, t);
              })))
             return rc;
         for (uint32_t q = 0; q < net->projs.size(); ++q) {
@@ -1038,6 +1309,57 @@ int knp_mi355x_add_bernoulli_source(knp_net *net, uint32_t proj, uint32_t first_
     return KNP_OK;
 }
 
+int knp_mi355x_add_dense_input_projection(knp_net *net, uint32_t post_pop, uint32_t n_instances,
+                                          const uint32_t *post_begin, const uint32_t *post_count, uint32_t n_in,
+                                          int format, const void *weights, const float *row_scale,
+                                          uint32_t threshold, uint64_t seed, uint32_t hash_offset,
+                                          uint32_t time_block, uint32_t *out_dense) {
+    int rc = check_pop(net, post_pop);
+    if (rc) return rc;
+    if (net->initialized) return fail(KNP_ERR_STATE, "projections must be loaded before the first step");
+    if (format != KNP_DENSE_FP8_E4M3 && format != KNP_DENSE_BF16) return fail(KNP_ERR_UNSUPPORTED, "dense format");
+    if (!n_instances || !post_begin || !post_count || !weights || n_in == 0)
+        return fail(KNP_ERR_INVALID_ARGUMENT, "bad dense projection arguments");
+    if (time_block % 32 != 0) return fail(KNP_ERR_INVALID_ARGUMENT, "time_block must be a multiple of 32");
+    if (net->dense.size() >= (size_t)kMaxDense) return fail(KNP_ERR_UNSUPPORTED, "too many dense projections");
+    const uint32_t n = net->pops[post_pop].n;
+    HostDense D;
+    D.post_pop = post_pop;
+    D.n_inst = n_instances;
+    D.n_in = n_in;
+    D.format = (uint32_t)format;
+    D.threshold = threshold;
+    D.seed = seed;
+    D.hash_offset = hash_offset;
+    D.T = time_block;
+    D.post_begin.assign(post_begin, post_begin + n_instances);
+    D.post_count.assign(post_count, post_count + n_instances);
+    uint64_t rows = 0;
+    for (uint32_t k = 0; k < n_instances; ++k) {
+        if ((uint64_t)post_begin[k] + post_count[k] > n)
+            return fail(KNP_ERR_INVALID_ARGUMENT, "dense instance range outside the population");
+        rows += post_count[k];
+    }
+    const size_t vb = format == KNP_DENSE_FP8_E4M3 ? 1 : 2;
+    D.weights.assign((const uint8_t *)weights, (const uint8_t *)weights + rows * n_in * vb);
+    D.row_scale.assign(rows, 1.f);
+    if (row_scale) D.row_scale.assign(row_scale, row_scale + rows);
+    for (uint64_t r = 0; r < rows; ++r)
+        for (uint32_t i = 0; i < n_in; ++i)
+            if (!std::isfinite(dense_value(D, r, i))) return fail(KNP_ERR_INVALID_ARGUMENT, "non-finite dense weight");
+    net->dense.push_back(std::move(D));
+    if (out_dense) *out_dense = (uint32_t)net->dense.size() - 1;
+    return KNP_OK;
+}
+
+int knp_mi355x_get_dense_time_block(knp_net *net, uint32_t dense, uint32_t *time_block) {
+    if (!net || !time_block || dense >= net->dense.size()) return fail(KNP_ERR_INVALID_ARGUMENT, "bad dense index");
+    int rc = ensure_init(net);
+    if (rc) return rc;
+    *time_block = net->dense[dense].T;
+    return KNP_OK;
+}
+
 int knp_mi355x_step(knp_net *net, uint64_t n_steps) {
     if (!net) return fail(KNP_ERR_INVALID_ARGUMENT, "null handle");
     return run_steps(net, n_steps);
@@ -1277,26 +1599,36 @@ int knp_mi355x_read_resource_rules(knp_net *net, uint32_t proj, knp_resource_rul
     return KNP_OK;
 }
 
+static int read_counters(knp_net *net, uint64_t *events, uint64_t *rows) {
+    unsigned long long c[2] = {0, 0};
+    *events = *rows = 0;
+    if (!net->initialized) return KNP_OK;
+    int rc = download(net, c, net->table.events, 2);
+    if (rc) return rc;
+    std::vector<unsigned long long> part(2 * (size_t)net->scatter_grid);
+    if ((rc = download(net, part.data(), net->table.part, part.size()))) return rc;
+    *events = c[0];
+    *rows = c[1];
+    for (size_t b = 0; b < net->scatter_grid; ++b) {
+        *events += part[2 * b];
+        *rows += part[2 * b + 1];
+    }
+    return KNP_OK;
+}
+
 int knp_mi355x_get_event_count(knp_net *net, uint64_t *events) {
     if (!net || !events) return fail(KNP_ERR_INVALID_ARGUMENT, "null pointer");
-    *events = 0;
-    if (!net->initialized) return KNP_OK;
-    unsigned long long e = 0;
-    int rc = download(net, &e, net->table.events, 1);
-    *events = e;
-    return rc;
+    uint64_t rows;
+    return read_counters(net, events, &rows);
 }
 
 int knp_mi355x_get_counters(knp_net *net, uint64_t *events, uint64_t *rows) {
     if (!net) return fail(KNP_ERR_INVALID_ARGUMENT, "null handle");
-    unsigned long long c[2] = {0, 0};
-    if (net->initialized) {
-        int rc = download(net, c, net->table.events, 2);
-        if (rc) return rc;
-    }
-    if (events) *events = c[0];
-    if (rows) *rows = c[1];
-    return KNP_OK;
+    uint64_t e, r;
+    int rc = read_counters(net, &e, &r);
+    if (events) *events = e;
+    if (rows) *rows = r;
+    return rc;
 }
 
 int knp_mi355x_get_accumulator_bits(knp_net *net, uint32_t pop, int32_t *frac_bits) {

@@ -66,6 +66,24 @@ __device__ __forceinline__ void wave_append(bool pred, uint32_t value, uint32_t 
     if (pred) list[base + __popcll(mask & ((1ull << lane) - 1ull))] = value;
 }
 
+// Ordered compaction of a workgroup's 256 candidates into its own segment: ascending order, no global
+// atomics, one count store.
+__device__ __forceinline__ void block_segment_write(bool pred, uint32_t value, uint32_t *segment, uint32_t *count) {
+    __shared__ uint32_t wcnt[kBlock / 64];
+    const uint64_t mask = __ballot(pred);
+    const uint32_t lane = lane_id(), w = threadIdx.x / 64;
+    if (lane == 0) wcnt[w] = (uint32_t)__popcll(mask);
+    __syncthreads();
+    uint32_t off = 0, total = 0;
+    for (uint32_t k = 0; k < kBlock / 64; ++k) {
+        if (k < w) off += wcnt[k];
+        total += wcnt[k];
+    }
+    if (pred) segment[off + __popcll(mask & ((1ull << lane) - 1ull))] = value;
+    if (threadIdx.x == 0) *count = total;
+    __syncthreads();
+}
+
 // Write the wave's 64-neuron spike mask into the bitmap words (and the record), bit i = neuron i.
 __device__ __forceinline__ void wave_bitmap(uint64_t mask, uint32_t first_neuron, const PopDev &P, uint64_t t) {
     uint32_t lane = lane_id();
@@ -91,7 +109,8 @@ __device__ __forceinline__ int64_t blocking_value(const NetDev *net, uint64_t ke
 
 // ---- phase 1: populations + sources ----------------------------------------------------------------
 template <uint32_t KIND>
-__device__ __forceinline__ void population_step(const NetDev *__restrict__ net, uint32_t p, uint32_t local_block, uint64_t t, Ctl *ctl) {
+__device__ __forceinline__ void population_step(const NetDev *__restrict__ net, uint32_t p, uint32_t local_block,
+                                                uint64_t t, Ctl *ctl, uint32_t *seg_count) {
     const PopDev &P = net->pops[p];
     const uint32_t i = local_block * kBlock + threadIdx.x;
     const bool active = i < P.n;
@@ -105,6 +124,7 @@ __device__ __forceinline__ void population_step(const NetDev *__restrict__ net, 
             a = P.acc_v[ai];
             P.acc_v[ai] = 0;
         }
+        if (P.ff) a += __float2ll_rn(P.ff[(size_t)(t % (2ull * P.ff_T)) * P.n + i] * P.scale);
         if (KIND == POP_LEAN) {
             // default BLIFAT (blifat.h defaults): V_pre = 0, no decays, no refractory period, no blocking.
             const bool can = !(P.lean_ts_max && t == P.t_first);  // step 0 rule (wrap of SIZE_MAX)
@@ -216,14 +236,14 @@ __device__ __forceinline__ void population_step(const NetDev *__restrict__ net, 
         }
     }
     // spike compaction and bitmaps (whole wave participates)
-    wave_append(spike, i, &ctl->pop_count[p], P.spike_list);
+    block_segment_write(spike, i, P.spike_list + (size_t)local_block * kBlock, &seg_count[P.seg_base + local_block]);
     const uint64_t mask = __ballot(spike);
     wave_bitmap(mask, local_block * kBlock + (threadIdx.x & ~63u), P, t);
     if (KIND == POP_RESOURCE) wave_append(cand, i, &ctl->cand_count[p], P.res.cand);
 }
 
 __device__ __forceinline__ void source_step(const NetDev *__restrict__ net, uint32_t s, uint32_t local_block,
-                                            uint64_t t, Ctl *ctl) {
+                                            uint64_t t, uint32_t *seg_count) {
     const SourceDev &S = net->srcs[s];
     const uint32_t c = local_block * kBlock + threadIdx.x;
     const uint32_t phase = (uint32_t)(t % S.steps_per_pattern);
@@ -232,15 +252,18 @@ __device__ __forceinline__ void source_step(const NetDev *__restrict__ net, uint
         const uint32_t pattern = (uint32_t)((t / S.steps_per_pattern) % S.n_patterns);
         spike = bernoulli_hash(S.seed, t, S.hash_offset + c) < S.thr[(size_t)pattern * S.n_ch + c];
     }
-    const ProjDev &Q = net->projs[S.proj];
-    wave_append(spike, S.first + c, &ctl->in_count[S.proj], Q.in_list);
+    block_segment_write(spike, S.first + c, S.list + (size_t)local_block * kBlock, &seg_count[S.seg_base + local_block]);
 }
 
 __global__ __launch_bounds__(kBlock) void k_phase1(const NetDev *__restrict__ net, uint64_t t) {
     Ctl *ctl = &net->ctl[t & 1];
+    uint32_t *seg_count = net->seg_count + (size_t)(t & 1) * net->n_seg;
     if (blockIdx.x == 0) {
         uint32_t *nxt = (uint32_t *)&net->ctl[(t + 1) & 1];
         for (uint32_t k = threadIdx.x; k < sizeof(Ctl) / 4; k += blockDim.x) nxt[k] = 0;
+        // next step's counts: host-pushed segments must start empty (every other segment is rewritten)
+        uint32_t *nc = net->seg_count + (size_t)((t + 1) & 1) * net->n_seg;
+        for (uint32_t k = threadIdx.x; k < net->n_projs; k += blockDim.x) nc[net->projs[k].push_seg] = 0;
     }
     const uint32_t b = blockIdx.x;
     uint32_t task = 0;
@@ -249,92 +272,130 @@ __global__ __launch_bounds__(kBlock) void k_phase1(const NetDev *__restrict__ ne
     const TaskDev &T = net->tasks[task];
     const uint32_t lb = b - T.block_begin;
     if (T.kind == 1) {
-        source_step(net, T.index, lb, t, ctl);
+        source_step(net, T.index, lb, t, seg_count);
         return;
     }
     switch (net->pops[T.index].kind) {
-        case POP_LEAN: population_step<POP_LEAN>(net, T.index, lb, t, ctl); break;
-        case POP_FULL: population_step<POP_FULL>(net, T.index, lb, t, ctl); break;
-        default: population_step<POP_RESOURCE>(net, T.index, lb, t, ctl); break;
+        case POP_LEAN: population_step<POP_LEAN>(net, T.index, lb, t, ctl, seg_count); break;
+        case POP_FULL: population_step<POP_FULL>(net, T.index, lb, t, ctl, seg_count); break;
+        default: population_step<POP_RESOURCE>(net, T.index, lb, t, ctl, seg_count); break;
     }
 }
 
 // ---- scatter: spike rows -> impacts in the delay rings (delta_impl.h:54-114) -------------------------
+// Every workgroup first scans the spike counts of all virtual segments into LDS (one parallel load per
+// thread), so the spiking rows of the step are numbered 0..total-1 without any global synchronization.
+// One workgroup then expands one row at a time: each thread loads up to 4 synapses before issuing its
+// atomics, so a 1,000-synapse row costs one memory round trip.
 // BLOCKING keys order impacts as the reference delivers them (the "last" impact wins, blifat_impl.h:86-88):
 // projection load order, then creation step (later = shorter delay), then message, then presynaptic index,
 // then position in the row.  key = 1 + [blocking projection:6][64-delay:6][message rank:5][pre:29][pos:17].
-__global__ __launch_bounds__(kBlock) void k_scatter(const NetDev *__restrict__ net, uint64_t t) {
-    __shared__ uint32_t prefix[kMaxRowSources + 1];
-    const Ctl *ctl = &net->ctl[t & 1];
-    const uint32_t n_rs = net->n_row_sources;
-    if (threadIdx.x == 0) {
-        uint32_t acc = 0;
-        for (uint32_t r = 0; r < n_rs; ++r) {
-            prefix[r] = acc;
-            const RowSource &R = net->rows[r];
-            acc += (R.from_pop == 0xFFFFFFFFu) ? ctl->in_count[R.proj] : ctl->pop_count[R.from_pop];
+__device__ __forceinline__ void impact(const PopDev &P, const ProjDev &Q, const RowSource &R, uint32_t slot0,
+                                       uint32_t pre, uint64_t pos, uint32_t pt, uint32_t d, float w) {
+    const uint32_t post = pt & 0x1FFFFFFFu;
+    const uint32_t type = pt >> 29;
+    uint32_t slot = slot0 + d;
+    if (slot >= P.D) slot -= P.D;
+    const size_t ai = (size_t)slot * P.n + post;
+    const long long v = __float2ll_rn(w * P.scale);
+    switch (type) {
+        case 0:  // EXCITATORY
+            atomicAdd((unsigned long long *)&P.acc_v[ai], (unsigned long long)v);
+            if (Q.forcing && P.acc_f) atomicOr(&P.acc_f[ai], 1u);
+            break;
+        case 1:  // INHIBITORY_CURRENT
+            atomicAdd((unsigned long long *)&P.acc_v[ai], (unsigned long long)(-v));
+            break;
+        case 2:  // INHIBITORY_CONDUCTANCE
+            atomicAdd((unsigned long long *)&P.acc_g[ai], (unsigned long long)v);
+            break;
+        case 3:  // DOPAMINE
+            atomicAdd((unsigned long long *)&P.acc_d[ai], (unsigned long long)v);
+            break;
+        default: {  // BLOCKING
+            const uint64_t key = 1ull + (((uint64_t)Q.blocking_id << 57) | ((uint64_t)(64u - d) << 51) |
+                                         ((uint64_t)R.rank << 46) | ((uint64_t)pre << 17) | pos);
+            atomicMax((unsigned long long *)&P.acc_b[ai], (unsigned long long)key);
+            break;
         }
-        prefix[n_rs] = acc;
     }
+}
+
+__global__ __launch_bounds__(kBlock) void k_scatter(const NetDev *__restrict__ net, uint64_t t) {
+    extern __shared__ uint32_t pfx[];  // [n_vseg + 1]
+    __shared__ uint32_t wsum[kBlock / 64];
+    const uint32_t nv = net->n_vseg;
+    const uint32_t *cnt = net->seg_count + (size_t)(t & 1) * net->n_seg;
+    const uint32_t lane = lane_id(), w = threadIdx.x / 64;
+    for (uint32_t v = threadIdx.x; v < nv; v += kBlock) pfx[v] = cnt[net->vseg[v]];
     __syncthreads();
-    const uint32_t total = prefix[n_rs];
-    const uint32_t lane = lane_id();
-    const uint32_t waves = gridDim.x * (kBlock / 64);
-    for (uint32_t row = blockIdx.x * (kBlock / 64) + threadIdx.x / 64; row < total; row += waves) {
-        uint32_t rs = 0;
-        while (rs + 1 < n_rs && prefix[rs + 1] <= row) ++rs;
-        const RowSource &R = net->rows[rs];
-        const uint32_t idx = row - prefix[rs];
+    uint32_t carry = 0;
+    for (uint32_t base = 0; base < nv; base += kBlock) {
+        const uint32_t v = base + threadIdx.x;
+        const uint32_t c = v < nv ? pfx[v] : 0u;
+        uint32_t x = c;
+        for (int off = 1; off < 64; off <<= 1) {
+            const uint32_t y = __shfl_up(x, off);
+            if ((int)lane >= off) x += y;
+        }
+        if (lane == 63) wsum[w] = x;
+        __syncthreads();
+        uint32_t woff = 0, btot = 0;
+        for (uint32_t k = 0; k < kBlock / 64; ++k) {
+            if (k < w) woff += wsum[k];
+            btot += wsum[k];
+        }
+        if (v < nv) pfx[v] = carry + woff + x - c;
+        carry += btot;
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) pfx[nv] = carry;
+    __syncthreads();
+    const uint32_t total = pfx[nv];
+    unsigned long long my_ev = 0, my_rows = 0;
+    for (uint32_t row = blockIdx.x; row < total; row += gridDim.x) {
+        uint32_t lo = 0, hi = nv;  // largest v with pfx[v] <= row
+        while (hi - lo > 1) {
+            const uint32_t mid = (lo + hi) >> 1;
+            if (pfx[mid] <= row) lo = mid;
+            else hi = mid;
+        }
+        const uint32_t seg = net->vseg[lo];
+        const RowSource &R = net->rows[net->vrs[lo]];
+        const uint32_t pre = net->seg_list[seg][row - pfx[lo]];
         const ProjDev &Q = net->projs[R.proj];
-        const uint32_t pre = (R.from_pop == 0xFFFFFFFFu) ? Q.in_list[idx] : net->pops[R.from_pop].spike_list[idx];
         if (pre >= Q.n_pre) continue;
         const uint64_t b = Q.row_ptr[pre], e = Q.row_ptr[pre + 1];
-        if (lane == 0) {
-            if (e > b) atomicAdd(&net->events[0], (unsigned long long)(e - b));
-            atomicAdd(&net->events[1], 1ull);
-        }
+        my_ev += e - b;
+        my_rows += 1;
         const PopDev &P = net->pops[Q.post_pop];
         const uint32_t slot0 = (uint32_t)(t % P.D);
-        const float scale = P.scale;
-        for (uint64_t k = b + lane; k < e; k += 64) {
-            const uint32_t pt = Q.post_type[k];
-            const uint32_t post = pt & 0x1FFFFFFFu;
-            const uint32_t type = pt >> 29;
-            const uint32_t d = Q.delay[k];
-            float w;
-            if (Q.kind == PROJ_DELTA) {
-                w = Q.w_csr[k];
-            } else {
-                const uint32_t s = Q.csr_syn[k];
-                w = Q.W[s];
-                if (Q.kind == PROJ_RESOURCE) Q.last_spike_step[s] = t;  // init_synapse (synaptic_resource_stdp.h:36-41)
-            }
-            uint32_t slot = slot0 + d;
-            if (slot >= P.D) slot -= P.D;
-            const size_t ai = (size_t)slot * P.n + post;
-            const long long v = __float2ll_rn(w * scale);
-            switch (type) {
-                case 0:  // EXCITATORY
-                    atomicAdd((unsigned long long *)&P.acc_v[ai], (unsigned long long)v);
-                    if (Q.forcing && P.acc_f) atomicOr(&P.acc_f[ai], 1u);
-                    break;
-                case 1:  // INHIBITORY_CURRENT
-                    atomicAdd((unsigned long long *)&P.acc_v[ai], (unsigned long long)(-v));
-                    break;
-                case 2:  // INHIBITORY_CONDUCTANCE
-                    atomicAdd((unsigned long long *)&P.acc_g[ai], (unsigned long long)v);
-                    break;
-                case 3:  // DOPAMINE
-                    atomicAdd((unsigned long long *)&P.acc_d[ai], (unsigned long long)v);
-                    break;
-                default: {  // BLOCKING
-                    const uint64_t key = 1ull + (((uint64_t)Q.blocking_id << 57) | ((uint64_t)(64u - d) << 51) |
-                                                 ((uint64_t)R.rank << 46) | ((uint64_t)pre << 17) | (uint64_t)(k - b));
-                    atomicMax((unsigned long long *)&P.acc_b[ai], (unsigned long long)key);
-                    break;
+        for (uint64_t base = b; base < e; base += 4 * kBlock) {
+            uint32_t pt[4], d[4];
+            float wv[4];
+            bool ok[4];
+            for (int j = 0; j < 4; ++j) {
+                const uint64_t k = base + threadIdx.x + (uint64_t)j * kBlock;
+                ok[j] = k < e;
+                if (ok[j]) {
+                    pt[j] = Q.post_type[k];
+                    d[j] = Q.delay[k];
+                    if (Q.kind == PROJ_DELTA) {
+                        wv[j] = Q.w_csr[k];
+                    } else {
+                        const uint32_t s = Q.csr_syn[k];
+                        wv[j] = Q.W[s];
+                        if (Q.kind == PROJ_RESOURCE) Q.last_spike_step[s] = t;  // init_synapse
+                    }
                 }
             }
+            for (int j = 0; j < 4; ++j)
+                if (ok[j]) impact(P, Q, R, slot0, pre, base + threadIdx.x + (uint64_t)j * kBlock - b, pt[j], d[j], wv[j]);
         }
+    }
+    if (threadIdx.x == 0) {
+        net->part[2 * blockIdx.x] += my_ev;
+        net->part[2 * blockIdx.x + 1] += my_rows;
     }
 }
 
@@ -547,17 +608,19 @@ __global__ __launch_bounds__(kBlock) void k_train(const NetDev *__restrict__ net
 // every synapse whose POSTsynaptic index equals the spiking neuron's index (additive_stdp.h:149-160).
 __global__ __launch_bounds__(kBlock) void k_additive_append(const NetDev *__restrict__ net, uint32_t q, uint64_t t) {
     const ProjDev &Q = net->projs[q];
-    const Ctl *ctl = &net->ctl[t & 1];
+    const uint32_t *cnt = net->seg_count + (size_t)(t & 1) * net->n_seg;
     const uint32_t lane = lane_id();
     const uint32_t waves = gridDim.x * (kBlock / 64);
     const uint32_t wave = blockIdx.x * (kBlock / 64) + threadIdx.x / 64;
     const uint32_t n_post = net->pops[Q.post_pop].n;
     for (uint32_t m = 0; m < Q.n_stdp; ++m) {
-        const uint32_t sp = Q.stdp_pop[m];
-        const uint32_t cnt = ctl->pop_count[sp];
-        const uint32_t *list = net->pops[sp].spike_list;
-        for (uint32_t r = wave; r < cnt; r += waves) {
-            const uint32_t nidx = list[r];
+        const PopDev &SP = net->pops[Q.stdp_pop[m]];
+        const uint32_t chunks = (SP.n + kBlock - 1) / kBlock;
+        // one wave per spike: walk (chunk, index) pairs
+        for (uint32_t item = wave; item < chunks * kBlock; item += waves) {
+            const uint32_t chunk = item / kBlock, r = item % kBlock;
+            if (r >= cnt[SP.seg_base + chunk]) continue;
+            const uint32_t nidx = SP.spike_list[(size_t)chunk * kBlock + r];
             if (nidx >= n_post) continue;
             const uint64_t cb = Q.col_ptr[nidx], ce = Q.col_ptr[nidx + 1];
             for (uint64_t k = cb + lane; k < ce; k += 64) {
@@ -596,6 +659,109 @@ __global__ __launch_bounds__(kBlock) void k_additive_modify(const NetDev *__rest
             Q.W[s] += w_j;
             Q.n_pre_t[s] = 0;
             Q.n_post_t[s] = 0;
+        }
+    }
+}
+
+
+// ---- dense input projections on MFMA ---------------------------------------------------------------
+// The binary spike matrix of a time block (T steps x n_in inputs per instance) is exact in FP8 (1.0 = 0x38)
+// and BF16 (1.0 = 0x3F80); the products with the weights are exact and FP32 accumulation of a few hundred
+// of them is exact for weights on a 2^-k grid, so impacts equal the reference's sum of the same weights.
+typedef float v16f __attribute__((ext_vector_type(16)));
+typedef short v8s __attribute__((ext_vector_type(8)));
+
+// Spike bits of steps t0 .. t0+T-1: byte b of (step, instance) holds inputs 8b .. 8b+7.
+__global__ __launch_bounds__(kBlock) void k_dense_spikes(const NetDev *__restrict__ net, uint32_t d, uint64_t t0) {
+    const DenseDev &Dn = net->dense[d];
+    const uint32_t bytes_per = 2u * Dn.ksteps;
+    const uint64_t total = (uint64_t)Dn.T * Dn.n_inst * bytes_per;
+    const uint64_t stride = (uint64_t)gridDim.x * kBlock;
+    for (uint64_t base = (uint64_t)blockIdx.x * kBlock; base < total; base += stride) {
+        const uint64_t idx = base + threadIdx.x;
+        uint32_t v = 0;
+        unsigned long long ev = 0, rows = 0;
+        if (idx < total) {
+            const uint32_t byte = (uint32_t)(idx % bytes_per);
+            const uint64_t rest = idx / bytes_per;
+            const uint32_t inst = (uint32_t)(rest % Dn.n_inst);
+            const uint64_t t = t0 + rest / Dn.n_inst;
+            for (uint32_t b = 0; b < 8; ++b) {
+                const uint32_t i = byte * 8 + b;
+                if (i < Dn.n_in && bernoulli_hash(Dn.seed, t, Dn.hash_offset + inst * Dn.n_in + i) < Dn.threshold)
+                    v |= 1u << b;
+            }
+            Dn.bits[idx] = (uint8_t)v;
+            rows = __popc(v);
+            ev = rows * Dn.inst_post_count[inst];
+        }
+        for (int off = 32; off > 0; off >>= 1) {
+            ev += __shfl_xor(ev, off);
+            rows += __shfl_xor(rows, off);
+        }
+        if (lane_id() == 0 && rows) {
+            atomicAdd(&net->events[0], ev);
+            atomicAdd(&net->events[1], rows);
+        }
+    }
+}
+
+__device__ __forceinline__ long spread_fp8_ones(uint32_t byte) {
+    uint64_t x = byte;
+    x = (x | (x << 28)) & 0x0000000F0000000Full;
+    x = (x | (x << 14)) & 0x0003000300030003ull;
+    x = (x | (x << 7)) & 0x0101010101010101ull;
+    return (long)(x * 0x38ull);  // fp8 e4m3 1.0 in every byte whose bit is set
+}
+
+__device__ __forceinline__ v8s spread_bf16_ones(uint32_t byte) {
+    v8s r;
+    for (int e = 0; e < 8; ++e) r[e] = ((byte >> e) & 1u) ? (short)0x3F80 : (short)0;
+    return r;
+}
+
+// One wave per 32-column tile; it walks the block's 32-step row tiles. Output element (row m, column n) of
+// the 32x32 accumulator sits in lane n + 32 * ((m / 4) % 2), register 4 * (m / 8) + m % 4.
+template <int FMT>
+__global__ __launch_bounds__(kBlock) void k_dense_mfma(const NetDev *__restrict__ net, uint32_t d, uint64_t t0) {
+    const DenseDev &Dn = net->dense[d];
+    const PopDev &P = net->pops[Dn.post_pop];
+    constexpr uint32_t FB = FMT == 0 ? 8 : 16;  // fragment bytes per lane per K-step
+    const uint32_t lane = lane_id();
+    const uint32_t waves = gridDim.x * (kBlock / 64);
+    const uint32_t bytes_per = 2u * Dn.ksteps;
+    const uint64_t ring = 2ull * Dn.T;
+    for (uint32_t tile = blockIdx.x * (kBlock / 64) + threadIdx.x / 64; tile < Dn.n_tiles; tile += waves) {
+        const uint32_t inst = Dn.tile_inst[tile];
+        const uint32_t j0 = Dn.tile_j0[tile];
+        const uint32_t ncol = Dn.tile_ncol[tile];
+        const uint8_t *fr = Dn.frag + (size_t)tile * Dn.ksteps * 64 * FB + (size_t)lane * FB;
+        const uint32_t col = lane & 31;
+        const float sc = col < ncol ? Dn.row_scale[j0 + col] : 0.f;
+        for (uint32_t tt = 0; tt < Dn.T / 32; ++tt) {
+            v16f acc;
+            for (int v = 0; v < 16; ++v) acc[v] = 0.f;
+            const uint32_t row = tt * 32 + (lane & 31);
+            const uint8_t *brow = Dn.bits + ((size_t)row * Dn.n_inst + inst) * bytes_per + (lane >> 5);
+            for (uint32_t kk = 0; kk < Dn.ksteps; ++kk) {
+                const uint32_t sb = brow[2 * kk];
+                if (FMT == 0) {
+                    const long a = spread_fp8_ones(sb);
+                    const long b = *(const long *)(fr + (size_t)kk * 64 * FB);
+                    acc = __builtin_amdgcn_mfma_f32_32x32x16_fp8_fp8(a, b, acc, 0, 0, 0);
+                } else {
+                    const v8s a = spread_bf16_ones(sb);
+                    const v8s b = *(const v8s *)(fr + (size_t)kk * 64 * FB);
+                    acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc, 0, 0, 0);
+                }
+            }
+            if (col < ncol) {
+                for (int v = 0; v < 16; ++v) {
+                    const uint32_t m = 8u * (v / 4) + 4u * (lane >> 5) + (v % 4);
+                    const uint64_t arrival = t0 + tt * 32 + m + 1;
+                    Dn.ff[(size_t)(arrival % ring) * P.n + j0 + col] = acc[v] * sc;
+                }
+            }
         }
     }
 }

@@ -171,6 +171,37 @@ class Oracle:
         self._chk(self.lib.ko_add_bernoulli_source(self.net, proj, first_index, n_ch, abi.u32p(thr), n_patterns,
                                                    steps_per_pattern, active_lo, hi, seed, hash_offset))
 
+    def add_dense_input_projection(self, post_pop, post_begin, post_count, n_in, fmt, weights, row_scale,
+                                   threshold, seed, hash_offset=0, time_block=0):
+        """The dense projection stated as what it stands for: a Projection<DeltaSynapse> from channel block k
+        (indexes k * n_in + i) to every neuron of instance k with weight row_scale[j] * w(j, i), EXCITATORY,
+        delay 1, fed by one Bernoulli source over all channels."""
+        pb = np.asarray(post_begin, dtype=np.int64)
+        pc = np.asarray(post_count, dtype=np.int64)
+        K = len(pb)
+        rows = int(pc.sum())
+        if fmt == abi.DENSE_FP8_E4M3:
+            w = abi.e4m3_decode(np.asarray(weights).reshape(rows, n_in))
+        else:
+            w = abi.bf16_decode(np.asarray(weights).reshape(rows, n_in)).astype(np.float64)
+        sc = np.ones(rows) if row_scale is None else np.asarray(row_scale, dtype=np.float64)
+        wf = (w * sc[:, None]).astype(np.float32)
+        pre, post, wl = [], [], []
+        r0 = 0
+        for k in range(K):
+            jj, ii = np.meshgrid(np.arange(pc[k]), np.arange(n_in), indexing="ij")
+            post.append((pb[k] + jj).ravel())
+            pre.append((k * n_in + ii).ravel())
+            wl.append(wf[r0:r0 + pc[k]].ravel())
+            r0 += pc[k]
+        pre = np.concatenate(pre)
+        post = np.concatenate(post)
+        wl = np.concatenate(wl)
+        q = self.add_delta_projection(abi.KNP_NO_POPULATION, K * n_in, post_pop, pre, post, wl,
+                                      np.ones(len(pre), np.uint32))
+        self.add_bernoulli_source(q, np.full(K * n_in, threshold, np.uint32), seed=seed, hash_offset=hash_offset)
+        return q
+
     def step(self, n=1):
         self._chk(self.lib.ko_step(self.net, n))
 

@@ -328,3 +328,28 @@ def test_batched_instances_match_standalone():
         assert agreement(sg[:, b * 1024:(b + 1) * 1024], so) >= 0.999
         n = part.proj_sizes[0]
         assert np.max(np.abs(wg[b * n:(b + 1) * n] - o.weights(0))) <= 1e-6
+
+
+@pytest.mark.parametrize("fmt,time_block", [(abi.DENSE_FP8_E4M3, 64), (abi.DENSE_BF16, 32), (abi.DENSE_FP8_E4M3, 0)])
+def test_dense_input_projection(fmt, time_block):
+    """Dense feed-forward projection on MFMA (uneven instances, n_in not a multiple of 16) against the same
+    projection stated as a sparse Delta projection on the oracle: spikes bit-exact, impacts counted equal."""
+    say(f"case: dense input projection fmt={fmt} T={time_block}")
+    spec = workloads.uneven_inference(n_instances=5, n_min=20, n_max=150, n_in=50, fmt=fmt, time_block=time_block,
+                                      p_in=0.1, threshold=0.6, w_inh=0.4)
+    steps = 150
+    g, o = both(spec)
+    g.record_spikes(0, steps)
+    o.record_spikes(0, steps)
+    g.step(steps)
+    o.step(steps)
+    sg, so = g.read_spike_record(0), o.read_spike_record(0)
+    assert so.sum() > 100
+    assert np.array_equal(sg, so), np.argwhere(sg != so)[:10]
+    assert rel_err(g.potentials(0), o.potentials(0)) <= POT_RTOL
+    if time_block:
+        # whole time blocks were expanded on the device: compare impact counts at a block boundary
+        g.step((-steps) % time_block)
+        o.step((-steps) % time_block)
+        assert g.events == o.events
+    assert g.dense_time_block(0) % 32 == 0
