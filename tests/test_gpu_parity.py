@@ -336,7 +336,7 @@ def test_dense_input_projection(fmt, time_block):
     projection stated as a sparse Delta projection on the oracle: spikes bit-exact, impacts counted equal."""
     say(f"case: dense input projection fmt={fmt} T={time_block}")
     spec = workloads.uneven_inference(n_instances=5, n_min=20, n_max=150, n_in=50, fmt=fmt, time_block=time_block,
-                                      p_in=0.1, threshold=0.6, w_inh=0.4)
+                                      p_in=0.4, threshold=0.6, w_inh=0.4)
     steps = 150
     g, o = both(spec)
     g.record_spikes(0, steps)
