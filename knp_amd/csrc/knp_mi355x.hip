@@ -1345,8 +1345,13 @@ int knp_mi355x_add_dense_input_projection(knp_net *net, uint32_t post_pop, uint3
     D.row_scale.assign(rows, 1.f);
     if (row_scale) D.row_scale.assign(row_scale, row_scale + rows);
     for (uint64_t r = 0; r < rows; ++r)
-        for (uint32_t i = 0; i < n_in; ++i)
+        for (uint32_t i = 0; i < n_in; ++i) {
             if (!std::isfinite(dense_value(D, r, i))) return fail(KNP_ERR_INVALID_ARGUMENT, "non-finite dense weight");
+            // the FP8 matrix instructions flush subnormal operands: such codes would not mean what they say
+            if (format == KNP_DENSE_FP8_E4M3 && (D.weights[r * n_in + i] & 0x78) == 0 && (D.weights[r * n_in + i] & 7))
+                return fail(KNP_ERR_UNSUPPORTED, "FP8 subnormal weight codes are flushed by the matrix cores; "
+                                                 "quantize without subnormals");
+        }
     net->dense.push_back(std::move(D));
     if (out_dense) *out_dense = (uint32_t)net->dense.size() - 1;
     return KNP_OK;

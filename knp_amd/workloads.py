@@ -309,13 +309,14 @@ def compose(parts, name, per_instance_steps):
 
 def fp8_rows(w):
     """Per-row FP8 e4m3 quantization with power-of-two row scales (row = postsynaptic neuron):
-    scale_j = 2^ceil(log2(max_i |w_ji| / 448)), codes = e4m3(w / scale). Power-of-two scales make
-    scale * decode(code) exact in FP32. Returns (codes uint8 [rows, n_in], scale float32 [rows])."""
+    scale_j = 2^ceil(log2(max_i |w_ji| / 448)), codes = e4m3(w / scale) without subnormal codes (the matrix
+    instructions flush them). Power-of-two scales make scale * decode(code) exact in FP32.
+    Returns (codes uint8 [rows, n_in], scale float32 [rows])."""
     w = np.asarray(w, dtype=np.float64)
     m = np.max(np.abs(w), axis=1)
     e = np.ceil(np.log2(np.maximum(m, 1e-30) / 448.0))
     scale = np.ldexp(1.0, e.astype(np.int64)).astype(np.float32)
-    codes = abi.e4m3_encode(w / scale.astype(np.float64)[:, None])
+    codes = abi.e4m3_encode(w / scale.astype(np.float64)[:, None], subnormals=False)
     return codes, scale
 
 
