@@ -307,17 +307,38 @@ def compose(parts, name, per_instance_steps):
     return spec
 
 
+# The matrix instructions form each 16-input dot product with about 13 binades below its largest product
+# (bits further down are dropped; the FP32 accumulation across K-blocks is exact). Weights whose nonzero
+# values stay within DENSE_WINDOW binades of the row maximum therefore multiply exactly; the quantizers
+# below flush the few smaller weights to zero, and the library rejects rows that break the window.
+FP8_WINDOW = 9   # codes in [0.5, 448]: LSB 2^-4 >= 2^(8-13)
+BF16_WINDOW = 5  # values in [2^-5, 1]: LSB 2^-12 >= 2^(0-13)
+
+
 def fp8_rows(w):
     """Per-row FP8 e4m3 quantization with power-of-two row scales (row = postsynaptic neuron):
-    scale_j = 2^ceil(log2(max_i |w_ji| / 448)), codes = e4m3(w / scale) without subnormal codes (the matrix
-    instructions flush them). Power-of-two scales make scale * decode(code) exact in FP32.
+    scale_j = 2^ceil(log2(max_i |w_ji| / 448)), codes = e4m3(w / scale) restricted to magnitudes >= 0.5
+    (nine binades below 448). Power-of-two scales make scale * decode(code) exact in FP32.
     Returns (codes uint8 [rows, n_in], scale float32 [rows])."""
     w = np.asarray(w, dtype=np.float64)
     m = np.max(np.abs(w), axis=1)
     e = np.ceil(np.log2(np.maximum(m, 1e-30) / 448.0))
     scale = np.ldexp(1.0, e.astype(np.int64)).astype(np.float32)
-    codes = abi.e4m3_encode(w / scale.astype(np.float64)[:, None], subnormals=False)
+    codes = abi.e4m3_encode(w / scale.astype(np.float64)[:, None], min_magnitude=0.5)
     return codes, scale
+
+
+def bf16_rows(w):
+    """Per-row BF16 quantization with power-of-two row scales: values w / scale in (0, 1], restricted to
+    magnitudes >= 2^-5 (smaller ones round to 0 or 2^-5). Returns (bf16 bits uint16, scale float32)."""
+    w = np.asarray(w, dtype=np.float64)
+    m = np.max(np.abs(w), axis=1)
+    e = np.ceil(np.log2(np.maximum(m, 1e-30)))
+    scale = np.ldexp(1.0, e.astype(np.int64))
+    v = w / scale[:, None]
+    lo = 2.0 ** -5
+    v = np.where(np.abs(v) < lo, np.where(np.abs(v) >= lo / 2, np.sign(v) * lo, 0.0), v)
+    return abi.bf16_encode(v.astype(np.float32)), scale.astype(np.float32)
 
 
 def uneven_inference(seed=4, n_instances=512, n_min=200, n_max=5000, n_in=784, p_in=0.02, w_ff=0.05,
@@ -359,7 +380,7 @@ def uneven_inference(seed=4, n_instances=512, n_min=200, n_max=5000, n_in=784, p
     if fmt == abi.DENSE_FP8_E4M3:
         codes, scale = fp8_rows(wff)
     else:
-        codes, scale = abi.bf16_encode(wff), np.ones(N, np.float32)
+        codes, scale = bf16_rows(wff)
     spec.ops.append(("add_dense_input_projection",
                      (pop, begin.astype(np.uint32), sizes.astype(np.uint32), n_in, fmt, codes, scale,
                       int(bernoulli_threshold(p_in)), seed * 7919 + 17),
