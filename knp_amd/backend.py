@@ -81,9 +81,11 @@ def _declare(lib):
     return lib
 
 
-def load_library(path=LIB_PATH):
+def load_library(path=None):
     """Load the built library (raises if it has not been built)."""
     global _LIB
+    if path is None:
+        path = LIB_PATH
     if _LIB is None:
         if not os.path.exists(path):
             raise BackendError(f"{path} is missing: build it first (python -c 'import __graft_entry__ as g; g.build()')")

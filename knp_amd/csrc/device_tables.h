@@ -178,6 +178,8 @@ struct NetDev {
     uint32_t *const *seg_list; // [n_seg] first element of every segment
     const uint32_t *vseg;      // [n_vseg] segment of every virtual segment
     const uint32_t *vrs;       // [n_vseg] row source of every virtual segment
+    const uint32_t *seg_vpos;  // [n_seg][4] virtual positions of every segment (UINT32_MAX = none)
+    uint32_t *vcount;          // [2][n_vseg] spikes per virtual segment, indexed by step parity
     unsigned long long *part;  // [2 * scatter grid] per-workgroup impact and row counters
     uint32_t blk_proj[64];     // projection index of each projection with BLOCKING synapses
     uint32_t n_dense;

@@ -101,6 +101,7 @@ struct knp_net {
     uint64_t allocated = 0, cap = 0;
     uint32_t scatter_grid = 1, train_grid = 1;
     std::vector<uint32_t *> seg_list_host;
+    std::vector<uint32_t> push_vpos;  // virtual position of each projection's push segment
     std::vector<uint32_t> staging;
     bool timing = false;
     std::vector<hipEvent_t> ev_pool;
@@ -786,6 +787,20 @@ int init_network(knp_net *net) {
         T.vseg = vs;
         T.vrs = vr;
         if ((rc = dev_alloc(net, (void **)&T.seg_count, (size_t)2 * T.n_seg * 4))) return rc;
+        if ((rc = dev_alloc(net, (void **)&T.vcount, (size_t)2 * std::max(1u, T.n_vseg) * 4))) return rc;
+        std::vector<uint32_t> vpos((size_t)4 * T.n_seg, 0xFFFFFFFFu);
+        for (uint32_t v = 0; v < T.n_vseg; ++v) {
+            uint32_t *slot = &vpos[(size_t)4 * vseg[v]];
+            int k = 0;
+            while (k < 4 && slot[k] != 0xFFFFFFFFu) ++k;
+            if (k == 4) return fail(KNP_ERR_UNSUPPORTED, "a spike source feeds more than 4 projections");
+            slot[k] = v;
+        }
+        uint32_t *vp;
+        if ((rc = upload(net, &vp, vpos.data(), vpos.size()))) return rc;
+        T.seg_vpos = vp;
+        net->push_vpos.assign(n_projs, 0xFFFFFFFFu);
+        for (uint32_t q = 0; q < n_projs; ++q) net->push_vpos[q] = vpos[(size_t)4 * T.projs[q].push_seg];
         uint64_t max_rows = 0;
         for (uint32_t v = 0; v < T.n_vseg; ++v) max_rows += kBlock;  // segments hold at most 256 rows, pushes grow
         net->scatter_grid = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(2048, max_rows));
@@ -937,6 +952,8 @@ int deliver_pushes(knp_net *net) {
         const uint32_t cnt = (uint32_t)L.size();
         uint32_t *slot = net->table.seg_count + (size_t)(t & 1) * net->table.n_seg + Q.push_seg;
         HIP_TRY(hipMemcpyAsync(slot, &cnt, 4, hipMemcpyHostToDevice, net->stream));
+        uint32_t *vslot = net->table.vcount + (size_t)(t & 1) * net->table.n_vseg + net->push_vpos[q];
+        HIP_TRY(hipMemcpyAsync(vslot, &cnt, 4, hipMemcpyHostToDevice, net->stream));
         HIP_TRY(hipStreamSynchronize(net->stream));
     }
     return KNP_OK;

@@ -68,7 +68,8 @@ __device__ __forceinline__ void wave_append(bool pred, uint32_t value, uint32_t 
 
 // Ordered compaction of a workgroup's 256 candidates into its own segment: ascending order, no global
 // atomics, one count store.
-__device__ __forceinline__ void block_segment_write(bool pred, uint32_t value, uint32_t *segment, uint32_t *count) {
+__device__ __forceinline__ void block_segment_write(const NetDev *__restrict__ net, uint64_t t, bool pred,
+                                                    uint32_t value, uint32_t *segment, uint32_t seg) {
     __shared__ uint32_t wcnt[kBlock / 64];
     const uint64_t mask = __ballot(pred);
     const uint32_t lane = lane_id(), w = threadIdx.x / 64;
@@ -80,7 +81,12 @@ __device__ __forceinline__ void block_segment_write(bool pred, uint32_t value, u
         total += wcnt[k];
     }
     if (pred) segment[off + __popcll(mask & ((1ull << lane) - 1ull))] = value;
-    if (threadIdx.x == 0) *count = total;
+    if (threadIdx.x < 4) {
+        const size_t par = (size_t)(t & 1);
+        if (threadIdx.x == 0) net->seg_count[par * net->n_seg + seg] = total;
+        const uint32_t vp = net->seg_vpos[4 * seg + threadIdx.x];
+        if (vp != 0xFFFFFFFFu) net->vcount[par * net->n_vseg + vp] = total;
+    }
     __syncthreads();
 }
 
@@ -236,7 +242,7 @@ __device__ __forceinline__ void population_step(const NetDev *__restrict__ net, 
         }
     }
     // spike compaction and bitmaps (whole wave participates)
-    block_segment_write(spike, i, P.spike_list + (size_t)local_block * kBlock, &seg_count[P.seg_base + local_block]);
+    block_segment_write(net, t, spike, i, P.spike_list + (size_t)local_block * kBlock, P.seg_base + local_block);
     const uint64_t mask = __ballot(spike);
     wave_bitmap(mask, local_block * kBlock + (threadIdx.x & ~63u), P, t);
     if (KIND == POP_RESOURCE) wave_append(cand, i, &ctl->cand_count[p], P.res.cand);
@@ -252,7 +258,7 @@ __device__ __forceinline__ void source_step(const NetDev *__restrict__ net, uint
         const uint32_t pattern = (uint32_t)((t / S.steps_per_pattern) % S.n_patterns);
         spike = bernoulli_hash(S.seed, t, S.hash_offset + c) < S.thr[(size_t)pattern * S.n_ch + c];
     }
-    block_segment_write(spike, S.first + c, S.list + (size_t)local_block * kBlock, &seg_count[S.seg_base + local_block]);
+    block_segment_write(net, t, spike, S.first + c, S.list + (size_t)local_block * kBlock, S.seg_base + local_block);
 }
 
 __global__ __launch_bounds__(kBlock) void k_phase1(const NetDev *__restrict__ net, uint64_t t) {
@@ -263,7 +269,15 @@ __global__ __launch_bounds__(kBlock) void k_phase1(const NetDev *__restrict__ ne
         for (uint32_t k = threadIdx.x; k < sizeof(Ctl) / 4; k += blockDim.x) nxt[k] = 0;
         // next step's counts: host-pushed segments must start empty (every other segment is rewritten)
         uint32_t *nc = net->seg_count + (size_t)((t + 1) & 1) * net->n_seg;
-        for (uint32_t k = threadIdx.x; k < net->n_projs; k += blockDim.x) nc[net->projs[k].push_seg] = 0;
+        uint32_t *nv = net->vcount + (size_t)((t + 1) & 1) * net->n_vseg;
+        for (uint32_t k = threadIdx.x; k < net->n_projs; k += blockDim.x) {
+            const uint32_t ps = net->projs[k].push_seg;
+            nc[ps] = 0;
+            for (uint32_t j = 0; j < 4; ++j) {
+                const uint32_t vp = net->seg_vpos[4 * ps + j];
+                if (vp != 0xFFFFFFFFu) nv[vp] = 0;
+            }
+        }
     }
     const uint32_t b = blockIdx.x;
     uint32_t task = 0;
@@ -290,6 +304,22 @@ __global__ __launch_bounds__(kBlock) void k_phase1(const NetDev *__restrict__ ne
 // BLOCKING keys order impacts as the reference delivers them (the "last" impact wins, blifat_impl.h:86-88):
 // projection load order, then creation step (later = shorter delay), then message, then presynaptic index,
 // then position in the row.  key = 1 + [blocking projection:6][64-delay:6][message rank:5][pre:29][pos:17].
+// Exact 64-bit addition done as two 32-bit atomics on the two halves of the 8-byte accumulator (64-bit
+// atomics are several times slower here): the low half returns its old value, which gives the carry that
+// the high half receives with the high part of the addend. The result equals the 64-bit sum mod 2^64.
+__device__ __forceinline__ void atomic_add_i64(int64_t *p, long long v) {
+    uint32_t *w = reinterpret_cast<uint32_t *>(p);
+    const unsigned long long u = (unsigned long long)v;
+    const uint32_t lo = (uint32_t)u;
+    const uint32_t hi = (uint32_t)(u >> 32);
+    uint32_t carry = 0;
+    if (lo) {
+        const uint32_t old = atomicAdd(&w[0], lo);
+        carry = (uint32_t)(old + lo) < old ? 1u : 0u;
+    }
+    if (hi + carry) atomicAdd(&w[1], hi + carry);
+}
+
 __device__ __forceinline__ void impact(const PopDev &P, const ProjDev &Q, const RowSource &R, uint32_t slot0,
                                        uint32_t pre, uint64_t pos, uint32_t pt, uint32_t d, float w) {
     const uint32_t post = pt & 0x1FFFFFFFu;
@@ -300,17 +330,17 @@ __device__ __forceinline__ void impact(const PopDev &P, const ProjDev &Q, const 
     const long long v = __float2ll_rn(w * P.scale);
     switch (type) {
         case 0:  // EXCITATORY
-            atomicAdd((unsigned long long *)&P.acc_v[ai], (unsigned long long)v);
+            atomic_add_i64(&P.acc_v[ai], v);
             if (Q.forcing && P.acc_f) atomicOr(&P.acc_f[ai], 1u);
             break;
         case 1:  // INHIBITORY_CURRENT
-            atomicAdd((unsigned long long *)&P.acc_v[ai], (unsigned long long)(-v));
+            atomic_add_i64(&P.acc_v[ai], -v);
             break;
         case 2:  // INHIBITORY_CONDUCTANCE
-            atomicAdd((unsigned long long *)&P.acc_g[ai], (unsigned long long)v);
+            atomic_add_i64(&P.acc_g[ai], v);
             break;
         case 3:  // DOPAMINE
-            atomicAdd((unsigned long long *)&P.acc_d[ai], (unsigned long long)v);
+            atomic_add_i64(&P.acc_d[ai], v);
             break;
         default: {  // BLOCKING
             const uint64_t key = 1ull + (((uint64_t)Q.blocking_id << 57) | ((uint64_t)(64u - d) << 51) |
@@ -325,9 +355,9 @@ __global__ __launch_bounds__(kBlock) void k_scatter(const NetDev *__restrict__ n
     extern __shared__ uint32_t pfx[];  // [n_vseg + 1]
     __shared__ uint32_t wsum[kBlock / 64];
     const uint32_t nv = net->n_vseg;
-    const uint32_t *cnt = net->seg_count + (size_t)(t & 1) * net->n_seg;
+    const uint32_t *vc = net->vcount + (size_t)(t & 1) * nv;
     const uint32_t lane = lane_id(), w = threadIdx.x / 64;
-    for (uint32_t v = threadIdx.x; v < nv; v += kBlock) pfx[v] = cnt[net->vseg[v]];
+    for (uint32_t v = threadIdx.x; v < nv; v += kBlock) pfx[v] = vc[v];
     __syncthreads();
     uint32_t carry = 0;
     for (uint32_t base = 0; base < nv; base += kBlock) {
