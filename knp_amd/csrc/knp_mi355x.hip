@@ -1295,6 +1295,14 @@ int knp_mi355x_push_spikes(knp_net *net, uint32_t proj, const uint32_t *idx, uin
     HostProj &H = net->projs[proj];
     for (uint32_t i = 0; i < n; ++i)
         if (idx[i] >= H.n_pre) return fail(KNP_ERR_INVALID_ARGUMENT, "spike index out of the presynaptic range");
+    // The exact accumulators keep a factor 4 of headroom over one impact per synapse and step: an index may
+    // repeat at most 4 times over the messages delivered in one step.
+    std::vector<uint32_t> all(idx, idx + n);
+    for (const auto &m : H.pending) all.insert(all.end(), m.begin(), m.end());
+    std::sort(all.begin(), all.end());
+    for (size_t i = 4; i < all.size(); ++i)
+        if (all[i] == all[i - 4])
+            return fail(KNP_ERR_RANGE, "a spike index repeats more than 4 times in the messages of one step");
     H.pending.emplace_back(idx, idx + n);
     return KNP_OK;
 }

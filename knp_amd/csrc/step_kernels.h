@@ -304,20 +304,11 @@ __global__ __launch_bounds__(kBlock) void k_phase1(const NetDev *__restrict__ ne
 // BLOCKING keys order impacts as the reference delivers them (the "last" impact wins, blifat_impl.h:86-88):
 // projection load order, then creation step (later = shorter delay), then message, then presynaptic index,
 // then position in the row.  key = 1 + [blocking projection:6][64-delay:6][message rank:5][pre:29][pos:17].
-// Exact 64-bit addition done as two 32-bit atomics on the two halves of the 8-byte accumulator (64-bit
-// atomics are several times slower here): the low half returns its old value, which gives the carry that
-// the high half receives with the high part of the addend. The result equals the 64-bit sum mod 2^64.
+// Exact accumulation: a 64-bit integer atomic per impact (non-returning). Splitting it into two 32-bit
+// atomics on the same 8 bytes measured slower: device-wide atomics are executed past the L2, one transaction
+// each, so the count of atomics, not their width, sets the cost.
 __device__ __forceinline__ void atomic_add_i64(int64_t *p, long long v) {
-    uint32_t *w = reinterpret_cast<uint32_t *>(p);
-    const unsigned long long u = (unsigned long long)v;
-    const uint32_t lo = (uint32_t)u;
-    const uint32_t hi = (uint32_t)(u >> 32);
-    uint32_t carry = 0;
-    if (lo) {
-        const uint32_t old = atomicAdd(&w[0], lo);
-        carry = (uint32_t)(old + lo) < old ? 1u : 0u;
-    }
-    if (hi + carry) atomicAdd(&w[1], hi + carry);
+    atomicAdd(reinterpret_cast<unsigned long long *>(p), (unsigned long long)v);
 }
 
 __device__ __forceinline__ void impact(const PopDev &P, const ProjDev &Q, const RowSource &R, uint32_t slot0,
