@@ -47,6 +47,8 @@ def main():
     ap.add_argument("--timing-steps", type=int, default=1000)
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="target CPU-baseline sample length")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--scatter", choices=["auto", "atomic"], default="auto",
+                    help="impact summation: tiled populations where eligible (auto) or atomics everywhere")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -63,7 +65,11 @@ def main():
     n_neurons = spec.pop_sizes[pop]
     log(f"generated {spec.name}: {n_neurons} neurons, {spec.n_synapses} synapses in {time.perf_counter() - t_gen:.1f}s")
 
-    gpu = spec.load_into(MI355XBackend(0))
+    from knp_amd import abi
+
+    gpu = MI355XBackend(0)
+    gpu.set_scatter_mode(abi.SCATTER_AUTO if args.scatter == "auto" else abi.SCATTER_ATOMIC)
+    gpu = spec.load_into(gpu)
     t_up = time.perf_counter()
     gpu._init()
     gpu.synchronize()
@@ -94,9 +100,11 @@ def main():
     tsteps = args.timing_steps
     gpu.enable_timing(True)
     e2, r2 = gpu.counters()
+    p2, c2 = gpu.tile_counters()
     gpu.step(tsteps)
     gpu.synchronize()
     e3, r3 = gpu.counters()
+    p3, c3 = gpu.tile_counters()
     kern = {}
     for name, kid in KERNEL_IDS.items():
         avg_ms, n = gpu.kernel_time(kid)
@@ -105,20 +113,32 @@ def main():
     gpu.enable_timing(False)
     ev_launch = (e3 - e2) / tsteps
     rows_launch = (r3 - r2) / tsteps
-    # scatter: 9 B per impact (post|type 4, weight 4, delay 1) + 20 B per expanded row (row_ptr pair 16, index 4)
-    scatter_bytes = ev_launch * 9.0 + rows_launch * 20.0
+    parts_launch = (p3 - p2) / tsteps
+    cells_launch = (c3 - c2) / tsteps
+    tiled = gpu.tiled(pop)
+    if tiled:
+        # tile-owner scatter: 9 B per impact (post|type 4, weight 4, delay 1), 8 B per row part (two tile
+        # directory entries), 20 B per spiking row (row metadata 16, spike-list entry 4), 16 B per ring cell
+        # updated in HBM (read 8 + write 8)
+        scatter_name, scatter_kernel = "tile_scatter", "k_tile_scatter"
+        scatter_bytes = ev_launch * 9.0 + parts_launch * 8.0 + rows_launch * 20.0 + cells_launch * 16.0
+    else:
+        # atomic scatter: 9 B per impact (post|type 4, weight 4, delay 1) + 20 B per expanded row (row_ptr pair
+        # 16, index 4); the atomics themselves are counted as 8 B each
+        scatter_name, scatter_kernel = "scatter", "k_scatter"
+        scatter_bytes = ev_launch * 17.0 + rows_launch * 20.0
     # population step (default BLIFAT): 20 B per neuron (ring slot read 8 + clear 8, potential write 4) + bitmap
     neuron_bytes = n_neurons * 20.0 + n_neurons / 8.0
-    sc_s = kern["scatter"]["avg_us"] * 1e-6
+    sc_s = kern[scatter_name]["avg_us"] * 1e-6
     ne_s = kern["neuron"]["avg_us"] * 1e-6
     sc_gbs = scatter_bytes / sc_s / 1e9
     ne_gbs = neuron_bytes / ne_s / 1e9
-    kern["scatter"].update(bytes_per_launch=scatter_bytes, achieved_GBs=sc_gbs)
+    kern[scatter_name].update(bytes_per_launch=scatter_bytes, achieved_GBs=sc_gbs)
     kern["neuron"].update(bytes_per_launch=neuron_bytes, achieved_GBs=ne_gbs)
-    dominant = "scatter" if kern["scatter"]["avg_us"] >= kern["neuron"]["avg_us"] else "neuron"
+    dominant = scatter_name if kern[scatter_name]["avg_us"] >= kern["neuron"]["avg_us"] else "neuron"
     dk = kern[dominant]
     roofline = {
-        "kernel": "k_scatter" if dominant == "scatter" else "k_phase1",
+        "kernel": scatter_kernel if dominant == scatter_name else "k_phase1",
         "bound": "hbm",
         "achieved": round(dk["achieved_GBs"], 2),
         "peak": HBM_PEAK_GBS,
@@ -128,8 +148,8 @@ def main():
         "avg_launch_us": round(dk["avg_us"], 3),
         "bytes_per_launch": round(dk["bytes_per_launch"], 1),
     }
-    traffic_file = os.path.join(ROOT, "profiles", "scatter_plain_gpu_scale_traffic.json")
-    if os.path.exists(traffic_file) and dominant == "scatter":
+    traffic_file = os.path.join(ROOT, "profiles", f"{roofline['kernel']}_plain_gpu_scale_traffic.json")
+    if os.path.exists(traffic_file):
         try:
             roofline["traffic"] = json.load(open(traffic_file)).get("hbm_bytes_per_launch")
         except (OSError, ValueError):
@@ -202,6 +222,7 @@ def main():
         },
         "net_steps_per_s": args.steps / dt,
         "events_per_step": events / args.steps,
+        "summation": "tile-owner (LDS) scatter" if tiled else "atomic scatter",
         "roofline": roofline,
         "kernels": {k: {kk: (round(vv, 3) if isinstance(vv, float) else vv) for kk, vv in v.items()}
                     for k, v in kern.items()},

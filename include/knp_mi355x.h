@@ -141,6 +141,14 @@ int knp_mi355x_create(int device, knp_net **out);
 int knp_mi355x_destroy(knp_net *net);
 /* Caps the device memory the handle may allocate (0 = no cap); exceeding it gives KNP_ERR_OUT_OF_MEMORY. */
 int knp_mi355x_set_memory_cap(knp_net *net, uint64_t bytes);
+/* How impacts are summed (no reference counterpart: an execution choice with identical results).
+ * AUTO: populations that only receive plain Delta EXCITATORY / INHIBITORY_CURRENT impacts with delays up to
+ * 32 are tiled (each 256-neuron tile owned by one workgroup, sums in LDS, no global atomics); the rest use
+ * per-impact 64-bit atomics. ATOMIC: atomics everywhere. Set before the first step. */
+enum knp_scatter_mode { KNP_SCATTER_AUTO = 0, KNP_SCATTER_ATOMIC = 1 };
+int knp_mi355x_set_scatter_mode(knp_net *net, int mode);
+/* 1 if the population's impacts are summed by tiles (valid after the first step or an explicit init). */
+int knp_mi355x_get_tiled(knp_net *net, uint32_t pop, int *tiled);
 
 /* ---- loading (Backend::load_all_populations / load_all_projections, backend.h:137,146;
  *      SingleThreadedCPUBackend::load_populations / load_projections, backend.cpp:144-170) ---------- */
@@ -242,6 +250,9 @@ int knp_mi355x_get_event_count(knp_net *net, uint64_t *events);
 int knp_mi355x_get_counters(knp_net *net, uint64_t *events, uint64_t *rows);
 /* Fixed-point fraction bits of the exact impact accumulator of a population (chosen at init). */
 int knp_mi355x_get_accumulator_bits(knp_net *net, uint32_t pop, int32_t *frac_bits);
+/* Work of the tiled path so far: row parts read (a spiking row's synapses in one tile) and ring cells
+ * updated in HBM (non-zero tile sums). */
+int knp_mi355x_get_tile_counters(knp_net *net, uint64_t *row_parts, uint64_t *ring_cells);
 
 /* ---- kernel timing (HIP events on the library's own stream) -------------------------------------- */
 enum knp_kernel_id {
@@ -249,7 +260,8 @@ enum knp_kernel_id {
     KNP_KERNEL_SCATTER = 1, /* spike -> impact scatter into the delay ring */
     KNP_KERNEL_TRAIN = 2,   /* synaptic-resource STDP */
     KNP_KERNEL_DENSE = 3,   /* dense input projection on MFMA (spike bits + product), per time block */
-    KNP_KERNEL_COUNT = 4
+    KNP_KERNEL_TILE_SCATTER = 4, /* tile-owner scatter (tiled populations) */
+    KNP_KERNEL_COUNT = 5
 };
 int knp_mi355x_enable_timing(knp_net *net, int enable);
 /* Average duration (ms) and launch count of a kernel class since timing was enabled. */

@@ -37,6 +37,9 @@ def _declare(lib):
         "knp_mi355x_create": (i32, [i32, C.POINTER(P)]),
         "knp_mi355x_destroy": (i32, [P]),
         "knp_mi355x_set_memory_cap": (i32, [P, u64]),
+        "knp_mi355x_set_scatter_mode": (i32, [P, i32]),
+        "knp_mi355x_get_tiled": (i32, [P, u32, C.POINTER(i32)]),
+        "knp_mi355x_get_tile_counters": (i32, [P, C.POINTER(u64), C.POINTER(u64)]),
         "knp_mi355x_add_blifat_population": (i32, [P, C.POINTER(abi.BLIFATNeuron), u32, pu32]),
         "knp_mi355x_add_resource_blifat_population": (i32, [P, C.POINTER(abi.ResourceBLIFATNeuron), u32, pu32]),
         "knp_mi355x_add_delta_projection": (i32, [P, u32, u32, u32, pu32, pu32, pf32, pu32, pu8, u64, pu32]),
@@ -265,6 +268,21 @@ class MI355XBackend:
         self._chk(self.lib.knp_mi355x_get_counters(self.net, C.byref(e), C.byref(r)))
         return e.value, r.value
 
+    def set_scatter_mode(self, mode):
+        """abi.SCATTER_AUTO (tiled populations where eligible) or abi.SCATTER_ATOMIC; before the first step."""
+        self._chk(self.lib.knp_mi355x_set_scatter_mode(self.net, mode))
+
+    def tiled(self, pop):
+        t = C.c_int()
+        self._chk(self.lib.knp_mi355x_get_tiled(self.net, pop, C.byref(t)))
+        return bool(t.value)
+
+    def tile_counters(self):
+        """(row parts read, ring cells updated) by the tile-owner scatter so far."""
+        a, b = C.c_uint64(), C.c_uint64()
+        self._chk(self.lib.knp_mi355x_get_tile_counters(self.net, C.byref(a), C.byref(b)))
+        return a.value, b.value
+
     def accumulator_bits(self, pop):
         b = C.c_int32()
         self._chk(self.lib.knp_mi355x_get_accumulator_bits(self.net, pop, C.byref(b)))
@@ -337,4 +355,4 @@ class MI355XBackend:
 
 
 KERNEL_IDS = {"neuron": abi.KERNEL_NEURON, "scatter": abi.KERNEL_SCATTER, "train": abi.KERNEL_TRAIN,
-              "dense": abi.KERNEL_DENSE}
+              "dense": abi.KERNEL_DENSE, "tile_scatter": abi.KERNEL_TILE_SCATTER}

@@ -18,6 +18,10 @@ constexpr int kMaxTasks = kMaxPops + kMaxSources;
 constexpr int kMaxWorking = 8;
 constexpr int kMaxDense = 4;
 constexpr int kBlock = 256;  // 4 wave64 per workgroup
+constexpr int kTileThreads = 512;   // workgroup of the tile-owner scatter (8 wave64)
+constexpr int kTileMaxD = 32;       // ring depth limit of tile-owned populations (LDS: D x 256 x 8 bytes)
+constexpr int kTileMaxList = 2048;  // virtual segments that may reach one tile
+constexpr uint32_t kTileShortPart = 16;  // row parts up to this length are expanded by the row's own thread
 
 enum PopKind : uint32_t { POP_LEAN = 0, POP_FULL = 1, POP_RESOURCE = 2 };
 enum ProjKind : uint32_t { PROJ_DELTA = 0, PROJ_RESOURCE = 1, PROJ_ADDITIVE = 2 };
@@ -77,6 +81,7 @@ struct PopDev {
     uint32_t *bitmap;      // spikes of the current step, bit i = neuron i
     uint32_t *rec;         // recorded bitmaps [rec_cap][words]
     uint64_t rec_t0, rec_cap;
+    uint32_t tiled;        // impacts into this population are added by the tile-owner scatter
     ResDev res;
 };
 
@@ -88,6 +93,13 @@ struct PopDev {
 struct RowSource {
     uint32_t proj;
     uint32_t rank;      // 0 = channel messages, 1 + population index for population messages
+};
+
+struct alignas(16) RowMeta {
+    uint32_t begin;      // first CSR position of the row
+    uint32_t first_tile; // tile of the row's first postsynaptic neuron
+    uint32_t n_tiles;    // tiles spanned (first .. last postsynaptic neuron)
+    uint32_t dir;        // offset of the row's directory in row_dir
 };
 
 struct ProjDev {
@@ -122,6 +134,15 @@ struct ProjDev {
     uint32_t n_stdp;
     uint32_t stdp_pop[4];
     uint32_t stdp_mode[4];
+    // tile-owner scatter (plain Delta into a tiled population): rows are sorted by postsynaptic neuron, so
+    // a tile's part of a row is contiguous; the owner of a tile finds it through a directory. Dense directory:
+    // [n_pre][tile_stride] CSR positions. Ranged directory (rows spanning few of many tiles): row_meta +
+    // row_dir. Neither: rows are short and every tile reads them whole and filters.
+    const uint64_t *syn8;        // [n_syn] CSR order: weight bits | delay << 32 | inhibitory << 40 | post << 41
+    const uint32_t *tile_dir;    // dense directory (null if none)
+    uint32_t tile_stride;        // tiles of the postsynaptic population + 1
+    const struct RowMeta *row_meta;  // [n_pre] ranged directory (null if none)
+    const uint32_t *row_dir;     // per row: tiles spanned + 1 offsets from the row start
 };
 
 struct SourceDev {
@@ -161,6 +182,20 @@ struct DenseDev {
     float *ff;             // [2T][population size] impacts by arrival step
 };
 
+// One 256-neuron tile of a tiled population: the workgroup that owns it reads every spiking row that can
+// reach the tile (the virtual segments tile_v[vbeg..vend)), accumulates the tile's impacts in LDS and adds
+// them to the ring without global atomics.
+struct TileDev {
+    uint32_t pop, tile, vbeg, vend;
+};
+
+// A spike segment that may reach a tile: its list, the index of its count in vcount, its projection.
+struct TileSeg {
+    const uint32_t *list;
+    uint32_t vpos;
+    uint32_t proj;
+};
+
 struct Ctl {
     uint32_t cand_count[kMaxPops];
 };
@@ -175,12 +210,21 @@ struct NetDev {
     Ctl *ctl;                  // [2], indexed by step parity
     uint32_t n_seg, n_vseg;
     uint32_t *seg_count;       // [2][n_seg] spikes per segment, indexed by step parity
-    uint32_t *const *seg_list; // [n_seg] first element of every segment
+    uint32_t *const *seg_list; // [n_seg] first element of every segment (step parity 0)
+    uint32_t list_pstride;     // offset of the parity-1 copy of every list
     const uint32_t *vseg;      // [n_vseg] segment of every virtual segment
     const uint32_t *vrs;       // [n_vseg] row source of every virtual segment
     const uint32_t *seg_vpos;  // [n_seg][4] virtual positions of every segment (UINT32_MAX = none)
     uint32_t *vcount;          // [2][n_vseg] spikes per virtual segment, indexed by step parity
     unsigned long long *part;  // [2 * scatter grid] per-workgroup impact and row counters
+    uint32_t n_vseg_atomic;    // virtual segments [0, n_vseg_atomic) go through the atomic scatter
+    uint32_t n_tiles;          // tiles of tiled populations (one tile-scatter workgroup each)
+    const TileDev *tiles;      // [n_tiles]
+    const TileSeg *tile_seg;   // segments reaching each tile (ranges in TileDev)
+    uint32_t tile_max_list;    // longest such range
+    uint32_t tile_max_d;       // deepest ring of a tiled population
+    uint32_t n_src_blocks;     // workgroups of the Bernoulli sources (256 channels each)
+    unsigned long long *tile_part;  // [4 * n_tiles] impacts, rows, row segments, ring cells updated
     uint32_t blk_proj[64];     // projection index of each projection with BLOCKING synapses
     uint32_t n_dense;
     DenseDev dense[kMaxDense];

@@ -47,6 +47,7 @@ struct HostPop {
     uint32_t D = 1;
     bool has_cond = false, has_dop = false, has_block = false, has_forcing = false;
     int32_t frac_bits = 48;
+    bool tiled = false;  // impacts summed by the tile-owner scatter
     uint64_t t_first = 0;
     bool ran = false;
     uint64_t rec_cap = 0, rec_t0 = 0;
@@ -100,6 +101,9 @@ struct knp_net {
     std::vector<void *> allocations;
     uint64_t allocated = 0, cap = 0;
     uint32_t scatter_grid = 1, train_grid = 1;
+    int scatter_mode = KNP_SCATTER_AUTO;
+    bool fused = false;     // every population tiled: phase 1 of step t + 1 runs inside the tile kernel of t
+    uint32_t tile_lds = 0;  // dynamic LDS bytes of the tile scatter (largest tiled ring)
     std::vector<uint32_t *> seg_list_host;
     std::vector<uint32_t> push_vpos;  // virtual position of each projection's push segment
     std::vector<uint32_t> staging;
@@ -449,6 +453,20 @@ int init_network(knp_net *net) {
             }
     }
 
+    // ---- spike lists: one pool holding every segment (population, source and channel lists) twice, by step
+    // parity, so a step's lists can be written while the previous step's are still being read
+    uint32_t *pool_cursor = nullptr;
+    {
+        uint64_t half = 0;
+        for (uint32_t p = 0; p < n_pops; ++p) half += (uint64_t)std::max(1u, (net->pops[p].n + kBlock - 1) / kBlock) * kBlock;
+        for (const auto &Sh : net->srcs) half += (uint64_t)std::max(1u, (Sh.n_ch + kBlock - 1) / kBlock) * kBlock;
+        for (const auto &Qh : net->projs) half += 4ull * std::max(Qh.n_pre, 1u);
+        if (half >= (uint64_t(1) << 31)) return fail(KNP_ERR_UNSUPPORTED, "spike lists above 2^31 entries");
+        int rc;
+        if ((rc = dev_alloc(net, (void **)&pool_cursor, (size_t)2 * half * 4))) return rc;
+        T.list_pstride = (uint32_t)half;
+    }
+
     // ---- populations
     for (uint32_t p = 0; p < n_pops; ++p) {
         HostPop &H = net->pops[p];
@@ -558,7 +576,8 @@ int init_network(knp_net *net) {
             if ((rc = make_param_u(net, n, [&](uint32_t i) { return nr[i].isi_max; }, &R.isi_max))) return rc;
         }
         const uint32_t chunks = std::max(1u, (n + kBlock - 1) / kBlock);
-        if ((rc = dev_alloc(net, (void **)&P.spike_list, (size_t)chunks * kBlock * 4))) return rc;
+        P.spike_list = pool_cursor;
+        pool_cursor += (size_t)chunks * kBlock;
         if ((rc = dev_alloc(net, (void **)&P.bitmap, (size_t)std::max(P.words, 1u) * 4))) return rc;
         if (H.rec_cap) {
             if ((rc = dev_alloc(net, (void **)&P.rec, (size_t)H.rec_cap * P.words * 4))) return rc;
@@ -568,6 +587,60 @@ int init_network(knp_net *net) {
         }
         P.t_first = net->step;
         H.t_first = net->step;
+    }
+
+    // ---- tile-owner scatter: which populations are tiled, which projections get tile directories
+    // A population is tiled when every impact it receives is a plain Delta EXCITATORY / INHIBITORY_CURRENT
+    // impact (one int64 ring), its ring fits in LDS and its indexes fit the packed synapse record (23 bits).
+    // Rows longer than 4 synapses on average get a tile directory: dense ([n_pre][tiles + 1]) unless rows span
+    // few of many tiles (ranged, per row); shorter rows are read whole by every tile they may reach and
+    // filtered.
+    enum { DIR_NONE = 0, DIR_RANGED = 1, DIR_DENSE = 2 };
+    std::vector<uint8_t> proj_dir(n_projs, DIR_NONE);
+    std::vector<std::vector<uint32_t>> row_lo(n_projs), row_hi(n_projs);  // postsynaptic range of every row
+    uint32_t tile_max_d = kTileMaxD;
+    {
+        // the LDS ring (D x 256 x 8 bytes), the segment table and the static LDS must fit one workgroup
+        int lds_limit = 0;
+        HIP_TRY(hipDeviceGetAttribute(&lds_limit, hipDeviceAttributeMaxSharedMemoryPerBlock, net->device));
+        hipFuncAttributes fa{};
+        HIP_TRY(hipFuncGetAttributes(&fa, reinterpret_cast<const void *>(&k_tile_scatter<true>)));
+        const int64_t room = (int64_t)lds_limit - (int64_t)fa.sharedSizeBytes - (int64_t)tile_lds_bytes(0, kTileMaxList);
+        tile_max_d = (uint32_t)std::max<int64_t>(0, std::min<int64_t>(kTileMaxD, room / (kBlock * 8)));
+    }
+    for (uint32_t p = 0; p < n_pops; ++p) {
+        HostPop &H = net->pops[p];
+        H.tiled = net->scatter_mode == KNP_SCATTER_AUTO && !H.resource && !H.has_cond && !H.has_dop &&
+                  !H.has_block && H.D <= tile_max_d && H.n > 0 && H.n < (1u << 23);
+    }
+    for (uint32_t q = 0; q < n_projs; ++q)
+        if (net->projs[q].kind != PROJ_DELTA) net->pops[net->projs[q].post_pop].tiled = false;
+    {
+        std::vector<uint64_t> dir_entries(n_pops, 0);
+        for (uint32_t q = 0; q < n_projs; ++q) {
+            const HostProj &H = net->projs[q];
+            if (!net->pops[H.post_pop].tiled) continue;
+            auto &lo = row_lo[q], &hi = row_hi[q];
+            lo.assign(H.n_pre, 0xFFFFFFFFu);
+            hi.assign(H.n_pre, 0);
+            for (uint64_t k = 0; k < H.n_syn; ++k) {
+                const uint32_t r = H.pre[k], j = H.post[k];
+                lo[r] = std::min(lo[r], j);
+                hi[r] = std::max(hi[r], j);
+            }
+            if (H.n_syn <= 4ull * H.n_pre) continue;
+            uint64_t ranged = 0;
+            for (uint32_t r = 0; r < H.n_pre; ++r)
+                ranged += 4 + (lo[r] <= hi[r] ? hi[r] / kBlock - lo[r] / kBlock + 2 : 0);  // + 16-byte row meta
+            const uint64_t tiles = (net->pops[H.post_pop].n + kBlock - 1) / kBlock;
+            const uint64_t dense = (uint64_t)H.n_pre * (tiles + 1);
+            proj_dir[q] = (dense <= 2 * ranged) ? DIR_DENSE : DIR_RANGED;
+            dir_entries[H.post_pop] += std::min(dense, ranged);
+        }
+        for (uint32_t p = 0; p < n_pops; ++p)
+            if (dir_entries[p] > (uint64_t(1) << 28)) net->pops[p].tiled = false;  // 1 GiB of directories
+        for (uint32_t q = 0; q < n_projs; ++q)
+            if (!net->pops[net->projs[q].post_pop].tiled) proj_dir[q] = DIR_NONE;
     }
 
     // ---- projections
@@ -590,7 +663,77 @@ int init_network(knp_net *net) {
         std::vector<uint32_t> csr_syn(S);
         {
             std::vector<uint64_t> fill(row_ptr.begin(), row_ptr.end() - 1);
-            for (uint64_t s = 0; s < S; ++s) csr_syn[fill[H.pre[s]]++] = (uint32_t)s;
+            if (net->pops[H.post_pop].tiled) {
+                // rows sorted by postsynaptic neuron, then synapse index (a tile's part of a row is contiguous):
+                // a stable counting sort by postsynaptic neuron, then the stable fill by presynaptic neuron
+                std::vector<uint64_t> col((size_t)n_post + 1, 0);
+                for (uint64_t s = 0; s < S; ++s) col[H.post[s] + 1]++;
+                for (uint32_t j = 0; j < n_post; ++j) col[j + 1] += col[j];
+                std::vector<uint32_t> by_post(S);
+                for (uint64_t s = 0; s < S; ++s) by_post[col[H.post[s]]++] = (uint32_t)s;
+                for (uint64_t k = 0; k < S; ++k) csr_syn[fill[H.pre[by_post[k]]]++] = by_post[k];
+            } else {
+                for (uint64_t s = 0; s < S; ++s) csr_syn[fill[H.pre[s]]++] = (uint32_t)s;
+            }
+        }
+        if (net->pops[H.post_pop].tiled) {
+            // packed records for the tile scatter: weight bits | delay << 32 | inhibitory << 40 | post << 41
+            std::vector<uint64_t> rec(S);
+            for (uint64_t k = 0; k < S; ++k) {
+                const uint32_t sy = csr_syn[k];
+                uint32_t wb;
+                std::memcpy(&wb, &H.w[sy], 4);
+                rec[k] = (uint64_t)wb | ((uint64_t)(H.delay[sy] & 0xFFu) << 32) |
+                         ((uint64_t)(H.type[sy] == KNP_INHIBITORY_CURRENT ? 1 : 0) << 40) | ((uint64_t)H.post[sy] << 41);
+            }
+            int rc2;
+            uint64_t *rd;
+            if ((rc2 = upload(net, &rd, rec.data(), S))) return rc2;
+            Q.syn8 = rd;
+        }
+        if (proj_dir[q] == DIR_DENSE) {
+            const uint32_t tiles = (n_post + kBlock - 1) / kBlock;
+            Q.tile_stride = tiles + 1;
+            std::vector<uint32_t> dir((size_t)H.n_pre * (tiles + 1));
+            for (uint32_t i = 0; i < H.n_pre; ++i) {
+                uint64_t k = row_ptr[i];
+                const uint64_t e = row_ptr[i + 1];
+                uint32_t *d = dir.data() + (size_t)i * (tiles + 1);
+                for (uint32_t x = 0; x <= tiles; ++x) {
+                    while (k < e && H.post[csr_syn[k]] / kBlock < x) ++k;
+                    d[x] = (uint32_t)k;
+                }
+            }
+            int rc2;
+            uint32_t *dd;
+            if ((rc2 = upload(net, &dd, dir.data(), dir.size()))) return rc2;
+            Q.tile_dir = dd;
+        } else if (proj_dir[q] == DIR_RANGED) {
+            std::vector<RowMeta> meta(H.n_pre);
+            std::vector<uint32_t> dir;
+            for (uint32_t i = 0; i < H.n_pre; ++i) {
+                const uint64_t b = row_ptr[i], e = row_ptr[i + 1];
+                RowMeta &m = meta[i];
+                m.begin = (uint32_t)b;
+                m.dir = (uint32_t)dir.size();
+                m.first_tile = 0;
+                m.n_tiles = 0;
+                if (b == e) continue;
+                m.first_tile = H.post[csr_syn[b]] / kBlock;
+                m.n_tiles = H.post[csr_syn[e - 1]] / kBlock - m.first_tile + 1;
+                uint64_t k = b;
+                for (uint32_t j = 0; j <= m.n_tiles; ++j) {
+                    while (k < e && H.post[csr_syn[k]] / kBlock < m.first_tile + j) ++k;
+                    dir.push_back((uint32_t)(k - b));
+                }
+            }
+            int rc2;
+            RowMeta *md;
+            if ((rc2 = upload(net, &md, meta.data(), meta.size()))) return rc2;
+            Q.row_meta = md;
+            uint32_t *dd;
+            if ((rc2 = upload(net, &dd, dir.data(), dir.size()))) return rc2;
+            Q.row_dir = dd;
         }
         if (H.has_blocking)
             for (uint32_t i = 0; i < H.n_pre; ++i)
@@ -692,8 +835,11 @@ int init_network(knp_net *net) {
                 Q.stdp_mode[m] = H.stdp_modes[m];
             }
         }
-        Q.in_cap = std::max(H.n_pre, 1u);
-        if ((rc = dev_alloc(net, (void **)&Q.in_list, (size_t)Q.in_cap * 4))) return rc;
+        // channel messages of one step: every index at most 4 times (checked when pushed), so this capacity
+        // is never exceeded and the list never moves (tile tables hold its address)
+        Q.in_cap = 4 * std::max(H.n_pre, 1u);
+        Q.in_list = pool_cursor;
+        pool_cursor += Q.in_cap;
     }
 
     // ---- sources
@@ -714,7 +860,8 @@ int init_network(knp_net *net) {
         if ((rc = upload(net, &th, H.thr.data(), H.thr.size()))) return rc;
         S.thr = th;
         const uint32_t chunks = std::max(1u, (H.n_ch + kBlock - 1) / kBlock);
-        if ((rc = dev_alloc(net, (void **)&S.list, (size_t)chunks * kBlock * 4))) return rc;
+        S.list = pool_cursor;
+        pool_cursor += (size_t)chunks * kBlock;
     }
 
     // ---- spike segments and the scatter's virtual segment list
@@ -728,7 +875,13 @@ int init_network(knp_net *net) {
             P.seg_base = (uint32_t)seg_list.size();
             for (uint32_t c = 0; c < pop_chunks[p]; ++c) seg_list.push_back(P.spike_list + (size_t)c * kBlock);
         }
-        std::vector<uint32_t> vseg, vrs;
+        // every (projection, segment) pair whose spikes the projection expands, with the presynaptic rows the
+        // segment can hold: for every projection (load order) its channel segments, then the segments of every
+        // population whose spike messages it receives
+        struct VDesc {
+            uint32_t seg, rs, q, row_lo, row_hi;
+        };
+        std::vector<VDesc> vd;
         uint32_t nrs = 0;
         for (uint32_t q = 0; q < n_projs; ++q) {
             const HostProj &H = net->projs[q];
@@ -738,16 +891,15 @@ int init_network(knp_net *net) {
             seg_list.push_back(Q.in_list);
             if (nrs >= (uint32_t)kMaxRowSources) return fail(KNP_ERR_UNSUPPORTED, "too many row sources");
             T.rows[nrs] = RowSource{q, 0u};
-            vseg.push_back(Q.push_seg);
-            vrs.push_back(nrs);
+            vd.push_back(VDesc{Q.push_seg, nrs, q, 0u, H.n_pre});
             for (uint32_t si = 0; si < T.n_srcs; ++si) {
                 SourceDev &S = T.srcs[si];
                 if (S.proj != q) continue;
                 const uint32_t chunks = std::max(1u, (S.n_ch + kBlock - 1) / kBlock);
                 S.seg_base = (uint32_t)seg_list.size();
                 for (uint32_t c = 0; c < chunks; ++c) {
-                    vseg.push_back((uint32_t)seg_list.size());
-                    vrs.push_back(nrs);
+                    const uint32_t r0 = S.first + c * kBlock, r1 = S.first + std::min((c + 1) * kBlock, S.n_ch);
+                    vd.push_back(VDesc{(uint32_t)seg_list.size(), nrs, q, r0, r1});
                     seg_list.push_back(S.list + (size_t)c * kBlock);
                 }
             }
@@ -766,14 +918,85 @@ int init_network(knp_net *net) {
                 if (!sub || mode == KNP_STDP_ONLY) continue;
                 if (nrs >= (uint32_t)kMaxRowSources) return fail(KNP_ERR_UNSUPPORTED, "too many row sources");
                 T.rows[nrs] = RowSource{q, 1u + p};
-                for (uint32_t c = 0; c < pop_chunks[p]; ++c) {
-                    vseg.push_back(T.pops[p].seg_base + c);
-                    vrs.push_back(nrs);
-                }
+                for (uint32_t c = 0; c < pop_chunks[p]; ++c)
+                    vd.push_back(VDesc{T.pops[p].seg_base + c, nrs, q, c * kBlock,
+                                       std::min((c + 1) * kBlock, T.pops[p].n)});
                 ++nrs;
             }
         }
-        if (vseg.size() > (size_t)kMaxVSeg) return fail(KNP_ERR_UNSUPPORTED, "too many spike segments for one step");
+        // tiles of tiled populations and the segments that can reach each of them (through the postsynaptic
+        // range of the rows a segment can hold); a population with a tile reached by more than kTileMaxList
+        // segments stays on the atomic path
+        std::vector<std::pair<uint32_t, uint32_t>> vd_tiles(vd.size(), {1u, 0u});  // tile range, empty = {1,0}
+        for (size_t k = 0; k < vd.size(); ++k) {
+            const VDesc &d = vd[k];
+            const uint32_t post = net->projs[d.q].post_pop;
+            if (!net->pops[post].tiled) continue;
+            const auto &lo = row_lo[d.q], &hi = row_hi[d.q];
+            uint32_t tlo = 0xFFFFFFFFu, thi = 0;
+            for (uint32_t r = d.row_lo; r < d.row_hi && r < (uint32_t)lo.size(); ++r)
+                if (lo[r] <= hi[r]) {
+                    tlo = std::min(tlo, lo[r] / kBlock);
+                    thi = std::max(thi, hi[r] / kBlock);
+                }
+            if (tlo <= thi) vd_tiles[k] = {tlo, thi};
+        }
+        for (uint32_t p = 0; p < n_pops; ++p) {
+            if (!net->pops[p].tiled) continue;
+            std::vector<uint32_t> reach((net->pops[p].n + kBlock - 1) / kBlock, 0);
+            for (size_t k = 0; k < vd.size(); ++k)
+                if (net->projs[vd[k].q].post_pop == p)
+                    for (uint32_t x = vd_tiles[k].first; x <= vd_tiles[k].second && vd_tiles[k].first <= vd_tiles[k].second; ++x)
+                        reach[x]++;
+            for (uint32_t c : reach)
+                if (c > (uint32_t)kTileMaxList) net->pops[p].tiled = false;
+        }
+        // virtual segment order: the atomic scatter's segments first, then the tiled populations'
+        std::vector<uint32_t> vseg, vrs, vidx(vd.size());
+        for (int pass = 0; pass < 2; ++pass)
+            for (size_t k = 0; k < vd.size(); ++k) {
+                const bool tl = net->pops[net->projs[vd[k].q].post_pop].tiled;
+                if (tl != (pass == 1)) continue;
+                vidx[k] = (uint32_t)vseg.size();
+                vseg.push_back(vd[k].seg);
+                vrs.push_back(vd[k].rs);
+            }
+        uint32_t n_atomic = 0;
+        for (size_t k = 0; k < vd.size(); ++k)
+            if (!net->pops[net->projs[vd[k].q].post_pop].tiled) ++n_atomic;
+        if (n_atomic > (uint32_t)kMaxVSeg) return fail(KNP_ERR_UNSUPPORTED, "too many spike segments for one step");
+        T.n_vseg_atomic = n_atomic;
+        std::vector<TileDev> tiles;
+        std::vector<TileSeg> tile_seg;
+        uint32_t maxD = 0, maxL = 0;
+        for (uint32_t p = 0; p < n_pops; ++p) {
+            T.pops[p].tiled = net->pops[p].tiled ? 1u : 0u;
+            if (!net->pops[p].tiled) continue;
+            maxD = std::max(maxD, T.pops[p].D);
+            const uint32_t nt = (net->pops[p].n + kBlock - 1) / kBlock;
+            std::vector<std::vector<uint32_t>> reach(nt);
+            for (size_t k = 0; k < vd.size(); ++k)
+                if (net->projs[vd[k].q].post_pop == p && vd_tiles[k].first <= vd_tiles[k].second)
+                    for (uint32_t x = vd_tiles[k].first; x <= vd_tiles[k].second; ++x) reach[x].push_back((uint32_t)k);
+            for (uint32_t x = 0; x < nt; ++x) {
+                tiles.push_back(TileDev{p, x, (uint32_t)tile_seg.size(), (uint32_t)(tile_seg.size() + reach[x].size())});
+                maxL = std::max(maxL, (uint32_t)reach[x].size());
+                for (uint32_t k : reach[x]) tile_seg.push_back(TileSeg{seg_list[vd[k].seg], vidx[k], vd[k].q});
+            }
+        }
+        T.n_tiles = (uint32_t)tiles.size();
+        T.tile_max_d = maxD;
+        T.tile_max_list = maxL;
+        net->tile_lds = (uint32_t)tile_lds_bytes(maxD, maxL);
+        if (T.n_tiles) {
+            TileDev *td;
+            if ((rc = upload(net, &td, tiles.data(), tiles.size()))) return rc;
+            T.tiles = td;
+            TileSeg *tsd;
+            if ((rc = upload(net, &tsd, tile_seg.data(), tile_seg.size()))) return rc;
+            T.tile_seg = tsd;
+        }
+        if ((rc = dev_alloc(net, (void **)&T.tile_part, (size_t)4 * std::max(1u, T.n_tiles) * 8))) return rc;
         T.n_row_sources = nrs;
         T.n_seg = (uint32_t)seg_list.size();
         T.n_vseg = (uint32_t)vseg.size();
@@ -801,8 +1024,8 @@ int init_network(knp_net *net) {
         T.seg_vpos = vp;
         net->push_vpos.assign(n_projs, 0xFFFFFFFFu);
         for (uint32_t q = 0; q < n_projs; ++q) net->push_vpos[q] = vpos[(size_t)4 * T.projs[q].push_seg];
-        uint64_t max_rows = 0;
-        for (uint32_t v = 0; v < T.n_vseg; ++v) max_rows += kBlock;  // segments hold at most 256 rows, pushes grow
+        // segments hold at most 256 rows (pushes more, handled by the grid-stride loop)
+        const uint64_t max_rows = (uint64_t)n_atomic * kBlock;
         net->scatter_grid = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(2048, max_rows));
         if ((rc = dev_alloc(net, (void **)&T.part, (size_t)2 * net->scatter_grid * 8))) return rc;
     }
@@ -821,6 +1044,10 @@ int init_network(knp_net *net) {
     }
     T.n_tasks = nt;
     T.total_blocks = blocks;
+    T.n_src_blocks = 0;
+    for (uint32_t s = 0; s < T.n_srcs; ++s) T.n_src_blocks += std::max(1u, (net->srcs[s].n_ch + kBlock - 1) / kBlock);
+    net->fused = T.n_tiles > 0;
+    for (const auto &H : net->pops) net->fused = net->fused && H.tiled;
 
     // ---- control blocks
     int rc;
@@ -936,19 +1163,10 @@ int deliver_pushes(knp_net *net) {
         for (auto &m : H.pending) L.insert(L.end(), m.begin(), m.end());
         H.pending.clear();
         ProjDev &Q = net->table.projs[q];
-        if (L.size() > Q.in_cap) {
-            dev_free(net, Q.in_list);
-            Q.in_cap = (uint32_t)L.size();
-            int rc = dev_alloc(net, (void **)&Q.in_list, (size_t)Q.in_cap * 4);
-            if (rc) return rc;
-            net->seg_list_host[Q.push_seg] = Q.in_list;
-            HIP_TRY(hipMemcpyAsync((void *)(net->table.seg_list + Q.push_seg), &Q.in_list, sizeof(uint32_t *),
-                                   hipMemcpyHostToDevice, net->stream));
-            HIP_TRY(hipStreamSynchronize(net->stream));
-            if ((rc = upload_table(net))) return rc;
-        }
+        if (L.size() > Q.in_cap) return fail(KNP_ERR_RANGE, "more channel spikes in one step than the list holds");
         if (!L.empty())
-            HIP_TRY(hipMemcpyAsync(Q.in_list, L.data(), L.size() * 4, hipMemcpyHostToDevice, net->stream));
+            HIP_TRY(hipMemcpyAsync(Q.in_list + (size_t)(t & 1) * net->table.list_pstride, L.data(), L.size() * 4,
+                                   hipMemcpyHostToDevice, net->stream));
         const uint32_t cnt = (uint32_t)L.size();
         uint32_t *slot = net->table.seg_count + (size_t)(t & 1) * net->table.n_seg + Q.push_seg;
         HIP_TRY(hipMemcpyAsync(slot, &cnt, 4, hipMemcpyHostToDevice, net->stream));
@@ -966,20 +1184,29 @@ int run_steps(knp_net *net, uint64_t n_steps) {
     }
     HIP_TRY(hipSetDevice(net->device));
     NetDev *D = net->dev_table;
+    auto launch_dense_due = [&](uint64_t t) -> int {
+        for (uint32_t d = 0; d < net->dense.size(); ++d) {
+            const uint64_t t_first = net->pops[net->dense[d].post_pop].t_first;
+            if ((t - t_first) % net->dense[d].T != 0) continue;
+            int rc = timed_launch(net, KNP_KERNEL_DENSE, [&] { launch_dense(net, d, t); });
+            if (rc) return rc;
+        }
+        return KNP_OK;
+    };
+    // phase 1 of the current step already ran inside the previous step's tile kernel
+    bool phase1_done = false;
     for (uint64_t it = 0; it < n_steps; ++it) {
         const uint64_t t = net->step;
         int rc = deliver_pushes(net);
         if (rc) return rc;
-        for (uint32_t d = 0; d < net->dense.size(); ++d) {
-            const uint64_t t_first = net->pops[net->dense[d].post_pop].t_first;
-            if ((t - t_first) % net->dense[d].T != 0) continue;
-            if ((rc = timed_launch(net, KNP_KERNEL_DENSE, [&] { launch_dense(net, d, t); }))) return rc;
+        if (!phase1_done) {
+            if ((rc = launch_dense_due(t))) return rc;
+            const uint32_t blocks = net->table.total_blocks;
+            if ((rc = timed_launch(net, KNP_KERNEL_NEURON, [&] {
+                     k_phase1<<<blocks, kBlock, 0, net->stream>>>(D, t);
+                 })))
+                return rc;
         }
-        const uint32_t blocks = net->table.total_blocks;
-        if ((rc = timed_launch(net, KNP_KERNEL_NEURON, [&] {
-                 k_phase1<<<blocks, kBlock, 0, net->stream>>>(D, t);
-             })))
-            return rc;
         for (uint32_t p = 0; p < net->pops.size(); ++p) {
             if (!net->pops[p].resource) continue;
             const uint32_t g = net->train_grid;
@@ -994,10 +1221,25 @@ int run_steps(knp_net *net, uint64_t n_steps) {
             HIP_TRY(hipGetLastError());
         }
         const uint32_t sg = net->scatter_grid;
-        if ((rc = timed_launch(net, KNP_KERNEL_SCATTER, [&] {
-                 k_scatter<<<sg, kBlock, (net->table.n_vseg + 1) * 4, net->stream>>>(D, t);
+        if (net->table.n_vseg_atomic &&
+            (rc = timed_launch(net, KNP_KERNEL_SCATTER, [&] {
+                 k_scatter<<<sg, kBlock, (net->table.n_vseg_atomic + 1) * 4, net->stream>>>(D, t);
              })))
             return rc;
+        // fused: the tile kernel of step t also runs phase 1 of step t + 1, except on the last step of the call
+        // (the state a caller reads between calls is the state after whole steps)
+        const bool p1 = net->fused && it + 1 < n_steps;
+        if (p1 && (rc = launch_dense_due(t + 1))) return rc;
+        const uint32_t nt = net->table.n_tiles;
+        if (nt && (rc = timed_launch(net, KNP_KERNEL_TILE_SCATTER, [&] {
+                       if (p1)
+                           k_tile_scatter<true><<<nt + net->table.n_src_blocks, kTileThreads, net->tile_lds,
+                                                  net->stream>>>(D, t);
+                       else
+                           k_tile_scatter<false><<<nt, kTileThreads, net->tile_lds, net->stream>>>(D, t);
+                   })))
+            return rc;
+        phase1_done = p1;
         for (uint32_t q = 0; q < net->projs.size(); ++q) {
             if (net->projs[q].kind != PROJ_ADDITIVE) continue;
             const uint32_t g = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(4096, (net->projs[q].n_syn + kBlock - 1) / kBlock));
@@ -1659,6 +1901,46 @@ static int read_counters(knp_net *net, uint64_t *events, uint64_t *rows) {
         *events += part[2 * b];
         *rows += part[2 * b + 1];
     }
+    if (net->table.n_tiles) {
+        std::vector<unsigned long long> tp(4 * (size_t)net->table.n_tiles);
+        if ((rc = download(net, tp.data(), net->table.tile_part, tp.size()))) return rc;
+        for (size_t b = 0; b < net->table.n_tiles; ++b) {
+            *events += tp[4 * b];
+            *rows += tp[4 * b + 1];
+        }
+    }
+    return KNP_OK;
+}
+
+int knp_mi355x_get_tile_counters(knp_net *net, uint64_t *row_parts, uint64_t *ring_cells) {
+    if (!net || !row_parts || !ring_cells) return fail(KNP_ERR_INVALID_ARGUMENT, "null pointer");
+    *row_parts = *ring_cells = 0;
+    if (!net->initialized || !net->table.n_tiles) return KNP_OK;
+    std::vector<unsigned long long> tp(4 * (size_t)net->table.n_tiles);
+    int rc = download(net, tp.data(), net->table.tile_part, tp.size());
+    if (rc) return rc;
+    for (size_t b = 0; b < net->table.n_tiles; ++b) {
+        *row_parts += tp[4 * b + 2];
+        *ring_cells += tp[4 * b + 3];
+    }
+    return KNP_OK;
+}
+
+int knp_mi355x_set_scatter_mode(knp_net *net, int mode) {
+    if (!net) return fail(KNP_ERR_INVALID_ARGUMENT, "null handle");
+    if (mode != KNP_SCATTER_AUTO && mode != KNP_SCATTER_ATOMIC)
+        return fail(KNP_ERR_INVALID_ARGUMENT, "unknown scatter mode");
+    if (net->initialized) return fail(KNP_ERR_STATE, "the scatter mode must be set before the first step");
+    net->scatter_mode = mode;
+    return KNP_OK;
+}
+
+int knp_mi355x_get_tiled(knp_net *net, uint32_t pop, int *tiled) {
+    int rc = check_pop(net, pop);
+    if (rc) return rc;
+    if (!tiled) return fail(KNP_ERR_INVALID_ARGUMENT, "null pointer");
+    if ((rc = ensure_init(net))) return rc;
+    *tiled = net->pops[pop].tiled ? 1 : 0;
     return KNP_OK;
 }
 

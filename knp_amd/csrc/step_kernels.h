@@ -67,16 +67,17 @@ __device__ __forceinline__ void wave_append(bool pred, uint32_t value, uint32_t 
 }
 
 // Ordered compaction of a workgroup's 256 candidates into its own segment: ascending order, no global
-// atomics, one count store.
+// atomics, one count store. NT = threads of the workgroup (threads from 256 on carry no candidate).
+template <uint32_t NT = kBlock>
 __device__ __forceinline__ void block_segment_write(const NetDev *__restrict__ net, uint64_t t, bool pred,
                                                     uint32_t value, uint32_t *segment, uint32_t seg) {
-    __shared__ uint32_t wcnt[kBlock / 64];
+    __shared__ uint32_t wcnt[NT / 64];
     const uint64_t mask = __ballot(pred);
     const uint32_t lane = lane_id(), w = threadIdx.x / 64;
     if (lane == 0) wcnt[w] = (uint32_t)__popcll(mask);
     __syncthreads();
     uint32_t off = 0, total = 0;
-    for (uint32_t k = 0; k < kBlock / 64; ++k) {
+    for (uint32_t k = 0; k < NT / 64; ++k) {
         if (k < w) off += wcnt[k];
         total += wcnt[k];
     }
@@ -114,19 +115,21 @@ __device__ __forceinline__ int64_t blocking_value(const NetDev *net, uint64_t ke
 }
 
 // ---- phase 1: populations + sources ----------------------------------------------------------------
-template <uint32_t KIND>
+// NT = threads of the workgroup (threads from 256 on have no neuron). FUSED: the exact sum of the step's
+// impacts is passed in (a_in) by the tile kernel instead of being read from the ring.
+template <uint32_t KIND, uint32_t NT = kBlock, bool FUSED = false>
 __device__ __forceinline__ void population_step(const NetDev *__restrict__ net, uint32_t p, uint32_t local_block,
-                                                uint64_t t, Ctl *ctl, uint32_t *seg_count) {
+                                                uint64_t t, Ctl *ctl, uint32_t *seg_count, int64_t a_in = 0) {
     const PopDev &P = net->pops[p];
     const uint32_t i = local_block * kBlock + threadIdx.x;
-    const bool active = i < P.n;
+    const bool active = threadIdx.x < (uint32_t)kBlock && i < P.n;
     const uint32_t slot = (uint32_t)(t % P.D);
     const size_t ai = (size_t)slot * P.n + i;
     bool spike = false;
     bool cand = false;
     if (active) {
-        int64_t a = 0;
-        if (P.acc_v) {
+        int64_t a = a_in;
+        if (!FUSED && P.acc_v) {
             a = P.acc_v[ai];
             P.acc_v[ai] = 0;
         }
@@ -242,23 +245,26 @@ __device__ __forceinline__ void population_step(const NetDev *__restrict__ net, 
         }
     }
     // spike compaction and bitmaps (whole wave participates)
-    block_segment_write(net, t, spike, i, P.spike_list + (size_t)local_block * kBlock, P.seg_base + local_block);
+    block_segment_write<NT>(net, t, spike, i, P.spike_list + (size_t)(t & 1) * net->list_pstride + (size_t)local_block * kBlock,
+                            P.seg_base + local_block);
     const uint64_t mask = __ballot(spike);
-    wave_bitmap(mask, local_block * kBlock + (threadIdx.x & ~63u), P, t);
+    if (threadIdx.x < (uint32_t)kBlock) wave_bitmap(mask, local_block * kBlock + (threadIdx.x & ~63u), P, t);
     if (KIND == POP_RESOURCE) wave_append(cand, i, &ctl->cand_count[p], P.res.cand);
 }
 
+template <uint32_t NT = kBlock>
 __device__ __forceinline__ void source_step(const NetDev *__restrict__ net, uint32_t s, uint32_t local_block,
                                             uint64_t t, uint32_t *seg_count) {
     const SourceDev &S = net->srcs[s];
     const uint32_t c = local_block * kBlock + threadIdx.x;
     const uint32_t phase = (uint32_t)(t % S.steps_per_pattern);
     bool spike = false;
-    if (c < S.n_ch && phase >= S.lo && phase <= S.hi) {
+    if (threadIdx.x < (uint32_t)kBlock && c < S.n_ch && phase >= S.lo && phase <= S.hi) {
         const uint32_t pattern = (uint32_t)((t / S.steps_per_pattern) % S.n_patterns);
         spike = bernoulli_hash(S.seed, t, S.hash_offset + c) < S.thr[(size_t)pattern * S.n_ch + c];
     }
-    block_segment_write(net, t, spike, S.first + c, S.list + (size_t)local_block * kBlock, S.seg_base + local_block);
+    block_segment_write<NT>(net, t, spike, S.first + c, S.list + (size_t)(t & 1) * net->list_pstride + (size_t)local_block * kBlock,
+                            S.seg_base + local_block);
 }
 
 __global__ __launch_bounds__(kBlock) void k_phase1(const NetDev *__restrict__ net, uint64_t t) {
@@ -345,8 +351,8 @@ __device__ __forceinline__ void impact(const PopDev &P, const ProjDev &Q, const 
 __global__ __launch_bounds__(kBlock) void k_scatter(const NetDev *__restrict__ net, uint64_t t) {
     extern __shared__ uint32_t pfx[];  // [n_vseg + 1]
     __shared__ uint32_t wsum[kBlock / 64];
-    const uint32_t nv = net->n_vseg;
-    const uint32_t *vc = net->vcount + (size_t)(t & 1) * nv;
+    const uint32_t nv = net->n_vseg_atomic;
+    const uint32_t *vc = net->vcount + (size_t)(t & 1) * net->n_vseg;
     const uint32_t lane = lane_id(), w = threadIdx.x / 64;
     for (uint32_t v = threadIdx.x; v < nv; v += kBlock) pfx[v] = vc[v];
     __syncthreads();
@@ -383,7 +389,7 @@ __global__ __launch_bounds__(kBlock) void k_scatter(const NetDev *__restrict__ n
         }
         const uint32_t seg = net->vseg[lo];
         const RowSource &R = net->rows[net->vrs[lo]];
-        const uint32_t pre = net->seg_list[seg][row - pfx[lo]];
+        const uint32_t pre = net->seg_list[seg][(size_t)(t & 1) * net->list_pstride + row - pfx[lo]];
         const ProjDev &Q = net->projs[R.proj];
         if (pre >= Q.n_pre) continue;
         const uint64_t b = Q.row_ptr[pre], e = Q.row_ptr[pre + 1];
@@ -417,6 +423,289 @@ __global__ __launch_bounds__(kBlock) void k_scatter(const NetDev *__restrict__ n
     if (threadIdx.x == 0) {
         net->part[2 * blockIdx.x] += my_ev;
         net->part[2 * blockIdx.x + 1] += my_rows;
+    }
+}
+
+// ---- tile-owner scatter: exact sums per 256-neuron tile in LDS, no global atomics ----------------------
+// Device-wide atomics are executed past the L2 at one memory transaction each, which bounds the atomic
+// scatter at roughly 10^10 impacts/s. A tiled population instead gives each of its 256-neuron tiles to one
+// workgroup: the workgroup reads the part of every spiking row that lands in its tile (rows are sorted by
+// postsynaptic neuron and carry a tile directory), adds the impacts into a [D][256] int64 LDS ring with LDS
+// atomics, and finally adds the non-zero sums into the population's ring in HBM; it is the only writer of
+// those cells, so plain loads and stores suffice. The sums are the same exact integers as the atomic scatter's.
+template <uint32_t NT>
+__device__ __forceinline__ uint32_t block_excl_scan(uint32_t x, uint32_t *wsum, uint32_t *total) {
+    const uint32_t lane = lane_id(), w = threadIdx.x / 64;
+    uint32_t inc = x;
+    for (int off = 1; off < 64; off <<= 1) {
+        const uint32_t y = __shfl_up(inc, off);
+        if ((int)lane >= off) inc += y;
+    }
+    if (lane == 63) wsum[w] = inc;
+    __syncthreads();
+    uint32_t woff = 0, tot = 0;
+    for (uint32_t k = 0; k < NT / 64; ++k) {
+        const uint32_t v = wsum[k];
+        if (k < w) woff += v;
+        tot += v;
+    }
+    __syncthreads();  // wsum may be reused by the next scan
+    *total = tot;
+    return woff + inc - x;
+}
+
+__device__ __forceinline__ uint32_t wave_sum_u32(uint32_t v) {
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+    return v;
+}
+
+// Per-projection pointers the tile scatter needs, staged in LDS.
+struct TileProj {
+    const uint64_t *syn8;
+    const uint32_t *tile_dir;
+    const RowMeta *row_meta;
+    const uint32_t *row_dir;
+    const uint64_t *row_ptr;
+    uint32_t stride, n_pre;
+};
+
+// Dynamic LDS of k_tile_scatter: the ring sums [tile_max_d][256] (int64), then for the tile's segments their
+// spike lists [tile_max_list] (pointers), row-count prefix [tile_max_list + 1] and projection [tile_max_list].
+__host__ __device__ inline size_t tile_lds_bytes(uint32_t max_d, uint32_t max_list) {
+    return (size_t)max_d * kBlock * 8 + (size_t)max_list * 8 + ((size_t)max_list + 1) * 4 + max_list;
+}
+
+// PHASE1: the same launch also runs phase 1 of step t + 1 for the tile's neurons (their ring slot for t + 1
+// is complete once this step's impacts are summed) and, in the workgroups after the tiles, the Bernoulli
+// sources of step t + 1; the host uses it when every population is tiled, saving one launch per step.
+template <bool PHASE1>
+__global__ __launch_bounds__(kTileThreads) void k_tile_scatter(const NetDev *__restrict__ net, uint64_t t) {
+    extern __shared__ unsigned long long tacc[];
+    __shared__ uint32_t rbeg[kTileThreads];  // CSR position of each row's part in the tile
+    __shared__ uint32_t rpfx[kTileThreads];  // exclusive prefix of the parts' lengths
+    __shared__ uint8_t rproj[kTileThreads];  // projection of each row
+    __shared__ uint32_t wsum[kTileThreads / 64];
+    __shared__ unsigned long long cnt[3];
+    __shared__ TileProj tp[kMaxProjs];
+    if (blockIdx.x >= net->n_tiles) {  // Bernoulli sources of step t + 1 (PHASE1 grids only)
+        uint32_t b = blockIdx.x - net->n_tiles;
+        for (uint32_t s = 0; s < net->n_srcs; ++s) {
+            const uint32_t nb = max(1u, (net->srcs[s].n_ch + kBlock - 1) / kBlock);
+            if (b < nb) {
+                source_step<kTileThreads>(net, s, b, t + 1, nullptr);
+                break;
+            }
+            b -= nb;
+        }
+        return;
+    }
+    if (blockIdx.x == 0) {
+        // channel segments of step t + 1 start empty (the host writes the counts of pushed messages later)
+        uint32_t *nc = net->seg_count + (size_t)((t + 1) & 1) * net->n_seg;
+        uint32_t *nvc = net->vcount + (size_t)((t + 1) & 1) * net->n_vseg;
+        for (uint32_t k = threadIdx.x; k < net->n_projs; k += kTileThreads) {
+            const uint32_t ps = net->projs[k].push_seg;
+            nc[ps] = 0;
+            for (uint32_t j = 0; j < 4; ++j) {
+                const uint32_t vp = net->seg_vpos[4 * ps + j];
+                if (vp != 0xFFFFFFFFu) nvc[vp] = 0;
+            }
+        }
+    }
+    const TileDev td = net->tiles[blockIdx.x];
+    const PopDev &P = net->pops[td.pop];
+    // population fields in registers: stores through the ring would otherwise force reloads of the table
+    const uint32_t D = P.D, n_pop = P.n;
+    int64_t *const acc_v = P.acc_v;
+    const float scale = P.scale;
+    const uint32_t tid = threadIdx.x;
+    const uint32_t L = net->tile_max_list;
+    unsigned long long *ring = tacc;
+    const uint32_t **vlist = reinterpret_cast<const uint32_t **>(tacc + (size_t)net->tile_max_d * kBlock);
+    uint32_t *vpfx = reinterpret_cast<uint32_t *>(vlist + L);
+    uint8_t *vproj = reinterpret_cast<uint8_t *>(vpfx + L + 1);
+    for (uint32_t k = tid; k < D * kBlock; k += kTileThreads) ring[k] = 0;
+    if (tid < 3) cnt[tid] = 0;
+    if (tid < net->n_projs) {
+        const ProjDev &Q = net->projs[tid];
+        tp[tid] = TileProj{Q.syn8, Q.tile_dir, Q.row_meta, Q.row_dir, Q.row_ptr, Q.tile_stride, Q.n_pre};
+    }
+    // 1. the spiking rows that may reach the tile, numbered through a prefix over its segments' counts
+    const uint32_t nv = td.vend - td.vbeg;
+    const TileSeg *ts = net->tile_seg + td.vbeg;
+    const uint32_t *vc = net->vcount + (size_t)(t & 1) * net->n_vseg;
+    uint32_t total = 0;
+    for (uint32_t base = 0; base < nv; base += kTileThreads) {
+        const uint32_t v = base + tid;
+        uint32_t c = 0;
+        if (v < nv) {
+            const TileSeg e = ts[v];
+            vlist[v] = e.list + (size_t)(t & 1) * net->list_pstride;
+            vproj[v] = (uint8_t)e.proj;
+            c = vc[e.vpos];
+        }
+        uint32_t tot;
+        const uint32_t ex = block_excl_scan<kTileThreads>(c, wsum, &tot);
+        if (v < nv) vpfx[v] = total + ex;
+        total += tot;
+    }
+    __syncthreads();
+    const uint32_t tile_lo = td.tile * kBlock;
+    const uint32_t slot0 = (uint32_t)(t % D);
+    uint32_t my_ev = 0, my_parts = 0;
+    // exact sum of one packed synapse record into the LDS ring (false: a neuron of another tile)
+    auto add_record = [&](uint64_t rec) -> bool {
+        const uint32_t j = (uint32_t)(rec >> 41) - tile_lo;
+        if (j >= (uint32_t)kBlock) return false;  // a short row read whole: neuron of another tile
+        uint32_t slot = slot0 + ((uint32_t)(rec >> 32) & 0xFFu);
+        if (slot >= D) slot -= D;
+        long long v = __float2ll_rn(__uint_as_float((uint32_t)rec) * scale);
+        if ((rec >> 40) & 1u) v = -v;  // INHIBITORY_CURRENT, the only other type a tiled population receives
+        atomicAdd(&ring[slot * kBlock + j], (unsigned long long)v);
+        return true;
+    };
+    for (uint32_t rb = 0; rb < total; rb += kTileThreads) {
+        // 2. one thread per row: where the row's part for this tile lies
+        const uint32_t row = rb + tid;
+        uint32_t b = 0, len = 0, q = 0;
+        if (row < total) {
+            uint32_t lo = 0, hi = nv;  // largest v with vpfx[v] <= row
+            while (hi - lo > 1) {
+                const uint32_t mid = (lo + hi) >> 1;
+                if (vpfx[mid] <= row) lo = mid;
+                else hi = mid;
+            }
+            q = vproj[lo];
+            const uint32_t pre = vlist[lo][row - vpfx[lo]];
+            const TileProj &Q = tp[q];
+            if (pre < Q.n_pre) {
+                if (Q.tile_dir) {
+                    const uint32_t *d = Q.tile_dir + (size_t)pre * Q.stride + td.tile;
+                    const uint32_t d0 = d[0], d1 = d[1];
+                    b = d0;
+                    len = d1 - d0;
+                } else if (Q.row_meta) {
+                    const RowMeta m = Q.row_meta[pre];
+                    const uint32_t k = td.tile - m.first_tile;  // wraps above n_tiles for tiles before the row
+                    if (k < m.n_tiles) {
+                        const uint32_t o0 = Q.row_dir[m.dir + k], o1 = Q.row_dir[m.dir + k + 1];
+                        b = m.begin + o0;
+                        len = o1 - o0;
+                    }
+                } else {
+                    const uint64_t r0 = Q.row_ptr[pre], r1 = Q.row_ptr[pre + 1];
+                    b = (uint32_t)r0;
+                    len = (uint32_t)(r1 - r0);
+                }
+            }
+        }
+        my_parts += len ? 1u : 0u;
+        // 3a. short parts (the common case): the row's thread loads its records, 8 at a time, then adds them
+        if (len <= kTileShortPart) {
+            const uint64_t *src = tp[q].syn8 + b;
+            for (uint32_t k0 = 0; k0 < len; k0 += 8) {
+                uint64_t rec[8];
+#pragma unroll
+                for (uint32_t u = 0; u < 8; ++u)
+                    if (k0 + u < len) rec[u] = src[k0 + u];
+#pragma unroll
+                for (uint32_t u = 0; u < 8; ++u)
+                    if (k0 + u < len && add_record(rec[u])) ++my_ev;
+            }
+        }
+        // 3b. long parts: spread over the workgroup, one thread per synapse
+        const uint32_t llen = len > kTileShortPart ? len : 0u;
+        if (__syncthreads_or(llen != 0)) {
+            uint32_t n_syn;
+            const uint32_t ex = block_excl_scan<kTileThreads>(llen, wsum, &n_syn);
+            rbeg[tid] = b;
+            rpfx[tid] = ex;
+            rproj[tid] = (uint8_t)q;
+            __syncthreads();
+            const uint32_t nrows = min((uint32_t)kTileThreads, total - rb);
+            for (uint32_t s = tid; s < n_syn; s += kTileThreads) {
+                uint32_t lo = 0, hi = nrows;  // largest r with rpfx[r] <= s: a row whose part holds s
+                while (hi - lo > 1) {
+                    const uint32_t mid = (lo + hi) >> 1;
+                    if (rpfx[mid] <= s) lo = mid;
+                    else hi = mid;
+                }
+                if (add_record(tp[rproj[lo]].syn8[rbeg[lo] + (s - rpfx[lo])])) ++my_ev;
+            }
+        }
+        __syncthreads();
+    }
+    // 4. add the tile's sums into the ring; no other thread of the device writes these cells in this kernel.
+    // All loads first, then all stores, so the round trips overlap.
+    constexpr uint32_t kCells = kTileMaxD * kBlock / kTileThreads;
+    unsigned long long x[kCells];
+    int64_t old[kCells];
+    int64_t *cell[kCells];
+    uint32_t my_cells = 0;
+    const uint32_t s1 = PHASE1 ? (uint32_t)((t + 1) % D) : 0xFFFFFFFFu;  // consumed below, not written back
+#pragma unroll
+    for (uint32_t u = 0; u < kCells; ++u) {
+        const uint32_t c = tid + u * kTileThreads;
+        const uint32_t i = tile_lo + (c % kBlock);
+        x[u] = 0;
+        cell[u] = nullptr;
+        if (c < D * kBlock && i < n_pop && c / kBlock != s1) {
+            x[u] = ring[c];
+            cell[u] = acc_v + (size_t)(c / kBlock) * n_pop + i;
+        }
+    }
+#pragma unroll
+    for (uint32_t u = 0; u < kCells; ++u) old[u] = x[u] ? *cell[u] : 0;
+#pragma unroll
+    for (uint32_t u = 0; u < kCells; ++u)
+        if (x[u]) {
+            *cell[u] = old[u] + (long long)x[u];
+            ++my_cells;
+        }
+    my_ev = wave_sum_u32(my_ev);
+    my_parts = wave_sum_u32(my_parts);
+    my_cells = wave_sum_u32(my_cells);
+    if (lane_id() == 0) {
+        atomicAdd(&cnt[0], (unsigned long long)my_ev);
+        atomicAdd(&cnt[1], (unsigned long long)my_parts);
+        atomicAdd(&cnt[2], (unsigned long long)my_cells);
+    }
+    __syncthreads();
+    if (tid == 0) {
+        unsigned long long *tpart = net->tile_part + 4 * (size_t)blockIdx.x;
+        tpart[0] += cnt[0];
+        tpart[2] += cnt[1];
+        tpart[3] += cnt[2];
+    }
+    // rows expanded this step by the tiled path: block 0 sums the counts of every tiled segment
+    if (blockIdx.x == 0) {
+        uint32_t r = 0;
+        for (uint32_t v = net->n_vseg_atomic + tid; v < net->n_vseg; v += kTileThreads) r += vc[v];
+        r = wave_sum_u32(r);
+        if (lane_id() == 0) wsum[tid / 64] = r;
+        __syncthreads();
+        if (tid == 0) {
+            unsigned long long rows = 0;
+            for (uint32_t k = 0; k < kTileThreads / 64; ++k) rows += wsum[k];
+            net->tile_part[1] += rows;
+        }
+    }
+    if (PHASE1) {
+        // 5. phase 1 of step t + 1: slot s1 = earlier steps' sums in HBM + this step's sums in LDS
+        const uint64_t t1 = t + 1;
+        const uint32_t i = tile_lo + tid;
+        int64_t a = 0;
+        if (tid < (uint32_t)kBlock && i < n_pop) {
+            int64_t *c1 = acc_v + (size_t)s1 * n_pop + i;
+            a = *c1 + (int64_t)ring[s1 * kBlock + tid];
+            *c1 = 0;
+        }
+        Ctl *ctl = &net->ctl[t1 & 1];
+        uint32_t *seg_count = net->seg_count + (size_t)(t1 & 1) * net->n_seg;
+        if (P.kind == POP_LEAN)
+            population_step<POP_LEAN, kTileThreads, true>(net, td.pop, td.tile, t1, ctl, seg_count, a);
+        else
+            population_step<POP_FULL, kTileThreads, true>(net, td.pop, td.tile, t1, ctl, seg_count, a);
     }
 }
 
@@ -641,7 +930,7 @@ __global__ __launch_bounds__(kBlock) void k_additive_append(const NetDev *__rest
         for (uint32_t item = wave; item < chunks * kBlock; item += waves) {
             const uint32_t chunk = item / kBlock, r = item % kBlock;
             if (r >= cnt[SP.seg_base + chunk]) continue;
-            const uint32_t nidx = SP.spike_list[(size_t)chunk * kBlock + r];
+            const uint32_t nidx = SP.spike_list[(size_t)(t & 1) * net->list_pstride + (size_t)chunk * kBlock + r];
             if (nidx >= n_post) continue;
             const uint64_t cb = Q.col_ptr[nidx], ce = Q.col_ptr[nidx + 1];
             for (uint64_t k = cb + lane; k < ce; k += 64) {

@@ -234,6 +234,141 @@ def test_random_full_network(blocking):
     assert g.events == o.events
 
 
+def mixed_network(o, seed):
+    """Populations on both summation paths: A (default BLIFAT, delays up to 20) only receives plain Delta
+    EXCITATORY / INHIBITORY_CURRENT impacts, so it is tiled: long recurrent rows (tile directories), short rows
+    from B (read whole), host pushes and a Bernoulli source over scattered targets; B (dopamine input) and
+    C (a delay of 40) stay on the atomic path."""
+    rng = np.random.default_rng(seed)
+    na, nb, nc = 1500, 700, 300
+    a = o.add_blifat_population(abi.neuron_array(abi.blifat_defaults(), na))
+    b = o.add_blifat_population(abi.neuron_array(abi.blifat_defaults(), nb))
+    c = o.add_blifat_population(abi.neuron_array(abi.blifat_defaults(), nc))
+    ident = np.arange(na, dtype=np.uint32)
+    push = o.add_delta_projection(abi.KNP_NO_POPULATION, na, a, ident, ident, np.full(na, 1.0, np.float32),
+                                  np.ones(na, np.uint32))
+    S = 60_000
+    o.add_delta_projection(a, na, a, rng.integers(0, na, S).astype(np.uint32), rng.integers(0, na, S).astype(np.uint32),
+                           (rng.integers(1, 48, S) / 256.0).astype(np.float32), rng.integers(1, 21, S).astype(np.uint32),
+                           rng.choice([abi.EXCITATORY, abi.INHIBITORY_CURRENT], S, p=[0.8, 0.2]).astype(np.uint8))
+    S2 = 1400
+    o.add_delta_projection(b, nb, a, rng.integers(0, nb, S2).astype(np.uint32), rng.integers(0, na, S2).astype(np.uint32),
+                           (rng.integers(1, 64, S2) / 128.0).astype(np.float32), rng.integers(1, 6, S2).astype(np.uint32))
+    S3 = 40_000
+    o.add_delta_projection(a, na, b, rng.integers(0, na, S3).astype(np.uint32), rng.integers(0, nb, S3).astype(np.uint32),
+                           (rng.integers(64, 160, S3) / 128.0).astype(np.float32), rng.integers(1, 6, S3).astype(np.uint32),
+                           rng.choice([abi.EXCITATORY, abi.DOPAMINE], S3, p=[0.7, 0.3]).astype(np.uint8))
+    S4 = 6000
+    o.add_delta_projection(a, na, c, rng.integers(0, na, S4).astype(np.uint32), rng.integers(0, nc, S4).astype(np.uint32),
+                           (rng.integers(1, 64, S4) / 128.0).astype(np.float32), rng.integers(30, 41, S4).astype(np.uint32))
+    nch = 900
+    src = o.add_delta_projection(abi.KNP_NO_POPULATION, nch, a, np.arange(nch, dtype=np.uint32),
+                                 rng.integers(0, na, nch).astype(np.uint32), np.full(nch, 0.75, np.float32),
+                                 rng.integers(1, 4, nch).astype(np.uint32))
+    o.add_bernoulli_source(src, np.full(nch, workloads.bernoulli_threshold(0.03), np.uint32), seed=11)
+    return (a, b, c), push
+
+
+def test_tiled_scatter_matches_atomic_and_oracle():
+    say("case: tile-owner scatter vs atomic scatter vs oracle (mixed network)")
+    steps = 200
+    runs = {}
+    for name in ("tiled", "atomic", "oracle"):
+        o = Oracle("exact") if name == "oracle" else MI355XBackend(0)
+        if name == "atomic":
+            o.set_scatter_mode(abi.SCATTER_ATOMIC)
+        pops, push = mixed_network(o, 5)
+        if name == "tiled":
+            assert [o.tiled(p) for p in pops] == [True, False, False]
+        if name == "atomic":
+            assert not any(o.tiled(p) for p in pops)
+        for p in pops:
+            o.record_spikes(p, steps)
+        rng = np.random.default_rng(21)
+        for t in range(steps):
+            idx = np.nonzero(rng.random(1500) < 0.02)[0].astype(np.uint32)
+            if len(idx):
+                o.push_spikes(push, idx)
+            o.step()
+        runs[name] = ([o.read_spike_record(p) for p in pops], [o.potentials(p) for p in pops], o.events)
+        if name == "tiled":
+            parts, cells = o.tile_counters()
+            say(f"  tiled path: {parts} row parts, {cells} ring cells")
+            assert parts > 0 and cells > 0
+    for k in range(3):
+        so = runs["oracle"][0][k]
+        assert so.sum() > 50, k
+        for name in ("tiled", "atomic"):
+            assert np.array_equal(runs[name][0][k], so), (name, k)
+            assert rel_err(runs[name][1][k], runs["oracle"][1][k]) <= POT_RTOL, (name, k)
+    assert runs["tiled"][2] == runs["atomic"][2] == runs["oracle"][2]
+
+
+def all_tiled_network(o, seed):
+    """Every population tiled (only plain Delta EXCITATORY / INHIBITORY_CURRENT input, delays <= 24): the
+    device runs phase 1 of step t + 1 inside the tile kernel of step t. A (default BLIFAT) and B (refractory
+    period, postsynaptic traces, potential floor: the FULL state path) with recurrent, cross and channel
+    projections and a Bernoulli source. Potentials stay sums of weights on a 2^-8 grid, so FP32 is exact."""
+    rng = np.random.default_rng(seed)
+    na, nb = 1200, 600
+    a = o.add_blifat_population(abi.neuron_array(abi.blifat_defaults(), na))
+    b = o.add_blifat_population(abi.neuron_array(abi.blifat_defaults(
+        absolute_refractory_period=2, postsynaptic_trace_decay=0.5, postsynaptic_trace_increment=1.0,
+        min_potential=-2.0), nb))
+    ident = np.arange(na, dtype=np.uint32)
+    push = o.add_delta_projection(abi.KNP_NO_POPULATION, na, a, ident, ident, np.full(na, 1.0, np.float32),
+                                  np.ones(na, np.uint32))
+    def proj(pre_pop, n_pre, post_pop, n_post, S, wlo, whi, dmax, inh):
+        o.add_delta_projection(pre_pop, n_pre, post_pop, rng.integers(0, n_pre, S).astype(np.uint32),
+                               rng.integers(0, n_post, S).astype(np.uint32),
+                               (rng.integers(wlo, whi, S) / 256.0).astype(np.float32),
+                               rng.integers(1, dmax + 1, S).astype(np.uint32),
+                               rng.choice([abi.EXCITATORY, abi.INHIBITORY_CURRENT], S, p=[1 - inh, inh]).astype(np.uint8))
+    proj(a, na, a, na, 48_000, 4, 96, 24, 0.2)
+    proj(a, na, b, nb, 30_000, 16, 128, 6, 0.1)
+    proj(b, nb, a, na, 1_000, 32, 200, 3, 0.5)
+    nch = 600
+    src = o.add_delta_projection(abi.KNP_NO_POPULATION, nch, b, np.arange(nch, dtype=np.uint32),
+                                 np.arange(nch, dtype=np.uint32) % nb, np.full(nch, 1.0, np.float32),
+                                 np.ones(nch, np.uint32))
+    o.add_bernoulli_source(src, np.full(nch, workloads.bernoulli_threshold(0.02), np.uint32), seed=5)
+    return (a, b), push
+
+
+def test_fused_steps_match_oracle():
+    say("case: fused tile steps (all populations tiled), calls of 1..40 steps with pushes between calls")
+    chunks = [1, 7, 40, 2, 1, 25, 3, 31, 1, 9]
+    steps = sum(chunks)
+    runs = {}
+    for name in ("fused", "atomic", "oracle"):
+        o = Oracle("exact") if name == "oracle" else MI355XBackend(0)
+        if name == "atomic":
+            o.set_scatter_mode(abi.SCATTER_ATOMIC)
+        pops, push = all_tiled_network(o, 8)
+        if name == "fused":
+            assert all(o.tiled(p) for p in pops)
+        for p in pops:
+            o.record_spikes(p, steps)
+        rng = np.random.default_rng(4)
+        mids = []
+        for c in chunks:
+            idx = np.nonzero(rng.random(1200) < 0.05)[0].astype(np.uint32)
+            o.push_spikes(push, idx)
+            o.step(c)
+            mids.append((o.potentials(pops[0]).copy(), o.last_spikes(pops[1]).copy()))
+        runs[name] = ([o.read_spike_record(p) for p in pops], mids, o.events)
+    for k in range(2):
+        so = runs["oracle"][0][k]
+        assert so.sum() > 100, k
+        for name in ("fused", "atomic"):
+            assert np.array_equal(runs[name][0][k], so), (name, k)
+    for j in range(len(chunks)):
+        for name in ("fused", "atomic"):
+            assert rel_err(runs[name][1][j][0], runs["oracle"][1][j][0]) <= POT_RTOL, (name, j)
+            assert np.array_equal(runs[name][1][j][1], runs["oracle"][1][j][1]), (name, j)
+    assert runs["fused"][2] == runs["atomic"][2] == runs["oracle"][2]
+
+
 def agreement(a, b):
     inter = np.logical_and(a, b).sum()
     union = np.logical_or(a, b).sum()
