@@ -47,8 +47,9 @@ def main():
     ap.add_argument("--timing-steps", type=int, default=1000)
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="target CPU-baseline sample length")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--scatter", choices=["auto", "atomic"], default="auto",
-                    help="impact summation: tiled populations where eligible (auto) or atomics everywhere")
+    ap.add_argument("--scatter", choices=["auto", "stepwise", "atomic"], default="auto",
+                    help="impact summation: tiled populations and persistent launches where eligible (auto), "
+                         "tiled with one launch per step (stepwise), or atomics everywhere")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -68,7 +69,8 @@ def main():
     from knp_amd import abi
 
     gpu = MI355XBackend(0)
-    gpu.set_scatter_mode(abi.SCATTER_AUTO if args.scatter == "auto" else abi.SCATTER_ATOMIC)
+    gpu.set_scatter_mode({"auto": abi.SCATTER_AUTO, "stepwise": abi.SCATTER_STEPWISE,
+                          "atomic": abi.SCATTER_ATOMIC}[args.scatter])
     gpu = spec.load_into(gpu)
     t_up = time.perf_counter()
     gpu._init()
@@ -109,36 +111,39 @@ def main():
     for name, kid in KERNEL_IDS.items():
         avg_ms, n = gpu.kernel_time(kid)
         if n:
-            kern[name] = {"avg_us": avg_ms * 1e3, "launches": n}
+            kern[name] = {"avg_us": avg_ms * 1e3, "launches": n, "us_per_step": avg_ms * 1e3 * n / tsteps}
     gpu.enable_timing(False)
-    ev_launch = (e3 - e2) / tsteps
-    rows_launch = (r3 - r2) / tsteps
-    parts_launch = (p3 - p2) / tsteps
-    cells_launch = (c3 - c2) / tsteps
-    tiled = gpu.tiled(pop)
-    if tiled:
-        # tile-owner scatter: 9 B per impact (post|type 4, weight 4, delay 1), 8 B per row part (two tile
-        # directory entries), 20 B per spiking row (row metadata 16, spike-list entry 4), 16 B per ring cell
-        # updated in HBM (read 8 + write 8)
-        scatter_name, scatter_kernel = "tile_scatter", "k_tile_scatter"
-        scatter_bytes = ev_launch * 9.0 + parts_launch * 8.0 + rows_launch * 20.0 + cells_launch * 16.0
+    ev_step = (e3 - e2) / tsteps
+    rows_step = (r3 - r2) / tsteps
+    parts_step = (p3 - p2) / tsteps
+    cells_step = (c3 - c2) / tsteps
+    # Algorithmic HBM bytes per step of each kernel class:
+    #   impacts: 8 B each on the tiled path (one packed record: weight, delay, type, post), 9 B on the atomic
+    #            path (post|type 4, weight 4, delay 1)
+    #   tiled path: 8 B per row part (two tile-directory entries), 4 B per spiking row (spike-list entry),
+    #            16 B per ring cell updated in HBM (read 8 + write 8; none in the persistent launch, whose ring
+    #            stays in LDS)
+    #   atomic path: 20 B per expanded row (row_ptr pair 16, index 4) + 8 B per impact atomic
+    #   population step (default BLIFAT): 4 B potential write + 1/8 B bitmap per neuron, plus 16 B per neuron
+    #            for the ring slot (read 8 + clear 8) when the ring is in HBM
+    lean_bytes = n_neurons * (4.0 + 1.0 / 8.0)
+    ring_slot_bytes = n_neurons * 16.0
+    if "tile_run" in kern:
+        dominant, kname = "tile_run", "k_tile_run"
+        step_bytes = ev_step * 8.0 + parts_step * 8.0 + rows_step * 4.0 + cells_step * 16.0 + lean_bytes
+    elif "tile_scatter" in kern:
+        dominant, kname = "tile_scatter", "k_tile_scatter"
+        step_bytes = ev_step * 8.0 + parts_step * 8.0 + rows_step * 4.0 + cells_step * 16.0
+        if kern.get("neuron", {}).get("launches", 0) < tsteps:  # phase 1 fused into the tile launch
+            step_bytes += lean_bytes + ring_slot_bytes
     else:
-        # atomic scatter: 9 B per impact (post|type 4, weight 4, delay 1) + 20 B per expanded row (row_ptr pair
-        # 16, index 4); the atomics themselves are counted as 8 B each
-        scatter_name, scatter_kernel = "scatter", "k_scatter"
-        scatter_bytes = ev_launch * 17.0 + rows_launch * 20.0
-    # population step (default BLIFAT): 20 B per neuron (ring slot read 8 + clear 8, potential write 4) + bitmap
-    neuron_bytes = n_neurons * 20.0 + n_neurons / 8.0
-    sc_s = kern[scatter_name]["avg_us"] * 1e-6
-    ne_s = kern["neuron"]["avg_us"] * 1e-6
-    sc_gbs = scatter_bytes / sc_s / 1e9
-    ne_gbs = neuron_bytes / ne_s / 1e9
-    kern[scatter_name].update(bytes_per_launch=scatter_bytes, achieved_GBs=sc_gbs)
-    kern["neuron"].update(bytes_per_launch=neuron_bytes, achieved_GBs=ne_gbs)
-    dominant = scatter_name if kern[scatter_name]["avg_us"] >= kern["neuron"]["avg_us"] else "neuron"
+        dominant, kname = "scatter", "k_scatter"
+        step_bytes = ev_step * 17.0 + rows_step * 20.0
     dk = kern[dominant]
+    per_launch = tsteps / dk["launches"]  # steps per launch
+    dk.update(bytes_per_step=step_bytes, achieved_GBs=step_bytes / (dk["us_per_step"] * 1e-6) / 1e9)
     roofline = {
-        "kernel": scatter_kernel if dominant == scatter_name else "k_phase1",
+        "kernel": kname,
         "bound": "hbm",
         "achieved": round(dk["achieved_GBs"], 2),
         "peak": HBM_PEAK_GBS,
@@ -146,13 +151,16 @@ def main():
         "frac": round(dk["achieved_GBs"] / HBM_PEAK_GBS, 5),
         "traffic": None,
         "avg_launch_us": round(dk["avg_us"], 3),
-        "bytes_per_launch": round(dk["bytes_per_launch"], 1),
+        "steps_per_launch": per_launch,
+        "bytes_per_launch": round(step_bytes * per_launch, 1),
+        "us_per_step": round(dk["us_per_step"], 3),
     }
-    traffic_file = os.path.join(ROOT, "profiles", f"{roofline['kernel']}_plain_gpu_scale_traffic.json")
+    traffic_file = os.path.join(ROOT, "profiles", f"{kname}_plain_gpu_scale_traffic.json")
     if os.path.exists(traffic_file):
         try:
-            roofline["traffic"] = json.load(open(traffic_file)).get("hbm_bytes_per_launch")
-        except (OSError, ValueError):
+            # HBM bytes per step measured with rocprofv3 counters, scaled to this launch length
+            roofline["traffic"] = json.load(open(traffic_file))["hbm_bytes_per_step"] * per_launch
+        except (OSError, ValueError, KeyError):
             pass
 
     # ---- CPU baseline: the oracle (restated single-threaded backend), bounded sample, 1 thread
@@ -222,7 +230,9 @@ def main():
         },
         "net_steps_per_s": args.steps / dt,
         "events_per_step": events / args.steps,
-        "summation": "tile-owner (LDS) scatter" if tiled else "atomic scatter",
+        "summation": {"tile_run": "tile-owner scatter, persistent launch (ring in LDS)",
+                      "tile_scatter": "tile-owner scatter, one launch per step",
+                      "scatter": "atomic scatter"}[dominant],
         "roofline": roofline,
         "kernels": {k: {kk: (round(vv, 3) if isinstance(vv, float) else vv) for kk, vv in v.items()}
                     for k, v in kern.items()},

@@ -144,8 +144,10 @@ int knp_mi355x_set_memory_cap(knp_net *net, uint64_t bytes);
 /* How impacts are summed (no reference counterpart: an execution choice with identical results).
  * AUTO: populations that only receive plain Delta EXCITATORY / INHIBITORY_CURRENT impacts with delays up to
  * 32 are tiled (each 256-neuron tile owned by one workgroup, sums in LDS, no global atomics); the rest use
- * per-impact 64-bit atomics. ATOMIC: atomics everywhere. Set before the first step. */
-enum knp_scatter_mode { KNP_SCATTER_AUTO = 0, KNP_SCATTER_ATOMIC = 1 };
+ * per-impact 64-bit atomics. When every population is tiled and all tiles fit on the device at once, a
+ * multi-step call runs as one persistent launch (ring kept in LDS, one grid barrier per step).
+ * STEPWISE: as AUTO but one launch per step. ATOMIC: atomics everywhere. Set before the first step. */
+enum knp_scatter_mode { KNP_SCATTER_AUTO = 0, KNP_SCATTER_ATOMIC = 1, KNP_SCATTER_STEPWISE = 2 };
 int knp_mi355x_set_scatter_mode(knp_net *net, int mode);
 /* 1 if the population's impacts are summed by tiles (valid after the first step or an explicit init). */
 int knp_mi355x_get_tiled(knp_net *net, uint32_t pop, int *tiled);
@@ -260,8 +262,9 @@ enum knp_kernel_id {
     KNP_KERNEL_SCATTER = 1, /* spike -> impact scatter into the delay ring */
     KNP_KERNEL_TRAIN = 2,   /* synaptic-resource STDP */
     KNP_KERNEL_DENSE = 3,   /* dense input projection on MFMA (spike bits + product), per time block */
-    KNP_KERNEL_TILE_SCATTER = 4, /* tile-owner scatter (tiled populations) */
-    KNP_KERNEL_COUNT = 5
+    KNP_KERNEL_TILE_SCATTER = 4, /* tile-owner scatter (tiled populations), one step per launch */
+    KNP_KERNEL_TILE_RUN = 5,     /* persistent tile launch: several steps per launch */
+    KNP_KERNEL_COUNT = 6
 };
 int knp_mi355x_enable_timing(knp_net *net, int enable);
 /* Average duration (ms) and launch count of a kernel class since timing was enabled. */

@@ -36,10 +36,12 @@ KERNEL_SCATTER = 1
 KERNEL_TRAIN = 2
 KERNEL_DENSE = 3
 KERNEL_TILE_SCATTER = 4
+KERNEL_TILE_RUN = 5
 
 # knp_scatter_mode
 SCATTER_AUTO = 0
 SCATTER_ATOMIC = 1
+SCATTER_STEPWISE = 2
 
 # knp_dense_format
 DENSE_FP8_E4M3 = 0

@@ -269,7 +269,9 @@ class MI355XBackend:
         return e.value, r.value
 
     def set_scatter_mode(self, mode):
-        """abi.SCATTER_AUTO (tiled populations where eligible) or abi.SCATTER_ATOMIC; before the first step."""
+        """abi.SCATTER_AUTO (tiled populations where eligible, persistent multi-step launches when every
+        population is tiled), abi.SCATTER_STEPWISE (tiled, one launch per step) or abi.SCATTER_ATOMIC;
+        before the first step."""
         self._chk(self.lib.knp_mi355x_set_scatter_mode(self.net, mode))
 
     def tiled(self, pop):
@@ -355,4 +357,4 @@ class MI355XBackend:
 
 
 KERNEL_IDS = {"neuron": abi.KERNEL_NEURON, "scatter": abi.KERNEL_SCATTER, "train": abi.KERNEL_TRAIN,
-              "dense": abi.KERNEL_DENSE, "tile_scatter": abi.KERNEL_TILE_SCATTER}
+              "dense": abi.KERNEL_DENSE, "tile_scatter": abi.KERNEL_TILE_SCATTER, "tile_run": abi.KERNEL_TILE_RUN}

@@ -224,6 +224,7 @@ struct NetDev {
     uint32_t tile_max_list;    // longest such range
     uint32_t tile_max_d;       // deepest ring of a tiled population
     uint32_t n_src_blocks;     // workgroups of the Bernoulli sources (256 channels each)
+    uint32_t *grid_bar;        // [2] barrier counter and abort flag of persistent launches
     unsigned long long *tile_part;  // [4 * n_tiles] impacts, rows, row segments, ring cells updated
     uint32_t blk_proj[64];     // projection index of each projection with BLOCKING synapses
     uint32_t n_dense;

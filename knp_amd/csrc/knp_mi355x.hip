@@ -103,6 +103,7 @@ struct knp_net {
     uint32_t scatter_grid = 1, train_grid = 1;
     int scatter_mode = KNP_SCATTER_AUTO;
     bool fused = false;     // every population tiled: phase 1 of step t + 1 runs inside the tile kernel of t
+    bool persistent = false;  // fused, and every tile fits on the device at once (cooperative launch)
     uint32_t tile_lds = 0;  // dynamic LDS bytes of the tile scatter (largest tiled ring)
     std::vector<uint32_t *> seg_list_host;
     std::vector<uint32_t> push_vpos;  // virtual position of each projection's push segment
@@ -610,7 +611,7 @@ int init_network(knp_net *net) {
     }
     for (uint32_t p = 0; p < n_pops; ++p) {
         HostPop &H = net->pops[p];
-        H.tiled = net->scatter_mode == KNP_SCATTER_AUTO && !H.resource && !H.has_cond && !H.has_dop &&
+        H.tiled = net->scatter_mode != KNP_SCATTER_ATOMIC && !H.resource && !H.has_cond && !H.has_dop &&
                   !H.has_block && H.D <= tile_max_d && H.n > 0 && H.n < (1u << 23);
     }
     for (uint32_t q = 0; q < n_projs; ++q)
@@ -1048,6 +1049,19 @@ int init_network(knp_net *net) {
     for (uint32_t s = 0; s < T.n_srcs; ++s) T.n_src_blocks += std::max(1u, (net->srcs[s].n_ch + kBlock - 1) / kBlock);
     net->fused = T.n_tiles > 0;
     for (const auto &H : net->pops) net->fused = net->fused && H.tiled;
+    net->persistent = false;
+    if (net->fused && net->scatter_mode == KNP_SCATTER_AUTO) {
+        int coop = 0, cus = 0, per_cu = 0;
+        HIP_TRY(hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, net->device));
+        HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, net->device));
+        HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void *>(&k_tile_run),
+                                                             kTileThreads, net->tile_lds));
+        net->persistent = coop && (uint64_t)per_cu * cus >= T.n_tiles;
+    }
+    {
+        int rc;
+        if ((rc = dev_alloc(net, (void **)&T.grid_bar, kBarWords * 4))) return rc;
+    }
 
     // ---- control blocks
     int rc;
@@ -1193,6 +1207,47 @@ int run_steps(knp_net *net, uint64_t n_steps) {
         }
         return KNP_OK;
     };
+    if (net->persistent && n_steps >= 2) {
+        // persistent launches, split where a dense projection starts a new time block (its product must be
+        // ready before phase 1 of that step)
+        uint64_t done = 0;
+        while (done < n_steps) {
+            const uint64_t t = net->step;
+            int rc = deliver_pushes(net);
+            if (rc) return rc;
+            if ((rc = launch_dense_due(t))) return rc;
+            // counters of the grid barrier grow by one per step and workgroup: 2^22 steps per launch at most
+            uint64_t len = std::min<uint64_t>(n_steps - done, uint64_t(1) << 22);
+            for (const auto &Dh : net->dense) {
+                const uint64_t t_first = net->pops[Dh.post_pop].t_first;
+                const uint64_t next = t_first + ((t - t_first) / Dh.T + 1) * Dh.T;
+                len = std::min<uint64_t>(len, next - t);
+            }
+            HIP_TRY(hipMemsetAsync(net->table.grid_bar, 0, kBarWords * 4, net->stream));
+            hipError_t le = hipSuccess;
+            const uint32_t nt = net->table.n_tiles, steps = (uint32_t)len;
+            uint64_t t0 = t;
+            void *args[] = {(void *)&D, (void *)&t0, (void *)&steps};
+            if ((rc = timed_launch(net, KNP_KERNEL_TILE_RUN, [&] {
+                     le = hipLaunchCooperativeKernel(reinterpret_cast<const void *>(&k_tile_run), dim3(nt),
+                                                     dim3(kTileThreads), args, net->tile_lds, net->stream);
+                 })))
+                return rc;
+            if (le != hipSuccess) return fail(KNP_ERR_HIP, std::string("persistent launch: ") + hipGetErrorString(le));
+            net->step += len;
+            done += len;
+            for (auto &P : net->pops) P.ran = true;
+        }
+        uint32_t flag = 0;
+        HIP_TRY(hipMemcpyAsync(&flag, net->table.grid_bar + kBarAbort, 4, hipMemcpyDeviceToHost, net->stream));
+        HIP_TRY(hipStreamSynchronize(net->stream));
+        if (flag) return fail(KNP_ERR_HIP, "persistent launch: a grid barrier timed out (state is undefined)");
+        if (net->timing && net->ev_used > 8192) {
+            int rc = collect_timing(net);
+            if (rc) return rc;
+        }
+        return KNP_OK;
+    }
     // phase 1 of the current step already ran inside the previous step's tile kernel
     bool phase1_done = false;
     for (uint64_t it = 0; it < n_steps; ++it) {
@@ -1233,8 +1288,7 @@ int run_steps(knp_net *net, uint64_t n_steps) {
         const uint32_t nt = net->table.n_tiles;
         if (nt && (rc = timed_launch(net, KNP_KERNEL_TILE_SCATTER, [&] {
                        if (p1)
-                           k_tile_scatter<true><<<nt + net->table.n_src_blocks, kTileThreads, net->tile_lds,
-                                                  net->stream>>>(D, t);
+                           k_tile_scatter<true><<<nt, kTileThreads, net->tile_lds, net->stream>>>(D, t);
                        else
                            k_tile_scatter<false><<<nt, kTileThreads, net->tile_lds, net->stream>>>(D, t);
                    })))
@@ -1928,7 +1982,7 @@ int knp_mi355x_get_tile_counters(knp_net *net, uint64_t *row_parts, uint64_t *ri
 
 int knp_mi355x_set_scatter_mode(knp_net *net, int mode) {
     if (!net) return fail(KNP_ERR_INVALID_ARGUMENT, "null handle");
-    if (mode != KNP_SCATTER_AUTO && mode != KNP_SCATTER_ATOMIC)
+    if (mode != KNP_SCATTER_AUTO && mode != KNP_SCATTER_ATOMIC && mode != KNP_SCATTER_STEPWISE)
         return fail(KNP_ERR_INVALID_ARGUMENT, "unknown scatter mode");
     if (net->initialized) return fail(KNP_ERR_STATE, "the scatter mode must be set before the first step");
     net->scatter_mode = mode;

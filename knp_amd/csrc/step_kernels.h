@@ -18,6 +18,15 @@
 
 namespace knp_mi355x {
 
+// Pointers read from the network tables are generic (flat) to the compiler; flat accesses also count on the
+// LDS counter, so every LDS wait and workgroup barrier would wait for them. Accesses through gp() are
+// global-memory instructions.
+#define KNP_GLOBAL __attribute__((address_space(1)))
+template <typename T>
+__device__ __forceinline__ KNP_GLOBAL T *gp(T *p) {
+    return (KNP_GLOBAL T *)p;
+}
+
 __host__ __device__ inline uint64_t mix64(uint64_t z) {
     // splitmix64 finalizer (Steele, Lea, Flood 2014)
     z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
@@ -31,8 +40,8 @@ __host__ __device__ inline uint32_t bernoulli_hash(uint64_t seed, uint64_t step,
     return (uint32_t)(b >> 32);
 }
 
-__device__ __forceinline__ float pf(const ParamF &p, uint32_t i) { return p.a ? p.a[i] : p.v; }
-__device__ __forceinline__ uint32_t pu(const ParamU &p, uint32_t i) { return p.a ? p.a[i] : p.v; }
+__device__ __forceinline__ float pf(const ParamF &p, uint32_t i) { return p.a ? gp(p.a)[i] : p.v; }
+__device__ __forceinline__ uint32_t pu(const ParamU &p, uint32_t i) { return p.a ? gp(p.a)[i] : p.v; }
 
 // n_time_steps_since_last_firing_ kept in 32 bits: 0xFFFFFFFF stands for SIZE_MAX (wraps to 0 on the next
 // increment, blifat_impl.h:34 with the default of blifat.h:53); counts saturate at 0xFFFFFFFE.
@@ -66,9 +75,24 @@ __device__ __forceinline__ void wave_append(bool pred, uint32_t value, uint32_t 
     if (pred) list[base + __popcll(mask & ((1ull << lane) - 1ull))] = value;
 }
 
+// Stores and loads of data that another workgroup reads or wrote within the same persistent launch:
+// device-scope relaxed atomics, i.e. written through and read past this XCD's L2 (the L2s of the XCDs are
+// not coherent with each other inside a kernel). COHERENT = false: plain accesses (kernel boundaries order
+// them).
+template <bool COHERENT>
+__device__ __forceinline__ void st_u32(uint32_t *p, uint32_t v) {
+    if (COHERENT) __hip_atomic_store(gp(p), v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    else *gp(p) = v;
+}
+template <bool COHERENT>
+__device__ __forceinline__ uint32_t ld_u32(const uint32_t *p) {
+    if (COHERENT) return __hip_atomic_load(gp(const_cast<uint32_t *>(p)), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    return *gp(p);
+}
+
 // Ordered compaction of a workgroup's 256 candidates into its own segment: ascending order, no global
 // atomics, one count store. NT = threads of the workgroup (threads from 256 on carry no candidate).
-template <uint32_t NT = kBlock>
+template <uint32_t NT = kBlock, bool COHERENT = false>
 __device__ __forceinline__ void block_segment_write(const NetDev *__restrict__ net, uint64_t t, bool pred,
                                                     uint32_t value, uint32_t *segment, uint32_t seg) {
     __shared__ uint32_t wcnt[NT / 64];
@@ -81,12 +105,12 @@ __device__ __forceinline__ void block_segment_write(const NetDev *__restrict__ n
         if (k < w) off += wcnt[k];
         total += wcnt[k];
     }
-    if (pred) segment[off + __popcll(mask & ((1ull << lane) - 1ull))] = value;
+    if (pred) st_u32<COHERENT>(&segment[off + __popcll(mask & ((1ull << lane) - 1ull))], value);
     if (threadIdx.x < 4) {
         const size_t par = (size_t)(t & 1);
-        if (threadIdx.x == 0) net->seg_count[par * net->n_seg + seg] = total;
-        const uint32_t vp = net->seg_vpos[4 * seg + threadIdx.x];
-        if (vp != 0xFFFFFFFFu) net->vcount[par * net->n_vseg + vp] = total;
+        if (threadIdx.x == 0) st_u32<COHERENT>(&net->seg_count[par * net->n_seg + seg], total);
+        const uint32_t vp = gp(net->seg_vpos)[4 * seg + threadIdx.x];
+        if (vp != 0xFFFFFFFFu) st_u32<COHERENT>(&net->vcount[par * net->n_vseg + vp], total);
     }
     __syncthreads();
 }
@@ -97,8 +121,8 @@ __device__ __forceinline__ void wave_bitmap(uint64_t mask, uint32_t first_neuron
     uint32_t w0 = first_neuron >> 5;
     if (lane < 2 && w0 + lane < P.words) {
         uint32_t word = lane == 0 ? (uint32_t)mask : (uint32_t)(mask >> 32);
-        P.bitmap[w0 + lane] = word;
-        if (P.rec && t >= P.rec_t0 && t - P.rec_t0 < P.rec_cap) P.rec[(t - P.rec_t0) * P.words + w0 + lane] = word;
+        gp(P.bitmap)[w0 + lane] = word;
+        if (P.rec && t >= P.rec_t0 && t - P.rec_t0 < P.rec_cap) gp(P.rec)[(t - P.rec_t0) * P.words + w0 + lane] = word;
     }
 }
 
@@ -117,7 +141,7 @@ __device__ __forceinline__ int64_t blocking_value(const NetDev *net, uint64_t ke
 // ---- phase 1: populations + sources ----------------------------------------------------------------
 // NT = threads of the workgroup (threads from 256 on have no neuron). FUSED: the exact sum of the step's
 // impacts is passed in (a_in) by the tile kernel instead of being read from the ring.
-template <uint32_t KIND, uint32_t NT = kBlock, bool FUSED = false>
+template <uint32_t KIND, uint32_t NT = kBlock, bool FUSED = false, bool COHERENT = false>
 __device__ __forceinline__ void population_step(const NetDev *__restrict__ net, uint32_t p, uint32_t local_block,
                                                 uint64_t t, Ctl *ctl, uint32_t *seg_count, int64_t a_in = 0) {
     const PopDev &P = net->pops[p];
@@ -130,10 +154,10 @@ __device__ __forceinline__ void population_step(const NetDev *__restrict__ net, 
     if (active) {
         int64_t a = a_in;
         if (!FUSED && P.acc_v) {
-            a = P.acc_v[ai];
-            P.acc_v[ai] = 0;
+            a = gp(P.acc_v)[ai];
+            gp(P.acc_v)[ai] = 0;
         }
-        if (P.ff) a += __float2ll_rn(P.ff[(size_t)(t % (2ull * P.ff_T)) * P.n + i] * P.scale);
+        if (P.ff) a += __float2ll_rn(gp(P.ff)[(size_t)(t % (2ull * P.ff_T)) * P.n + i] * P.scale);
         if (KIND == POP_LEAN) {
             // default BLIFAT (blifat.h defaults): V_pre = 0, no decays, no refractory period, no blocking.
             const bool can = !(P.lean_ts_max && t == P.t_first);  // step 0 rule (wrap of SIZE_MAX)
@@ -142,46 +166,46 @@ __device__ __forceinline__ void population_step(const NetDev *__restrict__ net, 
             spike = can && fixed_ge(a, thr, P.scale, V);
             if (spike) {
                 V = pf(P.V_reset, i);
-                P.last_spike[i] = (uint32_t)t;
+                gp(P.last_spike)[i] = (uint32_t)t;
             }
             const float vmin = pf(P.V_min, i);
             if (V < vmin) V = vmin;
-            P.V[i] = V;
+            gp(P.V)[i] = V;
         } else {
             // pre-impact (blifat_impl.h:31-65)
-            uint32_t n_ts = ts_inc(P.n_ts[i]);
-            float dyn = P.dyn[i] * pf(P.thr_decay, i);
-            float trace = P.trace[i] * pf(P.trace_decay, i);
-            float g = P.g[i] * pf(P.g_decay, i);
-            float V = P.V[i] * pf(P.V_decay, i);
-            uint32_t bphase = P.bphase[i];
+            uint32_t n_ts = ts_inc(gp(P.n_ts)[i]);
+            float dyn = gp(P.dyn)[i] * pf(P.thr_decay, i);
+            float trace = gp(P.trace)[i] * pf(P.trace_decay, i);
+            float g = gp(P.g)[i] * pf(P.g_decay, i);
+            float V = gp(P.V)[i] * pf(P.V_decay, i);
+            uint32_t bphase = gp(P.bphase)[i];
             if (bphase == 1) {
                 if (KIND != POP_RESOURCE) bphase = 0;  // resource neurons keep the phase (blifat_impl.h:62)
                 V += pf(P.reflexive, i);
             }
             const float V_pre = V;
-            P.Vpre[i] = V_pre;
+            gp(P.Vpre)[i] = V_pre;
             // impacts (blifat_impl.h:68-107): exact sums of this step's impacts
-            int64_t tbp = P.track_tbp ? P.tbp[i] : INT64_MAX;
+            int64_t tbp = P.track_tbp ? gp(P.tbp)[i] : INT64_MAX;
             if (P.acc_g) {
-                int64_t ag = P.acc_g[ai];
-                P.acc_g[ai] = 0;
+                int64_t ag = gp(P.acc_g)[ai];
+                gp(P.acc_g)[ai] = 0;
                 if (ag) g += fixed_to_float(ag, P.inv_scale);
             }
             float dop_now = 0.f;
             if (P.acc_d) {
-                int64_t ad = P.acc_d[ai];
-                P.acc_d[ai] = 0;
+                int64_t ad = gp(P.acc_d)[ai];
+                gp(P.acc_d)[ai] = 0;
                 dop_now = fixed_to_float(ad, P.inv_scale);
             }
             uint32_t forced = 0;
             if (P.acc_f) {
-                forced = P.acc_f[ai];
-                P.acc_f[ai] = 0;
+                forced = gp(P.acc_f)[ai];
+                gp(P.acc_f)[ai] = 0;
             }
             if (P.acc_b) {
-                uint64_t key = P.acc_b[ai];
-                P.acc_b[ai] = 0;
+                uint64_t key = gp(P.acc_b)[ai];
+                gp(P.acc_b)[ai] = 0;
                 if (key) tbp = blocking_value(net, key);
             }
             V = (V_pre == 0.f) ? fixed_to_float(a, P.inv_scale) : V_pre + fixed_to_float(a, P.inv_scale);
@@ -218,41 +242,41 @@ __device__ __forceinline__ void population_step(const NetDev *__restrict__ net, 
             }
             const float vmin = pf(P.V_min, i);
             if (V < vmin) V = vmin;
-            P.V[i] = V;
-            P.n_ts[i] = n_ts;
-            P.dyn[i] = dyn;
-            P.trace[i] = trace;
-            P.g[i] = g;
-            P.bphase[i] = bphase;
-            if (P.track_tbp) P.tbp[i] = tbp;
+            gp(P.V)[i] = V;
+            gp(P.n_ts)[i] = n_ts;
+            gp(P.dyn)[i] = dyn;
+            gp(P.trace)[i] = trace;
+            gp(P.g)[i] = g;
+            gp(P.bphase)[i] = bphase;
+            if (P.track_tbp) gp(P.tbp)[i] = tbp;
             if (KIND == POP_RESOURCE) {
                 const ResDev &R = P.res;
-                R.dop_now[i] = dop_now;  // dopamine_value_ is reset every step for resource neurons
-                R.forced_now[i] = (uint8_t)(forced != 0);
+                gp(R.dop_now)[i] = dop_now;  // dopamine_value_ is reset every step for resource neurons
+                gp(R.forced_now)[i] = (uint8_t)(forced != 0);
                 // training candidate: spiked, has dopamine, or due for resource renormalization
                 // (the renormalization test only depends on state training leaves unchanged for others)
                 bool renorm = false;
-                const float fr = R.free_res[i];
+                const float fr = gp(R.free_res)[i];
                 if (fabsf(fr) >= pf(R.res_thr, i)) {
-                    const uint8_t st = R.isi_status[i];
-                    const bool in_isi = (t - R.last_step[i] <= (uint64_t)pu(R.isi_max, i)) && st != 0;
+                    const uint8_t st = gp(R.isi_status)[i];
+                    const bool in_isi = (t - gp(R.last_step)[i] <= (uint64_t)pu(R.isi_max, i)) && st != 0;
                     renorm = !in_isi;
                 }
                 cand = spike || dop_now != 0.f || renorm;
             } else if (P.acc_d) {
-                P.dop[i] += dop_now;  // plain BLIFAT keeps summing dopamine (never reset for it)
+                gp(P.dop)[i] += dop_now;  // plain BLIFAT keeps summing dopamine (never reset for it)
             }
         }
     }
     // spike compaction and bitmaps (whole wave participates)
-    block_segment_write<NT>(net, t, spike, i, P.spike_list + (size_t)(t & 1) * net->list_pstride + (size_t)local_block * kBlock,
+    block_segment_write<NT, COHERENT>(net, t, spike, i, P.spike_list + (size_t)(t & 1) * net->list_pstride + (size_t)local_block * kBlock,
                             P.seg_base + local_block);
     const uint64_t mask = __ballot(spike);
     if (threadIdx.x < (uint32_t)kBlock) wave_bitmap(mask, local_block * kBlock + (threadIdx.x & ~63u), P, t);
     if (KIND == POP_RESOURCE) wave_append(cand, i, &ctl->cand_count[p], P.res.cand);
 }
 
-template <uint32_t NT = kBlock>
+template <uint32_t NT = kBlock, bool COHERENT = false>
 __device__ __forceinline__ void source_step(const NetDev *__restrict__ net, uint32_t s, uint32_t local_block,
                                             uint64_t t, uint32_t *seg_count) {
     const SourceDev &S = net->srcs[s];
@@ -261,9 +285,9 @@ __device__ __forceinline__ void source_step(const NetDev *__restrict__ net, uint
     bool spike = false;
     if (threadIdx.x < (uint32_t)kBlock && c < S.n_ch && phase >= S.lo && phase <= S.hi) {
         const uint32_t pattern = (uint32_t)((t / S.steps_per_pattern) % S.n_patterns);
-        spike = bernoulli_hash(S.seed, t, S.hash_offset + c) < S.thr[(size_t)pattern * S.n_ch + c];
+        spike = bernoulli_hash(S.seed, t, S.hash_offset + c) < gp(S.thr)[(size_t)pattern * S.n_ch + c];
     }
-    block_segment_write<NT>(net, t, spike, S.first + c, S.list + (size_t)(t & 1) * net->list_pstride + (size_t)local_block * kBlock,
+    block_segment_write<NT, COHERENT>(net, t, spike, S.first + c, S.list + (size_t)(t & 1) * net->list_pstride + (size_t)local_block * kBlock,
                             S.seg_base + local_block);
 }
 
@@ -469,99 +493,108 @@ struct TileProj {
     uint32_t stride, n_pre;
 };
 
-// Dynamic LDS of k_tile_scatter: the ring sums [tile_max_d][256] (int64), then for the tile's segments their
-// spike lists [tile_max_list] (pointers), row-count prefix [tile_max_list + 1] and projection [tile_max_list].
+// Dynamic LDS of the tile kernels: the ring sums [tile_max_d][256] (int64), then for the tile's segments
+// their spike lists [tile_max_list] (pointers, parity 0), count positions [tile_max_list], row-count prefix
+// [tile_max_list + 1] and projection [tile_max_list].
 __host__ __device__ inline size_t tile_lds_bytes(uint32_t max_d, uint32_t max_list) {
-    return (size_t)max_d * kBlock * 8 + (size_t)max_list * 8 + ((size_t)max_list + 1) * 4 + max_list;
+    return (size_t)max_d * kBlock * 8 + (size_t)max_list * 8 + (size_t)max_list * 4 + ((size_t)max_list + 1) * 4 +
+           max_list;
 }
 
-// PHASE1: the same launch also runs phase 1 of step t + 1 for the tile's neurons (their ring slot for t + 1
-// is complete once this step's impacts are summed) and, in the workgroups after the tiles, the Bernoulli
-// sources of step t + 1; the host uses it when every population is tiled, saving one launch per step.
-template <bool PHASE1>
-__global__ __launch_bounds__(kTileThreads) void k_tile_scatter(const NetDev *__restrict__ net, uint64_t t) {
-    extern __shared__ unsigned long long tacc[];
-    __shared__ uint32_t rbeg[kTileThreads];  // CSR position of each row's part in the tile
-    __shared__ uint32_t rpfx[kTileThreads];  // exclusive prefix of the parts' lengths
-    __shared__ uint8_t rproj[kTileThreads];  // projection of each row
-    __shared__ uint32_t wsum[kTileThreads / 64];
-    __shared__ unsigned long long cnt[3];
-    __shared__ TileProj tp[kMaxProjs];
-    if (blockIdx.x >= net->n_tiles) {  // Bernoulli sources of step t + 1 (PHASE1 grids only)
-        uint32_t b = blockIdx.x - net->n_tiles;
-        for (uint32_t s = 0; s < net->n_srcs; ++s) {
-            const uint32_t nb = max(1u, (net->srcs[s].n_ch + kBlock - 1) / kBlock);
-            if (b < nb) {
-                source_step<kTileThreads>(net, s, b, t + 1, nullptr);
-                break;
-            }
-            b -= nb;
-        }
-        return;
+// State of one tile workgroup: its tile, its population's fields (in registers: stores through the ring
+// would otherwise force reloads of the table) and its LDS tables.
+struct TileCtx {
+    TileDev td;
+    uint32_t D, n_pop, tile_lo, kind;
+    int64_t *acc_v;
+    float scale;
+    unsigned long long *ring;
+    const uint32_t **vlist;
+    uint32_t *vpos, *vpfx;
+    uint8_t *vproj;
+    uint32_t nv;
+};
+
+// Shared-memory scratch of the tile kernels.
+struct TileShared {
+    uint32_t rbeg[kTileThreads];  // CSR position of each row's part in the tile
+    uint32_t rpfx[kTileThreads];  // exclusive prefix of the parts' lengths
+    uint8_t rproj[kTileThreads];  // projection of each row
+    uint32_t wsum[kTileThreads / 64];
+    unsigned long long cnt[4];
+    TileProj tp[kMaxProjs];
+};
+
+// Loads the tile's description, the projection pointers and the list of segments that can reach the tile.
+__device__ __forceinline__ TileCtx tile_setup(const NetDev *__restrict__ net, unsigned long long *dyn,
+                                              TileShared &sh) {
+    TileCtx c;
+    {
+        const KNP_GLOBAL TileDev *tdp = gp(net->tiles) + blockIdx.x;
+        c.td.pop = tdp->pop;
+        c.td.tile = tdp->tile;
+        c.td.vbeg = tdp->vbeg;
+        c.td.vend = tdp->vend;
     }
-    if (blockIdx.x == 0) {
-        // channel segments of step t + 1 start empty (the host writes the counts of pushed messages later)
-        uint32_t *nc = net->seg_count + (size_t)((t + 1) & 1) * net->n_seg;
-        uint32_t *nvc = net->vcount + (size_t)((t + 1) & 1) * net->n_vseg;
-        for (uint32_t k = threadIdx.x; k < net->n_projs; k += kTileThreads) {
-            const uint32_t ps = net->projs[k].push_seg;
-            nc[ps] = 0;
-            for (uint32_t j = 0; j < 4; ++j) {
-                const uint32_t vp = net->seg_vpos[4 * ps + j];
-                if (vp != 0xFFFFFFFFu) nvc[vp] = 0;
-            }
-        }
-    }
-    const TileDev td = net->tiles[blockIdx.x];
-    const PopDev &P = net->pops[td.pop];
-    // population fields in registers: stores through the ring would otherwise force reloads of the table
-    const uint32_t D = P.D, n_pop = P.n;
-    int64_t *const acc_v = P.acc_v;
-    const float scale = P.scale;
-    const uint32_t tid = threadIdx.x;
+    const PopDev &P = net->pops[c.td.pop];
+    c.D = P.D;
+    c.n_pop = P.n;
+    c.kind = P.kind;
+    c.acc_v = P.acc_v;
+    c.scale = P.scale;
+    c.tile_lo = c.td.tile * kBlock;
     const uint32_t L = net->tile_max_list;
-    unsigned long long *ring = tacc;
-    const uint32_t **vlist = reinterpret_cast<const uint32_t **>(tacc + (size_t)net->tile_max_d * kBlock);
-    uint32_t *vpfx = reinterpret_cast<uint32_t *>(vlist + L);
-    uint8_t *vproj = reinterpret_cast<uint8_t *>(vpfx + L + 1);
-    for (uint32_t k = tid; k < D * kBlock; k += kTileThreads) ring[k] = 0;
-    if (tid < 3) cnt[tid] = 0;
+    c.ring = dyn;
+    c.vlist = reinterpret_cast<const uint32_t **>(dyn + (size_t)net->tile_max_d * kBlock);
+    c.vpos = reinterpret_cast<uint32_t *>(c.vlist + L);
+    c.vpfx = c.vpos + L;
+    c.vproj = reinterpret_cast<uint8_t *>(c.vpfx + L + 1);
+    c.nv = c.td.vend - c.td.vbeg;
+    const uint32_t tid = threadIdx.x;
+    if (tid < 4) sh.cnt[tid] = 0;
     if (tid < net->n_projs) {
         const ProjDev &Q = net->projs[tid];
-        tp[tid] = TileProj{Q.syn8, Q.tile_dir, Q.row_meta, Q.row_dir, Q.row_ptr, Q.tile_stride, Q.n_pre};
+        sh.tp[tid] = TileProj{Q.syn8, Q.tile_dir, Q.row_meta, Q.row_dir, Q.row_ptr, Q.tile_stride, Q.n_pre};
     }
-    // 1. the spiking rows that may reach the tile, numbered through a prefix over its segments' counts
-    const uint32_t nv = td.vend - td.vbeg;
-    const TileSeg *ts = net->tile_seg + td.vbeg;
+    const TileSeg *ts = net->tile_seg + c.td.vbeg;
+    for (uint32_t v = tid; v < c.nv; v += kTileThreads) {
+        const KNP_GLOBAL TileSeg *e = gp(ts) + v;
+        c.vlist[v] = e->list;
+        c.vpos[v] = e->vpos;
+        c.vproj[v] = (uint8_t)e->proj;
+    }
+    return c;
+}
+
+// Stages 1-3 of the tile scatter of step t: numbers the spiking rows that may reach the tile (prefix over
+// the counts of its segments), finds every row's part in the tile, and adds the part's impacts into the LDS
+// ring with LDS atomics. Ends with a workgroup barrier. Returns impacts (ev) and row parts (parts) added.
+template <bool COHERENT>
+__device__ __forceinline__ void tile_gather(const NetDev *__restrict__ net, const TileCtx &c, TileShared &sh,
+                                            uint64_t t, uint32_t &my_ev, uint32_t &my_parts) {
+    const uint32_t tid = threadIdx.x;
     const uint32_t *vc = net->vcount + (size_t)(t & 1) * net->n_vseg;
+    const size_t par_off = (size_t)(t & 1) * net->list_pstride;
     uint32_t total = 0;
-    for (uint32_t base = 0; base < nv; base += kTileThreads) {
+    for (uint32_t base = 0; base < c.nv; base += kTileThreads) {
         const uint32_t v = base + tid;
-        uint32_t c = 0;
-        if (v < nv) {
-            const TileSeg e = ts[v];
-            vlist[v] = e.list + (size_t)(t & 1) * net->list_pstride;
-            vproj[v] = (uint8_t)e.proj;
-            c = vc[e.vpos];
-        }
+        const uint32_t cv = v < c.nv ? ld_u32<COHERENT>(&vc[c.vpos[v]]) : 0u;
         uint32_t tot;
-        const uint32_t ex = block_excl_scan<kTileThreads>(c, wsum, &tot);
-        if (v < nv) vpfx[v] = total + ex;
+        const uint32_t ex = block_excl_scan<kTileThreads>(cv, sh.wsum, &tot);
+        if (v < c.nv) c.vpfx[v] = total + ex;
         total += tot;
     }
     __syncthreads();
-    const uint32_t tile_lo = td.tile * kBlock;
-    const uint32_t slot0 = (uint32_t)(t % D);
-    uint32_t my_ev = 0, my_parts = 0;
+    const uint32_t slot0 = (uint32_t)(t % c.D);
     // exact sum of one packed synapse record into the LDS ring (false: a neuron of another tile)
     auto add_record = [&](uint64_t rec) -> bool {
-        const uint32_t j = (uint32_t)(rec >> 41) - tile_lo;
+        const uint32_t j = (uint32_t)(rec >> 41) - c.tile_lo;
         if (j >= (uint32_t)kBlock) return false;  // a short row read whole: neuron of another tile
         uint32_t slot = slot0 + ((uint32_t)(rec >> 32) & 0xFFu);
-        if (slot >= D) slot -= D;
-        long long v = __float2ll_rn(__uint_as_float((uint32_t)rec) * scale);
+        if (slot >= c.D) slot -= c.D;
+        long long v = __float2ll_rn(__uint_as_float((uint32_t)rec) * c.scale);
         if ((rec >> 40) & 1u) v = -v;  // INHIBITORY_CURRENT, the only other type a tiled population receives
-        atomicAdd(&ring[slot * kBlock + j], (unsigned long long)v);
+        atomicAdd(&c.ring[slot * kBlock + j], (unsigned long long)v);
         return true;
     };
     for (uint32_t rb = 0; rb < total; rb += kTileThreads) {
@@ -569,31 +602,32 @@ __global__ __launch_bounds__(kTileThreads) void k_tile_scatter(const NetDev *__r
         const uint32_t row = rb + tid;
         uint32_t b = 0, len = 0, q = 0;
         if (row < total) {
-            uint32_t lo = 0, hi = nv;  // largest v with vpfx[v] <= row
+            uint32_t lo = 0, hi = c.nv;  // largest v with vpfx[v] <= row
             while (hi - lo > 1) {
                 const uint32_t mid = (lo + hi) >> 1;
-                if (vpfx[mid] <= row) lo = mid;
+                if (c.vpfx[mid] <= row) lo = mid;
                 else hi = mid;
             }
-            q = vproj[lo];
-            const uint32_t pre = vlist[lo][row - vpfx[lo]];
-            const TileProj &Q = tp[q];
+            q = c.vproj[lo];
+            const uint32_t pre = ld_u32<COHERENT>(c.vlist[lo] + par_off + (row - c.vpfx[lo]));
+            const TileProj &Q = sh.tp[q];
             if (pre < Q.n_pre) {
                 if (Q.tile_dir) {
-                    const uint32_t *d = Q.tile_dir + (size_t)pre * Q.stride + td.tile;
+                    const KNP_GLOBAL uint32_t *d = gp(Q.tile_dir) + (size_t)pre * Q.stride + c.td.tile;
                     const uint32_t d0 = d[0], d1 = d[1];
                     b = d0;
                     len = d1 - d0;
                 } else if (Q.row_meta) {
-                    const RowMeta m = Q.row_meta[pre];
-                    const uint32_t k = td.tile - m.first_tile;  // wraps above n_tiles for tiles before the row
-                    if (k < m.n_tiles) {
-                        const uint32_t o0 = Q.row_dir[m.dir + k], o1 = Q.row_dir[m.dir + k + 1];
-                        b = m.begin + o0;
+                    const KNP_GLOBAL RowMeta *m = gp(Q.row_meta) + pre;
+                    const uint32_t m_begin = m->begin, m_first = m->first_tile, m_n = m->n_tiles, m_dir = m->dir;
+                    const uint32_t k = c.td.tile - m_first;  // wraps above n_tiles for tiles before the row
+                    if (k < m_n) {
+                        const uint32_t o0 = gp(Q.row_dir)[m_dir + k], o1 = gp(Q.row_dir)[m_dir + k + 1];
+                        b = m_begin + o0;
                         len = o1 - o0;
                     }
                 } else {
-                    const uint64_t r0 = Q.row_ptr[pre], r1 = Q.row_ptr[pre + 1];
+                    const uint64_t r0 = gp(Q.row_ptr)[pre], r1 = gp(Q.row_ptr)[pre + 1];
                     b = (uint32_t)r0;
                     len = (uint32_t)(r1 - r0);
                 }
@@ -602,7 +636,7 @@ __global__ __launch_bounds__(kTileThreads) void k_tile_scatter(const NetDev *__r
         my_parts += len ? 1u : 0u;
         // 3a. short parts (the common case): the row's thread loads its records, 8 at a time, then adds them
         if (len <= kTileShortPart) {
-            const uint64_t *src = tp[q].syn8 + b;
+            const KNP_GLOBAL uint64_t *src = gp(sh.tp[q].syn8) + b;
             for (uint32_t k0 = 0; k0 < len; k0 += 8) {
                 uint64_t rec[8];
 #pragma unroll
@@ -617,96 +651,252 @@ __global__ __launch_bounds__(kTileThreads) void k_tile_scatter(const NetDev *__r
         const uint32_t llen = len > kTileShortPart ? len : 0u;
         if (__syncthreads_or(llen != 0)) {
             uint32_t n_syn;
-            const uint32_t ex = block_excl_scan<kTileThreads>(llen, wsum, &n_syn);
-            rbeg[tid] = b;
-            rpfx[tid] = ex;
-            rproj[tid] = (uint8_t)q;
+            const uint32_t ex = block_excl_scan<kTileThreads>(llen, sh.wsum, &n_syn);
+            sh.rbeg[tid] = b;
+            sh.rpfx[tid] = ex;
+            sh.rproj[tid] = (uint8_t)q;
             __syncthreads();
             const uint32_t nrows = min((uint32_t)kTileThreads, total - rb);
             for (uint32_t s = tid; s < n_syn; s += kTileThreads) {
                 uint32_t lo = 0, hi = nrows;  // largest r with rpfx[r] <= s: a row whose part holds s
                 while (hi - lo > 1) {
                     const uint32_t mid = (lo + hi) >> 1;
-                    if (rpfx[mid] <= s) lo = mid;
+                    if (sh.rpfx[mid] <= s) lo = mid;
                     else hi = mid;
                 }
-                if (add_record(tp[rproj[lo]].syn8[rbeg[lo] + (s - rpfx[lo])])) ++my_ev;
+                if (add_record(gp(sh.tp[sh.rproj[lo]].syn8)[sh.rbeg[lo] + (s - sh.rpfx[lo])])) ++my_ev;
             }
         }
         __syncthreads();
     }
-    // 4. add the tile's sums into the ring; no other thread of the device writes these cells in this kernel.
-    // All loads first, then all stores, so the round trips overlap.
+}
+
+// Channel segments of step t + 1 start empty (the host writes the counts of pushed messages before that
+// step); run by one workgroup.
+template <bool COHERENT>
+__device__ __forceinline__ void clear_push_counts(const NetDev *__restrict__ net, uint64_t t1) {
+    uint32_t *nc = net->seg_count + (size_t)(t1 & 1) * net->n_seg;
+    uint32_t *nvc = net->vcount + (size_t)(t1 & 1) * net->n_vseg;
+    for (uint32_t k = threadIdx.x; k < net->n_projs; k += kTileThreads) {
+        const uint32_t ps = net->projs[k].push_seg;
+        st_u32<COHERENT>(&nc[ps], 0u);
+        for (uint32_t j = 0; j < 4; ++j) {
+            const uint32_t vp = gp(net->seg_vpos)[4 * ps + j];
+            if (vp != 0xFFFFFFFFu) st_u32<COHERENT>(&nvc[vp], 0u);
+        }
+    }
+}
+
+// Phase 1 of step t1 for the tile's neurons (a = exact sum of the impacts arriving at t1 for this thread's
+// neuron) and the Bernoulli sources of t1 whose 256-channel blocks fall to this workgroup (block b is run by
+// tile b mod n_tiles).
+template <bool COHERENT>
+__device__ __forceinline__ void tile_phase1(const NetDev *__restrict__ net, const TileCtx &c, uint64_t t1, int64_t a) {
+    Ctl *ctl = &net->ctl[t1 & 1];
+    uint32_t *seg_count = net->seg_count + (size_t)(t1 & 1) * net->n_seg;
+    if (c.kind == POP_LEAN)
+        population_step<POP_LEAN, kTileThreads, true, COHERENT>(net, c.td.pop, c.td.tile, t1, ctl, seg_count, a);
+    else
+        population_step<POP_FULL, kTileThreads, true, COHERENT>(net, c.td.pop, c.td.tile, t1, ctl, seg_count, a);
+    uint32_t b0 = 0;
+    for (uint32_t s = 0; s < net->n_srcs; ++s) {
+        const uint32_t nb = max(1u, (net->srcs[s].n_ch + kBlock - 1) / kBlock);
+        for (uint32_t b = blockIdx.x; b < b0 + nb; b += net->n_tiles)
+            if (b >= b0) source_step<kTileThreads, COHERENT>(net, s, b - b0, t1, nullptr);
+        b0 += nb;  // the next source's blocks continue the round robin
+    }
+}
+
+// Counter totals of the workgroup: impacts, row parts, ring cells (block 0 also adds the rows of the step).
+__device__ __forceinline__ void tile_count(TileShared &sh, uint32_t ev, uint32_t parts, uint32_t cells) {
+    ev = wave_sum_u32(ev);
+    parts = wave_sum_u32(parts);
+    cells = wave_sum_u32(cells);
+    if (lane_id() == 0) {
+        atomicAdd(&sh.cnt[0], (unsigned long long)ev);
+        atomicAdd(&sh.cnt[1], (unsigned long long)parts);
+        atomicAdd(&sh.cnt[2], (unsigned long long)cells);
+    }
+}
+
+// Rows expanded at step t by the tiled path (sum of the counts of every tiled segment), added by block 0.
+template <bool COHERENT>
+__device__ __forceinline__ void tile_count_rows(const NetDev *__restrict__ net, TileShared &sh, uint64_t t) {
+    const uint32_t *vc = net->vcount + (size_t)(t & 1) * net->n_vseg;
+    uint32_t r = 0;
+    for (uint32_t v = net->n_vseg_atomic + threadIdx.x; v < net->n_vseg; v += kTileThreads) r += ld_u32<COHERENT>(&vc[v]);
+    r = wave_sum_u32(r);
+    if (lane_id() == 0) atomicAdd(&sh.cnt[3], (unsigned long long)r);
+}
+
+__device__ __forceinline__ void tile_flush_counts(const NetDev *__restrict__ net, TileShared &sh) {
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long *tpart = net->tile_part + 4 * (size_t)blockIdx.x;
+        tpart[0] += sh.cnt[0];
+        tpart[1] += sh.cnt[3];
+        tpart[2] += sh.cnt[1];
+        tpart[3] += sh.cnt[2];
+    }
+}
+
+// One step's tile scatter (the ring in HBM, this step's sums in LDS). PHASE1: the same launch also runs
+// phase 1 of step t + 1 for the tile's neurons (their ring slot for t + 1 is complete once this step's
+// impacts are summed) and the sources of t + 1; the host uses it when every population is tiled, saving
+// one launch per step.
+template <bool PHASE1>
+__global__ __launch_bounds__(kTileThreads) void k_tile_scatter(const NetDev *__restrict__ net, uint64_t t) {
+    extern __shared__ unsigned long long tacc[];
+    __shared__ TileShared sh;
+    if (blockIdx.x == 0) clear_push_counts<false>(net, t + 1);
+    const TileCtx c = tile_setup(net, tacc, sh);
+    const uint32_t tid = threadIdx.x;
+    const uint32_t s1 = PHASE1 ? (uint32_t)((t + 1) % c.D) : 0xFFFFFFFFu;  // slot consumed by phase 1 of t + 1
+    // The ring cells of the tile are written by no one else during this kernel: their values are loaded now,
+    // overlapping the rest, so that adding this step's sums at the end needs no further round trip.
     constexpr uint32_t kCells = kTileMaxD * kBlock / kTileThreads;
-    unsigned long long x[kCells];
     int64_t old[kCells];
-    int64_t *cell[kCells];
-    uint32_t my_cells = 0;
-    const uint32_t s1 = PHASE1 ? (uint32_t)((t + 1) % D) : 0xFFFFFFFFu;  // consumed below, not written back
 #pragma unroll
     for (uint32_t u = 0; u < kCells; ++u) {
-        const uint32_t c = tid + u * kTileThreads;
-        const uint32_t i = tile_lo + (c % kBlock);
-        x[u] = 0;
-        cell[u] = nullptr;
-        if (c < D * kBlock && i < n_pop && c / kBlock != s1) {
-            x[u] = ring[c];
-            cell[u] = acc_v + (size_t)(c / kBlock) * n_pop + i;
-        }
+        const uint32_t cc = tid + u * kTileThreads;
+        const uint32_t i = c.tile_lo + (cc % kBlock);
+        old[u] = 0;
+        if (cc < c.D * kBlock && i < c.n_pop && cc / kBlock != s1) old[u] = gp(c.acc_v)[(size_t)(cc / kBlock) * c.n_pop + i];
     }
-#pragma unroll
-    for (uint32_t u = 0; u < kCells; ++u) old[u] = x[u] ? *cell[u] : 0;
-#pragma unroll
-    for (uint32_t u = 0; u < kCells; ++u)
-        if (x[u]) {
-            *cell[u] = old[u] + (long long)x[u];
-            ++my_cells;
-        }
-    my_ev = wave_sum_u32(my_ev);
-    my_parts = wave_sum_u32(my_parts);
-    my_cells = wave_sum_u32(my_cells);
-    if (lane_id() == 0) {
-        atomicAdd(&cnt[0], (unsigned long long)my_ev);
-        atomicAdd(&cnt[1], (unsigned long long)my_parts);
-        atomicAdd(&cnt[2], (unsigned long long)my_cells);
-    }
+    int64_t a1 = 0;  // earlier steps' sum in slot s1 for this thread's neuron (phase 1 of t + 1)
+    if (PHASE1 && tid < (uint32_t)kBlock && c.tile_lo + tid < c.n_pop)
+        a1 = gp(c.acc_v)[(size_t)s1 * c.n_pop + c.tile_lo + tid];
+    for (uint32_t k = tid; k < c.D * kBlock; k += kTileThreads) c.ring[k] = 0;
     __syncthreads();
-    if (tid == 0) {
-        unsigned long long *tpart = net->tile_part + 4 * (size_t)blockIdx.x;
-        tpart[0] += cnt[0];
-        tpart[2] += cnt[1];
-        tpart[3] += cnt[2];
-    }
-    // rows expanded this step by the tiled path: block 0 sums the counts of every tiled segment
-    if (blockIdx.x == 0) {
-        uint32_t r = 0;
-        for (uint32_t v = net->n_vseg_atomic + tid; v < net->n_vseg; v += kTileThreads) r += vc[v];
-        r = wave_sum_u32(r);
-        if (lane_id() == 0) wsum[tid / 64] = r;
-        __syncthreads();
-        if (tid == 0) {
-            unsigned long long rows = 0;
-            for (uint32_t k = 0; k < kTileThreads / 64; ++k) rows += wsum[k];
-            net->tile_part[1] += rows;
+    uint32_t my_ev = 0, my_parts = 0, my_cells = 0;
+    tile_gather<false>(net, c, sh, t, my_ev, my_parts);
+    // 4. add the tile's sums into the ring (old values loaded at the start)
+#pragma unroll
+    for (uint32_t u = 0; u < kCells; ++u) {
+        const uint32_t cc = tid + u * kTileThreads;
+        const uint32_t i = c.tile_lo + (cc % kBlock);
+        if (cc < c.D * kBlock && i < c.n_pop && cc / kBlock != s1) {
+            const unsigned long long x = c.ring[cc];
+            if (x != 0) {
+                gp(c.acc_v)[(size_t)(cc / kBlock) * c.n_pop + i] = old[u] + (long long)x;
+                ++my_cells;
+            }
         }
     }
+    tile_count(sh, my_ev, my_parts, my_cells);
+    if (blockIdx.x == 0) tile_count_rows<false>(net, sh, t);
+    tile_flush_counts(net, sh);
     if (PHASE1) {
         // 5. phase 1 of step t + 1: slot s1 = earlier steps' sums in HBM + this step's sums in LDS
-        const uint64_t t1 = t + 1;
-        const uint32_t i = tile_lo + tid;
+        const uint32_t i = c.tile_lo + tid;
         int64_t a = 0;
-        if (tid < (uint32_t)kBlock && i < n_pop) {
-            int64_t *c1 = acc_v + (size_t)s1 * n_pop + i;
-            a = *c1 + (int64_t)ring[s1 * kBlock + tid];
-            *c1 = 0;
+        if (tid < (uint32_t)kBlock && i < c.n_pop) {
+            a = a1 + (int64_t)c.ring[s1 * kBlock + tid];
+            gp(c.acc_v)[(size_t)s1 * c.n_pop + i] = 0;
         }
-        Ctl *ctl = &net->ctl[t1 & 1];
-        uint32_t *seg_count = net->seg_count + (size_t)(t1 & 1) * net->n_seg;
-        if (P.kind == POP_LEAN)
-            population_step<POP_LEAN, kTileThreads, true>(net, td.pop, td.tile, t1, ctl, seg_count, a);
-        else
-            population_step<POP_FULL, kTileThreads, true>(net, td.pop, td.tile, t1, ctl, seg_count, a);
+        tile_phase1<false>(net, c, t + 1, a);
     }
+}
+
+// Grid-wide barrier of a persistent launch. Workgroups arrive on one of kBarGroups group counters (block
+// index mod kBarGroups, each counter on its own 128-byte line, so arrivals do not queue on one address);
+// the last arrival of a group counts on the top counter, and the last group publishes the barrier's number
+// in the generation word that every workgroup polls. Counters only grow within a launch (the host clears
+// them before each launch). Before arriving, every thread waits for its stores to complete, so data written
+// (through to memory) before the barrier is visible to device-scope loads after it. The wait is bounded:
+// after kSpinLimit polls, or when another workgroup has given up, the workgroup returns false (the launch
+// then stops and the host reports the failure), so no wait lasts forever.
+constexpr uint32_t kSpinLimit = 1u << 20;
+constexpr uint32_t kBarGroups = 8;
+// word offsets in grid_bar: generation, abort flag, group counters, top counter
+constexpr uint32_t kBarGen = 0, kBarAbort = 1, kBarGroup0 = 32, kBarTop = kBarGroup0 + 32 * kBarGroups;
+constexpr uint32_t kBarWords = kBarTop + 32;
+__device__ __forceinline__ bool grid_barrier(uint32_t *bar, uint32_t n_blocks, uint32_t j) {
+    __shared__ uint32_t ok;
+    __builtin_amdgcn_s_waitcnt(0);  // this wave's stores are complete
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        KNP_GLOBAL uint32_t *gb = gp(bar);
+        const uint32_t g = blockIdx.x % kBarGroups;
+        const uint32_t m = n_blocks / kBarGroups + (g < n_blocks % kBarGroups ? 1u : 0u);
+        const uint32_t groups = n_blocks < kBarGroups ? n_blocks : kBarGroups;
+        const uint32_t old = __hip_atomic_fetch_add(gb + kBarGroup0 + 32 * g, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (old + 1 == m * j) {
+            const uint32_t old2 = __hip_atomic_fetch_add(gb + kBarTop, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if (old2 + 1 == groups * j) __hip_atomic_store(gb + kBarGen, j, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+        uint32_t spins = 0;
+        bool good = true;
+        while (__hip_atomic_load(gb + kBarGen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < j) {
+            __builtin_amdgcn_s_sleep(1);
+            if ((++spins & 63u) == 0 &&
+                (spins > kSpinLimit || __hip_atomic_load(gb + kBarAbort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0)) {
+                good = false;
+                __hip_atomic_store(gb + kBarAbort, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                break;
+            }
+        }
+        ok = good ? 1u : 0u;
+    }
+    __syncthreads();
+    return ok != 0;
+}
+
+// Persistent run of steps t0 .. t0 + n - 1 when every population is tiled and all tiles fit on the device at
+// once (cooperative launch): the ring lives in LDS for the whole launch (loaded from HBM at the start,
+// stored back at the end), and one grid barrier per step separates phase 1 of step t (spike lists and
+// counts, written through to memory) from the scatter of step t that reads them.
+__global__ __launch_bounds__(kTileThreads) void k_tile_run(const NetDev *__restrict__ net, uint64_t t0, uint32_t n) {
+    extern __shared__ unsigned long long tacc[];
+    __shared__ TileShared sh;
+    const TileCtx c = tile_setup(net, tacc, sh);
+    const uint32_t tid = threadIdx.x;
+    uint32_t *bar = net->grid_bar;
+    // the ring of the tile from HBM
+    for (uint32_t cc = tid; cc < c.D * kBlock; cc += kTileThreads) {
+        const uint32_t i = c.tile_lo + (cc % kBlock);
+        c.ring[cc] = i < c.n_pop ? (unsigned long long)gp(c.acc_v)[(size_t)(cc / kBlock) * c.n_pop + i] : 0ull;
+    }
+    __syncthreads();
+    uint32_t my_ev = 0, my_parts = 0;
+    uint32_t barriers = 0;
+    bool alive = true;
+    // phase 1 of t0
+    {
+        const uint32_t s0 = (uint32_t)(t0 % c.D);
+        int64_t a = 0;
+        if (tid < (uint32_t)kBlock) {
+            a = (int64_t)c.ring[s0 * kBlock + tid];
+            c.ring[s0 * kBlock + tid] = 0;
+        }
+        tile_phase1<true>(net, c, t0, a);
+        alive = grid_barrier(bar, net->n_tiles, ++barriers);
+    }
+    for (uint32_t k = 0; k < n && alive; ++k) {
+        const uint64_t t = t0 + k;
+        if (blockIdx.x == 0) clear_push_counts<true>(net, t + 1);
+        tile_gather<true>(net, c, sh, t, my_ev, my_parts);
+        if (blockIdx.x == 0) tile_count_rows<true>(net, sh, t);
+        if (k + 1 < n) {
+            const uint32_t s1 = (uint32_t)((t + 1) % c.D);
+            int64_t a = 0;
+            if (tid < (uint32_t)kBlock) {
+                a = (int64_t)c.ring[s1 * kBlock + tid];
+                c.ring[s1 * kBlock + tid] = 0;
+            }
+            tile_phase1<true>(net, c, t + 1, a);
+            alive = grid_barrier(bar, net->n_tiles, ++barriers);
+        }
+    }
+    // the ring back to HBM
+    __syncthreads();
+    uint32_t my_cells = 0;
+    for (uint32_t cc = tid; cc < c.D * kBlock; cc += kTileThreads) {
+        const uint32_t i = c.tile_lo + (cc % kBlock);
+        if (i < c.n_pop) gp(c.acc_v)[(size_t)(cc / kBlock) * c.n_pop + i] = (int64_t)c.ring[cc];
+    }
+    tile_count(sh, my_ev, my_parts, my_cells);
+    tile_flush_counts(net, sh);
 }
 
 // ---- synaptic-resource STDP (blifat_stdp.h:35-172, training/stdp.h:40-186) ---------------------------

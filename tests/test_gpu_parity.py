@@ -336,16 +336,20 @@ def all_tiled_network(o, seed):
 
 
 def test_fused_steps_match_oracle():
+    """All populations tiled: persistent multi-step launches (auto), one fused launch per step (stepwise) and
+    the atomic path against the oracle, over calls of 1..40 steps with pushed spikes between calls."""
     say("case: fused tile steps (all populations tiled), calls of 1..40 steps with pushes between calls")
     chunks = [1, 7, 40, 2, 1, 25, 3, 31, 1, 9]
     steps = sum(chunks)
     runs = {}
-    for name in ("fused", "atomic", "oracle"):
+    modes = {"persistent": abi.SCATTER_AUTO, "stepwise": abi.SCATTER_STEPWISE, "atomic": abi.SCATTER_ATOMIC}
+    for name in ("persistent", "stepwise", "atomic", "oracle"):
         o = Oracle("exact") if name == "oracle" else MI355XBackend(0)
-        if name == "atomic":
-            o.set_scatter_mode(abi.SCATTER_ATOMIC)
+        if name != "oracle":
+            o.set_scatter_mode(modes[name])
+            o.enable_timing(True)
         pops, push = all_tiled_network(o, 8)
-        if name == "fused":
+        if name in ("persistent", "stepwise"):
             assert all(o.tiled(p) for p in pops)
         for p in pops:
             o.record_spikes(p, steps)
@@ -357,16 +361,21 @@ def test_fused_steps_match_oracle():
             o.step(c)
             mids.append((o.potentials(pops[0]).copy(), o.last_spikes(pops[1]).copy()))
         runs[name] = ([o.read_spike_record(p) for p in pops], mids, o.events)
+        if name == "persistent":
+            assert o.kernel_time(abi.KERNEL_TILE_RUN)[1] > 0  # the persistent launch ran
+        if name == "stepwise":
+            assert o.kernel_time(abi.KERNEL_TILE_RUN)[1] == 0
+    devs = ("persistent", "stepwise", "atomic")
     for k in range(2):
         so = runs["oracle"][0][k]
         assert so.sum() > 100, k
-        for name in ("fused", "atomic"):
+        for name in devs:
             assert np.array_equal(runs[name][0][k], so), (name, k)
     for j in range(len(chunks)):
-        for name in ("fused", "atomic"):
+        for name in devs:
             assert rel_err(runs[name][1][j][0], runs["oracle"][1][j][0]) <= POT_RTOL, (name, j)
             assert np.array_equal(runs[name][1][j][1], runs["oracle"][1][j][1]), (name, j)
-    assert runs["fused"][2] == runs["atomic"][2] == runs["oracle"][2]
+    assert runs["persistent"][2] == runs["stepwise"][2] == runs["atomic"][2] == runs["oracle"][2]
 
 
 def agreement(a, b):
