@@ -92,9 +92,12 @@ __device__ __forceinline__ uint32_t ld_u32(const uint32_t *p) {
 
 // Ordered compaction of a workgroup's 256 candidates into its own segment: ascending order, no global
 // atomics, one count store. NT = threads of the workgroup (threads from 256 on carry no candidate).
+// Returns (in thread 0) the rows published: the segment's count times the projections that read it.
+// vp_pre: the segment's 4 virtual positions if the caller holds them (else they are loaded).
 template <uint32_t NT = kBlock, bool COHERENT = false>
-__device__ __forceinline__ void block_segment_write(const NetDev *__restrict__ net, uint64_t t, bool pred,
-                                                    uint32_t value, uint32_t *segment, uint32_t seg) {
+__device__ __forceinline__ uint32_t block_segment_write(const NetDev *__restrict__ net, uint64_t t, bool pred,
+                                                        uint32_t value, uint32_t *segment, uint32_t seg,
+                                                        const uint32_t *vp_pre = nullptr) {
     __shared__ uint32_t wcnt[NT / 64];
     const uint64_t mask = __ballot(pred);
     const uint32_t lane = lane_id(), w = threadIdx.x / 64;
@@ -106,13 +109,19 @@ __device__ __forceinline__ void block_segment_write(const NetDev *__restrict__ n
         total += wcnt[k];
     }
     if (pred) st_u32<COHERENT>(&segment[off + __popcll(mask & ((1ull << lane) - 1ull))], value);
+    uint32_t rows = 0;
     if (threadIdx.x < 4) {
         const size_t par = (size_t)(t & 1);
         if (threadIdx.x == 0) st_u32<COHERENT>(&net->seg_count[par * net->n_seg + seg], total);
-        const uint32_t vp = gp(net->seg_vpos)[4 * seg + threadIdx.x];
+        uint32_t vps[4];
+        for (uint32_t k = 0; k < 4; ++k) vps[k] = vp_pre ? vp_pre[k] : gp(net->seg_vpos)[4 * seg + k];
+        const uint32_t vp = vps[threadIdx.x & 3];
         if (vp != 0xFFFFFFFFu) st_u32<COHERENT>(&net->vcount[par * net->n_vseg + vp], total);
+        if (threadIdx.x == 0)
+            for (uint32_t k = 0; k < 4; ++k) rows += vps[k] != 0xFFFFFFFFu ? total : 0u;
     }
     __syncthreads();
+    return rows;
 }
 
 // Write the wave's 64-neuron spike mask into the bitmap words (and the record), bit i = neuron i.
@@ -142,8 +151,9 @@ __device__ __forceinline__ int64_t blocking_value(const NetDev *net, uint64_t ke
 // NT = threads of the workgroup (threads from 256 on have no neuron). FUSED: the exact sum of the step's
 // impacts is passed in (a_in) by the tile kernel instead of being read from the ring.
 template <uint32_t KIND, uint32_t NT = kBlock, bool FUSED = false, bool COHERENT = false>
-__device__ __forceinline__ void population_step(const NetDev *__restrict__ net, uint32_t p, uint32_t local_block,
-                                                uint64_t t, Ctl *ctl, uint32_t *seg_count, int64_t a_in = 0) {
+__device__ __forceinline__ uint32_t population_step(const NetDev *__restrict__ net, uint32_t p, uint32_t local_block,
+                                                    uint64_t t, Ctl *ctl, uint32_t *seg_count, int64_t a_in = 0,
+                                                    const uint32_t *vp_pre = nullptr) {
     const PopDev &P = net->pops[p];
     const uint32_t i = local_block * kBlock + threadIdx.x;
     const bool active = threadIdx.x < (uint32_t)kBlock && i < P.n;
@@ -269,26 +279,33 @@ __device__ __forceinline__ void population_step(const NetDev *__restrict__ net, 
         }
     }
     // spike compaction and bitmaps (whole wave participates)
-    block_segment_write<NT, COHERENT>(net, t, spike, i, P.spike_list + (size_t)(t & 1) * net->list_pstride + (size_t)local_block * kBlock,
-                            P.seg_base + local_block);
+    const uint32_t rows = block_segment_write<NT, COHERENT>(
+        net, t, spike, i, P.spike_list + (size_t)(t & 1) * net->list_pstride + (size_t)local_block * kBlock,
+        P.seg_base + local_block, vp_pre);
     const uint64_t mask = __ballot(spike);
     if (threadIdx.x < (uint32_t)kBlock) wave_bitmap(mask, local_block * kBlock + (threadIdx.x & ~63u), P, t);
     if (KIND == POP_RESOURCE) wave_append(cand, i, &ctl->cand_count[p], P.res.cand);
+    return rows;
 }
 
+// thr_pre: this thread's channel threshold when the caller holds it (single-pattern sources), else
+// UINT32_MAX and it is loaded; vp_pre as in block_segment_write.
 template <uint32_t NT = kBlock, bool COHERENT = false>
-__device__ __forceinline__ void source_step(const NetDev *__restrict__ net, uint32_t s, uint32_t local_block,
-                                            uint64_t t, uint32_t *seg_count) {
+__device__ __forceinline__ uint32_t source_step(const NetDev *__restrict__ net, uint32_t s, uint32_t local_block,
+                                                uint64_t t, uint32_t *seg_count, uint32_t thr_pre = 0xFFFFFFFFu,
+                                                const uint32_t *vp_pre = nullptr) {
     const SourceDev &S = net->srcs[s];
     const uint32_t c = local_block * kBlock + threadIdx.x;
     const uint32_t phase = (uint32_t)(t % S.steps_per_pattern);
     bool spike = false;
     if (threadIdx.x < (uint32_t)kBlock && c < S.n_ch && phase >= S.lo && phase <= S.hi) {
         const uint32_t pattern = (uint32_t)((t / S.steps_per_pattern) % S.n_patterns);
-        spike = bernoulli_hash(S.seed, t, S.hash_offset + c) < gp(S.thr)[(size_t)pattern * S.n_ch + c];
+        const uint32_t thr = thr_pre != 0xFFFFFFFFu ? thr_pre : gp(S.thr)[(size_t)pattern * S.n_ch + c];
+        spike = bernoulli_hash(S.seed, t, S.hash_offset + c) < thr;
     }
-    block_segment_write<NT, COHERENT>(net, t, spike, S.first + c, S.list + (size_t)(t & 1) * net->list_pstride + (size_t)local_block * kBlock,
-                            S.seg_base + local_block);
+    return block_segment_write<NT, COHERENT>(
+        net, t, spike, S.first + c, S.list + (size_t)(t & 1) * net->list_pstride + (size_t)local_block * kBlock,
+        S.seg_base + local_block, vp_pre);
 }
 
 __global__ __launch_bounds__(kBlock) void k_phase1(const NetDev *__restrict__ net, uint64_t t) {
@@ -691,20 +708,87 @@ __device__ __forceinline__ void clear_push_counts(const NetDev *__restrict__ net
 // neuron) and the Bernoulli sources of t1 whose 256-channel blocks fall to this workgroup (block b is run by
 // tile b mod n_tiles).
 template <bool COHERENT>
-__device__ __forceinline__ void tile_phase1(const NetDev *__restrict__ net, const TileCtx &c, uint64_t t1, int64_t a) {
+__device__ __forceinline__ uint32_t tile_phase1(const NetDev *__restrict__ net, const TileCtx &c, uint64_t t1,
+                                                int64_t a) {
     Ctl *ctl = &net->ctl[t1 & 1];
     uint32_t *seg_count = net->seg_count + (size_t)(t1 & 1) * net->n_seg;
+    uint32_t rows;
     if (c.kind == POP_LEAN)
-        population_step<POP_LEAN, kTileThreads, true, COHERENT>(net, c.td.pop, c.td.tile, t1, ctl, seg_count, a);
+        rows = population_step<POP_LEAN, kTileThreads, true, COHERENT>(net, c.td.pop, c.td.tile, t1, ctl, seg_count, a);
     else
-        population_step<POP_FULL, kTileThreads, true, COHERENT>(net, c.td.pop, c.td.tile, t1, ctl, seg_count, a);
+        rows = population_step<POP_FULL, kTileThreads, true, COHERENT>(net, c.td.pop, c.td.tile, t1, ctl, seg_count, a);
     uint32_t b0 = 0;
     for (uint32_t s = 0; s < net->n_srcs; ++s) {
         const uint32_t nb = max(1u, (net->srcs[s].n_ch + kBlock - 1) / kBlock);
         for (uint32_t b = blockIdx.x; b < b0 + nb; b += net->n_tiles)
-            if (b >= b0) source_step<kTileThreads, COHERENT>(net, s, b - b0, t1, nullptr);
+            if (b >= b0) rows += source_step<kTileThreads, COHERENT>(net, s, b - b0, t1, nullptr);
         b0 += nb;  // the next source's blocks continue the round robin
     }
+    return rows;  // rows published for step t1 (thread 0)
+}
+
+// What phase 1 of a persistent workgroup writes to, fixed for the launch: the virtual positions of its
+// population segment and of its source blocks (in LDS), and this thread's channel threshold for
+// single-pattern sources (in registers), so that a step's phase 1 loads nothing but its own state.
+constexpr uint32_t kCachedSrcBlocks = 4;
+struct PhaseCache {
+    uint32_t pop_vp[4];
+    uint32_t n_src;  // source blocks of this workgroup; cached only if <= kCachedSrcBlocks
+    uint32_t src[kCachedSrcBlocks], blk[kCachedSrcBlocks];
+    uint32_t vp[kCachedSrcBlocks][4];
+};
+
+__device__ __forceinline__ void phase_cache(const NetDev *__restrict__ net, const TileCtx &c, PhaseCache &pc,
+                                            uint32_t (&thr)[kCachedSrcBlocks]) {
+    const uint32_t pseg = net->pops[c.td.pop].seg_base + c.td.tile;
+    if (threadIdx.x < 4) pc.pop_vp[threadIdx.x] = gp(net->seg_vpos)[4 * pseg + threadIdx.x];
+    uint32_t n = 0, b0 = 0;
+#pragma unroll
+    for (uint32_t j = 0; j < kCachedSrcBlocks; ++j) thr[j] = 0xFFFFFFFFu;
+    for (uint32_t s = 0; s < net->n_srcs; ++s) {
+        const SourceDev &S = net->srcs[s];
+        const uint32_t nb = max(1u, (S.n_ch + kBlock - 1) / kBlock);
+        for (uint32_t b = blockIdx.x; b < b0 + nb; b += net->n_tiles) {
+            if (b < b0) continue;
+            const uint32_t j = n++;
+            if (j >= kCachedSrcBlocks) continue;
+            if (threadIdx.x == 0) {
+                pc.src[j] = s;
+                pc.blk[j] = b - b0;
+            }
+            if (threadIdx.x < 4) pc.vp[j][threadIdx.x] = gp(net->seg_vpos)[4 * (S.seg_base + b - b0) + threadIdx.x];
+            const uint32_t ch = (b - b0) * kBlock + threadIdx.x;
+            const uint32_t v = (S.n_patterns == 1 && threadIdx.x < (uint32_t)kBlock && ch < S.n_ch) ? gp(S.thr)[ch]
+                                                                                                  : 0xFFFFFFFFu;
+#pragma unroll
+            for (uint32_t k = 0; k < kCachedSrcBlocks; ++k)
+                if (k == j) thr[k] = v;
+        }
+        b0 += nb;
+    }
+    if (threadIdx.x == 0) pc.n_src = n;
+    __syncthreads();
+}
+
+// tile_phase1 with the launch's fixed targets at hand (falls back to it with many source blocks).
+__device__ __forceinline__ uint32_t tile_phase1_cached(const NetDev *__restrict__ net, const TileCtx &c,
+                                                       const PhaseCache &pc, const uint32_t (&thr)[kCachedSrcBlocks],
+                                                       uint64_t t1, int64_t a) {
+    const uint32_t n_src = pc.n_src;
+    if (n_src > kCachedSrcBlocks) return tile_phase1<true>(net, c, t1, a);
+    Ctl *ctl = &net->ctl[t1 & 1];
+    uint32_t *seg_count = net->seg_count + (size_t)(t1 & 1) * net->n_seg;
+    uint32_t rows;
+    if (c.kind == POP_LEAN)
+        rows = population_step<POP_LEAN, kTileThreads, true, true>(net, c.td.pop, c.td.tile, t1, ctl, seg_count, a,
+                                                                   pc.pop_vp);
+    else
+        rows = population_step<POP_FULL, kTileThreads, true, true>(net, c.td.pop, c.td.tile, t1, ctl, seg_count, a,
+                                                                   pc.pop_vp);
+#pragma unroll
+    for (uint32_t j = 0; j < kCachedSrcBlocks; ++j)
+        if (j < n_src) rows += source_step<kTileThreads, true>(net, pc.src[j], pc.blk[j], t1, nullptr, thr[j], pc.vp[j]);
+    return rows;
 }
 
 // Counter totals of the workgroup: impacts, row parts, ring cells (block 0 also adds the rows of the step).
@@ -858,9 +942,24 @@ __global__ __launch_bounds__(kTileThreads) void k_tile_run(const NetDev *__restr
         c.ring[cc] = i < c.n_pop ? (unsigned long long)gp(c.acc_v)[(size_t)(cc / kBlock) * c.n_pop + i] : 0ull;
     }
     __syncthreads();
+    __shared__ PhaseCache pc;
+    uint32_t thr[kCachedSrcBlocks];
+    phase_cache(net, c, pc, thr);
     uint32_t my_ev = 0, my_parts = 0;
+    unsigned long long my_rows = 0;  // thread 0: rows published by this workgroup's phase 1
     uint32_t barriers = 0;
     bool alive = true;
+    if (blockIdx.x == 0) {
+        // Channel messages exist only for t0 (pushed by the host before the launch): their rows are counted
+        // here, and the channel counts of t0 + 1 are cleared now, those of t0 (parity of t0 + 2) once the
+        // scatter of t0 is done everywhere (step t0 + 1 below).
+        if (threadIdx.x == 0)
+            for (uint32_t q = 0; q < net->n_projs; ++q) {
+                const uint32_t vp = gp(net->seg_vpos)[4 * net->projs[q].push_seg];
+                if (vp != 0xFFFFFFFFu) my_rows += gp(net->vcount)[(size_t)(t0 & 1) * net->n_vseg + vp];
+            }
+        clear_push_counts<true>(net, t0 + 1);
+    }
     // phase 1 of t0
     {
         const uint32_t s0 = (uint32_t)(t0 % c.D);
@@ -869,14 +968,13 @@ __global__ __launch_bounds__(kTileThreads) void k_tile_run(const NetDev *__restr
             a = (int64_t)c.ring[s0 * kBlock + tid];
             c.ring[s0 * kBlock + tid] = 0;
         }
-        tile_phase1<true>(net, c, t0, a);
+        my_rows += tile_phase1_cached(net, c, pc, thr, t0, a);
         alive = grid_barrier(bar, net->n_tiles, ++barriers);
     }
     for (uint32_t k = 0; k < n && alive; ++k) {
         const uint64_t t = t0 + k;
-        if (blockIdx.x == 0) clear_push_counts<true>(net, t + 1);
+        if (k == 1 && blockIdx.x == 0) clear_push_counts<true>(net, t0);
         tile_gather<true>(net, c, sh, t, my_ev, my_parts);
-        if (blockIdx.x == 0) tile_count_rows<true>(net, sh, t);
         if (k + 1 < n) {
             const uint32_t s1 = (uint32_t)((t + 1) % c.D);
             int64_t a = 0;
@@ -884,7 +982,7 @@ __global__ __launch_bounds__(kTileThreads) void k_tile_run(const NetDev *__restr
                 a = (int64_t)c.ring[s1 * kBlock + tid];
                 c.ring[s1 * kBlock + tid] = 0;
             }
-            tile_phase1<true>(net, c, t + 1, a);
+            my_rows += tile_phase1_cached(net, c, pc, thr, t + 1, a);
             alive = grid_barrier(bar, net->n_tiles, ++barriers);
         }
     }
@@ -896,6 +994,7 @@ __global__ __launch_bounds__(kTileThreads) void k_tile_run(const NetDev *__restr
         if (i < c.n_pop) gp(c.acc_v)[(size_t)(cc / kBlock) * c.n_pop + i] = (int64_t)c.ring[cc];
     }
     tile_count(sh, my_ev, my_parts, my_cells);
+    if (threadIdx.x == 0) atomicAdd(&sh.cnt[3], my_rows);
     tile_flush_counts(net, sh);
 }
 
