@@ -117,31 +117,29 @@ def main():
     rows_step = (r3 - r2) / tsteps
     parts_step = (p3 - p2) / tsteps
     cells_step = (c3 - c2) / tsteps
-    # Algorithmic HBM bytes per step of each kernel class:
-    #   impacts: 8 B each on the tiled path (one packed record: weight, delay, type, post), 9 B on the atomic
-    #            path (post|type 4, weight 4, delay 1)
-    #   tiled path: 8 B per row part (two tile-directory entries), 4 B per spiking row (spike-list entry),
-    #            16 B per ring cell updated in HBM (read 8 + write 8; none in the persistent launch, whose ring
-    #            stays in LDS)
-    #   atomic path: 20 B per expanded row (row_ptr pair 16, index 4) + 8 B per impact atomic
-    #   population step (default BLIFAT): 4 B potential write + 1/8 B bitmap per neuron, plus 16 B per neuron
-    #            for the ring slot (read 8 + clear 8) when the ring is in HBM
-    lean_bytes = n_neurons * (4.0 + 1.0 / 8.0)
-    ring_slot_bytes = n_neurons * 16.0
-    if "tile_run" in kern:
-        dominant, kname = "tile_run", "k_tile_run"
-        step_bytes = ev_step * 8.0 + parts_step * 8.0 + rows_step * 4.0 + cells_step * 16.0 + lean_bytes
+    # Algorithmic HBM bytes per step (the per-unit figures of the measurement plan, identical for every path):
+    #   spike -> impact scatter: 9 B per impact (post index 4, weight 4, delay/type 1) + 8 B per expanded
+    #            presynaptic row (row pointer pair); the ring accumulators are on-chip state and not counted
+    #   population update (default BLIFAT, FP32 SoA): 24 B per neuron (ring slot read 8 + clear 8, potential
+    #            read/write 4 + 4) + 1 bit of spike output
+    scatter_bytes = ev_step * 9.0 + rows_step * 8.0
+    neuron_bytes = n_neurons * (24.0 + 1.0 / 8.0)
+    if "tile_run" in kern:  # one persistent launch runs both
+        dominant, kname, step_bytes = "tile_run", "k_tile_run", scatter_bytes + neuron_bytes
     elif "tile_scatter" in kern:
-        dominant, kname = "tile_scatter", "k_tile_scatter"
-        step_bytes = ev_step * 8.0 + parts_step * 8.0 + rows_step * 4.0 + cells_step * 16.0
+        dominant, kname, step_bytes = "tile_scatter", "k_tile_scatter", scatter_bytes
         if kern.get("neuron", {}).get("launches", 0) < tsteps:  # phase 1 fused into the tile launch
-            step_bytes += lean_bytes + ring_slot_bytes
+            step_bytes += neuron_bytes
     else:
-        dominant, kname = "scatter", "k_scatter"
-        step_bytes = ev_step * 17.0 + rows_step * 20.0
+        dominant, kname, step_bytes = "scatter", "k_scatter", scatter_bytes
+    if "neuron" in kern and kern["neuron"]["launches"] >= tsteps:
+        kn = kern["neuron"]
+        kn.update(bytes_per_step=neuron_bytes, achieved_GBs=neuron_bytes / (kn["us_per_step"] * 1e-6) / 1e9)
     dk = kern[dominant]
     per_launch = tsteps / dk["launches"]  # steps per launch
     dk.update(bytes_per_step=step_bytes, achieved_GBs=step_bytes / (dk["us_per_step"] * 1e-6) / 1e9)
+    if dominant != "scatter":
+        dk.update(row_parts_per_step=parts_step, ring_cells_per_step=cells_step)
     roofline = {
         "kernel": kname,
         "bound": "hbm",
