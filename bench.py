@@ -47,6 +47,13 @@ def main():
     ap.add_argument("--timing-steps", type=int, default=1000)
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="target CPU-baseline sample length")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--workload", choices=["c1", "c0", "c2", "c3", "c4"], default="c1",
+                    help="c1 (default): the plain FP32 network at GPU scale, the headline line. The others print "
+                         "their own measurement line: c0 small plain network, c2 single STDP instance, c3 batched "
+                         "STDP instances (instance-steps/s and ratio to c2), c4 uneven instances with the FP8 dense "
+                         "projection on MFMA")
+    ap.add_argument("--batch", type=int, default=256, help="c3: instances")
+    ap.add_argument("--agree-steps", type=int, default=400, help="c2/c3/c4: steps compared with the FP64 oracle")
     ap.add_argument("--scatter", choices=["auto", "stepwise", "atomic"], default="auto",
                     help="impact summation: tiled populations and persistent launches where eligible (auto), "
                          "tiled with one launch per step (stepwise), or atomics everywhere")
@@ -59,6 +66,10 @@ def main():
 
     from knp_amd import workloads
     from knp_amd.backend import MI355XBackend, device_name, KERNEL_IDS
+
+    if args.workload != "c1":
+        other_workload(args)
+        return
 
     t_gen = time.perf_counter()
     spec = workloads.plain_gpu_scale()
@@ -239,6 +250,183 @@ def main():
         "device": device_name(0),
     }
     gpu.close()
+    print(json.dumps(line), flush=True)
+
+
+def agreement(a, b):
+    """1 - |A xor B| / max(1, |A or B|) over (step, neuron) spike triples, and the Hamming accuracy."""
+    import numpy as np
+
+    union = int(np.logical_or(a, b).sum())
+    diff = int(np.logical_xor(a, b).sum())
+    return 1.0 - diff / max(1, union), 1.0 - diff / max(1, a.size)
+
+
+def pot_error(g, o):
+    import numpy as np
+
+    g = np.asarray(g, np.float64)
+    o = np.asarray(o, np.float64)
+    rel = np.abs(g - o) / np.maximum(np.abs(o), 1e-6)
+    return float(rel.max()) if rel.size else 0.0, float(rel.mean()) if rel.size else 0.0
+
+
+def timed_steps(gpu, steps):
+    gpu.synchronize()
+    e0, _ = gpu.counters()
+    t0 = time.perf_counter()
+    gpu.step(steps)
+    gpu.synchronize()
+    dt = time.perf_counter() - t0
+    e1, _ = gpu.counters()
+    return dt, e1 - e0
+
+
+def compare_f64(spec, gpu_rec, gpu_pot, pop, steps, neurons=None):
+    """Spike agreement and potential error of a GPU run against the FP64 oracle on the same spec."""
+    from oracle.pyoracle import Oracle
+
+    o = spec.load_into(Oracle("f64"))
+    o.record_spikes(pop, steps)
+    o.step(steps)
+    rec = o.read_spike_record(pop)
+    pot = o.potentials(pop)
+    o.close()
+    if neurons is not None:
+        rec, pot = rec[:, neurons], pot[neurons]
+        gpu_rec, gpu_pot = gpu_rec[:, neurons], gpu_pot[neurons]
+    agr, ham = agreement(gpu_rec, rec)
+    pmax, pmean = pot_error(gpu_pot, pot)
+    return {"steps": steps, "spike_agreement": agr, "hamming_accuracy": ham, "spikes_oracle": int(rec.sum()),
+            "spikes_gpu": int(gpu_rec.sum()), "potential_rel_err_max": pmax, "potential_rel_err_mean": pmean}
+
+
+def other_workload(args):
+    """Measurement lines of the other configurations (not the headline line)."""
+    import numpy as np
+
+    from knp_amd import abi, workloads
+    from knp_amd.backend import KERNEL_IDS, MI355XBackend, device_name
+
+    def load(spec):
+        g = MI355XBackend(0)
+        g.set_scatter_mode({"auto": abi.SCATTER_AUTO, "stepwise": abi.SCATTER_STEPWISE,
+                            "atomic": abi.SCATTER_ATOMIC}[args.scatter])
+        spec.load_into(g)
+        g._init()
+        g.synchronize()
+        return g
+
+    def kernels(g, steps):
+        g.enable_timing(True)
+        g.step(steps)
+        g.synchronize()
+        out = {}
+        for name, kid in KERNEL_IDS.items():
+            avg_ms, n = g.kernel_time(kid)
+            if n:
+                out[name] = {"avg_us": round(avg_ms * 1e3, 3), "launches": n,
+                             "us_per_step": round(avg_ms * 1e3 * n / steps, 3)}
+        g.enable_timing(False)
+        return out
+
+    line = {"workload": args.workload, "device": device_name(0), "data": "synthetic (seeded generators)"}
+    if args.workload == "c0":
+        from oracle.pyoracle import Oracle
+
+        spec = workloads.plain_small()
+        steps = spec.meta["steps"]
+        g = load(spec)
+        g.record_spikes(0, steps)
+        dt, ev = timed_steps(g, steps)
+        rec, pot = g.read_spike_record(0), g.potentials(0)
+        o = spec.load_into(Oracle("exact"))
+        o.record_spikes(0, steps)
+        tc = time.perf_counter()
+        o.step(steps)
+        tcpu = time.perf_counter() - tc
+        line.update(metric="synaptic events/s", value=ev / dt, net_steps_per_s=steps / dt, steps=steps,
+                    events=ev, dtype="fp32",
+                    parity_knp_exact={"spike_trains_identical": bool(np.array_equal(rec, o.read_spike_record(0))),
+                                      "events_equal": ev == o.events,
+                                      "potential_rel_err_max": pot_error(pot, o.potentials(0))[0]},
+                    fp64=compare_f64(spec, rec, pot, 0, steps),
+                    cpu_baseline={"value": o.events / tcpu, "unit": "events/s", "cores": 1, "kind": "port",
+                                  "net_steps_per_s": steps / tcpu, "sample": f"all {steps} steps"})
+        o.close()
+    elif args.workload in ("c2", "c3"):
+        steps = args.steps if args.steps != 10_000 else 2000
+        warm = max(args.warmup, args.agree_steps)  # both runs time the same step range
+        n_img = (warm + steps) // 20 + 1
+        # the plain single-instance path (the denominator of the batching ratio)
+        spec2 = workloads.stdp_learning(n_images=n_img)
+        g2 = load(spec2)
+        g2.step(warm)
+        dt2, ev2 = timed_steps(g2, steps)
+        k2 = kernels(g2, min(steps, 500))
+        g2.close()
+        line.update(c2={"net_steps_per_s": steps / dt2, "events_per_s": ev2 / dt2, "steps": steps,
+                        "ms_per_step": dt2 / steps * 1e3, "kernels": k2})
+        if args.workload == "c2":
+            g = load(spec2)
+            g.record_spikes(0, args.agree_steps)
+            g.step(args.agree_steps)
+            line.update(metric="network steps/s", value=steps / dt2, unit="net-steps/s", dtype="fp32",
+                        fp64=compare_f64(spec2, g.read_spike_record(0), g.potentials(0), 0, args.agree_steps))
+            g.close()
+        else:
+            B = args.batch
+            tg = time.perf_counter()
+            spec3 = workloads.batched_stdp(batch=B, n_images=n_img)
+            log(f"generated {B} instances ({spec3.n_synapses} synapses) in {time.perf_counter() - tg:.1f}s")
+            g = load(spec3)
+            g.record_spikes(0, args.agree_steps)
+            g.step(args.agree_steps)
+            rec0 = g.read_spike_record(0)[:, :1024]
+            pot0 = g.potentials(0)[:1024]
+            g.step(warm - args.agree_steps)
+            dt3, ev3 = timed_steps(g, steps)
+            k3 = kernels(g, min(steps, 500))
+            g.close()
+            # instance 0 of the batch is the standalone network with the batch's seeds and streams
+            inst0 = workloads.stdp_learning(seed=30, n_images=n_img, source_seed=30 * 7919 + 17, instance=0)
+            inst_steps_per_s = B * steps / dt3
+            line.update(metric="instance-steps/s", value=inst_steps_per_s, unit="instance-steps/s", dtype="fp32",
+                        batch=B, steps=steps, ms_per_step=dt3 / steps * 1e3, events_per_s=ev3 / dt3,
+                        ratio_to_single_instance=inst_steps_per_s / (steps / dt2), kernels=k3,
+                        fp64_instance0=compare_f64(inst0, rec0, pot0, 0, args.agree_steps))
+    elif args.workload == "c4":
+        steps = args.steps if args.steps != 10_000 else 2000
+        tg = time.perf_counter()
+        spec = workloads.uneven_inference(steps=steps)
+        log(f"generated c4 ({spec.n_synapses} sparse synapses) in {time.perf_counter() - tg:.1f}s")
+        N = spec.pop_sizes[0]
+        g = load(spec)
+        g.step(args.warmup)
+        dt, ev = timed_steps(g, steps)
+        k4 = kernels(g, min(steps, 1024))
+        T = g.dense_time_block(0)
+        g.close()
+        d = k4.get("dense", {})
+        # dense product per launch: 2 x T steps x 784 inputs x N neurons (binary spikes x FP8 weights)
+        flops = 2.0 * T * 784 * N
+        mfma = None
+        if d:
+            ach = flops / (d["avg_us"] * 1e-6) / 1e12
+            mfma = {"kernel": "k_dense_mfma (+ spike bits)", "bound": "mfma", "achieved": round(ach, 2),
+                    "peak": 5000.0, "unit": "TFLOP/s", "frac": round(ach / 5000.0, 5), "time_block": T,
+                    "flops_per_launch": flops}
+        # agreement on a reduced batch (the FP64 oracle expands every dense weight into a synapse)
+        small = workloads.uneven_inference(n_instances=16, steps=args.agree_steps)
+        gs = load(small)
+        gs.record_spikes(0, args.agree_steps)
+        gs.step(args.agree_steps)
+        fp64 = compare_f64(small, gs.read_spike_record(0), gs.potentials(0), 0, args.agree_steps)
+        fp64["instances"] = 16
+        gs.close()
+        line.update(metric="synaptic events/s", value=ev / dt, unit="events/s", dtype="fp32 sparse + fp8 dense",
+                    neurons=N, instances=512, steps=steps, ms_per_step=dt / steps * 1e3, net_steps_per_s=steps / dt,
+                    instance_steps_per_s=512 * steps / dt, dense_roofline=mfma, kernels=k4, fp64=fp64)
     print(json.dumps(line), flush=True)
 
 

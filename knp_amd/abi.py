@@ -173,9 +173,9 @@ def resource_rule_defaults(**overrides):
 def neuron_array(proto, n):
     """A ctypes array of n copies of `proto` (a BLIFATNeuron or ResourceBLIFATNeuron)."""
     arr = (type(proto) * n)()
-    raw = bytes(proto)
-    buf = (C.c_char * (C.sizeof(proto) * n)).from_buffer(arr)
-    buf[:] = raw * n
+    if n:
+        view = np.frombuffer(arr, dtype=np.uint8).reshape(n, C.sizeof(proto))
+        view[:] = np.frombuffer(bytes(proto), dtype=np.uint8)
     return arr
 
 
