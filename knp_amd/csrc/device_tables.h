@@ -18,6 +18,10 @@ constexpr int kMaxTasks = kMaxPops + kMaxSources;
 constexpr int kMaxWorking = 8;
 constexpr int kMaxDense = 4;
 constexpr int kBlock = 256;  // 4 wave64 per workgroup
+// A spike segment: word 0 = count, words 1.. = ascending spike indexes. Population and source segments hold
+// 256 entries; the stride keeps every segment 64-byte aligned, so the count and the first entries share one
+// line and can be read with one 16-byte load.
+constexpr uint32_t kSegStride = 272;
 constexpr int kTileThreads = 512;   // workgroup of the tile-owner scatter (8 wave64)
 constexpr int kTileMaxD = 32;       // ring depth limit of tile-owned populations (LDS: D x 256 x 8 bytes)
 constexpr int kTileMaxList = 2048;  // virtual segments that may reach one tile
@@ -212,6 +216,7 @@ struct NetDev {
     uint32_t *seg_count;       // [2][n_seg] spikes per segment, indexed by step parity
     uint32_t *const *seg_list; // [n_seg] first element of every segment (step parity 0)
     uint32_t list_pstride;     // offset of the parity-1 copy of every list
+    const uint32_t *list_pool; // base of the spike-list pool (both parities)
     const uint32_t *vseg;      // [n_vseg] segment of every virtual segment
     const uint32_t *vrs;       // [n_vseg] row source of every virtual segment
     const uint32_t *seg_vpos;  // [n_seg][4] virtual positions of every segment (UINT32_MAX = none)

@@ -104,6 +104,7 @@ struct knp_net {
     int scatter_mode = KNP_SCATTER_AUTO;
     bool fused = false;     // every population tiled: phase 1 of step t + 1 runs inside the tile kernel of t
     bool persistent = false;  // fused, and every tile fits on the device at once (cooperative launch)
+    uint32_t run_kind = POP_LEAN;  // population kind of the persistent launch
     uint32_t tile_lds = 0;  // dynamic LDS bytes of the tile scatter (largest tiled ring)
     std::vector<uint32_t *> seg_list_host;
     std::vector<uint32_t> push_vpos;  // virtual position of each projection's push segment
@@ -214,6 +215,19 @@ int32_t choose_frac_bits(double bound) {
     std::frexp(bound + 1.0, &e);  // bound + 1 < 2^e
     int f = 61 - e - 2;
     return std::min(52, f);
+}
+
+// Words of a channel (host-pushed) segment: the count, up to 4 entries per presynaptic index (each index at
+// most 4 times per step, checked when pushed), rounded to 64 bytes.
+uint64_t channel_seg_words(uint32_t n_pre) {
+    return (1ull + 4ull * std::max(n_pre, 1u) + 15) / 16 * 16;
+}
+
+// The persistent kernel for the population kind of every tile (or kMixedKinds).
+const void *tile_run_kernel(uint32_t kind) {
+    if (kind == POP_LEAN) return reinterpret_cast<const void *>(&k_tile_run<POP_LEAN>);
+    if (kind == POP_FULL) return reinterpret_cast<const void *>(&k_tile_run<POP_FULL>);
+    return reinterpret_cast<const void *>(&k_tile_run<kMixedKinds>);
 }
 
 int upload_working_sets(knp_net *net);
@@ -459,13 +473,15 @@ int init_network(knp_net *net) {
     uint32_t *pool_cursor = nullptr;
     {
         uint64_t half = 0;
-        for (uint32_t p = 0; p < n_pops; ++p) half += (uint64_t)std::max(1u, (net->pops[p].n + kBlock - 1) / kBlock) * kBlock;
-        for (const auto &Sh : net->srcs) half += (uint64_t)std::max(1u, (Sh.n_ch + kBlock - 1) / kBlock) * kBlock;
-        for (const auto &Qh : net->projs) half += 4ull * std::max(Qh.n_pre, 1u);
-        if (half >= (uint64_t(1) << 31)) return fail(KNP_ERR_UNSUPPORTED, "spike lists above 2^31 entries");
+        for (uint32_t p = 0; p < n_pops; ++p) half += (uint64_t)std::max(1u, (net->pops[p].n + kBlock - 1) / kBlock) * kSegStride;
+        for (const auto &Sh : net->srcs) half += (uint64_t)std::max(1u, (Sh.n_ch + kBlock - 1) / kBlock) * kSegStride;
+        for (const auto &Qh : net->projs) half += channel_seg_words(Qh.n_pre);
+        // byte offsets into the pool (both parities) are 32-bit in the tile kernels' buffer loads
+        if (2 * half * 4 >= (uint64_t(1) << 32)) return fail(KNP_ERR_UNSUPPORTED, "spike lists above 2^29 entries");
         int rc;
         if ((rc = dev_alloc(net, (void **)&pool_cursor, (size_t)2 * half * 4))) return rc;
         T.list_pstride = (uint32_t)half;
+        T.list_pool = pool_cursor;
     }
 
     // ---- populations
@@ -578,7 +594,7 @@ int init_network(knp_net *net) {
         }
         const uint32_t chunks = std::max(1u, (n + kBlock - 1) / kBlock);
         P.spike_list = pool_cursor;
-        pool_cursor += (size_t)chunks * kBlock;
+        pool_cursor += (size_t)chunks * kSegStride;
         if ((rc = dev_alloc(net, (void **)&P.bitmap, (size_t)std::max(P.words, 1u) * 4))) return rc;
         if (H.rec_cap) {
             if ((rc = dev_alloc(net, (void **)&P.rec, (size_t)H.rec_cap * P.words * 4))) return rc;
@@ -840,7 +856,7 @@ int init_network(knp_net *net) {
         // is never exceeded and the list never moves (tile tables hold its address)
         Q.in_cap = 4 * std::max(H.n_pre, 1u);
         Q.in_list = pool_cursor;
-        pool_cursor += Q.in_cap;
+        pool_cursor += channel_seg_words(H.n_pre);
     }
 
     // ---- sources
@@ -862,7 +878,7 @@ int init_network(knp_net *net) {
         S.thr = th;
         const uint32_t chunks = std::max(1u, (H.n_ch + kBlock - 1) / kBlock);
         S.list = pool_cursor;
-        pool_cursor += (size_t)chunks * kBlock;
+        pool_cursor += (size_t)chunks * kSegStride;
     }
 
     // ---- spike segments and the scatter's virtual segment list
@@ -874,7 +890,7 @@ int init_network(knp_net *net) {
             PopDev &P = T.pops[p];
             pop_chunks[p] = std::max(1u, (P.n + kBlock - 1) / kBlock);
             P.seg_base = (uint32_t)seg_list.size();
-            for (uint32_t c = 0; c < pop_chunks[p]; ++c) seg_list.push_back(P.spike_list + (size_t)c * kBlock);
+            for (uint32_t c = 0; c < pop_chunks[p]; ++c) seg_list.push_back(P.spike_list + (size_t)c * kSegStride);
         }
         // every (projection, segment) pair whose spikes the projection expands, with the presynaptic rows the
         // segment can hold: for every projection (load order) its channel segments, then the segments of every
@@ -901,7 +917,7 @@ int init_network(knp_net *net) {
                 for (uint32_t c = 0; c < chunks; ++c) {
                     const uint32_t r0 = S.first + c * kBlock, r1 = S.first + std::min((c + 1) * kBlock, S.n_ch);
                     vd.push_back(VDesc{(uint32_t)seg_list.size(), nrs, q, r0, r1});
-                    seg_list.push_back(S.list + (size_t)c * kBlock);
+                    seg_list.push_back(S.list + (size_t)c * kSegStride);
                 }
             }
             ++nrs;
@@ -1049,14 +1065,21 @@ int init_network(knp_net *net) {
     for (uint32_t s = 0; s < T.n_srcs; ++s) T.n_src_blocks += std::max(1u, (net->srcs[s].n_ch + kBlock - 1) / kBlock);
     net->fused = T.n_tiles > 0;
     for (const auto &H : net->pops) net->fused = net->fused && H.tiled;
+    // persistent launches: every population tiled, every tile on the device at once, and
+    // at most kCachedSrcBlocks source blocks per tile
     net->persistent = false;
+    net->run_kind = POP_LEAN;
     if (net->fused && net->scatter_mode == KNP_SCATTER_AUTO) {
+        bool one_kind = true;
+        for (uint32_t p = 0; p < n_pops; ++p) one_kind = one_kind && T.pops[p].kind == T.pops[0].kind;
+        net->run_kind = one_kind ? T.pops[0].kind : kMixedKinds;
+        const void *fn = tile_run_kernel(net->run_kind);
         int coop = 0, cus = 0, per_cu = 0;
         HIP_TRY(hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, net->device));
         HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, net->device));
-        HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void *>(&k_tile_run),
-                                                             kTileThreads, net->tile_lds));
-        net->persistent = coop && (uint64_t)per_cu * cus >= T.n_tiles;
+        HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, kTileThreads, net->tile_lds));
+        const bool src_fit = T.n_src_blocks <= (uint64_t)kCachedSrcBlocks * T.n_tiles;
+        net->persistent = coop && src_fit && (uint64_t)per_cu * cus >= T.n_tiles;
     }
     {
         int rc;
@@ -1178,10 +1201,10 @@ int deliver_pushes(knp_net *net) {
         H.pending.clear();
         ProjDev &Q = net->table.projs[q];
         if (L.size() > Q.in_cap) return fail(KNP_ERR_RANGE, "more channel spikes in one step than the list holds");
-        if (!L.empty())
-            HIP_TRY(hipMemcpyAsync(Q.in_list + (size_t)(t & 1) * net->table.list_pstride, L.data(), L.size() * 4,
-                                   hipMemcpyHostToDevice, net->stream));
         const uint32_t cnt = (uint32_t)L.size();
+        L.insert(L.begin(), cnt);  // segment layout: count, then the indexes
+        HIP_TRY(hipMemcpyAsync(Q.in_list + (size_t)(t & 1) * net->table.list_pstride, L.data(), L.size() * 4,
+                               hipMemcpyHostToDevice, net->stream));
         uint32_t *slot = net->table.seg_count + (size_t)(t & 1) * net->table.n_seg + Q.push_seg;
         HIP_TRY(hipMemcpyAsync(slot, &cnt, 4, hipMemcpyHostToDevice, net->stream));
         uint32_t *vslot = net->table.vcount + (size_t)(t & 1) * net->table.n_vseg + net->push_vpos[q];
@@ -1229,8 +1252,8 @@ int run_steps(knp_net *net, uint64_t n_steps) {
             uint64_t t0 = t;
             void *args[] = {(void *)&D, (void *)&t0, (void *)&steps};
             if ((rc = timed_launch(net, KNP_KERNEL_TILE_RUN, [&] {
-                     le = hipLaunchCooperativeKernel(reinterpret_cast<const void *>(&k_tile_run), dim3(nt),
-                                                     dim3(kTileThreads), args, net->tile_lds, net->stream);
+                     le = hipLaunchCooperativeKernel(tile_run_kernel(net->run_kind), dim3(nt), dim3(kTileThreads),
+                                                     args, net->tile_lds, net->stream);
                  })))
                 return rc;
             if (le != hipSuccess) return fail(KNP_ERR_HIP, std::string("persistent launch: ") + hipGetErrorString(le));

@@ -22,6 +22,7 @@ namespace knp_mi355x {
 // LDS counter, so every LDS wait and workgroup barrier would wait for them. Accesses through gp() are
 // global-memory instructions.
 #define KNP_GLOBAL __attribute__((address_space(1)))
+typedef unsigned int v4u __attribute__((ext_vector_type(4)));
 template <typename T>
 __device__ __forceinline__ KNP_GLOBAL T *gp(T *p) {
     return (KNP_GLOBAL T *)p;
@@ -90,8 +91,9 @@ __device__ __forceinline__ uint32_t ld_u32(const uint32_t *p) {
     return *gp(p);
 }
 
-// Ordered compaction of a workgroup's 256 candidates into its own segment: ascending order, no global
-// atomics, one count store. NT = threads of the workgroup (threads from 256 on carry no candidate).
+// Ordered compaction of a workgroup's 256 candidates into its own segment (count in word 0, indexes from
+// word 1): ascending order, no global atomics. NT = threads of the workgroup (threads from 256 on carry no
+// candidate).
 // Returns (in thread 0) the rows published: the segment's count times the projections that read it.
 // vp_pre: the segment's 4 virtual positions if the caller holds them (else they are loaded).
 template <uint32_t NT = kBlock, bool COHERENT = false>
@@ -108,11 +110,14 @@ __device__ __forceinline__ uint32_t block_segment_write(const NetDev *__restrict
         if (k < w) off += wcnt[k];
         total += wcnt[k];
     }
-    if (pred) st_u32<COHERENT>(&segment[off + __popcll(mask & ((1ull << lane) - 1ull))], value);
+    if (pred) st_u32<COHERENT>(&segment[1 + off + __popcll(mask & ((1ull << lane) - 1ull))], value);
     uint32_t rows = 0;
     if (threadIdx.x < 4) {
         const size_t par = (size_t)(t & 1);
-        if (threadIdx.x == 0) st_u32<COHERENT>(&net->seg_count[par * net->n_seg + seg], total);
+        if (threadIdx.x == 0) {
+            st_u32<COHERENT>(&segment[0], total);
+            st_u32<COHERENT>(&net->seg_count[par * net->n_seg + seg], total);
+        }
         uint32_t vps[4];
         for (uint32_t k = 0; k < 4; ++k) vps[k] = vp_pre ? vp_pre[k] : gp(net->seg_vpos)[4 * seg + k];
         const uint32_t vp = vps[threadIdx.x & 3];
@@ -150,11 +155,19 @@ __device__ __forceinline__ int64_t blocking_value(const NetDev *net, uint64_t ke
 // ---- phase 1: populations + sources ----------------------------------------------------------------
 // NT = threads of the workgroup (threads from 256 on have no neuron). FUSED: the exact sum of the step's
 // impacts is passed in (a_in) by the tile kernel instead of being read from the ring.
-template <uint32_t KIND, uint32_t NT = kBlock, bool FUSED = false, bool COHERENT = false>
+// The table entry of population p, or the caller's LDS copy of it (COPY: known at compile time, so that
+// the compiler addresses the copy as LDS).
+template <bool COPY, typename T>
+__device__ __forceinline__ const T &entry_of(const T *table, uint32_t i, const T *copy) {
+    if constexpr (COPY) return *copy;
+    else return table[i];
+}
+
+template <uint32_t KIND, uint32_t NT = kBlock, bool FUSED = false, bool COHERENT = false, bool COPY = false>
 __device__ __forceinline__ uint32_t population_step(const NetDev *__restrict__ net, uint32_t p, uint32_t local_block,
                                                     uint64_t t, Ctl *ctl, uint32_t *seg_count, int64_t a_in = 0,
-                                                    const uint32_t *vp_pre = nullptr) {
-    const PopDev &P = net->pops[p];
+                                                    const uint32_t *vp_pre = nullptr, const PopDev *P_copy = nullptr) {
+    const PopDev &P = entry_of<COPY>(net->pops, p, P_copy);
     const uint32_t i = local_block * kBlock + threadIdx.x;
     const bool active = threadIdx.x < (uint32_t)kBlock && i < P.n;
     const uint32_t slot = (uint32_t)(t % P.D);
@@ -280,7 +293,7 @@ __device__ __forceinline__ uint32_t population_step(const NetDev *__restrict__ n
     }
     // spike compaction and bitmaps (whole wave participates)
     const uint32_t rows = block_segment_write<NT, COHERENT>(
-        net, t, spike, i, P.spike_list + (size_t)(t & 1) * net->list_pstride + (size_t)local_block * kBlock,
+        net, t, spike, i, P.spike_list + (size_t)(t & 1) * net->list_pstride + (size_t)local_block * kSegStride,
         P.seg_base + local_block, vp_pre);
     const uint64_t mask = __ballot(spike);
     if (threadIdx.x < (uint32_t)kBlock) wave_bitmap(mask, local_block * kBlock + (threadIdx.x & ~63u), P, t);
@@ -290,11 +303,11 @@ __device__ __forceinline__ uint32_t population_step(const NetDev *__restrict__ n
 
 // thr_pre: this thread's channel threshold when the caller holds it (single-pattern sources), else
 // UINT32_MAX and it is loaded; vp_pre as in block_segment_write.
-template <uint32_t NT = kBlock, bool COHERENT = false>
+template <uint32_t NT = kBlock, bool COHERENT = false, bool COPY = false>
 __device__ __forceinline__ uint32_t source_step(const NetDev *__restrict__ net, uint32_t s, uint32_t local_block,
                                                 uint64_t t, uint32_t *seg_count, uint32_t thr_pre = 0xFFFFFFFFu,
-                                                const uint32_t *vp_pre = nullptr) {
-    const SourceDev &S = net->srcs[s];
+                                                const uint32_t *vp_pre = nullptr, const SourceDev *S_copy = nullptr) {
+    const SourceDev &S = entry_of<COPY>(net->srcs, s, S_copy);
     const uint32_t c = local_block * kBlock + threadIdx.x;
     const uint32_t phase = (uint32_t)(t % S.steps_per_pattern);
     bool spike = false;
@@ -304,7 +317,7 @@ __device__ __forceinline__ uint32_t source_step(const NetDev *__restrict__ net, 
         spike = bernoulli_hash(S.seed, t, S.hash_offset + c) < thr;
     }
     return block_segment_write<NT, COHERENT>(
-        net, t, spike, S.first + c, S.list + (size_t)(t & 1) * net->list_pstride + (size_t)local_block * kBlock,
+        net, t, spike, S.first + c, S.list + (size_t)(t & 1) * net->list_pstride + (size_t)local_block * kSegStride,
         S.seg_base + local_block, vp_pre);
 }
 
@@ -320,6 +333,7 @@ __global__ __launch_bounds__(kBlock) void k_phase1(const NetDev *__restrict__ ne
         for (uint32_t k = threadIdx.x; k < net->n_projs; k += blockDim.x) {
             const uint32_t ps = net->projs[k].push_seg;
             nc[ps] = 0;
+            net->projs[k].in_list[(size_t)((t + 1) & 1) * net->list_pstride] = 0;  // segment count word
             for (uint32_t j = 0; j < 4; ++j) {
                 const uint32_t vp = net->seg_vpos[4 * ps + j];
                 if (vp != 0xFFFFFFFFu) nv[vp] = 0;
@@ -430,7 +444,7 @@ __global__ __launch_bounds__(kBlock) void k_scatter(const NetDev *__restrict__ n
         }
         const uint32_t seg = net->vseg[lo];
         const RowSource &R = net->rows[net->vrs[lo]];
-        const uint32_t pre = net->seg_list[seg][(size_t)(t & 1) * net->list_pstride + row - pfx[lo]];
+        const uint32_t pre = net->seg_list[seg][(size_t)(t & 1) * net->list_pstride + 1 + row - pfx[lo]];
         const ProjDev &Q = net->projs[R.proj];
         if (pre >= Q.n_pre) continue;
         const uint64_t b = Q.row_ptr[pre], e = Q.row_ptr[pre + 1];
@@ -512,10 +526,10 @@ struct TileProj {
 
 // Dynamic LDS of the tile kernels: the ring sums [tile_max_d][256] (int64), then for the tile's segments
 // their spike lists [tile_max_list] (pointers, parity 0), count positions [tile_max_list], row-count prefix
-// [tile_max_list + 1] and projection [tile_max_list].
+// [tile_max_list + 1], first three spike indexes [tile_max_list][3] and projection [tile_max_list].
 __host__ __device__ inline size_t tile_lds_bytes(uint32_t max_d, uint32_t max_list) {
     return (size_t)max_d * kBlock * 8 + (size_t)max_list * 8 + (size_t)max_list * 4 + ((size_t)max_list + 1) * 4 +
-           max_list;
+           (size_t)max_list * 12 + max_list;
 }
 
 // State of one tile workgroup: its tile, its population's fields (in registers: stores through the ring
@@ -529,6 +543,7 @@ struct TileCtx {
     const uint32_t **vlist;
     uint32_t *vpos, *vpfx;
     uint8_t *vproj;
+    uint32_t *vstash;  // [tile_max_list][3]: the first 3 spike indexes of every segment this step
     uint32_t nv;
 };
 
@@ -565,7 +580,8 @@ __device__ __forceinline__ TileCtx tile_setup(const NetDev *__restrict__ net, un
     c.vlist = reinterpret_cast<const uint32_t **>(dyn + (size_t)net->tile_max_d * kBlock);
     c.vpos = reinterpret_cast<uint32_t *>(c.vlist + L);
     c.vpfx = c.vpos + L;
-    c.vproj = reinterpret_cast<uint8_t *>(c.vpfx + L + 1);
+    c.vstash = c.vpfx + L + 1;
+    c.vproj = reinterpret_cast<uint8_t *>(c.vstash + 3 * L);
     c.nv = c.td.vend - c.td.vbeg;
     const uint32_t tid = threadIdx.x;
     if (tid < 4) sh.cnt[tid] = 0;
@@ -590,18 +606,39 @@ template <bool COHERENT>
 __device__ __forceinline__ void tile_gather(const NetDev *__restrict__ net, const TileCtx &c, TileShared &sh,
                                             uint64_t t, uint32_t &my_ev, uint32_t &my_parts) {
     const uint32_t tid = threadIdx.x;
-    const uint32_t *vc = net->vcount + (size_t)(t & 1) * net->n_vseg;
     const size_t par_off = (size_t)(t & 1) * net->list_pstride;
+    // the pool as a buffer: one 16-byte load gives a segment's count and first three spike indexes
+    // (device-scope in a persistent launch, where other workgroups wrote them during this launch)
+    const __amdgpu_buffer_rsrc_t pool = __builtin_amdgcn_make_buffer_rsrc(
+        const_cast<uint32_t *>(net->list_pool), 0, (int)(2u * net->list_pstride * 4u), 0x00020000);
     uint32_t total = 0;
     for (uint32_t base = 0; base < c.nv; base += kTileThreads) {
         const uint32_t v = base + tid;
-        const uint32_t cv = v < c.nv ? ld_u32<COHERENT>(&vc[c.vpos[v]]) : 0u;
+        uint32_t cv = 0;
+        if (v < c.nv) {
+            const uint32_t off = (uint32_t)((const char *)(c.vlist[v] + par_off) - (const char *)net->list_pool);
+            const v4u h = __builtin_amdgcn_raw_buffer_load_b128(pool, off, 0, COHERENT ? 16 : 0);
+            cv = h.x;
+            c.vstash[3 * v + 0] = h.y;
+            c.vstash[3 * v + 1] = h.z;
+            c.vstash[3 * v + 2] = h.w;
+        }
         uint32_t tot;
         const uint32_t ex = block_excl_scan<kTileThreads>(cv, sh.wsum, &tot);
         if (v < c.nv) c.vpfx[v] = total + ex;
         total += tot;
     }
     __syncthreads();
+    // rows -> segment map for one pass of rows (instead of a binary search per row): sh.rbeg serves as the
+    // map until the long-part pass below reuses it
+    const bool mapped = total <= (uint32_t)kTileThreads;
+    if (mapped) {
+        for (uint32_t v = tid; v < c.nv; v += kTileThreads) {
+            const uint32_t r0 = c.vpfx[v], r1 = v + 1 < c.nv ? c.vpfx[v + 1] : total;
+            for (uint32_t r = r0; r < r1; ++r) sh.rbeg[r] = v;
+        }
+        __syncthreads();
+    }
     const uint32_t slot0 = (uint32_t)(t % c.D);
     // exact sum of one packed synapse record into the LDS ring (false: a neuron of another tile)
     auto add_record = [&](uint64_t rec) -> bool {
@@ -620,13 +657,18 @@ __device__ __forceinline__ void tile_gather(const NetDev *__restrict__ net, cons
         uint32_t b = 0, len = 0, q = 0;
         if (row < total) {
             uint32_t lo = 0, hi = c.nv;  // largest v with vpfx[v] <= row
-            while (hi - lo > 1) {
-                const uint32_t mid = (lo + hi) >> 1;
-                if (c.vpfx[mid] <= row) lo = mid;
-                else hi = mid;
+            if (mapped) {
+                lo = sh.rbeg[row];
+            } else {
+                while (hi - lo > 1) {
+                    const uint32_t mid = (lo + hi) >> 1;
+                    if (c.vpfx[mid] <= row) lo = mid;
+                    else hi = mid;
+                }
             }
             q = c.vproj[lo];
-            const uint32_t pre = ld_u32<COHERENT>(c.vlist[lo] + par_off + (row - c.vpfx[lo]));
+            const uint32_t idx = row - c.vpfx[lo];
+            const uint32_t pre = idx < 3 ? c.vstash[3 * lo + idx] : ld_u32<COHERENT>(c.vlist[lo] + par_off + 1 + idx);
             const TileProj &Q = sh.tp[q];
             if (pre < Q.n_pre) {
                 if (Q.tile_dir) {
@@ -697,6 +739,7 @@ __device__ __forceinline__ void clear_push_counts(const NetDev *__restrict__ net
     for (uint32_t k = threadIdx.x; k < net->n_projs; k += kTileThreads) {
         const uint32_t ps = net->projs[k].push_seg;
         st_u32<COHERENT>(&nc[ps], 0u);
+        st_u32<COHERENT>(net->projs[k].in_list + (size_t)(t1 & 1) * net->list_pstride, 0u);  // segment count word
         for (uint32_t j = 0; j < 4; ++j) {
             const uint32_t vp = gp(net->seg_vpos)[4 * ps + j];
             if (vp != 0xFFFFFFFFu) st_u32<COHERENT>(&nvc[vp], 0u);
@@ -727,67 +770,83 @@ __device__ __forceinline__ uint32_t tile_phase1(const NetDev *__restrict__ net, 
     return rows;  // rows published for step t1 (thread 0)
 }
 
-// What phase 1 of a persistent workgroup writes to, fixed for the launch: the virtual positions of its
-// population segment and of its source blocks (in LDS), and this thread's channel threshold for
-// single-pattern sources (in registers), so that a step's phase 1 loads nothing but its own state.
-constexpr uint32_t kCachedSrcBlocks = 4;
+// What phase 1 of a persistent workgroup writes to, fixed for the launch, kept in LDS so that a step's
+// phase 1 loads nothing but its own state: the virtual positions of its population segment and of its
+// source blocks (block b of the sources is run by tile b mod n_tiles), copies of their table entries (LDS
+// loads cannot alias the step's global stores) and every thread's channel threshold for single-pattern
+// sources. The host uses the persistent launch only when no tile runs more than kCachedSrcBlocks blocks.
+constexpr uint32_t kCachedSrcBlocks = 8;
 struct PhaseCache {
     uint32_t pop_vp[4];
-    uint32_t n_src;  // source blocks of this workgroup; cached only if <= kCachedSrcBlocks
+    uint32_t n_src;
     uint32_t src[kCachedSrcBlocks], blk[kCachedSrcBlocks];
     uint32_t vp[kCachedSrcBlocks][4];
+    PopDev pop;
+    SourceDev srcd[kCachedSrcBlocks];
+    uint32_t thr[kCachedSrcBlocks][kBlock];  // UINT32_MAX: load the threshold (several patterns)
 };
 
-__device__ __forceinline__ void phase_cache(const NetDev *__restrict__ net, const TileCtx &c, PhaseCache &pc,
-                                            uint32_t (&thr)[kCachedSrcBlocks]) {
+// Word-wise copy of a table entry into LDS by the workgroup (the caller synchronizes).
+template <typename T>
+__device__ __forceinline__ void lds_copy(T *dst, const T *src) {
+    static_assert(sizeof(T) % 4 == 0, "word copy");
+    uint32_t *d = reinterpret_cast<uint32_t *>(dst);
+    const KNP_GLOBAL uint32_t *g = gp(reinterpret_cast<const uint32_t *>(src));
+    for (uint32_t k = threadIdx.x; k < sizeof(T) / 4; k += blockDim.x) d[k] = g[k];
+}
+
+__device__ __forceinline__ void phase_cache(const NetDev *__restrict__ net, const TileCtx &c, PhaseCache &pc) {
     const uint32_t pseg = net->pops[c.td.pop].seg_base + c.td.tile;
     if (threadIdx.x < 4) pc.pop_vp[threadIdx.x] = gp(net->seg_vpos)[4 * pseg + threadIdx.x];
+    lds_copy(&pc.pop, &net->pops[c.td.pop]);
     uint32_t n = 0, b0 = 0;
-#pragma unroll
-    for (uint32_t j = 0; j < kCachedSrcBlocks; ++j) thr[j] = 0xFFFFFFFFu;
     for (uint32_t s = 0; s < net->n_srcs; ++s) {
         const SourceDev &S = net->srcs[s];
         const uint32_t nb = max(1u, (S.n_ch + kBlock - 1) / kBlock);
         for (uint32_t b = blockIdx.x; b < b0 + nb; b += net->n_tiles) {
             if (b < b0) continue;
             const uint32_t j = n++;
-            if (j >= kCachedSrcBlocks) continue;
+            if (j >= kCachedSrcBlocks) continue;  // not reached: the host checks the bound
             if (threadIdx.x == 0) {
                 pc.src[j] = s;
                 pc.blk[j] = b - b0;
             }
             if (threadIdx.x < 4) pc.vp[j][threadIdx.x] = gp(net->seg_vpos)[4 * (S.seg_base + b - b0) + threadIdx.x];
+            lds_copy(&pc.srcd[j], &net->srcs[s]);
             const uint32_t ch = (b - b0) * kBlock + threadIdx.x;
-            const uint32_t v = (S.n_patterns == 1 && threadIdx.x < (uint32_t)kBlock && ch < S.n_ch) ? gp(S.thr)[ch]
-                                                                                                  : 0xFFFFFFFFu;
-#pragma unroll
-            for (uint32_t k = 0; k < kCachedSrcBlocks; ++k)
-                if (k == j) thr[k] = v;
+            if (threadIdx.x < (uint32_t)kBlock)
+                pc.thr[j][threadIdx.x] = (S.n_patterns == 1 && ch < S.n_ch) ? gp(S.thr)[ch] : 0xFFFFFFFFu;
         }
         b0 += nb;
     }
-    if (threadIdx.x == 0) pc.n_src = n;
+    if (threadIdx.x == 0) pc.n_src = n < kCachedSrcBlocks ? n : kCachedSrcBlocks;
     __syncthreads();
 }
 
-// tile_phase1 with the launch's fixed targets at hand (falls back to it with many source blocks).
+// Phase 1 of step t1 for a persistent workgroup: its tile's neurons (a = exact sum of the impacts arriving
+// at t1) and its source blocks. KIND: the population kind of every tile of the launch, or kMixedKinds (the
+// kind is read per tile; a larger kernel).
+constexpr uint32_t kMixedKinds = 0xFF;
+template <uint32_t KIND>
 __device__ __forceinline__ uint32_t tile_phase1_cached(const NetDev *__restrict__ net, const TileCtx &c,
-                                                       const PhaseCache &pc, const uint32_t (&thr)[kCachedSrcBlocks],
-                                                       uint64_t t1, int64_t a) {
-    const uint32_t n_src = pc.n_src;
-    if (n_src > kCachedSrcBlocks) return tile_phase1<true>(net, c, t1, a);
+                                                       const PhaseCache &pc, uint64_t t1, int64_t a) {
     Ctl *ctl = &net->ctl[t1 & 1];
     uint32_t *seg_count = net->seg_count + (size_t)(t1 & 1) * net->n_seg;
     uint32_t rows;
-    if (c.kind == POP_LEAN)
-        rows = population_step<POP_LEAN, kTileThreads, true, true>(net, c.td.pop, c.td.tile, t1, ctl, seg_count, a,
-                                                                   pc.pop_vp);
+    if (KIND != kMixedKinds)
+        rows = population_step<KIND, kTileThreads, true, true, true>(net, c.td.pop, c.td.tile, t1, ctl, seg_count, a,
+                                                                     pc.pop_vp, &pc.pop);
+    else if (c.kind == POP_LEAN)
+        rows = population_step<POP_LEAN, kTileThreads, true, true, true>(net, c.td.pop, c.td.tile, t1, ctl, seg_count,
+                                                                         a, pc.pop_vp, &pc.pop);
     else
-        rows = population_step<POP_FULL, kTileThreads, true, true>(net, c.td.pop, c.td.tile, t1, ctl, seg_count, a,
-                                                                   pc.pop_vp);
-#pragma unroll
-    for (uint32_t j = 0; j < kCachedSrcBlocks; ++j)
-        if (j < n_src) rows += source_step<kTileThreads, true>(net, pc.src[j], pc.blk[j], t1, nullptr, thr[j], pc.vp[j]);
+        rows = population_step<POP_FULL, kTileThreads, true, true, true>(net, c.td.pop, c.td.tile, t1, ctl, seg_count,
+                                                                         a, pc.pop_vp, &pc.pop);
+    const uint32_t n_src = pc.n_src;
+    for (uint32_t j = 0; j < n_src; ++j) {
+        const uint32_t thr = threadIdx.x < (uint32_t)kBlock ? pc.thr[j][threadIdx.x] : 0xFFFFFFFFu;
+        rows += source_step<kTileThreads, true, true>(net, pc.src[j], pc.blk[j], t1, nullptr, thr, pc.vp[j], &pc.srcd[j]);
+    }
     return rows;
 }
 
@@ -930,6 +989,7 @@ __device__ __forceinline__ bool grid_barrier(uint32_t *bar, uint32_t n_blocks, u
 // once (cooperative launch): the ring lives in LDS for the whole launch (loaded from HBM at the start,
 // stored back at the end), and one grid barrier per step separates phase 1 of step t (spike lists and
 // counts, written through to memory) from the scatter of step t that reads them.
+template <uint32_t KIND>
 __global__ __launch_bounds__(kTileThreads) void k_tile_run(const NetDev *__restrict__ net, uint64_t t0, uint32_t n) {
     extern __shared__ unsigned long long tacc[];
     __shared__ TileShared sh;
@@ -943,8 +1003,7 @@ __global__ __launch_bounds__(kTileThreads) void k_tile_run(const NetDev *__restr
     }
     __syncthreads();
     __shared__ PhaseCache pc;
-    uint32_t thr[kCachedSrcBlocks];
-    phase_cache(net, c, pc, thr);
+    phase_cache(net, c, pc);
     uint32_t my_ev = 0, my_parts = 0;
     unsigned long long my_rows = 0;  // thread 0: rows published by this workgroup's phase 1
     uint32_t barriers = 0;
@@ -968,7 +1027,7 @@ __global__ __launch_bounds__(kTileThreads) void k_tile_run(const NetDev *__restr
             a = (int64_t)c.ring[s0 * kBlock + tid];
             c.ring[s0 * kBlock + tid] = 0;
         }
-        my_rows += tile_phase1_cached(net, c, pc, thr, t0, a);
+        my_rows += tile_phase1_cached<KIND>(net, c, pc, t0, a);
         alive = grid_barrier(bar, net->n_tiles, ++barriers);
     }
     for (uint32_t k = 0; k < n && alive; ++k) {
@@ -982,7 +1041,7 @@ __global__ __launch_bounds__(kTileThreads) void k_tile_run(const NetDev *__restr
                 a = (int64_t)c.ring[s1 * kBlock + tid];
                 c.ring[s1 * kBlock + tid] = 0;
             }
-            my_rows += tile_phase1_cached(net, c, pc, thr, t + 1, a);
+            my_rows += tile_phase1_cached<KIND>(net, c, pc, t + 1, a);
             alive = grid_barrier(bar, net->n_tiles, ++barriers);
         }
     }
@@ -1219,7 +1278,7 @@ __global__ __launch_bounds__(kBlock) void k_additive_append(const NetDev *__rest
         for (uint32_t item = wave; item < chunks * kBlock; item += waves) {
             const uint32_t chunk = item / kBlock, r = item % kBlock;
             if (r >= cnt[SP.seg_base + chunk]) continue;
-            const uint32_t nidx = SP.spike_list[(size_t)(t & 1) * net->list_pstride + (size_t)chunk * kBlock + r];
+            const uint32_t nidx = SP.spike_list[(size_t)(t & 1) * net->list_pstride + (size_t)chunk * kSegStride + 1 + r];
             if (nidx >= n_post) continue;
             const uint64_t cb = Q.col_ptr[nidx], ce = Q.col_ptr[nidx + 1];
             for (uint64_t k = cb + lane; k < ce; k += 64) {
