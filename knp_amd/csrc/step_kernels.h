@@ -163,17 +163,16 @@ __device__ __forceinline__ const T &entry_of(const T *table, uint32_t i, const T
     else return table[i];
 }
 
-template <uint32_t KIND, uint32_t NT = kBlock, bool FUSED = false, bool COHERENT = false, bool COPY = false>
-__device__ __forceinline__ uint32_t population_step(const NetDev *__restrict__ net, uint32_t p, uint32_t local_block,
-                                                    uint64_t t, Ctl *ctl, uint32_t *seg_count, int64_t a_in = 0,
-                                                    const uint32_t *vp_pre = nullptr, const PopDev *P_copy = nullptr) {
-    const PopDev &P = entry_of<COPY>(net->pops, p, P_copy);
-    const uint32_t i = local_block * kBlock + threadIdx.x;
-    const bool active = threadIdx.x < (uint32_t)kBlock && i < P.n;
+// The BLIFAT update of neuron i of population P at step t (blifat_impl.h:31-168): returns the spike and sets
+// cand (synaptic-resource neurons: a training candidate). FUSED: a_in is the exact sum of the step's impacts
+// (else it is read from the ring and the slot cleared).
+template <uint32_t KIND, bool FUSED>
+__device__ __forceinline__ bool population_update(const NetDev *__restrict__ net, const PopDev &P, uint32_t i,
+                                                  bool active, uint64_t t, int64_t a_in, bool &cand) {
     const uint32_t slot = (uint32_t)(t % P.D);
     const size_t ai = (size_t)slot * P.n + i;
     bool spike = false;
-    bool cand = false;
+    cand = false;
     if (active) {
         int64_t a = a_in;
         if (!FUSED && P.acc_v) {
@@ -291,6 +290,18 @@ __device__ __forceinline__ uint32_t population_step(const NetDev *__restrict__ n
             }
         }
     }
+    return spike;
+}
+
+template <uint32_t KIND, uint32_t NT = kBlock, bool FUSED = false, bool COHERENT = false, bool COPY = false>
+__device__ __forceinline__ uint32_t population_step(const NetDev *__restrict__ net, uint32_t p, uint32_t local_block,
+                                                    uint64_t t, Ctl *ctl, uint32_t *seg_count, int64_t a_in = 0,
+                                                    const uint32_t *vp_pre = nullptr, const PopDev *P_copy = nullptr) {
+    const PopDev &P = entry_of<COPY>(net->pops, p, P_copy);
+    const uint32_t i = local_block * kBlock + threadIdx.x;
+    const bool active = threadIdx.x < (uint32_t)kBlock && i < P.n;
+    bool cand;
+    const bool spike = population_update<KIND, FUSED>(net, P, i, active, t, a_in, cand);
     // spike compaction and bitmaps (whole wave participates)
     const uint32_t rows = block_segment_write<NT, COHERENT>(
         net, t, spike, i, P.spike_list + (size_t)(t & 1) * net->list_pstride + (size_t)local_block * kSegStride,
@@ -301,21 +312,26 @@ __device__ __forceinline__ uint32_t population_step(const NetDev *__restrict__ n
     return rows;
 }
 
-// thr_pre: this thread's channel threshold when the caller holds it (single-pattern sources), else
-// UINT32_MAX and it is loaded; vp_pre as in block_segment_write.
+// Does channel c of source S fire at step t (counter-based Bernoulli draw against the channel's threshold
+// for the current pattern)? thr_pre: the threshold if the caller holds it (single-pattern sources), else
+// UINT32_MAX and it is loaded.
+__device__ __forceinline__ bool source_spike(const SourceDev &S, uint32_t c, bool active, uint64_t t,
+                                             uint32_t thr_pre) {
+    const uint32_t phase = (uint32_t)(t % S.steps_per_pattern);
+    if (!active || c >= S.n_ch || phase < S.lo || phase > S.hi) return false;
+    const uint32_t pattern = (uint32_t)((t / S.steps_per_pattern) % S.n_patterns);
+    const uint32_t thr = thr_pre != 0xFFFFFFFFu ? thr_pre : gp(S.thr)[(size_t)pattern * S.n_ch + c];
+    return bernoulli_hash(S.seed, t, S.hash_offset + c) < thr;
+}
+
+// thr_pre as in source_spike; vp_pre as in block_segment_write.
 template <uint32_t NT = kBlock, bool COHERENT = false, bool COPY = false>
 __device__ __forceinline__ uint32_t source_step(const NetDev *__restrict__ net, uint32_t s, uint32_t local_block,
                                                 uint64_t t, uint32_t *seg_count, uint32_t thr_pre = 0xFFFFFFFFu,
                                                 const uint32_t *vp_pre = nullptr, const SourceDev *S_copy = nullptr) {
     const SourceDev &S = entry_of<COPY>(net->srcs, s, S_copy);
     const uint32_t c = local_block * kBlock + threadIdx.x;
-    const uint32_t phase = (uint32_t)(t % S.steps_per_pattern);
-    bool spike = false;
-    if (threadIdx.x < (uint32_t)kBlock && c < S.n_ch && phase >= S.lo && phase <= S.hi) {
-        const uint32_t pattern = (uint32_t)((t / S.steps_per_pattern) % S.n_patterns);
-        const uint32_t thr = thr_pre != 0xFFFFFFFFu ? thr_pre : gp(S.thr)[(size_t)pattern * S.n_ch + c];
-        spike = bernoulli_hash(S.seed, t, S.hash_offset + c) < thr;
-    }
+    const bool spike = source_spike(S, c, threadIdx.x < (uint32_t)kBlock, t, thr_pre);
     return block_segment_write<NT, COHERENT>(
         net, t, spike, S.first + c, S.list + (size_t)(t & 1) * net->list_pstride + (size_t)local_block * kSegStride,
         S.seg_base + local_block, vp_pre);
@@ -823,28 +839,81 @@ __device__ __forceinline__ void phase_cache(const NetDev *__restrict__ net, cons
     __syncthreads();
 }
 
-// Phase 1 of step t1 for a persistent workgroup: its tile's neurons (a = exact sum of the impacts arriving
-// at t1) and its source blocks. KIND: the population kind of every tile of the launch, or kMixedKinds (the
-// kind is read per tile; a larger kernel).
+// Population kind of every tile of a persistent launch, or kMixedKinds (the kind is read per tile; a larger
+// kernel).
 constexpr uint32_t kMixedKinds = 0xFF;
+
+// Compaction of the two 256-thread halves of a 512-thread workgroup into two segments at once: threads
+// 0-255 into seg_a, threads 256-511 into seg_b (null: that half has no candidates), each in ascending order,
+// with one pair of barriers. Returns the rows published (threads 0 and 256).
+template <bool COHERENT>
+__device__ __forceinline__ uint32_t dual_segment_write(const NetDev *__restrict__ net, uint64_t t, bool pred,
+                                                       uint32_t value, uint32_t *seg_a, uint32_t sa,
+                                                       const uint32_t *vp_a, uint32_t *seg_b, uint32_t sb,
+                                                       const uint32_t *vp_b) {
+    __shared__ uint32_t wcnt[kTileThreads / 64];
+    const uint64_t mask = __ballot(pred);
+    const uint32_t lane = lane_id(), w = threadIdx.x / 64;
+    if (lane == 0) wcnt[w] = (uint32_t)__popcll(mask);
+    __syncthreads();
+    const bool upper = w >= 4;
+    const uint32_t w0 = upper ? 4u : 0u;
+    uint32_t off = 0, total = 0;
+    for (uint32_t k = w0; k < w0 + 4; ++k) {
+        if (k < w) off += wcnt[k];
+        total += wcnt[k];
+    }
+    uint32_t *seg = upper ? seg_b : seg_a;
+    if (pred) st_u32<COHERENT>(&seg[1 + off + __popcll(mask & ((1ull << lane) - 1ull))], value);
+    uint32_t rows = 0;
+    const uint32_t h = threadIdx.x & (kBlock - 1);
+    if (h < 4 && seg != nullptr) {
+        const uint32_t sg = upper ? sb : sa;
+        const uint32_t *vp = upper ? vp_b : vp_a;
+        const size_t par = (size_t)(t & 1);
+        if (h == 0) {
+            st_u32<COHERENT>(&seg[0], total);
+            st_u32<COHERENT>(&net->seg_count[par * net->n_seg + sg], total);
+            for (uint32_t k = 0; k < 4; ++k) rows += vp[k] != 0xFFFFFFFFu ? total : 0u;
+        }
+        if (vp[h] != 0xFFFFFFFFu) st_u32<COHERENT>(&net->vcount[par * net->n_vseg + vp[h]], total);
+    }
+    __syncthreads();
+    return rows;
+}
+
+// Phase 1 of step t1 for a persistent workgroup with both halves busy: the tile's neurons on threads 0-255
+// and its first source block on threads 256-511, compacted together; further source blocks follow.
 template <uint32_t KIND>
-__device__ __forceinline__ uint32_t tile_phase1_cached(const NetDev *__restrict__ net, const TileCtx &c,
-                                                       const PhaseCache &pc, uint64_t t1, int64_t a) {
-    Ctl *ctl = &net->ctl[t1 & 1];
-    uint32_t *seg_count = net->seg_count + (size_t)(t1 & 1) * net->n_seg;
-    uint32_t rows;
-    if (KIND != kMixedKinds)
-        rows = population_step<KIND, kTileThreads, true, true, true>(net, c.td.pop, c.td.tile, t1, ctl, seg_count, a,
-                                                                     pc.pop_vp, &pc.pop);
-    else if (c.kind == POP_LEAN)
-        rows = population_step<POP_LEAN, kTileThreads, true, true, true>(net, c.td.pop, c.td.tile, t1, ctl, seg_count,
-                                                                         a, pc.pop_vp, &pc.pop);
-    else
-        rows = population_step<POP_FULL, kTileThreads, true, true, true>(net, c.td.pop, c.td.tile, t1, ctl, seg_count,
-                                                                         a, pc.pop_vp, &pc.pop);
+__device__ __forceinline__ uint32_t tile_phase1_dual(const NetDev *__restrict__ net, const TileCtx &c,
+                                                     const PhaseCache &pc, uint64_t t1, int64_t a) {
+    const uint32_t tid = threadIdx.x;
+    const size_t par_off = (size_t)(t1 & 1) * net->list_pstride;
     const uint32_t n_src = pc.n_src;
-    for (uint32_t j = 0; j < n_src; ++j) {
-        const uint32_t thr = threadIdx.x < (uint32_t)kBlock ? pc.thr[j][threadIdx.x] : 0xFFFFFFFFu;
+    bool pred = false;
+    uint32_t value = 0;
+    if (tid < (uint32_t)kBlock) {
+        const uint32_t i = c.tile_lo + tid;
+        bool cand;
+        if (KIND == POP_LEAN || (KIND == kMixedKinds && c.kind == POP_LEAN))
+            pred = population_update<POP_LEAN, true>(net, pc.pop, i, i < c.n_pop, t1, a, cand);
+        else
+            pred = population_update<POP_FULL, true>(net, pc.pop, i, i < c.n_pop, t1, a, cand);
+        value = i;
+    } else if (n_src > 0) {
+        const uint32_t ch = pc.blk[0] * kBlock + (tid - kBlock);
+        pred = source_spike(pc.srcd[0], ch, true, t1, pc.thr[0][tid - kBlock]);
+        value = pc.srcd[0].first + ch;
+    }
+    uint32_t *seg_a = pc.pop.spike_list + par_off + (size_t)c.td.tile * kSegStride;
+    uint32_t *seg_b = n_src > 0 ? pc.srcd[0].list + par_off + (size_t)pc.blk[0] * kSegStride : nullptr;
+    const uint32_t sb = n_src > 0 ? pc.srcd[0].seg_base + pc.blk[0] : 0u;
+    uint32_t rows = dual_segment_write<true>(net, t1, pred, value, seg_a, pc.pop.seg_base + c.td.tile, pc.pop_vp,
+                                             seg_b, sb, pc.vp[0]);
+    const uint64_t mask = __ballot(pred);
+    if (tid < (uint32_t)kBlock) wave_bitmap(mask, c.tile_lo + (tid & ~63u), pc.pop, t1);
+    for (uint32_t j = 1; j < n_src; ++j) {
+        const uint32_t thr = tid < (uint32_t)kBlock ? pc.thr[j][tid] : 0xFFFFFFFFu;
         rows += source_step<kTileThreads, true, true>(net, pc.src[j], pc.blk[j], t1, nullptr, thr, pc.vp[j], &pc.srcd[j]);
     }
     return rows;
@@ -1027,7 +1096,7 @@ __global__ __launch_bounds__(kTileThreads) void k_tile_run(const NetDev *__restr
             a = (int64_t)c.ring[s0 * kBlock + tid];
             c.ring[s0 * kBlock + tid] = 0;
         }
-        my_rows += tile_phase1_cached<KIND>(net, c, pc, t0, a);
+        my_rows += tile_phase1_dual<KIND>(net, c, pc, t0, a);
         alive = grid_barrier(bar, net->n_tiles, ++barriers);
     }
     for (uint32_t k = 0; k < n && alive; ++k) {
@@ -1041,7 +1110,7 @@ __global__ __launch_bounds__(kTileThreads) void k_tile_run(const NetDev *__restr
                 a = (int64_t)c.ring[s1 * kBlock + tid];
                 c.ring[s1 * kBlock + tid] = 0;
             }
-            my_rows += tile_phase1_cached<KIND>(net, c, pc, t + 1, a);
+            my_rows += tile_phase1_dual<KIND>(net, c, pc, t + 1, a);
             alive = grid_barrier(bar, net->n_tiles, ++barriers);
         }
     }
@@ -1053,7 +1122,7 @@ __global__ __launch_bounds__(kTileThreads) void k_tile_run(const NetDev *__restr
         if (i < c.n_pop) gp(c.acc_v)[(size_t)(cc / kBlock) * c.n_pop + i] = (int64_t)c.ring[cc];
     }
     tile_count(sh, my_ev, my_parts, my_cells);
-    if (threadIdx.x == 0) atomicAdd(&sh.cnt[3], my_rows);
+    if (my_rows) atomicAdd(&sh.cnt[3], my_rows);
     tile_flush_counts(net, sh);
 }
 
