@@ -193,10 +193,12 @@ struct TileDev {
     uint32_t pop, tile, vbeg, vend;
 };
 
-// A spike segment that may reach a tile: its list, the index of its count in vcount, its projection.
+// A spike segment that may reach a tile: its list and its projection (with kTileSegChannel set for a
+// channel segment, whose messages the host pushes).
+constexpr uint32_t kTileSegChannel = 0x80;
+static_assert(kMaxProjs <= (int)kTileSegChannel / 2, "projection indexes stay below the channel and row flags");
 struct TileSeg {
     const uint32_t *list;
-    uint32_t vpos;
     uint32_t proj;
 };
 
@@ -229,7 +231,7 @@ struct NetDev {
     uint32_t tile_max_list;    // longest such range
     uint32_t tile_max_d;       // deepest ring of a tiled population
     uint32_t n_src_blocks;     // workgroups of the Bernoulli sources (256 channels each)
-    uint32_t *grid_bar;        // [2] barrier counter and abort flag of persistent launches
+    uint32_t *grid_bar;        // abort flag of persistent launches (word kBarAbort)
     unsigned long long *tile_part;  // [4 * n_tiles] impacts, rows, row segments, ring cells updated
     uint32_t blk_proj[64];     // projection index of each projection with BLOCKING synapses
     uint32_t n_dense;

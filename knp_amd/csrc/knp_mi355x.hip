@@ -998,7 +998,10 @@ int init_network(knp_net *net) {
             for (uint32_t x = 0; x < nt; ++x) {
                 tiles.push_back(TileDev{p, x, (uint32_t)tile_seg.size(), (uint32_t)(tile_seg.size() + reach[x].size())});
                 maxL = std::max(maxL, (uint32_t)reach[x].size());
-                for (uint32_t k : reach[x]) tile_seg.push_back(TileSeg{seg_list[vd[k].seg], vidx[k], vd[k].q});
+                for (uint32_t k : reach[x]) {
+                    const bool chan = vd[k].seg == T.projs[vd[k].q].push_seg;
+                    tile_seg.push_back(TileSeg{seg_list[vd[k].seg], vd[k].q | (chan ? kTileSegChannel : 0u)});
+                }
             }
         }
         T.n_tiles = (uint32_t)tiles.size();
@@ -1239,8 +1242,8 @@ int run_steps(knp_net *net, uint64_t n_steps) {
             int rc = deliver_pushes(net);
             if (rc) return rc;
             if ((rc = launch_dense_due(t))) return rc;
-            // counters of the grid barrier grow by one per step and workgroup: 2^22 steps per launch at most
-            uint64_t len = std::min<uint64_t>(n_steps - done, uint64_t(1) << 22);
+            // 2^16 steps per launch at most: the launch's per-thread impact counters are 32-bit
+            uint64_t len = std::min<uint64_t>(n_steps - done, uint64_t(1) << 16);
             for (const auto &Dh : net->dense) {
                 const uint64_t t_first = net->pops[Dh.post_pop].t_first;
                 const uint64_t next = t_first + ((t - t_first) / Dh.T + 1) * Dh.T;
@@ -1264,7 +1267,7 @@ int run_steps(knp_net *net, uint64_t n_steps) {
         uint32_t flag = 0;
         HIP_TRY(hipMemcpyAsync(&flag, net->table.grid_bar + kBarAbort, 4, hipMemcpyDeviceToHost, net->stream));
         HIP_TRY(hipStreamSynchronize(net->stream));
-        if (flag) return fail(KNP_ERR_HIP, "persistent launch: a grid barrier timed out (state is undefined)");
+        if (flag) return fail(KNP_ERR_HIP, "persistent launch: a wait for another workgroup's spikes timed out (state is undefined)");
         if (net->timing && net->ev_used > 8192) {
             int rc = collect_timing(net);
             if (rc) return rc;

@@ -91,6 +91,21 @@ __device__ __forceinline__ uint32_t ld_u32(const uint32_t *p) {
     return *gp(p);
 }
 
+// Word 0 of a population or source segment: the count (at most 256) in the low kStampShift bits; entries
+// from word 1. In a persistent launch, where the readers of a segment run in other workgroups while it is
+// written, the writer stores word 0 only once the entries have reached memory, with the step's stamp above
+// the count (the step modulo 2^20 - 1, plus 1: never 0, the stamp of a plain count, and different for steps
+// t and t - 2, which share the segment), so a reader that sees the stamp of step t also reads complete
+// entries. Channel segments hold the host's plain counts.
+constexpr uint32_t kStampShift = 12, kCountMask = (1u << kStampShift) - 1u;
+__device__ __forceinline__ uint32_t list_stamp(uint64_t t) { return (uint32_t)(t % 0xFFFFFull) + 1u; }
+
+// Bounded waits of a persistent launch: after kSpinLimit polls, or once another workgroup has given up, a
+// wait ends with a failure, raises the abort flag (word kBarAbort of grid_bar) and the launch stops (the
+// host reports the failure), so no wait lasts forever.
+constexpr uint32_t kSpinLimit = 1u << 20;
+constexpr uint32_t kBarAbort = 0, kBarWords = 32;  // words of grid_bar: the abort flag
+
 // Ordered compaction of a workgroup's 256 candidates into its own segment (count in word 0, indexes from
 // word 1): ascending order, no global atomics. NT = threads of the workgroup (threads from 256 on carry no
 // candidate).
@@ -111,11 +126,15 @@ __device__ __forceinline__ uint32_t block_segment_write(const NetDev *__restrict
         total += wcnt[k];
     }
     if (pred) st_u32<COHERENT>(&segment[1 + off + __popcll(mask & ((1ull << lane) - 1ull))], value);
+    if (COHERENT) {
+        __builtin_amdgcn_s_waitcnt(0);  // the entries have reached memory before the stamped count
+        __syncthreads();
+    }
     uint32_t rows = 0;
     if (threadIdx.x < 4) {
         const size_t par = (size_t)(t & 1);
         if (threadIdx.x == 0) {
-            st_u32<COHERENT>(&segment[0], total);
+            st_u32<COHERENT>(&segment[0], COHERENT ? total | list_stamp(t) << kStampShift : total);
             st_u32<COHERENT>(&net->seg_count[par * net->n_seg + seg], total);
         }
         uint32_t vps[4];
@@ -544,8 +563,7 @@ struct TileProj {
 // their spike lists [tile_max_list] (pointers, parity 0), count positions [tile_max_list], row-count prefix
 // [tile_max_list + 1], first three spike indexes [tile_max_list][3] and projection [tile_max_list].
 __host__ __device__ inline size_t tile_lds_bytes(uint32_t max_d, uint32_t max_list) {
-    return (size_t)max_d * kBlock * 8 + (size_t)max_list * 8 + (size_t)max_list * 4 + ((size_t)max_list + 1) * 4 +
-           (size_t)max_list * 12 + max_list;
+    return (size_t)max_d * kBlock * 8 + (size_t)max_list * 8 + ((size_t)max_list + 1) * 4 + max_list;
 }
 
 // State of one tile workgroup: its tile, its population's fields (in registers: stores through the ring
@@ -557,28 +575,38 @@ struct TileCtx {
     float scale;
     unsigned long long *ring;
     const uint32_t **vlist;
-    uint32_t *vpos, *vpfx;
+    uint32_t *vpfx;  // prefix of the segments' counts this step
     uint8_t *vproj;
-    uint32_t *vstash;  // [tile_max_list][3]: the first 3 spike indexes of every segment this step
     uint32_t nv;
 };
 
-// Shared-memory scratch of the tile kernels.
+// Shared-memory scratch of the tile kernels. One pass of the gather takes kGatherRows rows (one per thread);
+// the segment entries stashed by the count stage (kStash per segment) fill the row tables of the first pass.
+constexpr uint32_t kGatherRows = kTileThreads, kStash = 3;
+constexpr uint32_t kRowLoad = 0x80;  // rq flag: the row's entry is loaded in stage 2 (rval = its segment)
 struct TileShared {
-    uint32_t rbeg[kTileThreads];  // CSR position of each row's part in the tile
-    uint32_t rpfx[kTileThreads];  // exclusive prefix of the parts' lengths
-    uint8_t rproj[kTileThreads];  // projection of each row
+    uint32_t rval[kGatherRows];  // stage 2: presynaptic index (or segment) of each row; 3b: CSR position
+    uint32_t rpfx[kGatherRows];  // 3b: exclusive prefix of the long parts' lengths
+    uint8_t rq[kGatherRows];     // projection of each row (| kRowLoad)
     uint32_t wsum[kTileThreads / 64];
     unsigned long long cnt[4];
     TileProj tp[kMaxProjs];
 };
+
+// The tile of workgroup b. Workgroups are dispatched to the 8 XCDs in turn (workgroup b on XCD b mod 8), so
+// the tiles are dealt out in 8 contiguous ranges, one per XCD: neighbouring tiles read neighbouring
+// directory entries and row parts (rows are sorted by postsynaptic neuron), which then share the XCD's L2.
+__device__ __forceinline__ uint32_t tile_of_block(uint32_t b, uint32_t n_tiles) {
+    const uint32_t q = n_tiles / 8, r = n_tiles % 8, x = b % 8;
+    return x * q + min(x, r) + b / 8;
+}
 
 // Loads the tile's description, the projection pointers and the list of segments that can reach the tile.
 __device__ __forceinline__ TileCtx tile_setup(const NetDev *__restrict__ net, unsigned long long *dyn,
                                               TileShared &sh) {
     TileCtx c;
     {
-        const KNP_GLOBAL TileDev *tdp = gp(net->tiles) + blockIdx.x;
+        const KNP_GLOBAL TileDev *tdp = gp(net->tiles) + tile_of_block(blockIdx.x, gridDim.x);
         c.td.pop = tdp->pop;
         c.td.tile = tdp->tile;
         c.td.vbeg = tdp->vbeg;
@@ -594,10 +622,8 @@ __device__ __forceinline__ TileCtx tile_setup(const NetDev *__restrict__ net, un
     const uint32_t L = net->tile_max_list;
     c.ring = dyn;
     c.vlist = reinterpret_cast<const uint32_t **>(dyn + (size_t)net->tile_max_d * kBlock);
-    c.vpos = reinterpret_cast<uint32_t *>(c.vlist + L);
-    c.vpfx = c.vpos + L;
-    c.vstash = c.vpfx + L + 1;
-    c.vproj = reinterpret_cast<uint8_t *>(c.vstash + 3 * L);
+    c.vpfx = reinterpret_cast<uint32_t *>(c.vlist + L);
+    c.vproj = reinterpret_cast<uint8_t *>(c.vpfx + L + 1);
     c.nv = c.td.vend - c.td.vbeg;
     const uint32_t tid = threadIdx.x;
     if (tid < 4) sh.cnt[tid] = 0;
@@ -609,7 +635,6 @@ __device__ __forceinline__ TileCtx tile_setup(const NetDev *__restrict__ net, un
     for (uint32_t v = tid; v < c.nv; v += kTileThreads) {
         const KNP_GLOBAL TileSeg *e = gp(ts) + v;
         c.vlist[v] = e->list;
-        c.vpos[v] = e->vpos;
         c.vproj[v] = (uint8_t)e->proj;
     }
     return c;
@@ -617,42 +642,73 @@ __device__ __forceinline__ TileCtx tile_setup(const NetDev *__restrict__ net, un
 
 // Stages 1-3 of the tile scatter of step t: numbers the spiking rows that may reach the tile (prefix over
 // the counts of its segments), finds every row's part in the tile, and adds the part's impacts into the LDS
-// ring with LDS atomics. Ends with a workgroup barrier. Returns impacts (ev) and row parts (parts) added.
+// ring with LDS atomics. Ends with a workgroup barrier. Adds the impacts (ev) and row parts (parts).
+// COHERENT (persistent launch): the population and source segments of step t are written by other
+// workgroups during the launch, and each is read once its stamp says step t is complete; channel segments
+// are read only when `channels` (the launch's first step: the host pushes channel messages before the
+// launch). Returns false if a wait ended without the stamp (the launch is being aborted).
 template <bool COHERENT>
-__device__ __forceinline__ void tile_gather(const NetDev *__restrict__ net, const TileCtx &c, TileShared &sh,
-                                            uint64_t t, uint32_t &my_ev, uint32_t &my_parts) {
+__device__ __forceinline__ bool tile_gather(const NetDev *__restrict__ net, const TileCtx &c, TileShared &sh,
+                                            uint64_t t, uint32_t &my_ev, uint32_t &my_parts, bool channels = true) {
     const uint32_t tid = threadIdx.x;
     const size_t par_off = (size_t)(t & 1) * net->list_pstride;
-    // the pool as a buffer: one 16-byte load gives a segment's count and first three spike indexes
-    // (device-scope in a persistent launch, where other workgroups wrote them during this launch)
+    // the pool as a buffer: one 16-byte load gives a segment's count and first kStash entries (device-scope
+    // in a persistent launch)
     const __amdgpu_buffer_rsrc_t pool = __builtin_amdgcn_make_buffer_rsrc(
         const_cast<uint32_t *>(net->list_pool), 0, (int)(2u * net->list_pstride * 4u), 0x00020000);
+    constexpr int kAux = COHERENT ? 16 : 0;
+    const uint32_t stamp = list_stamp(t);
+    bool ok = true;
     uint32_t total = 0;
+    // 1. counts and stashed entries of every segment; the rows of the first pass into the row tables (the
+    // row's presynaptic index and projection, or kRowLoad and its segment when the entry is not stashed)
     for (uint32_t base = 0; base < c.nv; base += kTileThreads) {
         const uint32_t v = base + tid;
-        uint32_t cv = 0;
-        if (v < c.nv) {
+        uint32_t cv = 0, w[kStash];
+        const bool chan = v < c.nv && (c.vproj[v] & kTileSegChannel) != 0;
+        const bool stamped = COHERENT && !chan;
+        if (v < c.nv && (!COHERENT || !chan || channels)) {
             const uint32_t off = (uint32_t)((const char *)(c.vlist[v] + par_off) - (const char *)net->list_pool);
-            const v4u h = __builtin_amdgcn_raw_buffer_load_b128(pool, off, 0, COHERENT ? 16 : 0);
-            cv = h.x;
-            c.vstash[3 * v + 0] = h.y;
-            c.vstash[3 * v + 1] = h.z;
-            c.vstash[3 * v + 2] = h.w;
+            v4u h = __builtin_amdgcn_raw_buffer_load_b128(pool, off, 0, kAux);
+            if (stamped) {
+                uint32_t spins = 0;
+                while ((h.x >> kStampShift) != stamp) {  // the writer has not published step t yet
+                    __builtin_amdgcn_s_sleep(1);
+                    if ((++spins & 63u) == 0 &&
+                        (spins > kSpinLimit || ld_u32<true>(net->grid_bar + kBarAbort) != 0)) {
+                        st_u32<true>(net->grid_bar + kBarAbort, 1u);
+                        ok = false;
+                        break;
+                    }
+                    h = __builtin_amdgcn_raw_buffer_load_b128(pool, off, 0, kAux);
+                }
+            }
+            cv = chan ? h.x : h.x & kCountMask;
+            w[0] = h.y, w[1] = h.z, w[2] = h.w;
         }
+        if (!ok) cv = 0;
         uint32_t tot;
         const uint32_t ex = block_excl_scan<kTileThreads>(cv, sh.wsum, &tot);
-        if (v < c.nv) c.vpfx[v] = total + ex;
+        if (v < c.nv) {
+            const uint32_t r0 = total + ex, q = c.vproj[v] & (kTileSegChannel - 1u);
+            c.vpfx[v] = r0;
+            const uint32_t n = r0 < kGatherRows ? min(cv, kGatherRows - r0) : 0u;
+#pragma unroll
+            for (uint32_t k = 0; k < kStash; ++k)
+                if (k < n) {
+                    sh.rval[r0 + k] = w[k];
+                    sh.rq[r0 + k] = (uint8_t)q;
+                }
+            for (uint32_t k = kStash; k < n; ++k) {
+                sh.rval[r0 + k] = v;
+                sh.rq[r0 + k] = (uint8_t)(q | kRowLoad);
+            }
+        }
         total += tot;
     }
-    __syncthreads();
-    // rows -> segment map for one pass of rows (instead of a binary search per row): sh.rbeg serves as the
-    // map until the long-part pass below reuses it
-    const bool mapped = total <= (uint32_t)kTileThreads;
-    if (mapped) {
-        for (uint32_t v = tid; v < c.nv; v += kTileThreads) {
-            const uint32_t r0 = c.vpfx[v], r1 = v + 1 < c.nv ? c.vpfx[v + 1] : total;
-            for (uint32_t r = r0; r < r1; ++r) sh.rbeg[r] = v;
-        }
+    if (COHERENT) {
+        if (!__syncthreads_and(ok)) return false;
+    } else {
         __syncthreads();
     }
     const uint32_t slot0 = (uint32_t)(t % c.D);
@@ -667,24 +723,28 @@ __device__ __forceinline__ void tile_gather(const NetDev *__restrict__ net, cons
         atomicAdd(&c.ring[slot * kBlock + j], (unsigned long long)v);
         return true;
     };
-    for (uint32_t rb = 0; rb < total; rb += kTileThreads) {
-        // 2. one thread per row: where the row's part for this tile lies
+    for (uint32_t rb = 0; rb < total; rb += kGatherRows) {
+        // 2. one thread per row: its presynaptic index and projection, and where its part for this tile lies
         const uint32_t row = rb + tid;
-        uint32_t b = 0, len = 0, q = 0;
+        uint32_t b = 0, len = 0, q = 0, pre = 0xFFFFFFFFu;
         if (row < total) {
-            uint32_t lo = 0, hi = c.nv;  // largest v with vpfx[v] <= row
-            if (mapped) {
-                lo = sh.rbeg[row];
+            uint32_t lo = 0xFFFFFFFFu;  // the row's segment when its entry is loaded here
+            if (row < kGatherRows) {
+                const uint32_t rq = sh.rq[row];
+                q = rq & (kTileSegChannel - 1u);
+                if (rq & kRowLoad) lo = sh.rval[row];
+                else pre = sh.rval[row];
             } else {
+                uint32_t hi = c.nv;  // largest v with vpfx[v] <= row
+                lo = 0;
                 while (hi - lo > 1) {
                     const uint32_t mid = (lo + hi) >> 1;
                     if (c.vpfx[mid] <= row) lo = mid;
                     else hi = mid;
                 }
+                q = c.vproj[lo] & (kTileSegChannel - 1u);
             }
-            q = c.vproj[lo];
-            const uint32_t idx = row - c.vpfx[lo];
-            const uint32_t pre = idx < 3 ? c.vstash[3 * lo + idx] : ld_u32<COHERENT>(c.vlist[lo] + par_off + 1 + idx);
+            if (lo != 0xFFFFFFFFu) pre = ld_u32<COHERENT>(c.vlist[lo] + par_off + 1 + (row - c.vpfx[lo]));
             const TileProj &Q = sh.tp[q];
             if (pre < Q.n_pre) {
                 if (Q.tile_dir) {
@@ -722,14 +782,14 @@ __device__ __forceinline__ void tile_gather(const NetDev *__restrict__ net, cons
                     if (k0 + u < len && add_record(rec[u])) ++my_ev;
             }
         }
-        // 3b. long parts: spread over the workgroup, one thread per synapse
+        // 3b. long parts: spread over the workgroup, one thread per synapse (the row tables are free now)
         const uint32_t llen = len > kTileShortPart ? len : 0u;
         if (__syncthreads_or(llen != 0)) {
             uint32_t n_syn;
             const uint32_t ex = block_excl_scan<kTileThreads>(llen, sh.wsum, &n_syn);
-            sh.rbeg[tid] = b;
+            sh.rval[tid] = b;
             sh.rpfx[tid] = ex;
-            sh.rproj[tid] = (uint8_t)q;
+            sh.rq[tid] = (uint8_t)q;
             __syncthreads();
             const uint32_t nrows = min((uint32_t)kTileThreads, total - rb);
             for (uint32_t s = tid; s < n_syn; s += kTileThreads) {
@@ -739,11 +799,12 @@ __device__ __forceinline__ void tile_gather(const NetDev *__restrict__ net, cons
                     if (sh.rpfx[mid] <= s) lo = mid;
                     else hi = mid;
                 }
-                if (add_record(gp(sh.tp[sh.rproj[lo]].syn8)[sh.rbeg[lo] + (s - sh.rpfx[lo])])) ++my_ev;
+                if (add_record(gp(sh.tp[sh.rq[lo]].syn8)[sh.rval[lo] + (s - sh.rpfx[lo])])) ++my_ev;
             }
         }
         __syncthreads();
     }
+    return true;
 }
 
 // Channel segments of step t + 1 start empty (the host writes the counts of pushed messages before that
@@ -865,6 +926,10 @@ __device__ __forceinline__ uint32_t dual_segment_write(const NetDev *__restrict_
     }
     uint32_t *seg = upper ? seg_b : seg_a;
     if (pred) st_u32<COHERENT>(&seg[1 + off + __popcll(mask & ((1ull << lane) - 1ull))], value);
+    if (COHERENT) {
+        __builtin_amdgcn_s_waitcnt(0);  // the entries have reached memory before the stamped counts
+        __syncthreads();
+    }
     uint32_t rows = 0;
     const uint32_t h = threadIdx.x & (kBlock - 1);
     if (h < 4 && seg != nullptr) {
@@ -872,7 +937,7 @@ __device__ __forceinline__ uint32_t dual_segment_write(const NetDev *__restrict_
         const uint32_t *vp = upper ? vp_b : vp_a;
         const size_t par = (size_t)(t & 1);
         if (h == 0) {
-            st_u32<COHERENT>(&seg[0], total);
+            st_u32<COHERENT>(&seg[0], COHERENT ? total | list_stamp(t) << kStampShift : total);
             st_u32<COHERENT>(&net->seg_count[par * net->n_seg + sg], total);
             for (uint32_t k = 0; k < 4; ++k) rows += vp[k] != 0xFFFFFFFFu ? total : 0u;
         }
@@ -1010,61 +1075,18 @@ __global__ __launch_bounds__(kTileThreads) void k_tile_scatter(const NetDev *__r
     }
 }
 
-// Grid-wide barrier of a persistent launch. Workgroups arrive on one of kBarGroups group counters (block
-// index mod kBarGroups, each counter on its own 128-byte line, so arrivals do not queue on one address);
-// the last arrival of a group counts on the top counter, and the last group publishes the barrier's number
-// in the generation word that every workgroup polls. Counters only grow within a launch (the host clears
-// them before each launch). Before arriving, every thread waits for its stores to complete, so data written
-// (through to memory) before the barrier is visible to device-scope loads after it. The wait is bounded:
-// after kSpinLimit polls, or when another workgroup has given up, the workgroup returns false (the launch
-// then stops and the host reports the failure), so no wait lasts forever.
-constexpr uint32_t kSpinLimit = 1u << 20;
-constexpr uint32_t kBarGroups = 8;
-// word offsets in grid_bar: generation, abort flag, group counters, top counter
-constexpr uint32_t kBarGen = 0, kBarAbort = 1, kBarGroup0 = 32, kBarTop = kBarGroup0 + 32 * kBarGroups;
-constexpr uint32_t kBarWords = kBarTop + 32;
-__device__ __forceinline__ bool grid_barrier(uint32_t *bar, uint32_t n_blocks, uint32_t j) {
-    __shared__ uint32_t ok;
-    __builtin_amdgcn_s_waitcnt(0);  // this wave's stores are complete
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        KNP_GLOBAL uint32_t *gb = gp(bar);
-        const uint32_t g = blockIdx.x % kBarGroups;
-        const uint32_t m = n_blocks / kBarGroups + (g < n_blocks % kBarGroups ? 1u : 0u);
-        const uint32_t groups = n_blocks < kBarGroups ? n_blocks : kBarGroups;
-        const uint32_t old = __hip_atomic_fetch_add(gb + kBarGroup0 + 32 * g, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        if (old + 1 == m * j) {
-            const uint32_t old2 = __hip_atomic_fetch_add(gb + kBarTop, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            if (old2 + 1 == groups * j) __hip_atomic_store(gb + kBarGen, j, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        }
-        uint32_t spins = 0;
-        bool good = true;
-        while (__hip_atomic_load(gb + kBarGen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < j) {
-            __builtin_amdgcn_s_sleep(1);
-            if ((++spins & 63u) == 0 &&
-                (spins > kSpinLimit || __hip_atomic_load(gb + kBarAbort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0)) {
-                good = false;
-                __hip_atomic_store(gb + kBarAbort, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                break;
-            }
-        }
-        ok = good ? 1u : 0u;
-    }
-    __syncthreads();
-    return ok != 0;
-}
-
 // Persistent run of steps t0 .. t0 + n - 1 when every population is tiled and all tiles fit on the device at
 // once (cooperative launch): the ring lives in LDS for the whole launch (loaded from HBM at the start,
-// stored back at the end), and one grid barrier per step separates phase 1 of step t (spike lists and
-// counts, written through to memory) from the scatter of step t that reads them.
+// stored back at the end). There is no grid barrier: the scatter of step t waits for the stamp of step t on
+// each segment it reads (written through to memory by phase 1 of t in the segment's workgroup). A
+// workgroup overwrites the segments of step t (same parity) at phase 1 of t + 2, after its own scatter of
+// t + 1 has seen every segment of t + 1, whose writers had finished reading step t before publishing them.
 template <uint32_t KIND>
 __global__ __launch_bounds__(kTileThreads) void k_tile_run(const NetDev *__restrict__ net, uint64_t t0, uint32_t n) {
     extern __shared__ unsigned long long tacc[];
     __shared__ TileShared sh;
     const TileCtx c = tile_setup(net, tacc, sh);
     const uint32_t tid = threadIdx.x;
-    uint32_t *bar = net->grid_bar;
     // the ring of the tile from HBM
     for (uint32_t cc = tid; cc < c.D * kBlock; cc += kTileThreads) {
         const uint32_t i = c.tile_lo + (cc % kBlock);
@@ -1075,12 +1097,11 @@ __global__ __launch_bounds__(kTileThreads) void k_tile_run(const NetDev *__restr
     phase_cache(net, c, pc);
     uint32_t my_ev = 0, my_parts = 0;
     unsigned long long my_rows = 0;  // thread 0: rows published by this workgroup's phase 1
-    uint32_t barriers = 0;
-    bool alive = true;
     if (blockIdx.x == 0) {
         // Channel messages exist only for t0 (pushed by the host before the launch): their rows are counted
         // here, and the channel counts of t0 + 1 are cleared now, those of t0 (parity of t0 + 2) once the
-        // scatter of t0 is done everywhere (step t0 + 1 below).
+        // scatter of t0 is done everywhere (after this workgroup has seen every segment of t0 + 1); the
+        // launch reads no channel segment after t0, so the cleared counts are for the steps that follow it.
         if (threadIdx.x == 0)
             for (uint32_t q = 0; q < net->n_projs; ++q) {
                 const uint32_t vp = gp(net->seg_vpos)[4 * net->projs[q].push_seg];
@@ -1097,12 +1118,11 @@ __global__ __launch_bounds__(kTileThreads) void k_tile_run(const NetDev *__restr
             c.ring[s0 * kBlock + tid] = 0;
         }
         my_rows += tile_phase1_dual<KIND>(net, c, pc, t0, a);
-        alive = grid_barrier(bar, net->n_tiles, ++barriers);
     }
-    for (uint32_t k = 0; k < n && alive; ++k) {
+    for (uint32_t k = 0; k < n; ++k) {
         const uint64_t t = t0 + k;
+        if (!tile_gather<true>(net, c, sh, t, my_ev, my_parts, k == 0)) break;
         if (k == 1 && blockIdx.x == 0) clear_push_counts<true>(net, t0);
-        tile_gather<true>(net, c, sh, t, my_ev, my_parts);
         if (k + 1 < n) {
             const uint32_t s1 = (uint32_t)((t + 1) % c.D);
             int64_t a = 0;
@@ -1111,7 +1131,6 @@ __global__ __launch_bounds__(kTileThreads) void k_tile_run(const NetDev *__restr
                 c.ring[s1 * kBlock + tid] = 0;
             }
             my_rows += tile_phase1_dual<KIND>(net, c, pc, t + 1, a);
-            alive = grid_barrier(bar, net->n_tiles, ++barriers);
         }
     }
     // the ring back to HBM
