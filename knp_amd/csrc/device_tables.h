@@ -168,12 +168,15 @@ struct TaskDev {
 // Weights are stored in MFMA fragment order: for tile c (32 postsynaptic neurons of one instance) and K-step
 // kk (16 inputs), 64 lanes x (8 fp8 | 8 bf16) values, lane l holding rows j0 + (l % 32), inputs
 // 16 kk + 8 (l / 32) + 0..7.
+// Row blocks (32 steps each) of a dense projection's time block that one wave multiplies with the same
+// weight fragments.
+constexpr uint32_t kDenseGroup = 4;
 struct DenseDev {
     uint32_t format;       // 0 fp8 e4m3, 1 bf16
     uint32_t post_pop;
     uint32_t n_in, ksteps;  // inputs, ceil(n_in / 16)
     uint32_t n_inst, n_tiles, T;  // instances, 32-column tiles, steps per time block
-    uint32_t words;        // spike-bit words per (step, instance): 2 * ksteps (16 bits per K-step)
+    uint32_t pitch;        // bytes per (step, instance) row of spike bits: 2 * ksteps rounded up to 16
     uint32_t threshold, hash_offset;
     uint64_t seed;
     const uint8_t *frag;   // [n_tiles][ksteps][64][8 or 16 bytes]
@@ -182,7 +185,7 @@ struct DenseDev {
     const uint32_t *tile_ncol;  // [n_tiles] valid columns (<= 32)
     const float *row_scale;     // [population size] (1 outside instances)
     const uint32_t *inst_post_count;  // [n_inst]
-    uint8_t *bits;         // [T][n_inst][2 ksteps] spike bits of the current block (bit b of byte = input 8 byte + b)
+    uint8_t *bits;         // [T][n_inst][pitch] spike bits of the current block (bit b of byte = input 8 byte + b)
     float *ff;             // [2T][population size] impacts by arrival step
 };
 

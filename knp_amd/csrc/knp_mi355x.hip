@@ -251,6 +251,7 @@ double dense_value(const HostDense &D, uint64_t row, uint32_t i) {
 }
 
 uint32_t dense_ksteps(const HostDense &D) { return (D.n_in + 15) / 16; }
+uint32_t dense_pitch(const HostDense &D) { return (2 * dense_ksteps(D) + 15) / 16 * 16; }
 
 int alloc_dense_buffers(knp_net *net, uint32_t d, uint32_t T) {
     DenseDev &Dn = net->table.dense[d];
@@ -262,7 +263,7 @@ int alloc_dense_buffers(knp_net *net, uint32_t d, uint32_t T) {
     Dn.ff = nullptr;
     Dn.T = T;
     int rc;
-    if ((rc = dev_alloc(net, (void **)&Dn.bits, (size_t)T * Dh.n_inst * 2 * dense_ksteps(Dh)))) return rc;
+    if ((rc = dev_alloc(net, (void **)&Dn.bits, (size_t)T * Dh.n_inst * dense_pitch(Dh)))) return rc;
     if ((rc = dev_alloc(net, (void **)&Dn.ff, (size_t)2 * T * n * sizeof(float)))) return rc;
     PopDev &P = net->table.pops[Dh.post_pop];
     P.ff = Dn.ff;
@@ -272,10 +273,12 @@ int alloc_dense_buffers(knp_net *net, uint32_t d, uint32_t T) {
 
 void launch_dense(knp_net *net, uint32_t d, uint64_t t0) {
     const DenseDev &Dn = net->table.dense[d];
-    const uint64_t bytes = (uint64_t)Dn.T * Dn.n_inst * 2 * Dn.ksteps;
+    const uint64_t bytes = (uint64_t)Dn.T * Dn.n_inst * Dn.pitch;
     const uint32_t g1 = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(8192, (bytes + kBlock - 1) / kBlock));
     k_dense_spikes<<<g1, kBlock, 0, net->stream>>>(net->dev_table, d, t0);
-    const uint32_t g2 = std::max(1u, std::min(4096u, (Dn.n_tiles + 3) / 4));
+    // one wave per (tile, group of kDenseGroup row blocks), four waves per workgroup
+    const uint64_t units = (uint64_t)Dn.n_tiles * ((Dn.T / 32 + kDenseGroup - 1) / kDenseGroup);
+    const uint32_t g2 = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(65536, (units + 3) / 4));
     if (Dn.format == 0) k_dense_mfma<0><<<g2, kBlock, 0, net->stream>>>(net->dev_table, d, t0);
     else k_dense_mfma<1><<<g2, kBlock, 0, net->stream>>>(net->dev_table, d, t0);
 }
@@ -397,7 +400,7 @@ int init_dense(knp_net *net, uint32_t d) {
     Dn.ksteps = ks;
     Dn.n_inst = Dh.n_inst;
     Dn.n_tiles = n_tiles;
-    Dn.words = 2 * ks;
+    Dn.pitch = dense_pitch(Dh);
     Dn.threshold = Dh.threshold;
     Dn.hash_offset = Dh.hash_offset;
     Dn.seed = Dh.seed;

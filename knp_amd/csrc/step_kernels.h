@@ -673,7 +673,7 @@ __device__ __forceinline__ bool tile_gather(const NetDev *__restrict__ net, cons
             if (stamped) {
                 uint32_t spins = 0;
                 while ((h.x >> kStampShift) != stamp) {  // the writer has not published step t yet
-                    __builtin_amdgcn_s_sleep(1);
+                    __builtin_amdgcn_s_sleep(16);  // polls are spaced out: frequent ones slow the stores
                     if ((++spins & 63u) == 0 &&
                         (spins > kSpinLimit || ld_u32<true>(net->grid_bar + kBarAbort) != 0)) {
                         st_u32<true>(net->grid_bar + kBarAbort, 1u);
@@ -1417,95 +1417,135 @@ __global__ __launch_bounds__(kBlock) void k_additive_modify(const NetDev *__rest
 typedef float v16f __attribute__((ext_vector_type(16)));
 typedef short v8s __attribute__((ext_vector_type(8)));
 
-// Spike bits of steps t0 .. t0+T-1: byte b of (step, instance) holds inputs 8b .. 8b+7.
+// Spike bits of steps t0 .. t0+T-1: byte b of (step, instance) holds inputs 8b .. 8b+7 (bytes from 2 ksteps
+// up to the row pitch are 0). Counts the dense synaptic events and rows of the block (one atomic pair per
+// wave).
 __global__ __launch_bounds__(kBlock) void k_dense_spikes(const NetDev *__restrict__ net, uint32_t d, uint64_t t0) {
     const DenseDev &Dn = net->dense[d];
-    const uint32_t bytes_per = 2u * Dn.ksteps;
-    const uint64_t total = (uint64_t)Dn.T * Dn.n_inst * bytes_per;
+    const uint32_t pitch = Dn.pitch, used = 2u * Dn.ksteps;
+    const uint64_t total = (uint64_t)Dn.T * Dn.n_inst * pitch;
     const uint64_t stride = (uint64_t)gridDim.x * kBlock;
-    for (uint64_t base = (uint64_t)blockIdx.x * kBlock; base < total; base += stride) {
-        const uint64_t idx = base + threadIdx.x;
+    unsigned long long ev = 0, rows = 0;
+    for (uint64_t idx = (uint64_t)blockIdx.x * kBlock + threadIdx.x; idx < total; idx += stride) {
+        const uint32_t byte = (uint32_t)(idx % pitch);
+        const uint64_t rest = idx / pitch;
+        const uint32_t inst = (uint32_t)(rest % Dn.n_inst);
+        const uint64_t t = t0 + rest / Dn.n_inst;
         uint32_t v = 0;
-        unsigned long long ev = 0, rows = 0;
-        if (idx < total) {
-            const uint32_t byte = (uint32_t)(idx % bytes_per);
-            const uint64_t rest = idx / bytes_per;
-            const uint32_t inst = (uint32_t)(rest % Dn.n_inst);
-            const uint64_t t = t0 + rest / Dn.n_inst;
+        if (byte < used)
             for (uint32_t b = 0; b < 8; ++b) {
                 const uint32_t i = byte * 8 + b;
                 if (i < Dn.n_in && bernoulli_hash(Dn.seed, t, Dn.hash_offset + inst * Dn.n_in + i) < Dn.threshold)
                     v |= 1u << b;
             }
-            Dn.bits[idx] = (uint8_t)v;
-            rows = __popc(v);
-            ev = rows * Dn.inst_post_count[inst];
-        }
-        for (int off = 32; off > 0; off >>= 1) {
-            ev += __shfl_xor(ev, off);
-            rows += __shfl_xor(rows, off);
-        }
-        if (lane_id() == 0 && rows) {
-            atomicAdd(&net->events[0], ev);
-            atomicAdd(&net->events[1], rows);
-        }
+        Dn.bits[idx] = (uint8_t)v;
+        const uint32_t c = __popc(v);
+        rows += c;
+        ev += (unsigned long long)c * Dn.inst_post_count[inst];
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        ev += __shfl_xor(ev, off);
+        rows += __shfl_xor(rows, off);
+    }
+    if (lane_id() == 0 && rows) {
+        atomicAdd(&net->events[0], ev);
+        atomicAdd(&net->events[1], rows);
     }
 }
 
+// A operands from spike bits. Four bits x (x < 16) to one byte each: x * 0x204081 places bit k of x at bit
+// 8k (the shifted copies x << 7k do not overlap, so the product has no carries), the mask keeps those bits.
+__device__ __forceinline__ uint32_t spread4_bytes(uint32_t x) { return __umul24(x, 0x204081u) & 0x01010101u; }
+
+// 8 spike bits (bit e = input e) to 8 fp8 e4m3 values: 1.0 (0x38) where the bit is set, else 0.
 __device__ __forceinline__ long spread_fp8_ones(uint32_t byte) {
-    uint64_t x = byte;
-    x = (x | (x << 28)) & 0x0000000F0000000Full;
-    x = (x | (x << 14)) & 0x0003000300030003ull;
-    x = (x | (x << 7)) & 0x0101010101010101ull;
-    return (long)(x * 0x38ull);  // fp8 e4m3 1.0 in every byte whose bit is set
+    const uint32_t lo = spread4_bytes(byte & 0xFu) * 0x38u, hi = spread4_bytes((byte >> 4) & 0xFu) * 0x38u;
+    return (long)((uint64_t)hi << 32 | lo);
 }
 
+// 8 spike bits to 8 bf16 values: 1.0 (0x3F80) where the bit is set, else 0. Two bits x (x < 4) to the two
+// halves of a word: x * 0x8001 places bit 1 at bit 16, the mask keeps bits 0 and 16.
 __device__ __forceinline__ v8s spread_bf16_ones(uint32_t byte) {
-    v8s r;
-    for (int e = 0; e < 8; ++e) r[e] = ((byte >> e) & 1u) ? (short)0x3F80 : (short)0;
-    return r;
+    typedef unsigned int v4w __attribute__((ext_vector_type(4)));
+    v4w w;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) w[k] = (__umul24((byte >> (2 * k)) & 3u, 0x8001u) & 0x00010001u) * 0x3F80u;
+    return __builtin_bit_cast(v8s, w);
 }
 
-// One wave per 32-column tile; it walks the block's 32-step row tiles. Output element (row m, column n) of
-// the 32x32 accumulator sits in lane n + 32 * ((m / 4) % 2), register 4 * (m / 8) + m % 4.
+// One wave per (32-column tile, group of kDenseGroup 32-step row blocks): A = the spike bits of 32 steps (row
+// m = step, K = inputs), B = the tile's weight fragments, read once per group and shared by its row blocks.
+// Each lane loads the spike bits of its step rows 16 bytes (8 K-steps) at a time. Output element (row m,
+// column n) of a 32x32 accumulator sits in lane n + 32 * ((m / 4) % 2), register 4 * (m / 8) + m % 4.
 template <int FMT>
 __global__ __launch_bounds__(kBlock) void k_dense_mfma(const NetDev *__restrict__ net, uint32_t d, uint64_t t0) {
     const DenseDev &Dn = net->dense[d];
     const PopDev &P = net->pops[Dn.post_pop];
     constexpr uint32_t FB = FMT == 0 ? 8 : 16;  // fragment bytes per lane per K-step
-    const uint32_t lane = lane_id();
-    const uint32_t waves = gridDim.x * (kBlock / 64);
-    const uint32_t bytes_per = 2u * Dn.ksteps;
+    constexpr uint32_t G = kDenseGroup;
+    const uint32_t lane = lane_id(), h = lane >> 5, col = lane & 31;
+    const uint32_t n_rb = Dn.T / 32, n_grp = (n_rb + G - 1) / G;
+    const uint64_t units = (uint64_t)Dn.n_tiles * n_grp;
     const uint64_t ring = 2ull * Dn.T;
-    for (uint32_t tile = blockIdx.x * (kBlock / 64) + threadIdx.x / 64; tile < Dn.n_tiles; tile += waves) {
-        const uint32_t inst = Dn.tile_inst[tile];
-        const uint32_t j0 = Dn.tile_j0[tile];
-        const uint32_t ncol = Dn.tile_ncol[tile];
-        const uint8_t *fr = Dn.frag + (size_t)tile * Dn.ksteps * 64 * FB + (size_t)lane * FB;
-        const uint32_t col = lane & 31;
-        const float sc = col < ncol ? Dn.row_scale[j0 + col] : 0.f;
-        for (uint32_t tt = 0; tt < Dn.T / 32; ++tt) {
-            v16f acc;
-            for (int v = 0; v < 16; ++v) acc[v] = 0.f;
-            const uint32_t row = tt * 32 + (lane & 31);
-            const uint8_t *brow = Dn.bits + ((size_t)row * Dn.n_inst + inst) * bytes_per + (lane >> 5);
-            for (uint32_t kk = 0; kk < Dn.ksteps; ++kk) {
-                const uint32_t sb = brow[2 * kk];
-                if (FMT == 0) {
-                    const long a = spread_fp8_ones(sb);
-                    const long b = *(const long *)(fr + (size_t)kk * 64 * FB);
-                    acc = __builtin_amdgcn_mfma_f32_32x32x16_fp8_fp8(a, b, acc, 0, 0, 0);
-                } else {
-                    const v8s a = spread_bf16_ones(sb);
-                    const v8s b = *(const v8s *)(fr + (size_t)kk * 64 * FB);
-                    acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc, 0, 0, 0);
+    const uint64_t waves = (uint64_t)gridDim.x * (kBlock / 64);
+    for (uint64_t u = (uint64_t)blockIdx.x * (kBlock / 64) + threadIdx.x / 64; u < units; u += waves) {
+        const uint32_t tile = (uint32_t)(u / n_grp), rb0 = (uint32_t)(u % n_grp) * G;
+        const uint32_t nb = min(G, n_rb - rb0);
+        const uint32_t inst = Dn.tile_inst[tile], j0 = Dn.tile_j0[tile], ncol = Dn.tile_ncol[tile];
+        const KNP_GLOBAL uint8_t *fr = gp(Dn.frag) + (size_t)tile * Dn.ksteps * 64 * FB + (size_t)lane * FB;
+        const KNP_GLOBAL uint8_t *brow[G];
+#pragma unroll
+        for (uint32_t g = 0; g < G; ++g)
+            brow[g] = gp(Dn.bits) + ((size_t)((rb0 + min(g, nb - 1)) * 32 + col) * Dn.n_inst + inst) * Dn.pitch;
+        v16f acc[G];
+#pragma unroll
+        for (uint32_t g = 0; g < G; ++g)
+            for (int v = 0; v < 16; ++v) acc[g][v] = 0.f;
+        for (uint32_t k8 = 0; k8 < Dn.ksteps; k8 += 8) {
+            v4u sb[G];
+#pragma unroll
+            for (uint32_t g = 0; g < G; ++g) sb[g] = *(const KNP_GLOBAL v4u *)(brow[g] + 2 * k8);
+            const uint32_t ne = min(8u, Dn.ksteps - k8);
+            if (FMT == 0) {
+                long b[8];
+#pragma unroll
+                for (uint32_t e = 0; e < 8; ++e)
+                    b[e] = e < ne ? *(const KNP_GLOBAL long *)(fr + (size_t)(k8 + e) * 64 * FB) : 0;
+#pragma unroll
+                for (uint32_t e = 0; e < 8; ++e) {
+                    if (e >= ne) break;
+#pragma unroll
+                    for (uint32_t g = 0; g < G; ++g) {
+                        const uint32_t byte = (sb[g][e >> 1] >> (16u * (e & 1u) + 8u * h)) & 0xFFu;
+                        acc[g] = __builtin_amdgcn_mfma_f32_32x32x16_fp8_fp8(spread_fp8_ones(byte), b[e], acc[g], 0, 0, 0);
+                    }
+                }
+            } else {
+                v8s b[8];
+#pragma unroll
+                for (uint32_t e = 0; e < 8; ++e)
+                    if (e < ne) b[e] = *(const KNP_GLOBAL v8s *)(fr + (size_t)(k8 + e) * 64 * FB);
+#pragma unroll
+                for (uint32_t e = 0; e < 8; ++e) {
+                    if (e >= ne) break;
+#pragma unroll
+                    for (uint32_t g = 0; g < G; ++g) {
+                        const uint32_t byte = (sb[g][e >> 1] >> (16u * (e & 1u) + 8u * h)) & 0xFFu;
+                        acc[g] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(spread_bf16_ones(byte), b[e], acc[g], 0, 0, 0);
+                    }
                 }
             }
-            if (col < ncol) {
+        }
+        if (col < ncol) {
+            const float sc = Dn.row_scale[j0 + col];
+#pragma unroll
+            for (uint32_t g = 0; g < G; ++g) {
+                if (g >= nb) break;
+#pragma unroll
                 for (int v = 0; v < 16; ++v) {
-                    const uint32_t m = 8u * (v / 4) + 4u * (lane >> 5) + (v % 4);
-                    const uint64_t arrival = t0 + tt * 32 + m + 1;
-                    Dn.ff[(size_t)(arrival % ring) * P.n + j0 + col] = acc[v] * sc;
+                    const uint32_t m = 8u * (v / 4) + 4u * h + (v % 4);
+                    const uint64_t arrival = t0 + (uint64_t)(rb0 + g) * 32 + m + 1;
+                    gp(Dn.ff)[(size_t)(arrival % ring) * P.n + j0 + col] = acc[g][v] * sc;
                 }
             }
         }
