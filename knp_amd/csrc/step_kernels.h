@@ -13,6 +13,7 @@
 // spike trains do not depend on thread scheduling.
 #pragma once
 #include <hip/hip_runtime.h>
+#include <type_traits>
 
 #include "device_tables.h"
 
@@ -1501,40 +1502,45 @@ __global__ __launch_bounds__(kBlock) void k_dense_mfma(const NetDev *__restrict_
 #pragma unroll
         for (uint32_t g = 0; g < G; ++g)
             for (int v = 0; v < 16; ++v) acc[g][v] = 0.f;
-        for (uint32_t k8 = 0; k8 < Dn.ksteps; k8 += 8) {
+        // Chunks of 8 K-steps, double-buffered: the loads of chunk c + 1 are in flight while chunk c is
+        // multiplied. Loads are unconditional (beyond the last chunk they repeat it, and K-steps beyond the
+        // last repeat its fragment), so the wait before a chunk's products covers only that chunk's loads.
+        typedef typename std::conditional<FMT == 0, long, v8s>::type Frag;
+        const uint32_t nck = (Dn.ksteps + 7) / 8;
+        struct Chunk {
             v4u sb[G];
+            Frag b[8];
+        };
+        auto load = [&](uint32_t ck, Chunk &x) {
+            ck = min(ck, nck - 1);
 #pragma unroll
-            for (uint32_t g = 0; g < G; ++g) sb[g] = *(const KNP_GLOBAL v4u *)(brow[g] + 2 * k8);
-            const uint32_t ne = min(8u, Dn.ksteps - k8);
-            if (FMT == 0) {
-                long b[8];
+            for (uint32_t g = 0; g < G; ++g) x.sb[g] = *(const KNP_GLOBAL v4u *)(brow[g] + 16 * ck);
 #pragma unroll
-                for (uint32_t e = 0; e < 8; ++e)
-                    b[e] = e < ne ? *(const KNP_GLOBAL long *)(fr + (size_t)(k8 + e) * 64 * FB) : 0;
+            for (uint32_t e = 0; e < 8; ++e)
+                x.b[e] = *(const KNP_GLOBAL Frag *)(fr + (size_t)min(8 * ck + e, Dn.ksteps - 1) * 64 * FB);
+        };
+        auto multiply = [&](uint32_t ck, const Chunk &x) {
+            const uint32_t ne = min(8u, Dn.ksteps - 8 * ck);
 #pragma unroll
-                for (uint32_t e = 0; e < 8; ++e) {
-                    if (e >= ne) break;
+            for (uint32_t e = 0; e < 8; ++e) {
+                if (e >= ne) break;
 #pragma unroll
-                    for (uint32_t g = 0; g < G; ++g) {
-                        const uint32_t byte = (sb[g][e >> 1] >> (16u * (e & 1u) + 8u * h)) & 0xFFu;
-                        acc[g] = __builtin_amdgcn_mfma_f32_32x32x16_fp8_fp8(spread_fp8_ones(byte), b[e], acc[g], 0, 0, 0);
-                    }
-                }
-            } else {
-                v8s b[8];
-#pragma unroll
-                for (uint32_t e = 0; e < 8; ++e)
-                    if (e < ne) b[e] = *(const KNP_GLOBAL v8s *)(fr + (size_t)(k8 + e) * 64 * FB);
-#pragma unroll
-                for (uint32_t e = 0; e < 8; ++e) {
-                    if (e >= ne) break;
-#pragma unroll
-                    for (uint32_t g = 0; g < G; ++g) {
-                        const uint32_t byte = (sb[g][e >> 1] >> (16u * (e & 1u) + 8u * h)) & 0xFFu;
-                        acc[g] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(spread_bf16_ones(byte), b[e], acc[g], 0, 0, 0);
-                    }
+                for (uint32_t g = 0; g < G; ++g) {
+                    const uint32_t byte = (x.sb[g][e >> 1] >> (16u * (e & 1u) + 8u * h)) & 0xFFu;
+                    if constexpr (FMT == 0)
+                        acc[g] = __builtin_amdgcn_mfma_f32_32x32x16_fp8_fp8(spread_fp8_ones(byte), x.b[e], acc[g], 0, 0, 0);
+                    else
+                        acc[g] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(spread_bf16_ones(byte), x.b[e], acc[g], 0, 0, 0);
                 }
             }
+        };
+        Chunk c0, c1;
+        load(0, c0);
+        for (uint32_t ck = 0; ck < nck; ck += 2) {
+            load(ck + 1, c1);
+            multiply(ck, c0);
+            load(ck + 2, c0);
+            if (ck + 1 < nck) multiply(ck + 1, c1);
         }
         if (col < ncol) {
             const float sc = Dn.row_scale[j0 + col];
