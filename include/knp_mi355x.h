@@ -145,7 +145,8 @@ int knp_mi355x_set_memory_cap(knp_net *net, uint64_t bytes);
  * AUTO: populations that only receive plain Delta EXCITATORY / INHIBITORY_CURRENT impacts with delays up to
  * 32 are tiled (each 256-neuron tile owned by one workgroup, sums in LDS, no global atomics); the rest use
  * per-impact 64-bit atomics. When every population is tiled and all tiles fit on the device at once, a
- * multi-step call runs as one persistent launch (ring kept in LDS, one grid barrier per step).
+ * multi-step call runs as one persistent launch (ring kept in LDS; each step's spike lists carry a step
+ * stamp that the readers wait for).
  * STEPWISE: as AUTO but one launch per step. ATOMIC: atomics everywhere. Set before the first step. */
 enum knp_scatter_mode { KNP_SCATTER_AUTO = 0, KNP_SCATTER_ATOMIC = 1, KNP_SCATTER_STEPWISE = 2 };
 int knp_mi355x_set_scatter_mode(knp_net *net, int mode);
