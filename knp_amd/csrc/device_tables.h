@@ -142,7 +142,7 @@ struct ProjDev {
     // a tile's part of a row is contiguous; the owner of a tile finds it through a directory. Dense directory:
     // [n_pre][tile_stride] CSR positions. Ranged directory (rows spanning few of many tiles): row_meta +
     // row_dir. Neither: rows are short and every tile reads them whole and filters.
-    const uint64_t *syn8;        // [n_syn] CSR order: weight bits | delay << 32 | inhibitory << 40 | post << 41
+    const uint64_t *syn8;        // [n_syn] CSR order: impact m * 2^s (m: bits 0-24 signed, s: 25-30) | delay << 32 | post << 41
     const uint32_t *tile_dir;    // dense directory (null if none)
     uint32_t tile_stride;        // tiles of the postsynaptic population + 1
     const struct RowMeta *row_meta;  // [n_pre] ranged directory (null if none)

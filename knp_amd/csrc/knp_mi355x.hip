@@ -697,14 +697,23 @@ int init_network(knp_net *net) {
             }
         }
         if (net->pops[H.post_pop].tiled) {
-            // packed records for the tile scatter: weight bits | delay << 32 | inhibitory << 40 | post << 41
+            // packed records for the tile scatter: the synapse's signed fixed-point impact m * 2^s (m in the low
+            // 25 bits, two's complement; s in bits 25-30) | delay << 32 | post << 41. The impact is
+            // round-half-even(w * 2^F), the value the float product w * 2^F (exact) converted to int64 gives,
+            // negated for INHIBITORY_CURRENT; it has at most 24 significant bits, so m * 2^s holds it exactly.
+            const int F = net->pops[H.post_pop].frac_bits;
             std::vector<uint64_t> rec(S);
             for (uint64_t k = 0; k < S; ++k) {
                 const uint32_t sy = csr_syn[k];
-                uint32_t wb;
-                std::memcpy(&wb, &H.w[sy], 4);
-                rec[k] = (uint64_t)wb | ((uint64_t)(H.delay[sy] & 0xFFu) << 32) |
-                         ((uint64_t)(H.type[sy] == KNP_INHIBITORY_CURRENT ? 1 : 0) << 40) | ((uint64_t)H.post[sy] << 41);
+                long long v = std::llrint(std::ldexp((double)H.w[sy], F));
+                if (H.type[sy] == KNP_INHIBITORY_CURRENT) v = -v;
+                uint32_t sh = 0;
+                while (v >= (1ll << 24) || v <= -(1ll << 24)) {
+                    v /= 2;  // exact: an impact this large is a multiple of 2^(its magnitude's binade - 23)
+                    ++sh;
+                }
+                rec[k] = ((uint64_t)v & 0x1FFFFFFull) | ((uint64_t)sh << 25) | ((uint64_t)(H.delay[sy] & 0xFFu) << 32) |
+                         ((uint64_t)H.post[sy] << 41);
             }
             int rc2;
             uint64_t *rd;

@@ -573,7 +573,6 @@ struct TileCtx {
     TileDev td;
     uint32_t D, n_pop, tile_lo, kind;
     int64_t *acc_v;
-    float scale;
     unsigned long long *ring;
     const uint32_t **vlist;
     uint32_t *vpfx;  // prefix of the segments' counts this step
@@ -618,7 +617,6 @@ __device__ __forceinline__ TileCtx tile_setup(const NetDev *__restrict__ net, un
     c.n_pop = P.n;
     c.kind = P.kind;
     c.acc_v = P.acc_v;
-    c.scale = P.scale;
     c.tile_lo = c.td.tile * kBlock;
     const uint32_t L = net->tile_max_list;
     c.ring = dyn;
@@ -713,15 +711,16 @@ __device__ __forceinline__ bool tile_gather(const NetDev *__restrict__ net, cons
         __syncthreads();
     }
     const uint32_t slot0 = (uint32_t)(t % c.D);
-    // exact sum of one packed synapse record into the LDS ring (false: a neuron of another tile)
+    // exact sum of one packed synapse record into the LDS ring (false: a neuron of another tile); the record
+    // holds the signed fixed-point impact as m * 2^s
     auto add_record = [&](uint64_t rec) -> bool {
         const uint32_t j = (uint32_t)(rec >> 41) - c.tile_lo;
         if (j >= (uint32_t)kBlock) return false;  // a short row read whole: neuron of another tile
         uint32_t slot = slot0 + ((uint32_t)(rec >> 32) & 0xFFu);
         if (slot >= c.D) slot -= c.D;
-        long long v = __float2ll_rn(__uint_as_float((uint32_t)rec) * c.scale);
-        if ((rec >> 40) & 1u) v = -v;  // INHIBITORY_CURRENT, the only other type a tiled population receives
-        atomicAdd(&c.ring[slot * kBlock + j], (unsigned long long)v);
+        const int32_t m = (int32_t)((uint32_t)rec << 7) >> 7;
+        const unsigned long long v = (unsigned long long)(long long)m << (((uint32_t)rec >> 25) & 63u);
+        atomicAdd(&c.ring[slot * kBlock + j], v);
         return true;
     };
     for (uint32_t rb = 0; rb < total; rb += kGatherRows) {
