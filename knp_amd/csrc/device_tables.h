@@ -147,6 +147,9 @@ struct ProjDev {
     uint32_t tile_stride;        // tiles of the postsynaptic population + 1
     const struct RowMeta *row_meta;  // [n_pre] ranged directory (null if none)
     const uint32_t *row_dir;     // per row: tiles spanned + 1 offsets from the row start
+    // Records of a row's part in a tile start with its delay-1 records; their number per part, indexed as
+    // the directory entry of the part (tile_dir or row_dir), or null (every record counts as delay 1).
+    const uint8_t *part_urgent;
 };
 
 struct SourceDev {

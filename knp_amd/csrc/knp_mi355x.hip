@@ -684,34 +684,45 @@ int init_network(knp_net *net) {
         {
             std::vector<uint64_t> fill(row_ptr.begin(), row_ptr.end() - 1);
             if (net->pops[H.post_pop].tiled) {
-                // rows sorted by postsynaptic neuron, then synapse index (a tile's part of a row is contiguous):
-                // a stable counting sort by postsynaptic neuron, then the stable fill by presynaptic neuron
-                std::vector<uint64_t> col((size_t)n_post + 1, 0);
-                for (uint64_t s = 0; s < S; ++s) col[H.post[s] + 1]++;
-                for (uint32_t j = 0; j < n_post; ++j) col[j + 1] += col[j];
-                std::vector<uint32_t> by_post(S);
-                for (uint64_t s = 0; s < S; ++s) by_post[col[H.post[s]]++] = (uint32_t)s;
-                for (uint64_t k = 0; k < S; ++k) csr_syn[fill[H.pre[by_post[k]]]++] = by_post[k];
+                // rows ordered by the postsynaptic tile (a tile's part of a row is contiguous), in each part the
+                // delay-1 records first (a persistent launch adds them before phase 1 of the next step and the
+                // others after it), then by synapse index: a stable counting sort by that key, then the stable
+                // fill by presynaptic neuron
+                const uint32_t n_keys = 2 * ((n_post + kBlock - 1) / kBlock);
+                auto key = [&](uint64_t s) { return 2 * (H.post[s] / kBlock) + (H.delay[s] == 1 ? 0u : 1u); };
+                std::vector<uint64_t> col((size_t)n_keys + 1, 0);
+                for (uint64_t s = 0; s < S; ++s) col[key(s) + 1]++;
+                for (uint32_t j = 0; j < n_keys; ++j) col[j + 1] += col[j];
+                std::vector<uint32_t> by_key(S);
+                for (uint64_t s = 0; s < S; ++s) by_key[col[key(s)]++] = (uint32_t)s;
+                for (uint64_t k = 0; k < S; ++k) csr_syn[fill[H.pre[by_key[k]]]++] = by_key[k];
             } else {
                 for (uint64_t s = 0; s < S; ++s) csr_syn[fill[H.pre[s]]++] = (uint32_t)s;
             }
         }
+        // delay-1 records at the start of [b, e) (at most 255, else no split for the projection)
+        bool urgent_ok = true;
+        auto urgent_count = [&](uint64_t b, uint64_t e) -> uint8_t {
+            uint64_t k = b;
+            while (k < e && H.delay[csr_syn[k]] == 1) ++k;
+            if (k - b > 255) urgent_ok = false;
+            return (uint8_t)std::min<uint64_t>(k - b, 255);
+        };
         if (net->pops[H.post_pop].tiled) {
             // packed records for the tile scatter: the synapse's signed fixed-point impact m * 2^s (m in the low
             // 25 bits, two's complement; s in bits 25-30) | delay << 32 | post << 41. The impact is
             // round-half-even(w * 2^F), the value the float product w * 2^F (exact) converted to int64 gives,
             // negated for INHIBITORY_CURRENT; it has at most 24 significant bits, so m * 2^s holds it exactly.
-            const int F = net->pops[H.post_pop].frac_bits;
+            const double scale = std::ldexp(1.0, net->pops[H.post_pop].frac_bits);
             std::vector<uint64_t> rec(S);
             for (uint64_t k = 0; k < S; ++k) {
                 const uint32_t sy = csr_syn[k];
-                long long v = std::llrint(std::ldexp((double)H.w[sy], F));
+                long long v = std::llrint((double)H.w[sy] * scale);
                 if (H.type[sy] == KNP_INHIBITORY_CURRENT) v = -v;
-                uint32_t sh = 0;
-                while (v >= (1ll << 24) || v <= -(1ll << 24)) {
-                    v /= 2;  // exact: an impact this large is a multiple of 2^(its magnitude's binade - 23)
-                    ++sh;
-                }
+                const uint64_t mag = (uint64_t)(v < 0 ? -v : v);
+                // exact: an impact of 2^24 or more is a multiple of 2^(its binade - 23)
+                const uint32_t sh = mag >= (1ull << 24) ? (uint32_t)(63 - __builtin_clzll(mag)) - 23 : 0;
+                v /= (1ll << sh);
                 rec[k] = ((uint64_t)v & 0x1FFFFFFull) | ((uint64_t)sh << 25) | ((uint64_t)(H.delay[sy] & 0xFFu) << 32) |
                          ((uint64_t)H.post[sy] << 41);
             }
@@ -733,10 +744,19 @@ int init_network(knp_net *net) {
                     d[x] = (uint32_t)k;
                 }
             }
+            std::vector<uint8_t> urg(dir.size(), 0);
+            for (uint32_t i = 0; i < H.n_pre; ++i)
+                for (uint32_t x = 0; x < tiles; ++x) {
+                    const size_t at = (size_t)i * (tiles + 1) + x;
+                    urg[at] = urgent_count(dir[at], dir[at + 1]);
+                }
             int rc2;
             uint32_t *dd;
             if ((rc2 = upload(net, &dd, dir.data(), dir.size()))) return rc2;
             Q.tile_dir = dd;
+            uint8_t *ud;
+            if (urgent_ok && (rc2 = upload(net, &ud, urg.data(), urg.size()))) return rc2;
+            Q.part_urgent = urgent_ok ? ud : nullptr;
         } else if (proj_dir[q] == DIR_RANGED) {
             std::vector<RowMeta> meta(H.n_pre);
             std::vector<uint32_t> dir;
@@ -756,6 +776,12 @@ int init_network(knp_net *net) {
                     dir.push_back((uint32_t)(k - b));
                 }
             }
+            std::vector<uint8_t> urg(dir.size(), 0);
+            for (uint32_t i = 0; i < H.n_pre; ++i)
+                for (uint32_t j = 0; j < meta[i].n_tiles; ++j) {
+                    const size_t at = (size_t)meta[i].dir + j;
+                    urg[at] = urgent_count(meta[i].begin + dir[at], meta[i].begin + dir[at + 1]);
+                }
             int rc2;
             RowMeta *md;
             if ((rc2 = upload(net, &md, meta.data(), meta.size()))) return rc2;
@@ -763,6 +789,9 @@ int init_network(knp_net *net) {
             uint32_t *dd;
             if ((rc2 = upload(net, &dd, dir.data(), dir.size()))) return rc2;
             Q.row_dir = dd;
+            uint8_t *ud;
+            if (urgent_ok && (rc2 = upload(net, &ud, urg.data(), urg.size()))) return rc2;
+            Q.part_urgent = urgent_ok ? ud : nullptr;
         }
         if (H.has_blocking)
             for (uint32_t i = 0; i < H.n_pre; ++i)

@@ -557,7 +557,13 @@ struct TileProj {
     const RowMeta *row_meta;
     const uint32_t *row_dir;
     const uint64_t *row_ptr;
+    const uint8_t *urgent;
     uint32_t stride, n_pre;
+};
+
+// A thread's row part whose records are added later: [b, b + len) of projection q.
+struct TilePart {
+    uint32_t b, len, q;
 };
 
 // Dynamic LDS of the tile kernels: the ring sums [tile_max_d][256] (int64), then for the tile's segments
@@ -628,7 +634,7 @@ __device__ __forceinline__ TileCtx tile_setup(const NetDev *__restrict__ net, un
     if (tid < 4) sh.cnt[tid] = 0;
     if (tid < net->n_projs) {
         const ProjDev &Q = net->projs[tid];
-        sh.tp[tid] = TileProj{Q.syn8, Q.tile_dir, Q.row_meta, Q.row_dir, Q.row_ptr, Q.tile_stride, Q.n_pre};
+        sh.tp[tid] = TileProj{Q.syn8, Q.tile_dir, Q.row_meta, Q.row_dir, Q.row_ptr, Q.part_urgent, Q.tile_stride, Q.n_pre};
     }
     const TileSeg *ts = net->tile_seg + c.td.vbeg;
     for (uint32_t v = tid; v < c.nv; v += kTileThreads) {
@@ -639,6 +645,61 @@ __device__ __forceinline__ TileCtx tile_setup(const NetDev *__restrict__ net, un
     return c;
 }
 
+// Exact sum of one packed synapse record of a step whose ring slot is slot0 into the LDS ring (false: a
+// neuron of another tile). The record holds the signed fixed-point impact as m * 2^s.
+__device__ __forceinline__ bool add_record(const TileCtx &c, uint32_t slot0, uint64_t rec) {
+    const uint32_t j = (uint32_t)(rec >> 41) - c.tile_lo;
+    if (j >= (uint32_t)kBlock) return false;  // a short row read whole: neuron of another tile
+    uint32_t slot = slot0 + ((uint32_t)(rec >> 32) & 0xFFu);
+    if (slot >= c.D) slot -= c.D;
+    const int32_t m = (int32_t)((uint32_t)rec << 7) >> 7;
+    const unsigned long long v = (unsigned long long)(long long)m << (((uint32_t)rec >> 25) & 63u);
+    atomicAdd(&c.ring[slot * kBlock + j], v);
+    return true;
+}
+
+// The thread adds the n records at src, 8 loads in flight at a time.
+__device__ __forceinline__ void thread_add_records(const TileCtx &c, uint32_t slot0, const KNP_GLOBAL uint64_t *src,
+                                                   uint32_t n, uint32_t &my_ev) {
+    for (uint32_t k0 = 0; k0 < n; k0 += 8) {
+        uint64_t rec[8];
+#pragma unroll
+        for (uint32_t u = 0; u < 8; ++u)
+            if (k0 + u < n) rec[u] = src[k0 + u];
+#pragma unroll
+        for (uint32_t u = 0; u < 8; ++u)
+            if (k0 + u < n && add_record(c, slot0, rec[u])) ++my_ev;
+    }
+}
+
+// Stage 3: every thread's row part (records [b, b + len) of projection q) into the ring. Short parts (the
+// common case) by their own thread; long parts spread over the workgroup, one thread per record, through
+// the row tables. Ends with a workgroup barrier.
+__device__ __forceinline__ void tile_add_parts(const TileCtx &c, TileShared &sh, uint32_t slot0, uint32_t b,
+                                               uint32_t len, uint32_t q, uint32_t &my_ev) {
+    const uint32_t tid = threadIdx.x;
+    if (len <= kTileShortPart) thread_add_records(c, slot0, gp(sh.tp[q].syn8) + b, len, my_ev);
+    const uint32_t llen = len > kTileShortPart ? len : 0u;
+    if (__syncthreads_or(llen != 0)) {
+        uint32_t n_syn;
+        const uint32_t ex = block_excl_scan<kTileThreads>(llen, sh.wsum, &n_syn);
+        sh.rval[tid] = b;
+        sh.rpfx[tid] = ex;
+        sh.rq[tid] = (uint8_t)q;
+        __syncthreads();
+        for (uint32_t s = tid; s < n_syn; s += kTileThreads) {
+            uint32_t lo = 0, hi = kTileThreads;  // largest r with rpfx[r] <= s: a row whose part holds s
+            while (hi - lo > 1) {
+                const uint32_t mid = (lo + hi) >> 1;
+                if (sh.rpfx[mid] <= s) lo = mid;
+                else hi = mid;
+            }
+            if (add_record(c, slot0, gp(sh.tp[sh.rq[lo]].syn8)[sh.rval[lo] + (s - sh.rpfx[lo])])) ++my_ev;
+        }
+    }
+    __syncthreads();
+}
+
 // Stages 1-3 of the tile scatter of step t: numbers the spiking rows that may reach the tile (prefix over
 // the counts of its segments), finds every row's part in the tile, and adds the part's impacts into the LDS
 // ring with LDS atomics. Ends with a workgroup barrier. Adds the impacts (ev) and row parts (parts).
@@ -646,9 +707,13 @@ __device__ __forceinline__ TileCtx tile_setup(const NetDev *__restrict__ net, un
 // workgroups during the launch, and each is read once its stamp says step t is complete; channel segments
 // are read only when `channels` (the launch's first step: the host pushes channel messages before the
 // launch). Returns false if a wait ended without the stamp (the launch is being aborted).
-template <bool COHERENT>
+// SPLIT: of the first pass's row parts only the delay-1 records are added (the ring slot of step t + 1 is
+// then complete); the rest of each thread's part is left in `part`, for tile_add_parts after phase 1.
+template <bool COHERENT, bool SPLIT = false>
 __device__ __forceinline__ bool tile_gather(const NetDev *__restrict__ net, const TileCtx &c, TileShared &sh,
-                                            uint64_t t, uint32_t &my_ev, uint32_t &my_parts, bool channels = true) {
+                                            uint64_t t, uint32_t &my_ev, uint32_t &my_parts, bool channels = true,
+                                            TilePart *part = nullptr) {
+    if (SPLIT) *part = TilePart{0u, 0u, 0u};
     const uint32_t tid = threadIdx.x;
     const size_t par_off = (size_t)(t & 1) * net->list_pstride;
     // the pool as a buffer: one 16-byte load gives a segment's count and first kStash entries (device-scope
@@ -711,22 +776,10 @@ __device__ __forceinline__ bool tile_gather(const NetDev *__restrict__ net, cons
         __syncthreads();
     }
     const uint32_t slot0 = (uint32_t)(t % c.D);
-    // exact sum of one packed synapse record into the LDS ring (false: a neuron of another tile); the record
-    // holds the signed fixed-point impact as m * 2^s
-    auto add_record = [&](uint64_t rec) -> bool {
-        const uint32_t j = (uint32_t)(rec >> 41) - c.tile_lo;
-        if (j >= (uint32_t)kBlock) return false;  // a short row read whole: neuron of another tile
-        uint32_t slot = slot0 + ((uint32_t)(rec >> 32) & 0xFFu);
-        if (slot >= c.D) slot -= c.D;
-        const int32_t m = (int32_t)((uint32_t)rec << 7) >> 7;
-        const unsigned long long v = (unsigned long long)(long long)m << (((uint32_t)rec >> 25) & 63u);
-        atomicAdd(&c.ring[slot * kBlock + j], v);
-        return true;
-    };
     for (uint32_t rb = 0; rb < total; rb += kGatherRows) {
         // 2. one thread per row: its presynaptic index and projection, and where its part for this tile lies
         const uint32_t row = rb + tid;
-        uint32_t b = 0, len = 0, q = 0, pre = 0xFFFFFFFFu;
+        uint32_t b = 0, len = 0, q = 0, pre = 0xFFFFFFFFu, urgent = 0;
         if (row < total) {
             uint32_t lo = 0xFFFFFFFFu;  // the row's segment when its entry is loaded here
             if (row < kGatherRows) {
@@ -748,10 +801,12 @@ __device__ __forceinline__ bool tile_gather(const NetDev *__restrict__ net, cons
             const TileProj &Q = sh.tp[q];
             if (pre < Q.n_pre) {
                 if (Q.tile_dir) {
-                    const KNP_GLOBAL uint32_t *d = gp(Q.tile_dir) + (size_t)pre * Q.stride + c.td.tile;
+                    const size_t at = (size_t)pre * Q.stride + c.td.tile;
+                    const KNP_GLOBAL uint32_t *d = gp(Q.tile_dir) + at;
                     const uint32_t d0 = d[0], d1 = d[1];
                     b = d0;
                     len = d1 - d0;
+                    urgent = SPLIT && Q.urgent ? (uint32_t)gp(Q.urgent)[at] : len;
                 } else if (Q.row_meta) {
                     const KNP_GLOBAL RowMeta *m = gp(Q.row_meta) + pre;
                     const uint32_t m_begin = m->begin, m_first = m->first_tile, m_n = m->n_tiles, m_dir = m->dir;
@@ -760,49 +815,25 @@ __device__ __forceinline__ bool tile_gather(const NetDev *__restrict__ net, cons
                         const uint32_t o0 = gp(Q.row_dir)[m_dir + k], o1 = gp(Q.row_dir)[m_dir + k + 1];
                         b = m_begin + o0;
                         len = o1 - o0;
+                        urgent = SPLIT && Q.urgent ? (uint32_t)gp(Q.urgent)[m_dir + k] : len;
                     }
                 } else {
                     const uint64_t r0 = gp(Q.row_ptr)[pre], r1 = gp(Q.row_ptr)[pre + 1];
                     b = (uint32_t)r0;
                     len = (uint32_t)(r1 - r0);
+                    urgent = len;
                 }
             }
         }
         my_parts += len ? 1u : 0u;
-        // 3a. short parts (the common case): the row's thread loads its records, 8 at a time, then adds them
-        if (len <= kTileShortPart) {
-            const KNP_GLOBAL uint64_t *src = gp(sh.tp[q].syn8) + b;
-            for (uint32_t k0 = 0; k0 < len; k0 += 8) {
-                uint64_t rec[8];
-#pragma unroll
-                for (uint32_t u = 0; u < 8; ++u)
-                    if (k0 + u < len) rec[u] = src[k0 + u];
-#pragma unroll
-                for (uint32_t u = 0; u < 8; ++u)
-                    if (k0 + u < len && add_record(rec[u])) ++my_ev;
-            }
+        if (SPLIT && rb == 0) {
+            // the part's delay-1 records now, the rest after phase 1 of t + 1
+            thread_add_records(c, slot0, gp(sh.tp[q].syn8) + b, urgent, my_ev);
+            *part = TilePart{b + urgent, len - urgent, q};
+            __syncthreads();  // the sums are complete before phase 1 reads them
+            continue;
         }
-        // 3b. long parts: spread over the workgroup, one thread per synapse (the row tables are free now)
-        const uint32_t llen = len > kTileShortPart ? len : 0u;
-        if (__syncthreads_or(llen != 0)) {
-            uint32_t n_syn;
-            const uint32_t ex = block_excl_scan<kTileThreads>(llen, sh.wsum, &n_syn);
-            sh.rval[tid] = b;
-            sh.rpfx[tid] = ex;
-            sh.rq[tid] = (uint8_t)q;
-            __syncthreads();
-            const uint32_t nrows = min((uint32_t)kTileThreads, total - rb);
-            for (uint32_t s = tid; s < n_syn; s += kTileThreads) {
-                uint32_t lo = 0, hi = nrows;  // largest r with rpfx[r] <= s: a row whose part holds s
-                while (hi - lo > 1) {
-                    const uint32_t mid = (lo + hi) >> 1;
-                    if (sh.rpfx[mid] <= s) lo = mid;
-                    else hi = mid;
-                }
-                if (add_record(gp(sh.tp[sh.rq[lo]].syn8)[sh.rval[lo] + (s - sh.rpfx[lo])])) ++my_ev;
-            }
-        }
-        __syncthreads();
+        tile_add_parts(c, sh, slot0, b, len, q, my_ev);
     }
     return true;
 }
@@ -1121,7 +1152,10 @@ __global__ __launch_bounds__(kTileThreads) void k_tile_run(const NetDev *__restr
     }
     for (uint32_t k = 0; k < n; ++k) {
         const uint64_t t = t0 + k;
-        if (!tile_gather<true>(net, c, sh, t, my_ev, my_parts, k == 0)) break;
+        // the impacts of step t that phase 1 of t + 1 needs (delay 1), then phase 1, then the others of step
+        // t (delays 2 and up: later slots), while the other workgroups finish phase 1
+        TilePart part;
+        if (!tile_gather<true, true>(net, c, sh, t, my_ev, my_parts, k == 0, &part)) break;
         if (k == 1 && blockIdx.x == 0) clear_push_counts<true>(net, t0);
         if (k + 1 < n) {
             const uint32_t s1 = (uint32_t)((t + 1) % c.D);
@@ -1132,6 +1166,7 @@ __global__ __launch_bounds__(kTileThreads) void k_tile_run(const NetDev *__restr
             }
             my_rows += tile_phase1_dual<KIND>(net, c, pc, t + 1, a);
         }
+        tile_add_parts(c, sh, (uint32_t)(t % c.D), part.b, part.len, part.q, my_ev);
     }
     // the ring back to HBM
     __syncthreads();
