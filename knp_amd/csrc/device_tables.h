@@ -138,19 +138,24 @@ struct ProjDev {
     uint32_t n_stdp;
     uint32_t stdp_pop[4];
     uint32_t stdp_mode[4];
-    // tile-owner scatter (plain Delta into a tiled population): rows are sorted by postsynaptic neuron, so
-    // a tile's part of a row is contiguous; the owner of a tile finds it through a directory. Dense directory:
-    // [n_pre][tile_stride] CSR positions. Ranged directory (rows spanning few of many tiles): row_meta +
-    // row_dir. Neither: rows are short and every tile reads them whole and filters.
+    // tile-owner scatter (plain Delta into a tiled population): rows are ordered by postsynaptic tile, so a
+    // tile's part of a row is contiguous, with the part's delay-1 records first; the owner of a tile finds
+    // the part through a directory of part entries (dir_entry). Dense directory: [n_pre][tile_stride].
+    // Ranged directory (rows spanning few of many tiles): row_meta + row_dir. Neither: rows are short and
+    // every tile reads them whole and filters.
     const uint64_t *syn8;        // [n_syn] CSR order: impact m * 2^s (m: bits 0-24 signed, s: 25-30) | delay << 32 | post << 41
-    const uint32_t *tile_dir;    // dense directory (null if none)
-    uint32_t tile_stride;        // tiles of the postsynaptic population + 1
+    const uint64_t *tile_dir;    // dense directory (null if none): CSR positions of the parts
+    uint32_t tile_stride;        // tiles of the postsynaptic population
     const struct RowMeta *row_meta;  // [n_pre] ranged directory (null if none)
-    const uint32_t *row_dir;     // per row: tiles spanned + 1 offsets from the row start
-    // Records of a row's part in a tile start with its delay-1 records; their number per part, indexed as
-    // the directory entry of the part (tile_dir or row_dir), or null (every record counts as delay 1).
-    const uint8_t *part_urgent;
+    const uint64_t *row_dir;     // per row: one entry per tile spanned, positions from the row start
 };
+
+// A part entry of a tile directory: start | length << 32 | delay-1 records << 56 (255: all of the part's
+// records count as delay 1, at 255 or more of them). Lengths stay below 2^24.
+constexpr uint32_t kPartLenBits = 24, kUrgentAll = 255;
+__host__ __device__ inline uint64_t dir_entry(uint32_t start, uint32_t len, uint32_t urgent) {
+    return (uint64_t)start | (uint64_t)len << 32 | (uint64_t)(urgent < kUrgentAll ? urgent : kUrgentAll) << 56;
+}
 
 struct SourceDev {
     uint32_t proj, first, n_ch, n_patterns, steps_per_pattern, lo, hi, hash_offset;

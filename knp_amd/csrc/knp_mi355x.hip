@@ -643,17 +643,23 @@ int init_network(knp_net *net) {
             auto &lo = row_lo[q], &hi = row_hi[q];
             lo.assign(H.n_pre, 0xFFFFFFFFu);
             hi.assign(H.n_pre, 0);
+            std::vector<uint32_t> row_len(H.n_pre, 0);
             for (uint64_t k = 0; k < H.n_syn; ++k) {
                 const uint32_t r = H.pre[k], j = H.post[k];
                 lo[r] = std::min(lo[r], j);
                 hi[r] = std::max(hi[r], j);
+                row_len[r]++;
             }
+            // part lengths in directory entries have kPartLenBits bits
+            for (uint32_t r = 0; r < H.n_pre; ++r)
+                if (row_len[r] >= (1u << kPartLenBits)) net->pops[H.post_pop].tiled = false;
             if (H.n_syn <= 4ull * H.n_pre) continue;
+            // directory sizes in 4-byte words: 8-byte part entries, a 16-byte meta per ranged row
             uint64_t ranged = 0;
             for (uint32_t r = 0; r < H.n_pre; ++r)
-                ranged += 4 + (lo[r] <= hi[r] ? hi[r] / kBlock - lo[r] / kBlock + 2 : 0);  // + 16-byte row meta
+                ranged += 4 + (lo[r] <= hi[r] ? 2 * (hi[r] / kBlock - lo[r] / kBlock + 1) : 0);
             const uint64_t tiles = (net->pops[H.post_pop].n + kBlock - 1) / kBlock;
-            const uint64_t dense = (uint64_t)H.n_pre * (tiles + 1);
+            const uint64_t dense = 2 * (uint64_t)H.n_pre * tiles;
             proj_dir[q] = (dense <= 2 * ranged) ? DIR_DENSE : DIR_RANGED;
             dir_entries[H.post_pop] += std::min(dense, ranged);
         }
@@ -700,13 +706,11 @@ int init_network(knp_net *net) {
                 for (uint64_t s = 0; s < S; ++s) csr_syn[fill[H.pre[s]]++] = (uint32_t)s;
             }
         }
-        // delay-1 records at the start of [b, e) (at most 255, else no split for the projection)
-        bool urgent_ok = true;
-        auto urgent_count = [&](uint64_t b, uint64_t e) -> uint8_t {
+        // delay-1 records at the start of the part [b, e)
+        auto urgent_count = [&](uint64_t b, uint64_t e) -> uint32_t {
             uint64_t k = b;
             while (k < e && H.delay[csr_syn[k]] == 1) ++k;
-            if (k - b > 255) urgent_ok = false;
-            return (uint8_t)std::min<uint64_t>(k - b, 255);
+            return (uint32_t)(k - b);
         };
         if (net->pops[H.post_pop].tiled) {
             // packed records for the tile scatter: the synapse's signed fixed-point impact m * 2^s (m in the low
@@ -733,33 +737,24 @@ int init_network(knp_net *net) {
         }
         if (proj_dir[q] == DIR_DENSE) {
             const uint32_t tiles = (n_post + kBlock - 1) / kBlock;
-            Q.tile_stride = tiles + 1;
-            std::vector<uint32_t> dir((size_t)H.n_pre * (tiles + 1));
+            Q.tile_stride = tiles;
+            std::vector<uint64_t> dir((size_t)H.n_pre * tiles);
             for (uint32_t i = 0; i < H.n_pre; ++i) {
                 uint64_t k = row_ptr[i];
                 const uint64_t e = row_ptr[i + 1];
-                uint32_t *d = dir.data() + (size_t)i * (tiles + 1);
-                for (uint32_t x = 0; x <= tiles; ++x) {
-                    while (k < e && H.post[csr_syn[k]] / kBlock < x) ++k;
-                    d[x] = (uint32_t)k;
+                for (uint32_t x = 0; x < tiles; ++x) {
+                    const uint64_t b = k;
+                    while (k < e && H.post[csr_syn[k]] / kBlock <= x) ++k;
+                    dir[(size_t)i * tiles + x] = dir_entry((uint32_t)b, (uint32_t)(k - b), urgent_count(b, k));
                 }
             }
-            std::vector<uint8_t> urg(dir.size(), 0);
-            for (uint32_t i = 0; i < H.n_pre; ++i)
-                for (uint32_t x = 0; x < tiles; ++x) {
-                    const size_t at = (size_t)i * (tiles + 1) + x;
-                    urg[at] = urgent_count(dir[at], dir[at + 1]);
-                }
             int rc2;
-            uint32_t *dd;
+            uint64_t *dd;
             if ((rc2 = upload(net, &dd, dir.data(), dir.size()))) return rc2;
             Q.tile_dir = dd;
-            uint8_t *ud;
-            if (urgent_ok && (rc2 = upload(net, &ud, urg.data(), urg.size()))) return rc2;
-            Q.part_urgent = urgent_ok ? ud : nullptr;
         } else if (proj_dir[q] == DIR_RANGED) {
             std::vector<RowMeta> meta(H.n_pre);
-            std::vector<uint32_t> dir;
+            std::vector<uint64_t> dir;
             for (uint32_t i = 0; i < H.n_pre; ++i) {
                 const uint64_t b = row_ptr[i], e = row_ptr[i + 1];
                 RowMeta &m = meta[i];
@@ -771,27 +766,19 @@ int init_network(knp_net *net) {
                 m.first_tile = H.post[csr_syn[b]] / kBlock;
                 m.n_tiles = H.post[csr_syn[e - 1]] / kBlock - m.first_tile + 1;
                 uint64_t k = b;
-                for (uint32_t j = 0; j <= m.n_tiles; ++j) {
-                    while (k < e && H.post[csr_syn[k]] / kBlock < m.first_tile + j) ++k;
-                    dir.push_back((uint32_t)(k - b));
+                for (uint32_t j = 0; j < m.n_tiles; ++j) {
+                    const uint64_t pb = k;
+                    while (k < e && H.post[csr_syn[k]] / kBlock <= m.first_tile + j) ++k;
+                    dir.push_back(dir_entry((uint32_t)(pb - b), (uint32_t)(k - pb), urgent_count(pb, k)));
                 }
             }
-            std::vector<uint8_t> urg(dir.size(), 0);
-            for (uint32_t i = 0; i < H.n_pre; ++i)
-                for (uint32_t j = 0; j < meta[i].n_tiles; ++j) {
-                    const size_t at = (size_t)meta[i].dir + j;
-                    urg[at] = urgent_count(meta[i].begin + dir[at], meta[i].begin + dir[at + 1]);
-                }
             int rc2;
             RowMeta *md;
             if ((rc2 = upload(net, &md, meta.data(), meta.size()))) return rc2;
             Q.row_meta = md;
-            uint32_t *dd;
+            uint64_t *dd;
             if ((rc2 = upload(net, &dd, dir.data(), dir.size()))) return rc2;
             Q.row_dir = dd;
-            uint8_t *ud;
-            if (urgent_ok && (rc2 = upload(net, &ud, urg.data(), urg.size()))) return rc2;
-            Q.part_urgent = urgent_ok ? ud : nullptr;
         }
         if (H.has_blocking)
             for (uint32_t i = 0; i < H.n_pre; ++i)

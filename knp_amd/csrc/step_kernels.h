@@ -553,11 +553,10 @@ __device__ __forceinline__ uint32_t wave_sum_u32(uint32_t v) {
 // Per-projection pointers the tile scatter needs, staged in LDS.
 struct TileProj {
     const uint64_t *syn8;
-    const uint32_t *tile_dir;
+    const uint64_t *tile_dir;
     const RowMeta *row_meta;
-    const uint32_t *row_dir;
+    const uint64_t *row_dir;
     const uint64_t *row_ptr;
-    const uint8_t *urgent;
     uint32_t stride, n_pre;
 };
 
@@ -634,7 +633,7 @@ __device__ __forceinline__ TileCtx tile_setup(const NetDev *__restrict__ net, un
     if (tid < 4) sh.cnt[tid] = 0;
     if (tid < net->n_projs) {
         const ProjDev &Q = net->projs[tid];
-        sh.tp[tid] = TileProj{Q.syn8, Q.tile_dir, Q.row_meta, Q.row_dir, Q.row_ptr, Q.part_urgent, Q.tile_stride, Q.n_pre};
+        sh.tp[tid] = TileProj{Q.syn8, Q.tile_dir, Q.row_meta, Q.row_dir, Q.row_ptr, Q.tile_stride, Q.n_pre};
     }
     const TileSeg *ts = net->tile_seg + c.td.vbeg;
     for (uint32_t v = tid; v < c.nv; v += kTileThreads) {
@@ -698,6 +697,33 @@ __device__ __forceinline__ void tile_add_parts(const TileCtx &c, TileShared &sh,
         }
     }
     __syncthreads();
+}
+
+// Where row pre's part in the tile lies: records [b, b + len) of the projection, the first `urgent` of
+// them with delay 1. No part: len = 0. One 8-byte directory entry per part (two dependent loads with a
+// ranged directory).
+__device__ __forceinline__ void part_of(const TileProj &Q, uint32_t pre, uint32_t tile, uint32_t &b, uint32_t &len,
+                                        uint32_t &urgent) {
+    uint64_t e = 0;
+    uint32_t base = 0;
+    if (Q.tile_dir) {
+        e = gp(Q.tile_dir)[(size_t)pre * Q.stride + tile];
+    } else if (Q.row_meta) {
+        const KNP_GLOBAL RowMeta *m = gp(Q.row_meta) + pre;
+        const uint32_t m_begin = m->begin, m_first = m->first_tile, m_n = m->n_tiles, m_dir = m->dir;
+        const uint32_t k = tile - m_first;  // wraps above n_tiles for tiles before the row
+        if (k < m_n) e = gp(Q.row_dir)[m_dir + k];
+        base = m_begin;
+    } else {
+        const uint64_t r0 = gp(Q.row_ptr)[pre], r1 = gp(Q.row_ptr)[pre + 1];
+        b = (uint32_t)r0;
+        len = urgent = (uint32_t)(r1 - r0);
+        return;
+    }
+    b = base + (uint32_t)e;
+    len = (uint32_t)(e >> 32) & ((1u << kPartLenBits) - 1u);
+    const uint32_t u = (uint32_t)(e >> 56);
+    urgent = u == kUrgentAll ? len : u;
 }
 
 // Stages 1-3 of the tile scatter of step t: numbers the spiking rows that may reach the tile (prefix over
@@ -798,32 +824,7 @@ __device__ __forceinline__ bool tile_gather(const NetDev *__restrict__ net, cons
                 q = c.vproj[lo] & (kTileSegChannel - 1u);
             }
             if (lo != 0xFFFFFFFFu) pre = ld_u32<COHERENT>(c.vlist[lo] + par_off + 1 + (row - c.vpfx[lo]));
-            const TileProj &Q = sh.tp[q];
-            if (pre < Q.n_pre) {
-                if (Q.tile_dir) {
-                    const size_t at = (size_t)pre * Q.stride + c.td.tile;
-                    const KNP_GLOBAL uint32_t *d = gp(Q.tile_dir) + at;
-                    const uint32_t d0 = d[0], d1 = d[1];
-                    b = d0;
-                    len = d1 - d0;
-                    urgent = SPLIT && Q.urgent ? (uint32_t)gp(Q.urgent)[at] : len;
-                } else if (Q.row_meta) {
-                    const KNP_GLOBAL RowMeta *m = gp(Q.row_meta) + pre;
-                    const uint32_t m_begin = m->begin, m_first = m->first_tile, m_n = m->n_tiles, m_dir = m->dir;
-                    const uint32_t k = c.td.tile - m_first;  // wraps above n_tiles for tiles before the row
-                    if (k < m_n) {
-                        const uint32_t o0 = gp(Q.row_dir)[m_dir + k], o1 = gp(Q.row_dir)[m_dir + k + 1];
-                        b = m_begin + o0;
-                        len = o1 - o0;
-                        urgent = SPLIT && Q.urgent ? (uint32_t)gp(Q.urgent)[m_dir + k] : len;
-                    }
-                } else {
-                    const uint64_t r0 = gp(Q.row_ptr)[pre], r1 = gp(Q.row_ptr)[pre + 1];
-                    b = (uint32_t)r0;
-                    len = (uint32_t)(r1 - r0);
-                    urgent = len;
-                }
-            }
+            if (pre < sh.tp[q].n_pre) part_of(sh.tp[q], pre, c.td.tile, b, len, urgent);
         }
         my_parts += len ? 1u : 0u;
         if (SPLIT && rb == 0) {
