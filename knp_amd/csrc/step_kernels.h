@@ -185,10 +185,14 @@ __device__ __forceinline__ const T &entry_of(const T *table, uint32_t i, const T
 
 // The BLIFAT update of neuron i of population P at step t (blifat_impl.h:31-168): returns the spike and sets
 // cand (synaptic-resource neurons: a training candidate). FUSED: a_in is the exact sum of the step's impacts
-// (else it is read from the ring and the slot cleared).
-template <uint32_t KIND, bool FUSED>
+// (else it is read from the ring and the slot cleared). KEEP (default BLIFAT only): the potential and the
+// last spike step go to the caller's registers (V_keep, ls_keep), which store them once a run of steps is
+// done; the default BLIFAT step reads neither.
+template <uint32_t KIND, bool FUSED, bool KEEP = false>
 __device__ __forceinline__ bool population_update(const NetDev *__restrict__ net, const PopDev &P, uint32_t i,
-                                                  bool active, uint64_t t, int64_t a_in, bool &cand) {
+                                                  bool active, uint64_t t, int64_t a_in, bool &cand,
+                                                  float *V_keep = nullptr, uint32_t *ls_keep = nullptr) {
+    static_assert(!KEEP || KIND == POP_LEAN, "only the default BLIFAT step keeps its state in registers");
     const uint32_t slot = (uint32_t)(t % P.D);
     const size_t ai = (size_t)slot * P.n + i;
     bool spike = false;
@@ -208,11 +212,13 @@ __device__ __forceinline__ bool population_update(const NetDev *__restrict__ net
             spike = can && fixed_ge(a, thr, P.scale, V);
             if (spike) {
                 V = pf(P.V_reset, i);
-                gp(P.last_spike)[i] = (uint32_t)t;
+                if (KEEP) *ls_keep = (uint32_t)t;
+                else gp(P.last_spike)[i] = (uint32_t)t;
             }
             const float vmin = pf(P.V_min, i);
             if (V < vmin) V = vmin;
-            gp(P.V)[i] = V;
+            if (KEEP) *V_keep = V;
+            else gp(P.V)[i] = V;
         } else {
             // pre-impact (blifat_impl.h:31-65)
             uint32_t n_ts = ts_inc(gp(P.n_ts)[i]);
@@ -981,9 +987,11 @@ __device__ __forceinline__ uint32_t dual_segment_write(const NetDev *__restrict_
 
 // Phase 1 of step t1 for a persistent workgroup with both halves busy: the tile's neurons on threads 0-255
 // and its first source block on threads 256-511, compacted together; further source blocks follow.
+// KIND POP_LEAN: the potential and last spike step of the thread's neuron stay in V_keep / ls_keep.
 template <uint32_t KIND>
 __device__ __forceinline__ uint32_t tile_phase1_dual(const NetDev *__restrict__ net, const TileCtx &c,
-                                                     const PhaseCache &pc, uint64_t t1, int64_t a) {
+                                                     const PhaseCache &pc, uint64_t t1, int64_t a, float &V_keep,
+                                                     uint32_t &ls_keep) {
     const uint32_t tid = threadIdx.x;
     const size_t par_off = (size_t)(t1 & 1) * net->list_pstride;
     const uint32_t n_src = pc.n_src;
@@ -992,7 +1000,9 @@ __device__ __forceinline__ uint32_t tile_phase1_dual(const NetDev *__restrict__ 
     if (tid < (uint32_t)kBlock) {
         const uint32_t i = c.tile_lo + tid;
         bool cand;
-        if (KIND == POP_LEAN || (KIND == kMixedKinds && c.kind == POP_LEAN))
+        if (KIND == POP_LEAN)
+            pred = population_update<POP_LEAN, true, true>(net, pc.pop, i, i < c.n_pop, t1, a, cand, &V_keep, &ls_keep);
+        else if (KIND == kMixedKinds && c.kind == POP_LEAN)
             pred = population_update<POP_LEAN, true>(net, pc.pop, i, i < c.n_pop, t1, a, cand);
         else
             pred = population_update<POP_FULL, true>(net, pc.pop, i, i < c.n_pop, t1, a, cand);
@@ -1127,6 +1137,14 @@ __global__ __launch_bounds__(kTileThreads) void k_tile_run(const NetDev *__restr
     __syncthreads();
     __shared__ PhaseCache pc;
     phase_cache(net, c, pc);
+    // default BLIFAT: the thread's neuron's potential and last spike step, kept in registers for the launch
+    const bool keeper = KIND == POP_LEAN && tid < (uint32_t)kBlock && c.tile_lo + tid < c.n_pop;
+    float V_keep = 0.f;
+    uint32_t ls_keep = 0;
+    if (keeper) {
+        V_keep = gp(pc.pop.V)[c.tile_lo + tid];
+        ls_keep = gp(pc.pop.last_spike)[c.tile_lo + tid];
+    }
     uint32_t my_ev = 0, my_parts = 0;
     unsigned long long my_rows = 0;  // thread 0: rows published by this workgroup's phase 1
     if (blockIdx.x == 0) {
@@ -1149,7 +1167,7 @@ __global__ __launch_bounds__(kTileThreads) void k_tile_run(const NetDev *__restr
             a = (int64_t)c.ring[s0 * kBlock + tid];
             c.ring[s0 * kBlock + tid] = 0;
         }
-        my_rows += tile_phase1_dual<KIND>(net, c, pc, t0, a);
+        my_rows += tile_phase1_dual<KIND>(net, c, pc, t0, a, V_keep, ls_keep);
     }
     for (uint32_t k = 0; k < n; ++k) {
         const uint64_t t = t0 + k;
@@ -1165,11 +1183,15 @@ __global__ __launch_bounds__(kTileThreads) void k_tile_run(const NetDev *__restr
                 a = (int64_t)c.ring[s1 * kBlock + tid];
                 c.ring[s1 * kBlock + tid] = 0;
             }
-            my_rows += tile_phase1_dual<KIND>(net, c, pc, t + 1, a);
+            my_rows += tile_phase1_dual<KIND>(net, c, pc, t + 1, a, V_keep, ls_keep);
         }
         tile_add_parts(c, sh, (uint32_t)(t % c.D), part.b, part.len, part.q, my_ev);
     }
-    // the ring back to HBM
+    // the ring and the kept neuron state back to HBM
+    if (keeper) {
+        gp(pc.pop.V)[c.tile_lo + tid] = V_keep;
+        gp(pc.pop.last_spike)[c.tile_lo + tid] = ls_keep;
+    }
     __syncthreads();
     uint32_t my_cells = 0;
     for (uint32_t cc = tid; cc < c.D * kBlock; cc += kTileThreads) {
