@@ -679,7 +679,7 @@ __device__ __forceinline__ void thread_add_records(const TileCtx &c, uint32_t sl
 
 // Stage 3: every thread's row part (records [b, b + len) of projection q) into the ring. Short parts (the
 // common case) by their own thread; long parts spread over the workgroup, one thread per record, through
-// the row tables. Ends with a workgroup barrier.
+// the row tables. Ends with a workgroup barrier (after which the row tables are free again).
 __device__ __forceinline__ void tile_add_parts(const TileCtx &c, TileShared &sh, uint32_t slot0, uint32_t b,
                                                uint32_t len, uint32_t q, uint32_t &my_ev) {
     const uint32_t tid = threadIdx.x;
@@ -701,8 +701,8 @@ __device__ __forceinline__ void tile_add_parts(const TileCtx &c, TileShared &sh,
             }
             if (add_record(c, slot0, gp(sh.tp[sh.rq[lo]].syn8)[sh.rval[lo] + (s - sh.rpfx[lo])])) ++my_ev;
         }
+        __syncthreads();
     }
-    __syncthreads();
 }
 
 // Where row pre's part in the tile lies: records [b, b + len) of the projection, the first `urgent` of
@@ -981,7 +981,7 @@ __device__ __forceinline__ uint32_t dual_segment_write(const NetDev *__restrict_
         }
         if (vp[h] != 0xFFFFFFFFu) st_u32<COHERENT>(&net->vcount[par * net->n_vseg + vp[h]], total);
     }
-    __syncthreads();
+    // wcnt is next written at the following call, after the caller's barriers in between
     return rows;
 }
 
