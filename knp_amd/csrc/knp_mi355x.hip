@@ -732,7 +732,8 @@ int init_network(knp_net *net) {
             }
             int rc2;
             uint64_t *rd;
-            if ((rc2 = upload(net, &rd, rec.data(), S))) return rc2;
+            rec.push_back(0);  // one record past the end: the device loads records in aligned pairs
+            if ((rc2 = upload(net, &rd, rec.data(), S + 1))) return rc2;
             Q.syn8 = rd;
         }
         if (proj_dir[q] == DIR_DENSE) {

@@ -663,27 +663,53 @@ __device__ __forceinline__ bool add_record(const TileCtx &c, uint32_t slot0, uin
     return true;
 }
 
-// The thread adds the n records at src, 8 loads in flight at a time.
-__device__ __forceinline__ void thread_add_records(const TileCtx &c, uint32_t slot0, const KNP_GLOBAL uint64_t *src,
-                                                   uint32_t n, uint32_t &my_ev) {
-    for (uint32_t k0 = 0; k0 < n; k0 += 8) {
-        uint64_t rec[8];
+// The thread adds the records [b, b + n) of syn8, 8 loads in flight at a time. PAIRS: 16 bytes (an aligned
+// pair of records) per load, for long parts (fewer memory requests); the array holds one record past its
+// end, so the pair of its last record is readable. Else 8 bytes per load (fewer instructions for the short
+// parts of sparse networks).
+template <bool PAIRS>
+__device__ __forceinline__ void thread_add_records(const TileCtx &c, uint32_t slot0, const KNP_GLOBAL uint64_t *syn8,
+                                                   uint32_t b, uint32_t n, uint32_t &my_ev) {
+    if constexpr (!PAIRS) {
+        const KNP_GLOBAL uint64_t *src = syn8 + b;
+        for (uint32_t k0 = 0; k0 < n; k0 += 8) {
+            uint64_t rec[8];
 #pragma unroll
-        for (uint32_t u = 0; u < 8; ++u)
-            if (k0 + u < n) rec[u] = src[k0 + u];
+            for (uint32_t u = 0; u < 8; ++u)
+                if (k0 + u < n) rec[u] = src[k0 + u];
 #pragma unroll
-        for (uint32_t u = 0; u < 8; ++u)
-            if (k0 + u < n && add_record(c, slot0, rec[u])) ++my_ev;
+            for (uint32_t u = 0; u < 8; ++u)
+                if (k0 + u < n && add_record(c, slot0, rec[u])) ++my_ev;
+        }
+        return;
+    }
+    const uint32_t end = b + n;
+    for (uint32_t p0 = n ? (b & ~1u) : end; p0 < end; p0 += 8) {
+        v4u pr[4];
+#pragma unroll
+        for (uint32_t u = 0; u < 4; ++u)
+            if (p0 + 2 * u < end) pr[u] = *reinterpret_cast<const KNP_GLOBAL v4u *>(syn8 + p0 + 2 * u);
+#pragma unroll
+        for (uint32_t u = 0; u < 4; ++u)
+#pragma unroll
+            for (uint32_t h = 0; h < 2; ++h) {
+                const uint32_t k = p0 + 2 * u + h;
+                if (k >= b && k < end) {
+                    const uint64_t rec = h ? ((uint64_t)pr[u].w << 32 | pr[u].z) : ((uint64_t)pr[u].y << 32 | pr[u].x);
+                    if (add_record(c, slot0, rec)) ++my_ev;
+                }
+            }
     }
 }
 
 // Stage 3: every thread's row part (records [b, b + len) of projection q) into the ring. Short parts (the
 // common case) by their own thread; long parts spread over the workgroup, one thread per record, through
 // the row tables. Ends with a workgroup barrier (after which the row tables are free again).
+template <bool PAIRS>
 __device__ __forceinline__ void tile_add_parts(const TileCtx &c, TileShared &sh, uint32_t slot0, uint32_t b,
                                                uint32_t len, uint32_t q, uint32_t &my_ev) {
     const uint32_t tid = threadIdx.x;
-    if (len <= kTileShortPart) thread_add_records(c, slot0, gp(sh.tp[q].syn8) + b, len, my_ev);
+    if (len <= kTileShortPart) thread_add_records<PAIRS>(c, slot0, gp(sh.tp[q].syn8), b, len, my_ev);
     const uint32_t llen = len > kTileShortPart ? len : 0u;
     if (__syncthreads_or(llen != 0)) {
         uint32_t n_syn;
@@ -835,12 +861,12 @@ __device__ __forceinline__ bool tile_gather(const NetDev *__restrict__ net, cons
         my_parts += len ? 1u : 0u;
         if (SPLIT && rb == 0) {
             // the part's delay-1 records now, the rest after phase 1 of t + 1
-            thread_add_records(c, slot0, gp(sh.tp[q].syn8) + b, urgent, my_ev);
+            thread_add_records<false>(c, slot0, gp(sh.tp[q].syn8), b, urgent, my_ev);
             *part = TilePart{b + urgent, len - urgent, q};
             __syncthreads();  // the sums are complete before phase 1 reads them
             continue;
         }
-        tile_add_parts(c, sh, slot0, b, len, q, my_ev);
+        tile_add_parts<!COHERENT>(c, sh, slot0, b, len, q, my_ev);
     }
     return true;
 }
@@ -1185,7 +1211,7 @@ __global__ __launch_bounds__(kTileThreads) void k_tile_run(const NetDev *__restr
             }
             my_rows += tile_phase1_dual<KIND>(net, c, pc, t + 1, a, V_keep, ls_keep);
         }
-        tile_add_parts(c, sh, (uint32_t)(t % c.D), part.b, part.len, part.q, my_ev);
+        tile_add_parts<false>(c, sh, (uint32_t)(t % c.D), part.b, part.len, part.q, my_ev);
     }
     // the ring and the kept neuron state back to HBM
     if (keeper) {
