@@ -378,6 +378,87 @@ def test_fused_steps_match_oracle():
     assert runs["persistent"][2] == runs["stepwise"][2] == runs["atomic"][2] == runs["oracle"][2]
 
 
+def heavy_parts_network(o, seed):
+    """Every population tiled, with the persistent kernel's rarer paths in use: more than 512 spiking rows per
+    tile and step (a 2,048-channel source at p = 0.3 read whole by every tile: several gather passes), row
+    parts of 300 records (200 of delay 1) and of 270 records of delay 1 (more than 255: the whole part is
+    added before phase 1), long parts on both sides of phase 1, a ranged directory (rows of A spanning one tile of the
+    16-tile population B), pushed channel messages, and two source blocks per tile. Weights on a 2^-8 grid
+    keep FP32 potentials exact."""
+    rng = np.random.default_rng(seed)
+    na, nb = 1024, 4096
+    a = o.add_blifat_population(abi.neuron_array(abi.blifat_defaults(), na))
+    b = o.add_blifat_population(abi.neuron_array(abi.blifat_defaults(min_potential=-4.0), nb))
+    nch = 2048
+    src = o.add_delta_projection(abi.KNP_NO_POPULATION, nch, a, np.arange(nch, dtype=np.uint32),
+                                 (np.arange(nch, dtype=np.uint32) * 7) % na, np.full(nch, 0.375, np.float32),
+                                 np.ones(nch, np.uint32))
+    o.add_bernoulli_source(src, np.full(nch, workloads.bernoulli_threshold(0.3), np.uint32), seed=11)
+    ident = np.arange(na, dtype=np.uint32)
+    push = o.add_delta_projection(abi.KNP_NO_POPULATION, na, a, ident, ident, np.full(na, 1.0, np.float32),
+                                  np.ones(na, np.uint32))
+    # A -> A: every row 300 synapses into tile 0 (200 of delay 1, 100 longer: long parts on both sides of
+    # phase 1) and 270 of delay 1 into tile 1 (more than 255: the whole part before phase 1), half inhibitory
+    for tile, n_d1, per in ((0, 200, 300), (1, 270, 270)):
+        pre = np.repeat(np.arange(na, dtype=np.uint32), per)
+        post = (tile * 256 + rng.integers(0, 256, na * per)).astype(np.uint32)
+        delay = np.tile(np.concatenate([np.ones(n_d1), rng.integers(2, 6, per - n_d1)]), na).astype(np.uint32)
+        w = (rng.integers(1, 3, na * per) / 256.0).astype(np.float32)
+        ty = rng.choice([abi.EXCITATORY, abi.INHIBITORY_CURRENT], na * per).astype(np.uint8)
+        o.add_delta_projection(a, na, a, pre, post, w, delay, ty)
+    # A -> A: random, 64 per row, delays 1..8
+    S = na * 64
+    o.add_delta_projection(a, na, a, rng.integers(0, na, S).astype(np.uint32), rng.integers(0, na, S).astype(np.uint32),
+                           (rng.integers(4, 64, S) / 256.0).astype(np.float32), rng.integers(1, 9, S).astype(np.uint32),
+                           rng.choice([abi.EXCITATORY, abi.INHIBITORY_CURRENT], S, p=[0.7, 0.3]).astype(np.uint8))
+    # A -> B: row i into tile i % 16 of B only (ranged directory), 48 per row, delays 1..3
+    S = na * 48
+    pre = np.repeat(np.arange(na, dtype=np.uint32), 48)
+    post = ((pre % 16) * 256 + rng.integers(0, 256, S)).astype(np.uint32)
+    o.add_delta_projection(a, na, b, pre, post, (rng.integers(128, 320, S) / 256.0).astype(np.float32),
+                           rng.integers(1, 4, S).astype(np.uint32),
+                           rng.choice([abi.EXCITATORY, abi.INHIBITORY_CURRENT], S, p=[0.8, 0.2]).astype(np.uint8))
+    return (a, b), push
+
+
+def test_persistent_rare_paths_match_oracle():
+    """heavy_parts_network: persistent launches, one launch per step and atomics against the oracle."""
+    say("case: persistent kernel rare paths (several passes, parts of 300 records, ranged directory)")
+    chunks = [1, 12, 3, 20]
+    steps = sum(chunks)
+    runs = {}
+    modes = {"persistent": abi.SCATTER_AUTO, "stepwise": abi.SCATTER_STEPWISE, "atomic": abi.SCATTER_ATOMIC}
+    for name in ("persistent", "stepwise", "atomic", "oracle"):
+        o = Oracle("exact") if name == "oracle" else MI355XBackend(0)
+        if name != "oracle":
+            o.set_scatter_mode(modes[name])
+            o.enable_timing(True)
+        pops, push = heavy_parts_network(o, 21)
+        if name in ("persistent", "stepwise"):
+            assert all(o.tiled(p) for p in pops)
+        for p in pops:
+            o.record_spikes(p, steps)
+        rng = np.random.default_rng(9)
+        pots = []
+        for c in chunks:
+            o.push_spikes(push, np.nonzero(rng.random(1024) < 0.1)[0].astype(np.uint32))
+            o.step(c)
+            pots.append([o.potentials(p).copy() for p in pops])
+        runs[name] = ([o.read_spike_record(p) for p in pops], pots, o.events)
+        if name == "persistent":
+            assert o.kernel_time(abi.KERNEL_TILE_RUN)[1] > 0
+    for k in range(2):
+        so = runs["oracle"][0][k]
+        assert so.sum() > 1000, k
+        for name in ("persistent", "stepwise", "atomic"):
+            assert np.array_equal(runs[name][0][k], so), (name, k, np.argwhere(runs[name][0][k] != so)[:5])
+    for j in range(len(chunks)):
+        for k in range(2):
+            for name in ("persistent", "stepwise", "atomic"):
+                assert rel_err(runs[name][1][j][k], runs["oracle"][1][j][k]) <= POT_RTOL, (name, j, k)
+    assert runs["persistent"][2] == runs["stepwise"][2] == runs["atomic"][2] == runs["oracle"][2]
+
+
 def agreement(a, b):
     inter = np.logical_and(a, b).sum()
     union = np.logical_or(a, b).sum()
