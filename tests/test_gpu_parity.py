@@ -383,8 +383,8 @@ def heavy_parts_network(o, seed):
     tile and step (a 2,048-channel source at p = 0.3 read whole by every tile: several gather passes), row
     parts of 300 records (200 of delay 1) and of 270 records of delay 1 (more than 255: the whole part is
     added before phase 1), long parts on both sides of phase 1, a ranged directory (rows of A spanning one tile of the
-    16-tile population B), pushed channel messages, and two source blocks per tile. Weights on a 2^-8 grid
-    keep FP32 potentials exact."""
+    16-tile population B), pushed channel messages, and source blocks run by the workgroups of both
+    populations' tiles. Weights on a 2^-8 grid keep FP32 potentials exact."""
     rng = np.random.default_rng(seed)
     na, nb = 1024, 4096
     a = o.add_blifat_population(abi.neuron_array(abi.blifat_defaults(), na))
