@@ -572,8 +572,8 @@ struct TilePart {
 };
 
 // Dynamic LDS of the tile kernels: the ring sums [tile_max_d][256] (int64), then for the tile's segments
-// their spike lists [tile_max_list] (pointers, parity 0), count positions [tile_max_list], row-count prefix
-// [tile_max_list + 1], first three spike indexes [tile_max_list][3] and projection [tile_max_list].
+// their spike lists [tile_max_list] (pointers, parity 0), row-count prefix [tile_max_list + 1] and
+// projection [tile_max_list].
 __host__ __device__ inline size_t tile_lds_bytes(uint32_t max_d, uint32_t max_list) {
     return (size_t)max_d * kBlock * 8 + (size_t)max_list * 8 + ((size_t)max_list + 1) * 4 + max_list;
 }
