@@ -243,6 +243,8 @@ struct NetDev {
     uint32_t tile_max_d;       // deepest ring of a tiled population
     uint32_t n_src_blocks;     // workgroups of the Bernoulli sources (256 channels each)
     uint32_t *grid_bar;        // abort flag of persistent launches (word kBarAbort)
+    uint32_t spin_limit;       // polls before a persistent launch's wait gives up (kSpinLimit)
+    uint32_t hold_block;       // testing: this persistent workgroup withholds its spike counts (none: ~0u)
     unsigned long long *tile_part;  // [4 * n_tiles] impacts, rows, row segments, ring cells updated
     uint32_t blk_proj[64];     // projection index of each projection with BLOCKING synapses
     uint32_t n_dense;

@@ -104,6 +104,7 @@ struct knp_net {
     int scatter_mode = KNP_SCATTER_AUTO;
     bool fused = false;     // every population tiled: phase 1 of step t + 1 runs inside the tile kernel of t
     bool persistent = false;  // fused, and every tile fits on the device at once (cooperative launch)
+    uint32_t test_hold_block = 0xFFFFFFFFu, test_spin_limit = kSpinLimit;  // knp_mi355x_set_wait_test
     uint32_t run_kind = POP_LEAN;  // population kind of the persistent launch
     uint32_t tile_lds = 0;  // dynamic LDS bytes of the tile scatter (largest tiled ring)
     std::vector<uint32_t *> seg_list_host;
@@ -1116,6 +1117,8 @@ int init_network(knp_net *net) {
     {
         int rc;
         if ((rc = dev_alloc(net, (void **)&T.grid_bar, kBarWords * 4))) return rc;
+        T.spin_limit = net->test_spin_limit;
+        T.hold_block = net->test_hold_block;
     }
 
     // ---- control blocks
@@ -2033,6 +2036,17 @@ int knp_mi355x_get_tile_counters(knp_net *net, uint64_t *row_parts, uint64_t *ri
         *ring_cells += tp[4 * b + 3];
     }
     return KNP_OK;
+}
+
+int knp_mi355x_set_wait_test(knp_net *net, uint32_t hold_block, uint32_t spin_limit) {
+    if (!net) return fail(KNP_ERR_INVALID_ARGUMENT, "null handle");
+    if (spin_limit < 64) return fail(KNP_ERR_INVALID_ARGUMENT, "spin limit below 64 polls");
+    net->test_hold_block = hold_block;
+    net->test_spin_limit = spin_limit;
+    if (!net->initialized) return KNP_OK;
+    net->table.hold_block = hold_block;
+    net->table.spin_limit = spin_limit;
+    return upload_table(net);
 }
 
 int knp_mi355x_set_scatter_mode(knp_net *net, int mode) {

@@ -101,9 +101,9 @@ __device__ __forceinline__ uint32_t ld_u32(const uint32_t *p) {
 constexpr uint32_t kStampShift = 12, kCountMask = (1u << kStampShift) - 1u;
 __device__ __forceinline__ uint32_t list_stamp(uint64_t t) { return (uint32_t)(t % 0xFFFFFull) + 1u; }
 
-// Bounded waits of a persistent launch: after kSpinLimit polls, or once another workgroup has given up, a
-// wait ends with a failure, raises the abort flag (word kBarAbort of grid_bar) and the launch stops (the
-// host reports the failure), so no wait lasts forever.
+// Bounded waits of a persistent launch: after kSpinLimit polls (NetDev::spin_limit), or once another
+// workgroup has given up, a wait ends with a failure, raises the abort flag (word kBarAbort of grid_bar) and
+// the launch stops (the host reports the failure), so no wait lasts forever.
 constexpr uint32_t kSpinLimit = 1u << 20;
 constexpr uint32_t kBarAbort = 0, kBarWords = 32;  // words of grid_bar: the abort flag
 
@@ -797,7 +797,7 @@ __device__ __forceinline__ bool tile_gather(const NetDev *__restrict__ net, cons
                 while ((h.x >> kStampShift) != stamp) {  // the writer has not published step t yet
                     __builtin_amdgcn_s_sleep(16);  // polls are spaced out: frequent ones slow the stores
                     if ((++spins & 63u) == 0 &&
-                        (spins > kSpinLimit || ld_u32<true>(net->grid_bar + kBarAbort) != 0)) {
+                        (spins > net->spin_limit || ld_u32<true>(net->grid_bar + kBarAbort) != 0)) {
                         st_u32<true>(net->grid_bar + kBarAbort, 1u);
                         ok = false;
                         break;
@@ -1001,7 +1001,8 @@ __device__ __forceinline__ uint32_t dual_segment_write(const NetDev *__restrict_
         const uint32_t *vp = upper ? vp_b : vp_a;
         const size_t par = (size_t)(t & 1);
         if (h == 0) {
-            st_u32<COHERENT>(&seg[0], COHERENT ? total | list_stamp(t) << kStampShift : total);
+            if (!COHERENT || blockIdx.x != net->hold_block)  // (a test may have one workgroup withhold)
+                st_u32<COHERENT>(&seg[0], COHERENT ? total | list_stamp(t) << kStampShift : total);
             st_u32<COHERENT>(&net->seg_count[par * net->n_seg + sg], total);
             for (uint32_t k = 0; k < 4; ++k) rows += vp[k] != 0xFFFFFFFFu ? total : 0u;
         }

@@ -378,6 +378,27 @@ def test_fused_steps_match_oracle():
     assert runs["persistent"][2] == runs["stepwise"][2] == runs["atomic"][2] == runs["oracle"][2]
 
 
+def test_persistent_wait_ends_with_an_error():
+    """A persistent launch in which one workgroup withholds its spike counts (testing hook, a small bound on
+    every wait): the waits end, the call reports the failure instead of hanging, and a new network on the
+    same device steps normally afterwards."""
+    say("case: persistent launch with a withheld segment (bounded waits end with an error)")
+    g = MI355XBackend(0)
+    pops, push = all_tiled_network(g, 8)
+    g.step(3)
+    assert g.tiled(pops[0])
+    g.set_wait_test(0, 4096)
+    with pytest.raises(BackendError, match="timed out"):
+        g.step(20)
+    g.close()
+    o, h = MI355XBackend(0), Oracle("exact")
+    for x in (o, h):
+        all_tiled_network(x, 8)
+        x.record_spikes(0, 12)
+        x.step(12)
+    assert np.array_equal(o.read_spike_record(0), h.read_spike_record(0))
+
+
 def heavy_parts_network(o, seed):
     """Every population tiled, with the persistent kernel's rarer paths in use: more than 512 spiking rows per
     tile and step (a 2,048-channel source at p = 0.3 read whole by every tile: several gather passes), row
