@@ -25,7 +25,9 @@ constexpr uint32_t kSegStride = 272;
 constexpr int kTileThreads = 512;   // workgroup of the tile-owner scatter (8 wave64)
 constexpr int kTileMaxD = 32;       // ring depth limit of tile-owned populations (LDS: D x 256 x 8 bytes)
 constexpr int kTileMaxList = 2048;  // virtual segments that may reach one tile
-constexpr uint32_t kTileShortPart = 16;  // row parts up to this length are expanded by the row's own thread
+// Row parts up to this length are expanded by the row's own thread (longer ones by the whole workgroup):
+// in the persistent kernel, and in the one-launch-per-step kernel (where more workgroups share each CU).
+constexpr uint32_t kTileShortPart = 16, kTileShortPartStepwise = 32;
 
 enum PopKind : uint32_t { POP_LEAN = 0, POP_FULL = 1, POP_RESOURCE = 2 };
 enum ProjKind : uint32_t { PROJ_DELTA = 0, PROJ_RESOURCE = 1, PROJ_ADDITIVE = 2 };

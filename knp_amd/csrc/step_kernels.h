@@ -704,13 +704,15 @@ __device__ __forceinline__ void thread_add_records(const TileCtx &c, uint32_t sl
 
 // Stage 3: every thread's row part (records [b, b + len) of projection q) into the ring. Short parts (the
 // common case) by their own thread; long parts spread over the workgroup, one thread per record, through
-// the row tables. Ends with a workgroup barrier (after which the row tables are free again).
-template <bool PAIRS>
+// the row tables. Ends with a workgroup barrier (after which the row tables are free again). STEPWISE (the
+// one-launch-per-step kernel): records loaded in 16-byte pairs, parts up to kTileShortPartStepwise short.
+template <bool STEPWISE>
 __device__ __forceinline__ void tile_add_parts(const TileCtx &c, TileShared &sh, uint32_t slot0, uint32_t b,
                                                uint32_t len, uint32_t q, uint32_t &my_ev) {
+    constexpr uint32_t kShort = STEPWISE ? kTileShortPartStepwise : kTileShortPart;
     const uint32_t tid = threadIdx.x;
-    if (len <= kTileShortPart) thread_add_records<PAIRS>(c, slot0, gp(sh.tp[q].syn8), b, len, my_ev);
-    const uint32_t llen = len > kTileShortPart ? len : 0u;
+    if (len <= kShort) thread_add_records<STEPWISE>(c, slot0, gp(sh.tp[q].syn8), b, len, my_ev);
+    const uint32_t llen = len > kShort ? len : 0u;
     if (__syncthreads_or(llen != 0)) {
         uint32_t n_syn;
         const uint32_t ex = block_excl_scan<kTileThreads>(llen, sh.wsum, &n_syn);
