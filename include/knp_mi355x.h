@@ -120,8 +120,20 @@ typedef struct knp_resource_rule {
     uint8_t has_contributed;
 } knp_resource_rule;
 
+/* neuron_parameters<LIFNeuron>, knp/neuron-traits-library/include/knp/neuron-traits/lif.h:72-117.
+ * Same fields, types and meaning; knp_lif_defaults() returns default_values<LIFNeuron> (lif.h:43-67). */
+typedef struct knp_lif_neuron {
+    float potential;
+    float potential_reset_value;
+    float activation_threshold;
+    float leak_coefficient;
+    uint32_t refract_counter;
+    uint32_t refract_period;
+} knp_lif_neuron;
+
 /* Defaults of the reference trait structs. */
 void knp_blifat_defaults(knp_blifat_neuron *out);
+void knp_lif_defaults(knp_lif_neuron *out);
 void knp_resource_blifat_defaults(knp_resource_blifat_neuron *out);
 void knp_resource_rule_defaults(knp_resource_rule *out);
 
@@ -164,6 +176,10 @@ int knp_mi355x_get_tiled(knp_net *net, uint32_t pop, int *tiled);
 int knp_mi355x_add_blifat_population(knp_net *net, const knp_blifat_neuron *neurons, uint32_t n, uint32_t *out_pop);
 int knp_mi355x_add_resource_blifat_population(
     knp_net *net, const knp_resource_blifat_neuron *neurons, uint32_t n, uint32_t *out_pop);
+/* Population<LIFNeuron> (step: lif_impl.h:31-84). It accepts EXCITATORY and INHIBITORY_CURRENT impacts only:
+ * the reference throws "Unhandled synapse type." when another type reaches it (lif_impl.h:62-64); here the
+ * projection carrying it is rejected, with the same message, when the network is built. */
+int knp_mi355x_add_lif_population(knp_net *net, const knp_lif_neuron *neurons, uint32_t n, uint32_t *out_pop);
 
 /* Projection<DeltaSynapse>. Synapse i of the reference container is (pre[i], post[i], weight[i], delay[i],
  * output_type[i]); output_type may be NULL (all KNP_EXCITATORY). pre_pop is a population index or
@@ -249,6 +265,7 @@ int knp_mi355x_read_last_spikes(knp_net *net, uint32_t pop, uint32_t *idx, uint3
 int knp_mi355x_read_potentials(knp_net *net, uint32_t pop, float *out, uint32_t n);
 int knp_mi355x_read_blifat_neurons(knp_net *net, uint32_t pop, knp_blifat_neuron *out, uint32_t n);
 int knp_mi355x_read_resource_neurons(knp_net *net, uint32_t pop, knp_resource_blifat_neuron *out, uint32_t n);
+int knp_mi355x_read_lif_neurons(knp_net *net, uint32_t pop, knp_lif_neuron *out, uint32_t n);
 int knp_mi355x_read_weights(knp_net *net, uint32_t proj, float *out, uint64_t n_syn);
 int knp_mi355x_read_resource_rules(knp_net *net, uint32_t proj, knp_resource_rule *out, uint64_t n_syn);
 /* Number of SynapticImpacts created so far (sum over steps of spikes x out-degree over all projections). */

@@ -103,6 +103,18 @@ class ResourceBLIFATNeuron(C.Structure):
     ]
 
 
+class LIFNeuron(C.Structure):
+    """neuron_parameters<LIFNeuron> (lif.h:72-117)."""
+    _fields_ = [
+        ("potential", C.c_float),
+        ("potential_reset_value", C.c_float),
+        ("activation_threshold", C.c_float),
+        ("leak_coefficient", C.c_float),
+        ("refract_counter", C.c_uint32),
+        ("refract_period", C.c_uint32),
+    ]
+
+
 class ResourceRule(C.Structure):
     _fields_ = [
         ("synaptic_resource", C.c_float),
@@ -124,6 +136,16 @@ def blifat_defaults(**overrides):
     n.reversal_inhibitory_potential = -0.3
     n.min_potential = -1.0e9
     n.total_blocking_period = INT64_MAX
+    for k, v in overrides.items():
+        setattr(n, k, v)
+    return n
+
+
+def lif_defaults(**overrides):
+    """default_values<LIFNeuron> (lif.h:43-67) as a LIFNeuron struct."""
+    n = LIFNeuron()
+    n.activation_threshold = 1.0
+    n.leak_coefficient = 1.0
     for k, v in overrides.items():
         setattr(n, k, v)
     return n
@@ -171,7 +193,7 @@ def resource_rule_defaults(**overrides):
 
 
 def neuron_array(proto, n):
-    """A ctypes array of n copies of `proto` (a BLIFATNeuron or ResourceBLIFATNeuron)."""
+    """A ctypes array of n copies of `proto` (a BLIFATNeuron, ResourceBLIFATNeuron or LIFNeuron)."""
     arr = (type(proto) * n)()
     if n:
         view = np.frombuffer(arr, dtype=np.uint8).reshape(n, C.sizeof(proto))

@@ -88,6 +88,7 @@ struct PopDev {
     uint32_t *rec;         // recorded bitmaps [rec_cap][words]
     uint64_t rec_t0, rec_cap;
     uint32_t tiled;        // impacts into this population are added by the tile-owner scatter
+    uint32_t lif;          // FULL: LIFNeuron step on V, n_ts (refract counter), act, V_reset, V_decay (leak), refr
     ResDev res;
 };
 

@@ -40,9 +40,11 @@ int fail(int code, const std::string &msg) {
 
 struct HostPop {
     bool resource = false;
+    bool lif = false;  // LIFNeuron population (state in nl; nb stays empty)
     uint32_t n = 0;
     std::vector<knp_blifat_neuron> nb;           // initial state as loaded
     std::vector<knp_resource_blifat_neuron> nr;  // resource populations
+    std::vector<knp_lif_neuron> nl;              // LIF populations
     uint32_t kind = POP_FULL;
     uint32_t D = 1;
     bool has_cond = false, has_dop = false, has_block = false, has_forcing = false;
@@ -192,7 +194,7 @@ uint64_t ts_from_device(uint32_t v) {
 }
 
 bool lean_eligible(const HostPop &P) {
-    if (P.resource || P.has_cond || P.has_dop || P.has_block) return false;
+    if (P.resource || P.lif || P.has_cond || P.has_dop || P.has_block) return false;
     bool any_max = false, any_other = false;
     for (const auto &n : P.nb) {
         if (n.n_time_steps_since_last_firing == std::numeric_limits<uint64_t>::max()) any_max = true;
@@ -514,55 +516,74 @@ int init_network(knp_net *net) {
         if (H.has_dop && (rc = dev_alloc(net, (void **)&P.acc_d, ring * 8))) return rc;
         if (H.has_forcing && (rc = dev_alloc(net, (void **)&P.acc_f, ring * 4))) return rc;
         if (H.has_block && (rc = dev_alloc(net, (void **)&P.acc_b, ring * 8))) return rc;
-        const auto &nb = H.nb;
-        std::vector<float> Vh(n);
-        for (uint32_t i = 0; i < n; ++i) Vh[i] = (float)nb[i].potential;
-        if ((rc = upload(net, &P.V, Vh.data(), n))) return rc;
-        if ((rc = make_param_f(net, n, [&](uint32_t i) { return (float)nb[i].activation_threshold; }, &P.act))) return rc;
-        if ((rc = make_param_f(net, n, [&](uint32_t i) { return (float)nb[i].potential_reset_value; }, &P.V_reset))) return rc;
-        if ((rc = make_param_f(net, n, [&](uint32_t i) { return (float)nb[i].min_potential; }, &P.V_min))) return rc;
-        if (H.kind != POP_RESOURCE)
-            if ((rc = make_param_f(net, n, [&](uint32_t i) { return (float)nb[i].additional_threshold; }, &P.add_thr))) return rc;
-        if (H.kind == POP_LEAN) {
-            P.lean_ts_max = nb.empty() ? 0 : (nb[0].n_time_steps_since_last_firing == std::numeric_limits<uint64_t>::max());
-            std::vector<uint32_t> ls(n, 0xFFFFFFFFu);
-            if ((rc = upload(net, &P.last_spike, ls.data(), n))) return rc;
-        } else {
-            std::vector<float> f(n);
-            std::vector<uint32_t> u(n);
-            auto upf = [&](float **dst, auto get) {
-                for (uint32_t i = 0; i < n; ++i) f[i] = get(i);
-                return upload(net, dst, f.data(), n);
-            };
-            auto upu = [&](uint32_t **dst, auto get) {
-                for (uint32_t i = 0; i < n; ++i) u[i] = get(i);
-                return upload(net, dst, u.data(), n);
-            };
-            if ((rc = upf(&P.Vpre, [&](uint32_t i) { return (float)nb[i].pre_impact_potential; }))) return rc;
-            if ((rc = upf(&P.dyn, [&](uint32_t i) { return (float)nb[i].dynamic_threshold; }))) return rc;
-            if ((rc = upf(&P.trace, [&](uint32_t i) { return (float)nb[i].postsynaptic_trace; }))) return rc;
-            if ((rc = upf(&P.g, [&](uint32_t i) { return (float)nb[i].inhibitory_conductance; }))) return rc;
-            if ((rc = upf(&P.dop, [&](uint32_t i) { return (float)nb[i].dopamine_value; }))) return rc;
-            if ((rc = upu(&P.n_ts, [&](uint32_t i) { return ts_to_device(nb[i].n_time_steps_since_last_firing); }))) return rc;
-            if ((rc = upu(&P.bphase, [&](uint32_t i) { return nb[i].bursting_phase; }))) return rc;
-            bool track = H.has_block;
-            for (const auto &x : nb) track |= x.total_blocking_period < (int64_t(1) << 62);
-            P.track_tbp = track ? 1 : 0;
-            if (track) {
-                std::vector<int64_t> tb(n);
-                for (uint32_t i = 0; i < n; ++i) tb[i] = nb[i].total_blocking_period;
-                if ((rc = upload(net, &P.tbp, tb.data(), n))) return rc;
+        if (H.lif) {
+            // LIFNeuron (lif.h:72-117) on the FULL state arrays: V = potential_, n_ts = refract_counter_,
+            // V_decay = leak_coefficient_, refr = refract_period_
+            const auto &nl = H.nl;
+            P.lif = 1;
+            std::vector<float> Vh(n);
+            std::vector<uint32_t> rcnt(n);
+            for (uint32_t i = 0; i < n; ++i) {
+                Vh[i] = nl[i].potential;
+                rcnt[i] = nl[i].refract_counter;
             }
-            if ((rc = make_param_f(net, n, [&](uint32_t i) { return (float)nb[i].threshold_decay; }, &P.thr_decay))) return rc;
-            if ((rc = make_param_f(net, n, [&](uint32_t i) { return (float)nb[i].threshold_increment; }, &P.thr_inc))) return rc;
-            if ((rc = make_param_f(net, n, [&](uint32_t i) { return (float)nb[i].postsynaptic_trace_decay; }, &P.trace_decay))) return rc;
-            if ((rc = make_param_f(net, n, [&](uint32_t i) { return (float)nb[i].postsynaptic_trace_increment; }, &P.trace_inc))) return rc;
-            if ((rc = make_param_f(net, n, [&](uint32_t i) { return (float)nb[i].inhibitory_conductance_decay; }, &P.g_decay))) return rc;
-            if ((rc = make_param_f(net, n, [&](uint32_t i) { return (float)nb[i].potential_decay; }, &P.V_decay))) return rc;
-            if ((rc = make_param_f(net, n, [&](uint32_t i) { return (float)nb[i].reflexive_weight; }, &P.reflexive))) return rc;
-            if ((rc = make_param_f(net, n, [&](uint32_t i) { return (float)nb[i].reversal_inhibitory_potential; }, &P.E_inh))) return rc;
-            if ((rc = make_param_u(net, n, [&](uint32_t i) { return nb[i].bursting_period; }, &P.bperiod))) return rc;
-            if ((rc = make_param_u(net, n, [&](uint32_t i) { return nb[i].absolute_refractory_period; }, &P.refr))) return rc;
+            if ((rc = upload(net, &P.V, Vh.data(), n))) return rc;
+            if ((rc = upload(net, &P.n_ts, rcnt.data(), n))) return rc;
+            if ((rc = make_param_f(net, n, [&](uint32_t i) { return nl[i].activation_threshold; }, &P.act))) return rc;
+            if ((rc = make_param_f(net, n, [&](uint32_t i) { return nl[i].potential_reset_value; }, &P.V_reset))) return rc;
+            if ((rc = make_param_f(net, n, [&](uint32_t i) { return nl[i].leak_coefficient; }, &P.V_decay))) return rc;
+            if ((rc = make_param_u(net, n, [&](uint32_t i) { return nl[i].refract_period; }, &P.refr))) return rc;
+        } else {
+            const auto &nb = H.nb;
+            std::vector<float> Vh(n);
+            for (uint32_t i = 0; i < n; ++i) Vh[i] = (float)nb[i].potential;
+            if ((rc = upload(net, &P.V, Vh.data(), n))) return rc;
+            if ((rc = make_param_f(net, n, [&](uint32_t i) { return (float)nb[i].activation_threshold; }, &P.act))) return rc;
+            if ((rc = make_param_f(net, n, [&](uint32_t i) { return (float)nb[i].potential_reset_value; }, &P.V_reset))) return rc;
+            if ((rc = make_param_f(net, n, [&](uint32_t i) { return (float)nb[i].min_potential; }, &P.V_min))) return rc;
+            if (H.kind != POP_RESOURCE)
+                if ((rc = make_param_f(net, n, [&](uint32_t i) { return (float)nb[i].additional_threshold; }, &P.add_thr))) return rc;
+            if (H.kind == POP_LEAN) {
+                P.lean_ts_max = nb.empty() ? 0 : (nb[0].n_time_steps_since_last_firing == std::numeric_limits<uint64_t>::max());
+                std::vector<uint32_t> ls(n, 0xFFFFFFFFu);
+                if ((rc = upload(net, &P.last_spike, ls.data(), n))) return rc;
+            } else {
+                std::vector<float> f(n);
+                std::vector<uint32_t> u(n);
+                auto upf = [&](float **dst, auto get) {
+                    for (uint32_t i = 0; i < n; ++i) f[i] = get(i);
+                    return upload(net, dst, f.data(), n);
+                };
+                auto upu = [&](uint32_t **dst, auto get) {
+                    for (uint32_t i = 0; i < n; ++i) u[i] = get(i);
+                    return upload(net, dst, u.data(), n);
+                };
+                if ((rc = upf(&P.Vpre, [&](uint32_t i) { return (float)nb[i].pre_impact_potential; }))) return rc;
+                if ((rc = upf(&P.dyn, [&](uint32_t i) { return (float)nb[i].dynamic_threshold; }))) return rc;
+                if ((rc = upf(&P.trace, [&](uint32_t i) { return (float)nb[i].postsynaptic_trace; }))) return rc;
+                if ((rc = upf(&P.g, [&](uint32_t i) { return (float)nb[i].inhibitory_conductance; }))) return rc;
+                if ((rc = upf(&P.dop, [&](uint32_t i) { return (float)nb[i].dopamine_value; }))) return rc;
+                if ((rc = upu(&P.n_ts, [&](uint32_t i) { return ts_to_device(nb[i].n_time_steps_since_last_firing); }))) return rc;
+                if ((rc = upu(&P.bphase, [&](uint32_t i) { return nb[i].bursting_phase; }))) return rc;
+                bool track = H.has_block;
+                for (const auto &x : nb) track |= x.total_blocking_period < (int64_t(1) << 62);
+                P.track_tbp = track ? 1 : 0;
+                if (track) {
+                    std::vector<int64_t> tb(n);
+                    for (uint32_t i = 0; i < n; ++i) tb[i] = nb[i].total_blocking_period;
+                    if ((rc = upload(net, &P.tbp, tb.data(), n))) return rc;
+                }
+                if ((rc = make_param_f(net, n, [&](uint32_t i) { return (float)nb[i].threshold_decay; }, &P.thr_decay))) return rc;
+                if ((rc = make_param_f(net, n, [&](uint32_t i) { return (float)nb[i].threshold_increment; }, &P.thr_inc))) return rc;
+                if ((rc = make_param_f(net, n, [&](uint32_t i) { return (float)nb[i].postsynaptic_trace_decay; }, &P.trace_decay))) return rc;
+                if ((rc = make_param_f(net, n, [&](uint32_t i) { return (float)nb[i].postsynaptic_trace_increment; }, &P.trace_inc))) return rc;
+                if ((rc = make_param_f(net, n, [&](uint32_t i) { return (float)nb[i].inhibitory_conductance_decay; }, &P.g_decay))) return rc;
+                if ((rc = make_param_f(net, n, [&](uint32_t i) { return (float)nb[i].potential_decay; }, &P.V_decay))) return rc;
+                if ((rc = make_param_f(net, n, [&](uint32_t i) { return (float)nb[i].reflexive_weight; }, &P.reflexive))) return rc;
+                if ((rc = make_param_f(net, n, [&](uint32_t i) { return (float)nb[i].reversal_inhibitory_potential; }, &P.E_inh))) return rc;
+                if ((rc = make_param_u(net, n, [&](uint32_t i) { return nb[i].bursting_period; }, &P.bperiod))) return rc;
+                if ((rc = make_param_u(net, n, [&](uint32_t i) { return nb[i].absolute_refractory_period; }, &P.refr))) return rc;
+            }
         }
         if (H.kind == POP_RESOURCE) {
             ResDev &R = P.res;
@@ -570,7 +591,7 @@ int init_network(knp_net *net) {
             std::vector<float> f(n);
             std::vector<uint64_t> u64(n);
             std::vector<uint8_t> u8(n);
-            for (uint32_t i = 0; i < n; ++i) f[i] = (float)nb[i].additional_threshold;
+            for (uint32_t i = 0; i < n; ++i) f[i] = (float)H.nb[i].additional_threshold;
             if ((rc = upload(net, &R.add_thr, f.data(), n))) return rc;
             for (uint32_t i = 0; i < n; ++i) f[i] = nr[i].free_synaptic_resource;
             if ((rc = upload(net, &R.free_res, f.data(), n))) return rc;
@@ -1422,6 +1443,9 @@ int add_projection_common(knp_net *net, uint32_t kind, uint32_t pre_pop, uint32_
         if (H.delay[s] == 0 || H.delay[s] > 255)
             return fail(KNP_ERR_UNSUPPORTED, "synaptic delay must be in 1..255 (delay 0 is lost by the reference)");
         if (H.type[s] > KNP_BLOCKING) return fail(KNP_ERR_UNSUPPORTED, "Unhandled synapse type.");
+        // LIF neurons take EXCITATORY and INHIBITORY_CURRENT impacts only (lif_impl.h:44-64)
+        if (net->pops[post_pop].lif && H.type[s] > KNP_INHIBITORY_CURRENT)
+            return fail(KNP_ERR_UNSUPPORTED, "Unhandled synapse type.");
         if (!std::isfinite(H.w[s])) return fail(KNP_ERR_INVALID_ARGUMENT, "non-finite weight");
         H.max_delay = std::max(H.max_delay, H.delay[s]);
         if (H.type[s] == KNP_BLOCKING) H.has_blocking = true;
@@ -1443,6 +1467,12 @@ void knp_blifat_defaults(knp_blifat_neuron *o) {
     o->reversal_inhibitory_potential = -0.3;
     o->min_potential = -1.0e9;
     o->total_blocking_period = std::numeric_limits<int64_t>::max();
+}
+
+void knp_lif_defaults(knp_lif_neuron *o) {
+    std::memset(o, 0, sizeof(*o));
+    o->activation_threshold = 1.f;
+    o->leak_coefficient = 1.f;
 }
 
 void knp_resource_blifat_defaults(knp_resource_blifat_neuron *o) {
@@ -1535,6 +1565,20 @@ int knp_mi355x_add_blifat_population(knp_net *net, const knp_blifat_neuron *neur
     HostPop P;
     P.n = n;
     P.nb.assign(neurons, neurons + n);
+    net->pops.push_back(std::move(P));
+    if (out_pop) *out_pop = (uint32_t)net->pops.size() - 1;
+    return KNP_OK;
+}
+
+int knp_mi355x_add_lif_population(knp_net *net, const knp_lif_neuron *neurons, uint32_t n, uint32_t *out_pop) {
+    if (!net) return fail(KNP_ERR_INVALID_ARGUMENT, "null handle");
+    if (net->initialized) return fail(KNP_ERR_STATE, "populations must be loaded before the first step");
+    if (n && !neurons) return fail(KNP_ERR_INVALID_ARGUMENT, "null neuron array");
+    if (net->pops.size() >= (size_t)kMaxPops) return fail(KNP_ERR_UNSUPPORTED, "too many populations");
+    HostPop P;
+    P.lif = true;
+    P.n = n;
+    P.nl.assign(neurons, neurons + n);
     net->pops.push_back(std::move(P));
     if (out_pop) *out_pop = (uint32_t)net->pops.size() - 1;
     return KNP_OK;
@@ -1904,6 +1948,7 @@ static int export_blifat(knp_net *net, uint32_t pop, std::vector<knp_blifat_neur
 int knp_mi355x_read_blifat_neurons(knp_net *net, uint32_t pop, knp_blifat_neuron *out, uint32_t n) {
     int rc = check_pop(net, pop);
     if (rc) return rc;
+    if (net->pops[pop].lif) return fail(KNP_ERR_INVALID_ARGUMENT, "not a BLIFAT population");
     if (n > net->pops[pop].n || (n && !out)) return fail(KNP_ERR_INVALID_ARGUMENT, "bad output buffer");
     if ((rc = ensure_init(net))) return rc;
     std::vector<knp_blifat_neuron> v;
@@ -1914,6 +1959,25 @@ int knp_mi355x_read_blifat_neurons(knp_net *net, uint32_t pop, knp_blifat_neuron
         for (uint32_t i = 0; i < v.size(); ++i) v[i].additional_threshold = add[i];
     }
     std::copy(v.begin(), v.begin() + n, out);
+    return KNP_OK;
+}
+
+int knp_mi355x_read_lif_neurons(knp_net *net, uint32_t pop, knp_lif_neuron *out, uint32_t n) {
+    int rc = check_pop(net, pop);
+    if (rc) return rc;
+    if (!net->pops[pop].lif) return fail(KNP_ERR_INVALID_ARGUMENT, "not a LIF population");
+    if (n > net->pops[pop].n || (n && !out)) return fail(KNP_ERR_INVALID_ARGUMENT, "bad output buffer");
+    if ((rc = ensure_init(net))) return rc;
+    const PopDev &P = net->table.pops[pop];
+    std::vector<float> V(n);
+    std::vector<uint32_t> rcnt(n);
+    if ((rc = download(net, V.data(), P.V, n))) return rc;
+    if ((rc = download(net, rcnt.data(), P.n_ts, n))) return rc;
+    for (uint32_t i = 0; i < n; ++i) {
+        out[i] = net->pops[pop].nl[i];
+        out[i].potential = V[i];
+        out[i].refract_counter = rcnt[i];
+    }
     return KNP_OK;
 }
 

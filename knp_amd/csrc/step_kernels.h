@@ -219,6 +219,25 @@ __device__ __forceinline__ bool population_update(const NetDev *__restrict__ net
             if (V < vmin) V = vmin;
             if (KEEP) *V_keep = V;
             else gp(P.V)[i] = V;
+        } else if (P.lif) {
+            // LIFNeuron (lif_impl.h:31-84): while the refractory counter is non-zero the potential neither
+            // leaks nor takes impacts and the counter counts down; else leak, add the step's impacts and fire
+            // strictly above the threshold.
+            uint32_t rc = gp(P.n_ts)[i];
+            float V = gp(P.V)[i];
+            if (rc == 0) {
+                V *= pf(P.V_decay, i);
+                V += fixed_to_float(a, P.inv_scale);
+                if (V > pf(P.act, i)) {
+                    V = pf(P.V_reset, i);
+                    rc = pu(P.refr, i);
+                    spike = true;
+                }
+            } else {
+                --rc;
+            }
+            gp(P.V)[i] = V;
+            gp(P.n_ts)[i] = rc;
         } else {
             // pre-impact (blifat_impl.h:31-65)
             uint32_t n_ts = ts_inc(gp(P.n_ts)[i]);

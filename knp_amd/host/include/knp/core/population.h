@@ -9,6 +9,7 @@
 #include "uid.h"
 #include "../neuron-traits/blifat.h"
 #include "../neuron-traits/stdp_synaptic_resource_rule.h"
+#include "../neuron-traits/lif.h"
 
 namespace knp::core {
 

@@ -13,6 +13,18 @@ namespace {
 
 using BLIFAT = neuron_traits::neuron_parameters<neuron_traits::BLIFATNeuron>;
 using ResBLIFAT = neuron_traits::neuron_parameters<neuron_traits::SynapticResourceSTDPBLIFATNeuron>;
+using LIF = neuron_traits::neuron_parameters<neuron_traits::LIFNeuron>;
+
+knp_lif_neuron to_abi(const LIF &n) {
+    knp_lif_neuron o;
+    o.potential = n.potential_;
+    o.potential_reset_value = n.potential_reset_value_;
+    o.activation_threshold = n.activation_threshold_;
+    o.leak_coefficient = n.leak_coefficient_;
+    o.refract_counter = n.refract_counter_;
+    o.refract_period = n.refract_period_;
+    return o;
+}
 
 knp_blifat_neuron to_abi(const BLIFAT &n) {
     knp_blifat_neuron o;
@@ -112,7 +124,7 @@ MI355XBackend::~MI355XBackend() {
 }
 
 std::vector<std::string> MI355XBackend::get_supported_neurons() const {
-    return {"BLIFATNeuron", "SynapticResourceSTDPBLIFATNeuron"};
+    return {"BLIFATNeuron", "SynapticResourceSTDPBLIFATNeuron", "LIFNeuron"};
 }
 
 std::vector<std::string> MI355XBackend::get_supported_synapses() const {
@@ -184,6 +196,11 @@ void MI355XBackend::_init() {
             for (const auto &n : *p) a.push_back(to_abi(n));
             check(knp_mi355x_add_blifat_population(net_, a.data(), (uint32_t)a.size(), &idx), "load population");
             pop_uids_.push_back(p->get_uid());
+        } else if (auto *pl = std::get_if<core::Population<neuron_traits::LIFNeuron>>(&v)) {
+            std::vector<knp_lif_neuron> a;
+            for (const auto &n : *pl) a.push_back(to_abi(n));
+            check(knp_mi355x_add_lif_population(net_, a.data(), (uint32_t)a.size(), &idx), "load population");
+            pop_uids_.push_back(pl->get_uid());
         } else {
             const auto &p2 = std::get<core::Population<neuron_traits::SynapticResourceSTDPBLIFATNeuron>>(v);
             std::vector<knp_resource_blifat_neuron> a;
@@ -305,6 +322,13 @@ MI355XBackend::get_network_data() {
             std::vector<knp_blifat_neuron> a(p->size());
             check(knp_mi355x_read_blifat_neurons(net_, i, a.data(), (uint32_t)a.size()), "read neurons");
             for (size_t k = 0; k < a.size(); ++k) from_abi(a[k], (*p)[k]);
+        } else if (auto *pl = std::get_if<core::Population<neuron_traits::LIFNeuron>>(&pops[i])) {
+            std::vector<knp_lif_neuron> a(pl->size());
+            check(knp_mi355x_read_lif_neurons(net_, i, a.data(), (uint32_t)a.size()), "read neurons");
+            for (size_t k = 0; k < a.size(); ++k) {
+                (*pl)[k].potential_ = a[k].potential;
+                (*pl)[k].refract_counter_ = a[k].refract_counter;
+            }
         } else {
             auto &p2 = std::get<core::Population<neuron_traits::SynapticResourceSTDPBLIFATNeuron>>(pops[i]);
             std::vector<knp_resource_blifat_neuron> a(p2.size());

@@ -8,6 +8,7 @@
  *   - SingleThreadedCPUBackend::_step          knp/backends/cpu/cpu-single-threaded-backend/impl/backend.cpp:89-141
  *   - calculate_any_population / resource pop   knp/backends/cpu/cpu-library/include/knp/backends/cpu-library/populations_old.h:104-217
  *   - BLIFAT pre / impact / post                cpu-library/impl/populations/blifat/blifat_impl.h:31-168
+ *   - LIF pre / impact / post                   cpu-library/impl/populations/lif/lif_impl.h:31-84
  *   - resource STDP training                    cpu-library/impl/populations/blifat/blifat_stdp.h:35-172
  *                                               cpu-library/impl/populations/training/stdp.h:40-186
  *   - delta projection + delay queue            cpu-library/projections.h:58-84, impl/projections/delta/delta_impl.h:54-114
@@ -23,6 +24,9 @@
  *
  * Fixed choices where the reference leaves order to Boost's hashed index (projection.h:450-460):
  * synapses of one presynaptic (or postsynaptic) neuron are visited in ascending synapse index.
+ * A projection onto a LIF population with synapses other than EXCITATORY / INHIBITORY_CURRENT is rejected
+ * when it is added; the reference throws "Unhandled synapse type." once such an impact is delivered
+ * (lif_impl.h:62-64).
  */
 #include <math.h>
 #include <stdint.h>
@@ -85,9 +89,17 @@ typedef struct {
     int from_pop; /* population index of the sender, -1 for a channel */
 } ko_spike_msg;
 
+/* neuron_parameters<LIFNeuron> (lif.h:72-117); the reference keeps the reals in float */
+typedef struct {
+    wreal potential, potential_reset_value, activation_threshold, leak_coefficient;
+    uint32_t refract_counter, refract_period;
+} ko_lif;
+
 typedef struct {
     int resource;
+    int lif;
     uint32_t n;
+    ko_lif *nl;            /* LIF populations */
     knp_blifat_neuron *nb; /* BLIFAT fields, double as in the reference */
     ko_res_ext *rx;        /* resource fields (resource populations only) */
     int record;
@@ -180,6 +192,7 @@ void ko_destroy(ko_net *net) {
     if (!net) return;
     for (uint32_t i = 0; i < net->n_pops; ++i) {
         free(net->pops[i].nb);
+        free(net->pops[i].nl);
         free(net->pops[i].rx);
         free(net->pops[i].rec);
         free(net->pops[i].last_spikes);
@@ -221,6 +234,23 @@ static int add_pop(ko_net *net, uint32_t n, int resource) {
 int ko_add_blifat_population(ko_net *net, const knp_blifat_neuron *neurons, uint32_t n) {
     int id = add_pop(net, n, 0);
     memcpy(net->pops[id].nb, neurons, (size_t)n * sizeof(knp_blifat_neuron));
+    return id;
+}
+
+int ko_add_lif_population(ko_net *net, const knp_lif_neuron *neurons, uint32_t n) {
+    int id = add_pop(net, n, 0);
+    ko_pop *p = &net->pops[id];
+    p->lif = 1;
+    p->nl = (ko_lif *)xcalloc(n, sizeof(ko_lif));
+    for (uint32_t i = 0; i < n; ++i) {
+        ko_lif *x = &p->nl[i];
+        x->potential = neurons[i].potential;
+        x->potential_reset_value = neurons[i].potential_reset_value;
+        x->activation_threshold = neurons[i].activation_threshold;
+        x->leak_coefficient = neurons[i].leak_coefficient;
+        x->refract_counter = neurons[i].refract_counter;
+        x->refract_period = neurons[i].refract_period;
+    }
     return id;
 }
 
@@ -282,7 +312,10 @@ static int add_proj(ko_net *net, int kind, uint32_t pre_pop, uint32_t n_pre, uin
         p->w[s] = (wreal)weight[s];
         p->delay[s] = delay[s];
         p->type[s] = type ? type[s] : KNP_EXCITATORY;
-        if (p->type[s] > KNP_BLOCKING) { snprintf(net->err, sizeof net->err, "Unhandled synapse type."); return -1; }
+        if (p->type[s] > KNP_BLOCKING || (net->pops[post_pop].lif && p->type[s] > KNP_INHIBITORY_CURRENT)) {
+            snprintf(net->err, sizeof net->err, "Unhandled synapse type.");
+            return -1;
+        }
         if (delay[s] > p->max_delay) p->max_delay = delay[s];
     }
     /* by-presynaptic and by-postsynaptic indexes (counting sort keeps ascending synapse index) */
@@ -496,6 +529,30 @@ static int blifat_post(knp_blifat_neuron *n) {
     }
     if (n->potential < n->min_potential) n->potential = n->min_potential;
     return spike;
+}
+
+/* ---- LIF (lif_impl.h:31-84) ------------------------------------------------------------------- */
+static void lif_pre(ko_lif *n) {
+    if (0 == n->refract_counter) n->potential *= n->leak_coefficient;
+}
+
+static void lif_impact(ko_lif *n, const ko_impact *imp) {
+    if (0 != n->refract_counter) return; /* impacts are ignored while refractory */
+    if (imp->type == KNP_EXCITATORY) n->potential += imp->w;
+    else if (imp->type == KNP_INHIBITORY_CURRENT) n->potential -= imp->w;
+}
+
+static int lif_post(ko_lif *n) {
+    if (0 == n->refract_counter) {
+        if (n->potential > n->activation_threshold) {
+            n->potential = n->potential_reset_value;
+            n->refract_counter = n->refract_period;
+            return 1;
+        }
+    } else {
+        --n->refract_counter;
+    }
+    return 0;
 }
 
 /* ---- resource STDP training (blifat_stdp.h, training/stdp.h) --------------------------------- */
@@ -738,7 +795,8 @@ static void calculate_projection(ko_net *net, ko_proj *p, ko_spike_msg *msgs, ui
 static void population_step(ko_net *net, uint32_t pop, uint64_t step) {
     ko_pop *P = &net->pops[pop];
     for (uint32_t i = 0; i < P->n; ++i) {
-        if (P->resource) resource_pre(&P->nb[i], &P->rx[i]);
+        if (P->lif) lif_pre(&P->nl[i]);
+        else if (P->resource) resource_pre(&P->nb[i], &P->rx[i]);
         else blifat_pre(&P->nb[i]);
     }
     /* impact messages in delivery order: projections that sent last step, in load order */
@@ -748,6 +806,10 @@ static void population_step(ko_net *net, uint32_t pop, uint64_t step) {
         int is_forcing = p->kind == KO_PROJ_DELTA; /* stdp_dispatcher.h:54-58 */
         for (size_t k = 0; k < p->outbox.n; ++k) {
             const ko_impact *imp = &p->outbox.v[k];
+            if (P->lif) {
+                lif_impact(&P->nl[imp->post], imp);
+                continue;
+            }
             knp_blifat_neuron *n = &P->nb[imp->post];
             blifat_impact(n, imp);
             if (P->resource && imp->type == KNP_EXCITATORY) P->rx[imp->post].is_being_forced |= is_forcing;
@@ -755,7 +817,7 @@ static void population_step(ko_net *net, uint32_t pop, uint64_t step) {
     }
     P->n_last = 0;
     for (uint32_t i = 0; i < P->n; ++i)
-        if (blifat_post(&P->nb[i])) P->last_spikes[P->n_last++] = i;
+        if (P->lif ? lif_post(&P->nl[i]) : blifat_post(&P->nb[i])) P->last_spikes[P->n_last++] = i;
     if (P->resource) train_population(net, pop, P->last_spikes, P->n_last, step);
     if (P->record && P->rec_steps < P->rec_cap) {
         uint32_t words = (P->n + 31) / 32;
@@ -824,7 +886,22 @@ uint64_t ko_get_event_count(ko_net *net) { return net->events; }
 
 int ko_read_potentials(ko_net *net, uint32_t pop, double *out, uint32_t n) {
     if (pop >= net->n_pops || n > net->pops[pop].n) return -1;
-    for (uint32_t i = 0; i < n; ++i) out[i] = net->pops[pop].nb[i].potential;
+    const ko_pop *P = &net->pops[pop];
+    for (uint32_t i = 0; i < n; ++i) out[i] = P->lif ? (double)P->nl[i].potential : P->nb[i].potential;
+    return 0;
+}
+
+int ko_read_lif_neurons(ko_net *net, uint32_t pop, knp_lif_neuron *out, uint32_t n) {
+    if (pop >= net->n_pops || n > net->pops[pop].n || !net->pops[pop].lif) return -1;
+    const ko_lif *x = net->pops[pop].nl;
+    for (uint32_t i = 0; i < n; ++i) {
+        out[i].potential = (float)x[i].potential;
+        out[i].potential_reset_value = (float)x[i].potential_reset_value;
+        out[i].activation_threshold = (float)x[i].activation_threshold;
+        out[i].leak_coefficient = (float)x[i].leak_coefficient;
+        out[i].refract_counter = x[i].refract_counter;
+        out[i].refract_period = x[i].refract_period;
+    }
     return 0;
 }
 
@@ -838,7 +915,7 @@ int ko_read_last_spikes(ko_net *net, uint32_t pop, uint32_t *idx, uint32_t cap, 
 }
 
 int ko_read_blifat_neurons(ko_net *net, uint32_t pop, knp_blifat_neuron *out, uint32_t n) {
-    if (pop >= net->n_pops || n > net->pops[pop].n) return -1;
+    if (pop >= net->n_pops || n > net->pops[pop].n || net->pops[pop].lif) return -1;
     memcpy(out, net->pops[pop].nb, (size_t)n * sizeof(knp_blifat_neuron));
     return 0;
 }

@@ -39,6 +39,7 @@ def _load(policy):
         "ko_last_error": (C.c_char_p, [P]),
         "ko_add_blifat_population": (i32, [P, C.POINTER(abi.BLIFATNeuron), u32]),
         "ko_add_resource_blifat_population": (i32, [P, C.POINTER(abi.ResourceBLIFATNeuron), u32]),
+        "ko_add_lif_population": (i32, [P, C.POINTER(abi.LIFNeuron), u32]),
         "ko_add_delta_projection": (i32, [P, u32, u32, u32, pu32, pu32, pf32, pu32, pu8, u64]),
         "ko_add_resource_stdp_projection": (
             i32, [P, u32, u32, u32, pu32, pu32, pf32, pu32, pu8, C.POINTER(abi.ResourceRule), u64]),
@@ -56,6 +57,7 @@ def _load(policy):
         "ko_read_last_spikes": (i32, [P, u32, pu32, u32, C.POINTER(u32)]),
         "ko_read_blifat_neurons": (i32, [P, u32, C.POINTER(abi.BLIFATNeuron), u32]),
         "ko_read_resource_neurons": (i32, [P, u32, C.POINTER(abi.ResourceBLIFATNeuron), u32]),
+        "ko_read_lif_neurons": (i32, [P, u32, C.POINTER(abi.LIFNeuron), u32]),
         "ko_read_weights": (i32, [P, u32, pf64, u64]),
         "ko_read_resources": (i32, [P, u32, pf64, u64]),
         "ko_read_resource_rules": (i32, [P, u32, C.POINTER(abi.ResourceRule), u64]),
@@ -110,6 +112,12 @@ class Oracle:
     def add_resource_blifat_population(self, neurons):
         n = len(neurons)
         pid = self._chk(self.lib.ko_add_resource_blifat_population(self.net, neurons, n))
+        self.pop_sizes.append(n)
+        return pid
+
+    def add_lif_population(self, neurons):
+        n = len(neurons)
+        pid = self._chk(self.lib.ko_add_lif_population(self.net, neurons, n))
         self.pop_sizes.append(n)
         return pid
 
@@ -252,6 +260,12 @@ class Oracle:
         n = self.pop_sizes[pop]
         out = (abi.ResourceBLIFATNeuron * n)()
         self._chk(self.lib.ko_read_resource_neurons(self.net, pop, out, n))
+        return out
+
+    def lif_neurons(self, pop):
+        n = self.pop_sizes[pop]
+        out = (abi.LIFNeuron * n)()
+        self._chk(self.lib.ko_read_lif_neurons(self.net, pop, out, n))
         return out
 
     def weights(self, proj):

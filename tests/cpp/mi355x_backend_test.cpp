@@ -1,6 +1,7 @@
 // mi355x_backend_test.cpp - the reference's backend integration tests, run against the MI355X backend
 // through the knp::core API: SmallestNetwork, AdditiveSTDPBLIFATNetwork and ResourceSTDPBLIFATNetwork
 // (knp/tests/backend/single_thread_cpu_test.cpp:54-264, generators knp/tests/common/generators.h:48-100),
+// a LIF neuron trace (lif_impl.h:31-84),
 // the supported-type names (:362-381), BackendLoader (knp/tests/framework/backend_loader_test.cpp:29-70)
 // and the Backend::start run loop with a stop predicate.
 // Usage: mi355x_backend_test <path to libknp-mi355x-backend.so>. Prints one PASS/FAIL line per case.
@@ -165,9 +166,47 @@ int main(int argc, char **argv) {
         CHECK(std::get<knp::core::synapse_data>(loop_after[0]).weight_ != 1.0f);
     });
 
+    run_case("LIFNetwork", [] {
+        // one LIF neuron (leak 0.5, refractory period 2) fed 0.75 every step: it fires when the leaked
+        // potential plus the impact exceeds 1 (steps 2, 6, 10, ...), then ignores two steps of impacts
+        using LIFPopulation = knp::core::Population<knp::neuron_traits::LIFNeuron>;
+        Backend backend;
+        LIFPopulation population{[](size_t) {
+                                     knp::neuron_traits::neuron_parameters<knp::neuron_traits::LIFNeuron> n;
+                                     n.leak_coefficient_ = 0.5f;
+                                     n.refract_period_ = 2;
+                                     return n;
+                                 },
+                                 1};
+        DeltaProjection input{UID{false}, population.get_uid(),
+                              [](size_t) { return DeltaProjection::Synapse{{0.75, 1, knp::synapse_traits::OutputType::EXCITATORY}, 0, 0}; },
+                              1};
+        const UID input_uid = input.get_uid();
+        backend.load_populations({population});
+        backend.load_projections({input});
+        backend._init();
+        auto endpoint = backend.get_message_bus().create_endpoint();
+        const UID in_channel_uid, out_channel_uid;
+        backend.subscribe<SpikeMessage>(input_uid, {in_channel_uid});
+        endpoint.subscribe<SpikeMessage>(out_channel_uid, {population.get_uid()});
+        std::vector<knp::core::Step> results;
+        for (knp::core::Step step = 0; step < 16; ++step) {
+            endpoint.send_message(SpikeMessage{{in_channel_uid, 0}, {0}});
+            backend._step();
+            endpoint.receive_all_messages();
+            if (!endpoint.unload_messages<SpikeMessage>(out_channel_uid).empty()) results.push_back(step);
+        }
+        CHECK((results == std::vector<knp::core::Step>{2, 6, 10, 14}));
+        const auto data = backend.get_network_data();
+        const auto &after = std::get<LIFPopulation>(data.first[0]);
+        CHECK(after[0].refract_counter_ == 1);  // fired at step 14, one refractory step done
+        CHECK(after[0].potential_ == 0.0f);
+    });
+
     run_case("NeuronsAndSynapsesGetting", [] {
         Backend backend;
         CHECK(backend.get_supported_neurons()[0] == "BLIFATNeuron");
+        CHECK(backend.get_supported_neurons().back() == "LIFNeuron");
         CHECK(backend.get_supported_synapses()[0] == "DeltaSynapse");
         CHECK(backend.plasticity_supported());
     });

@@ -57,6 +57,12 @@ def test_defaults_match_reference_values():
     rr = abi.ResourceRule()
     lib.knp_resource_rule_defaults(C.byref(rr))
     assert (rr.w_min, rr.w_max, rr.d_u, rr.synaptic_resource) == (0.0, 1.0, 0.0, 0.0)
+    lf = abi.LIFNeuron()
+    lib.knp_lif_defaults(C.byref(lf))
+    assert bytes(lf) == bytes(abi.lif_defaults())
+    # lif.h:43-67: potential 0, reset 0, threshold 1, leak 1, counter 0, period 0
+    assert (lf.potential, lf.potential_reset_value, lf.activation_threshold, lf.leak_coefficient) == (0, 0, 1, 1)
+    assert (lf.refract_counter, lf.refract_period) == (0, 0)
 
 
 def test_bernoulli_hash_matches_oracle(oracle_built):
@@ -80,6 +86,8 @@ int main(void) {
   printf("knp_blifat_neuron %zu\n", sizeof(knp_blifat_neuron));
   printf("knp_resource_blifat_neuron %zu\n", sizeof(knp_resource_blifat_neuron));
   printf("knp_resource_rule %zu\n", sizeof(knp_resource_rule));
+  printf("knp_lif_neuron %zu\n", sizeof(knp_lif_neuron));
+  P(knp_lif_neuron, refract_counter) P(knp_lif_neuron, refract_period)
   P(knp_blifat_neuron, potential) P(knp_blifat_neuron, total_blocking_period)
   P(knp_blifat_neuron, random_number_generator_state)
   P(knp_resource_blifat_neuron, d_h) P(knp_resource_blifat_neuron, is_being_forced)
@@ -97,6 +105,9 @@ int main(void) {
     assert int(vals["knp_blifat_neuron"]) == C.sizeof(abi.BLIFATNeuron)
     assert int(vals["knp_resource_blifat_neuron"]) == C.sizeof(abi.ResourceBLIFATNeuron)
     assert int(vals["knp_resource_rule"]) == C.sizeof(abi.ResourceRule)
+    assert int(vals["knp_lif_neuron"]) == C.sizeof(abi.LIFNeuron)
+    assert int(vals["knp_lif_neuron.refract_counter"]) == abi.LIFNeuron.refract_counter.offset
+    assert int(vals["knp_lif_neuron.refract_period"]) == abi.LIFNeuron.refract_period.offset
     assert int(vals["knp_blifat_neuron.potential"]) == abi.BLIFATNeuron.potential.offset
     assert int(vals["knp_blifat_neuron.total_blocking_period"]) == abi.BLIFATNeuron.total_blocking_period.offset
     assert int(vals["knp_blifat_neuron.random_number_generator_state"]) == \

@@ -553,6 +553,79 @@ def test_errors_are_reported():
         h.step(1)
 
 
+def lif_network(o, seed):
+    """LIF population A (leak 1, reset -0.5, refractory period 2, some neurons loaded refractory) with a
+    default BLIFAT population B: recurrent A->A (delays up to 12), A->B, B->A, host pushes into A and a
+    Bernoulli source into B. Only plain Delta EXCITATORY / INHIBITORY_CURRENT input, so both are tiled;
+    weights on a 2^-8 grid keep every FP32 sum exact."""
+    rng = np.random.default_rng(seed)
+    na, nb = 1500, 500
+    cells = abi.neuron_array(abi.lif_defaults(potential_reset_value=-0.5, refract_period=2), na)
+    for i in range(0, na, 7):
+        cells[i].refract_counter = 1 + i % 3
+        cells[i].potential = 0.25
+    a = o.add_lif_population(cells)
+    b = o.add_blifat_population(abi.neuron_array(abi.blifat_defaults(), nb))
+    ident = np.arange(na, dtype=np.uint32)
+    push = o.add_delta_projection(abi.KNP_NO_POPULATION, na, a, ident, ident, np.full(na, 0.75, np.float32),
+                                  np.ones(na, np.uint32))
+    def proj(pre_pop, n_pre, post_pop, n_post, S, wlo, whi, dmax, inh):
+        o.add_delta_projection(pre_pop, n_pre, post_pop, rng.integers(0, n_pre, S).astype(np.uint32),
+                               rng.integers(0, n_post, S).astype(np.uint32),
+                               (rng.integers(wlo, whi, S) / 256.0).astype(np.float32),
+                               rng.integers(1, dmax + 1, S).astype(np.uint32),
+                               rng.choice([abi.EXCITATORY, abi.INHIBITORY_CURRENT], S, p=[1 - inh, inh]).astype(np.uint8))
+    proj(a, na, a, na, 60_000, 4, 64, 12, 0.25)
+    proj(a, na, b, nb, 20_000, 16, 128, 5, 0.1)
+    proj(b, nb, a, na, 2_000, 32, 160, 3, 0.5)
+    src = o.add_delta_projection(abi.KNP_NO_POPULATION, nb, b, np.arange(nb, dtype=np.uint32),
+                                 np.arange(nb, dtype=np.uint32), np.full(nb, 1.0, np.float32), np.ones(nb, np.uint32))
+    o.add_bernoulli_source(src, np.full(nb, workloads.bernoulli_threshold(0.03), np.uint32), seed=9)
+    return (a, b), push
+
+
+def test_lif_population_matches_oracle():
+    """LIFNeuron populations (lif_impl.h:31-84) on the persistent, stepwise and atomic paths against the
+    oracle: spikes, potentials and refractory counters identical; EXCITATORY / INHIBITORY_CURRENT only."""
+    say("case: LIF + BLIFAT network, persistent / stepwise / atomic, calls of 1..30 steps")
+    chunks = [1, 5, 30, 2, 17, 1, 24]
+    steps = sum(chunks)
+    runs = {}
+    modes = {"persistent": abi.SCATTER_AUTO, "stepwise": abi.SCATTER_STEPWISE, "atomic": abi.SCATTER_ATOMIC}
+    for name in ("persistent", "stepwise", "atomic", "oracle"):
+        o = Oracle("exact") if name == "oracle" else MI355XBackend(0)
+        if name != "oracle":
+            o.set_scatter_mode(modes[name])
+        pops, push = lif_network(o, 21)
+        if name in ("persistent", "stepwise"):
+            assert all(o.tiled(p) for p in pops)
+        for p in pops:
+            o.record_spikes(p, steps)
+        rng = np.random.default_rng(6)
+        mids = []
+        for c in chunks:
+            o.push_spikes(push, np.nonzero(rng.random(1500) < 0.08)[0].astype(np.uint32))
+            o.step(c)
+            lf = o.lif_neurons(pops[0])
+            mids.append((np.array([x.potential for x in lf]), np.array([x.refract_counter for x in lf])))
+        runs[name] = ([o.read_spike_record(p) for p in pops], mids, o.events)
+    so = runs["oracle"][0]
+    assert so[0].sum() > 200 and so[1].sum() > 50
+    for name in ("persistent", "stepwise", "atomic"):
+        for k in range(2):
+            assert np.array_equal(runs[name][0][k], so[k]), (name, k)
+        for j in range(len(chunks)):
+            assert np.array_equal(runs[name][1][j][0], runs["oracle"][1][j][0]), (name, j)
+            assert np.array_equal(runs[name][1][j][1], runs["oracle"][1][j][1]), (name, j)
+        assert runs[name][2] == runs["oracle"][2]
+    g = MI355XBackend(0)
+    lp = g.add_lif_population(abi.neuron_array(abi.lif_defaults(), 4))
+    with pytest.raises(BackendError, match="Unhandled synapse type"):
+        g.add_delta_projection(abi.KNP_NO_POPULATION, 4, lp, [0], [1], [0.5], [1], [abi.DOPAMINE])
+    with pytest.raises(BackendError, match="not a BLIFAT population"):
+        g.blifat_neurons(lp)
+
+
 def test_batched_instances_match_standalone():
     """Instances composed into one device network (config 3 layout, small batch) evolve exactly as the same
     networks run alone on the oracle: composition changes launch shapes, not results."""

@@ -11,6 +11,8 @@ Checks, in order:
   * the closed form of default (memoryless) BLIFAT networks against an independent numpy simulation;
   * hand-computed single-neuron traces for decay, refractory period, conductance, blocking, threshold
     increments and bursting (blifat_impl.h:31-168);
+  * hand-computed LIF traces (leak, strict threshold, reset, refractory counter: lif_impl.h:31-84) and a
+    random LIF network against an independent numpy simulation of the same rules;
   * the Bernoulli channel hash against an independent numpy implementation.
 """
 import numpy as np
@@ -354,3 +356,111 @@ def test_exact_and_f64_agree_without_plasticity(oracle_built):
         x.step(100)
     assert np.array_equal(a.read_spike_record(0), b.read_spike_record(0))
     assert np.array_equal(a.potentials(0), b.potentials(0))
+
+
+# ---- LIF (lif_impl.h:31-84) ------------------------------------------------------------------------
+def lif_trace(neuron, w, steps, every=1, otype=abi.EXCITATORY):
+    """One LIF neuron fed by a channel that fires every `every` steps with weight w, delay 1."""
+    o = Oracle("exact")
+    pop = o.add_lif_population(abi.neuron_array(neuron, 1))
+    inp = o.add_delta_projection(abi.KNP_NO_POPULATION, 1, pop, [0], [0], [w], [1], [otype])
+    spikes, pots, counters = [], [], []
+    for t in range(steps):
+        if t % every == 0:
+            o.push_spikes(inp, [0])
+        o.step()
+        if len(o.last_spikes(pop)):
+            spikes.append(t)
+        n = o.lif_neurons(pop)[0]
+        pots.append(n.potential)
+        counters.append(n.refract_counter)
+    return spikes, pots, counters
+
+
+def test_hand_lif_leak_and_refractory(oracle_built):
+    # leak 0.5, threshold 1, refractory period 2, an impact of 0.75 every step from step 1 on:
+    # t1: 0.75; t2: 0.375 + 0.75 = 1.125 > 1 -> spike, V = 0, counter 2; t3, t4: no leak, impacts
+    # ignored, counter 2 -> 1 -> 0; t5: 0.75; t6: spike again -> period 4
+    sp, pots, cnt = lif_trace(abi.lif_defaults(leak_coefficient=0.5, refract_period=2), 0.75, 16)
+    assert sp == [2, 6, 10, 14]
+    assert pots[:8] == [0.0, 0.75, 0.0, 0.0, 0.0, 0.75, 0.0, 0.0]
+    assert cnt[:8] == [0, 0, 2, 1, 0, 0, 2, 1]
+
+
+def test_hand_lif_strict_threshold_and_reset(oracle_built):
+    # no leak, 0.5 per step: V reaches exactly 1.0 at t2 without firing (V > threshold, not >=), fires at
+    # t3 and resets to -0.25; then -0.25 + 0.5 + 0.5 + 0.5 = 1.25 fires at t6
+    sp, pots, _ = lif_trace(abi.lif_defaults(potential_reset_value=-0.25), 0.5, 13)
+    assert sp == [3, 6, 9, 12]
+    assert pots[:7] == [0.0, 0.5, 1.0, -0.25, 0.25, 0.75, -0.25]
+
+
+def test_hand_lif_inhibitory_and_initial_counter(oracle_built):
+    # INHIBITORY_CURRENT subtracts; a neuron loaded with refract_counter 3 ignores the first impacts
+    sp, pots, cnt = lif_trace(abi.lif_defaults(refract_counter=3, potential=0.5), 0.25, 7,
+                              otype=abi.INHIBITORY_CURRENT)
+    assert sp == []
+    assert cnt[:4] == [2, 1, 0, 0]
+    assert pots == [0.5, 0.5, 0.5, 0.25, 0.0, -0.25, -0.5]
+
+
+def test_lif_rejects_other_synapse_types(oracle_built):
+    o = Oracle("exact")
+    pop = o.add_lif_population(abi.neuron_array(abi.lif_defaults(), 2))
+    for ty in (abi.INHIBITORY_CONDUCTANCE, abi.DOPAMINE, abi.BLOCKING):
+        with pytest.raises(RuntimeError, match="Unhandled synapse type"):
+            o.add_delta_projection(abi.KNP_NO_POPULATION, 1, pop, [0], [1], [0.5], [1], [ty])
+
+
+def lif_numpy(n, pre, post, w, d, otype, reset, thr, period, inputs, steps):
+    """Independent simulation of a recurrent LIF network (leak 1): per step, the sum of the weights of the
+    synapses whose presynaptic neuron spiked `delay` steps earlier (channel inputs: one step earlier), added
+    only to neurons whose refractory counter is 0."""
+    sign = np.where(otype == abi.INHIBITORY_CURRENT, -1.0, 1.0)
+    V = np.zeros(n)
+    cnt = np.zeros(n, dtype=np.int64)
+    hist = []
+    out = np.zeros((steps, n), dtype=bool)
+    for t in range(steps):
+        s = np.zeros(n)
+        if t >= 1:
+            np.add.at(s, inputs[t - 1], 1.0)
+        for k in range(len(pre)):
+            if t - d[k] >= 0 and hist[t - d[k]][pre[k]]:
+                s[post[k]] += sign[k] * w[k]
+        free = cnt == 0
+        V = np.where(free, V + s, V)
+        fire = free & (V > thr)
+        V = np.where(fire, reset, V)
+        cnt = np.where(free, np.where(fire, period, 0), cnt - 1)
+        hist.append(fire)
+        out[t] = fire
+    return out, V, cnt
+
+
+def test_lif_network_matches_numpy(oracle_built):
+    rng = np.random.default_rng(11)
+    n, S, steps = 40, 400, 60
+    pre = rng.integers(0, n, S)
+    post = rng.integers(0, n, S)
+    w = rng.integers(1, 64, S) / 64.0
+    d = rng.integers(1, 6, S)
+    otype = rng.choice([abi.EXCITATORY, abi.INHIBITORY_CURRENT], S, p=[0.7, 0.3]).astype(np.uint8)
+    inputs = [np.nonzero(rng.random(n) < 0.15)[0] for _ in range(steps)]
+    o = Oracle("exact")
+    pop = o.add_lif_population(abi.neuron_array(abi.lif_defaults(potential_reset_value=-0.5, refract_period=2), n))
+    ident = np.arange(n, dtype=np.uint32)
+    inp = o.add_delta_projection(abi.KNP_NO_POPULATION, n, pop, ident, ident, np.ones(n, np.float32),
+                                 np.ones(n, np.uint32))
+    o.add_delta_projection(pop, n, pop, pre, post, w, d, otype)
+    o.record_spikes(pop, steps)
+    for t in range(steps):
+        if len(inputs[t]):
+            o.push_spikes(inp, inputs[t].astype(np.uint32))
+        o.step()
+    ref, V, cnt = lif_numpy(n, pre, post, w, d, otype, -0.5, 1.0, 2, inputs, steps)
+    assert ref.sum() > 50
+    assert np.array_equal(o.read_spike_record(pop), ref)
+    lf = o.lif_neurons(pop)
+    assert np.array_equal([x.potential for x in lf], V)
+    assert np.array_equal([x.refract_counter for x in lf], cnt)
