@@ -131,9 +131,30 @@ typedef struct knp_lif_neuron {
     uint32_t refract_period;
 } knp_lif_neuron;
 
+/* neuron_parameters<AltAILIF>, knp/neuron-traits-library/include/knp/neuron-traits/altai_lif.h:128-354.
+ * Same fields, order and meaning (the five bool flags as bytes); knp_altai_lif_defaults() returns
+ * default_values<AltAILIF> (altai_lif.h:43-122). */
+typedef struct knp_altai_lif_neuron {
+    uint8_t is_diff;
+    uint8_t is_reset;
+    uint8_t leak_rev;
+    uint8_t saturate;
+    uint8_t do_not_save;
+    float potential;
+    float pre_impact_potential;
+    uint16_t activation_threshold;
+    uint16_t negative_activation_threshold;
+    int16_t potential_leak;
+    uint16_t potential_reset_value;
+    double dopamine_value;
+    double additional_threshold;
+    int64_t activity_time;
+} knp_altai_lif_neuron;
+
 /* Defaults of the reference trait structs. */
 void knp_blifat_defaults(knp_blifat_neuron *out);
 void knp_lif_defaults(knp_lif_neuron *out);
+void knp_altai_lif_defaults(knp_altai_lif_neuron *out);
 void knp_resource_blifat_defaults(knp_resource_blifat_neuron *out);
 void knp_resource_rule_defaults(knp_resource_rule *out);
 
@@ -180,6 +201,12 @@ int knp_mi355x_add_resource_blifat_population(
  * the reference throws "Unhandled synapse type." when another type reaches it (lif_impl.h:62-64); here the
  * projection carrying it is rejected, with the same message, when the network is built. */
 int knp_mi355x_add_lif_population(knp_net *net, const knp_lif_neuron *neurons, uint32_t n, uint32_t *out_pop);
+/* Population<AltAILIF> (step: altai_impl.h:31-150). Accepts EXCITATORY, INHIBITORY_CURRENT and DOPAMINE
+ * impacts. BLOCKING impacts (altai_impl.h:72-79: a fold over the step's impacts in delivery order whose
+ * result depends on that order) and INHIBITORY_CONDUCTANCE (the reference throws) are rejected when the
+ * projection is added. With is_diff_ set, activation_threshold_ + additional_threshold_ must be a float. */
+int knp_mi355x_add_altai_lif_population(
+    knp_net *net, const knp_altai_lif_neuron *neurons, uint32_t n, uint32_t *out_pop);
 
 /* Projection<DeltaSynapse>. Synapse i of the reference container is (pre[i], post[i], weight[i], delay[i],
  * output_type[i]); output_type may be NULL (all KNP_EXCITATORY). pre_pop is a population index or
@@ -266,6 +293,7 @@ int knp_mi355x_read_potentials(knp_net *net, uint32_t pop, float *out, uint32_t 
 int knp_mi355x_read_blifat_neurons(knp_net *net, uint32_t pop, knp_blifat_neuron *out, uint32_t n);
 int knp_mi355x_read_resource_neurons(knp_net *net, uint32_t pop, knp_resource_blifat_neuron *out, uint32_t n);
 int knp_mi355x_read_lif_neurons(knp_net *net, uint32_t pop, knp_lif_neuron *out, uint32_t n);
+int knp_mi355x_read_altai_lif_neurons(knp_net *net, uint32_t pop, knp_altai_lif_neuron *out, uint32_t n);
 int knp_mi355x_read_weights(knp_net *net, uint32_t proj, float *out, uint64_t n_syn);
 int knp_mi355x_read_resource_rules(knp_net *net, uint32_t proj, knp_resource_rule *out, uint64_t n_syn);
 /* Number of SynapticImpacts created so far (sum over steps of spikes x out-degree over all projections). */

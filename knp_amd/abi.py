@@ -6,6 +6,8 @@ The structs follow the reference trait structs field by field:
                                                                    knp/neuron-traits-library/include/knp/neuron-traits/stdp_synaptic_resource_rule.h:91-192
   knp_resource_rule           STDPSynapticResourceRule<DeltaSynapse>
                                                                    knp/synapse-traits-library/include/knp/synapse-traits/stdp_synaptic_resource_rule.h:43-100
+  knp_lif_neuron              neuron_parameters<LIFNeuron>         knp/neuron-traits-library/include/knp/neuron-traits/lif.h:72-117
+  knp_altai_lif_neuron        neuron_parameters<AltAILIF>          knp/neuron-traits-library/include/knp/neuron-traits/altai_lif.h:128-354
 """
 import ctypes as C
 
@@ -115,6 +117,26 @@ class LIFNeuron(C.Structure):
     ]
 
 
+class AltAILIFNeuron(C.Structure):
+    """neuron_parameters<AltAILIF> (altai_lif.h:128-354); the bool flags are bytes."""
+    _fields_ = [
+        ("is_diff", C.c_uint8),
+        ("is_reset", C.c_uint8),
+        ("leak_rev", C.c_uint8),
+        ("saturate", C.c_uint8),
+        ("do_not_save", C.c_uint8),
+        ("potential", C.c_float),
+        ("pre_impact_potential", C.c_float),
+        ("activation_threshold", C.c_uint16),
+        ("negative_activation_threshold", C.c_uint16),
+        ("potential_leak", C.c_int16),
+        ("potential_reset_value", C.c_uint16),
+        ("dopamine_value", C.c_double),
+        ("additional_threshold", C.c_double),
+        ("activity_time", C.c_int64),
+    ]
+
+
 class ResourceRule(C.Structure):
     _fields_ = [
         ("synaptic_resource", C.c_float),
@@ -146,6 +168,19 @@ def lif_defaults(**overrides):
     n = LIFNeuron()
     n.activation_threshold = 1.0
     n.leak_coefficient = 1.0
+    for k, v in overrides.items():
+        setattr(n, k, v)
+    return n
+
+
+def altai_lif_defaults(**overrides):
+    """default_values<AltAILIF> (altai_lif.h:43-122) as an AltAILIFNeuron struct."""
+    n = AltAILIFNeuron()
+    n.is_reset = 1
+    n.leak_rev = 1
+    n.saturate = 1
+    n.activation_threshold = 1
+    n.negative_activation_threshold = 30000
     for k, v in overrides.items():
         setattr(n, k, v)
     return n
@@ -193,7 +228,7 @@ def resource_rule_defaults(**overrides):
 
 
 def neuron_array(proto, n):
-    """A ctypes array of n copies of `proto` (a BLIFATNeuron, ResourceBLIFATNeuron or LIFNeuron)."""
+    """A ctypes array of n copies of `proto` (one of the neuron structs above)."""
     arr = (type(proto) * n)()
     if n:
         view = np.frombuffer(arr, dtype=np.uint8).reshape(n, C.sizeof(proto))

@@ -30,6 +30,11 @@ constexpr int kTileMaxList = 2048;  // virtual segments that may reach one tile
 constexpr uint32_t kTileShortPart = 16, kTileShortPartStepwise = 32;
 
 enum PopKind : uint32_t { POP_LEAN = 0, POP_FULL = 1, POP_RESOURCE = 2 };
+// Neuron model of a FULL population. LIF: V, n_ts = refract counter, act, V_reset, V_decay = leak, refr.
+// AltAI-LIF: V, Vpre, tbp = activity time, dop, act = activation + additional threshold, V_reset,
+// V_decay = potential leak, V_min = -negative threshold, alt_flags.
+enum PopModel : uint32_t { MODEL_BLIFAT = 0, MODEL_LIF = 1, MODEL_ALTAI = 2 };
+constexpr uint32_t kAltDiff = 1, kAltReset = 2, kAltLeakRev = 4, kAltSaturate = 8, kAltDoNotSave = 16;
 enum ProjKind : uint32_t { PROJ_DELTA = 0, PROJ_RESOURCE = 1, PROJ_ADDITIVE = 2 };
 
 // A parameter that is either one value for the whole population or one value per neuron.
@@ -76,6 +81,7 @@ struct PopDev {
     ParamF act, thr_decay, thr_inc, trace_decay, trace_inc, g_decay, V_decay, reflexive, E_inh, V_reset, V_min,
         add_thr;
     ParamU bperiod, refr;
+    ParamU alt_flags;      // AltAI-LIF: kAlt* bits of is_diff_, is_reset_, leak_rev_, saturate_, do_not_save_
     uint32_t lean_ts_max;  // LEAN: every neuron started with n_time_steps_since_last_firing = SIZE_MAX
     uint32_t seg_base;     // first spike segment (one per 256 neurons)
     const float *ff;       // dense input impacts [2 ff_T][n] by arrival step (null if none)
@@ -88,7 +94,7 @@ struct PopDev {
     uint32_t *rec;         // recorded bitmaps [rec_cap][words]
     uint64_t rec_t0, rec_cap;
     uint32_t tiled;        // impacts into this population are added by the tile-owner scatter
-    uint32_t lif;          // FULL: LIFNeuron step on V, n_ts (refract counter), act, V_reset, V_decay (leak), refr
+    uint32_t model;        // FULL: MODEL_BLIFAT, MODEL_LIF or MODEL_ALTAI (see population_update)
     ResDev res;
 };
 

@@ -40,11 +40,12 @@ int fail(int code, const std::string &msg) {
 
 struct HostPop {
     bool resource = false;
-    bool lif = false;  // LIFNeuron population (state in nl; nb stays empty)
+    uint32_t model = MODEL_BLIFAT;  // LIF: state in nl, AltAI-LIF: in na (nb stays empty for both)
     uint32_t n = 0;
     std::vector<knp_blifat_neuron> nb;           // initial state as loaded
     std::vector<knp_resource_blifat_neuron> nr;  // resource populations
     std::vector<knp_lif_neuron> nl;              // LIF populations
+    std::vector<knp_altai_lif_neuron> na;        // AltAI-LIF populations
     uint32_t kind = POP_FULL;
     uint32_t D = 1;
     bool has_cond = false, has_dop = false, has_block = false, has_forcing = false;
@@ -193,8 +194,17 @@ uint64_t ts_from_device(uint32_t v) {
     return v;
 }
 
+// activation_threshold_ + additional_threshold_ (a double in the reference) as the smallest float not below
+// it: for a float potential V, V >= T exactly when V >= that float (altai_impl.h:116).
+float altai_threshold(const knp_altai_lif_neuron &x) {
+    const double T = (double)x.activation_threshold + x.additional_threshold;
+    float f = (float)T;
+    if ((double)f < T) f = std::nextafter(f, std::numeric_limits<float>::infinity());
+    return f;
+}
+
 bool lean_eligible(const HostPop &P) {
-    if (P.resource || P.lif || P.has_cond || P.has_dop || P.has_block) return false;
+    if (P.resource || P.model != MODEL_BLIFAT || P.has_cond || P.has_dop || P.has_block) return false;
     bool any_max = false, any_other = false;
     for (const auto &n : P.nb) {
         if (n.n_time_steps_since_last_firing == std::numeric_limits<uint64_t>::max()) any_max = true;
@@ -516,11 +526,34 @@ int init_network(knp_net *net) {
         if (H.has_dop && (rc = dev_alloc(net, (void **)&P.acc_d, ring * 8))) return rc;
         if (H.has_forcing && (rc = dev_alloc(net, (void **)&P.acc_f, ring * 4))) return rc;
         if (H.has_block && (rc = dev_alloc(net, (void **)&P.acc_b, ring * 8))) return rc;
-        if (H.lif) {
+        P.model = H.model;
+        if (H.model == MODEL_ALTAI) {
+            // AltAILIF (altai_lif.h:128-354) on the FULL state arrays (see PopModel)
+            const auto &na = H.na;
+            std::vector<float> f(n);
+            std::vector<int64_t> at(n);
+            for (uint32_t i = 0; i < n; ++i) f[i] = na[i].potential;
+            if ((rc = upload(net, &P.V, f.data(), n))) return rc;
+            for (uint32_t i = 0; i < n; ++i) f[i] = na[i].pre_impact_potential;
+            if ((rc = upload(net, &P.Vpre, f.data(), n))) return rc;
+            for (uint32_t i = 0; i < n; ++i) f[i] = (float)na[i].dopamine_value;
+            if ((rc = upload(net, &P.dop, f.data(), n))) return rc;
+            for (uint32_t i = 0; i < n; ++i) at[i] = na[i].activity_time;
+            if ((rc = upload(net, &P.tbp, at.data(), n))) return rc;
+            if ((rc = make_param_f(net, n, [&](uint32_t i) { return altai_threshold(na[i]); }, &P.act))) return rc;
+            if ((rc = make_param_f(net, n, [&](uint32_t i) { return (float)na[i].potential_reset_value; }, &P.V_reset))) return rc;
+            if ((rc = make_param_f(net, n, [&](uint32_t i) { return (float)na[i].potential_leak; }, &P.V_decay))) return rc;
+            if ((rc = make_param_f(net, n, [&](uint32_t i) { return -(float)na[i].negative_activation_threshold; }, &P.V_min))) return rc;
+            auto flags = [&](uint32_t i) {
+                const auto &x = na[i];
+                return (x.is_diff ? kAltDiff : 0u) | (x.is_reset ? kAltReset : 0u) | (x.leak_rev ? kAltLeakRev : 0u) |
+                       (x.saturate ? kAltSaturate : 0u) | (x.do_not_save ? kAltDoNotSave : 0u);
+            };
+            if ((rc = make_param_u(net, n, flags, &P.alt_flags))) return rc;
+        } else if (H.model == MODEL_LIF) {
             // LIFNeuron (lif.h:72-117) on the FULL state arrays: V = potential_, n_ts = refract_counter_,
             // V_decay = leak_coefficient_, refr = refract_period_
             const auto &nl = H.nl;
-            P.lif = 1;
             std::vector<float> Vh(n);
             std::vector<uint32_t> rcnt(n);
             for (uint32_t i = 0; i < n; ++i) {
@@ -1444,8 +1477,14 @@ int add_projection_common(knp_net *net, uint32_t kind, uint32_t pre_pop, uint32_
             return fail(KNP_ERR_UNSUPPORTED, "synaptic delay must be in 1..255 (delay 0 is lost by the reference)");
         if (H.type[s] > KNP_BLOCKING) return fail(KNP_ERR_UNSUPPORTED, "Unhandled synapse type.");
         // LIF neurons take EXCITATORY and INHIBITORY_CURRENT impacts only (lif_impl.h:44-64)
-        if (net->pops[post_pop].lif && H.type[s] > KNP_INHIBITORY_CURRENT)
+        if (net->pops[post_pop].model == MODEL_LIF && H.type[s] > KNP_INHIBITORY_CURRENT)
             return fail(KNP_ERR_UNSUPPORTED, "Unhandled synapse type.");
+        // AltAI-LIF: conductance throws in the reference (altai_impl.h:80-82); BLOCKING is order-dependent
+        if (net->pops[post_pop].model == MODEL_ALTAI) {
+            if (H.type[s] == KNP_INHIBITORY_CONDUCTANCE) return fail(KNP_ERR_UNSUPPORTED, "Unhandled synapse type.");
+            if (H.type[s] == KNP_BLOCKING)
+                return fail(KNP_ERR_UNSUPPORTED, "BLOCKING synapses onto AltAI-LIF populations are not supported");
+        }
         if (!std::isfinite(H.w[s])) return fail(KNP_ERR_INVALID_ARGUMENT, "non-finite weight");
         H.max_delay = std::max(H.max_delay, H.delay[s]);
         if (H.type[s] == KNP_BLOCKING) H.has_blocking = true;
@@ -1473,6 +1512,15 @@ void knp_lif_defaults(knp_lif_neuron *o) {
     std::memset(o, 0, sizeof(*o));
     o->activation_threshold = 1.f;
     o->leak_coefficient = 1.f;
+}
+
+void knp_altai_lif_defaults(knp_altai_lif_neuron *o) {
+    std::memset(o, 0, sizeof(*o));
+    o->is_reset = 1;
+    o->leak_rev = 1;
+    o->saturate = 1;
+    o->activation_threshold = 1;
+    o->negative_activation_threshold = 30000;
 }
 
 void knp_resource_blifat_defaults(knp_resource_blifat_neuron *o) {
@@ -1576,9 +1624,30 @@ int knp_mi355x_add_lif_population(knp_net *net, const knp_lif_neuron *neurons, u
     if (n && !neurons) return fail(KNP_ERR_INVALID_ARGUMENT, "null neuron array");
     if (net->pops.size() >= (size_t)kMaxPops) return fail(KNP_ERR_UNSUPPORTED, "too many populations");
     HostPop P;
-    P.lif = true;
+    P.model = MODEL_LIF;
     P.n = n;
     P.nl.assign(neurons, neurons + n);
+    net->pops.push_back(std::move(P));
+    if (out_pop) *out_pop = (uint32_t)net->pops.size() - 1;
+    return KNP_OK;
+}
+
+int knp_mi355x_add_altai_lif_population(knp_net *net, const knp_altai_lif_neuron *neurons, uint32_t n,
+                                        uint32_t *out_pop) {
+    if (!net) return fail(KNP_ERR_INVALID_ARGUMENT, "null handle");
+    if (net->initialized) return fail(KNP_ERR_STATE, "populations must be loaded before the first step");
+    if (n && !neurons) return fail(KNP_ERR_INVALID_ARGUMENT, "null neuron array");
+    if (net->pops.size() >= (size_t)kMaxPops) return fail(KNP_ERR_UNSUPPORTED, "too many populations");
+    for (uint32_t i = 0; i < n; ++i) {
+        const auto &x = neurons[i];
+        if (x.is_diff && (double)altai_threshold(x) != (double)x.activation_threshold + x.additional_threshold)
+            return fail(KNP_ERR_UNSUPPORTED,
+                        "AltAI-LIF with is_diff_: activation_threshold_ + additional_threshold_ must be a float");
+    }
+    HostPop P;
+    P.model = MODEL_ALTAI;
+    P.n = n;
+    P.na.assign(neurons, neurons + n);
     net->pops.push_back(std::move(P));
     if (out_pop) *out_pop = (uint32_t)net->pops.size() - 1;
     return KNP_OK;
@@ -1948,7 +2017,7 @@ static int export_blifat(knp_net *net, uint32_t pop, std::vector<knp_blifat_neur
 int knp_mi355x_read_blifat_neurons(knp_net *net, uint32_t pop, knp_blifat_neuron *out, uint32_t n) {
     int rc = check_pop(net, pop);
     if (rc) return rc;
-    if (net->pops[pop].lif) return fail(KNP_ERR_INVALID_ARGUMENT, "not a BLIFAT population");
+    if (net->pops[pop].model != MODEL_BLIFAT) return fail(KNP_ERR_INVALID_ARGUMENT, "not a BLIFAT population");
     if (n > net->pops[pop].n || (n && !out)) return fail(KNP_ERR_INVALID_ARGUMENT, "bad output buffer");
     if ((rc = ensure_init(net))) return rc;
     std::vector<knp_blifat_neuron> v;
@@ -1965,7 +2034,7 @@ int knp_mi355x_read_blifat_neurons(knp_net *net, uint32_t pop, knp_blifat_neuron
 int knp_mi355x_read_lif_neurons(knp_net *net, uint32_t pop, knp_lif_neuron *out, uint32_t n) {
     int rc = check_pop(net, pop);
     if (rc) return rc;
-    if (!net->pops[pop].lif) return fail(KNP_ERR_INVALID_ARGUMENT, "not a LIF population");
+    if (net->pops[pop].model != MODEL_LIF) return fail(KNP_ERR_INVALID_ARGUMENT, "not a LIF population");
     if (n > net->pops[pop].n || (n && !out)) return fail(KNP_ERR_INVALID_ARGUMENT, "bad output buffer");
     if ((rc = ensure_init(net))) return rc;
     const PopDev &P = net->table.pops[pop];
@@ -1977,6 +2046,29 @@ int knp_mi355x_read_lif_neurons(knp_net *net, uint32_t pop, knp_lif_neuron *out,
         out[i] = net->pops[pop].nl[i];
         out[i].potential = V[i];
         out[i].refract_counter = rcnt[i];
+    }
+    return KNP_OK;
+}
+
+int knp_mi355x_read_altai_lif_neurons(knp_net *net, uint32_t pop, knp_altai_lif_neuron *out, uint32_t n) {
+    int rc = check_pop(net, pop);
+    if (rc) return rc;
+    if (net->pops[pop].model != MODEL_ALTAI) return fail(KNP_ERR_INVALID_ARGUMENT, "not an AltAI-LIF population");
+    if (n > net->pops[pop].n || (n && !out)) return fail(KNP_ERR_INVALID_ARGUMENT, "bad output buffer");
+    if ((rc = ensure_init(net))) return rc;
+    const PopDev &P = net->table.pops[pop];
+    std::vector<float> V(n), Vpre(n), dop(n);
+    std::vector<int64_t> at(n);
+    if ((rc = download(net, V.data(), P.V, n))) return rc;
+    if ((rc = download(net, Vpre.data(), P.Vpre, n))) return rc;
+    if ((rc = download(net, dop.data(), P.dop, n))) return rc;
+    if ((rc = download(net, at.data(), P.tbp, n))) return rc;
+    for (uint32_t i = 0; i < n; ++i) {
+        out[i] = net->pops[pop].na[i];
+        out[i].potential = V[i];
+        out[i].pre_impact_potential = Vpre[i];
+        out[i].dopamine_value = dop[i];
+        out[i].activity_time = at[i];
     }
     return KNP_OK;
 }

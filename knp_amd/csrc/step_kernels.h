@@ -219,7 +219,44 @@ __device__ __forceinline__ bool population_update(const NetDev *__restrict__ net
             if (V < vmin) V = vmin;
             if (KEEP) *V_keep = V;
             else gp(P.V)[i] = V;
-        } else if (P.lif) {
+        } else if (P.model == MODEL_ALTAI) {
+            // AltAI-LIF (altai_impl.h:31-150). pre: the potential is rounded (or reset when do_not_save_);
+            // impacts; post: signed leak, activity-time countdown (0 means unlimited activity), spike when
+            // the potential reaches the threshold while active, then the positive and negative resets.
+            const uint32_t fl = pu(P.alt_flags, i);
+            float V = (fl & kAltDoNotSave) ? pf(P.V_reset, i) : roundf(gp(P.V)[i]);
+            gp(P.Vpre)[i] = V;
+            V += fixed_to_float(a, P.inv_scale);
+            if (P.acc_d) {
+                const int64_t ad = gp(P.acc_d)[ai];
+                gp(P.acc_d)[ai] = 0;
+                if (ad) gp(P.dop)[i] += fixed_to_float(ad, P.inv_scale);
+            }
+            const float leak = pf(P.V_decay, i);
+            V += ((fl & kAltLeakRev) && V < 0.f) ? -leak : leak;
+            int64_t at = gp(P.tbp)[i];
+            if (at > 0) --at;
+            else if (at < 0) ++at;
+            if (at == 0) at = INT64_MAX;
+            bool was_reset = false;
+            const float thr = pf(P.act, i);
+            if (V >= thr) {
+                spike = at > 0;
+                if (fl & kAltDiff) V -= thr;
+                if (fl & kAltReset) {
+                    V = pf(P.V_reset, i);
+                    was_reset = true;
+                }
+            }
+            const float neg = pf(P.V_min, i);  // -negative_activation_threshold_
+            if (V <= neg && !was_reset) {
+                if (fl & kAltSaturate) V = neg;
+                else if (fl & kAltReset) V = -pf(P.V_reset, i);
+                else if (fl & kAltDiff) V -= neg;
+            }
+            gp(P.V)[i] = V;
+            gp(P.tbp)[i] = at;
+        } else if (P.model == MODEL_LIF) {
             // LIFNeuron (lif_impl.h:31-84): while the refractory counter is non-zero the potential neither
             // leaks nor takes impacts and the counter counts down; else leak, add the step's impacts and fire
             // strictly above the threshold.

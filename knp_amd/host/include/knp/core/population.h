@@ -10,6 +10,7 @@
 #include "../neuron-traits/blifat.h"
 #include "../neuron-traits/stdp_synaptic_resource_rule.h"
 #include "../neuron-traits/lif.h"
+#include "../neuron-traits/altai_lif.h"
 
 namespace knp::core {
 

@@ -14,6 +14,26 @@ namespace {
 using BLIFAT = neuron_traits::neuron_parameters<neuron_traits::BLIFATNeuron>;
 using ResBLIFAT = neuron_traits::neuron_parameters<neuron_traits::SynapticResourceSTDPBLIFATNeuron>;
 using LIF = neuron_traits::neuron_parameters<neuron_traits::LIFNeuron>;
+using AltAI = neuron_traits::neuron_parameters<neuron_traits::AltAILIF>;
+
+knp_altai_lif_neuron to_abi(const AltAI &n) {
+    knp_altai_lif_neuron o;
+    o.is_diff = n.is_diff_;
+    o.is_reset = n.is_reset_;
+    o.leak_rev = n.leak_rev_;
+    o.saturate = n.saturate_;
+    o.do_not_save = n.do_not_save_;
+    o.potential = n.potential_;
+    o.pre_impact_potential = n.pre_impact_potential_;
+    o.activation_threshold = n.activation_threshold_;
+    o.negative_activation_threshold = n.negative_activation_threshold_;
+    o.potential_leak = n.potential_leak_;
+    o.potential_reset_value = n.potential_reset_value_;
+    o.dopamine_value = n.dopamine_value_;
+    o.additional_threshold = n.additional_threshold_;
+    o.activity_time = n.activity_time_;
+    return o;
+}
 
 knp_lif_neuron to_abi(const LIF &n) {
     knp_lif_neuron o;
@@ -124,7 +144,7 @@ MI355XBackend::~MI355XBackend() {
 }
 
 std::vector<std::string> MI355XBackend::get_supported_neurons() const {
-    return {"BLIFATNeuron", "SynapticResourceSTDPBLIFATNeuron", "LIFNeuron"};
+    return {"BLIFATNeuron", "SynapticResourceSTDPBLIFATNeuron", "AltAILIF", "LIFNeuron"};
 }
 
 std::vector<std::string> MI355XBackend::get_supported_synapses() const {
@@ -201,6 +221,11 @@ void MI355XBackend::_init() {
             for (const auto &n : *pl) a.push_back(to_abi(n));
             check(knp_mi355x_add_lif_population(net_, a.data(), (uint32_t)a.size(), &idx), "load population");
             pop_uids_.push_back(pl->get_uid());
+        } else if (auto *pa = std::get_if<core::Population<neuron_traits::AltAILIF>>(&v)) {
+            std::vector<knp_altai_lif_neuron> a;
+            for (const auto &n : *pa) a.push_back(to_abi(n));
+            check(knp_mi355x_add_altai_lif_population(net_, a.data(), (uint32_t)a.size(), &idx), "load population");
+            pop_uids_.push_back(pa->get_uid());
         } else {
             const auto &p2 = std::get<core::Population<neuron_traits::SynapticResourceSTDPBLIFATNeuron>>(v);
             std::vector<knp_resource_blifat_neuron> a;
@@ -328,6 +353,15 @@ MI355XBackend::get_network_data() {
             for (size_t k = 0; k < a.size(); ++k) {
                 (*pl)[k].potential_ = a[k].potential;
                 (*pl)[k].refract_counter_ = a[k].refract_counter;
+            }
+        } else if (auto *pa = std::get_if<core::Population<neuron_traits::AltAILIF>>(&pops[i])) {
+            std::vector<knp_altai_lif_neuron> a(pa->size());
+            check(knp_mi355x_read_altai_lif_neurons(net_, i, a.data(), (uint32_t)a.size()), "read neurons");
+            for (size_t k = 0; k < a.size(); ++k) {
+                (*pa)[k].potential_ = a[k].potential;
+                (*pa)[k].pre_impact_potential_ = a[k].pre_impact_potential;
+                (*pa)[k].dopamine_value_ = a[k].dopamine_value;
+                (*pa)[k].activity_time_ = a[k].activity_time;
             }
         } else {
             auto &p2 = std::get<core::Population<neuron_traits::SynapticResourceSTDPBLIFATNeuron>>(pops[i]);

@@ -9,6 +9,7 @@
  *   - calculate_any_population / resource pop   knp/backends/cpu/cpu-library/include/knp/backends/cpu-library/populations_old.h:104-217
  *   - BLIFAT pre / impact / post                cpu-library/impl/populations/blifat/blifat_impl.h:31-168
  *   - LIF pre / impact / post                   cpu-library/impl/populations/lif/lif_impl.h:31-84
+ *   - AltAI-LIF pre / impact / post             cpu-library/impl/populations/altai/altai_impl.h:31-150
  *   - resource STDP training                    cpu-library/impl/populations/blifat/blifat_stdp.h:35-172
  *                                               cpu-library/impl/populations/training/stdp.h:40-186
  *   - delta projection + delay queue            cpu-library/projections.h:58-84, impl/projections/delta/delta_impl.h:54-114
@@ -26,7 +27,8 @@
  * synapses of one presynaptic (or postsynaptic) neuron are visited in ascending synapse index.
  * A projection onto a LIF population with synapses other than EXCITATORY / INHIBITORY_CURRENT is rejected
  * when it is added; the reference throws "Unhandled synapse type." once such an impact is delivered
- * (lif_impl.h:62-64).
+ * (lif_impl.h:62-64). Onto AltAI-LIF populations, INHIBITORY_CONDUCTANCE (the reference throws) and BLOCKING
+ * (order-dependent, altai_impl.h:72-79; not supported by the device) are rejected the same way.
  */
 #include <math.h>
 #include <stdint.h>
@@ -95,11 +97,25 @@ typedef struct {
     uint32_t refract_counter, refract_period;
 } ko_lif;
 
+/* neuron_parameters<AltAILIF> (altai_lif.h:128-354); reals as the reference keeps them */
+typedef struct {
+    int is_diff, is_reset, leak_rev, saturate, do_not_save;
+    wreal potential, pre_impact_potential;
+    uint16_t activation_threshold, negative_activation_threshold;
+    int16_t potential_leak;
+    uint16_t potential_reset_value;
+    double dopamine_value, additional_threshold;
+    int64_t activity_time;
+} ko_altai;
+
 typedef struct {
     int resource;
-    int lif;
+    int lif, altai;
     uint32_t n;
     ko_lif *nl;            /* LIF populations */
+    ko_altai *na;          /* AltAI-LIF populations */
+    ko_impact *injected;   /* impacts delivered straight to the population at the next step */
+    size_t n_injected;
     knp_blifat_neuron *nb; /* BLIFAT fields, double as in the reference */
     ko_res_ext *rx;        /* resource fields (resource populations only) */
     int record;
@@ -193,6 +209,8 @@ void ko_destroy(ko_net *net) {
     for (uint32_t i = 0; i < net->n_pops; ++i) {
         free(net->pops[i].nb);
         free(net->pops[i].nl);
+        free(net->pops[i].na);
+        free(net->pops[i].injected);
         free(net->pops[i].rx);
         free(net->pops[i].rec);
         free(net->pops[i].last_spikes);
@@ -254,6 +272,51 @@ int ko_add_lif_population(ko_net *net, const knp_lif_neuron *neurons, uint32_t n
     return id;
 }
 
+int ko_add_altai_lif_population(ko_net *net, const knp_altai_lif_neuron *neurons, uint32_t n) {
+    int id = add_pop(net, n, 0);
+    ko_pop *p = &net->pops[id];
+    p->altai = 1;
+    p->na = (ko_altai *)xcalloc(n, sizeof(ko_altai));
+    for (uint32_t i = 0; i < n; ++i) {
+        const knp_altai_lif_neuron *s = &neurons[i];
+        ko_altai *x = &p->na[i];
+        x->is_diff = s->is_diff != 0;
+        x->is_reset = s->is_reset != 0;
+        x->leak_rev = s->leak_rev != 0;
+        x->saturate = s->saturate != 0;
+        x->do_not_save = s->do_not_save != 0;
+        x->potential = s->potential;
+        x->pre_impact_potential = s->pre_impact_potential;
+        x->activation_threshold = s->activation_threshold;
+        x->negative_activation_threshold = s->negative_activation_threshold;
+        x->potential_leak = s->potential_leak;
+        x->potential_reset_value = s->potential_reset_value;
+        x->dopamine_value = s->dopamine_value;
+        x->additional_threshold = s->additional_threshold;
+        x->activity_time = s->activity_time;
+    }
+    return id;
+}
+
+/* Impacts (neuron, weight, type) delivered to `pop` at the next step, before the projections' messages: the
+ * reference's population tests send SynapticImpactMessages straight to a population
+ * (knp/tests/framework/altai_lif_test.cpp:60-100). Test hook; no projection is involved. */
+int ko_inject_impacts(ko_net *net, uint32_t pop, const uint32_t *neuron, const float *w, const uint8_t *type,
+                      uint32_t n) {
+    if (pop >= net->n_pops) return -1;
+    ko_pop *P = &net->pops[pop];
+    P->injected = (ko_impact *)realloc(P->injected, (P->n_injected + n) * sizeof(ko_impact));
+    for (uint32_t k = 0; k < n; ++k) {
+        if (neuron[k] >= P->n) return -1;
+        ko_impact *imp = &P->injected[P->n_injected++];
+        memset(imp, 0, sizeof(*imp));
+        imp->w = (wreal)w[k];
+        imp->type = type ? type[k] : KNP_EXCITATORY;
+        imp->post = neuron[k];
+    }
+    return 0;
+}
+
 int ko_add_resource_blifat_population(ko_net *net, const knp_resource_blifat_neuron *neurons, uint32_t n) {
     int id = add_pop(net, n, 1);
     ko_pop *p = &net->pops[id];
@@ -312,7 +375,10 @@ static int add_proj(ko_net *net, int kind, uint32_t pre_pop, uint32_t n_pre, uin
         p->w[s] = (wreal)weight[s];
         p->delay[s] = delay[s];
         p->type[s] = type ? type[s] : KNP_EXCITATORY;
-        if (p->type[s] > KNP_BLOCKING || (net->pops[post_pop].lif && p->type[s] > KNP_INHIBITORY_CURRENT)) {
+        const int t = p->type[s];
+        const ko_pop *Pp = &net->pops[post_pop];
+        if (t > KNP_BLOCKING || (Pp->lif && t > KNP_INHIBITORY_CURRENT) ||
+            (Pp->altai && (t == KNP_INHIBITORY_CONDUCTANCE || t == KNP_BLOCKING))) {
             snprintf(net->err, sizeof net->err, "Unhandled synapse type.");
             return -1;
         }
@@ -553,6 +619,43 @@ static int lif_post(ko_lif *n) {
         --n->refract_counter;
     }
     return 0;
+}
+
+/* ---- AltAI-LIF (altai_impl.h:31-150) ---------------------------------------------------------- */
+static void altai_pre(ko_altai *n) {
+    n->potential = n->do_not_save ? (wreal)n->potential_reset_value : (wreal)round(n->potential);
+    n->pre_impact_potential = n->potential;
+}
+
+static void altai_impact(ko_altai *n, const ko_impact *imp) {
+    if (imp->type == KNP_EXCITATORY) n->potential += imp->w;
+    else if (imp->type == KNP_INHIBITORY_CURRENT) n->potential -= imp->w;
+    else if (imp->type == KNP_DOPAMINE) n->dopamine_value += imp->w;
+}
+
+static int altai_post(ko_altai *n) {
+    const int sign = (n->leak_rev && n->potential < 0) ? -1 : 1;
+    n->potential += (wreal)(n->potential_leak * sign);
+    int spiked = 0;
+    if (n->activity_time > 0) --n->activity_time;
+    else if (n->activity_time < 0) ++n->activity_time;
+    if (n->activity_time == 0) n->activity_time = INT64_MAX;
+    int was_reset = 0;
+    const double thr = (double)n->activation_threshold + n->additional_threshold;
+    if ((double)n->potential >= thr) {
+        if (n->activity_time > 0) spiked = 1;
+        if (n->is_diff) n->potential = (wreal)((double)n->potential - thr);
+        if (n->is_reset) {
+            n->potential = (wreal)n->potential_reset_value;
+            was_reset = 1;
+        }
+    }
+    if (n->potential <= -(wreal)n->negative_activation_threshold && !was_reset) {
+        if (n->saturate) n->potential = -(wreal)n->negative_activation_threshold;
+        else if (n->is_reset) n->potential = (wreal)(-(int)n->potential_reset_value);
+        else if (n->is_diff) n->potential += (wreal)n->negative_activation_threshold;
+    }
+    return spiked;
 }
 
 /* ---- resource STDP training (blifat_stdp.h, training/stdp.h) --------------------------------- */
@@ -796,10 +899,19 @@ static void population_step(ko_net *net, uint32_t pop, uint64_t step) {
     ko_pop *P = &net->pops[pop];
     for (uint32_t i = 0; i < P->n; ++i) {
         if (P->lif) lif_pre(&P->nl[i]);
+        else if (P->altai) altai_pre(&P->na[i]);
         else if (P->resource) resource_pre(&P->nb[i], &P->rx[i]);
         else blifat_pre(&P->nb[i]);
     }
-    /* impact messages in delivery order: projections that sent last step, in load order */
+    /* impacts injected by a test, then impact messages in delivery order: projections that sent last step,
+     * in load order */
+    for (size_t k = 0; k < P->n_injected; ++k) {
+        const ko_impact *imp = &P->injected[k];
+        if (P->lif) lif_impact(&P->nl[imp->post], imp);
+        else if (P->altai) altai_impact(&P->na[imp->post], imp);
+        else blifat_impact(&P->nb[imp->post], imp);
+    }
+    P->n_injected = 0;
     for (uint32_t q = 0; q < net->n_projs; ++q) {
         ko_proj *p = &net->projs[q];
         if (p->post_pop != pop || !p->has_outbox) continue;
@@ -810,6 +922,10 @@ static void population_step(ko_net *net, uint32_t pop, uint64_t step) {
                 lif_impact(&P->nl[imp->post], imp);
                 continue;
             }
+            if (P->altai) {
+                altai_impact(&P->na[imp->post], imp);
+                continue;
+            }
             knp_blifat_neuron *n = &P->nb[imp->post];
             blifat_impact(n, imp);
             if (P->resource && imp->type == KNP_EXCITATORY) P->rx[imp->post].is_being_forced |= is_forcing;
@@ -817,7 +933,8 @@ static void population_step(ko_net *net, uint32_t pop, uint64_t step) {
     }
     P->n_last = 0;
     for (uint32_t i = 0; i < P->n; ++i)
-        if (P->lif ? lif_post(&P->nl[i]) : blifat_post(&P->nb[i])) P->last_spikes[P->n_last++] = i;
+        if (P->lif ? lif_post(&P->nl[i]) : P->altai ? altai_post(&P->na[i]) : blifat_post(&P->nb[i]))
+            P->last_spikes[P->n_last++] = i;
     if (P->resource) train_population(net, pop, P->last_spikes, P->n_last, step);
     if (P->record && P->rec_steps < P->rec_cap) {
         uint32_t words = (P->n + 31) / 32;
@@ -887,7 +1004,8 @@ uint64_t ko_get_event_count(ko_net *net) { return net->events; }
 int ko_read_potentials(ko_net *net, uint32_t pop, double *out, uint32_t n) {
     if (pop >= net->n_pops || n > net->pops[pop].n) return -1;
     const ko_pop *P = &net->pops[pop];
-    for (uint32_t i = 0; i < n; ++i) out[i] = P->lif ? (double)P->nl[i].potential : P->nb[i].potential;
+    for (uint32_t i = 0; i < n; ++i)
+        out[i] = P->lif ? (double)P->nl[i].potential : P->altai ? (double)P->na[i].potential : P->nb[i].potential;
     return 0;
 }
 
@@ -914,8 +1032,32 @@ int ko_read_last_spikes(ko_net *net, uint32_t pop, uint32_t *idx, uint32_t cap, 
     return 0;
 }
 
+int ko_read_altai_lif_neurons(ko_net *net, uint32_t pop, knp_altai_lif_neuron *out, uint32_t n) {
+    if (pop >= net->n_pops || n > net->pops[pop].n || !net->pops[pop].altai) return -1;
+    const ko_altai *x = net->pops[pop].na;
+    for (uint32_t i = 0; i < n; ++i) {
+        knp_altai_lif_neuron *o = &out[i];
+        memset(o, 0, sizeof(*o));
+        o->is_diff = (uint8_t)x[i].is_diff;
+        o->is_reset = (uint8_t)x[i].is_reset;
+        o->leak_rev = (uint8_t)x[i].leak_rev;
+        o->saturate = (uint8_t)x[i].saturate;
+        o->do_not_save = (uint8_t)x[i].do_not_save;
+        o->potential = (float)x[i].potential;
+        o->pre_impact_potential = (float)x[i].pre_impact_potential;
+        o->activation_threshold = x[i].activation_threshold;
+        o->negative_activation_threshold = x[i].negative_activation_threshold;
+        o->potential_leak = x[i].potential_leak;
+        o->potential_reset_value = x[i].potential_reset_value;
+        o->dopamine_value = x[i].dopamine_value;
+        o->additional_threshold = x[i].additional_threshold;
+        o->activity_time = x[i].activity_time;
+    }
+    return 0;
+}
+
 int ko_read_blifat_neurons(ko_net *net, uint32_t pop, knp_blifat_neuron *out, uint32_t n) {
-    if (pop >= net->n_pops || n > net->pops[pop].n || net->pops[pop].lif) return -1;
+    if (pop >= net->n_pops || n > net->pops[pop].n || net->pops[pop].lif || net->pops[pop].altai) return -1;
     memcpy(out, net->pops[pop].nb, (size_t)n * sizeof(knp_blifat_neuron));
     return 0;
 }

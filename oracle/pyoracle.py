@@ -40,6 +40,8 @@ def _load(policy):
         "ko_add_blifat_population": (i32, [P, C.POINTER(abi.BLIFATNeuron), u32]),
         "ko_add_resource_blifat_population": (i32, [P, C.POINTER(abi.ResourceBLIFATNeuron), u32]),
         "ko_add_lif_population": (i32, [P, C.POINTER(abi.LIFNeuron), u32]),
+        "ko_add_altai_lif_population": (i32, [P, C.POINTER(abi.AltAILIFNeuron), u32]),
+        "ko_inject_impacts": (i32, [P, u32, pu32, pf32, pu8, u32]),
         "ko_add_delta_projection": (i32, [P, u32, u32, u32, pu32, pu32, pf32, pu32, pu8, u64]),
         "ko_add_resource_stdp_projection": (
             i32, [P, u32, u32, u32, pu32, pu32, pf32, pu32, pu8, C.POINTER(abi.ResourceRule), u64]),
@@ -58,6 +60,7 @@ def _load(policy):
         "ko_read_blifat_neurons": (i32, [P, u32, C.POINTER(abi.BLIFATNeuron), u32]),
         "ko_read_resource_neurons": (i32, [P, u32, C.POINTER(abi.ResourceBLIFATNeuron), u32]),
         "ko_read_lif_neurons": (i32, [P, u32, C.POINTER(abi.LIFNeuron), u32]),
+        "ko_read_altai_lif_neurons": (i32, [P, u32, C.POINTER(abi.AltAILIFNeuron), u32]),
         "ko_read_weights": (i32, [P, u32, pf64, u64]),
         "ko_read_resources": (i32, [P, u32, pf64, u64]),
         "ko_read_resource_rules": (i32, [P, u32, C.POINTER(abi.ResourceRule), u64]),
@@ -120,6 +123,20 @@ class Oracle:
         pid = self._chk(self.lib.ko_add_lif_population(self.net, neurons, n))
         self.pop_sizes.append(n)
         return pid
+
+    def add_altai_lif_population(self, neurons):
+        n = len(neurons)
+        pid = self._chk(self.lib.ko_add_altai_lif_population(self.net, neurons, n))
+        self.pop_sizes.append(n)
+        return pid
+
+    def inject_impacts(self, pop, neurons, weights, otype=None):
+        """Impacts delivered straight to `pop` at the next step (a test hook with no projection)."""
+        neurons = np.ascontiguousarray(neurons, dtype=np.uint32)
+        weights = np.ascontiguousarray(weights, dtype=np.float32)
+        otype = None if otype is None else np.ascontiguousarray(otype, dtype=np.uint8)
+        self._chk(self.lib.ko_inject_impacts(self.net, pop, abi.u32p(neurons), abi.f32p(weights),
+                                             abi.u8p(otype), len(neurons)))
 
     @staticmethod
     def _syn(pre, post, weight, delay, otype):
@@ -266,6 +283,12 @@ class Oracle:
         n = self.pop_sizes[pop]
         out = (abi.LIFNeuron * n)()
         self._chk(self.lib.ko_read_lif_neurons(self.net, pop, out, n))
+        return out
+
+    def altai_lif_neurons(self, pop):
+        n = self.pop_sizes[pop]
+        out = (abi.AltAILIFNeuron * n)()
+        self._chk(self.lib.ko_read_altai_lif_neurons(self.net, pop, out, n))
         return out
 
     def weights(self, proj):

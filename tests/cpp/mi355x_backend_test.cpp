@@ -1,7 +1,7 @@
 // mi355x_backend_test.cpp - the reference's backend integration tests, run against the MI355X backend
 // through the knp::core API: SmallestNetwork, AdditiveSTDPBLIFATNetwork and ResourceSTDPBLIFATNetwork
 // (knp/tests/backend/single_thread_cpu_test.cpp:54-264, generators knp/tests/common/generators.h:48-100),
-// a LIF neuron trace (lif_impl.h:31-84),
+// a LIF neuron trace (lif_impl.h:31-84), an AltAI-LIF known answer (knp/tests/framework/altai_lif_test.cpp),
 // the supported-type names (:362-381), BackendLoader (knp/tests/framework/backend_loader_test.cpp:29-70)
 // and the Backend::start run loop with a stop predicate.
 // Usage: mi355x_backend_test <path to libknp-mi355x-backend.so>. Prints one PASS/FAIL line per case.
@@ -201,6 +201,36 @@ int main(int argc, char **argv) {
         const auto &after = std::get<LIFPopulation>(data.first[0]);
         CHECK(after[0].refract_counter_ == 1);  // fired at step 14, one refractory step done
         CHECK(after[0].potential_ == 0.0f);
+    });
+
+    run_case("AltAILIFNetwork", [] {
+        // SpikeGenerationTest of knp/tests/framework/altai_lif_test.cpp:162-175: leak 50, threshold 110,
+        // no input: the potential climbs 0, 50, 100 and fires on steps 2, 5, 8
+        using AltAIPopulation = knp::core::Population<knp::neuron_traits::AltAILIF>;
+        Backend backend;
+        AltAIPopulation population{[](size_t) {
+                                       knp::neuron_traits::neuron_parameters<knp::neuron_traits::AltAILIF> n;
+                                       n.potential_leak_ = 50;
+                                       n.activation_threshold_ = 110;
+                                       return n;
+                                   },
+                                   1};
+        backend.load_populations({population});
+        backend.load_projections({});
+        backend._init();
+        auto endpoint = backend.get_message_bus().create_endpoint();
+        const UID out_channel_uid;
+        endpoint.subscribe<SpikeMessage>(out_channel_uid, {population.get_uid()});
+        std::vector<knp::core::Step> results;
+        std::vector<float> potentials;
+        for (knp::core::Step step = 0; step < 10; ++step) {
+            potentials.push_back(std::get<AltAIPopulation>(backend.get_network_data().first[0])[0].potential_);
+            backend._step();
+            endpoint.receive_all_messages();
+            if (!endpoint.unload_messages<SpikeMessage>(out_channel_uid).empty()) results.push_back(step);
+        }
+        CHECK((results == std::vector<knp::core::Step>{2, 5, 8}));
+        CHECK((potentials == std::vector<float>{0, 50, 100, 0, 50, 100, 0, 50, 100, 0}));
     });
 
     run_case("NeuronsAndSynapsesGetting", [] {

@@ -63,6 +63,13 @@ def test_defaults_match_reference_values():
     # lif.h:43-67: potential 0, reset 0, threshold 1, leak 1, counter 0, period 0
     assert (lf.potential, lf.potential_reset_value, lf.activation_threshold, lf.leak_coefficient) == (0, 0, 1, 1)
     assert (lf.refract_counter, lf.refract_period) == (0, 0)
+    al = abi.AltAILIFNeuron()
+    lib.knp_altai_lif_defaults(C.byref(al))
+    assert bytes(al) == bytes(abi.altai_lif_defaults())
+    # altai_lif.h:43-122
+    assert (al.is_diff, al.is_reset, al.leak_rev, al.saturate, al.do_not_save) == (0, 1, 1, 1, 0)
+    assert (al.activation_threshold, al.negative_activation_threshold, al.potential_leak) == (1, 30000, 0)
+    assert al.activity_time == 0 and al.additional_threshold == 0.0
 
 
 def test_bernoulli_hash_matches_oracle(oracle_built):
@@ -88,6 +95,9 @@ int main(void) {
   printf("knp_resource_rule %zu\n", sizeof(knp_resource_rule));
   printf("knp_lif_neuron %zu\n", sizeof(knp_lif_neuron));
   P(knp_lif_neuron, refract_counter) P(knp_lif_neuron, refract_period)
+  printf("knp_altai_lif_neuron %zu\n", sizeof(knp_altai_lif_neuron));
+  P(knp_altai_lif_neuron, potential) P(knp_altai_lif_neuron, potential_reset_value)
+  P(knp_altai_lif_neuron, dopamine_value) P(knp_altai_lif_neuron, activity_time)
   P(knp_blifat_neuron, potential) P(knp_blifat_neuron, total_blocking_period)
   P(knp_blifat_neuron, random_number_generator_state)
   P(knp_resource_blifat_neuron, d_h) P(knp_resource_blifat_neuron, is_being_forced)
@@ -108,6 +118,9 @@ int main(void) {
     assert int(vals["knp_lif_neuron"]) == C.sizeof(abi.LIFNeuron)
     assert int(vals["knp_lif_neuron.refract_counter"]) == abi.LIFNeuron.refract_counter.offset
     assert int(vals["knp_lif_neuron.refract_period"]) == abi.LIFNeuron.refract_period.offset
+    assert int(vals["knp_altai_lif_neuron"]) == C.sizeof(abi.AltAILIFNeuron)
+    for f in ("potential", "potential_reset_value", "dopamine_value", "activity_time"):
+        assert int(vals["knp_altai_lif_neuron." + f]) == getattr(abi.AltAILIFNeuron, f).offset, f
     assert int(vals["knp_blifat_neuron.potential"]) == abi.BLIFATNeuron.potential.offset
     assert int(vals["knp_blifat_neuron.total_blocking_period"]) == abi.BLIFATNeuron.total_blocking_period.offset
     assert int(vals["knp_blifat_neuron.random_number_generator_state"]) == \

@@ -626,6 +626,92 @@ def test_lif_population_matches_oracle():
         g.blifat_neurons(lp)
 
 
+def altai_network(o, seed, dopamine):
+    """AltAI-LIF population A with per-neuron thresholds, leaks, negative thresholds, resets, activity times
+    and flag sets, next to a default BLIFAT population B: recurrent A->A (integer weights, delays up to 8),
+    A->B, B->A, host pushes into A, a Bernoulli source into B; dopamine=True adds a DOPAMINE projection A->A
+    (A is then summed by atomics). Weights on a 2^-2 grid (2^-8 into B) keep every FP32 sum exact."""
+    rng = np.random.default_rng(seed)
+    na, nb = 1200, 400
+    cells = abi.neuron_array(abi.altai_lif_defaults(), na)
+    for i in range(na):
+        c = cells[i]
+        c.activation_threshold = int(rng.integers(4, 16))
+        c.additional_threshold = float(rng.choice([0.0, 0.5]))
+        c.negative_activation_threshold = int(rng.integers(6, 30))
+        c.potential_leak = int(rng.integers(-2, 3))
+        c.potential_reset_value = int(rng.integers(0, 3))
+        c.is_diff, c.is_reset, c.leak_rev, c.saturate = (int(b) for b in rng.integers(0, 2, 4))
+        c.do_not_save = int(rng.random() < 0.05)
+        c.activity_time = int(rng.choice([0, 0, 0, 2, -2, 5]))
+        c.potential = float(rng.integers(-4, 8))
+    a = o.add_altai_lif_population(cells)
+    b = o.add_blifat_population(abi.neuron_array(abi.blifat_defaults(), nb))
+    ident = np.arange(na, dtype=np.uint32)
+    push = o.add_delta_projection(abi.KNP_NO_POPULATION, na, a, ident, ident, np.full(na, 6.0, np.float32),
+                                  np.ones(na, np.uint32))
+    def proj(pre_pop, n_pre, post_pop, n_post, S, w, dmax, types, p):
+        o.add_delta_projection(pre_pop, n_pre, post_pop, rng.integers(0, n_pre, S).astype(np.uint32),
+                               rng.integers(0, n_post, S).astype(np.uint32), w(S).astype(np.float32),
+                               rng.integers(1, dmax + 1, S).astype(np.uint32), rng.choice(types, S, p=p).astype(np.uint8))
+    EI = [abi.EXCITATORY, abi.INHIBITORY_CURRENT]
+    proj(a, na, a, na, 12_000, lambda S: rng.integers(1, 12, S) / 4.0, 8, EI, [0.55, 0.45])
+    proj(a, na, b, nb, 6_000, lambda S: rng.integers(8, 64, S) / 256.0, 4, EI, [0.7, 0.3])
+    proj(b, nb, a, na, 3_000, lambda S: rng.integers(1, 20, S) / 4.0, 3, EI, [0.5, 0.5])
+    if dopamine:
+        proj(a, na, a, na, 2_000, lambda S: rng.integers(1, 8, S) / 4.0, 2, [abi.DOPAMINE], [1.0])
+    src = o.add_delta_projection(abi.KNP_NO_POPULATION, nb, b, np.arange(nb, dtype=np.uint32),
+                                 np.arange(nb, dtype=np.uint32), np.full(nb, 1.0, np.float32), np.ones(nb, np.uint32))
+    o.add_bernoulli_source(src, np.full(nb, workloads.bernoulli_threshold(0.03), np.uint32), seed=13)
+    return (a, b), push
+
+
+@pytest.mark.parametrize("dopamine", [False, True])
+def test_altai_lif_population_matches_oracle(dopamine):
+    """AltAI-LIF populations (altai_impl.h:31-150) against the oracle: spikes, potentials, pre-impact
+    potentials, activity times and dopamine identical, on every summation path the network allows."""
+    say(f"case: AltAI-LIF + BLIFAT network dopamine={dopamine}, calls of 1..25 steps")
+    chunks = [1, 4, 25, 2, 13, 1, 20]
+    steps = sum(chunks)
+    names = ("auto", "atomic") if dopamine else ("auto", "stepwise", "atomic")
+    modes = {"auto": abi.SCATTER_AUTO, "stepwise": abi.SCATTER_STEPWISE, "atomic": abi.SCATTER_ATOMIC}
+    fields = ("potential", "pre_impact_potential", "activity_time", "dopamine_value")
+    runs = {}
+    for name in names + ("oracle",):
+        o = Oracle("exact") if name == "oracle" else MI355XBackend(0)
+        if name != "oracle":
+            o.set_scatter_mode(modes[name])
+        pops, push = altai_network(o, 31, dopamine)
+        if name in ("auto", "stepwise"):
+            assert o.tiled(pops[0]) == (not dopamine)
+        for p in pops:
+            o.record_spikes(p, steps)
+        rng = np.random.default_rng(8)
+        mids = []
+        for c in chunks:
+            o.push_spikes(push, np.nonzero(rng.random(1200) < 0.06)[0].astype(np.uint32))
+            o.step(c)
+            x = o.altai_lif_neurons(pops[0])
+            mids.append({f: np.array([getattr(v, f) for v in x]) for f in fields})
+        runs[name] = ([o.read_spike_record(p) for p in pops], mids, o.events)
+    so = runs["oracle"][0]
+    assert so[0].sum() > 200 and so[1].sum() > 50
+    for name in names:
+        for k in range(2):
+            assert np.array_equal(runs[name][0][k], so[k]), (name, k)
+        for j in range(len(chunks)):
+            for f in fields:
+                assert np.array_equal(runs[name][1][j][f], runs["oracle"][1][j][f]), (name, j, f)
+        assert runs[name][2] == runs["oracle"][2]
+    if not dopamine:
+        g = MI355XBackend(0)
+        ap = g.add_altai_lif_population(abi.neuron_array(abi.altai_lif_defaults(), 4))
+        with pytest.raises(BackendError, match="BLOCKING"):
+            g.add_delta_projection(abi.KNP_NO_POPULATION, 4, ap, [0], [1], [2.0], [1], [abi.BLOCKING])
+        with pytest.raises(BackendError, match="must be a float"):
+            g.add_altai_lif_population(abi.neuron_array(abi.altai_lif_defaults(is_diff=1, additional_threshold=0.1), 2))
+
+
 def test_batched_instances_match_standalone():
     """Instances composed into one device network (config 3 layout, small batch) evolve exactly as the same
     networks run alone on the oracle: composition changes launch shapes, not results."""

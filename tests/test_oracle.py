@@ -13,6 +13,9 @@ Checks, in order:
     increments and bursting (blifat_impl.h:31-168);
   * hand-computed LIF traces (leak, strict threshold, reset, refractory counter: lif_impl.h:31-84) and a
     random LIF network against an independent numpy simulation of the same rules;
+  * the reference's AltAI-LIF known answers (knp/tests/framework/altai_lif_test.cpp:103-196: reversed and
+    plain leak, saturation, spike generation, single-neuron impacts) plus hand traces of is_diff, negative
+    resets, rounding, do_not_save and activity time (altai_impl.h:31-150);
   * the Bernoulli channel hash against an independent numpy implementation.
 """
 import numpy as np
@@ -464,3 +467,112 @@ def test_lif_network_matches_numpy(oracle_built):
     lf = o.lif_neurons(pop)
     assert np.array_equal([x.potential for x in lf], V)
     assert np.array_equal([x.refract_counter for x in lf], cnt)
+
+
+# ---- AltAI-LIF (altai_impl.h:31-150) ---------------------------------------------------------------
+def run_altai_neuron(neuron, steps, impacts=(), num_neurons=1, neuron_index=0):
+    """As run_altai_neuron of knp/tests/framework/altai_lif_test.cpp:60-100: impacts[t] is delivered straight
+    to the neuron at step t; the potential is logged (as int16) before every step."""
+    o = Oracle("exact")
+    pop = o.add_altai_lif_population(abi.neuron_array(neuron, num_neurons))
+    pots, spikes = [], []
+    for t in range(steps):
+        if t < len(impacts):
+            o.inject_impacts(pop, [neuron_index], [impacts[t]])
+        pots.append(int(np.int16(o.altai_lif_neurons(pop)[neuron_index].potential)))
+        o.step()
+        if neuron_index in o.last_spikes(pop):
+            spikes.append(t)
+    return pots, spikes
+
+
+def test_altai_reference_leak_rev(oracle_built):
+    n = abi.altai_lif_defaults(activation_threshold=200, potential_leak=-25, potential=100, leak_rev=1)
+    pots, spikes = run_altai_neuron(n, 10, num_neurons=3, neuron_index=2)
+    assert pots == [100, 75, 50, 25, 0, -25, 0, -25, 0, -25]
+    assert spikes == []
+
+
+def test_altai_reference_leak_no_rev(oracle_built):
+    # negative_activation_threshold_ = -200 stored in uint16 (altai_lif_test.cpp:125)
+    n = abi.altai_lif_defaults(activation_threshold=200, negative_activation_threshold=(-200) & 0xFFFF,
+                               potential_leak=-25, potential=100, leak_rev=0)
+    pots, spikes = run_altai_neuron(n, 10)
+    assert pots == [100, 75, 50, 25, 0, -25, -50, -75, -100, -125]
+    assert spikes == []
+
+
+def test_altai_reference_saturate(oracle_built):
+    n = abi.altai_lif_defaults(potential=100, activation_threshold=200, negative_activation_threshold=60,
+                               potential_leak=-50, leak_rev=0, saturate=1)
+    pots, spikes = run_altai_neuron(n, 10)
+    assert pots == [100, 50, 0, -50, -60, -60, -60, -60, -60, -60]
+    assert spikes == []
+
+
+def test_altai_reference_spike_generation(oracle_built):
+    n = abi.altai_lif_defaults(potential_leak=50, activation_threshold=110)
+    pots, spikes = run_altai_neuron(n, 10)
+    assert pots == [0, 50, 100, 0, 50, 100, 0, 50, 100, 0]
+    assert spikes == [2, 5, 8]
+
+
+def test_altai_reference_single_neuron_impact(oracle_built):
+    n = abi.altai_lif_defaults(potential_leak=-30, activation_threshold=100)
+    pots, spikes = run_altai_neuron(n, 10, [130, 50, 50, 50, 50, 0, 100, 80, 80, 80], num_neurons=5,
+                                    neuron_index=3)
+    assert pots == [0, 0, 20, 40, 60, 80, 50, 0, 50, 0]
+    assert spikes == [0, 6, 8]
+
+
+def test_hand_altai_diff_rounding_and_negative_reset(oracle_built):
+    # is_diff without reset keeps the excess: 0 + 7.5 = 7.5 >= 5 -> spike, 2.5; next step rounds 2.5 to 3
+    # (half away from zero) and adds 7.5 -> 10.5 -> spike, 5.5; then 6 + 7.5 = 13.5 -> 8.5; without input,
+    # round(8.5) = 9 still fires and leaves 4, which does not
+    n = abi.altai_lif_defaults(is_diff=1, is_reset=0, activation_threshold=5)
+    pots, spikes = run_altai_neuron(n, 5, [7.5, 7.5, 7.5])
+    assert spikes == [0, 1, 2, 3]
+    assert pots == [0, 2, 5, 8, 4]  # logged as int16 before each step (2.5, 5.5, 8.5 truncate)
+    o = Oracle("exact")
+    pop = o.add_altai_lif_population(abi.neuron_array(n, 1))
+    for w in (7.5, 7.5, 7.5):
+        o.inject_impacts(pop, [0], [w])
+        o.step()
+    x = o.altai_lif_neurons(pop)[0]
+    assert x.potential == 8.5 and x.pre_impact_potential == 6.0
+    # negative side without saturation: reset gives -reset_value, diff adds the negative threshold
+    for flags, expect in (({"is_reset": 1}, -2.0), ({"is_reset": 0, "is_diff": 1}, -1.0)):
+        n = abi.altai_lif_defaults(saturate=0, negative_activation_threshold=10, potential_reset_value=2,
+                                   activation_threshold=100, **flags)
+        o = Oracle("exact")
+        pop = o.add_altai_lif_population(abi.neuron_array(n, 1))
+        o.inject_impacts(pop, [0], [11.0], [abi.INHIBITORY_CURRENT])
+        o.step()
+        assert o.altai_lif_neurons(pop)[0].potential == expect, flags
+
+
+def test_hand_altai_activity_time_and_do_not_save(oracle_built):
+    # activity_time_ = -2 blocks spikes for two steps (-2 -> -1 -> 0 -> unlimited): a neuron above threshold
+    # every step (leak 5, threshold 1, reset) fires from the second step on
+    n = abi.altai_lif_defaults(potential_leak=5, activity_time=-2)
+    pots, spikes = run_altai_neuron(n, 4)
+    assert spikes == [1, 2, 3]
+    # do_not_save_: the potential restarts from the reset value every step, so 0.75 per step never adds up
+    n = abi.altai_lif_defaults(do_not_save=1, potential_reset_value=0, activation_threshold=1)
+    pots, spikes = run_altai_neuron(n, 4, [0.75] * 4)
+    assert spikes == [] and pots == [0, 0, 0, 0]
+    o = Oracle("exact")
+    pop = o.add_altai_lif_population(abi.neuron_array(abi.altai_lif_defaults(), 2))
+    o.inject_impacts(pop, [1], [0.5], [abi.DOPAMINE])
+    o.step()
+    x = o.altai_lif_neurons(pop)
+    assert (x[0].dopamine_value, x[1].dopamine_value) == (0.0, 0.5)
+    assert x[0].activity_time == abi.INT64_MAX
+
+
+def test_altai_rejects_conductance_and_blocking(oracle_built):
+    o = Oracle("exact")
+    pop = o.add_altai_lif_population(abi.neuron_array(abi.altai_lif_defaults(), 2))
+    for ty in (abi.INHIBITORY_CONDUCTANCE, abi.BLOCKING):
+        with pytest.raises(RuntimeError, match="Unhandled synapse type"):
+            o.add_delta_projection(abi.KNP_NO_POPULATION, 1, pop, [0], [1], [0.5], [1], [ty])
