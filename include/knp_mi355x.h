@@ -151,11 +151,34 @@ typedef struct knp_altai_lif_neuron {
     int64_t activity_time;
 } knp_altai_lif_neuron;
 
+/* neuron_parameters<SynapticResourceSTDPNeuron<AltAILIF>> (SynapticResourceSTDPAltAILIFNeuron,
+ * stdp_synaptic_resource_rule.h:91-192 over altai_lif.h): the AltAI-LIF fields, then the same resource fields
+ * as knp_resource_blifat_neuron. */
+typedef struct knp_resource_altai_lif_neuron {
+    knp_altai_lif_neuron base;
+    float synapse_sum_threshold_coefficient;
+    uint32_t dopamine_plasticity_time;
+    float free_synaptic_resource;
+    float synaptic_resource_threshold;
+    uint32_t resource_drain_coefficient;
+    float stability;
+    float stability_change_parameter;
+    float stability_change_at_isi;
+    uint32_t isi_max;
+    float d_h;
+    int32_t isi_status; /* enum knp_isi_status */
+    uint64_t last_step;
+    uint64_t last_spike_step;
+    uint64_t first_isi_spike;
+    uint8_t is_being_forced;
+} knp_resource_altai_lif_neuron;
+
 /* Defaults of the reference trait structs. */
 void knp_blifat_defaults(knp_blifat_neuron *out);
 void knp_lif_defaults(knp_lif_neuron *out);
 void knp_altai_lif_defaults(knp_altai_lif_neuron *out);
 void knp_resource_blifat_defaults(knp_resource_blifat_neuron *out);
+void knp_resource_altai_lif_defaults(knp_resource_altai_lif_neuron *out);
 void knp_resource_rule_defaults(knp_resource_rule *out);
 
 /* ---- library / devices ------------------------------------------------------------------------ */
@@ -207,6 +230,11 @@ int knp_mi355x_add_lif_population(knp_net *net, const knp_lif_neuron *neurons, u
  * projection is added. With is_diff_ set, activation_threshold_ + additional_threshold_ must be a float. */
 int knp_mi355x_add_altai_lif_population(
     knp_net *net, const knp_altai_lif_neuron *neurons, uint32_t n, uint32_t *out_pop);
+/* Population<SynapticResourceSTDPAltAILIFNeuron> (SingleThreadedCPUBackend::calculate_population,
+ * backend.cpp:244-251; training altai_stdp.h:35-149, the same rules as the BLIFAT variant). The AltAI-LIF
+ * restrictions above apply; the threshold is activation_threshold_ + the trained additional threshold. */
+int knp_mi355x_add_resource_altai_lif_population(
+    knp_net *net, const knp_resource_altai_lif_neuron *neurons, uint32_t n, uint32_t *out_pop);
 
 /* Projection<DeltaSynapse>. Synapse i of the reference container is (pre[i], post[i], weight[i], delay[i],
  * output_type[i]); output_type may be NULL (all KNP_EXCITATORY). pre_pop is a population index or
@@ -294,6 +322,8 @@ int knp_mi355x_read_blifat_neurons(knp_net *net, uint32_t pop, knp_blifat_neuron
 int knp_mi355x_read_resource_neurons(knp_net *net, uint32_t pop, knp_resource_blifat_neuron *out, uint32_t n);
 int knp_mi355x_read_lif_neurons(knp_net *net, uint32_t pop, knp_lif_neuron *out, uint32_t n);
 int knp_mi355x_read_altai_lif_neurons(knp_net *net, uint32_t pop, knp_altai_lif_neuron *out, uint32_t n);
+int knp_mi355x_read_resource_altai_lif_neurons(knp_net *net, uint32_t pop, knp_resource_altai_lif_neuron *out,
+                                               uint32_t n);
 int knp_mi355x_read_weights(knp_net *net, uint32_t proj, float *out, uint64_t n_syn);
 int knp_mi355x_read_resource_rules(knp_net *net, uint32_t proj, knp_resource_rule *out, uint64_t n_syn);
 /* Number of SynapticImpacts created so far (sum over steps of spikes x out-degree over all projections). */

@@ -137,6 +137,11 @@ class AltAILIFNeuron(C.Structure):
     ]
 
 
+class ResourceAltAILIFNeuron(C.Structure):
+    """neuron_parameters<SynapticResourceSTDPAltAILIFNeuron>: AltAI-LIF fields, then the resource fields."""
+    _fields_ = [("base", AltAILIFNeuron)] + ResourceBLIFATNeuron._fields_[1:]
+
+
 class ResourceRule(C.Structure):
     _fields_ = [
         ("synaptic_resource", C.c_float),
@@ -206,6 +211,18 @@ def resource_blifat_defaults(base=None, **overrides):
     r.last_spike_step = 0
     r.first_isi_spike = 0
     r.is_being_forced = 0
+    for k, v in overrides.items():
+        setattr(r, k, v)
+    return r
+
+
+def resource_altai_lif_defaults(base=None, **overrides):
+    """neuron_parameters<SynapticResourceSTDPAltAILIFNeuron> defaults (resource fields as for BLIFAT)."""
+    r = ResourceAltAILIFNeuron()
+    r.base = base if base is not None else altai_lif_defaults()
+    ref = resource_blifat_defaults()
+    for f, _ in ResourceBLIFATNeuron._fields_[1:]:
+        setattr(r, f, getattr(ref, f))
     for k, v in overrides.items():
         setattr(r, k, v)
     return r

@@ -45,6 +45,7 @@ def _declare(lib):
         "knp_mi355x_add_resource_blifat_population": (i32, [P, C.POINTER(abi.ResourceBLIFATNeuron), u32, pu32]),
         "knp_mi355x_add_lif_population": (i32, [P, C.POINTER(abi.LIFNeuron), u32, pu32]),
         "knp_mi355x_add_altai_lif_population": (i32, [P, C.POINTER(abi.AltAILIFNeuron), u32, pu32]),
+        "knp_mi355x_add_resource_altai_lif_population": (i32, [P, C.POINTER(abi.ResourceAltAILIFNeuron), u32, pu32]),
         "knp_mi355x_add_delta_projection": (i32, [P, u32, u32, u32, pu32, pu32, pf32, pu32, pu8, u64, pu32]),
         "knp_mi355x_add_resource_stdp_projection": (
             i32, [P, u32, u32, u32, pu32, pu32, pf32, pu32, pu8, C.POINTER(abi.ResourceRule), u64, pu32]),
@@ -70,6 +71,7 @@ def _declare(lib):
         "knp_mi355x_read_resource_neurons": (i32, [P, u32, C.POINTER(abi.ResourceBLIFATNeuron), u32]),
         "knp_mi355x_read_lif_neurons": (i32, [P, u32, C.POINTER(abi.LIFNeuron), u32]),
         "knp_mi355x_read_altai_lif_neurons": (i32, [P, u32, C.POINTER(abi.AltAILIFNeuron), u32]),
+        "knp_mi355x_read_resource_altai_lif_neurons": (i32, [P, u32, C.POINTER(abi.ResourceAltAILIFNeuron), u32]),
         "knp_mi355x_read_weights": (i32, [P, u32, pf32, u64]),
         "knp_mi355x_read_resource_rules": (i32, [P, u32, C.POINTER(abi.ResourceRule), u64]),
         "knp_mi355x_get_event_count": (i32, [P, C.POINTER(u64)]),
@@ -81,6 +83,7 @@ def _declare(lib):
         "knp_resource_blifat_defaults": (None, [C.POINTER(abi.ResourceBLIFATNeuron)]),
         "knp_lif_defaults": (None, [C.POINTER(abi.LIFNeuron)]),
         "knp_altai_lif_defaults": (None, [C.POINTER(abi.AltAILIFNeuron)]),
+        "knp_resource_altai_lif_defaults": (None, [C.POINTER(abi.ResourceAltAILIFNeuron)]),
         "knp_resource_rule_defaults": (None, [C.POINTER(abi.ResourceRule)]),
         "knp_bernoulli_hash": (u32, [u64, u64, u32]),
     }
@@ -172,6 +175,13 @@ class MI355XBackend:
     def add_altai_lif_population(self, neurons):
         out = C.c_uint32()
         self._chk(self.lib.knp_mi355x_add_altai_lif_population(self.net, neurons, len(neurons), C.byref(out)))
+        self.pop_sizes.append(len(neurons))
+        return out.value
+
+    def add_resource_altai_lif_population(self, neurons):
+        out = C.c_uint32()
+        self._chk(self.lib.knp_mi355x_add_resource_altai_lif_population(self.net, neurons, len(neurons),
+                                                                        C.byref(out)))
         self.pop_sizes.append(len(neurons))
         return out.value
 
@@ -363,6 +373,12 @@ class MI355XBackend:
         n = self.pop_sizes[pop]
         out = (abi.AltAILIFNeuron * n)()
         self._chk(self.lib.knp_mi355x_read_altai_lif_neurons(self.net, pop, out, n))
+        return out
+
+    def resource_altai_lif_neurons(self, pop):
+        n = self.pop_sizes[pop]
+        out = (abi.ResourceAltAILIFNeuron * n)()
+        self._chk(self.lib.knp_mi355x_read_resource_altai_lif_neurons(self.net, pop, out, n))
         return out
 
     def weights(self, proj):

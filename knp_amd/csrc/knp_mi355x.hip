@@ -203,6 +203,26 @@ float altai_threshold(const knp_altai_lif_neuron &x) {
     return f;
 }
 
+// The resource fields (everything after `base`) of one resource neuron struct into another.
+template <class A, class B>
+void copy_resource_fields(const A &s, B &d) {
+    d.synapse_sum_threshold_coefficient = s.synapse_sum_threshold_coefficient;
+    d.dopamine_plasticity_time = s.dopamine_plasticity_time;
+    d.free_synaptic_resource = s.free_synaptic_resource;
+    d.synaptic_resource_threshold = s.synaptic_resource_threshold;
+    d.resource_drain_coefficient = s.resource_drain_coefficient;
+    d.stability = s.stability;
+    d.stability_change_parameter = s.stability_change_parameter;
+    d.stability_change_at_isi = s.stability_change_at_isi;
+    d.isi_max = s.isi_max;
+    d.d_h = s.d_h;
+    d.isi_status = s.isi_status;
+    d.last_step = s.last_step;
+    d.last_spike_step = s.last_spike_step;
+    d.first_isi_spike = s.first_isi_spike;
+    d.is_being_forced = s.is_being_forced;
+}
+
 bool lean_eligible(const HostPop &P) {
     if (P.resource || P.model != MODEL_BLIFAT || P.has_cond || P.has_dop || P.has_block) return false;
     bool any_max = false, any_other = false;
@@ -540,7 +560,9 @@ int init_network(knp_net *net) {
             if ((rc = upload(net, &P.dop, f.data(), n))) return rc;
             for (uint32_t i = 0; i < n; ++i) at[i] = na[i].activity_time;
             if ((rc = upload(net, &P.tbp, at.data(), n))) return rc;
-            if ((rc = make_param_f(net, n, [&](uint32_t i) { return altai_threshold(na[i]); }, &P.act))) return rc;
+            // resource neurons add their trained additional threshold on the device (res.add_thr)
+            auto act = [&](uint32_t i) { return H.resource ? (float)na[i].activation_threshold : altai_threshold(na[i]); };
+            if ((rc = make_param_f(net, n, act, &P.act))) return rc;
             if ((rc = make_param_f(net, n, [&](uint32_t i) { return (float)na[i].potential_reset_value; }, &P.V_reset))) return rc;
             if ((rc = make_param_f(net, n, [&](uint32_t i) { return (float)na[i].potential_leak; }, &P.V_decay))) return rc;
             if ((rc = make_param_f(net, n, [&](uint32_t i) { return -(float)na[i].negative_activation_threshold; }, &P.V_min))) return rc;
@@ -624,7 +646,8 @@ int init_network(knp_net *net) {
             std::vector<float> f(n);
             std::vector<uint64_t> u64(n);
             std::vector<uint8_t> u8(n);
-            for (uint32_t i = 0; i < n; ++i) f[i] = (float)H.nb[i].additional_threshold;
+            for (uint32_t i = 0; i < n; ++i)
+                f[i] = (float)(H.model == MODEL_ALTAI ? H.na[i].additional_threshold : H.nb[i].additional_threshold);
             if ((rc = upload(net, &R.add_thr, f.data(), n))) return rc;
             for (uint32_t i = 0; i < n; ++i) f[i] = nr[i].free_synaptic_resource;
             if ((rc = upload(net, &R.free_res, f.data(), n))) return rc;
@@ -1523,6 +1546,14 @@ void knp_altai_lif_defaults(knp_altai_lif_neuron *o) {
     o->negative_activation_threshold = 30000;
 }
 
+void knp_resource_altai_lif_defaults(knp_resource_altai_lif_neuron *o) {
+    knp_resource_blifat_neuron r;
+    knp_resource_blifat_defaults(&r);
+    std::memset(o, 0, sizeof(*o));
+    knp_altai_lif_defaults(&o->base);
+    copy_resource_fields(r, *o);
+}
+
 void knp_resource_blifat_defaults(knp_resource_blifat_neuron *o) {
     std::memset(o, 0, sizeof(*o));
     knp_blifat_defaults(&o->base);
@@ -1648,6 +1679,28 @@ int knp_mi355x_add_altai_lif_population(knp_net *net, const knp_altai_lif_neuron
     P.model = MODEL_ALTAI;
     P.n = n;
     P.na.assign(neurons, neurons + n);
+    net->pops.push_back(std::move(P));
+    if (out_pop) *out_pop = (uint32_t)net->pops.size() - 1;
+    return KNP_OK;
+}
+
+int knp_mi355x_add_resource_altai_lif_population(knp_net *net, const knp_resource_altai_lif_neuron *neurons,
+                                                 uint32_t n, uint32_t *out_pop) {
+    if (!net) return fail(KNP_ERR_INVALID_ARGUMENT, "null handle");
+    if (net->initialized) return fail(KNP_ERR_STATE, "populations must be loaded before the first step");
+    if (n && !neurons) return fail(KNP_ERR_INVALID_ARGUMENT, "null neuron array");
+    if (net->pops.size() >= (size_t)kMaxPops) return fail(KNP_ERR_UNSUPPORTED, "too many populations");
+    HostPop P;
+    P.model = MODEL_ALTAI;
+    P.resource = true;
+    P.n = n;
+    P.na.resize(n);
+    P.nr.resize(n);
+    for (uint32_t i = 0; i < n; ++i) {
+        P.na[i] = neurons[i].base;
+        knp_resource_blifat_defaults(&P.nr[i]);  // only the resource fields of nr are used
+        copy_resource_fields(neurons[i], P.nr[i]);
+    }
     net->pops.push_back(std::move(P));
     if (out_pop) *out_pop = (uint32_t)net->pops.size() - 1;
     return KNP_OK;
@@ -2063,29 +2116,30 @@ int knp_mi355x_read_altai_lif_neurons(knp_net *net, uint32_t pop, knp_altai_lif_
     if ((rc = download(net, Vpre.data(), P.Vpre, n))) return rc;
     if ((rc = download(net, dop.data(), P.dop, n))) return rc;
     if ((rc = download(net, at.data(), P.tbp, n))) return rc;
+    std::vector<float> add;
+    if (net->pops[pop].resource) {
+        add.resize(n);
+        if ((rc = download(net, add.data(), P.res.add_thr, n))) return rc;
+    }
     for (uint32_t i = 0; i < n; ++i) {
         out[i] = net->pops[pop].na[i];
         out[i].potential = V[i];
         out[i].pre_impact_potential = Vpre[i];
         out[i].dopamine_value = dop[i];
         out[i].activity_time = at[i];
+        if (!add.empty()) out[i].additional_threshold = add[i];
     }
     return KNP_OK;
 }
 
-int knp_mi355x_read_resource_neurons(knp_net *net, uint32_t pop, knp_resource_blifat_neuron *out, uint32_t n) {
-    int rc = check_pop(net, pop);
-    if (rc) return rc;
-    if (!net->pops[pop].resource) return fail(KNP_ERR_INVALID_ARGUMENT, "not a resource population");
-    if (n > net->pops[pop].n || (n && !out)) return fail(KNP_ERR_INVALID_ARGUMENT, "bad output buffer");
-    if ((rc = ensure_init(net))) return rc;
+// The resource fields of every neuron of a resource population (device state over the loaded values).
+static int export_resource_fields(knp_net *net, uint32_t pop, std::vector<knp_resource_blifat_neuron> &out) {
     const uint32_t N = net->pops[pop].n;
-    std::vector<knp_blifat_neuron> base(N);
-    if ((rc = knp_mi355x_read_blifat_neurons(net, pop, base.data(), N))) return rc;
     const ResDev &R = net->table.pops[pop].res;
     std::vector<float> fr(N), st(N);
     std::vector<uint8_t> is(N), fo(N);
     std::vector<uint64_t> ls(N), lss(N), fis(N);
+    int rc;
     if ((rc = download(net, fr.data(), R.free_res, N))) return rc;
     if ((rc = download(net, st.data(), R.stability, N))) return rc;
     if ((rc = download(net, is.data(), R.isi_status, N))) return rc;
@@ -2093,9 +2147,8 @@ int knp_mi355x_read_resource_neurons(knp_net *net, uint32_t pop, knp_resource_bl
     if ((rc = download(net, ls.data(), R.last_step, N))) return rc;
     if ((rc = download(net, lss.data(), R.last_spike_step, N))) return rc;
     if ((rc = download(net, fis.data(), R.first_isi_spike, N))) return rc;
-    for (uint32_t i = 0; i < n; ++i) {
-        out[i] = net->pops[pop].nr[i];
-        out[i].base = base[i];
+    out = net->pops[pop].nr;
+    for (uint32_t i = 0; i < N; ++i) {
         out[i].free_synaptic_resource = fr[i];
         out[i].stability = st[i];
         out[i].isi_status = is[i];
@@ -2103,6 +2156,45 @@ int knp_mi355x_read_resource_neurons(knp_net *net, uint32_t pop, knp_resource_bl
         out[i].last_step = ls[i];
         out[i].last_spike_step = lss[i];
         out[i].first_isi_spike = fis[i];
+    }
+    return KNP_OK;
+}
+
+int knp_mi355x_read_resource_altai_lif_neurons(knp_net *net, uint32_t pop, knp_resource_altai_lif_neuron *out,
+                                               uint32_t n) {
+    int rc = check_pop(net, pop);
+    if (rc) return rc;
+    if (!net->pops[pop].resource || net->pops[pop].model != MODEL_ALTAI)
+        return fail(KNP_ERR_INVALID_ARGUMENT, "not a resource AltAI-LIF population");
+    if (n > net->pops[pop].n || (n && !out)) return fail(KNP_ERR_INVALID_ARGUMENT, "bad output buffer");
+    if ((rc = ensure_init(net))) return rc;
+    const uint32_t N = net->pops[pop].n;
+    std::vector<knp_altai_lif_neuron> base(N);
+    std::vector<knp_resource_blifat_neuron> res;
+    if ((rc = knp_mi355x_read_altai_lif_neurons(net, pop, base.data(), N))) return rc;
+    if ((rc = export_resource_fields(net, pop, res))) return rc;
+    for (uint32_t i = 0; i < n; ++i) {
+        out[i].base = base[i];
+        copy_resource_fields(res[i], out[i]);
+    }
+    return KNP_OK;
+}
+
+int knp_mi355x_read_resource_neurons(knp_net *net, uint32_t pop, knp_resource_blifat_neuron *out, uint32_t n) {
+    int rc = check_pop(net, pop);
+    if (rc) return rc;
+    if (!net->pops[pop].resource || net->pops[pop].model != MODEL_BLIFAT)
+        return fail(KNP_ERR_INVALID_ARGUMENT, "not a resource BLIFAT population");
+    if (n > net->pops[pop].n || (n && !out)) return fail(KNP_ERR_INVALID_ARGUMENT, "bad output buffer");
+    if ((rc = ensure_init(net))) return rc;
+    const uint32_t N = net->pops[pop].n;
+    std::vector<knp_blifat_neuron> base(N);
+    std::vector<knp_resource_blifat_neuron> res;
+    if ((rc = knp_mi355x_read_blifat_neurons(net, pop, base.data(), N))) return rc;
+    if ((rc = export_resource_fields(net, pop, res))) return rc;
+    for (uint32_t i = 0; i < n; ++i) {
+        out[i] = res[i];
+        out[i].base = base[i];
     }
     return KNP_OK;
 }

@@ -183,6 +183,25 @@ __device__ __forceinline__ const T &entry_of(const T *table, uint32_t i, const T
     else return table[i];
 }
 
+// Synaptic-resource neurons after their update: store the step's dopamine and forcing for training and
+// return whether the neuron is a training candidate: it spiked, has dopamine, or is due for resource
+// renormalization (training/stdp.h:152-186; the test only depends on state that training leaves unchanged
+// for other neurons).
+__device__ __forceinline__ bool resource_candidate(const PopDev &P, uint32_t i, uint64_t t, bool spike,
+                                                   float dop_now, uint32_t forced) {
+    const ResDev &R = P.res;
+    gp(R.dop_now)[i] = dop_now;  // dopamine_value_ is reset every step for resource neurons
+    gp(R.forced_now)[i] = (uint8_t)(forced != 0);
+    bool renorm = false;
+    const float fr = gp(R.free_res)[i];
+    if (fabsf(fr) >= pf(R.res_thr, i)) {
+        const uint8_t st = gp(R.isi_status)[i];
+        const bool in_isi = (t - gp(R.last_step)[i] <= (uint64_t)pu(R.isi_max, i)) && st != 0;
+        renorm = !in_isi;
+    }
+    return spike || dop_now != 0.f || renorm;
+}
+
 // The BLIFAT update of neuron i of population P at step t (blifat_impl.h:31-168): returns the spike and sets
 // cand (synaptic-resource neurons: a training candidate). FUSED: a_in is the exact sum of the step's impacts
 // (else it is read from the ring and the slot cleared). KEEP (default BLIFAT only): the potential and the
@@ -223,14 +242,23 @@ __device__ __forceinline__ bool population_update(const NetDev *__restrict__ net
             // AltAI-LIF (altai_impl.h:31-150). pre: the potential is rounded (or reset when do_not_save_);
             // impacts; post: signed leak, activity-time countdown (0 means unlimited activity), spike when
             // the potential reaches the threshold while active, then the positive and negative resets.
+            // Synaptic-resource AltAI-LIF neurons (altai_impl.h:43-52, 85-94) also reset their dopamine
+            // every step and note forcing, and their threshold adds the trained additional threshold.
             const uint32_t fl = pu(P.alt_flags, i);
             float V = (fl & kAltDoNotSave) ? pf(P.V_reset, i) : roundf(gp(P.V)[i]);
             gp(P.Vpre)[i] = V;
             V += fixed_to_float(a, P.inv_scale);
+            float dop_now = 0.f;
             if (P.acc_d) {
                 const int64_t ad = gp(P.acc_d)[ai];
                 gp(P.acc_d)[ai] = 0;
-                if (ad) gp(P.dop)[i] += fixed_to_float(ad, P.inv_scale);
+                dop_now = fixed_to_float(ad, P.inv_scale);
+                if (KIND != POP_RESOURCE && ad) gp(P.dop)[i] += dop_now;
+            }
+            uint32_t forced = 0;
+            if (KIND == POP_RESOURCE && P.acc_f) {
+                forced = gp(P.acc_f)[ai];
+                gp(P.acc_f)[ai] = 0;
             }
             const float leak = pf(P.V_decay, i);
             V += ((fl & kAltLeakRev) && V < 0.f) ? -leak : leak;
@@ -239,10 +267,13 @@ __device__ __forceinline__ bool population_update(const NetDev *__restrict__ net
             else if (at < 0) ++at;
             if (at == 0) at = INT64_MAX;
             bool was_reset = false;
-            const float thr = pf(P.act, i);
-            if (V >= thr) {
+            // plain: the smallest float not below activation + additional threshold (exact for V >= T);
+            // resource: activation_threshold_ + the trained threshold, compared in double as the reference
+            const double thr = (KIND == POP_RESOURCE) ? (double)pf(P.act, i) + (double)P.res.add_thr[i]
+                                                      : (double)pf(P.act, i);
+            if ((double)V >= thr) {
                 spike = at > 0;
-                if (fl & kAltDiff) V -= thr;
+                if (fl & kAltDiff) V = (float)((double)V - thr);
                 if (fl & kAltReset) {
                     V = pf(P.V_reset, i);
                     was_reset = true;
@@ -256,6 +287,7 @@ __device__ __forceinline__ bool population_update(const NetDev *__restrict__ net
             }
             gp(P.V)[i] = V;
             gp(P.tbp)[i] = at;
+            if (KIND == POP_RESOURCE) cand = resource_candidate(P, i, t, spike, dop_now, forced);
         } else if (P.model == MODEL_LIF) {
             // LIFNeuron (lif_impl.h:31-84): while the refractory counter is non-zero the potential neither
             // leaks nor takes impacts and the counter counts down; else leak, add the step's impacts and fire
@@ -354,19 +386,7 @@ __device__ __forceinline__ bool population_update(const NetDev *__restrict__ net
             gp(P.bphase)[i] = bphase;
             if (P.track_tbp) gp(P.tbp)[i] = tbp;
             if (KIND == POP_RESOURCE) {
-                const ResDev &R = P.res;
-                gp(R.dop_now)[i] = dop_now;  // dopamine_value_ is reset every step for resource neurons
-                gp(R.forced_now)[i] = (uint8_t)(forced != 0);
-                // training candidate: spiked, has dopamine, or due for resource renormalization
-                // (the renormalization test only depends on state training leaves unchanged for others)
-                bool renorm = false;
-                const float fr = gp(R.free_res)[i];
-                if (fabsf(fr) >= pf(R.res_thr, i)) {
-                    const uint8_t st = gp(R.isi_status)[i];
-                    const bool in_isi = (t - gp(R.last_step)[i] <= (uint64_t)pu(R.isi_max, i)) && st != 0;
-                    renorm = !in_isi;
-                }
-                cand = spike || dop_now != 0.f || renorm;
+                cand = resource_candidate(P, i, t, spike, dop_now, forced);
             } else if (P.acc_d) {
                 gp(P.dop)[i] += dop_now;  // plain BLIFAT keeps summing dopamine (never reset for it)
             }

@@ -24,6 +24,7 @@ namespace knp::core {
 using AllPopulationsVariant = std::variant<Population<neuron_traits::BLIFATNeuron>,
                                            Population<neuron_traits::SynapticResourceSTDPBLIFATNeuron>,
                                            Population<neuron_traits::AltAILIF>,
+                                           Population<neuron_traits::SynapticResourceSTDPAltAILIFNeuron>,
                                            Population<neuron_traits::LIFNeuron>>;
 using AllProjectionsVariant = std::variant<Projection<synapse_traits::DeltaSynapse>,
                                            Projection<synapse_traits::AdditiveSTDPDeltaSynapse>,

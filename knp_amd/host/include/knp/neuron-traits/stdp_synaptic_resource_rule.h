@@ -1,10 +1,12 @@
 // knp/neuron-traits/stdp_synaptic_resource_rule.h - synaptic-resource STDP neuron.
 // Interface of knp/neuron-traits-library/include/knp/neuron-traits/stdp_synaptic_resource_rule.h:46-192
-// (field names, types, defaults) and stdp_type_traits.h (SynapticResourceSTDPBLIFATNeuron).
+// (field names, types, defaults) and stdp_type_traits.h (SynapticResourceSTDPBLIFATNeuron,
+// SynapticResourceSTDPAltAILIFNeuron).
 #pragma once
 #include <cstdint>
 #include <limits>
 
+#include "altai_lif.h"
 #include "blifat.h"
 
 namespace knp::neuron_traits {
@@ -36,5 +38,6 @@ struct neuron_parameters<SynapticResourceSTDPNeuron<NeuronType>> : public neuron
 };
 
 using SynapticResourceSTDPBLIFATNeuron = SynapticResourceSTDPNeuron<BLIFATNeuron>;
+using SynapticResourceSTDPAltAILIFNeuron = SynapticResourceSTDPNeuron<AltAILIF>;
 
 }  // namespace knp::neuron_traits

@@ -89,9 +89,9 @@ void from_abi(const knp_blifat_neuron &o, BLIFAT &n) {
     n.dopamine_value_ = o.dopamine_value;
 }
 
-knp_resource_blifat_neuron to_abi(const ResBLIFAT &n) {
-    knp_resource_blifat_neuron o;
-    o.base = to_abi(static_cast<const BLIFAT &>(n));
+// The resource fields of a synaptic-resource neuron (any base model) into its C ABI struct.
+template <class Res, class Out>
+void resource_to_abi(const Res &n, Out &o) {
     o.synapse_sum_threshold_coefficient = n.synapse_sum_threshold_coefficient_;
     o.dopamine_plasticity_time = n.dopamine_plasticity_time_;
     o.free_synaptic_resource = n.free_synaptic_resource_;
@@ -107,6 +107,24 @@ knp_resource_blifat_neuron to_abi(const ResBLIFAT &n) {
     o.last_spike_step = n.last_spike_step_;
     o.first_isi_spike = n.first_isi_spike_;
     o.is_being_forced = n.is_being_forced_ ? 1 : 0;
+}
+
+// The resource state that training changes, from the C ABI struct.
+template <class In, class Res>
+void resource_from_abi(const In &a, Res &n) {
+    n.free_synaptic_resource_ = a.free_synaptic_resource;
+    n.stability_ = a.stability;
+    n.isi_status_ = static_cast<neuron_traits::ISIPeriodType>(a.isi_status);
+    n.last_step_ = a.last_step;
+    n.last_spike_step_ = a.last_spike_step;
+    n.first_isi_spike_ = a.first_isi_spike;
+    n.is_being_forced_ = a.is_being_forced != 0;
+}
+
+knp_resource_blifat_neuron to_abi(const ResBLIFAT &n) {
+    knp_resource_blifat_neuron o;
+    o.base = to_abi(static_cast<const BLIFAT &>(n));
+    resource_to_abi(n, o);
     return o;
 }
 
@@ -144,7 +162,8 @@ MI355XBackend::~MI355XBackend() {
 }
 
 std::vector<std::string> MI355XBackend::get_supported_neurons() const {
-    return {"BLIFATNeuron", "SynapticResourceSTDPBLIFATNeuron", "AltAILIF", "LIFNeuron"};
+    return {"BLIFATNeuron", "SynapticResourceSTDPBLIFATNeuron", "AltAILIF", "SynapticResourceSTDPAltAILIFNeuron",
+            "LIFNeuron"};
 }
 
 std::vector<std::string> MI355XBackend::get_supported_synapses() const {
@@ -226,6 +245,17 @@ void MI355XBackend::_init() {
             for (const auto &n : *pa) a.push_back(to_abi(n));
             check(knp_mi355x_add_altai_lif_population(net_, a.data(), (uint32_t)a.size(), &idx), "load population");
             pop_uids_.push_back(pa->get_uid());
+        } else if (auto *pr = std::get_if<core::Population<neuron_traits::SynapticResourceSTDPAltAILIFNeuron>>(&v)) {
+            std::vector<knp_resource_altai_lif_neuron> a;
+            for (const auto &n : *pr) {
+                knp_resource_altai_lif_neuron o;
+                o.base = to_abi(static_cast<const AltAI &>(n));
+                resource_to_abi(n, o);
+                a.push_back(o);
+            }
+            check(knp_mi355x_add_resource_altai_lif_population(net_, a.data(), (uint32_t)a.size(), &idx),
+                  "load population");
+            pop_uids_.push_back(pr->get_uid());
         } else {
             const auto &p2 = std::get<core::Population<neuron_traits::SynapticResourceSTDPBLIFATNeuron>>(v);
             std::vector<knp_resource_blifat_neuron> a;
@@ -363,19 +393,25 @@ MI355XBackend::get_network_data() {
                 (*pa)[k].dopamine_value_ = a[k].dopamine_value;
                 (*pa)[k].activity_time_ = a[k].activity_time;
             }
+        } else if (auto *pr = std::get_if<core::Population<neuron_traits::SynapticResourceSTDPAltAILIFNeuron>>(&pops[i])) {
+            std::vector<knp_resource_altai_lif_neuron> a(pr->size());
+            check(knp_mi355x_read_resource_altai_lif_neurons(net_, i, a.data(), (uint32_t)a.size()), "read neurons");
+            for (size_t k = 0; k < a.size(); ++k) {
+                auto &n = (*pr)[k];
+                n.potential_ = a[k].base.potential;
+                n.pre_impact_potential_ = a[k].base.pre_impact_potential;
+                n.dopamine_value_ = a[k].base.dopamine_value;
+                n.additional_threshold_ = a[k].base.additional_threshold;
+                n.activity_time_ = a[k].base.activity_time;
+                resource_from_abi(a[k], n);
+            }
         } else {
             auto &p2 = std::get<core::Population<neuron_traits::SynapticResourceSTDPBLIFATNeuron>>(pops[i]);
             std::vector<knp_resource_blifat_neuron> a(p2.size());
             check(knp_mi355x_read_resource_neurons(net_, i, a.data(), (uint32_t)a.size()), "read neurons");
             for (size_t k = 0; k < a.size(); ++k) {
                 from_abi(a[k].base, p2[k]);
-                p2[k].free_synaptic_resource_ = a[k].free_synaptic_resource;
-                p2[k].stability_ = a[k].stability;
-                p2[k].isi_status_ = static_cast<neuron_traits::ISIPeriodType>(a[k].isi_status);
-                p2[k].last_step_ = a[k].last_step;
-                p2[k].last_spike_step_ = a[k].last_spike_step;
-                p2[k].first_isi_spike_ = a[k].first_isi_spike;
-                p2[k].is_being_forced_ = a[k].is_being_forced != 0;
+                resource_from_abi(a[k], p2[k]);
             }
         }
     }

@@ -317,28 +317,45 @@ int ko_inject_impacts(ko_net *net, uint32_t pop, const uint32_t *neuron, const f
     return 0;
 }
 
+/* the resource fields of a knp_resource_*_neuron (everything after `base`) into a ko_res_ext */
+#define KO_SET_RES_EXT(x, s)                                                     \
+    do {                                                                         \
+        (x)->synapse_sum_threshold_coefficient = (s)->synapse_sum_threshold_coefficient; \
+        (x)->dopamine_plasticity_time = (s)->dopamine_plasticity_time;           \
+        (x)->free_synaptic_resource = (s)->free_synaptic_resource;               \
+        (x)->synaptic_resource_threshold = (s)->synaptic_resource_threshold;     \
+        (x)->resource_drain_coefficient = (s)->resource_drain_coefficient;       \
+        (x)->stability = (s)->stability;                                         \
+        (x)->stability_change_parameter = (s)->stability_change_parameter;       \
+        (x)->stability_change_at_isi = (s)->stability_change_at_isi;             \
+        (x)->isi_max = (s)->isi_max;                                             \
+        (x)->d_h = (s)->d_h;                                                     \
+        (x)->isi_status = (s)->isi_status;                                       \
+        (x)->last_step = (s)->last_step;                                         \
+        (x)->last_spike_step = (s)->last_spike_step;                             \
+        (x)->first_isi_spike = (s)->first_isi_spike;                             \
+        (x)->is_being_forced = (s)->is_being_forced;                             \
+    } while (0)
+
+int ko_add_resource_altai_lif_population(ko_net *net, const knp_resource_altai_lif_neuron *neurons, uint32_t n) {
+    knp_altai_lif_neuron *base = (knp_altai_lif_neuron *)xcalloc(n ? n : 1, sizeof(knp_altai_lif_neuron));
+    for (uint32_t i = 0; i < n; ++i) base[i] = neurons[i].base;
+    int id = ko_add_altai_lif_population(net, base, n);
+    free(base);
+    ko_pop *p = &net->pops[id];
+    p->resource = 1;
+    p->rx = (ko_res_ext *)xcalloc(n, sizeof(ko_res_ext));
+    for (uint32_t i = 0; i < n; ++i) KO_SET_RES_EXT(&p->rx[i], &neurons[i]);
+    return id;
+}
+
 int ko_add_resource_blifat_population(ko_net *net, const knp_resource_blifat_neuron *neurons, uint32_t n) {
     int id = add_pop(net, n, 1);
     ko_pop *p = &net->pops[id];
     for (uint32_t i = 0; i < n; ++i) {
         const knp_resource_blifat_neuron *s = &neurons[i];
         p->nb[i] = s->base;
-        ko_res_ext *x = &p->rx[i];
-        x->synapse_sum_threshold_coefficient = s->synapse_sum_threshold_coefficient;
-        x->dopamine_plasticity_time = s->dopamine_plasticity_time;
-        x->free_synaptic_resource = s->free_synaptic_resource;
-        x->synaptic_resource_threshold = s->synaptic_resource_threshold;
-        x->resource_drain_coefficient = s->resource_drain_coefficient;
-        x->stability = s->stability;
-        x->stability_change_parameter = s->stability_change_parameter;
-        x->stability_change_at_isi = s->stability_change_at_isi;
-        x->isi_max = s->isi_max;
-        x->d_h = s->d_h;
-        x->isi_status = s->isi_status;
-        x->last_step = s->last_step;
-        x->last_spike_step = s->last_spike_step;
-        x->first_isi_spike = s->first_isi_spike;
-        x->is_being_forced = s->is_being_forced;
+        KO_SET_RES_EXT(&p->rx[i], s);
     }
     return id;
 }
@@ -693,7 +710,16 @@ static void recalc_weights(ko_synref *s, size_t k) {
     }
 }
 
-static int update_isi(knp_blifat_neuron *n, ko_res_ext *x, uint64_t step) {
+/* additional_threshold_ and dopamine_value_ of neuron i of a resource population (BLIFAT or AltAI-LIF base);
+ * the training rules read nothing else of the base neuron (blifat_stdp.h, altai_stdp.h) */
+static double *res_add_thr(ko_pop *P, uint32_t i) {
+    return P->altai ? &P->na[i].additional_threshold : &P->nb[i].additional_threshold;
+}
+static double res_dop(const ko_pop *P, uint32_t i) {
+    return P->altai ? P->na[i].dopamine_value : P->nb[i].dopamine_value;
+}
+
+static int update_isi(double dopamine, ko_res_ext *x, uint64_t step) {
     if (x->is_being_forced) {
         x->isi_status = KNP_ISI_IS_FORCED;
         return x->isi_status;
@@ -708,7 +734,7 @@ static int update_isi(knp_blifat_neuron *n, ko_res_ext *x, uint64_t step) {
             if (x->last_step - step < (uint64_t)x->isi_max) x->isi_status = KNP_ISI_PERIOD_CONTINUED;
             break;
         case KNP_ISI_PERIOD_CONTINUED:
-            if (x->last_step - step >= (uint64_t)x->isi_max || n->dopamine_value != 0) {
+            if (x->last_step - step >= (uint64_t)x->isi_max || dopamine != 0) {
                 x->isi_status = KNP_ISI_PERIOD_STARTED;
                 x->first_isi_spike = step;
             }
@@ -738,21 +764,21 @@ static void train_population(ko_net *net, uint32_t pop, const uint32_t *spikes, 
     for (uint32_t si = 0; si < n_spikes; ++si) {
         uint32_t idx = spikes[si];
         size_t k = connected_synapses(net, pop, idx, &syn, &cap);
-        knp_blifat_neuron *n = &P->nb[idx];
+        double *add_thr = res_add_thr(P, idx);
         ko_res_ext *x = &P->rx[idx];
         x->last_spike_step = step;
-        update_isi(n, x, step);
+        update_isi(res_dop(P, idx), x, step);
         if (x->isi_status == KNP_ISI_PERIOD_STARTED) x->stability -= x->stability_change_at_isi;
-        n->additional_threshold = 0.0;
+        *add_thr = 0.0;
         for (size_t i = 0; i < k; ++i) {
             ko_rule *r = &syn[i].proj->rule[syn[i].syn];
             wreal w = syn[i].proj->w[syn[i].syn];
-            n->additional_threshold += w * (wreal)(w > 0);
+            *add_thr += w * (wreal)(w > 0);
             int had_spike = point_in_interval(step - r->dopamine_plasticity_period, step,
                                               r->last_spike_step + syn[i].proj->delay[syn[i].syn] - 1);
             if (x->isi_status != KNP_ISI_PERIOD_CONTINUED || had_spike) r->has_contributed = had_spike;
         }
-        n->additional_threshold *= x->synapse_sum_threshold_coefficient;
+        *add_thr *= x->synapse_sum_threshold_coefficient;
         if (x->isi_status != KNP_ISI_PERIOD_CONTINUED)
             for (size_t i = 0; i < k; ++i) syn[i].proj->rule[syn[i].syn].had_hebbian_update = 0;
         if (x->isi_status != KNP_ISI_IS_FORCED) {
@@ -772,27 +798,27 @@ static void train_population(ko_net *net, uint32_t pop, const uint32_t *spikes, 
     }
     /* do_dopamine_plasticity_impl (blifat_stdp.h:106-155) */
     for (uint32_t idx = 0; idx < P->n; ++idx) {
-        knp_blifat_neuron *n = &P->nb[idx];
+        const double dopamine = res_dop(P, idx);
         ko_res_ext *x = &P->rx[idx];
-        if (n->dopamine_value > 0.0 || (n->dopamine_value < 0.0 && x->isi_status != KNP_ISI_IS_FORCED)) {
+        if (dopamine > 0.0 || (dopamine < 0.0 && x->isi_status != KNP_ISI_IS_FORCED)) {
             size_t k = connected_synapses(net, pop, idx, &syn, &cap);
             for (size_t i = 0; i < k; ++i) {
                 ko_rule *r = &syn[i].proj->rule[syn[i].syn];
                 if (step - x->last_spike_step <= (uint64_t)x->dopamine_plasticity_time && r->has_contributed) {
-                    wreal resource_change = (wreal)(n->dopamine_value * stability_factor(x->stability));
+                    wreal resource_change = (wreal)(dopamine * stability_factor(x->stability));
                     r->synaptic_resource += resource_change;
                     x->free_synaptic_resource -= resource_change;
                 }
             }
-            if (x->is_being_forced || n->dopamine_value < 0) {
-                x->stability = (wreal)(x->stability - n->dopamine_value * x->stability_change_parameter);
+            if (x->is_being_forced || dopamine < 0) {
+                x->stability = (wreal)(x->stability - dopamine * x->stability_change_parameter);
                 x->stability = x->stability > (wreal)0 ? x->stability : (wreal)0;
             } else {
                 const double dopamine_constant = 2.0;
                 const double difference = fabs((double)(step - x->first_isi_spike - (uint64_t)x->isi_max));
                 double f = dopamine_constant - difference / (double)x->isi_max;
                 if (f < -1.0) f = -1.0;
-                x->stability = (wreal)(x->stability + x->stability_change_parameter * n->dopamine_value * f);
+                x->stability = (wreal)(x->stability + x->stability_change_parameter * dopamine * f);
             }
             recalc_weights(syn, k);
         }
@@ -899,7 +925,13 @@ static void population_step(ko_net *net, uint32_t pop, uint64_t step) {
     ko_pop *P = &net->pops[pop];
     for (uint32_t i = 0; i < P->n; ++i) {
         if (P->lif) lif_pre(&P->nl[i]);
-        else if (P->altai) altai_pre(&P->na[i]);
+        else if (P->altai) {
+            altai_pre(&P->na[i]);
+            if (P->resource) { /* altai_impl.h:43-52 */
+                P->na[i].dopamine_value = 0.0;
+                P->rx[i].is_being_forced = 0;
+            }
+        }
         else if (P->resource) resource_pre(&P->nb[i], &P->rx[i]);
         else blifat_pre(&P->nb[i]);
     }
@@ -924,6 +956,7 @@ static void population_step(ko_net *net, uint32_t pop, uint64_t step) {
             }
             if (P->altai) {
                 altai_impact(&P->na[imp->post], imp);
+                if (P->resource && imp->type == KNP_EXCITATORY) P->rx[imp->post].is_being_forced |= is_forcing;
                 continue;
             }
             knp_blifat_neuron *n = &P->nb[imp->post];
@@ -1056,6 +1089,36 @@ int ko_read_altai_lif_neurons(ko_net *net, uint32_t pop, knp_altai_lif_neuron *o
     return 0;
 }
 
+int ko_read_resource_altai_lif_neurons(ko_net *net, uint32_t pop, knp_resource_altai_lif_neuron *out, uint32_t n) {
+    if (pop >= net->n_pops || n > net->pops[pop].n || !net->pops[pop].altai || !net->pops[pop].resource) return -1;
+    knp_altai_lif_neuron *base = (knp_altai_lif_neuron *)xcalloc(n ? n : 1, sizeof(knp_altai_lif_neuron));
+    ko_read_altai_lif_neurons(net, pop, base, n);
+    const ko_res_ext *rx = net->pops[pop].rx;
+    for (uint32_t i = 0; i < n; ++i) {
+        knp_resource_altai_lif_neuron *o = &out[i];
+        const ko_res_ext *x = &rx[i];
+        memset(o, 0, sizeof(*o));
+        o->base = base[i];
+        o->synapse_sum_threshold_coefficient = (float)x->synapse_sum_threshold_coefficient;
+        o->dopamine_plasticity_time = x->dopamine_plasticity_time;
+        o->free_synaptic_resource = (float)x->free_synaptic_resource;
+        o->synaptic_resource_threshold = (float)x->synaptic_resource_threshold;
+        o->resource_drain_coefficient = x->resource_drain_coefficient;
+        o->stability = (float)x->stability;
+        o->stability_change_parameter = (float)x->stability_change_parameter;
+        o->stability_change_at_isi = (float)x->stability_change_at_isi;
+        o->isi_max = x->isi_max;
+        o->d_h = (float)x->d_h;
+        o->isi_status = x->isi_status;
+        o->last_step = x->last_step;
+        o->last_spike_step = x->last_spike_step;
+        o->first_isi_spike = x->first_isi_spike;
+        o->is_being_forced = (uint8_t)x->is_being_forced;
+    }
+    free(base);
+    return 0;
+}
+
 int ko_read_blifat_neurons(ko_net *net, uint32_t pop, knp_blifat_neuron *out, uint32_t n) {
     if (pop >= net->n_pops || n > net->pops[pop].n || net->pops[pop].lif || net->pops[pop].altai) return -1;
     memcpy(out, net->pops[pop].nb, (size_t)n * sizeof(knp_blifat_neuron));
@@ -1063,7 +1126,7 @@ int ko_read_blifat_neurons(ko_net *net, uint32_t pop, knp_blifat_neuron *out, ui
 }
 
 int ko_read_resource_neurons(ko_net *net, uint32_t pop, knp_resource_blifat_neuron *out, uint32_t n) {
-    if (pop >= net->n_pops || n > net->pops[pop].n || !net->pops[pop].resource) return -1;
+    if (pop >= net->n_pops || n > net->pops[pop].n || !net->pops[pop].resource || net->pops[pop].altai) return -1;
     ko_pop *P = &net->pops[pop];
     for (uint32_t i = 0; i < n; ++i) {
         out[i].base = P->nb[i];

@@ -41,6 +41,7 @@ def _load(policy):
         "ko_add_resource_blifat_population": (i32, [P, C.POINTER(abi.ResourceBLIFATNeuron), u32]),
         "ko_add_lif_population": (i32, [P, C.POINTER(abi.LIFNeuron), u32]),
         "ko_add_altai_lif_population": (i32, [P, C.POINTER(abi.AltAILIFNeuron), u32]),
+        "ko_add_resource_altai_lif_population": (i32, [P, C.POINTER(abi.ResourceAltAILIFNeuron), u32]),
         "ko_inject_impacts": (i32, [P, u32, pu32, pf32, pu8, u32]),
         "ko_add_delta_projection": (i32, [P, u32, u32, u32, pu32, pu32, pf32, pu32, pu8, u64]),
         "ko_add_resource_stdp_projection": (
@@ -61,6 +62,7 @@ def _load(policy):
         "ko_read_resource_neurons": (i32, [P, u32, C.POINTER(abi.ResourceBLIFATNeuron), u32]),
         "ko_read_lif_neurons": (i32, [P, u32, C.POINTER(abi.LIFNeuron), u32]),
         "ko_read_altai_lif_neurons": (i32, [P, u32, C.POINTER(abi.AltAILIFNeuron), u32]),
+        "ko_read_resource_altai_lif_neurons": (i32, [P, u32, C.POINTER(abi.ResourceAltAILIFNeuron), u32]),
         "ko_read_weights": (i32, [P, u32, pf64, u64]),
         "ko_read_resources": (i32, [P, u32, pf64, u64]),
         "ko_read_resource_rules": (i32, [P, u32, C.POINTER(abi.ResourceRule), u64]),
@@ -127,6 +129,12 @@ class Oracle:
     def add_altai_lif_population(self, neurons):
         n = len(neurons)
         pid = self._chk(self.lib.ko_add_altai_lif_population(self.net, neurons, n))
+        self.pop_sizes.append(n)
+        return pid
+
+    def add_resource_altai_lif_population(self, neurons):
+        n = len(neurons)
+        pid = self._chk(self.lib.ko_add_resource_altai_lif_population(self.net, neurons, n))
         self.pop_sizes.append(n)
         return pid
 
@@ -289,6 +297,12 @@ class Oracle:
         n = self.pop_sizes[pop]
         out = (abi.AltAILIFNeuron * n)()
         self._chk(self.lib.ko_read_altai_lif_neurons(self.net, pop, out, n))
+        return out
+
+    def resource_altai_lif_neurons(self, pop):
+        n = self.pop_sizes[pop]
+        out = (abi.ResourceAltAILIFNeuron * n)()
+        self._chk(self.lib.ko_read_resource_altai_lif_neurons(self.net, pop, out, n))
         return out
 
     def weights(self, proj):

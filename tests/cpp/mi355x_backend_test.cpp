@@ -1,6 +1,7 @@
 // mi355x_backend_test.cpp - the reference's backend integration tests, run against the MI355X backend
-// through the knp::core API: SmallestNetwork, AdditiveSTDPBLIFATNetwork and ResourceSTDPBLIFATNetwork
-// (knp/tests/backend/single_thread_cpu_test.cpp:54-264, generators knp/tests/common/generators.h:48-100),
+// through the knp::core API: SmallestNetwork, AdditiveSTDPBLIFATNetwork, ResourceSTDPBLIFATNetwork and
+// ResourceSTDPAltAILIFNetwork (knp/tests/backend/single_thread_cpu_test.cpp:54-358, generators
+// knp/tests/common/generators.h:48-100),
 // a LIF neuron trace (lif_impl.h:31-84), an AltAI-LIF known answer (knp/tests/framework/altai_lif_test.cpp),
 // the supported-type names (:362-381), BackendLoader (knp/tests/framework/backend_loader_test.cpp:29-70)
 // and the Backend::start run loop with a stop predicate.
@@ -59,6 +60,43 @@ static std::vector<knp::core::Step> run_smallest(knp::core::Backend &backend, co
         if (!output.empty()) results.push_back(step);
     }
     return results;
+}
+
+// ResourceSTDPBLIFATNetwork / ResourceSTDPAltAILIFNetwork (single_thread_cpu_test.cpp:179-358): one
+// synaptic-resource neuron of the given model with a plastic input and a plastic delay-6 loop; same spike
+// steps as SmallestNetwork, and training changes the loop weight.
+template <class NeuronType>
+static void run_resource_network() {
+    using STDPDeltaProjection = knp::core::Projection<knp::synapse_traits::SynapticResourceSTDPDeltaSynapse>;
+    using StdpPopulation = knp::core::Population<NeuronType>;
+    Backend backend;
+    StdpPopulation population{
+        UID(),
+        [](uint64_t) -> std::optional<typename StdpPopulation::NeuronParameters> {
+            typename StdpPopulation::NeuronParameters neuron{{}};
+            neuron.synaptic_resource_threshold_ = 1;
+            neuron.free_synaptic_resource_ = 2;
+            neuron.isi_max_ = 0;
+            return neuron;
+        },
+        1};
+    auto loop = STDPDeltaProjection{
+        population.get_uid(), population.get_uid(),
+        [](size_t) { return STDPDeltaProjection::Synapse{{{1.0, 6, knp::synapse_traits::OutputType::EXCITATORY}, {0, 1, 2}}, 0, 0}; },
+        1};
+    STDPDeltaProjection input{
+        UID{false}, population.get_uid(),
+        [](size_t) { return STDPDeltaProjection::Synapse{{{1.0, 1, knp::synapse_traits::OutputType::EXCITATORY}, {0, 1, 2, 0.1F}}, 0, 0}; },
+        1};
+    const UID input_uid = input.get_uid();
+    backend.load_populations({population});
+    backend.load_projections({input, loop});
+    backend._init();
+    backend.start_learning();
+    CHECK(run_smallest(backend, input_uid, population.get_uid()) == kExpected);
+    auto data = backend.get_network_data();
+    const auto &loop_after = std::get<STDPDeltaProjection>(data.second[1]);
+    CHECK(std::get<knp::core::synapse_data>(loop_after[0]).weight_ != 1.0f);
 }
 
 int main(int argc, char **argv) {
@@ -133,38 +171,8 @@ int main(int argc, char **argv) {
         CHECK(std::get<knp::core::synapse_data>(loop_after[0]).weight_ != 1.0f);
     });
 
-    run_case("ResourceSTDPBLIFATNetwork", [] {
-        using STDPDeltaProjection = knp::core::Projection<knp::synapse_traits::SynapticResourceSTDPDeltaSynapse>;
-        using BlifatStdpPopulation = knp::core::Population<knp::neuron_traits::SynapticResourceSTDPBLIFATNeuron>;
-        Backend backend;
-        BlifatStdpPopulation population{
-            UID(),
-            [](uint64_t) -> std::optional<BlifatStdpPopulation::NeuronParameters> {
-                BlifatStdpPopulation::NeuronParameters neuron{{}};
-                neuron.synaptic_resource_threshold_ = 1;
-                neuron.free_synaptic_resource_ = 2;
-                neuron.isi_max_ = 0;
-                return neuron;
-            },
-            1};
-        auto loop = STDPDeltaProjection{
-            population.get_uid(), population.get_uid(),
-            [](size_t) { return STDPDeltaProjection::Synapse{{{1.0, 6, knp::synapse_traits::OutputType::EXCITATORY}, {0, 1, 2}}, 0, 0}; },
-            1};
-        STDPDeltaProjection input{
-            UID{false}, population.get_uid(),
-            [](size_t) { return STDPDeltaProjection::Synapse{{{1.0, 1, knp::synapse_traits::OutputType::EXCITATORY}, {0, 1, 2, 0.1F}}, 0, 0}; },
-            1};
-        const UID input_uid = input.get_uid();
-        backend.load_populations({population});
-        backend.load_projections({input, loop});
-        backend._init();
-        backend.start_learning();
-        CHECK(run_smallest(backend, input_uid, population.get_uid()) == kExpected);
-        auto data = backend.get_network_data();
-        const auto &loop_after = std::get<STDPDeltaProjection>(data.second[1]);
-        CHECK(std::get<knp::core::synapse_data>(loop_after[0]).weight_ != 1.0f);
-    });
+    run_case("ResourceSTDPBLIFATNetwork", [] { run_resource_network<knp::neuron_traits::SynapticResourceSTDPBLIFATNeuron>(); });
+    run_case("ResourceSTDPAltAILIFNetwork", [] { run_resource_network<knp::neuron_traits::SynapticResourceSTDPAltAILIFNeuron>(); });
 
     run_case("LIFNetwork", [] {
         // one LIF neuron (leak 0.5, refractory period 2) fed 0.75 every step: it fires when the leaked

@@ -81,9 +81,13 @@ def test_additive_stdp_network():
     assert res["gpu"][1] == pytest.approx(res["oracle"][1], rel=1e-6)
 
 
-def resource_smallest(o):
-    neuron = abi.resource_blifat_defaults(synaptic_resource_threshold=1.0, free_synaptic_resource=2.0, isi_max=0)
-    pop = o.add_resource_blifat_population(abi.neuron_array(neuron, 1))
+def resource_smallest(o, model="blifat"):
+    """ResourceSTDPBLIFATNetwork / ResourceSTDPAltAILIFNetwork (single_thread_cpu_test.cpp:179-358)."""
+    kw = dict(synaptic_resource_threshold=1.0, free_synaptic_resource=2.0, isi_max=0)
+    if model == "altai":
+        pop = o.add_resource_altai_lif_population(abi.neuron_array(abi.resource_altai_lif_defaults(**kw), 1))
+    else:
+        pop = o.add_resource_blifat_population(abi.neuron_array(abi.resource_blifat_defaults(**kw), 1))
     r_in = abi.rule_array(abi.resource_rule_defaults(synaptic_resource=0.0, w_min=1.0, w_max=2.0, d_u=0.1), 1)
     r_loop = abi.rule_array(abi.resource_rule_defaults(synaptic_resource=0.0, w_min=1.0, w_max=2.0), 1)
     inp = o.add_resource_stdp_projection(abi.KNP_NO_POPULATION, 1, pop, [0], [0], [1.0], [1], r_in)
@@ -92,11 +96,12 @@ def resource_smallest(o):
     return pop, inp, loop
 
 
-def test_resource_stdp_network():
-    say("case: ResourceSTDPBLIFATNetwork")
+@pytest.mark.parametrize("model", ["blifat", "altai"])
+def test_resource_stdp_network(model):
+    say(f"case: ResourceSTDP network, {model} neurons")
     g, o = MI355XBackend(0), Oracle("exact")
-    gp, gi, gl = resource_smallest(g)
-    op, oi, ol = resource_smallest(o)
+    gp, gi, gl = resource_smallest(g, model)
+    op, oi, ol = resource_smallest(o, model)
     assert run_smallest(g, gp, gi) == EXPECTED
     assert run_smallest(o, op, oi) == EXPECTED
     for proj_g, proj_o in ((gi, oi), (gl, ol)):
@@ -104,11 +109,69 @@ def test_resource_stdp_network():
         rg, ro = g.resource_rules(proj_g)[0], o.resource_rules(proj_o)[0]
         assert (rg.synaptic_resource, rg.last_spike_step, rg.had_hebbian_update, rg.has_contributed) == \
             (ro.synaptic_resource, ro.last_spike_step, ro.had_hebbian_update, ro.has_contributed)
-    ng, no = g.resource_neurons(gp)[0], o.resource_neurons(op)[0]
+    read = "resource_altai_lif_neurons" if model == "altai" else "resource_neurons"
+    ng, no = getattr(g, read)(gp)[0], getattr(o, read)(op)[0]
     for f in ("free_synaptic_resource", "stability", "isi_status", "last_step", "last_spike_step",
               "first_isi_spike"):
         assert getattr(ng, f) == getattr(no, f), f
+    if model == "altai":
+        for f in ("potential", "activity_time", "additional_threshold"):
+            assert getattr(ng.base, f) == getattr(no.base, f), f
     assert g.weights(gl)[0] != 1.0
+
+
+def resource_altai_network(o, seed):
+    """Synaptic-resource AltAI-LIF population (256 neurons, leak -1, threshold 4) trained by a plastic
+    channel projection (64 channels, Bernoulli source), with lateral inhibition and a dopamine channel."""
+    rng = np.random.default_rng(seed)
+    n, nch, S = 256, 64, 3000
+    base = abi.altai_lif_defaults(activation_threshold=4, potential_leak=-1)
+    cell = abi.resource_altai_lif_defaults(base, synaptic_resource_threshold=1.5, free_synaptic_resource=1.0,
+                                           isi_max=4, d_h=0.25, dopamine_plasticity_time=4,
+                                           stability_change_parameter=0.5, stability_change_at_isi=0.125)
+    pop = o.add_resource_altai_lif_population(abi.neuron_array(cell, n))
+    rules = abi.rule_array(abi.resource_rule_defaults(synaptic_resource=0.0, w_min=0.5, w_max=3.0, d_u=0.0625,
+                                                      dopamine_plasticity_period=5), S)
+    pre = rng.integers(0, nch, S).astype(np.uint32)
+    post = rng.integers(0, n, S).astype(np.uint32)
+    inp = o.add_resource_stdp_projection(abi.KNP_NO_POPULATION, nch, pop, pre, post, np.full(S, 0.5, np.float32),
+                                         np.ones(S, np.uint32), rules)
+    o.add_bernoulli_source(inp, np.full(nch, workloads.bernoulli_threshold(0.15), np.uint32), seed=21)
+    L = 4000
+    o.add_delta_projection(pop, n, pop, rng.integers(0, n, L).astype(np.uint32), rng.integers(0, n, L).astype(np.uint32),
+                           np.full(L, 1.0, np.float32), rng.integers(1, 3, L).astype(np.uint32),
+                           np.full(L, abi.INHIBITORY_CURRENT, np.uint8))
+    dop = o.add_delta_projection(abi.KNP_NO_POPULATION, 1, pop, np.zeros(n, np.uint32), np.arange(n, dtype=np.uint32),
+                                 np.full(n, 0.5, np.float32), np.ones(n, np.uint32), np.full(n, abi.DOPAMINE, np.uint8))
+    o.start_learning()
+    return pop, inp, dop
+
+
+def test_resource_altai_training_matches_oracle():
+    """Synaptic-resource AltAI-LIF training (altai_stdp.h:35-149) over 300 steps with dopamine every 10
+    steps: spike agreement >= 0.99, and weights within 1e-6 when the spike trains are identical."""
+    say("case: resource AltAI-LIF training, 300 steps")
+    runs = {}
+    for name in ("gpu", "oracle"):
+        o = Oracle("exact") if name == "oracle" else MI355XBackend(0)
+        pop, inp, dop = resource_altai_network(o, 5)
+        o.record_spikes(pop, 300)
+        for t in range(0, 300, 10):
+            o.step(9)
+            o.push_spikes(dop, [0])
+            o.step(1)
+        runs[name] = (o.read_spike_record(pop), o.weights(inp), o.resource_altai_lif_neurons(pop))
+    sg, so = runs["gpu"][0], runs["oracle"][0]
+    agr = agreement(sg, so)
+    say(f"  spikes gpu={sg.sum()} oracle={so.sum()} agreement={agr:.6f}")
+    assert so.sum() > 300
+    assert agr >= 0.99
+    wg, wo = runs["gpu"][1], runs["oracle"][1]
+    assert np.any(wo != 0.5)
+    if np.array_equal(sg, so):
+        assert np.max(np.abs(wg - wo)) <= 1e-6
+        ng, no = runs["gpu"][2], runs["oracle"][2]
+        assert [x.isi_status for x in ng] == [x.isi_status for x in no]
 
 
 def test_plain_small_bit_exact():

@@ -105,6 +105,25 @@ def test_resource_stdp_kat(oracle_built, policy):
     assert o.weights(loop)[0] != 1.0
 
 
+@pytest.mark.parametrize("policy", POLICIES)
+def test_resource_stdp_altai_kat(oracle_built, policy):
+    """ResourceSTDPAltAILIFNetwork (single_thread_cpu_test.cpp:267-358): the same network and expected spike
+    steps as the BLIFAT variant, on default AltAI-LIF neurons; the loop weight changes."""
+    o = Oracle(policy)
+    neuron = abi.resource_altai_lif_defaults(synaptic_resource_threshold=1.0, free_synaptic_resource=2.0, isi_max=0)
+    pop = o.add_resource_altai_lif_population(abi.neuron_array(neuron, 1))
+    r_in = abi.rule_array(abi.resource_rule_defaults(synaptic_resource=0.0, w_min=1.0, w_max=2.0, d_u=0.1), 1)
+    r_loop = abi.rule_array(abi.resource_rule_defaults(synaptic_resource=0.0, w_min=1.0, w_max=2.0), 1)
+    inp = o.add_resource_stdp_projection(abi.KNP_NO_POPULATION, 1, pop, [0], [0], [1.0], [1], r_in)
+    loop = o.add_resource_stdp_projection(pop, 1, pop, [0], [0], [1.0], [6], r_loop)
+    o.start_learning()
+    assert run_smallest(o, pop, inp) == EXPECTED
+    assert o.weights(loop)[0] != 1.0
+    x = o.resource_altai_lif_neurons(pop)[0]
+    # 0 -> unlimited (INT64_MAX) at step 0, then one step less of activity per step (altai_impl.h:103-114)
+    assert x.base.activity_time == abi.INT64_MAX - 19 and x.isi_status != abi.ISI_NOT_IN_PERIOD
+
+
 def test_resource_conservation(oracle_built):
     """free + sum of synaptic resources is invariant under resource-STDP training with drain 0."""
     o = Oracle("f64")
