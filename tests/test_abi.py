@@ -70,6 +70,12 @@ def test_defaults_match_reference_values():
     assert (al.is_diff, al.is_reset, al.leak_rev, al.saturate, al.do_not_save) == (0, 1, 1, 1, 0)
     assert (al.activation_threshold, al.negative_activation_threshold, al.potential_leak) == (1, 30000, 0)
     assert al.activity_time == 0 and al.additional_threshold == 0.0
+    ra = abi.ResourceAltAILIFNeuron()
+    lib.knp_resource_altai_lif_defaults(C.byref(ra))
+    assert bytes(ra.base) == bytes(al)
+    ref = abi.resource_altai_lif_defaults()
+    for f, _ in abi.ResourceAltAILIFNeuron._fields_[1:]:
+        assert getattr(ra, f) == getattr(ref, f) == getattr(r, f), f
 
 
 def test_bernoulli_hash_matches_oracle(oracle_built):
@@ -96,6 +102,8 @@ int main(void) {
   printf("knp_lif_neuron %zu\n", sizeof(knp_lif_neuron));
   P(knp_lif_neuron, refract_counter) P(knp_lif_neuron, refract_period)
   printf("knp_altai_lif_neuron %zu\n", sizeof(knp_altai_lif_neuron));
+  printf("knp_resource_altai_lif_neuron %zu\n", sizeof(knp_resource_altai_lif_neuron));
+  P(knp_resource_altai_lif_neuron, d_h) P(knp_resource_altai_lif_neuron, is_being_forced)
   P(knp_altai_lif_neuron, potential) P(knp_altai_lif_neuron, potential_reset_value)
   P(knp_altai_lif_neuron, dopamine_value) P(knp_altai_lif_neuron, activity_time)
   P(knp_blifat_neuron, potential) P(knp_blifat_neuron, total_blocking_period)
@@ -121,6 +129,9 @@ int main(void) {
     assert int(vals["knp_altai_lif_neuron"]) == C.sizeof(abi.AltAILIFNeuron)
     for f in ("potential", "potential_reset_value", "dopamine_value", "activity_time"):
         assert int(vals["knp_altai_lif_neuron." + f]) == getattr(abi.AltAILIFNeuron, f).offset, f
+    assert int(vals["knp_resource_altai_lif_neuron"]) == C.sizeof(abi.ResourceAltAILIFNeuron)
+    for f in ("d_h", "is_being_forced"):
+        assert int(vals["knp_resource_altai_lif_neuron." + f]) == getattr(abi.ResourceAltAILIFNeuron, f).offset, f
     assert int(vals["knp_blifat_neuron.potential"]) == abi.BLIFATNeuron.potential.offset
     assert int(vals["knp_blifat_neuron.total_blocking_period"]) == abi.BLIFATNeuron.total_blocking_period.offset
     assert int(vals["knp_blifat_neuron.random_number_generator_state"]) == \
