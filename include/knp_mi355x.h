@@ -326,6 +326,15 @@ int knp_mi355x_read_resource_altai_lif_neurons(knp_net *net, uint32_t pop, knp_r
                                                uint32_t n);
 int knp_mi355x_read_weights(knp_net *net, uint32_t proj, float *out, uint64_t n_syn);
 int knp_mi355x_read_resource_rules(knp_net *net, uint32_t proj, knp_resource_rule *out, uint64_t n_syn);
+/* Network file: save_network / load_network of knp/framework-library/include/knp/framework/sonata/
+ * network_io.h:41,51 (SONATA over HDF5 in the reference; HDF5 is absent here). The file holds every
+ * population (neuron model and the C ABI struct of each neuron with its current state) and projection
+ * (kind, presynaptic / postsynaptic populations, synapses with current weights, synaptic-resource rules
+ * or additive-STDP time constants, learning lock), in load order. As in the reference, impacts still in
+ * flight, spike records, channel sources and dense input projections are not part of the network file.
+ * load_network adds the file's populations and projections to a handle that has none yet. */
+int knp_mi355x_save_network(knp_net *net, const char *path);
+int knp_mi355x_load_network(knp_net *net, const char *path);
 /* Number of SynapticImpacts created so far (sum over steps of spikes x out-degree over all projections). */
 int knp_mi355x_get_event_count(knp_net *net, uint64_t *events);
 /* Counters of the spike -> impact expansion so far: impacts (as above) and presynaptic rows expanded

@@ -40,6 +40,11 @@ KERNEL_DENSE = 3
 KERNEL_TILE_SCATTER = 4
 KERNEL_TILE_RUN = 5
 
+# projection kinds in a network file
+PROJ_DELTA = 0
+PROJ_RESOURCE = 1
+PROJ_ADDITIVE = 2
+
 # knp_scatter_mode
 SCATTER_AUTO = 0
 SCATTER_ATOMIC = 1

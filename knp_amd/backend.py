@@ -75,6 +75,8 @@ def _declare(lib):
         "knp_mi355x_read_weights": (i32, [P, u32, pf32, u64]),
         "knp_mi355x_read_resource_rules": (i32, [P, u32, C.POINTER(abi.ResourceRule), u64]),
         "knp_mi355x_get_event_count": (i32, [P, C.POINTER(u64)]),
+        "knp_mi355x_save_network": (i32, [P, C.c_char_p]),
+        "knp_mi355x_load_network": (i32, [P, C.c_char_p]),
         "knp_mi355x_get_counters": (i32, [P, C.POINTER(u64), C.POINTER(u64)]),
         "knp_mi355x_get_accumulator_bits": (i32, [P, u32, C.POINTER(C.c_int32)]),
         "knp_mi355x_enable_timing": (i32, [P, i32]),
@@ -120,6 +122,34 @@ def device_name(dev=0):
     if lib.knp_mi355x_device_name(dev, buf, 256) != 0:
         raise BackendError(lib.knp_mi355x_last_error().decode())
     return buf.value.decode()
+
+
+def read_network_sizes(path):
+    """Population sizes and synapse counts of a network file, in load order (the layout written by
+    knp_mi355x_save_network: an 8-byte magic and version / population / projection counts, then per
+    population a model tag, count and struct size before its structs, per projection a fixed header before
+    its synapse arrays)."""
+    with open(path, "rb") as f:
+        head = f.read(20)
+        n_pops, n_projs = np.frombuffer(head[12:20], dtype=np.uint32)
+        pops, projs = [], []
+        for _ in range(int(n_pops)):
+            tag, n, size = np.frombuffer(f.read(12), dtype=np.uint32)
+            pops.append(int(n))
+            f.seek(int(n) * int(size), 1)
+        for _ in range(int(n_projs)):
+            kind, _pre, _npre, _post = np.frombuffer(f.read(16), dtype=np.uint32)
+            n = int(np.frombuffer(f.read(8), dtype=np.uint64)[0])
+            f.seek(4 + n * (4 + 4 + 4 + 4 + 1), 1)
+            if kind == abi.PROJ_RESOURCE:
+                size = int(np.frombuffer(f.read(4), dtype=np.uint32)[0])
+                f.seek(n * size, 1)
+            elif kind == abi.PROJ_ADDITIVE:
+                f.seek(8 * n, 1)
+                n_stdp = int(np.frombuffer(f.read(4), dtype=np.uint32)[0])
+                f.seek(5 * n_stdp, 1)
+            projs.append(n)
+    return pops, projs
 
 
 class MI355XBackend:
@@ -398,6 +428,15 @@ class MI355XBackend:
         return np.array([x.synaptic_resource for x in r], dtype=np.float32)
 
     # ---- kernel timing ------------------------------------------------------------------------
+    def save_network(self, path):
+        """Write every population and projection with its current state to `path`."""
+        self._chk(self.lib.knp_mi355x_save_network(self.net, os.fsencode(path)))
+
+    def load_network(self, path):
+        """Add the populations and projections of a network file to this (empty) backend."""
+        self._chk(self.lib.knp_mi355x_load_network(self.net, os.fsencode(path)))
+        self.pop_sizes, self.proj_sizes = read_network_sizes(path)
+
     def enable_timing(self, on=True):
         self._chk(self.lib.knp_mi355x_enable_timing(self.net, 1 if on else 0))
 

@@ -2361,3 +2361,176 @@ int knp_mi355x_kernel_time(knp_net *net, int kernel_id, double *avg_ms, uint64_t
 }
 
 }  // extern "C"
+
+// =====================================================================================================
+// Network file (save / load of populations and projections with their current state)
+// =====================================================================================================
+namespace {
+
+constexpr char kNetMagic[8] = {'K', 'N', 'P', 'N', 'E', 'T', '0', '1'};
+enum NetFileModel : uint32_t { NF_BLIFAT = 0, NF_RES_BLIFAT = 1, NF_LIF = 2, NF_ALTAI = 3, NF_RES_ALTAI = 4 };
+
+struct FileCloser {
+    FILE *f;
+    ~FileCloser() {
+        if (f) std::fclose(f);
+    }
+};
+
+bool put(FILE *f, const void *p, size_t n) { return n == 0 || std::fwrite(p, 1, n, f) == n; }
+template <class T>
+bool put_v(FILE *f, const T &v) { return put(f, &v, sizeof v); }
+template <class T>
+bool put_vec(FILE *f, const std::vector<T> &v) { return put(f, v.data(), v.size() * sizeof(T)); }
+bool get(FILE *f, void *p, size_t n) { return n == 0 || std::fread(p, 1, n, f) == n; }
+template <class T>
+bool get_v(FILE *f, T &v) { return get(f, &v, sizeof v); }
+template <class T>
+bool get_vec(FILE *f, std::vector<T> &v, uint64_t n) {
+    v.resize(n);
+    return get(f, v.data(), n * sizeof(T));
+}
+
+// One population: model tag, count, struct size, then the C ABI structs with the current state.
+template <class T>
+int save_pop_structs(knp_net *net, FILE *f, uint32_t pop, uint32_t tag,
+                     int (*read)(knp_net *, uint32_t, T *, uint32_t)) {
+    const uint32_t n = net->pops[pop].n, size = sizeof(T);
+    std::vector<T> v(n);
+    int rc = read(net, pop, v.data(), n);
+    if (rc) return rc;
+    if (!put_v(f, tag) || !put_v(f, n) || !put_v(f, size) || !put_vec(f, v))
+        return fail(KNP_ERR_INVALID_ARGUMENT, "network file: write failed");
+    return KNP_OK;
+}
+
+template <class T>
+int load_pop_structs(knp_net *net, FILE *f, uint32_t n, uint32_t size,
+                     int (*add)(knp_net *, const T *, uint32_t, uint32_t *)) {
+    if (size != sizeof(T)) return fail(KNP_ERR_INVALID_ARGUMENT, "network file: neuron struct size mismatch");
+    std::vector<T> v;
+    if (!get_vec(f, v, n)) return fail(KNP_ERR_INVALID_ARGUMENT, "network file: truncated population");
+    return add(net, v.data(), n, nullptr);
+}
+
+}  // namespace
+
+extern "C" {
+
+int knp_mi355x_save_network(knp_net *net, const char *path) {
+    if (!net || !path) return fail(KNP_ERR_INVALID_ARGUMENT, "null argument");
+    int rc;
+    if ((rc = ensure_init(net))) return rc;
+    FileCloser fc{std::fopen(path, "wb")};
+    FILE *f = fc.f;
+    if (!f) return fail(KNP_ERR_INVALID_ARGUMENT, std::string("network file: cannot open ") + path);
+    const uint32_t version = 1, n_pops = (uint32_t)net->pops.size(), n_projs = (uint32_t)net->projs.size();
+    if (!put(f, kNetMagic, 8) || !put_v(f, version) || !put_v(f, n_pops) || !put_v(f, n_projs))
+        return fail(KNP_ERR_INVALID_ARGUMENT, "network file: write failed");
+    for (uint32_t p = 0; p < n_pops; ++p) {
+        const HostPop &H = net->pops[p];
+        if (H.model == MODEL_LIF) rc = save_pop_structs(net, f, p, NF_LIF, knp_mi355x_read_lif_neurons);
+        else if (H.model == MODEL_ALTAI && H.resource)
+            rc = save_pop_structs(net, f, p, NF_RES_ALTAI, knp_mi355x_read_resource_altai_lif_neurons);
+        else if (H.model == MODEL_ALTAI) rc = save_pop_structs(net, f, p, NF_ALTAI, knp_mi355x_read_altai_lif_neurons);
+        else if (H.resource) rc = save_pop_structs(net, f, p, NF_RES_BLIFAT, knp_mi355x_read_resource_neurons);
+        else rc = save_pop_structs(net, f, p, NF_BLIFAT, knp_mi355x_read_blifat_neurons);
+        if (rc) return rc;
+    }
+    for (uint32_t q = 0; q < n_projs; ++q) {
+        const HostProj &H = net->projs[q];
+        std::vector<float> w(H.n_syn);
+        if ((rc = knp_mi355x_read_weights(net, q, w.data(), H.n_syn))) return rc;
+        const uint32_t locked = H.locked ? 1 : 0;
+        bool ok = put_v(f, H.kind) && put_v(f, H.pre_pop) && put_v(f, H.n_pre) && put_v(f, H.post_pop) &&
+                  put_v(f, H.n_syn) && put_v(f, locked) && put_vec(f, H.pre) && put_vec(f, H.post) && put_vec(f, w) &&
+                  put_vec(f, H.delay) && put_vec(f, H.type);
+        if (ok && H.kind == PROJ_RESOURCE) {
+            std::vector<knp_resource_rule> rules(H.n_syn);
+            if ((rc = knp_mi355x_read_resource_rules(net, q, rules.data(), H.n_syn))) return rc;
+            const uint32_t size = sizeof(knp_resource_rule);
+            ok = put_v(f, size) && put_vec(f, rules);
+        }
+        if (ok && H.kind == PROJ_ADDITIVE) {
+            const uint32_t n_stdp = (uint32_t)H.stdp_pops.size();
+            ok = put_vec(f, H.tau_p) && put_vec(f, H.tau_m) && put_v(f, n_stdp) && put_vec(f, H.stdp_pops) &&
+                 put_vec(f, H.stdp_modes);
+        }
+        if (!ok) return fail(KNP_ERR_INVALID_ARGUMENT, "network file: write failed");
+    }
+    if (std::fflush(f) != 0) return fail(KNP_ERR_INVALID_ARGUMENT, "network file: write failed");
+    return KNP_OK;
+}
+
+int knp_mi355x_load_network(knp_net *net, const char *path) {
+    if (!net || !path) return fail(KNP_ERR_INVALID_ARGUMENT, "null argument");
+    if (net->initialized || !net->pops.empty() || !net->projs.empty())
+        return fail(KNP_ERR_STATE, "network file: load into an empty handle");
+    FileCloser fc{std::fopen(path, "rb")};
+    FILE *f = fc.f;
+    if (!f) return fail(KNP_ERR_INVALID_ARGUMENT, std::string("network file: cannot open ") + path);
+    char magic[8];
+    uint32_t version = 0, n_pops = 0, n_projs = 0;
+    if (!get(f, magic, 8) || std::memcmp(magic, kNetMagic, 8) != 0 || !get_v(f, version) || version != 1 ||
+        !get_v(f, n_pops) || !get_v(f, n_projs))
+        return fail(KNP_ERR_INVALID_ARGUMENT, "network file: not a version-1 network file");
+    int rc;
+    for (uint32_t p = 0; p < n_pops; ++p) {
+        uint32_t tag = 0, n = 0, size = 0;
+        if (!get_v(f, tag) || !get_v(f, n) || !get_v(f, size))
+            return fail(KNP_ERR_INVALID_ARGUMENT, "network file: truncated population");
+        switch (tag) {
+            case NF_BLIFAT: rc = load_pop_structs(net, f, n, size, knp_mi355x_add_blifat_population); break;
+            case NF_RES_BLIFAT: rc = load_pop_structs(net, f, n, size, knp_mi355x_add_resource_blifat_population); break;
+            case NF_LIF: rc = load_pop_structs(net, f, n, size, knp_mi355x_add_lif_population); break;
+            case NF_ALTAI: rc = load_pop_structs(net, f, n, size, knp_mi355x_add_altai_lif_population); break;
+            case NF_RES_ALTAI:
+                rc = load_pop_structs(net, f, n, size, knp_mi355x_add_resource_altai_lif_population);
+                break;
+            default: return fail(KNP_ERR_INVALID_ARGUMENT, "network file: unknown neuron model");
+        }
+        if (rc) return rc;
+    }
+    for (uint32_t q = 0; q < n_projs; ++q) {
+        uint32_t kind = 0, pre_pop = 0, n_pre = 0, post_pop = 0, locked = 0;
+        uint64_t n = 0;
+        if (!get_v(f, kind) || !get_v(f, pre_pop) || !get_v(f, n_pre) || !get_v(f, post_pop) || !get_v(f, n) ||
+            !get_v(f, locked) || n > (uint64_t(1) << 40))
+            return fail(KNP_ERR_INVALID_ARGUMENT, "network file: truncated projection");
+        std::vector<uint32_t> pre, post, delay;
+        std::vector<float> w;
+        std::vector<uint8_t> type;
+        if (!get_vec(f, pre, n) || !get_vec(f, post, n) || !get_vec(f, w, n) || !get_vec(f, delay, n) ||
+            !get_vec(f, type, n))
+            return fail(KNP_ERR_INVALID_ARGUMENT, "network file: truncated projection");
+        if (kind == PROJ_DELTA) {
+            rc = knp_mi355x_add_delta_projection(net, pre_pop, n_pre, post_pop, pre.data(), post.data(), w.data(),
+                                                 delay.data(), type.data(), n, nullptr);
+        } else if (kind == PROJ_RESOURCE) {
+            uint32_t size = 0;
+            std::vector<knp_resource_rule> rules;
+            if (!get_v(f, size) || size != sizeof(knp_resource_rule) || !get_vec(f, rules, n))
+                return fail(KNP_ERR_INVALID_ARGUMENT, "network file: bad synaptic-resource rules");
+            rc = knp_mi355x_add_resource_stdp_projection(net, pre_pop, n_pre, post_pop, pre.data(), post.data(),
+                                                         w.data(), delay.data(), type.data(), rules.data(), n, nullptr);
+        } else if (kind == PROJ_ADDITIVE) {
+            std::vector<float> tp, tm;
+            std::vector<uint32_t> sp;
+            std::vector<uint8_t> sm;
+            uint32_t n_stdp = 0;
+            if (!get_vec(f, tp, n) || !get_vec(f, tm, n) || !get_v(f, n_stdp) || n_stdp > 4 || !get_vec(f, sp, n_stdp) ||
+                !get_vec(f, sm, n_stdp))
+                return fail(KNP_ERR_INVALID_ARGUMENT, "network file: bad additive STDP parameters");
+            rc = knp_mi355x_add_additive_stdp_projection(net, pre_pop, n_pre, post_pop, pre.data(), post.data(),
+                                                         w.data(), delay.data(), type.data(), tp.data(), tm.data(), n,
+                                                         sp.data(), sm.data(), n_stdp, nullptr);
+        } else {
+            return fail(KNP_ERR_INVALID_ARGUMENT, "network file: unknown projection kind");
+        }
+        if (rc) return rc;
+        if ((rc = knp_mi355x_set_locked(net, q, (int)locked))) return rc;
+    }
+    return KNP_OK;
+}
+
+}  // extern "C"

@@ -13,6 +13,8 @@ from knp_amd import abi, workloads
 from knp_amd.backend import BackendError, MI355XBackend, device_count
 from oracle.pyoracle import Oracle
 
+import netfile
+
 pytestmark = pytest.mark.gpu
 
 POT_RTOL = 1e-6
@@ -773,6 +775,63 @@ def test_altai_lif_population_matches_oracle(dopamine):
             g.add_delta_projection(abi.KNP_NO_POPULATION, 4, ap, [0], [1], [2.0], [1], [abi.BLOCKING])
         with pytest.raises(BackendError, match="must be a float"):
             g.add_altai_lif_population(abi.neuron_array(abi.altai_lif_defaults(is_diff=1, additional_threshold=0.1), 2))
+
+
+def test_network_file_round_trip(tmp_path):
+    """save_network / load_network (network_io.h:41,51): the file holds the state the device reports, and a
+    network loaded from it evolves on the device exactly as the oracle built from the same file does."""
+    say("case: network file round trip (LIF + BLIFAT; trained resource AltAI-LIF)")
+    g = MI355XBackend(0)
+    pops, push = lif_network(g, 21)
+    rng = np.random.default_rng(3)
+    for _ in range(5):
+        g.push_spikes(push, np.nonzero(rng.random(1500) < 0.08)[0].astype(np.uint32))
+        g.step(6)
+    path = str(tmp_path / "lif.knpnet")
+    g.save_network(path)
+    fpops, fprojs = netfile.parse(path)
+    assert bytes(fpops[0][1]) == bytes(g.lif_neurons(pops[0]))
+    assert bytes(fpops[1][1]) == bytes(g.blifat_neurons(pops[1]))
+    for q, p in enumerate(fprojs):
+        assert np.array_equal(p["weight"], g.weights(q)), q
+    g2, o = MI355XBackend(0), Oracle("exact")
+    g2.load_network(path)
+    netfile.load_into(o, path)
+    assert bytes(g2.lif_neurons(0)) == bytes(fpops[0][1])
+    steps = 40
+    for x in (g2, o):
+        x.record_spikes(0, steps)
+        x.record_spikes(1, steps)
+    for _ in range(steps):
+        idx = np.nonzero(rng.random(1500) < 0.08)[0].astype(np.uint32)
+        for x in (g2, o):
+            x.push_spikes(0, idx)
+            x.step()
+    for k in range(2):
+        sg, so = g2.read_spike_record(k), o.read_spike_record(k)
+        assert so.sum() > 50 and np.array_equal(sg, so), k
+    assert bytes(g2.lif_neurons(0)) == bytes(o.lif_neurons(0))
+
+    # a trained synaptic-resource network: rules, weights and neuron state survive the file
+    g = MI355XBackend(0)
+    pop, inp, dop = resource_altai_network(g, 5)
+    for _ in range(6):
+        g.step(9)
+        g.push_spikes(dop, [0])
+        g.step(1)
+    path = str(tmp_path / "res.knpnet")
+    g.save_network(path)
+    fpops, fprojs = netfile.parse(path)
+    assert bytes(fpops[0][1]) == bytes(g.resource_altai_lif_neurons(pop))
+    assert bytes(fprojs[inp]["rules"]) == bytes(g.resource_rules(inp))
+    assert np.any(fprojs[inp]["weight"] != 0.5) and fprojs[inp]["locked"] == 0
+    g3 = MI355XBackend(0)
+    g3.load_network(path)
+    assert bytes(g3.resource_altai_lif_neurons(0)) == bytes(fpops[0][1])
+    assert bytes(g3.resource_rules(inp)) == bytes(fprojs[inp]["rules"])
+    assert np.array_equal(g3.weights(inp), fprojs[inp]["weight"])
+    with pytest.raises(BackendError, match="empty handle"):
+        g3.load_network(path)
 
 
 def test_batched_instances_match_standalone():
