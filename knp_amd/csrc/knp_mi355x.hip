@@ -107,6 +107,7 @@ struct knp_net {
     int scatter_mode = KNP_SCATTER_AUTO;
     bool fused = false;     // every population tiled: phase 1 of step t + 1 runs inside the tile kernel of t
     bool persistent = false;  // fused, and every tile fits on the device at once (cooperative launch)
+    bool wide_tiles = false;  // tiles outnumber the workgroups resident at once: the 6-waves tile kernel
     uint32_t test_hold_block = 0xFFFFFFFFu, test_spin_limit = kSpinLimit;  // knp_mi355x_set_wait_test
     uint32_t run_kind = POP_LEAN;  // population kind of the persistent launch
     uint32_t tile_lds = 0;  // dynamic LDS bytes of the tile scatter (largest tiled ring)
@@ -702,7 +703,7 @@ int init_network(knp_net *net) {
         int lds_limit = 0;
         HIP_TRY(hipDeviceGetAttribute(&lds_limit, hipDeviceAttributeMaxSharedMemoryPerBlock, net->device));
         hipFuncAttributes fa{};
-        HIP_TRY(hipFuncGetAttributes(&fa, reinterpret_cast<const void *>(&k_tile_scatter<true>)));
+        HIP_TRY(hipFuncGetAttributes(&fa, reinterpret_cast<const void *>(&k_tile_scatter<true, false>)));
         const int64_t room = (int64_t)lds_limit - (int64_t)fa.sharedSizeBytes - (int64_t)tile_lds_bytes(0, kTileMaxList);
         tile_max_d = (uint32_t)std::max<int64_t>(0, std::min<int64_t>(kTileMaxD, room / (kBlock * 8)));
     }
@@ -1191,6 +1192,14 @@ int init_network(knp_net *net) {
         const bool src_fit = T.n_src_blocks <= (uint64_t)kCachedSrcBlocks * T.n_tiles;
         net->persistent = coop && src_fit && (uint64_t)per_cu * cus >= T.n_tiles;
     }
+    net->wide_tiles = false;
+    if (T.n_tiles) {
+        int cus = 0, per_cu = 0;
+        HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, net->device));
+        HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+            &per_cu, reinterpret_cast<const void *>(&k_tile_scatter<true, false>), kTileThreads, net->tile_lds));
+        net->wide_tiles = (uint64_t)per_cu * cus < T.n_tiles;
+    }
     {
         int rc;
         if ((rc = dev_alloc(net, (void **)&T.grid_bar, kBarWords * 4))) return rc;
@@ -1422,10 +1431,11 @@ int run_steps(knp_net *net, uint64_t n_steps) {
         if (p1 && (rc = launch_dense_due(t + 1))) return rc;
         const uint32_t nt = net->table.n_tiles;
         if (nt && (rc = timed_launch(net, KNP_KERNEL_TILE_SCATTER, [&] {
-                       if (p1)
-                           k_tile_scatter<true><<<nt, kTileThreads, net->tile_lds, net->stream>>>(D, t);
-                       else
-                           k_tile_scatter<false><<<nt, kTileThreads, net->tile_lds, net->stream>>>(D, t);
+                       const uint32_t lds = net->tile_lds;
+                       if (p1 && net->wide_tiles) k_tile_scatter<true, true><<<nt, kTileThreads, lds, net->stream>>>(D, t);
+                       else if (p1) k_tile_scatter<true, false><<<nt, kTileThreads, lds, net->stream>>>(D, t);
+                       else if (net->wide_tiles) k_tile_scatter<false, true><<<nt, kTileThreads, lds, net->stream>>>(D, t);
+                       else k_tile_scatter<false, false><<<nt, kTileThreads, lds, net->stream>>>(D, t);
                    })))
             return rc;
         phase1_done = p1;

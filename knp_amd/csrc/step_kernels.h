@@ -1167,9 +1167,13 @@ __device__ __forceinline__ void tile_flush_counts(const NetDev *__restrict__ net
 // One step's tile scatter (the ring in HBM, this step's sums in LDS). PHASE1: the same launch also runs
 // phase 1 of step t + 1 for the tile's neurons (their ring slot for t + 1 is complete once this step's
 // impacts are summed) and the sources of t + 1; the host uses it when every population is tiled, saving
-// one launch per step.
-template <bool PHASE1>
-__global__ __launch_bounds__(kTileThreads) void k_tile_scatter(const NetDev *__restrict__ net, uint64_t t) {
+// one launch per step. WIDE: at most 80 VGPRs (6 waves per SIMD), so three 512-thread workgroups share a CU
+// instead of two, at the cost of a few spilled registers; the host picks it when the tiles do not all fit
+// on the device at once (many-instance networks: 129 -> 126 us a step; with one round of tiles it is
+// slower).
+template <bool PHASE1, bool WIDE>
+__global__ __launch_bounds__(kTileThreads, WIDE ? 6 : 1) void k_tile_scatter(const NetDev *__restrict__ net,
+                                                                              uint64_t t) {
     extern __shared__ unsigned long long tacc[];
     __shared__ TileShared sh;
     if (blockIdx.x == 0) clear_push_counts<false>(net, t + 1);
