@@ -332,7 +332,8 @@ int knp_mi355x_read_resource_rules(knp_net *net, uint32_t proj, knp_resource_rul
  * (kind, presynaptic / postsynaptic populations, synapses with current weights, synaptic-resource rules
  * or additive-STDP time constants, learning lock), in load order. As in the reference, impacts still in
  * flight, spike records, channel sources and dense input projections are not part of the network file.
- * load_network adds the file's populations and projections to a handle that has none yet. */
+ * load_network adds the file's populations and projections to a handle that has none yet; when it fails,
+ * the handle may hold part of the file's network and should be destroyed. */
 int knp_mi355x_save_network(knp_net *net, const char *path);
 int knp_mi355x_load_network(knp_net *net, const char *path);
 /* Number of SynapticImpacts created so far (sum over steps of spikes x out-degree over all projections). */
