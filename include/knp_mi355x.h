@@ -222,7 +222,7 @@ int knp_mi355x_add_resource_blifat_population(
     knp_net *net, const knp_resource_blifat_neuron *neurons, uint32_t n, uint32_t *out_pop);
 /* Population<LIFNeuron> (step: lif_impl.h:31-84). It accepts EXCITATORY and INHIBITORY_CURRENT impacts only:
  * the reference throws "Unhandled synapse type." when another type reaches it (lif_impl.h:62-64); here the
- * projection carrying it is rejected, with the same message, when the network is built. */
+ * projection carrying it is rejected, with the same message, when it is added. */
 int knp_mi355x_add_lif_population(knp_net *net, const knp_lif_neuron *neurons, uint32_t n, uint32_t *out_pop);
 /* Population<AltAILIF> (step: altai_impl.h:31-150). Accepts EXCITATORY, INHIBITORY_CURRENT and DOPAMINE
  * impacts. BLOCKING impacts (altai_impl.h:72-79: a fold over the step's impacts in delivery order whose
