@@ -1343,10 +1343,14 @@ __device__ __forceinline__ long long wave_sum_i64(long long v) {
 // The reference updates free_synaptic_resource_ once per synapse in synapse order; the float result
 // depends on that order, so the wave replays it lane by lane: free += d_u of lane l; if lane l contributes:
 // free += delta. du_lane is each lane's d_u, read across lanes with readlane (no memory round trip); every
-// lane runs the same additions in the same order, so all lanes end with the same value.
-__device__ __forceinline__ float serial_free(float freer, uint64_t active, uint64_t contrib, float du_lane,
-                                             float delta, bool use_du) {
-    uint64_t m = use_du ? active : contrib;
+// lane runs the same additions in the same order, so all lanes end with the same value. Adding a zero d_u
+// changes nothing unless free is -0 (-0 + +0 = +0), and a non-negative-zero free never becomes -0 in the
+// replay, so lanes with a zero d_u (not in `nonzero`) that do not contribute are skipped unless free starts
+// as -0.
+__device__ __forceinline__ float serial_free(float freer, uint64_t active, uint64_t nonzero, uint64_t contrib,
+                                             float du_lane, float delta, bool use_du) {
+    const bool neg_zero = freer == 0.f && signbit(freer);
+    uint64_t m = use_du ? (neg_zero ? active : (nonzero | contrib)) : contrib;
     while (m) {
         const int l = __ffsll((unsigned long long)m) - 1;
         m &= m - 1;
@@ -1436,7 +1440,7 @@ __global__ __launch_bounds__(kBlock) void k_train(const NetDev *__restrict__ net
                             wmax[u] = Q.w_max[sn];
                         }
                     }
-                    uint64_t am[kTrainChunks], cm[kTrainChunks];
+                    uint64_t am[kTrainChunks], nz[kTrainChunks], cm[kTrainChunks];
 #pragma unroll
                     for (uint32_t u = 0; u < kTrainChunks; ++u) {
                         const uint64_t k = base + 64 * u + lane;
@@ -1464,12 +1468,13 @@ __global__ __launch_bounds__(kBlock) void k_train(const NetDev *__restrict__ net
                             du[u] = 0.f;
                         }
                         am[u] = __ballot(act);
+                        nz[u] = __ballot(act && du[u] != 0.f);
                         cm[u] = __ballot(contributing);
                     }
                     if (status != 0) {
 #pragma unroll
                         for (uint32_t u = 0; u < kTrainChunks; ++u)
-                            freer = serial_free(freer, am[u], cm[u], du[u], -d_h, true);
+                            freer = serial_free(freer, am[u], nz[u], cm[u], du[u], -d_h, true);
                     }
                 }
             }
@@ -1516,7 +1521,7 @@ __global__ __launch_bounds__(kBlock) void k_train(const NetDev *__restrict__ net
                         cm[u] = __ballot(contributing);
                     }
 #pragma unroll
-                    for (uint32_t u = 0; u < kTrainChunks; ++u) freer = serial_free(freer, cm[u], cm[u], 0.f, -rc, false);
+                    for (uint32_t u = 0; u < kTrainChunks; ++u) freer = serial_free(freer, cm[u], 0, cm[u], 0.f, -rc, false);
                 }
             }
             if (forced || dop < 0.f) {
