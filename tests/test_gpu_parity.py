@@ -122,6 +122,31 @@ def test_resource_stdp_network(model):
     assert g.weights(gl)[0] != 1.0
 
 
+def test_resource_free_negative_zero():
+    """The free-resource replay skips zero-d_u additions except when free is -0 (-0 + +0 = +0 in the
+    reference's float order): a neuron loaded with free = -0.0 and d_u = 0 must follow the oracle bit for bit."""
+    import struct
+    say("case: free synaptic resource starting at -0.0 with d_u = 0")
+    runs = []
+    for o in (MI355XBackend(0), Oracle("exact")):
+        cell = abi.resource_blifat_defaults(free_synaptic_resource=-0.0, isi_max=3, d_h=0.25)
+        pop = o.add_resource_blifat_population(abi.neuron_array(cell, 1))
+        rules = abi.rule_array(abi.resource_rule_defaults(synaptic_resource=0.0, w_min=1.0, w_max=2.0, d_u=0.0,
+                                                          dopamine_plasticity_period=2), 3)
+        inp = o.add_resource_stdp_projection(abi.KNP_NO_POPULATION, 3, pop, [0, 1, 2], [0, 0, 0], [1.0] * 3, [1] * 3,
+                                             rules)
+        o.start_learning()
+        bits = []
+        for t in range(12):
+            if t % 4 == 0:
+                o.push_spikes(inp, [0, 2])
+            o.step()
+            bits.append(struct.pack("<f", o.resource_neurons(pop)[0].free_synaptic_resource))
+        runs.append(bits)
+    assert runs[0] == runs[1]
+    assert runs[1][0] == struct.pack("<f", -0.0) and runs[1][-1] != struct.pack("<f", -0.0)
+
+
 def resource_altai_network(o, seed):
     """Synaptic-resource AltAI-LIF population (256 neurons, leak -1, threshold 4) trained by a plastic
     channel projection (64 channels, Bernoulli source), with lateral inhibition and a dopamine channel."""
