@@ -127,7 +127,11 @@ struct ProjDev {
     // synaptic-resource rule state, synapse-index order
     float *r, *w_min, *w_max, *d_u;
     uint32_t *period;
-    uint64_t *last_spike_step;
+    // last_spike_step_ (init_synapse, synaptic_resource_stdp.h:36-62, gives every synapse of a spiking row
+    // the step): the row's last spike step once the row has spiked since load, else the loaded value
+    uint64_t *last_spike_step;   // [n_syn] loaded values, synapse-index order
+    uint64_t *row_spike;         // [n_pre] last step each presynaptic row spiked (UINT64_MAX: not since load)
+    const uint32_t *col_pre;     // [n_syn] CSC order: presynaptic neuron of col_syn[k]
     uint8_t *flags;              // bit0 had_hebbian_update_, bit1 has_contributed_
     const uint8_t *delay_syn;    // [n_syn] delay in synapse-index order
     const uint64_t *col_ptr;     // [n_post + 1] CSC by postsynaptic neuron

@@ -912,6 +912,11 @@ int init_network(knp_net *net) {
             uint8_t *dsd;
             if ((rc = upload(net, &dsd, ds.data(), S))) return rc;
             Q.delay_syn = dsd;
+            std::vector<uint32_t> cpre(S);
+            for (uint64_t k = 0; k < S; ++k) cpre[k] = H.pre[col_syn[k]];
+            uint32_t *cpd;
+            if ((rc = upload(net, &cpd, cpre.data(), S))) return rc;
+            Q.col_pre = cpd;
         }
         if (H.kind == PROJ_RESOURCE) {
             std::vector<float> f(S);
@@ -930,6 +935,8 @@ int init_network(knp_net *net) {
             if ((rc = upload(net, &Q.period, u.data(), S))) return rc;
             for (uint64_t s = 0; s < S; ++s) u64[s] = H.rules[s].last_spike_step;
             if ((rc = upload(net, &Q.last_spike_step, u64.data(), S))) return rc;
+            std::vector<uint64_t> never((size_t)std::max(1u, H.n_pre), UINT64_MAX);
+            if ((rc = upload(net, &Q.row_spike, never.data(), never.size()))) return rc;
             for (uint64_t s = 0; s < S; ++s)
                 fl[s] = (uint8_t)((H.rules[s].had_hebbian_update ? 1 : 0) | (H.rules[s].has_contributed ? 2 : 0));
             if ((rc = upload(net, &Q.flags, fl.data(), S))) return rc;
@@ -2243,6 +2250,10 @@ int knp_mi355x_read_resource_rules(knp_net *net, uint32_t proj, knp_resource_rul
     if ((rc = download(net, du.data(), Q.d_u, n_syn))) return rc;
     if ((rc = download(net, per.data(), Q.period, n_syn))) return rc;
     if ((rc = download(net, lss.data(), Q.last_spike_step, n_syn))) return rc;
+    std::vector<uint64_t> row_spike(std::max(1u, H.n_pre));
+    if ((rc = download(net, row_spike.data(), Q.row_spike, row_spike.size()))) return rc;
+    for (uint64_t s = 0; s < n_syn; ++s)
+        if (row_spike[H.pre[s]] != UINT64_MAX) lss[s] = row_spike[H.pre[s]];
     if ((rc = download(net, fl.data(), Q.flags, n_syn))) return rc;
     for (uint64_t s = 0; s < n_syn; ++s) {
         out[s].synaptic_resource = r[s];

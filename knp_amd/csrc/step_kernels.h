@@ -570,6 +570,7 @@ __global__ __launch_bounds__(kBlock) void k_scatter(const NetDev *__restrict__ n
         my_rows += 1;
         const PopDev &P = net->pops[Q.post_pop];
         const uint32_t slot0 = (uint32_t)(t % P.D);
+        if (Q.kind == PROJ_RESOURCE && threadIdx.x == 0) Q.row_spike[pre] = t;  // init_synapse of the row
         for (uint64_t base = b; base < e; base += 4 * kBlock) {
             uint32_t pt[4], d[4];
             float wv[4];
@@ -585,7 +586,7 @@ __global__ __launch_bounds__(kBlock) void k_scatter(const NetDev *__restrict__ n
                     } else {
                         const uint32_t s = Q.csr_syn[k];
                         wv[j] = Q.W[s];
-                        if (Q.kind == PROJ_RESOURCE) Q.last_spike_step[s] = t;  // init_synapse
+
                     }
                 }
             }
@@ -1433,7 +1434,8 @@ __global__ __launch_bounds__(kBlock) void k_train(const NetDev *__restrict__ net
                             w[u] = Q.W[sn];
                             fl[u] = Q.flags[sn];
                             per[u] = Q.period[sn];
-                            spk[u] = Q.last_spike_step[sn] + Q.delay_syn[sn] - 1;
+                            const uint64_t rs = Q.row_spike[Q.col_pre[k]];
+                            spk[u] = (rs != UINT64_MAX ? rs : Q.last_spike_step[sn]) + Q.delay_syn[sn] - 1;
                             r[u] = Q.r[sn];
                             du[u] = status != 0 ? Q.d_u[sn] : 0.f;
                             wmin[u] = Q.w_min[sn];
