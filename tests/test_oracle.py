@@ -15,7 +15,9 @@ Checks, in order:
     random LIF network against an independent numpy simulation of the same rules;
   * the reference's AltAI-LIF known answers (knp/tests/framework/altai_lif_test.cpp:103-196: reversed and
     plain leak, saturation, spike generation, single-neuron impacts) plus hand traces of is_diff, negative
-    resets, rounding, do_not_save and activity time (altai_impl.h:31-150);
+    resets, rounding, do_not_save and activity time (altai_impl.h:31-150), a random AltAI-LIF network
+    against an independent numpy simulation, and ResourceSTDPAltAILIFNetwork
+    (single_thread_cpu_test.cpp:267-358);
   * the Bernoulli channel hash against an independent numpy implementation.
 """
 import numpy as np
@@ -595,3 +597,79 @@ def test_altai_rejects_conductance_and_blocking(oracle_built):
     for ty in (abi.INHIBITORY_CONDUCTANCE, abi.BLOCKING):
         with pytest.raises(RuntimeError, match="Unhandled synapse type"):
             o.add_delta_projection(abi.KNP_NO_POPULATION, 1, pop, [0], [1], [0.5], [1], [ty])
+
+
+def altai_numpy(n, pre, post, w, d, otype, cells, inputs, steps):
+    """Independent simulation of a recurrent AltAI-LIF network with integer weights (altai_impl.h:31-150):
+    per step round (or reset) the potential, add the weights of the synapses whose presynaptic neuron spiked
+    `delay` steps earlier (channel inputs one step earlier), then leak, count the activity time down, fire
+    and reset."""
+    g = lambda f: np.array([getattr(c, f) for c in cells], dtype=np.float64)
+    V = g("potential")
+    act = g("activation_threshold") + g("additional_threshold")
+    neg = g("negative_activation_threshold")
+    leak = g("potential_leak")
+    reset = g("potential_reset_value")
+    flags = {f: g(f).astype(bool) for f in ("is_diff", "is_reset", "leak_rev", "saturate", "do_not_save")}
+    at = np.array([c.activity_time for c in cells], dtype=np.int64)
+    sign = np.where(otype == abi.INHIBITORY_CURRENT, -1.0, 1.0)
+    hist, out = [], np.zeros((steps, n), dtype=bool)
+    for t in range(steps):
+        V = np.where(flags["do_not_save"], reset, np.sign(V) * np.floor(np.abs(V) + 0.5))
+        s = np.zeros(n)
+        if t >= 1:
+            np.add.at(s, inputs[t - 1], 4.0)
+        for k in range(len(pre)):
+            if t - d[k] >= 0 and hist[t - d[k]][pre[k]]:
+                s[post[k]] += sign[k] * w[k]
+        V = V + s
+        V = V + np.where(flags["leak_rev"] & (V < 0), -leak, leak)
+        at = np.where(at > 0, at - 1, np.where(at < 0, at + 1, at))
+        at = np.where(at == 0, np.iinfo(np.int64).max, at)
+        above = V >= act
+        fire = above & (at > 0)
+        V = np.where(above & flags["is_diff"], V - act, V)
+        was_reset = above & flags["is_reset"]
+        V = np.where(was_reset, reset, V)
+        low = (V <= -neg) & ~was_reset
+        V = np.where(low & flags["saturate"], -neg,
+                     np.where(low & flags["is_reset"], -reset, np.where(low & flags["is_diff"], V + neg, V)))
+        hist.append(fire)
+        out[t] = fire
+    return out, V, at
+
+
+def test_altai_network_matches_numpy(oracle_built):
+    rng = np.random.default_rng(17)
+    n, S, steps = 40, 300, 60
+    cells = abi.neuron_array(abi.altai_lif_defaults(), n)
+    for c in cells:
+        c.activation_threshold = int(rng.integers(3, 9))
+        c.negative_activation_threshold = int(rng.integers(4, 12))
+        c.potential_leak = int(rng.integers(-1, 2))
+        c.potential_reset_value = int(rng.integers(0, 2))
+        c.is_diff, c.is_reset, c.leak_rev, c.saturate = (int(b) for b in rng.integers(0, 2, 4))
+        c.activity_time = int(rng.choice([0, 0, 3, -2]))
+    pre = rng.integers(0, n, S)
+    post = rng.integers(0, n, S)
+    w = rng.integers(1, 5, S).astype(np.float64)
+    d = rng.integers(1, 5, S)
+    otype = rng.choice([abi.EXCITATORY, abi.INHIBITORY_CURRENT], S, p=[0.65, 0.35]).astype(np.uint8)
+    inputs = [np.nonzero(rng.random(n) < 0.2)[0] for _ in range(steps)]
+    o = Oracle("exact")
+    pop = o.add_altai_lif_population(cells)
+    ident = np.arange(n, dtype=np.uint32)
+    inp = o.add_delta_projection(abi.KNP_NO_POPULATION, n, pop, ident, ident, np.full(n, 4.0, np.float32),
+                                 np.ones(n, np.uint32))
+    o.add_delta_projection(pop, n, pop, pre, post, w, d, otype)
+    o.record_spikes(pop, steps)
+    for t in range(steps):
+        if len(inputs[t]):
+            o.push_spikes(inp, inputs[t].astype(np.uint32))
+        o.step()
+    ref, V, at = altai_numpy(n, pre, post, w, d, otype, cells, inputs, steps)
+    assert ref.sum() > 50
+    assert np.array_equal(o.read_spike_record(pop), ref)
+    x = o.altai_lif_neurons(pop)
+    assert np.array_equal([c.potential for c in x], V)
+    assert np.array_equal([c.activity_time for c in x], at)
