@@ -1362,10 +1362,9 @@ __device__ __forceinline__ float serial_free(float freer, uint64_t active, uint6
 }
 
 // 64-synapse chunks of one neuron whose loads k_train issues before their stores. Larger groups keep more
-// loads in flight per wave but cost registers (97 VGPRs at 2, 129 at 4): measured over training steps 100-400
-// of one instance and of 256 batched instances, one chunk (76 VGPRs) is fastest in both (train 42.6 / 95 us
-// a step against 45.0 / 124 us at 4), because the batch needs the waves more than the single instance
-// needs the loads.
+// loads in flight per wave but cost registers (97 VGPRs at 2, 129 at 4, 76 at 1); measured on one instance
+// and on 256 batched instances, one chunk was the fastest in both: the batch needs the waves more than the
+// single instance needs the loads.
 constexpr uint32_t kTrainChunks = 1;
 
 __global__ __launch_bounds__(kBlock) void k_train(const NetDev *__restrict__ net, uint32_t p, uint64_t t) {
