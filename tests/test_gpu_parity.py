@@ -858,6 +858,27 @@ def test_network_file_round_trip(tmp_path):
     with pytest.raises(BackendError, match="empty handle"):
         g3.load_network(path)
 
+    # additive STDP: time constants, STDP populations and processing types survive the file, and the loaded
+    # network (spike-time histories start empty, as after a fresh load in the reference) matches the oracle
+    g = MI355XBackend(0)
+    pop = g.add_blifat_population(abi.neuron_array(abi.blifat_defaults(), 1))
+    inp = g.add_additive_stdp_projection(abi.KNP_NO_POPULATION, 1, pop, [0], [0], [1.0], [1], [2.0], [2.0])
+    loop = g.add_additive_stdp_projection(pop, 1, pop, [0], [0], [1.0], [6], [1.0], [1.0],
+                                          stdp_pops=[pop], stdp_modes=[abi.STDP_AND_SPIKE])
+    run_smallest(g, pop, inp)
+    assert g.weights(loop)[0] != 1.0
+    path = str(tmp_path / "add.knpnet")
+    g.save_network(path)
+    fpops, fprojs = netfile.parse(path)
+    assert fprojs[loop]["kind"] == abi.PROJ_ADDITIVE and list(fprojs[loop]["stdp_pops"]) == [pop]
+    assert list(fprojs[loop]["stdp_modes"]) == [abi.STDP_AND_SPIKE] and fprojs[inp]["tau_plus"][0] == 2.0
+    assert fprojs[loop]["weight"][0] == g.weights(loop)[0]
+    g4, o = MI355XBackend(0), Oracle("exact")
+    g4.load_network(path)
+    netfile.load_into(o, path)
+    assert run_smallest(g4, 0, inp) == run_smallest(o, 0, inp)
+    assert g4.weights(loop)[0] == pytest.approx(o.weights(loop)[0], rel=1e-6)
+
 
 def test_batched_instances_match_standalone():
     """Instances composed into one device network (config 3 layout, small batch) evolve exactly as the same
