@@ -174,6 +174,7 @@ def main():
 
     # ---- CPU baseline: the oracle (restated single-threaded backend), bounded sample, 1 thread
     cpu = None
+    cpu_mt = None
     parity = {"steps_compared": 0}
     if not args.no_cpu_baseline:
         import numpy as np
@@ -216,6 +217,29 @@ def main():
         _ = ev_a
         log(f"cpu baseline {cpu['value']:.3e} events/s over {n_cpu} steps")
         ora.close()
+        # the multi-threaded backend's decomposition of the same step on the same sample (at most 16 threads:
+        # one GPU's share of the host); its spikes equal the single thread's
+        threads = max(1, min(16, os.cpu_count() or 1))
+        if threads > 1:
+            mt = spec.load_into(Oracle("exact", threads=threads))
+            mt.step(parity_steps + 10)
+            eb = mt.events
+            tm = time.perf_counter()
+            mt.step(n_cpu)
+            tmt = time.perf_counter() - tm
+            cpu_mt = {
+                "value": (mt.events - eb) / tmt,
+                "unit": "events/s",
+                "cores": threads,
+                "kind": "port",
+                "sample": f"the same {n_cpu} steps, {threads} threads: the multi-threaded backend's decomposition "
+                          f"(parallel neuron parts and projection chunks; cpu-multi-threaded-backend/impl/"
+                          f"backend.cpp:77-323) of the oracle, -O2 strict IEEE, OpenMP",
+                "net_steps_per_s": n_cpu / tmt,
+                "events_equal_single_thread": mt.events - eb == ev_c - ev_b,
+            }
+            mt.close()
+            log(f"cpu baseline ({threads} threads) {cpu_mt['value']:.3e} events/s")
 
     line = {
         "metric": "synaptic events/s",
@@ -246,6 +270,7 @@ def main():
         "kernels": {k: {kk: (round(vv, 3) if isinstance(vv, float) else vv) for kk, vv in v.items()}
                     for k, v in kern.items()},
         "cpu_baseline": cpu,
+        "cpu_baseline_mt": cpu_mt,
         "parity": parity,
         "device": device_name(0),
     }

@@ -16,6 +16,11 @@
  *   - projection STDP hooks                     impl/projections/training/stdp/{stdp_dispatcher.h,synaptic_resource_stdp.h,additive_stdp.h}
  *   - message delivery order of the CPU bus     knp/core-library/impl/message_bus_cpu_impl/message_bus_cpu_impl.cpp:41-88,
  *                                               knp/core-library/impl/message_endpoint.cpp:151-241
+ *   - the multi-threaded backend's decomposition knp/backends/cpu/cpu-multi-threaded-backend/impl/backend.cpp:77-323
+ *     (ko_set_threads, built with OpenMP): neuron pre / post states in parallel parts, impacts applied per
+ *     population in delivery order, projection impacts built from parallel chunks of the spike list. Thread
+ *     buffers are merged in chunk order, so impact order, sums and spike order equal the one-thread step
+ *     (the reference's MT merge order depends on thread timing; with exact sums its spike sets agree).
  *
  * Precision policy (compile-time): KO_WREAL is the type of every value the reference keeps in `float`
  * (weights, impacts, synaptic resources, the float fields of resource neurons).
@@ -32,6 +37,9 @@
  */
 #include <math.h>
 #include <stdint.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
@@ -166,6 +174,7 @@ typedef struct ko_net {
     uint64_t step;
     uint64_t events;
     int initialized;
+    int threads;  /* > 1: the multi-threaded step (OpenMP builds) */
     char err[256];
 } ko_net;
 
@@ -236,6 +245,13 @@ void ko_destroy(ko_net *net) {
 }
 
 const char *ko_last_error(ko_net *net) { return net->err; }
+
+/* Threads of the step (1: the single-threaded backend; more: the multi-threaded decomposition, effective in
+ * OpenMP builds). */
+int ko_set_threads(ko_net *net, int threads) {
+    net->threads = threads > 1 ? threads : 1;
+    return 0;
+}
 
 static int add_pop(ko_net *net, uint32_t n, int resource) {
     net->pops = (ko_pop *)realloc(net->pops, (net->n_pops + 1) * sizeof(ko_pop));
@@ -890,8 +906,62 @@ static void additive_modify_weights(ko_proj *p) {
     }
 }
 
+/* Impacts of the spikes [lo, hi) of the flattened message list into per-delay-slot buffers. */
+static uint64_t build_impacts(const ko_proj *p, const uint32_t *flat, size_t lo, size_t hi, ko_msg *slots,
+                              uint64_t step) {
+    uint64_t ev = 0;
+    for (size_t i = lo; i < hi; ++i) {
+        const uint32_t idx = flat[i];
+        if (idx >= p->n_pre) continue;
+        for (uint64_t j = p->row_ptr[idx]; j < p->row_ptr[idx + 1]; ++j) {
+            const uint64_t s = p->row_syn[j];
+            const uint64_t future_step = (uint64_t)p->delay[s] + step - 1;
+            const ko_impact imp = {s, p->w[s], p->type[s], p->pre[s], p->post[s]};
+            msg_push(&slots[future_step % p->max_delay], &imp);
+            ev++;
+        }
+    }
+    return ev;
+}
+
+/* The multi-threaded form of the spike loop below for plain Delta projections: contiguous chunks of the
+ * spike list build impacts into thread buffers, merged into the queue slots in chunk order. */
+static void calculate_projection_mt(ko_net *net, ko_proj *p, ko_spike_msg *msgs, uint32_t n_msgs, uint64_t step) {
+    size_t total = 0;
+    for (uint32_t m = 0; m < n_msgs; ++m) total += msgs[m].n;
+    uint32_t *flat = (uint32_t *)xcalloc(total ? total : 1, 4);
+    for (uint32_t m = 0, k = 0; m < n_msgs; ++m)
+        for (uint32_t i = 0; i < msgs[m].n; ++i) flat[k++] = msgs[m].idx[i];
+    const int T = net->threads;
+    ko_msg *buf = (ko_msg *)xcalloc((size_t)T * p->max_delay, sizeof(ko_msg));
+    uint64_t ev = 0;
+#pragma omp parallel num_threads(T) reduction(+ : ev)
+    {
+        int th = 0, nth = 1;
+#ifdef _OPENMP
+        th = omp_get_thread_num();
+        nth = omp_get_num_threads();
+#endif
+        const size_t lo = total * (size_t)th / (size_t)nth, hi = total * (size_t)(th + 1) / (size_t)nth;
+        ev += build_impacts(p, flat, lo, hi, buf + (size_t)th * p->max_delay, step);
+    }
+    for (uint32_t d = 0; d < p->max_delay; ++d)
+        for (int th = 0; th < T; ++th) {
+            ko_msg *b = &buf[(size_t)th * p->max_delay + d];
+            for (size_t k = 0; k < b->n; ++k) msg_push(&p->queue[d], &b->v[k]);
+            free(b->v);
+        }
+    free(buf);
+    free(flat);
+    net->events += ev;
+}
+
 static void calculate_projection(ko_net *net, ko_proj *p, ko_spike_msg *msgs, uint32_t n_msgs, uint64_t step) {
     if (p->kind == KO_PROJ_ADDITIVE) additive_init_projection(net, p, msgs, n_msgs);
+    if (net->threads > 1 && p->kind == KO_PROJ_DELTA) {
+        calculate_projection_mt(net, p, msgs, n_msgs, step);
+        n_msgs = 0;  /* impacts already built */
+    }
     for (uint32_t m = 0; m < n_msgs; ++m) {
         for (uint32_t i = 0; i < msgs[m].n; ++i) {
             uint32_t idx = msgs[m].idx[i];
@@ -923,6 +993,9 @@ static void calculate_projection(ko_net *net, ko_proj *p, ko_spike_msg *msgs, ui
 /* ---- the step (backend.cpp:89-141) ----------------------------------------------------------- */
 static void population_step(ko_net *net, uint32_t pop, uint64_t step) {
     ko_pop *P = &net->pops[pop];
+    const int T = net->threads > 1 ? net->threads : 1;
+    (void)T;
+#pragma omp parallel for num_threads(T) schedule(static) if (T > 1)
     for (uint32_t i = 0; i < P->n; ++i) {
         if (P->lif) lif_pre(&P->nl[i]);
         else if (P->altai) {
@@ -965,9 +1038,20 @@ static void population_step(ko_net *net, uint32_t pop, uint64_t step) {
         }
     }
     P->n_last = 0;
-    for (uint32_t i = 0; i < P->n; ++i)
-        if (P->lif ? lif_post(&P->nl[i]) : P->altai ? altai_post(&P->na[i]) : blifat_post(&P->nb[i]))
-            P->last_spikes[P->n_last++] = i;
+    if (T > 1) {
+        /* post-impact states in parallel parts, the spike message in ascending order */
+        uint8_t *fired = (uint8_t *)xcalloc(P->n ? P->n : 1, 1);
+#pragma omp parallel for num_threads(T) schedule(static)
+        for (uint32_t i = 0; i < P->n; ++i)
+            fired[i] = (uint8_t)(P->lif ? lif_post(&P->nl[i]) : P->altai ? altai_post(&P->na[i]) : blifat_post(&P->nb[i]));
+        for (uint32_t i = 0; i < P->n; ++i)
+            if (fired[i]) P->last_spikes[P->n_last++] = i;
+        free(fired);
+    } else {
+        for (uint32_t i = 0; i < P->n; ++i)
+            if (P->lif ? lif_post(&P->nl[i]) : P->altai ? altai_post(&P->na[i]) : blifat_post(&P->nb[i]))
+                P->last_spikes[P->n_last++] = i;
+    }
     if (P->resource) train_population(net, pop, P->last_spikes, P->n_last, step);
     if (P->record && P->rec_steps < P->rec_cap) {
         uint32_t words = (P->n + 31) / 32;

@@ -25,7 +25,7 @@ def build():
 def _load(policy):
     if policy in _LIBS:
         return _LIBS[policy]
-    name = {"exact": "liboracle_exact.so", "f64": "liboracle_f64.so"}[policy]
+    name = {"exact": "liboracle_exact.so", "f64": "liboracle_f64.so", "mt": "liboracle_mt.so"}[policy]
     path = os.path.join(_HERE, "lib", name)
     if not os.path.exists(path):
         build()
@@ -37,6 +37,7 @@ def _load(policy):
         "ko_create": (P, []),
         "ko_destroy": (None, [P]),
         "ko_last_error": (C.c_char_p, [P]),
+        "ko_set_threads": (i32, [P, i32]),
         "ko_add_blifat_population": (i32, [P, C.POINTER(abi.BLIFATNeuron), u32]),
         "ko_add_resource_blifat_population": (i32, [P, C.POINTER(abi.ResourceBLIFATNeuron), u32]),
         "ko_add_lif_population": (i32, [P, C.POINTER(abi.LIFNeuron), u32]),
@@ -82,11 +83,18 @@ def bernoulli_hash(seed, step, channel):
 
 
 class Oracle:
-    """One network instance of the restated single-threaded CPU backend."""
+    """One network instance of the restated single-threaded CPU backend. threads > 1: the multi-threaded
+    backend's decomposition of the same step (exact policy only; same results as one thread)."""
 
-    def __init__(self, policy="exact"):
+    def __init__(self, policy="exact", threads=1):
+        if threads > 1:
+            if policy != "exact":
+                raise ValueError("the multi-threaded step is built for the exact policy only")
+            policy = "mt"
         self.lib = _load(policy)
         self.net = self.lib.ko_create()
+        if threads > 1:
+            self.lib.ko_set_threads(self.net, int(threads))
         self.pop_sizes = []
         self.proj_sizes = []
         self._rec_cap = {}

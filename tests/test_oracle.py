@@ -673,3 +673,19 @@ def test_altai_network_matches_numpy(oracle_built):
     x = o.altai_lif_neurons(pop)
     assert np.array_equal([c.potential for c in x], V)
     assert np.array_equal([c.activity_time for c in x], at)
+
+
+def test_multi_threaded_step_equals_single_thread(oracle_built):
+    """The multi-threaded decomposition (cpu-multi-threaded-backend/impl/backend.cpp:77-323) gives the
+    single-threaded step's spikes, potentials and event count: its chunks are merged in spike order."""
+    for spec in (workloads.plain_small(seed=5, n=600, n_syn=30000),):
+        a = spec.load_into(Oracle("exact"))
+        b = spec.load_into(Oracle("exact", threads=4))
+        for x in (a, b):
+            x.record_spikes(0, 150)
+            x.step(150)
+        ra, rb = a.read_spike_record(0), b.read_spike_record(0)
+        assert ra.sum() > 100
+        assert np.array_equal(ra, rb)
+        assert np.array_equal(a.potentials(0), b.potentials(0))
+        assert a.events == b.events
