@@ -90,7 +90,9 @@ def main():
 
     parity_steps = min(args.parity_steps, args.warmup)
     gpu.record_spikes(pop, parity_steps)
-    gpu.step(args.warmup)
+    gpu.step(parity_steps)
+    gpu_pot = gpu.potentials(pop)  # state after the compared prefix
+    gpu.step(args.warmup - parity_steps)
     gpu.synchronize()
     gpu_prefix = gpu.read_spike_record(pop)
     log(f"warmup {args.warmup} steps done")
@@ -189,7 +191,9 @@ def main():
         ora_prefix = ora.read_spike_record(pop)
         same = bool(np.array_equal(ora_prefix, gpu_prefix))
         parity = {"steps_compared": int(parity_steps), "spike_trains_identical": same,
-                  "spikes_compared": int(ora_prefix.sum())}
+                  "spikes_compared": int(ora_prefix.sum()),
+                  "potential_rel_err_max": pot_error(gpu_pot, ora.potentials(pop))[0],
+                  "fp64": compare_f64(spec, gpu_prefix, gpu_pot, pop, parity_steps)}
         # probe, then a sample sized to ~cpu_seconds
         ev_a = ora.events
         tp = time.perf_counter()
