@@ -177,6 +177,7 @@ def main():
     # ---- CPU baseline: the oracle (restated single-threaded backend), bounded sample, 1 thread
     cpu = None
     cpu_mt = None
+    cpu_release = None
     parity = {"steps_compared": 0}
     if not args.no_cpu_baseline:
         import numpy as np
@@ -244,6 +245,21 @@ def main():
             }
             mt.close()
             log(f"cpu baseline ({threads} threads) {cpu_mt['value']:.3e} events/s")
+        # the same single-thread step built with Release-like flags (-Ofast, AVX2 / FMA): a speed figure only
+        rl = spec.load_into(Oracle("release"))
+        rl.step(parity_steps + 10)
+        eb = rl.events
+        tr = time.perf_counter()
+        rl.step(n_cpu)
+        trl = time.perf_counter() - tr
+        cpu_release = {
+            "value": (rl.events - eb) / trl, "unit": "events/s", "cores": 1, "kind": "port",
+            "sample": f"the same {n_cpu} steps, single thread, oracle built -Ofast -march=x86-64-v3 (the "
+                      f"reference's Release flags with a portable target; not bit-exact)",
+            "net_steps_per_s": n_cpu / trl,
+        }
+        rl.close()
+        log(f"cpu baseline (Release-like) {cpu_release['value']:.3e} events/s")
 
     line = {
         "metric": "synaptic events/s",
@@ -275,6 +291,7 @@ def main():
                     for k, v in kern.items()},
         "cpu_baseline": cpu,
         "cpu_baseline_mt": cpu_mt,
+        "cpu_baseline_release": cpu_release,
         "parity": parity,
         "device": device_name(0),
     }

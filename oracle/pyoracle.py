@@ -25,7 +25,8 @@ def build():
 def _load(policy):
     if policy in _LIBS:
         return _LIBS[policy]
-    name = {"exact": "liboracle_exact.so", "f64": "liboracle_f64.so", "mt": "liboracle_mt.so"}[policy]
+    name = {"exact": "liboracle_exact.so", "f64": "liboracle_f64.so", "mt": "liboracle_mt.so",
+            "release": "liboracle_release.so"}[policy]
     path = os.path.join(_HERE, "lib", name)
     if not os.path.exists(path):
         build()
