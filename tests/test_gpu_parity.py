@@ -2,7 +2,7 @@
 
 Bar (integer/index work and default BLIFAT): spike trains bit-exact, SynapticImpact counts equal.
 Floating point: FP32 device state against the oracle's double state within the tolerances written in each
-test (north-star bar for potentials: 1e-6 relative).
+test (the bar for potentials: 1e-6 relative).
 """
 import sys
 
