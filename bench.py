@@ -61,7 +61,7 @@ def main():
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: the step of one network runs on one MI355X "
-              "(multi-GPU sharding is a design note, not part of this build). Run with --gpus 1.", file=sys.stderr)
+              "(sharding over several GPUs is designed, not implemented). Run with --gpus 1.", file=sys.stderr)
         sys.exit(2)
 
     from knp_amd import workloads
