@@ -689,3 +689,22 @@ def test_multi_threaded_step_equals_single_thread(oracle_built):
         assert np.array_equal(ra, rb)
         assert np.array_equal(a.potentials(0), b.potentials(0))
         assert a.events == b.events
+    # several populations and projections, LIF and BLIFAT, pushed channels and a Bernoulli source
+    from networks import lif_network
+
+    runs = []
+    for threads in (1, 4):
+        o = Oracle("exact", threads=threads)
+        pops, push = lif_network(o, 21)
+        for p in pops:
+            o.record_spikes(p, 40)
+        rng = np.random.default_rng(6)
+        for _ in range(4):
+            o.push_spikes(push, np.nonzero(rng.random(1500) < 0.08)[0].astype(np.uint32))
+            o.step(10)
+        runs.append(([o.read_spike_record(p) for p in pops], o.potentials(pops[0]), o.potentials(pops[1]), o.events))
+    assert runs[0][0][0].sum() > 100
+    for k in range(2):
+        assert np.array_equal(runs[0][0][k], runs[1][0][k])
+    assert np.array_equal(runs[0][1], runs[1][1]) and np.array_equal(runs[0][2], runs[1][2])
+    assert runs[0][3] == runs[1][3]
