@@ -397,6 +397,23 @@ def other_workload(args):
         tc = time.perf_counter()
         o.step(steps)
         tcpu = time.perf_counter() - tc
+        # the same steps with the multi-threaded decomposition and with Release-like flags
+        others = {}
+        threads = max(1, min(16, os.cpu_count() or 1))
+        for key, kw, what in (("cpu_baseline_mt", {"threads": threads}, f"{threads} threads (the multi-threaded "
+                               "backend's decomposition, OpenMP)"),
+                              ("cpu_baseline_release", {"policy": "release"}, "single thread, -Ofast "
+                               "-march=x86-64-v3 (not bit-exact)")):
+            if key == "cpu_baseline_mt" and threads == 1:
+                continue
+            x = spec.load_into(Oracle(**kw))
+            tx = time.perf_counter()
+            x.step(steps)
+            tx = time.perf_counter() - tx
+            others[key] = {"value": x.events / tx, "unit": "events/s",
+                           "cores": threads if key == "cpu_baseline_mt" else 1, "kind": "port",
+                           "net_steps_per_s": steps / tx, "sample": f"all {steps} steps, {what}"}
+            x.close()
         line.update(metric="synaptic events/s", value=ev / dt, net_steps_per_s=steps / dt, steps=steps,
                     events=ev, dtype="fp32",
                     parity_knp_exact={"spike_trains_identical": bool(np.array_equal(rec, o.read_spike_record(0))),
@@ -404,7 +421,7 @@ def other_workload(args):
                                       "potential_rel_err_max": pot_error(pot, o.potentials(0))[0]},
                     fp64=compare_f64(spec, rec, pot, 0, steps),
                     cpu_baseline={"value": o.events / tcpu, "unit": "events/s", "cores": 1, "kind": "port",
-                                  "net_steps_per_s": steps / tcpu, "sample": f"all {steps} steps"})
+                                  "net_steps_per_s": steps / tcpu, "sample": f"all {steps} steps"}, **others)
         o.close()
     elif args.workload in ("c2", "c3"):
         steps = args.steps if args.steps != 10_000 else 2000
