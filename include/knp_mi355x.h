@@ -207,10 +207,11 @@ int knp_mi355x_set_memory_cap(knp_net *net, uint64_t bytes);
 enum knp_scatter_mode { KNP_SCATTER_AUTO = 0, KNP_SCATTER_ATOMIC = 1, KNP_SCATTER_STEPWISE = 2 };
 int knp_mi355x_set_scatter_mode(knp_net *net, int mode);
 /* Testing hook (no reference counterpart): in persistent launches, workgroup hold_block (~0u: none) does
- * not publish its spike counts, and every wait gives up after spin_limit polls (at least 64; default
- * 2^20). A launch whose workgroups cannot all proceed then ends with KNP_ERR_HIP instead of hanging, which
+ * not publish its spike counts at steps before hold_until (UINT64_MAX: every step), and every wait gives up
+ * after spin_limit polls (at least 64; default 2^20). A launch whose workgroups cannot all proceed then ends,
+ * later launches of the same call return at once, and the call returns KNP_ERR_HIP instead of hanging, which
  * is what the hook lets a test check. The network state is undefined after such an error. */
-int knp_mi355x_set_wait_test(knp_net *net, uint32_t hold_block, uint32_t spin_limit);
+int knp_mi355x_set_wait_test(knp_net *net, uint32_t hold_block, uint32_t spin_limit, uint64_t hold_until);
 /* 1 if the population's impacts are summed by tiles (valid after the first step or an explicit init). */
 int knp_mi355x_get_tiled(knp_net *net, uint32_t pop, int *tiled);
 

@@ -38,7 +38,7 @@ def _declare(lib):
         "knp_mi355x_destroy": (i32, [P]),
         "knp_mi355x_set_memory_cap": (i32, [P, u64]),
         "knp_mi355x_set_scatter_mode": (i32, [P, i32]),
-        "knp_mi355x_set_wait_test": (i32, [P, u32, u32]),
+        "knp_mi355x_set_wait_test": (i32, [P, u32, u32, u64]),
         "knp_mi355x_get_tiled": (i32, [P, u32, C.POINTER(i32)]),
         "knp_mi355x_get_tile_counters": (i32, [P, C.POINTER(u64), C.POINTER(u64)]),
         "knp_mi355x_add_blifat_population": (i32, [P, C.POINTER(abi.BLIFATNeuron), u32, pu32]),
@@ -333,10 +333,10 @@ class MI355XBackend:
         before the first step."""
         self._chk(self.lib.knp_mi355x_set_scatter_mode(self.net, mode))
 
-    def set_wait_test(self, hold_block, spin_limit):
-        """Testing hook: a persistent workgroup that withholds its spike counts, and a small bound on every
-        wait, so that a step call must end with an error (see knp_mi355x_set_wait_test)."""
-        self._chk(self.lib.knp_mi355x_set_wait_test(self.net, hold_block, spin_limit))
+    def set_wait_test(self, hold_block, spin_limit, hold_until=2**64 - 1):
+        """Testing hook: a persistent workgroup that withholds its spike counts at steps before hold_until, and
+        a small bound on every wait, so that a step call must end with an error (see knp_mi355x_set_wait_test)."""
+        self._chk(self.lib.knp_mi355x_set_wait_test(self.net, hold_block, spin_limit, hold_until))
 
     def tiled(self, pop):
         t = C.c_int()

@@ -46,6 +46,10 @@ struct ParamU {
     uint32_t v;
     const uint32_t *a;
 };
+struct ParamI64 {
+    int64_t v;
+    const int64_t *a;
+};
 
 // Synaptic-resource neuron state and parameters (neuron_parameters<SynapticResourceSTDPNeuron<...>>).
 struct ResDev {
@@ -78,8 +82,15 @@ struct PopDev {
     int64_t *tbp;
     // state (LEAN): step of the last spike, UINT32_MAX = none since load
     uint32_t *last_spike;
-    ParamF act, thr_decay, thr_inc, trace_decay, trace_inc, g_decay, V_decay, reflexive, E_inh, V_reset, V_min,
-        add_thr;
+    ParamF act, thr_decay, thr_inc, trace_decay, trace_inc, g_decay, V_decay, reflexive, E_inh, V_reset, V_min;
+    // BLIFAT spike threshold (blifat_impl.h:113-115): its static part T = activation_threshold_ +
+    // additional_threshold_ (synaptic-resource neurons: activation_threshold_ alone, their additional threshold
+    // is trained), summed in double as the reference sums it, in three forms: thr_fix = ceil(T * 2^frac_bits)
+    // (an exact accumulator sum a reaches T iff a >= thr_fix), thr_up = the smallest float >= T (a float
+    // potential V reaches T iff V >= thr_up), and thr_hi + thr_lo = T to 48 bits (thresholds with a dynamic
+    // part, compared in float-float arithmetic).
+    ParamI64 thr_fix;
+    ParamF thr_up, thr_hi, thr_lo;
     ParamU bperiod, refr;
     ParamU alt_flags;      // AltAI-LIF: kAlt* bits of is_diff_, is_reset_, leak_rev_, saturate_, do_not_save_
     uint32_t lean_ts_max;  // LEAN: every neuron started with n_time_steps_since_last_firing = SIZE_MAX
@@ -258,6 +269,7 @@ struct NetDev {
     uint32_t *grid_bar;        // abort flag of persistent launches (word kBarAbort)
     uint32_t spin_limit;       // polls before a persistent launch's wait gives up (kSpinLimit)
     uint32_t hold_block;       // testing: this persistent workgroup withholds its spike counts (none: ~0u)
+    uint64_t hold_until;       // testing: ... at steps before this one
     unsigned long long *tile_part;  // [4 * n_tiles] impacts, rows, row segments, ring cells updated
     uint32_t blk_proj[64];     // projection index of each projection with BLOCKING synapses
     uint32_t n_dense;

@@ -109,6 +109,7 @@ struct knp_net {
     bool persistent = false;  // fused, and every tile fits on the device at once (cooperative launch)
     bool wide_tiles = false;  // tiles outnumber the workgroups resident at once: the 6-waves tile kernel
     uint32_t test_hold_block = 0xFFFFFFFFu, test_spin_limit = kSpinLimit;  // knp_mi355x_set_wait_test
+    uint64_t test_hold_until = UINT64_MAX;
     uint32_t run_kind = POP_LEAN;  // population kind of the persistent launch
     uint32_t tile_lds = 0;  // dynamic LDS bytes of the tile scatter (largest tiled ring)
     std::vector<uint32_t *> seg_list_host;
@@ -182,6 +183,36 @@ int make_param_u(knp_net *net, uint32_t n, Get get, ParamU *out) {
     int rc = upload(net, &d, v.data(), n);
     out->a = d;
     return rc;
+}
+
+int make_param_i64(knp_net *net, const std::vector<int64_t> &v, ParamI64 *out) {
+    out->a = nullptr;
+    out->v = v.empty() ? 0 : v[0];
+    bool uniform = true;
+    for (size_t i = 1; i < v.size() && uniform; ++i) uniform = v[i] == out->v;
+    if (uniform) return KNP_OK;
+    int64_t *d = nullptr;
+    int rc = upload(net, &d, v.data(), v.size());
+    out->a = d;
+    return rc;
+}
+
+// The static part of a BLIFAT neuron's spike threshold, activation_threshold_ + additional_threshold_, in the
+// forms the device compares with (PopDev::thr_fix / thr_up / thr_hi / thr_lo). T is the reference's double sum.
+struct ThresholdForms {
+    int64_t fix;  // ceil(T * 2^frac_bits), kept within +-2^61 (accumulator sums stay below 2^60)
+    float up, hi, lo;
+};
+ThresholdForms threshold_forms(double T, int32_t frac_bits) {
+    ThresholdForms f;
+    const double lim = std::ldexp(1.0, 61);
+    const double x = std::ceil(std::ldexp(T, frac_bits));
+    f.fix = x >= lim ? (int64_t(1) << 61) : x <= -lim ? -(int64_t(1) << 61) : (int64_t)x;
+    f.up = (float)T;
+    if ((double)f.up < T) f.up = std::nextafter(f.up, std::numeric_limits<float>::infinity());
+    f.hi = (float)T;
+    f.lo = (float)(T - (double)f.hi);
+    return f;
 }
 
 uint32_t ts_to_device(uint64_t v) {
@@ -597,8 +628,20 @@ int init_network(knp_net *net) {
             if ((rc = make_param_f(net, n, [&](uint32_t i) { return (float)nb[i].activation_threshold; }, &P.act))) return rc;
             if ((rc = make_param_f(net, n, [&](uint32_t i) { return (float)nb[i].potential_reset_value; }, &P.V_reset))) return rc;
             if ((rc = make_param_f(net, n, [&](uint32_t i) { return (float)nb[i].min_potential; }, &P.V_min))) return rc;
-            if (H.kind != POP_RESOURCE)
-                if ((rc = make_param_f(net, n, [&](uint32_t i) { return (float)nb[i].additional_threshold; }, &P.add_thr))) return rc;
+            {
+                // static threshold: activation + additional threshold (resource neurons train the additional one)
+                std::vector<ThresholdForms> tf(n);
+                std::vector<int64_t> fix(n);
+                for (uint32_t i = 0; i < n; ++i) {
+                    const double T = nb[i].activation_threshold + (H.kind == POP_RESOURCE ? 0.0 : nb[i].additional_threshold);
+                    tf[i] = threshold_forms(T, H.frac_bits);
+                    fix[i] = tf[i].fix;
+                }
+                if ((rc = make_param_i64(net, fix, &P.thr_fix))) return rc;
+                if ((rc = make_param_f(net, n, [&](uint32_t i) { return tf[i].up; }, &P.thr_up))) return rc;
+                if ((rc = make_param_f(net, n, [&](uint32_t i) { return tf[i].hi; }, &P.thr_hi))) return rc;
+                if ((rc = make_param_f(net, n, [&](uint32_t i) { return tf[i].lo; }, &P.thr_lo))) return rc;
+            }
             if (H.kind == POP_LEAN) {
                 P.lean_ts_max = nb.empty() ? 0 : (nb[0].n_time_steps_since_last_firing == std::numeric_limits<uint64_t>::max());
                 std::vector<uint32_t> ls(n, 0xFFFFFFFFu);
@@ -1212,6 +1255,7 @@ int init_network(knp_net *net) {
         if ((rc = dev_alloc(net, (void **)&T.grid_bar, kBarWords * 4))) return rc;
         T.spin_limit = net->test_spin_limit;
         T.hold_block = net->test_hold_block;
+        T.hold_until = net->test_hold_until;
     }
 
     // ---- control blocks
@@ -1360,7 +1404,9 @@ int run_steps(knp_net *net, uint64_t n_steps) {
     };
     if (net->persistent && n_steps >= 2) {
         // persistent launches, split where a dense projection starts a new time block (its product must be
-        // ready before phase 1 of that step)
+        // ready before phase 1 of that step). The abort flag is cleared once per call: a launch that starts
+        // with it raised returns at once, and the host reads it after the last launch.
+        HIP_TRY(hipMemsetAsync(net->table.grid_bar, 0, kBarWords * 4, net->stream));
         uint64_t done = 0;
         while (done < n_steps) {
             const uint64_t t = net->step;
@@ -1374,7 +1420,6 @@ int run_steps(knp_net *net, uint64_t n_steps) {
                 const uint64_t next = t_first + ((t - t_first) / Dh.T + 1) * Dh.T;
                 len = std::min<uint64_t>(len, next - t);
             }
-            HIP_TRY(hipMemsetAsync(net->table.grid_bar, 0, kBarWords * 4, net->stream));
             hipError_t le = hipSuccess;
             const uint32_t nt = net->table.n_tiles, steps = (uint32_t)len;
             uint64_t t0 = t;
@@ -2307,14 +2352,16 @@ int knp_mi355x_get_tile_counters(knp_net *net, uint64_t *row_parts, uint64_t *ri
     return KNP_OK;
 }
 
-int knp_mi355x_set_wait_test(knp_net *net, uint32_t hold_block, uint32_t spin_limit) {
+int knp_mi355x_set_wait_test(knp_net *net, uint32_t hold_block, uint32_t spin_limit, uint64_t hold_until) {
     if (!net) return fail(KNP_ERR_INVALID_ARGUMENT, "null handle");
     if (spin_limit < 64) return fail(KNP_ERR_INVALID_ARGUMENT, "spin limit below 64 polls");
     net->test_hold_block = hold_block;
     net->test_spin_limit = spin_limit;
+    net->test_hold_until = hold_until;
     if (!net->initialized) return KNP_OK;
     net->table.hold_block = hold_block;
     net->table.spin_limit = spin_limit;
+    net->table.hold_until = hold_until;
     return upload_table(net);
 }
 

@@ -53,12 +53,42 @@ __device__ __forceinline__ uint32_t ts_inc(uint32_t x) {
 
 __device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
 
-// Exact "sum >= threshold" when the potential before impacts is 0: the accumulator holds the exact sum
-// on a 2^-frac grid, so comparing against the smallest grid point >= thr decides exactly.
-__device__ __forceinline__ bool fixed_ge(int64_t a, float thr, float scale, float v_float) {
-    float ts = thr * scale;
-    if (fabsf(ts) < 9.0e18f) return a >= (long long)ceilf(ts);
-    return v_float >= thr;
+__device__ __forceinline__ int64_t pi64(const ParamI64 &p, uint32_t i) { return p.a ? gp(p.a)[i] : p.v; }
+
+// s + e = x + y exactly (Knuth's TwoSum; the library is built without FMA contraction or reassociation).
+__device__ __forceinline__ void two_sum(float x, float y, float &s, float &e) {
+    s = x + y;
+    const float bb = s - x;
+    e = (x - (s - bb)) + (y - bb);
+}
+
+// A threshold with a dynamic part, hi + lo + x1 + x2, as a float-float pair s + l (about 48 bits).
+__device__ __forceinline__ void threshold_ff(float hi, float lo, float x1, float x2, float &s, float &l) {
+    float s1, e1, e2;
+    two_sum(hi, x1, s1, e1);
+    two_sum(s1, x2, s, e2);
+    l = lo + (e1 + e2);
+}
+
+// Does an exact accumulator sum a (on the 2^-frac_bits grid, |a| < 2^60) reach the threshold s + l?
+// scale = 2^frac_bits. The threshold times scale is split into integer parts and fractions, so the comparison
+// is made in int64 except for the fractions' sum.
+__device__ __forceinline__ bool fixed_ge_ff(int64_t a, float s, float l, float scale) {
+    const float H = s * scale, L = l * scale;
+    if (!(fabsf(H) < 2.0e18f)) return H < 0.f;  // beyond any sum the accumulator holds
+    const float Hf = floorf(H), Lf = floorf(L);
+    const int64_t d = a - (int64_t)Hf - (int64_t)Lf;
+    if (d >= 2) return true;
+    if (d < 0) return false;
+    return (float)d >= (H - Hf) + (L - Lf);  // the fractions sum to [0, 2)
+}
+
+// Does a float potential V reach the threshold s + l?
+__device__ __forceinline__ bool float_ge_ff(float V, float s, float l) {
+    float r, er, d, ed;
+    two_sum(V, -s, r, er);  // r + er = V - s
+    two_sum(r, -l, d, ed);  // d + ed = r - l
+    return d >= -(ed + er);
 }
 
 __device__ __forceinline__ float fixed_to_float(int64_t a, float inv_scale) {
@@ -226,9 +256,8 @@ __device__ __forceinline__ bool population_update(const NetDev *__restrict__ net
         if (KIND == POP_LEAN) {
             // default BLIFAT (blifat.h defaults): V_pre = 0, no decays, no refractory period, no blocking.
             const bool can = !(P.lean_ts_max && t == P.t_first);  // step 0 rule (wrap of SIZE_MAX)
-            const float thr = pf(P.act, i) + pf(P.add_thr, i);
             float V = fixed_to_float(a, P.inv_scale);
-            spike = can && fixed_ge(a, thr, P.scale, V);
+            spike = can && a >= pi64(P.thr_fix, i);
             if (spike) {
                 V = pf(P.V_reset, i);
                 if (KEEP) *ls_keep = (uint32_t)t;
@@ -268,12 +297,17 @@ __device__ __forceinline__ bool population_update(const NetDev *__restrict__ net
             if (at == 0) at = INT64_MAX;
             bool was_reset = false;
             // plain: the smallest float not below activation + additional threshold (exact for V >= T);
-            // resource: activation_threshold_ + the trained threshold, compared in double as the reference
-            const double thr = (KIND == POP_RESOURCE) ? (double)pf(P.act, i) + (double)P.res.add_thr[i]
-                                                      : (double)pf(P.act, i);
-            if ((double)V >= thr) {
+            // resource: activation_threshold_ + the trained threshold, as a float-float sum
+            float ts = pf(P.act, i), tl = 0.f;
+            if (KIND == POP_RESOURCE) two_sum(ts, P.res.add_thr[i], ts, tl);
+            if (KIND == POP_RESOURCE ? float_ge_ff(V, ts, tl) : V >= ts) {
                 spike = at > 0;
-                if (fl & kAltDiff) V = (float)((double)V - thr);
+                if (fl & kAltDiff) {
+                    // V - T rounded once: (V - ts) as r + er exactly, then r + (er - tl)
+                    float r, er;
+                    two_sum(V, -ts, r, er);
+                    V = r + (er - tl);
+                }
                 if (fl & kAltReset) {
                     V = pf(P.V_reset, i);
                     was_reset = true;
@@ -365,9 +399,17 @@ __device__ __forceinline__ bool population_update(const NetDev *__restrict__ net
                 V = pf(P.E_inh, i);
                 exact = false;
             }
-            const float add = (KIND == POP_RESOURCE) ? P.res.add_thr[i] : pf(P.add_thr, i);
-            const float thr = pf(P.act, i) + dyn + add;
-            const bool above = exact ? fixed_ge(a, thr, P.scale, V) : (V >= thr);
+            // threshold (blifat_impl.h:113-115): the static part alone when the dynamic threshold is 0 (exact
+            // tests), else static + dynamic (+ trained additional threshold) in float-float
+            bool above;
+            if (KIND != POP_RESOURCE && dyn == 0.f) {
+                above = exact ? a >= pi64(P.thr_fix, i) : V >= pf(P.thr_up, i);
+            } else {
+                float ts, tl;
+                threshold_ff(pf(P.thr_hi, i), pf(P.thr_lo, i), dyn, KIND == POP_RESOURCE ? P.res.add_thr[i] : 0.f,
+                             ts, tl);
+                above = exact ? fixed_ge_ff(a, ts, tl, P.scale) : float_ge_ff(V, ts, tl);
+            }
             spike = (n_ts > pu(P.refr, i)) && above;
             if (spike) {
                 dyn += pf(P.thr_inc, i);
@@ -1080,7 +1122,7 @@ __device__ __forceinline__ uint32_t dual_segment_write(const NetDev *__restrict_
         const uint32_t *vp = upper ? vp_b : vp_a;
         const size_t par = (size_t)(t & 1);
         if (h == 0) {
-            if (!COHERENT || blockIdx.x != net->hold_block)  // (a test may have one workgroup withhold)
+            if (!COHERENT || blockIdx.x != net->hold_block || t >= net->hold_until)  // (a test may have one workgroup withhold)
                 st_u32<COHERENT>(&seg[0], COHERENT ? total | list_stamp(t) << kStampShift : total);
             st_u32<COHERENT>(&net->seg_count[par * net->n_seg + sg], total);
             for (uint32_t k = 0; k < 4; ++k) rows += vp[k] != 0xFFFFFFFFu ? total : 0u;
@@ -1237,6 +1279,8 @@ template <uint32_t KIND>
 __global__ __launch_bounds__(kTileThreads) void k_tile_run(const NetDev *__restrict__ net, uint64_t t0, uint32_t n) {
     extern __shared__ unsigned long long tacc[];
     __shared__ TileShared sh;
+    // an earlier launch of the same call gave up a wait: the state is undefined, nothing more runs
+    if (ld_u32<true>(net->grid_bar + kBarAbort) != 0) return;
     const TileCtx c = tile_setup(net, tacc, sh);
     const uint32_t tid = threadIdx.x;
     // the ring of the tile from HBM

@@ -490,6 +490,25 @@ def test_persistent_wait_ends_with_an_error():
     assert np.array_equal(o.read_spike_record(0), h.read_spike_record(0))
 
 
+def test_persistent_wait_error_survives_later_launches():
+    """One call split into several persistent launches (a dense projection starts a new launch at every
+    32-step time block); a workgroup withholds its spike counts only before step 10, so only the first launch
+    gives up a wait: the later launches must not hide the failure."""
+    say("case: a wait given up in the first of several persistent launches of one call")
+    g = MI355XBackend(0)
+    pops, push = all_tiled_network(g, 8)
+    n = g.pop_sizes[pops[0]]
+    w = abi.bf16_encode(np.full((n, 16), 2.0 ** -5, np.float32))
+    g.add_dense_input_projection(pops[0], [0], [n], 16, abi.DENSE_BF16, w, None,
+                                 workloads.bernoulli_threshold(0.1), seed=3, time_block=32)
+    g.step(3)
+    assert g.tiled(pops[0]) and g.dense_time_block(0) == 32
+    g.set_wait_test(0, 4096, hold_until=10)
+    with pytest.raises(BackendError, match="timed out"):
+        g.step(100)
+    g.close()
+
+
 def heavy_parts_network(o, seed):
     """Every population tiled, with the persistent kernel's rarer paths in use: more than 512 spiking rows per
     tile and step (a 2,048-channel source at p = 0.3 read whole by every tile: several gather passes), row
