@@ -59,6 +59,7 @@ struct ResDev {
     uint8_t *isi_status; // ISIPeriodType
     uint64_t *last_step, *last_spike_step, *first_isi_spike;
     float *dop_now;      // dopamine_value_ of this step (written by the population kernel)
+    int64_t *dop_fix;    // the same as the exact accumulator sum (2^-frac_bits units), read by training
     uint8_t *forced_now; // is_being_forced_ of this step
     ParamF coeff, res_thr, change_param, change_at_isi, d_h;
     ParamU dpt, drain, isi_max;

@@ -706,6 +706,7 @@ int init_network(knp_net *net) {
             for (uint32_t i = 0; i < n; ++i) u64[i] = nr[i].first_isi_spike;
             if ((rc = upload(net, &R.first_isi_spike, u64.data(), n))) return rc;
             R.dop_now = P.dop;
+            if ((rc = dev_alloc(net, (void **)&R.dop_fix, (size_t)n * 8))) return rc;
             if ((rc = dev_alloc(net, (void **)&R.forced_now, n))) return rc;
             if ((rc = dev_alloc(net, (void **)&R.cand, (size_t)n * 4))) return rc;
             if ((rc = make_param_f(net, n, [&](uint32_t i) { return nr[i].synapse_sum_threshold_coefficient; }, &R.coeff))) return rc;
@@ -2194,6 +2195,20 @@ int knp_mi355x_read_altai_lif_neurons(knp_net *net, uint32_t pop, knp_altai_lif_
     return KNP_OK;
 }
 
+// dopamine_value_ of a resource population's neurons once it has run: the exact sum of the last step's
+// dopamine impacts (the value is reset every step, blifat_impl.h:50-65), in place of the FP32 copy.
+static int exact_resource_dopamine(knp_net *net, uint32_t pop, std::vector<double> &dop) {
+    dop.clear();
+    if (!net->pops[pop].ran) return KNP_OK;
+    const uint32_t N = net->pops[pop].n;
+    std::vector<int64_t> fix(N);
+    int rc = download(net, fix.data(), net->table.pops[pop].res.dop_fix, N);
+    if (rc) return rc;
+    dop.resize(N);
+    for (uint32_t i = 0; i < N; ++i) dop[i] = std::ldexp((double)fix[i], -net->pops[pop].frac_bits);
+    return KNP_OK;
+}
+
 // The resource fields of every neuron of a resource population (device state over the loaded values).
 static int export_resource_fields(knp_net *net, uint32_t pop, std::vector<knp_resource_blifat_neuron> &out) {
     const uint32_t N = net->pops[pop].n;
@@ -2233,10 +2248,13 @@ int knp_mi355x_read_resource_altai_lif_neurons(knp_net *net, uint32_t pop, knp_r
     const uint32_t N = net->pops[pop].n;
     std::vector<knp_altai_lif_neuron> base(N);
     std::vector<knp_resource_blifat_neuron> res;
+    std::vector<double> dop;
     if ((rc = knp_mi355x_read_altai_lif_neurons(net, pop, base.data(), N))) return rc;
     if ((rc = export_resource_fields(net, pop, res))) return rc;
+    if ((rc = exact_resource_dopamine(net, pop, dop))) return rc;
     for (uint32_t i = 0; i < n; ++i) {
         out[i].base = base[i];
+        if (!dop.empty()) out[i].base.dopamine_value = dop[i];
         copy_resource_fields(res[i], out[i]);
     }
     return KNP_OK;
@@ -2252,11 +2270,14 @@ int knp_mi355x_read_resource_neurons(knp_net *net, uint32_t pop, knp_resource_bl
     const uint32_t N = net->pops[pop].n;
     std::vector<knp_blifat_neuron> base(N);
     std::vector<knp_resource_blifat_neuron> res;
+    std::vector<double> dop;
     if ((rc = knp_mi355x_read_blifat_neurons(net, pop, base.data(), N))) return rc;
     if ((rc = export_resource_fields(net, pop, res))) return rc;
+    if ((rc = exact_resource_dopamine(net, pop, dop))) return rc;
     for (uint32_t i = 0; i < n; ++i) {
         out[i] = res[i];
         out[i].base = base[i];
+        if (!dop.empty()) out[i].base.dopamine_value = dop[i];
     }
     return KNP_OK;
 }

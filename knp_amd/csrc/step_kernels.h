@@ -218,9 +218,10 @@ __device__ __forceinline__ const T &entry_of(const T *table, uint32_t i, const T
 // renormalization (training/stdp.h:152-186; the test only depends on state that training leaves unchanged
 // for other neurons).
 __device__ __forceinline__ bool resource_candidate(const PopDev &P, uint32_t i, uint64_t t, bool spike,
-                                                   float dop_now, uint32_t forced) {
+                                                   float dop_now, int64_t dop_fix, uint32_t forced) {
     const ResDev &R = P.res;
     gp(R.dop_now)[i] = dop_now;  // dopamine_value_ is reset every step for resource neurons
+    gp(R.dop_fix)[i] = dop_fix;
     gp(R.forced_now)[i] = (uint8_t)(forced != 0);
     bool renorm = false;
     const float fr = gp(R.free_res)[i];
@@ -229,7 +230,7 @@ __device__ __forceinline__ bool resource_candidate(const PopDev &P, uint32_t i, 
         const bool in_isi = (t - gp(R.last_step)[i] <= (uint64_t)pu(R.isi_max, i)) && st != 0;
         renorm = !in_isi;
     }
-    return spike || dop_now != 0.f || renorm;
+    return spike || dop_fix != 0 || renorm;
 }
 
 // The BLIFAT update of neuron i of population P at step t (blifat_impl.h:31-168): returns the spike and sets
@@ -278,11 +279,12 @@ __device__ __forceinline__ bool population_update(const NetDev *__restrict__ net
             gp(P.Vpre)[i] = V;
             V += fixed_to_float(a, P.inv_scale);
             float dop_now = 0.f;
+            int64_t dop_fix = 0;
             if (P.acc_d) {
-                const int64_t ad = gp(P.acc_d)[ai];
+                dop_fix = gp(P.acc_d)[ai];
                 gp(P.acc_d)[ai] = 0;
-                dop_now = fixed_to_float(ad, P.inv_scale);
-                if (KIND != POP_RESOURCE && ad) gp(P.dop)[i] += dop_now;
+                dop_now = fixed_to_float(dop_fix, P.inv_scale);
+                if (KIND != POP_RESOURCE && dop_fix) gp(P.dop)[i] += dop_now;
             }
             uint32_t forced = 0;
             if (KIND == POP_RESOURCE && P.acc_f) {
@@ -321,7 +323,7 @@ __device__ __forceinline__ bool population_update(const NetDev *__restrict__ net
             }
             gp(P.V)[i] = V;
             gp(P.tbp)[i] = at;
-            if (KIND == POP_RESOURCE) cand = resource_candidate(P, i, t, spike, dop_now, forced);
+            if (KIND == POP_RESOURCE) cand = resource_candidate(P, i, t, spike, dop_now, dop_fix, forced);
         } else if (P.model == MODEL_LIF) {
             // LIFNeuron (lif_impl.h:31-84): while the refractory counter is non-zero the potential neither
             // leaks nor takes impacts and the counter counts down; else leak, add the step's impacts and fire
@@ -363,10 +365,11 @@ __device__ __forceinline__ bool population_update(const NetDev *__restrict__ net
                 if (ag) g += fixed_to_float(ag, P.inv_scale);
             }
             float dop_now = 0.f;
+            int64_t dop_fix = 0;
             if (P.acc_d) {
-                int64_t ad = gp(P.acc_d)[ai];
+                dop_fix = gp(P.acc_d)[ai];
                 gp(P.acc_d)[ai] = 0;
-                dop_now = fixed_to_float(ad, P.inv_scale);
+                dop_now = fixed_to_float(dop_fix, P.inv_scale);
             }
             uint32_t forced = 0;
             if (P.acc_f) {
@@ -428,7 +431,7 @@ __device__ __forceinline__ bool population_update(const NetDev *__restrict__ net
             gp(P.bphase)[i] = bphase;
             if (P.track_tbp) gp(P.tbp)[i] = tbp;
             if (KIND == POP_RESOURCE) {
-                cand = resource_candidate(P, i, t, spike, dop_now, forced);
+                cand = resource_candidate(P, i, t, spike, dop_now, dop_fix, forced);
             } else if (P.acc_d) {
                 gp(P.dop)[i] += dop_now;  // plain BLIFAT keeps summing dopamine (never reset for it)
             }
@@ -1366,9 +1369,11 @@ __device__ __forceinline__ float recalc_weight(float r, float w_min, float w_max
 }
 
 __device__ __forceinline__ float stability_factor(float stability) {
-    // std::min(static_cast<float>(std::pow(2, -stability)), 1.F)
-    const float v = exp2f(-stability);
-    return v < 1.f ? v : 1.f;
+    // std::min(static_cast<float>(std::pow(2, -stability)), 1.F): the power in double, rounded once to float
+    // (stability_ > 0: below 1; otherwise 1, or NaN for a NaN stability, as std::min returns it)
+    if (!(stability > 0.f)) return stability == stability ? 1.f : stability;
+    const float v = (float)exp2(-(double)stability);
+    return 1.f < v ? 1.f : v;
 }
 
 __device__ __forceinline__ bool in_interval(uint64_t b, uint64_t e, uint64_t p) {
@@ -1422,7 +1427,9 @@ __global__ __launch_bounds__(kBlock) void k_train(const NetDev *__restrict__ net
     for (uint32_t c = blockIdx.x * (kBlock / 64) + wv; c < count; c += waves) {
         const uint32_t n = R.cand[c];
         const bool spiked = (P.bitmap[n >> 5] >> (n & 31)) & 1u;
-        const float dop = R.dop_now[n];
+        // dopamine_value_ (a double sum of float impacts in the reference) from the exact accumulator sum
+        const int64_t dop_fix = gp(R.dop_fix)[n];
+        const double dop = (double)dop_fix * (double)P.inv_scale;
         const bool forced = R.forced_now[n] != 0;
         uint32_t status = R.isi_status[n];
         uint64_t last_step = R.last_step[n], lss = R.last_spike_step[n], fis = R.first_isi_spike[n];
@@ -1445,7 +1452,7 @@ __global__ __launch_bounds__(kBlock) void k_train(const NetDev *__restrict__ net
                         if (last_step - t < (uint64_t)isi_max) status = 2;
                         break;
                     case 2:  // period_continued
-                        if (last_step - t >= (uint64_t)isi_max || dop != 0.f) {
+                        if (last_step - t >= (uint64_t)isi_max || dop_fix != 0) {
                             status = 1;
                             fis = t;
                         }
@@ -1527,9 +1534,10 @@ __global__ __launch_bounds__(kBlock) void k_train(const NetDev *__restrict__ net
             if (lane == 0) R.add_thr[n] = fixed_to_float(sum, P.inv_scale) * pf(R.coeff, n);
         }
         // ---- do_dopamine_plasticity_impl
-        if (dop > 0.f || (dop < 0.f && status != 0)) {
+        if (dop > 0.0 || (dop < 0.0 && status != 0)) {
             const bool time_ok = (t - lss) <= (uint64_t)pu(R.dpt, n);
-            const float rc = dop * stability_factor(stab);
+            // float resource_change = dopamine_value_ * factor (double product, rounded once)
+            const float rc = (float)(dop * (double)stability_factor(stab));
             for (uint32_t j = 0; j < R.n_work; ++j) {
                 const ProjDev &Q = net->projs[R.work[j]];
                 const uint64_t cb = Q.col_ptr[n], ce = Q.col_ptr[n + 1];
@@ -1569,14 +1577,15 @@ __global__ __launch_bounds__(kBlock) void k_train(const NetDev *__restrict__ net
                     for (uint32_t u = 0; u < kTrainChunks; ++u) freer = serial_free(freer, cm[u], 0, cm[u], 0.f, -rc, false);
                 }
             }
-            if (forced || dop < 0.f) {
-                stab = fmaf(-dop, pf(R.change_param, n), stab);
-                stab = stab > 0.f ? stab : 0.f;
+            // stability_ updates in the reference's double promotions (blifat_stdp.h:135-150), rounded once
+            if (forced || dop < 0.0) {
+                stab = (float)((double)stab - dop * (double)pf(R.change_param, n));
+                stab = stab < 0.f ? 0.f : stab;  // std::max(stability_, 0.0F)
             } else {
-                const float difference = (float)(t - fis - (uint64_t)isi_max);
-                float f = 2.f - difference / (float)isi_max;
-                f = f > -1.f ? f : -1.f;
-                stab = stab + pf(R.change_param, n) * dop * f;
+                const double difference = (double)(t - fis - (uint64_t)isi_max);  // u64 arithmetic, as fabs sees it
+                double f = 2.0 - difference / (double)isi_max;
+                f = f < -1.0 ? -1.0 : f;  // std::max(..., -1.0)
+                stab = (float)((double)stab + (double)pf(R.change_param, n) * dop * f);
             }
         }
         // ---- renormalize_resource

@@ -769,7 +769,7 @@ static int point_in_interval(uint64_t b, uint64_t e, uint64_t p) {
 static wreal stability_factor(wreal stability) {
     /* std::min(static_cast<float>(std::pow(2, -stability)), 1.F) */
     wreal v = (wreal)pow(2.0, -(double)stability);
-    return v < (wreal)1 ? v : (wreal)1;
+    return (wreal)1 < v ? (wreal)1 : v; /* std::min(a, b) = (b < a) ? b : a */
 }
 
 static void train_population(ko_net *net, uint32_t pop, const uint32_t *spikes, uint32_t n_spikes, uint64_t step) {
@@ -828,7 +828,7 @@ static void train_population(ko_net *net, uint32_t pop, const uint32_t *spikes, 
             }
             if (x->is_being_forced || dopamine < 0) {
                 x->stability = (wreal)(x->stability - dopamine * x->stability_change_parameter);
-                x->stability = x->stability > (wreal)0 ? x->stability : (wreal)0;
+                x->stability = x->stability < (wreal)0 ? (wreal)0 : x->stability; /* std::max(a, b) = (a < b) ? b : a */
             } else {
                 const double dopamine_constant = 2.0;
                 const double difference = fabs((double)(step - x->first_isi_spike - (uint64_t)x->isi_max));

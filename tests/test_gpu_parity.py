@@ -123,6 +123,15 @@ def test_resource_stdp_network(model):
     assert g.weights(gl)[0] != 1.0
 
 
+def test_resource_stdp_hand_traces_on_device():
+    """The hand-computed synaptic-resource STDP traces (tests/resource_traces.py) on the device: every
+    expected resource, weight, free resource, stability, ISI field and flag, bit for bit."""
+    import resource_traces
+
+    say("case: synaptic-resource STDP hand traces on the device")
+    resource_traces.run_all_traces(lambda: MI355XBackend(0))
+
+
 def test_resource_free_negative_zero():
     """The free-resource replay skips zero-d_u additions except when free is -0 (-0 + +0 = +0 in the
     reference's float order): a neuron loaded with free = -0.0 and d_u = 0 must follow the oracle bit for bit."""
@@ -177,7 +186,7 @@ def resource_altai_network(o, seed):
 
 def test_resource_altai_training_matches_oracle():
     """Synaptic-resource AltAI-LIF training (altai_stdp.h:35-149) over 300 steps with dopamine every 10
-    steps: spike agreement >= 0.99, and weights within 1e-6 when the spike trains are identical."""
+    steps: spike trains identical, weights within 1e-6."""
     say("case: resource AltAI-LIF training, 300 steps")
     runs = {}
     for name in ("gpu", "oracle"):
@@ -191,15 +200,15 @@ def test_resource_altai_training_matches_oracle():
         runs[name] = (o.read_spike_record(pop), o.weights(inp), o.resource_altai_lif_neurons(pop))
     sg, so = runs["gpu"][0], runs["oracle"][0]
     agr = agreement(sg, so)
-    say(f"  spikes gpu={sg.sum()} oracle={so.sum()} agreement={agr:.6f}")
-    assert so.sum() > 300
-    assert agr >= 0.99
     wg, wo = runs["gpu"][1], runs["oracle"][1]
+    say(f"  spikes gpu={sg.sum()} oracle={so.sum()} agreement={agr:.6f} "
+        f"weight max abs diff {np.max(np.abs(wg - wo)):.3e}")
+    assert so.sum() > 300
     assert np.any(wo != 0.5)
-    if np.array_equal(sg, so):
-        assert np.max(np.abs(wg - wo)) <= 1e-6
-        ng, no = runs["gpu"][2], runs["oracle"][2]
-        assert [x.isi_status for x in ng] == [x.isi_status for x in no]
+    assert np.array_equal(sg, so), np.argwhere(sg != so)[:5].tolist()
+    assert np.max(np.abs(wg - wo)) <= 1e-6
+    ng, no = runs["gpu"][2], runs["oracle"][2]
+    assert [x.isi_status for x in ng] == [x.isi_status for x in no]
 
 
 def test_plain_small_bit_exact():
@@ -597,7 +606,8 @@ def agreement(a, b):
 
 
 def test_stdp_learning_short():
-    """Config 2 shape, 100 images (2,000 steps): FP32 device training against the KNP-exact oracle."""
+    """Config 2 shape, 100 images (2,000 steps): FP32 device training against the KNP-exact oracle, spikes
+    and weights identical."""
     say("case: stdp_learning 100 images")
     spec = workloads.stdp_learning(n_images=100)
     g, o = both(spec)
@@ -608,13 +618,14 @@ def test_stdp_learning_short():
     o.step(steps)
     sg, so = g.read_spike_record(0), o.read_spike_record(0)
     agr = agreement(sg, so)
-    say(f"  spikes gpu={sg.sum()} oracle={so.sum()} agreement={agr:.6f}")
-    assert so.sum() > 1000
-    assert agr >= 0.99
     wg, wo = g.weights(0), o.weights(0)
-    say(f"  weight max abs diff {np.max(np.abs(wg - wo)):.3e}")
-    if np.array_equal(sg, so):
-        assert np.max(np.abs(wg - wo)) <= 1e-6
+    first = np.argwhere(sg != so)
+    say(f"  spikes gpu={sg.sum()} oracle={so.sum()} agreement={agr:.6f} weight max abs diff "
+        f"{np.max(np.abs(wg - wo)):.3e} first differing (step, neuron) {first[:1].tolist()}")
+    assert so.sum() > 1000
+    # the device replays the reference's float order of training: identical spikes and weights
+    assert np.array_equal(sg, so)
+    assert np.array_equal(np.asarray(wg, np.float32), np.asarray(wo, np.float32))
 
 
 def test_plain_gpu_scale_prefix():

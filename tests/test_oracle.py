@@ -708,3 +708,13 @@ def test_multi_threaded_step_equals_single_thread(oracle_built):
         assert np.array_equal(runs[0][0][k], runs[1][0][k])
     assert np.array_equal(runs[0][1], runs[1][1]) and np.array_equal(runs[0][2], runs[1][2])
     assert runs[0][3] == runs[1][3]
+
+
+def test_resource_stdp_hand_traces(oracle_built):
+    """Synaptic-resource STDP arithmetic pinned by hand-computed traces (tests/resource_traces.py): Hebbian
+    updates at stability 0 and 1, the has_contributed window with its edges and the u64 wrap, dopamine reward
+    (both signs of the stability term), punishment and forced reward, renormalization with drains 0 and 2,
+    a negative free resource and the weight formula with w_min != 0."""
+    import resource_traces
+
+    resource_traces.run_all_traces(lambda: Oracle("exact"))
