@@ -302,6 +302,25 @@ int knp_mi355x_add_dense_input_projection(
 /* The time block the dense projection runs with (after init). */
 int knp_mi355x_get_dense_time_block(knp_net *net, uint32_t dense, uint32_t *time_block);
 
+/* Expand a loaded Projection<DeltaSynapse> or Projection<SynapticResourceSTDPDeltaSynapse> on the matrix cores
+ * instead of by the scatter: each step, the spike counts of its presynaptic indexes are multiplied by a BF16
+ * copy of its weights with FP32 accumulation (the batched dense projection of many instances: no counterpart
+ * class in the reference, which expands the same projection spike by spike in calculate_projection_impl,
+ * knp/backends/cpu/cpu-library/include/knp/backends/cpu-library/impl/projections/delta/delta_impl.h:54-114).
+ * The synapses must form dense blocks: one delay, one output type (EXCITATORY or INHIBITORY_CURRENT), and
+ * every postsynaptic neuron with synapses takes each presynaptic index of one range [lo, lo + n) exactly once,
+ * n the same for all; consecutive neurons with the same range form an instance, and instances do not share
+ * presynaptic indexes (an all-to-all projection, or a block-diagonal batch of them). Synaptic-resource STDP
+ * still trains the FP32 weights (read by knp_mi355x_read_weights) and refreshes the BF16 copy. The impacts
+ * then differ from the reference's by the rounding of the weights to BF16 (8 significant bits) and of the
+ * FP32 accumulation. flags: KNP_DENSE_SKIP_EMPTY multiplies only the groups of 8 presynaptic indexes that hold
+ * a spike (an exact shortcut: the others contribute 0). format: KNP_DENSE_BF16. Set before the first step;
+ * the postsynaptic population is then summed on the atomic path. */
+#define KNP_DENSE_SKIP_EMPTY 1u
+int knp_mi355x_set_projection_dense(knp_net *net, uint32_t proj, int format, uint32_t flags);
+/* The BF16 weight copy of a projection expanded on MFMA, in synapse-index order (bf16 bit patterns). */
+int knp_mi355x_read_dense_weights(knp_net *net, uint32_t proj, uint16_t *out, uint64_t n_syn);
+
 /* ---- stepping (Backend::_step, backend.cpp:89-141) ------------------------------------------------ */
 int knp_mi355x_step(knp_net *net, uint64_t n_steps);
 /* Backend::get_step (backend.h:290). */
@@ -356,7 +375,8 @@ enum knp_kernel_id {
     KNP_KERNEL_DENSE = 3,   /* dense input projection on MFMA (spike bits + product), per time block */
     KNP_KERNEL_TILE_SCATTER = 4, /* tile-owner scatter (tiled populations), one step per launch */
     KNP_KERNEL_TILE_RUN = 5,     /* persistent tile launch: several steps per launch */
-    KNP_KERNEL_COUNT = 6
+    KNP_KERNEL_DENSE_EXPAND = 6, /* projections expanded on MFMA: spike counts + product, per step */
+    KNP_KERNEL_COUNT = 7
 };
 int knp_mi355x_enable_timing(knp_net *net, int enable);
 /* Average duration (ms) and launch count of a kernel class since timing was enabled. */

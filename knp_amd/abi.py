@@ -39,6 +39,7 @@ KERNEL_TRAIN = 2
 KERNEL_DENSE = 3
 KERNEL_TILE_SCATTER = 4
 KERNEL_TILE_RUN = 5
+KERNEL_DENSE_EXPAND = 6
 
 # projection kinds in a network file
 PROJ_DELTA = 0
@@ -53,6 +54,7 @@ SCATTER_STEPWISE = 2
 # knp_dense_format
 DENSE_FP8_E4M3 = 0
 DENSE_BF16 = 1
+DENSE_SKIP_EMPTY = 1  # knp_mi355x_set_projection_dense flag
 
 SIZE_MAX = 0xFFFFFFFFFFFFFFFF
 INT64_MAX = 0x7FFFFFFFFFFFFFFF

@@ -60,6 +60,8 @@ def _declare(lib):
         "knp_mi355x_add_dense_input_projection": (
             i32, [P, u32, u32, pu32, pu32, u32, i32, C.c_void_p, pf32, u32, u64, u32, u32, pu32]),
         "knp_mi355x_get_dense_time_block": (i32, [P, u32, pu32]),
+        "knp_mi355x_set_projection_dense": (i32, [P, u32, i32, u32]),
+        "knp_mi355x_read_dense_weights": (i32, [P, u32, C.POINTER(C.c_uint16), u64]),
         "knp_mi355x_step": (i32, [P, u64]),
         "knp_mi355x_get_step": (i32, [P, C.POINTER(u64)]),
         "knp_mi355x_synchronize": (i32, [P]),
@@ -296,6 +298,17 @@ class MI355XBackend:
             abi.f32p(sc), int(threshold), seed, hash_offset, time_block, C.byref(out)))
         return out.value
 
+    def set_projection_dense(self, proj, fmt=abi.DENSE_BF16, flags=0):
+        """Expand projection `proj` on MFMA (spike counts x a BF16 copy of its weights; see the C ABI)."""
+        self._chk(self.lib.knp_mi355x_set_projection_dense(self.net, proj, fmt, flags))
+
+    def dense_weights(self, proj):
+        """The BF16 weight copy of a projection expanded on MFMA (bit patterns, synapse-index order)."""
+        n = self.proj_sizes[proj]
+        out = np.zeros(max(n, 1), dtype=np.uint16)
+        self._chk(self.lib.knp_mi355x_read_dense_weights(self.net, proj, out.ctypes.data_as(C.POINTER(C.c_uint16)), n))
+        return out[:n]
+
     def dense_time_block(self, dense=0):
         t = C.c_uint32()
         self._chk(self.lib.knp_mi355x_get_dense_time_block(self.net, dense, C.byref(t)))
@@ -448,4 +461,5 @@ class MI355XBackend:
 
 
 KERNEL_IDS = {"neuron": abi.KERNEL_NEURON, "scatter": abi.KERNEL_SCATTER, "train": abi.KERNEL_TRAIN,
-              "dense": abi.KERNEL_DENSE, "tile_scatter": abi.KERNEL_TILE_SCATTER, "tile_run": abi.KERNEL_TILE_RUN}
+              "dense": abi.KERNEL_DENSE, "tile_scatter": abi.KERNEL_TILE_SCATTER, "tile_run": abi.KERNEL_TILE_RUN,
+              "dense_expand": abi.KERNEL_DENSE_EXPAND}

@@ -173,7 +173,34 @@ struct ProjDev {
     uint32_t tile_stride;        // tiles of the postsynaptic population
     const struct RowMeta *row_meta;  // [n_pre] ranged directory (null if none)
     const uint64_t *row_dir;     // per row: one entry per tile spanned, positions from the row start
+    // Dense expansion on MFMA (knp_mi355x_set_projection_dense): the synapses form dense blocks (instances,
+    // DenseInst), every pair of an instance's presynaptic and postsynaptic ranges once, one delay and one
+    // output type. Each step
+    // the spikes of every presynaptic index are counted (k_dense_counts: 0 or 1, more when a channel message
+    // repeats an index) and the impacts, (spike counts) x (a BF16 copy of the weights), are summed on MFMA
+    // with FP32 accumulation (k_dense_expand) into the ring slot of their arrival step.
+    uint32_t dense;              // 0: expanded by the scatter; else 1 + kDense* flags
+    uint32_t dn_oct;             // octets (groups of 8 presynaptic indexes) over all instances
+    uint32_t dn_inst, dn_tiles;  // instances, 32-neuron tiles over all instances
+    uint32_t dn_delay;           // the projection's delay
+    float dn_sign;               // +1 EXCITATORY, -1 INHIBITORY_CURRENT
+    uint32_t dn_v0, dn_v1;       // the projection's virtual segments [dn_v0, dn_v1)
+    const struct DenseInst *dn_inst_tab;  // [dn_inst]
+    const uint32_t *dn_pre_inst;  // [n_pre] instance of each presynaptic index (UINT32_MAX: none)
+    const uint32_t *dn_post_inst; // [n_post] instance of each postsynaptic neuron (UINT32_MAX: none)
+    const uint32_t *dn_tile;      // [dn_tiles] instance << 16 | first neuron of the tile / 32
+    uint16_t *dn_w;              // BF16 weights; instance k from base: [octet][neuron (npad)][8 inputs]
+    uint32_t *dn_cnt;            // [2][dn_oct][8] spikes of each presynaptic index (one byte each) in the
+                                 // step, by step parity; instance k from octet cbase
 };
+
+// An instance (dense block) of a projection expanded on MFMA: presynaptic indexes pre_begin .. + n_in - 1
+// (n_oct octets) to postsynaptic neurons post_begin .. + post_count - 1; npad = post_count rounded up to 32.
+struct alignas(8) DenseInst {
+    uint32_t pre_begin, post_begin, post_count, npad, n_in, n_oct, cbase, pad_;
+    uint64_t base;  // first BF16 weight of the instance in dn_w
+};
+constexpr uint32_t kDenseSkipEmpty = 1;  // flag: multiply only the octets of presynaptic indexes that spiked
 
 // A part entry of a tile directory: start | length << 32 | delay-1 records << 56 (255: all of the part's
 // records count as delay 1, at 255 or more of them). Lengths stay below 2^24.

@@ -72,6 +72,16 @@ struct HostProj {
     bool has_blocking = false;
     std::vector<std::vector<uint32_t>> pending;  // channel messages for the next step
     uint32_t gen_channels = 0;                   // channels of device sources feeding it
+    int dense_format = -1;                       // knp_mi355x_set_projection_dense (-1: scatter)
+    uint32_t dense_flags = 0;
+};
+
+// The dense blocks (instances) of a projection expanded on MFMA.
+struct DenseBlocks {
+    uint32_t n_oct = 0, delay = 1;  // octets over all instances
+    float sign = 1.f;
+    std::vector<DenseInst> inst;
+    std::vector<uint32_t> pre_inst, post_inst;  // instance of every presynaptic / postsynaptic index
 };
 
 struct HostDense {
@@ -484,6 +494,87 @@ int init_dense(knp_net *net, uint32_t d) {
     return KNP_OK;
 }
 
+// Checks that the synapses of a projection marked for the MFMA expansion form dense blocks and finds them:
+// one delay and one output type (EXCITATORY or INHIBITORY_CURRENT; a plain EXCITATORY projection onto a
+// synaptic-resource population would also carry forcing, which the expansion does not); every postsynaptic
+// neuron with synapses takes each presynaptic index of one range [lo, hi] exactly once; consecutive neurons
+// with the same range form an instance; instance ranges do not overlap.
+int dense_blocks(const knp_net *net, const HostProj &H, DenseBlocks &B) {
+    const uint32_t n_post = net->pops[H.post_pop].n;
+    if (H.kind == PROJ_ADDITIVE) return fail(KNP_ERR_UNSUPPORTED, "additive STDP projections are not expanded on MFMA");
+    if (H.dense_format != KNP_DENSE_BF16) return fail(KNP_ERR_UNSUPPORTED, "the MFMA expansion takes BF16 weights");
+    if (H.n_syn == 0) return fail(KNP_ERR_UNSUPPORTED, "MFMA expansion of a projection without synapses");
+    const uint8_t ty = H.type[0];
+    const uint32_t d = H.delay[0];
+    for (uint64_t s = 0; s < H.n_syn; ++s)
+        if (H.type[s] != ty || H.delay[s] != d)
+            return fail(KNP_ERR_UNSUPPORTED, "MFMA expansion: all synapses need one delay and one output type");
+    if (ty != KNP_EXCITATORY && ty != KNP_INHIBITORY_CURRENT)
+        return fail(KNP_ERR_UNSUPPORTED, "MFMA expansion: EXCITATORY or INHIBITORY_CURRENT synapses only");
+    if (H.kind == PROJ_DELTA && ty == KNP_EXCITATORY && net->pops[H.post_pop].resource)
+        return fail(KNP_ERR_UNSUPPORTED, "MFMA expansion: a forcing projection onto a synaptic-resource population");
+    std::vector<uint32_t> lo(n_post, 0xFFFFFFFFu), hi(n_post, 0), cnt(n_post, 0);
+    std::vector<uint64_t> first(n_post + 1, 0);  // bit offsets of every neuron's range in `seen`
+    for (uint64_t s = 0; s < H.n_syn; ++s) {
+        const uint32_t j = H.post[s], i = H.pre[s];
+        lo[j] = std::min(lo[j], i);
+        hi[j] = std::max(hi[j], i);
+        cnt[j]++;
+    }
+    for (uint32_t j = 0; j < n_post; ++j) {
+        const uint64_t w = cnt[j] ? (uint64_t)hi[j] - lo[j] + 1 : 0;
+        if (w != cnt[j]) return fail(KNP_ERR_UNSUPPORTED, "MFMA expansion: every neuron needs a full range of presynaptic indexes");
+        first[j + 1] = first[j] + w;
+    }
+    std::vector<uint64_t> seen((first[n_post] + 63) / 64, 0);
+    for (uint64_t s = 0; s < H.n_syn; ++s) {
+        const uint64_t b = first[H.post[s]] + (H.pre[s] - lo[H.post[s]]);
+        if (seen[b >> 6] >> (b & 63) & 1) return fail(KNP_ERR_UNSUPPORTED, "MFMA expansion: a repeated synapse");
+        seen[b >> 6] |= 1ull << (b & 63);
+    }
+    B = DenseBlocks{};
+    B.delay = d;
+    B.sign = ty == KNP_INHIBITORY_CURRENT ? -1.f : 1.f;
+    B.post_inst.assign(n_post, 0xFFFFFFFFu);
+    B.pre_inst.assign(std::max(H.n_pre, 1u), 0xFFFFFFFFu);
+    uint64_t base = 0;
+    for (uint32_t j = 0; j < n_post; ++j) {
+        if (!cnt[j]) continue;
+        const bool same = !B.inst.empty() && B.inst.back().pre_begin == lo[j] && B.inst.back().n_in == cnt[j] &&
+                          B.inst.back().post_begin + B.inst.back().post_count == j;
+        if (!same) {
+            if (!B.inst.empty()) {
+                base += (uint64_t)B.inst.back().n_oct * B.inst.back().npad * 8;
+                B.n_oct += B.inst.back().n_oct;
+            }
+            DenseInst I{};
+            I.pre_begin = lo[j];
+            I.post_begin = j;
+            I.n_in = cnt[j];
+            I.n_oct = (cnt[j] + 7) / 8;
+            I.cbase = B.n_oct;
+            I.base = base;
+            B.inst.push_back(I);
+        }
+        DenseInst &I = B.inst.back();
+        I.post_count++;
+        I.npad = (I.post_count + 31) / 32 * 32;
+        B.post_inst[j] = (uint32_t)B.inst.size() - 1;
+    }
+    B.n_oct += B.inst.back().n_oct;
+    if (B.inst.size() >= (1u << 16)) return fail(KNP_ERR_UNSUPPORTED, "MFMA expansion: 65536 or more instances");
+    if (B.n_oct >= (1u << 30)) return fail(KNP_ERR_UNSUPPORTED, "MFMA expansion: too many presynaptic indexes");
+    for (uint32_t k = 0; k < B.inst.size(); ++k) {
+        if (B.inst[k].post_count > (1u << 21)) return fail(KNP_ERR_UNSUPPORTED, "MFMA expansion: instance too large");
+        for (uint32_t i = 0; i < B.inst[k].n_in; ++i) {
+            uint32_t &x = B.pre_inst[B.inst[k].pre_begin + i];
+            if (x != 0xFFFFFFFFu) return fail(KNP_ERR_UNSUPPORTED, "MFMA expansion: instances share presynaptic indexes");
+            x = k;
+        }
+    }
+    return KNP_OK;
+}
+
 int init_network(knp_net *net) {
     HIP_TRY(hipSetDevice(net->device));
     const uint32_t n_pops = (uint32_t)net->pops.size(), n_projs = (uint32_t)net->projs.size();
@@ -526,6 +617,13 @@ int init_network(knp_net *net) {
         }
     }
     if (n_pops > 31 && n_blocking) return fail(KNP_ERR_UNSUPPORTED, "BLOCKING ranks support up to 31 populations");
+    // projections expanded on MFMA: their dense blocks
+    std::vector<DenseBlocks> dense_info(n_projs);
+    for (uint32_t q = 0; q < n_projs; ++q)
+        if (net->projs[q].dense_format >= 0) {
+            int rc = dense_blocks(net, net->projs[q], dense_info[q]);
+            if (rc) return rc;
+        }
     for (const auto &Dh : net->dense) {
         uint64_t row = 0;
         for (uint32_t k = 0; k < Dh.n_inst; ++k)
@@ -757,7 +855,8 @@ int init_network(knp_net *net) {
                   !H.has_block && H.D <= tile_max_d && H.n > 0 && H.n < (1u << 23);
     }
     for (uint32_t q = 0; q < n_projs; ++q)
-        if (net->projs[q].kind != PROJ_DELTA) net->pops[net->projs[q].post_pop].tiled = false;
+        if (net->projs[q].kind != PROJ_DELTA || net->projs[q].dense_format >= 0)
+            net->pops[net->projs[q].post_pop].tiled = false;  // (MFMA expansions add into the ring in HBM)
     {
         std::vector<uint64_t> dir_entries(n_pops, 0);
         for (uint32_t q = 0; q < n_projs; ++q) {
@@ -1011,6 +1110,40 @@ int init_network(knp_net *net) {
                 Q.stdp_mode[m] = H.stdp_modes[m];
             }
         }
+        if (H.dense_format >= 0) {
+            // MFMA expansion: instance table, index maps, tiles, the BF16 weight copy and the spike counts
+            const DenseBlocks &B = dense_info[q];
+            std::vector<uint32_t> tiles;
+            for (uint32_t k = 0; k < B.inst.size(); ++k)
+                for (uint32_t j = 0; j < B.inst[k].post_count; j += 32) tiles.push_back(k << 16 | j / 32);
+            const DenseInst &L = B.inst.back();
+            std::vector<uint16_t> wb(L.base + (uint64_t)L.n_oct * L.npad * 8, 0);
+            for (uint64_t s2 = 0; s2 < S; ++s2) {
+                const DenseInst &I = B.inst[B.post_inst[H.post[s2]]];
+                const uint32_t i = H.pre[s2] - I.pre_begin, j = H.post[s2] - I.post_begin;
+                wb[I.base + ((uint64_t)(i >> 3) * I.npad + j) * 8 + (i & 7)] = bf16_rne(H.w[s2]);
+            }
+            DenseInst *it;
+            uint32_t *pi, *po, *tl;
+            uint16_t *wd;
+            if ((rc = upload(net, &it, B.inst.data(), B.inst.size()))) return rc;
+            if ((rc = upload(net, &pi, B.pre_inst.data(), B.pre_inst.size()))) return rc;
+            if ((rc = upload(net, &po, B.post_inst.data(), B.post_inst.size()))) return rc;
+            if ((rc = upload(net, &tl, tiles.data(), tiles.size()))) return rc;
+            if ((rc = upload(net, &wd, wb.data(), wb.size()))) return rc;
+            if ((rc = dev_alloc(net, (void **)&Q.dn_cnt, (size_t)2 * B.n_oct * 8))) return rc;
+            Q.dense = 1 + H.dense_flags;
+            Q.dn_oct = B.n_oct;
+            Q.dn_inst = (uint32_t)B.inst.size();
+            Q.dn_tiles = (uint32_t)tiles.size();
+            Q.dn_delay = B.delay;
+            Q.dn_sign = B.sign;
+            Q.dn_inst_tab = it;
+            Q.dn_pre_inst = pi;
+            Q.dn_post_inst = po;
+            Q.dn_tile = tl;
+            Q.dn_w = wd;
+        }
         // channel messages of one step: every index at most 4 times (checked when pushed), so this capacity
         // is never exceeded and the list never moves (tile tables hold its address)
         Q.in_cap = 4 * std::max(H.n_pre, 1u);
@@ -1127,19 +1260,28 @@ int init_network(knp_net *net) {
             for (uint32_t c : reach)
                 if (c > (uint32_t)kTileMaxList) net->pops[p].tiled = false;
         }
-        // virtual segment order: the atomic scatter's segments first, then the tiled populations'
+        // virtual segment order: the atomic scatter's segments first, then those of the projections expanded on
+        // MFMA (each projection's contiguous), then the tiled populations'
+        auto vclass = [&](size_t k) {
+            if (net->projs[vd[k].q].dense_format >= 0) return 1;
+            return net->pops[net->projs[vd[k].q].post_pop].tiled ? 2 : 0;
+        };
         std::vector<uint32_t> vseg, vrs, vidx(vd.size());
-        for (int pass = 0; pass < 2; ++pass)
+        for (int pass = 0; pass < 3; ++pass)
             for (size_t k = 0; k < vd.size(); ++k) {
-                const bool tl = net->pops[net->projs[vd[k].q].post_pop].tiled;
-                if (tl != (pass == 1)) continue;
+                if (vclass(k) != pass) continue;
                 vidx[k] = (uint32_t)vseg.size();
+                ProjDev &Qv = T.projs[vd[k].q];
+                if (pass == 1) {
+                    if (Qv.dn_v1 == Qv.dn_v0) Qv.dn_v0 = (uint32_t)vseg.size();
+                    Qv.dn_v1 = (uint32_t)vseg.size() + 1;
+                }
                 vseg.push_back(vd[k].seg);
                 vrs.push_back(vd[k].rs);
             }
         uint32_t n_atomic = 0;
         for (size_t k = 0; k < vd.size(); ++k)
-            if (!net->pops[net->projs[vd[k].q].post_pop].tiled) ++n_atomic;
+            if (vclass(k) == 0) ++n_atomic;
         if (n_atomic > (uint32_t)kMaxVSeg) return fail(KNP_ERR_UNSUPPORTED, "too many spike segments for one step");
         T.n_vseg_atomic = n_atomic;
         std::vector<TileDev> tiles;
@@ -1471,6 +1613,22 @@ int run_steps(knp_net *net, uint64_t n_steps) {
             if (net->projs[q].kind != PROJ_ADDITIVE || net->projs[q].stdp_pops.empty()) continue;
             k_additive_append<<<net->scatter_grid, kBlock, 0, net->stream>>>(D, q, t);
             HIP_TRY(hipGetLastError());
+        }
+        // projections expanded on MFMA (after training: they use the step's trained weights)
+        for (uint32_t q = 0; q < net->projs.size(); ++q) {
+            const ProjDev &Qd = net->table.projs[q];
+            if (!Qd.dense) continue;
+            const uint64_t words = (uint64_t)Qd.dn_oct * 2;
+            const uint32_t gc = (uint32_t)std::max<uint64_t>(
+                1, std::min<uint64_t>(1024, std::max<uint64_t>(Qd.dn_v1 - Qd.dn_v0, (words + kBlock - 1) / kBlock)));
+            const uint32_t ge = (Qd.dn_tiles + 3) / 4;
+            const bool skip = (Qd.dense - 1) & kDenseSkipEmpty;
+            if ((rc = timed_launch(net, KNP_KERNEL_DENSE_EXPAND, [&] {
+                     k_dense_counts<<<gc, kBlock, 0, net->stream>>>(D, q, t);
+                     if (skip) k_dense_expand<true><<<ge, kBlock, 0, net->stream>>>(D, q, t);
+                     else k_dense_expand<false><<<ge, kBlock, 0, net->stream>>>(D, q, t);
+                 })))
+                return rc;
         }
         const uint32_t sg = net->scatter_grid;
         if (net->table.n_vseg_atomic &&
@@ -2384,6 +2542,42 @@ int knp_mi355x_set_wait_test(knp_net *net, uint32_t hold_block, uint32_t spin_li
     net->table.spin_limit = spin_limit;
     net->table.hold_until = hold_until;
     return upload_table(net);
+}
+
+int knp_mi355x_set_projection_dense(knp_net *net, uint32_t proj, int format, uint32_t flags) {
+    int rc = check_proj(net, proj);
+    if (rc) return rc;
+    if (net->initialized) return fail(KNP_ERR_STATE, "set the MFMA expansion before the first step");
+    if (format != KNP_DENSE_BF16) return fail(KNP_ERR_UNSUPPORTED, "the MFMA expansion takes BF16 weights");
+    if (flags & ~(uint32_t)KNP_DENSE_SKIP_EMPTY) return fail(KNP_ERR_INVALID_ARGUMENT, "unknown flags");
+    const HostProj &H = net->projs[proj];
+    if (H.kind == PROJ_ADDITIVE) return fail(KNP_ERR_UNSUPPORTED, "additive STDP projections are not expanded on MFMA");
+    net->projs[proj].dense_format = format;
+    net->projs[proj].dense_flags = flags;
+    return KNP_OK;
+}
+
+int knp_mi355x_read_dense_weights(knp_net *net, uint32_t proj, uint16_t *out, uint64_t n_syn) {
+    int rc = check_proj(net, proj);
+    if (rc) return rc;
+    const HostProj &H = net->projs[proj];
+    if (n_syn != H.n_syn || (n_syn && !out)) return fail(KNP_ERR_INVALID_ARGUMENT, "bad output buffer size");
+    if (H.dense_format < 0) return fail(KNP_ERR_INVALID_ARGUMENT, "projection not expanded on MFMA");
+    if ((rc = ensure_init(net))) return rc;
+    const ProjDev &Q = net->table.projs[proj];
+    std::vector<DenseInst> inst(Q.dn_inst);
+    std::vector<uint32_t> post_inst(net->pops[H.post_pop].n);
+    if ((rc = download(net, inst.data(), Q.dn_inst_tab, inst.size()))) return rc;
+    if ((rc = download(net, post_inst.data(), Q.dn_post_inst, post_inst.size()))) return rc;
+    const DenseInst &L = inst.back();
+    std::vector<uint16_t> wb(L.base + (uint64_t)L.n_oct * L.npad * 8);
+    if ((rc = download(net, wb.data(), Q.dn_w, wb.size()))) return rc;
+    for (uint64_t s = 0; s < n_syn; ++s) {
+        const DenseInst &I = inst[post_inst[H.post[s]]];
+        const uint32_t i = H.pre[s] - I.pre_begin, j = H.post[s] - I.post_begin;
+        out[s] = wb[I.base + ((uint64_t)(i >> 3) * I.npad + j) * 8 + (i & 7)];
+    }
+    return KNP_OK;
 }
 
 int knp_mi355x_set_scatter_mode(knp_net *net, int mode) {

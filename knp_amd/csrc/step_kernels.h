@@ -1361,6 +1361,20 @@ __global__ __launch_bounds__(kTileThreads) void k_tile_run(const NetDev *__restr
 }
 
 // ---- synaptic-resource STDP (blifat_stdp.h:35-172, training/stdp.h:40-186) ---------------------------
+// float -> bf16, rounded to nearest even (NaN stays a NaN).
+__host__ __device__ inline uint16_t bf16_rne(float f) {
+    const uint32_t u = __builtin_bit_cast(uint32_t, f);
+    if ((u & 0x7FFFFFFFu) > 0x7F800000u) return (uint16_t)((u >> 16) | 0x40u);
+    return (uint16_t)((u + 0x7FFFu + ((u >> 16) & 1u)) >> 16);
+}
+
+// Position of the weight of synapse (pre -> post) in a dense projection's BF16 copy.
+__device__ __forceinline__ size_t dense_weight_pos(const ProjDev &Q, uint32_t post, uint32_t pre) {
+    const DenseInst &I = Q.dn_inst_tab[gp(Q.dn_post_inst)[post]];
+    const uint32_t i = pre - I.pre_begin, j = post - I.post_begin;
+    return I.base + ((size_t)(i >> 3) * I.npad + j) * 8 + (i & 7);
+}
+
 __device__ __forceinline__ float recalc_weight(float r, float w_min, float w_max) {
     // recalculate_synapse_weights (training/stdp.h:40-53)
     const float syn_w = r > 0.f ? r : 0.f;
@@ -1514,7 +1528,9 @@ __global__ __launch_bounds__(kBlock) void k_train(const NetDev *__restrict__ net
                                 }
                                 Q.r[sy[u]] = rr;
                             }
-                            Q.W[sy[u]] = recalc_weight(rr, wmin[u], wmax[u]);
+                            const float wn = recalc_weight(rr, wmin[u], wmax[u]);
+                            Q.W[sy[u]] = wn;
+                            if (Q.dense) gp(Q.dn_w)[dense_weight_pos(Q, n, gp(Q.col_pre)[k])] = bf16_rne(wn);
                             Q.flags[sy[u]] = (uint8_t)((hc ? 2 : 0) | (hh ? 1 : 0));
                         } else {
                             du[u] = 0.f;
@@ -1569,7 +1585,9 @@ __global__ __launch_bounds__(kBlock) void k_train(const NetDev *__restrict__ net
                                 Q.r[sy[u]] = rr;
                                 contributing = true;
                             }
-                            Q.W[sy[u]] = recalc_weight(rr, wmin[u], wmax[u]);
+                            const float wn = recalc_weight(rr, wmin[u], wmax[u]);
+                            Q.W[sy[u]] = wn;
+                            if (Q.dense) gp(Q.dn_w)[dense_weight_pos(Q, n, gp(Q.col_pre)[k])] = bf16_rne(wn);
                         }
                         cm[u] = __ballot(contributing);
                     }
@@ -1604,7 +1622,9 @@ __global__ __launch_bounds__(kBlock) void k_train(const NetDev *__restrict__ net
                     const uint32_t s = Q.col_syn[k];
                     const float r = Q.r[s] + add;
                     Q.r[s] = r;
-                    Q.W[s] = recalc_weight(r, Q.w_min[s], Q.w_max[s]);
+                    const float wn = recalc_weight(r, Q.w_min[s], Q.w_max[s]);
+                    Q.W[s] = wn;
+                    if (Q.dense) gp(Q.dn_w)[dense_weight_pos(Q, n, gp(Q.col_pre)[k])] = bf16_rne(wn);
                 }
             }
             freer = 0.f;
@@ -1825,6 +1845,127 @@ __global__ __launch_bounds__(kBlock) void k_dense_mfma(const NetDev *__restrict_
                 }
             }
         }
+    }
+}
+
+
+// ---- projections expanded on MFMA (ProjDev::dense) ---------------------------------------------------
+// A projection whose synapses form dense blocks (instances) is expanded each step as a product of the
+// instance's spike counts (K = presynaptic indexes) and a BF16 copy of its weights (N = postsynaptic
+// neurons) on the matrix cores, with FP32 accumulation; training keeps the FP32 weights and refreshes the copy.
+
+// Spike counts of a dense projection for step t: every spiking presynaptic index in the projection's virtual
+// segments adds 1 to its byte (parity t & 1); the counts of parity (t + 1) & 1, read by the previous step's
+// product, are cleared for the next step. Synaptic-resource projections note the step each row spiked
+// (init_synapse, synaptic_resource_stdp.h:36-62). Counts the impacts (the instance's neurons per spike) and rows.
+__global__ __launch_bounds__(kBlock) void k_dense_counts(const NetDev *__restrict__ net, uint32_t q, uint64_t t) {
+    const ProjDev &Q = net->projs[q];
+    const size_t words = (size_t)Q.dn_oct * 2;
+    uint32_t *cur = Q.dn_cnt + (size_t)(t & 1) * words;
+    uint32_t *nxt = Q.dn_cnt + (size_t)((t + 1) & 1) * words;
+    for (size_t k = (size_t)blockIdx.x * kBlock + threadIdx.x; k < words; k += (size_t)gridDim.x * kBlock) gp(nxt)[k] = 0;
+    const uint32_t *vc = net->vcount + (size_t)(t & 1) * net->n_vseg;
+    const size_t par = (size_t)(t & 1) * net->list_pstride;
+    unsigned long long ev = 0, rows = 0;
+    for (uint32_t v = Q.dn_v0 + blockIdx.x; v < Q.dn_v1; v += gridDim.x) {
+        const uint32_t cnt = gp(vc)[v];
+        const uint32_t *list = net->seg_list[net->vseg[v]] + par + 1;
+        for (uint32_t e = threadIdx.x; e < cnt; e += kBlock) {
+            const uint32_t pre = gp(list)[e];
+            if (pre >= Q.n_pre) continue;
+            const uint32_t k = gp(Q.dn_pre_inst)[pre];
+            if (k == 0xFFFFFFFFu) continue;
+            const DenseInst &I = Q.dn_inst_tab[k];
+            const uint32_t i = pre - I.pre_begin;
+            atomicAdd(&cur[(size_t)I.cbase * 2 + (i >> 2)], 1u << (8u * (i & 3u)));
+            if (Q.kind == PROJ_RESOURCE) gp(Q.row_spike)[pre] = t;
+            ev += I.post_count;
+            rows += 1;
+        }
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        ev += __shfl_xor(ev, off);
+        rows += __shfl_xor(rows, off);
+    }
+    if (lane_id() == 0 && rows) {
+        atomicAdd(&net->events[0], ev);
+        atomicAdd(&net->events[1], rows);
+    }
+}
+
+// 8 spike counts (one byte each, below 256) to 8 bf16 values (exact).
+__device__ __forceinline__ v8s counts_to_bf16(uint64_t c) {
+    typedef unsigned int v4w __attribute__((ext_vector_type(4)));
+    v4w w;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const uint32_t a = (uint32_t)(c >> (16 * k)) & 0xFFu, b = (uint32_t)(c >> (16 * k + 8)) & 0xFFu;
+        const uint32_t fa = __builtin_bit_cast(uint32_t, (float)a) >> 16, fb = __builtin_bit_cast(uint32_t, (float)b) >> 16;
+        w[k] = fa | fb << 16;
+    }
+    return __builtin_bit_cast(v8s, w);
+}
+
+// One wave per 32-neuron tile of an instance: C (32 x 32) += A (16 K: spike counts, the same in every row) x
+// B (16 K x 32 neurons: BF16 weights) over the instance's octets, two octets per K-step (lanes 0-31 the first,
+// 32-63 the second), four K-steps per round with their loads issued first. SKIP: only octets holding a spike.
+// Every column of C is the neuron's sum; lanes 0-31 add register 0 (row 0) to the ring as fixed point.
+template <bool SKIP>
+__global__ __launch_bounds__(kBlock) void k_dense_expand(const NetDev *__restrict__ net, uint32_t q, uint64_t t) {
+    const ProjDev &Q = net->projs[q];
+    const uint32_t tile = blockIdx.x * (kBlock / 64) + threadIdx.x / 64;
+    if (tile >= Q.dn_tiles) return;
+    const PopDev &P = net->pops[Q.post_pop];
+    const uint32_t lane = lane_id(), h = lane >> 5, col = lane & 31;
+    const uint32_t tv = gp(Q.dn_tile)[tile], k = tv >> 16, j0 = (tv & 0xFFFFu) * 32;
+    const DenseInst I = Q.dn_inst_tab[k];
+    const uint32_t n_oct = I.n_oct;
+    const KNP_GLOBAL uint64_t *cnt = (const KNP_GLOBAL uint64_t *)gp(Q.dn_cnt) + (size_t)(t & 1) * Q.dn_oct + I.cbase;
+    const KNP_GLOBAL uint16_t *wb = gp(Q.dn_w) + I.base + (size_t)(j0 + col) * 8;
+    const size_t ostride = (size_t)I.npad * 8;
+    constexpr uint32_t kNone = 0xFFFFFFFFu;
+    v16f acc;
+#pragma unroll
+    for (int v = 0; v < 16; ++v) acc[v] = 0.f;
+    const v8s zero = {0, 0, 0, 0, 0, 0, 0, 0};
+    for (uint32_t c0 = 0; c0 < n_oct; c0 += 64) {
+        const uint32_t o = c0 + lane;
+        const uint64_t mine = o < n_oct ? cnt[o] : 0ull;
+        uint64_t mask = SKIP ? __ballot(mine != 0ull) : __ballot(o < n_oct);
+        while (mask) {
+            uint32_t oct[4];
+#pragma unroll
+            for (int g = 0; g < 4; ++g) {
+                uint32_t a = kNone, b = kNone;
+                if (mask) {
+                    a = (uint32_t)__ffsll((unsigned long long)mask) - 1u;
+                    mask &= mask - 1ull;
+                }
+                if (mask) {
+                    b = (uint32_t)__ffsll((unsigned long long)mask) - 1u;
+                    mask &= mask - 1ull;
+                }
+                oct[g] = h ? b : a;
+            }
+            v8s frag[4];
+            uint64_t cv[4];
+#pragma unroll
+            for (int g = 0; g < 4; ++g) {
+                const uint32_t my = oct[g];
+                cv[g] = __shfl(mine, (int)(my & 63u));
+                if (my == kNone) cv[g] = 0ull;
+                frag[g] = my != kNone ? *(const KNP_GLOBAL v8s *)(wb + (size_t)(c0 + my) * ostride) : zero;
+            }
+#pragma unroll
+            for (int g = 0; g < 4; ++g)
+                acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(counts_to_bf16(cv[g]), frag[g], acc, 0, 0, 0);
+        }
+    }
+    if (h == 0 && j0 + col < I.post_count) {
+        const uint32_t n = I.post_begin + j0 + col;
+        const size_t ai = (size_t)((t + Q.dn_delay) % P.D) * P.n + n;
+        const long long v = __float2ll_rn(acc[0] * P.scale);
+        gp(P.acc_v)[ai] += Q.dn_sign < 0.f ? -v : v;
     }
 }
 

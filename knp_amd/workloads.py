@@ -35,6 +35,7 @@ class NetworkSpec:
         self.proj_sizes = []
         self.learning = False
         self.meta = {}
+        self.device_ops = []  # execution choices of the device backend (the oracle has none)
 
     # recipe construction (returns indexes in load order, like the targets do)
     def add_blifat_population(self, neurons):
@@ -68,11 +69,19 @@ class NetworkSpec:
     def start_learning(self):
         self.learning = True
 
-    def load_into(self, target):
+    def set_projection_dense(self, proj, fmt=abi.DENSE_BF16, flags=0):
+        """Device execution choice: expand projection `proj` on MFMA (BF16 weight copy)."""
+        self.device_ops.append(("set_projection_dense", (proj, fmt, flags), {}))
+
+    def load_into(self, target, device_options=True):
         for name, args, kw in self.ops:
             getattr(target, name)(*args, **kw)
         if self.learning:
             target.start_learning()
+        if device_options:
+            for name, args, kw in self.device_ops:
+                if hasattr(target, name):
+                    getattr(target, name)(*args, **kw)
         return target
 
     @property
@@ -198,8 +207,9 @@ def stdp_learning(seed=3, n_images=2000, steps_per_image=20, n_in=784, n_out=102
     dop = spec.add_delta_projection(abi.KNP_NO_POPULATION, 1, pop, np.zeros(n_out, np.uint32),
                                     np.arange(n_out, dtype=np.uint32), np.full(n_out, dopamine_w, np.float32),
                                     np.ones(n_out, np.uint32), np.full(n_out, abi.DOPAMINE, np.uint8))
-    images = blob_images(rng, n_images)
-    thr = bernoulli_threshold(p_max * images.reshape(n_images, n_in).astype(np.float64) / 255.0)
+    side = int(np.ceil(np.sqrt(n_in)))  # 28 for the 784 inputs of the configuration
+    images = blob_images(rng, n_images, side).reshape(n_images, side * side)[:, :n_in]
+    thr = bernoulli_threshold(p_max * images.astype(np.float64) / 255.0)
     spec.add_bernoulli_source(inp, thr, steps_per_pattern=steps_per_image, seed=src_seed,
                               hash_offset=instance * n_in)
     spec.add_bernoulli_source(dop, bernoulli_threshold([dopamine_p]), steps_per_pattern=steps_per_image,
@@ -216,15 +226,24 @@ def _op(spec, name):
     return [(a, kw) for n, a, kw in spec.ops if n == name]
 
 
-def batched_stdp(batch=256, n_images=250, steps_per_image=20, seed=30, **kw):
+def batched_stdp(batch=256, n_images=250, steps_per_image=20, seed=30, dense=None, **kw):
     """Config 3: `batch` independent copies of the config-2 network (own weights from seed + b, own image
     streams), composed into one device network: one population of batch x 1,024 neurons and block-diagonal
     projections, so that every instance advances in the same kernel launches. Instance b alone is
-    stdp_learning(seed=seed + b, source_seed=..., instance=b) - the same synapse values and input streams."""
+    stdp_learning(seed=seed + b, source_seed=..., instance=b) - the same synapse values and input streams.
+    dense: None (every projection on the event-driven scatter, FP32), "full" (the plastic input projection
+    expanded on MFMA as the instances' spike counts x their BF16 weight copies over all 784 inputs) or "skip"
+    (the same, multiplying only the groups of 8 inputs that hold a spike); with either, the lateral projection
+    (weight 0.5, exact in BF16) is expanded on MFMA too, over the groups of neurons that spiked."""
     src_seed = seed * 7919 + 17
     parts = [stdp_learning(seed=seed + b, n_images=n_images, steps_per_image=steps_per_image, source_seed=src_seed,
                            instance=b, **kw) for b in range(batch)]
-    return compose(parts, "batched_stdp", per_instance_steps=n_images * steps_per_image)
+    spec = compose(parts, "batched_stdp", per_instance_steps=n_images * steps_per_image)
+    if dense is not None:
+        meta = parts[0].meta
+        spec.set_projection_dense(meta["input_proj"], abi.DENSE_BF16, abi.DENSE_SKIP_EMPTY if dense == "skip" else 0)
+        spec.set_projection_dense(meta["lateral_proj"], abi.DENSE_BF16, abi.DENSE_SKIP_EMPTY)
+    return spec
 
 
 def compose(parts, name, per_instance_steps):
