@@ -18,6 +18,8 @@ import platform
 import sys
 import time
 
+import numpy as np
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
@@ -54,6 +56,13 @@ def main():
                          "projection on MFMA")
     ap.add_argument("--batch", type=int, default=256, help="c3: instances")
     ap.add_argument("--agree-steps", type=int, default=400, help="c2/c3/c4: steps compared with the FP64 oracle")
+    ap.add_argument("--agree-instances", type=int, default=4, help="c3: instances compared with the FP64 oracle")
+    ap.add_argument("--dense", choices=["full", "skip", "none"], default="full",
+                    help="c3: the plastic input projection expanded on MFMA over all inputs (full: binary spikes x "
+                         "BF16 weight copies), over the groups of 8 inputs holding a spike (skip), or by the FP32 "
+                         "event-driven scatter (none)")
+    ap.add_argument("--split", action="store_true",
+                    help="c3: the MFMA weight copies as two BF16 parts each (hi + lo, 16 significant bits)")
     ap.add_argument("--scatter", choices=["auto", "stepwise", "atomic"], default="auto",
                     help="impact summation: tiled populations and persistent launches where eligible (auto), "
                          "tiled with one launch per step (stepwise), or atomics everywhere")
@@ -347,6 +356,41 @@ def compare_f64(spec, gpu_rec, gpu_pot, pop, steps, neurons=None):
             "spikes_gpu": int(gpu_rec.sum()), "potential_rel_err_max": pmax, "potential_rel_err_mean": pmean}
 
 
+def _f64_worker(job):
+    from oracle.pyoracle import Oracle
+
+    spec, steps = job
+    o = spec.load_into(Oracle("f64"))
+    o.record_spikes(0, steps)
+    o.step(steps)
+    out = (o.read_spike_record(0), o.potentials(0))
+    o.close()
+    return out
+
+
+def f64_runs(specs, labels, steps):
+    """FP64 oracle runs of single-instance specs, in parallel threads (the oracle's step runs without the
+    interpreter lock): [(spike record, potentials)]."""
+    from multiprocessing.pool import ThreadPool
+
+    t0 = time.perf_counter()
+    n = max(1, min(len(specs), 8, os.cpu_count() or 1))
+    with ThreadPool(n) as pool:
+        out = pool.map(_f64_worker, [(s, steps) for s in specs])
+    log(f"fp64 oracle: {len(specs)} instances x {steps} steps in {time.perf_counter() - t0:.1f}s")
+    return out
+
+
+def agreement_report(g_rec, o_rec, g_pot, o_pot, steps):
+    agr, ham = agreement(g_rec[:steps], o_rec[:steps])
+    first = np.argwhere(g_rec[:steps] != o_rec[:steps])
+    pmax, pmean = pot_error(g_pot, o_pot)
+    return {"spike_agreement": round(agr, 6), "hamming_accuracy": round(ham, 6),
+            "spikes_gpu": int(g_rec[:steps].sum()), "spikes_oracle": int(o_rec[:steps].sum()),
+            "first_differing_step": int(first[0][0]) if len(first) else None,
+            "potential_rel_err_max_at_end": pmax, "potential_rel_err_mean_at_end": pmean}
+
+
 def other_workload(args):
     """Measurement lines of the other configurations (not the headline line)."""
     import numpy as np
@@ -424,46 +468,79 @@ def other_workload(args):
                                   "net_steps_per_s": steps / tcpu, "sample": f"all {steps} steps"}, **others)
         o.close()
     elif args.workload in ("c2", "c3"):
-        steps = args.steps if args.steps != 10_000 else 2000
-        warm = max(args.warmup, args.agree_steps)  # both runs time the same step range
+        steps = args.steps if args.steps != 10_000 else 1000
+        agree = args.agree_steps
+        warm = max(args.warmup, agree)  # the plain and the batched runs time the same step range
         n_img = (warm + steps) // 20 + 1
-        # the plain single-instance path (the denominator of the batching ratio)
-        spec2 = workloads.stdp_learning(n_images=n_img)
+        src_seed = 30 * 7919 + 17
+        # the plain FP32 path, one instance (the batch's instance 0): the denominator of the batching ratio
+        spec2 = workloads.stdp_learning(seed=30, n_images=n_img, source_seed=src_seed, instance=0)
         g2 = load(spec2)
-        g2.step(warm)
+        g2.record_spikes(0, agree)
+        g2.step(agree)
+        rec2, pot2 = g2.read_spike_record(0), g2.potentials(0)
+        g2.step(warm - agree)
         dt2, ev2 = timed_steps(g2, steps)
         k2 = kernels(g2, min(steps, 500))
         g2.close()
-        line.update(c2={"net_steps_per_s": steps / dt2, "events_per_s": ev2 / dt2, "steps": steps,
-                        "ms_per_step": dt2 / steps * 1e3, "kernels": k2})
+        c2_line = {"net_steps_per_s": steps / dt2, "events_per_s": ev2 / dt2, "steps": steps,
+                   "ms_per_step": dt2 / steps * 1e3, "kernels": k2, "path": "fp32 event-driven scatter"}
+        log(f"c2: {steps / dt2:.0f} steps/s")
         if args.workload == "c2":
-            g = load(spec2)
-            g.record_spikes(0, args.agree_steps)
-            g.step(args.agree_steps)
+            f64 = f64_runs([spec2], [0], agree)[0]
             line.update(metric="network steps/s", value=steps / dt2, unit="net-steps/s", dtype="fp32",
-                        fp64=compare_f64(spec2, g.read_spike_record(0), g.potentials(0), 0, args.agree_steps))
-            g.close()
+                        fp64=agreement_report(rec2, f64[0], pot2, f64[1], agree), **c2_line)
         else:
             B = args.batch
             tg = time.perf_counter()
-            spec3 = workloads.batched_stdp(batch=B, n_images=n_img)
+            spec3 = workloads.batched_stdp(batch=B, n_images=n_img, dense=None if args.dense == "none" else args.dense,
+                                           split=args.split)
             log(f"generated {B} instances ({spec3.n_synapses} synapses) in {time.perf_counter() - tg:.1f}s")
+            tl = time.perf_counter()
             g = load(spec3)
-            g.record_spikes(0, args.agree_steps)
-            g.step(args.agree_steps)
-            rec0 = g.read_spike_record(0)[:, :1024]
-            pot0 = g.potentials(0)[:1024]
-            g.step(warm - args.agree_steps)
+            log(f"loaded in {time.perf_counter() - tl:.1f}s")
+            check = list(range(min(B, args.agree_instances)))
+            g.record_spikes(0, agree)
+            g.step(agree)
+            rec = g.read_spike_record(0)
+            pot = g.potentials(0)
+            g.step(warm - agree)
             dt3, ev3 = timed_steps(g, steps)
+            wb0, full = g.dense_weight_bytes()
             k3 = kernels(g, min(steps, 500))
+            wb1, _ = g.dense_weight_bytes()
             g.close()
-            # instance 0 of the batch is the standalone network with the batch's seeds and streams
-            inst0 = workloads.stdp_learning(seed=30, n_images=n_img, source_seed=30 * 7919 + 17, instance=0)
+            ksteps = min(steps, 500)
             inst_steps_per_s = B * steps / dt3
-            line.update(metric="instance-steps/s", value=inst_steps_per_s, unit="instance-steps/s", dtype="fp32",
-                        batch=B, steps=steps, ms_per_step=dt3 / steps * 1e3, events_per_s=ev3 / dt3,
-                        ratio_to_single_instance=inst_steps_per_s / (steps / dt2), kernels=k3,
-                        fp64_instance0=compare_f64(inst0, rec0, pot0, 0, args.agree_steps))
+            parts = [workloads.stdp_learning(seed=30 + b, n_images=n_img, source_seed=src_seed, instance=b)
+                     for b in check]
+            f64 = f64_runs(parts, check, agree)
+            per = [agreement_report(rec[:, b * 1024:(b + 1) * 1024], f64[i][0], pot[b * 1024:(b + 1) * 1024],
+                                    f64[i][1], agree) for i, b in enumerate(check)]
+            allg = np.concatenate([rec[:, b * 1024:(b + 1) * 1024] for b in check], axis=1)
+            allo = np.concatenate([x[0] for x in f64], axis=1)
+            horizons = [h for h in (100, 500, 1000, 2000, 5000) if h <= agree]
+            batch_agr = {str(h): round(agreement(allg[:h], allo[:h])[0], 6) for h in horizons}
+            plain_agr = {str(h): round(agreement(rec2[:h], f64[0][0][:h])[0], 6) for h in horizons}
+            roof = None
+            if "dense_expand" in k3:
+                d = k3["dense_expand"]
+                bytes_step = full + (wb1 - wb0) / ksteps
+                ach = bytes_step / (d["us_per_step"] * 1e-6) / 1e9
+                roof = {"kernel": "k_dense_counts + k_dense_expand (both MFMA projections)", "bound": "hbm",
+                        "achieved": round(ach, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                        "frac": round(ach / HBM_PEAK_GBS, 5), "weight_bytes_per_step": bytes_step,
+                        "dense_equivalent_bytes_per_step": float(B * 784 * 1024 * (4 if args.split else 2)),
+                        "useful_mfma_flops_per_step": 2.0 * B * 784 * 1024,
+                        "us_per_step": d["us_per_step"]}
+            line.update(metric="instance-steps/s", value=inst_steps_per_s, unit="instance-steps/s",
+                        dtype="fp32" if args.dense == "none" else
+                        ("bf16 hi+lo" if args.split else "bf16") + " weights x spike counts, fp32 accumulate",
+                        dense=args.dense, split=args.split, batch=B, steps=steps, ms_per_step=dt3 / steps * 1e3,
+                        events_per_s=ev3 / dt3, ratio_to_plain_fp32=inst_steps_per_s / (steps / dt2),
+                        plain_fp32=c2_line, kernels=k3, dense_roofline=roof,
+                        fp64={"instances": check, "steps": agree, "spike_agreement_by_horizon": batch_agr,
+                              "plain_fp32_instance0_by_horizon": plain_agr, "per_instance": per})
     elif args.workload == "c4":
         steps = args.steps if args.steps != 10_000 else 2000
         tg = time.perf_counter()

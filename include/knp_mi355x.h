@@ -313,13 +313,21 @@ int knp_mi355x_get_dense_time_block(knp_net *net, uint32_t dense, uint32_t *time
  * presynaptic indexes (an all-to-all projection, or a block-diagonal batch of them). Synaptic-resource STDP
  * still trains the FP32 weights (read by knp_mi355x_read_weights) and refreshes the BF16 copy. The impacts
  * then differ from the reference's by the rounding of the weights to BF16 (8 significant bits) and of the
- * FP32 accumulation. flags: KNP_DENSE_SKIP_EMPTY multiplies only the groups of 8 presynaptic indexes that hold
+ * FP32 accumulation. flags (or-ed): KNP_DENSE_SKIP_EMPTY multiplies only the groups of 8 presynaptic indexes that hold
  * a spike (an exact shortcut: the others contribute 0). format: KNP_DENSE_BF16. Set before the first step;
  * the postsynaptic population is then summed on the atomic path. */
 #define KNP_DENSE_SKIP_EMPTY 1u
+/* KNP_DENSE_SPLIT: each weight is kept as two BF16 values, hi = bf16(w) and lo = bf16(w - hi) (16 significant
+ * bits), and multiplied twice; the impacts then differ from the reference's by about 2^-17 of a weight. */
+#define KNP_DENSE_SPLIT 2u
 int knp_mi355x_set_projection_dense(knp_net *net, uint32_t proj, int format, uint32_t flags);
-/* The BF16 weight copy of a projection expanded on MFMA, in synapse-index order (bf16 bit patterns). */
-int knp_mi355x_read_dense_weights(knp_net *net, uint32_t proj, uint16_t *out, uint64_t n_syn);
+/* Weight bytes the MFMA expansions read: so far by the KNP_DENSE_SKIP_EMPTY ones (a group of 8 presynaptic
+ * indexes with a spike reads 16 bytes per postsynaptic neuron of its instance, padded to 32), and per step by
+ * the others (every group). */
+int knp_mi355x_get_dense_weight_bytes(knp_net *net, uint64_t *skip_bytes, uint64_t *full_bytes_per_step);
+/* The BF16 weight copy of a projection expanded on MFMA, in synapse-index order (bf16 bit patterns); out_lo
+ * (may be NULL) receives the low parts of a KNP_DENSE_SPLIT copy (0 otherwise). */
+int knp_mi355x_read_dense_weights(knp_net *net, uint32_t proj, uint16_t *out, uint16_t *out_lo, uint64_t n_syn);
 
 /* ---- stepping (Backend::_step, backend.cpp:89-141) ------------------------------------------------ */
 int knp_mi355x_step(knp_net *net, uint64_t n_steps);

@@ -54,7 +54,8 @@ SCATTER_STEPWISE = 2
 # knp_dense_format
 DENSE_FP8_E4M3 = 0
 DENSE_BF16 = 1
-DENSE_SKIP_EMPTY = 1  # knp_mi355x_set_projection_dense flag
+DENSE_SKIP_EMPTY = 1  # knp_mi355x_set_projection_dense flags
+DENSE_SPLIT = 2
 
 SIZE_MAX = 0xFFFFFFFFFFFFFFFF
 INT64_MAX = 0x7FFFFFFFFFFFFFFF

@@ -61,7 +61,8 @@ def _declare(lib):
             i32, [P, u32, u32, pu32, pu32, u32, i32, C.c_void_p, pf32, u32, u64, u32, u32, pu32]),
         "knp_mi355x_get_dense_time_block": (i32, [P, u32, pu32]),
         "knp_mi355x_set_projection_dense": (i32, [P, u32, i32, u32]),
-        "knp_mi355x_read_dense_weights": (i32, [P, u32, C.POINTER(C.c_uint16), u64]),
+        "knp_mi355x_read_dense_weights": (i32, [P, u32, C.POINTER(C.c_uint16), C.POINTER(C.c_uint16), u64]),
+        "knp_mi355x_get_dense_weight_bytes": (i32, [P, C.POINTER(u64), C.POINTER(u64)]),
         "knp_mi355x_step": (i32, [P, u64]),
         "knp_mi355x_get_step": (i32, [P, C.POINTER(u64)]),
         "knp_mi355x_synchronize": (i32, [P]),
@@ -302,12 +303,22 @@ class MI355XBackend:
         """Expand projection `proj` on MFMA (spike counts x a BF16 copy of its weights; see the C ABI)."""
         self._chk(self.lib.knp_mi355x_set_projection_dense(self.net, proj, fmt, flags))
 
-    def dense_weights(self, proj):
-        """The BF16 weight copy of a projection expanded on MFMA (bit patterns, synapse-index order)."""
+    def dense_weights(self, proj, low=False):
+        """The BF16 weight copy of a projection expanded on MFMA (bit patterns, synapse-index order); with
+        low=True, (hi, lo) of a split copy."""
         n = self.proj_sizes[proj]
         out = np.zeros(max(n, 1), dtype=np.uint16)
-        self._chk(self.lib.knp_mi355x_read_dense_weights(self.net, proj, out.ctypes.data_as(C.POINTER(C.c_uint16)), n))
-        return out[:n]
+        lo = np.zeros(max(n, 1), dtype=np.uint16)
+        p16 = C.POINTER(C.c_uint16)
+        self._chk(self.lib.knp_mi355x_read_dense_weights(self.net, proj, out.ctypes.data_as(p16),
+                                                         lo.ctypes.data_as(p16) if low else None, n))
+        return (out[:n], lo[:n]) if low else out[:n]
+
+    def dense_weight_bytes(self):
+        """(bytes read so far by the skipping MFMA expansions, bytes per step of the full ones)."""
+        a, b = C.c_uint64(), C.c_uint64()
+        self._chk(self.lib.knp_mi355x_get_dense_weight_bytes(self.net, C.byref(a), C.byref(b)))
+        return a.value, b.value
 
     def dense_time_block(self, dense=0):
         t = C.c_uint32()

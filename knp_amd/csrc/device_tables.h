@@ -190,6 +190,7 @@ struct ProjDev {
     const uint32_t *dn_post_inst; // [n_post] instance of each postsynaptic neuron (UINT32_MAX: none)
     const uint32_t *dn_tile;      // [dn_tiles] instance << 16 | first neuron of the tile / 32
     uint16_t *dn_w;              // BF16 weights; instance k from base: [octet][neuron (npad)][8 inputs]
+    uint16_t *dn_w_lo;           // kDenseSplit: BF16 of w - dn_w (same layout), else null
     uint32_t *dn_cnt;            // [2][dn_oct][8] spikes of each presynaptic index (one byte each) in the
                                  // step, by step parity; instance k from octet cbase
 };
@@ -201,6 +202,7 @@ struct alignas(8) DenseInst {
     uint64_t base;  // first BF16 weight of the instance in dn_w
 };
 constexpr uint32_t kDenseSkipEmpty = 1;  // flag: multiply only the octets of presynaptic indexes that spiked
+constexpr uint32_t kDenseSplit = 2;      // flag: each weight as two BF16 values (hi + lo), two products
 
 // A part entry of a tile directory: start | length << 32 | delay-1 records << 56 (255: all of the part's
 // records count as delay 1, at 255 or more of them). Lengths stay below 2^24.

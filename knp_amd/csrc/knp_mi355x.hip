@@ -1117,12 +1117,16 @@ int init_network(knp_net *net) {
             for (uint32_t k = 0; k < B.inst.size(); ++k)
                 for (uint32_t j = 0; j < B.inst[k].post_count; j += 32) tiles.push_back(k << 16 | j / 32);
             const DenseInst &L = B.inst.back();
-            std::vector<uint16_t> wb(L.base + (uint64_t)L.n_oct * L.npad * 8, 0);
+            const bool split = H.dense_flags & KNP_DENSE_SPLIT;
+            std::vector<uint16_t> wb(L.base + (uint64_t)L.n_oct * L.npad * 8, 0), wlo(split ? wb.size() : 0, 0);
             for (uint64_t s2 = 0; s2 < S; ++s2) {
                 const DenseInst &I = B.inst[B.post_inst[H.post[s2]]];
                 const uint32_t i = H.pre[s2] - I.pre_begin, j = H.post[s2] - I.post_begin;
-                wb[I.base + ((uint64_t)(i >> 3) * I.npad + j) * 8 + (i & 7)] = bf16_rne(H.w[s2]);
+                const uint64_t pos = I.base + ((uint64_t)(i >> 3) * I.npad + j) * 8 + (i & 7);
+                wb[pos] = bf16_rne(H.w[s2]);
+                if (split) wlo[pos] = bf16_lo(H.w[s2], wb[pos]);
             }
+            if (split && (rc = upload(net, &Q.dn_w_lo, wlo.data(), wlo.size()))) return rc;
             DenseInst *it;
             uint32_t *pi, *po, *tl;
             uint16_t *wd;
@@ -1404,7 +1408,7 @@ int init_network(knp_net *net) {
     // ---- control blocks
     int rc;
     if ((rc = dev_alloc(net, (void **)&T.ctl, 2 * sizeof(Ctl)))) return rc;
-    if ((rc = dev_alloc(net, (void **)&T.events, 16))) return rc;
+    if ((rc = dev_alloc(net, (void **)&T.events, 32))) return rc;
 
     // ---- dense input projections
     T.n_dense = (uint32_t)net->dense.size();
@@ -1622,11 +1626,13 @@ int run_steps(knp_net *net, uint64_t n_steps) {
             const uint32_t gc = (uint32_t)std::max<uint64_t>(
                 1, std::min<uint64_t>(1024, std::max<uint64_t>(Qd.dn_v1 - Qd.dn_v0, (words + kBlock - 1) / kBlock)));
             const uint32_t ge = (Qd.dn_tiles + 3) / 4;
-            const bool skip = (Qd.dense - 1) & kDenseSkipEmpty;
+            const bool skip = (Qd.dense - 1) & kDenseSkipEmpty, split = Qd.dn_w_lo != nullptr;
             if ((rc = timed_launch(net, KNP_KERNEL_DENSE_EXPAND, [&] {
                      k_dense_counts<<<gc, kBlock, 0, net->stream>>>(D, q, t);
-                     if (skip) k_dense_expand<true><<<ge, kBlock, 0, net->stream>>>(D, q, t);
-                     else k_dense_expand<false><<<ge, kBlock, 0, net->stream>>>(D, q, t);
+                     if (skip && split) k_dense_expand<true, true><<<ge, kBlock, 0, net->stream>>>(D, q, t);
+                     else if (skip) k_dense_expand<true, false><<<ge, kBlock, 0, net->stream>>>(D, q, t);
+                     else if (split) k_dense_expand<false, true><<<ge, kBlock, 0, net->stream>>>(D, q, t);
+                     else k_dense_expand<false, false><<<ge, kBlock, 0, net->stream>>>(D, q, t);
                  })))
                 return rc;
         }
@@ -2549,7 +2555,7 @@ int knp_mi355x_set_projection_dense(knp_net *net, uint32_t proj, int format, uin
     if (rc) return rc;
     if (net->initialized) return fail(KNP_ERR_STATE, "set the MFMA expansion before the first step");
     if (format != KNP_DENSE_BF16) return fail(KNP_ERR_UNSUPPORTED, "the MFMA expansion takes BF16 weights");
-    if (flags & ~(uint32_t)KNP_DENSE_SKIP_EMPTY) return fail(KNP_ERR_INVALID_ARGUMENT, "unknown flags");
+    if (flags & ~(uint32_t)(KNP_DENSE_SKIP_EMPTY | KNP_DENSE_SPLIT)) return fail(KNP_ERR_INVALID_ARGUMENT, "unknown flags");
     const HostProj &H = net->projs[proj];
     if (H.kind == PROJ_ADDITIVE) return fail(KNP_ERR_UNSUPPORTED, "additive STDP projections are not expanded on MFMA");
     net->projs[proj].dense_format = format;
@@ -2557,7 +2563,28 @@ int knp_mi355x_set_projection_dense(knp_net *net, uint32_t proj, int format, uin
     return KNP_OK;
 }
 
-int knp_mi355x_read_dense_weights(knp_net *net, uint32_t proj, uint16_t *out, uint64_t n_syn) {
+int knp_mi355x_get_dense_weight_bytes(knp_net *net, uint64_t *skip_bytes, uint64_t *full_bytes_per_step) {
+    if (!net) return fail(KNP_ERR_INVALID_ARGUMENT, "null handle");
+    if (skip_bytes) *skip_bytes = 0;
+    if (full_bytes_per_step) *full_bytes_per_step = 0;
+    if (!net->initialized) return KNP_OK;
+    unsigned long long c[4];
+    int rc = download(net, c, net->table.events, 4);
+    if (rc) return rc;
+    uint64_t full = 0;
+    for (uint32_t q = 0; q < net->projs.size(); ++q) {
+        const ProjDev &Q = net->table.projs[q];
+        if (!Q.dense || ((Q.dense - 1) & kDenseSkipEmpty)) continue;
+        std::vector<DenseInst> inst(Q.dn_inst);
+        if ((rc = download(net, inst.data(), Q.dn_inst_tab, inst.size()))) return rc;
+        for (const auto &I : inst) full += (uint64_t)I.n_oct * I.npad * (Q.dn_w_lo ? 32 : 16);
+    }
+    if (skip_bytes) *skip_bytes = c[2];
+    if (full_bytes_per_step) *full_bytes_per_step = full;
+    return KNP_OK;
+}
+
+int knp_mi355x_read_dense_weights(knp_net *net, uint32_t proj, uint16_t *out, uint16_t *out_lo, uint64_t n_syn) {
     int rc = check_proj(net, proj);
     if (rc) return rc;
     const HostProj &H = net->projs[proj];
@@ -2570,12 +2597,18 @@ int knp_mi355x_read_dense_weights(knp_net *net, uint32_t proj, uint16_t *out, ui
     if ((rc = download(net, inst.data(), Q.dn_inst_tab, inst.size()))) return rc;
     if ((rc = download(net, post_inst.data(), Q.dn_post_inst, post_inst.size()))) return rc;
     const DenseInst &L = inst.back();
-    std::vector<uint16_t> wb(L.base + (uint64_t)L.n_oct * L.npad * 8);
+    std::vector<uint16_t> wb(L.base + (uint64_t)L.n_oct * L.npad * 8), wlo;
     if ((rc = download(net, wb.data(), Q.dn_w, wb.size()))) return rc;
+    if (out_lo && Q.dn_w_lo) {
+        wlo.resize(wb.size());
+        if ((rc = download(net, wlo.data(), Q.dn_w_lo, wlo.size()))) return rc;
+    }
     for (uint64_t s = 0; s < n_syn; ++s) {
         const DenseInst &I = inst[post_inst[H.post[s]]];
         const uint32_t i = H.pre[s] - I.pre_begin, j = H.post[s] - I.post_begin;
-        out[s] = wb[I.base + ((uint64_t)(i >> 3) * I.npad + j) * 8 + (i & 7)];
+        const uint64_t pos = I.base + ((uint64_t)(i >> 3) * I.npad + j) * 8 + (i & 7);
+        out[s] = wb[pos];
+        if (out_lo) out_lo[s] = wlo.empty() ? 0 : wlo[pos];
     }
     return KNP_OK;
 }

@@ -1368,11 +1368,24 @@ __host__ __device__ inline uint16_t bf16_rne(float f) {
     return (uint16_t)((u + 0x7FFFu + ((u >> 16) & 1u)) >> 16);
 }
 
+// The low BF16 part of a weight split in two: bf16(w - float(hi)) (w - float(hi) is exact in FP32).
+__host__ __device__ inline uint16_t bf16_lo(float f, uint16_t hi) {
+    return bf16_rne(f - __builtin_bit_cast(float, (uint32_t)hi << 16));
+}
+
+
 // Position of the weight of synapse (pre -> post) in a dense projection's BF16 copy.
 __device__ __forceinline__ size_t dense_weight_pos(const ProjDev &Q, uint32_t post, uint32_t pre) {
     const DenseInst &I = Q.dn_inst_tab[gp(Q.dn_post_inst)[post]];
     const uint32_t i = pre - I.pre_begin, j = post - I.post_begin;
     return I.base + ((size_t)(i >> 3) * I.npad + j) * 8 + (i & 7);
+}
+
+__device__ __forceinline__ void dense_store_weight(const ProjDev &Q, uint32_t post, uint32_t pre, float w) {
+    const size_t pos = dense_weight_pos(Q, post, pre);
+    const uint16_t hi = bf16_rne(w);
+    gp(Q.dn_w)[pos] = hi;
+    if (Q.dn_w_lo) gp(Q.dn_w_lo)[pos] = bf16_lo(w, hi);
 }
 
 __device__ __forceinline__ float recalc_weight(float r, float w_min, float w_max) {
@@ -1530,7 +1543,7 @@ __global__ __launch_bounds__(kBlock) void k_train(const NetDev *__restrict__ net
                             }
                             const float wn = recalc_weight(rr, wmin[u], wmax[u]);
                             Q.W[sy[u]] = wn;
-                            if (Q.dense) gp(Q.dn_w)[dense_weight_pos(Q, n, gp(Q.col_pre)[k])] = bf16_rne(wn);
+                            if (Q.dense) dense_store_weight(Q, n, gp(Q.col_pre)[k], wn);
                             Q.flags[sy[u]] = (uint8_t)((hc ? 2 : 0) | (hh ? 1 : 0));
                         } else {
                             du[u] = 0.f;
@@ -1587,7 +1600,7 @@ __global__ __launch_bounds__(kBlock) void k_train(const NetDev *__restrict__ net
                             }
                             const float wn = recalc_weight(rr, wmin[u], wmax[u]);
                             Q.W[sy[u]] = wn;
-                            if (Q.dense) gp(Q.dn_w)[dense_weight_pos(Q, n, gp(Q.col_pre)[k])] = bf16_rne(wn);
+                            if (Q.dense) dense_store_weight(Q, n, gp(Q.col_pre)[k], wn);
                         }
                         cm[u] = __ballot(contributing);
                     }
@@ -1624,7 +1637,7 @@ __global__ __launch_bounds__(kBlock) void k_train(const NetDev *__restrict__ net
                     Q.r[s] = r;
                     const float wn = recalc_weight(r, Q.w_min[s], Q.w_max[s]);
                     Q.W[s] = wn;
-                    if (Q.dense) gp(Q.dn_w)[dense_weight_pos(Q, n, gp(Q.col_pre)[k])] = bf16_rne(wn);
+                    if (Q.dense) dense_store_weight(Q, n, gp(Q.col_pre)[k], wn);
                 }
             }
             freer = 0.f;
@@ -1866,7 +1879,7 @@ __global__ __launch_bounds__(kBlock) void k_dense_counts(const NetDev *__restric
     for (size_t k = (size_t)blockIdx.x * kBlock + threadIdx.x; k < words; k += (size_t)gridDim.x * kBlock) gp(nxt)[k] = 0;
     const uint32_t *vc = net->vcount + (size_t)(t & 1) * net->n_vseg;
     const size_t par = (size_t)(t & 1) * net->list_pstride;
-    unsigned long long ev = 0, rows = 0;
+    unsigned long long ev = 0, rows = 0, oct_bytes = 0;
     for (uint32_t v = Q.dn_v0 + blockIdx.x; v < Q.dn_v1; v += gridDim.x) {
         const uint32_t cnt = gp(vc)[v];
         const uint32_t *list = net->seg_list[net->vseg[v]] + par + 1;
@@ -1877,7 +1890,11 @@ __global__ __launch_bounds__(kBlock) void k_dense_counts(const NetDev *__restric
             if (k == 0xFFFFFFFFu) continue;
             const DenseInst &I = Q.dn_inst_tab[k];
             const uint32_t i = pre - I.pre_begin;
-            atomicAdd(&cur[(size_t)I.cbase * 2 + (i >> 2)], 1u << (8u * (i & 3u)));
+            // the octet's 8 counts as one 64-bit word: the spike that makes it non-zero adds the weight bytes
+            // a KNP_DENSE_SKIP_EMPTY product reads for it
+            const unsigned long long old =
+                atomicAdd((unsigned long long *)&cur[((size_t)I.cbase + (i >> 3)) * 2], 1ull << (8u * (i & 7u)));
+            if (old == 0) oct_bytes += (unsigned long long)I.npad * (Q.dn_w_lo ? 32 : 16);
             if (Q.kind == PROJ_RESOURCE) gp(Q.row_spike)[pre] = t;
             ev += I.post_count;
             rows += 1;
@@ -1886,10 +1903,12 @@ __global__ __launch_bounds__(kBlock) void k_dense_counts(const NetDev *__restric
     for (int off = 32; off > 0; off >>= 1) {
         ev += __shfl_xor(ev, off);
         rows += __shfl_xor(rows, off);
+        oct_bytes += __shfl_xor(oct_bytes, off);
     }
     if (lane_id() == 0 && rows) {
         atomicAdd(&net->events[0], ev);
         atomicAdd(&net->events[1], rows);
+        if (Q.dense - 1 & kDenseSkipEmpty) atomicAdd(&net->events[2], oct_bytes);
     }
 }
 
@@ -1910,7 +1929,7 @@ __device__ __forceinline__ v8s counts_to_bf16(uint64_t c) {
 // B (16 K x 32 neurons: BF16 weights) over the instance's octets, two octets per K-step (lanes 0-31 the first,
 // 32-63 the second), four K-steps per round with their loads issued first. SKIP: only octets holding a spike.
 // Every column of C is the neuron's sum; lanes 0-31 add register 0 (row 0) to the ring as fixed point.
-template <bool SKIP>
+template <bool SKIP, bool SPLIT>
 __global__ __launch_bounds__(kBlock) void k_dense_expand(const NetDev *__restrict__ net, uint32_t q, uint64_t t) {
     const ProjDev &Q = net->projs[q];
     const uint32_t tile = blockIdx.x * (kBlock / 64) + threadIdx.x / 64;
@@ -1922,11 +1941,12 @@ __global__ __launch_bounds__(kBlock) void k_dense_expand(const NetDev *__restric
     const uint32_t n_oct = I.n_oct;
     const KNP_GLOBAL uint64_t *cnt = (const KNP_GLOBAL uint64_t *)gp(Q.dn_cnt) + (size_t)(t & 1) * Q.dn_oct + I.cbase;
     const KNP_GLOBAL uint16_t *wb = gp(Q.dn_w) + I.base + (size_t)(j0 + col) * 8;
+    const KNP_GLOBAL uint16_t *wl = SPLIT ? gp(Q.dn_w_lo) + I.base + (size_t)(j0 + col) * 8 : nullptr;
     const size_t ostride = (size_t)I.npad * 8;
     constexpr uint32_t kNone = 0xFFFFFFFFu;
-    v16f acc;
+    v16f acc, acc_lo;
 #pragma unroll
-    for (int v = 0; v < 16; ++v) acc[v] = 0.f;
+    for (int v = 0; v < 16; ++v) acc[v] = acc_lo[v] = 0.f;
     const v8s zero = {0, 0, 0, 0, 0, 0, 0, 0};
     for (uint32_t c0 = 0; c0 < n_oct; c0 += 64) {
         const uint32_t o = c0 + lane;
@@ -1947,7 +1967,7 @@ __global__ __launch_bounds__(kBlock) void k_dense_expand(const NetDev *__restric
                 }
                 oct[g] = h ? b : a;
             }
-            v8s frag[4];
+            v8s frag[4], frag_lo[4];
             uint64_t cv[4];
 #pragma unroll
             for (int g = 0; g < 4; ++g) {
@@ -1955,16 +1975,20 @@ __global__ __launch_bounds__(kBlock) void k_dense_expand(const NetDev *__restric
                 cv[g] = __shfl(mine, (int)(my & 63u));
                 if (my == kNone) cv[g] = 0ull;
                 frag[g] = my != kNone ? *(const KNP_GLOBAL v8s *)(wb + (size_t)(c0 + my) * ostride) : zero;
+                if (SPLIT) frag_lo[g] = my != kNone ? *(const KNP_GLOBAL v8s *)(wl + (size_t)(c0 + my) * ostride) : zero;
             }
 #pragma unroll
-            for (int g = 0; g < 4; ++g)
-                acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(counts_to_bf16(cv[g]), frag[g], acc, 0, 0, 0);
+            for (int g = 0; g < 4; ++g) {
+                const v8s a = counts_to_bf16(cv[g]);
+                acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, frag[g], acc, 0, 0, 0);
+                if (SPLIT) acc_lo = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, frag_lo[g], acc_lo, 0, 0, 0);
+            }
         }
     }
     if (h == 0 && j0 + col < I.post_count) {
         const uint32_t n = I.post_begin + j0 + col;
         const size_t ai = (size_t)((t + Q.dn_delay) % P.D) * P.n + n;
-        const long long v = __float2ll_rn(acc[0] * P.scale);
+        const long long v = __float2ll_rn((SPLIT ? acc[0] + acc_lo[0] : acc[0]) * P.scale);
         gp(P.acc_v)[ai] += Q.dn_sign < 0.f ? -v : v;
     }
 }

@@ -6,6 +6,7 @@
 // create_knp_backend alias that BackendLoader resolves (backend.cpp:340 of the reference ST backend).
 #pragma once
 #include <memory>
+#include <set>
 #include <string>
 #include <utility>
 #include <vector>
@@ -30,6 +31,9 @@ public:
     bool plasticity_supported() const override { return true; }
     std::vector<std::string> get_supported_neurons() const override;
     std::vector<std::string> get_supported_synapses() const override;
+    // every type of AllPopulationsVariant / AllProjectionsVariant
+    std::vector<size_t> get_supported_population_indexes() const override;
+    std::vector<size_t> get_supported_projection_indexes() const override;
 
     void load_populations(const std::vector<PopulationVariants> &populations);
     void load_projections(const std::vector<ProjectionVariants> &projections);
@@ -39,20 +43,27 @@ public:
     void remove_populations(const std::vector<core::UID> &uids) override;
 
     std::vector<std::unique_ptr<core::Device>> get_devices() const override;
+    // Selecting a device other than the one the backend was created for recreates the device network; that is
+    // allowed until the first step.
+    void select_devices(const std::set<core::UID> &uids) override;
+    void select_device(std::unique_ptr<core::Device> &&device) override;
 
     void stop_learning() override;
     void start_learning() override;
     void _init() override;
     void _step() override;
 
-    // Backend::get_network_data (knp/core-library/include/knp/core/backend.h:396-420): copies of the loaded
-    // populations and projections carrying the current device state.
-    std::pair<std::vector<PopulationVariants>, std::vector<ProjectionVariants>> get_network_data();
+    // Backend::get_network_data (knp/core-library/include/knp/core/backend.h:396-420): ranges over copies of
+    // the loaded populations and projections carrying the device state at the time of the call.
+    DataRanges get_network_data() const override;
+    // The same copies as two vectors.
+    std::pair<std::vector<PopulationVariants>, std::vector<ProjectionVariants>> network_copy() const;
 
     knp_net *native_handle() { return net_; }
 
 private:
     void check(int rc, const char *what) const;
+    void use_device(const core::UID &uid);
     int device_ = 0;
     knp_net *net_ = nullptr;
     bool learning_ = false;

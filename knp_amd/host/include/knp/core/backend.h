@@ -1,17 +1,22 @@
 // knp/core/backend.h - the backend interface.
 // Interface of knp/core-library/include/knp/core/backend.h:55-457 for the populations and projections this
 // build supports: the same pure virtuals (plasticity_supported :103, get_supported_neurons :109,
-// get_supported_synapses :115, load_all_projections :137, load_all_populations :146, remove_* :153,160,
-// get_devices :172, stop_learning :295, start_learning :300, _init :314, _step :321), the same run loop
+// get_supported_synapses :115, get_supported_population_indexes :121, get_supported_projection_indexes :127,
+// load_all_projections :137, load_all_populations :146, remove_* :153,160, get_devices :172,
+// stop_learning :295, start_learning :300, _init :314, _step :321, get_network_data :420 with its
+// BaseValueIterator / DataRanges types :336-413), the device selection (select_devices :200, select_device
+// :207, get_current_devices :183,193; knp/core-library/impl/backend.cpp:166-192), the same run loop
 // (start/stop/get_step, knp/core-library/impl/backend.cpp:57-141: _init once, then _step while running,
 // stop() from any thread, exceptions reset the running flag and propagate) and bus access (subscribe :222).
 #pragma once
 #include <atomic>
 #include <functional>
+#include <iterator>
 #include <memory>
 #include <set>
 #include <stdexcept>
 #include <string>
+#include <utility>
 #include <variant>
 #include <vector>
 
@@ -52,19 +57,55 @@ public:
     virtual bool plasticity_supported() const = 0;
     virtual std::vector<std::string> get_supported_neurons() const = 0;
     virtual std::vector<std::string> get_supported_synapses() const = 0;
+    // Indexes into AllPopulationsVariant / AllProjectionsVariant of the types the backend runs.
+    virtual std::vector<size_t> get_supported_population_indexes() const = 0;
+    virtual std::vector<size_t> get_supported_projection_indexes() const = 0;
     virtual void load_all_projections(const std::vector<AllProjectionsVariant> &projections) = 0;
     virtual void load_all_populations(const std::vector<AllPopulationsVariant> &populations) = 0;
     virtual void remove_projections(const std::vector<UID> &uids) = 0;
     virtual void remove_populations(const std::vector<UID> &uids) = 0;
     virtual std::vector<std::unique_ptr<Device>> get_devices() const = 0;
+    std::vector<std::unique_ptr<Device>> &get_current_devices() { return devices_; }
+    const std::vector<std::unique_ptr<Device>> &get_current_devices() const { return devices_; }
+    // As the reference: the first listed device whose UID is in `uids` is selected; the call fails when that
+    // does not select as many devices as were asked for.
     virtual void select_devices(const std::set<UID> &uids) {
-        auto devs = get_devices();
-        for (const auto &u : uids) {
-            bool found = false;
-            for (const auto &d : devs) found |= d->get_uid() == u;
-            if (!found) throw std::logic_error("Not all devices were selected.");
+        for (auto &&device : get_devices()) {
+            if (uids.find(device->get_uid()) != uids.end()) {
+                devices_.push_back(std::move(device));
+                return;
+            }
         }
+        if (uids.size() != devices_.size()) throw std::logic_error("Not all devices were selected.");
     }
+    virtual void select_device(std::unique_ptr<Device> &&device) {
+        devices_.clear();
+        devices_.push_back(std::move(device));
+    }
+
+    // Access to the network by value: iterators over populations or projections that yield copies.
+    template <class Type>
+    class BaseValueIterator {
+    public:
+        using iterator_category = std::input_iterator_tag;
+        using value_type = Type;
+        BaseValueIterator() = default;
+        virtual Type operator*() const = 0;
+        virtual BaseValueIterator &operator++() = 0;
+        virtual bool operator==(const BaseValueIterator &rhs) const = 0;
+        virtual bool operator!=(const BaseValueIterator &rhs) const { return !(*this == rhs); }
+        virtual ~BaseValueIterator() = default;
+    };
+    struct DataRanges {
+        std::pair<std::unique_ptr<BaseValueIterator<AllProjectionsVariant>>,
+                  std::unique_ptr<BaseValueIterator<AllProjectionsVariant>>>
+            projection_range;
+        std::pair<std::unique_ptr<BaseValueIterator<AllPopulationsVariant>>,
+                  std::unique_ptr<BaseValueIterator<AllPopulationsVariant>>>
+            population_range;
+    };
+    // Iterator ranges over copies of the populations and projections with their current state.
+    virtual DataRanges get_network_data() const = 0;
 
     template <typename MessageType>
     Subscription<MessageType> &subscribe(const UID &receiver, const std::vector<UID> &senders) {
@@ -114,6 +155,7 @@ protected:
     bool initialized_ = false;
 
 private:
+    std::vector<std::unique_ptr<Device>> devices_;
     std::shared_ptr<MessageBus> message_bus_;
     MessageEndpoint message_endpoint_;
     std::atomic<bool> started_{false};

@@ -170,6 +170,77 @@ std::vector<std::string> MI355XBackend::get_supported_synapses() const {
     return {"DeltaSynapse", "AdditiveSTDPDeltaSynapse", "SynapticResourceSTDPDeltaSynapse"};
 }
 
+std::vector<size_t> MI355XBackend::get_supported_population_indexes() const {
+    std::vector<size_t> out(std::variant_size_v<core::AllPopulationsVariant>);
+    for (size_t i = 0; i < out.size(); ++i) out[i] = i;
+    return out;
+}
+
+std::vector<size_t> MI355XBackend::get_supported_projection_indexes() const {
+    std::vector<size_t> out(std::variant_size_v<core::AllProjectionsVariant>);
+    for (size_t i = 0; i < out.size(); ++i) out[i] = i;
+    return out;
+}
+
+namespace {
+// Value iterators over a snapshot of the network taken by get_network_data (shared by both ends of a range).
+template <class Variant>
+class SnapshotIterator : public core::Backend::BaseValueIterator<Variant> {
+public:
+    SnapshotIterator(std::shared_ptr<const std::vector<Variant>> data, size_t pos) : data_(std::move(data)), pos_(pos) {}
+    Variant operator*() const override { return (*data_)[pos_]; }
+    core::Backend::BaseValueIterator<Variant> &operator++() override {
+        ++pos_;
+        return *this;
+    }
+    bool operator==(const core::Backend::BaseValueIterator<Variant> &rhs) const override {
+        if (typeid(*this) != typeid(rhs)) return false;
+        const auto &r = dynamic_cast<const SnapshotIterator &>(rhs);
+        return r.data_ == data_ && r.pos_ == pos_;
+    }
+
+private:
+    std::shared_ptr<const std::vector<Variant>> data_;
+    size_t pos_;
+};
+}  // namespace
+
+core::Backend::DataRanges MI355XBackend::get_network_data() const {
+    auto copy = network_copy();
+    auto pops = std::make_shared<const std::vector<PopulationVariants>>(std::move(copy.first));
+    auto projs = std::make_shared<const std::vector<ProjectionVariants>>(std::move(copy.second));
+    using PopIt = SnapshotIterator<core::AllPopulationsVariant>;
+    using ProjIt = SnapshotIterator<core::AllProjectionsVariant>;
+    DataRanges r;
+    r.population_range = {std::make_unique<PopIt>(pops, 0), std::make_unique<PopIt>(pops, pops->size())};
+    r.projection_range = {std::make_unique<ProjIt>(projs, 0), std::make_unique<ProjIt>(projs, projs->size())};
+    return r;
+}
+
+void MI355XBackend::use_device(const core::UID &uid) {
+    int n = 0;
+    check(knp_mi355x_device_count(&n), "device count");
+    int index = -1;
+    auto devs = get_devices();
+    for (int d = 0; d < (int)devs.size(); ++d)
+        if (devs[d]->get_uid() == uid) index = d;
+    if (index < 0) throw std::logic_error("Not all devices were selected.");
+    if (index == device_) return;
+    if (net_) throw std::logic_error("The device must be selected before the backend is initialized.");
+    device_ = index;
+}
+
+void MI355XBackend::select_devices(const std::set<core::UID> &uids) {
+    core::Backend::select_devices(uids);
+    if (!get_current_devices().empty()) use_device(get_current_devices().back()->get_uid());
+}
+
+void MI355XBackend::select_device(std::unique_ptr<core::Device> &&device) {
+    if (!device) throw std::logic_error("Not all devices were selected.");
+    use_device(device->get_uid());
+    core::Backend::select_device(std::move(device));
+}
+
 void MI355XBackend::load_populations(const std::vector<PopulationVariants> &populations) {
     if (net_) throw std::logic_error("Populations must be loaded before the backend is initialized.");
     populations_ = populations;
@@ -368,7 +439,7 @@ void MI355XBackend::_step() {
 }
 
 std::pair<std::vector<MI355XBackend::PopulationVariants>, std::vector<MI355XBackend::ProjectionVariants>>
-MI355XBackend::get_network_data() {
+MI355XBackend::network_copy() const {
     auto pops = populations_;
     auto projs = projections_;
     if (!net_) return {pops, projs};

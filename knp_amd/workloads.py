@@ -226,7 +226,7 @@ def _op(spec, name):
     return [(a, kw) for n, a, kw in spec.ops if n == name]
 
 
-def batched_stdp(batch=256, n_images=250, steps_per_image=20, seed=30, dense=None, **kw):
+def batched_stdp(batch=256, n_images=250, steps_per_image=20, seed=30, dense=None, split=False, **kw):
     """Config 3: `batch` independent copies of the config-2 network (own weights from seed + b, own image
     streams), composed into one device network: one population of batch x 1,024 neurons and block-diagonal
     projections, so that every instance advances in the same kernel launches. Instance b alone is
@@ -234,14 +234,16 @@ def batched_stdp(batch=256, n_images=250, steps_per_image=20, seed=30, dense=Non
     dense: None (every projection on the event-driven scatter, FP32), "full" (the plastic input projection
     expanded on MFMA as the instances' spike counts x their BF16 weight copies over all 784 inputs) or "skip"
     (the same, multiplying only the groups of 8 inputs that hold a spike); with either, the lateral projection
-    (weight 0.5, exact in BF16) is expanded on MFMA too, over the groups of neurons that spiked."""
+    (weight 0.5, exact in BF16) is expanded on MFMA too, over the groups of neurons that spiked. split: the
+    input weights as two BF16 parts (hi + lo, 16 significant bits)."""
     src_seed = seed * 7919 + 17
     parts = [stdp_learning(seed=seed + b, n_images=n_images, steps_per_image=steps_per_image, source_seed=src_seed,
                            instance=b, **kw) for b in range(batch)]
     spec = compose(parts, "batched_stdp", per_instance_steps=n_images * steps_per_image)
     if dense is not None:
         meta = parts[0].meta
-        spec.set_projection_dense(meta["input_proj"], abi.DENSE_BF16, abi.DENSE_SKIP_EMPTY if dense == "skip" else 0)
+        flags = (abi.DENSE_SKIP_EMPTY if dense == "skip" else 0) | (abi.DENSE_SPLIT if split else 0)
+        spec.set_projection_dense(meta["input_proj"], abi.DENSE_BF16, flags)
         spec.set_projection_dense(meta["lateral_proj"], abi.DENSE_BF16, abi.DENSE_SKIP_EMPTY)
     return spec
 

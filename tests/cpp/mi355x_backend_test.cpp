@@ -44,6 +44,19 @@ static void run_case(const char *name, const std::function<void()> &f) {
 
 static const std::vector<knp::core::Step> kExpected = {1, 6, 7, 11, 12, 13, 16, 17, 18, 19};
 
+// The network of a backend as get_network_copy builds it (knp/base-framework/impl/synchronization.cpp:26-41):
+// walk Backend::get_network_data()'s ranges through a const Backend and take every population and projection.
+static std::pair<std::vector<knp::core::AllPopulationsVariant>, std::vector<knp::core::AllProjectionsVariant>>
+network_copy(const knp::core::Backend &backend) {
+    auto data_ranges = backend.get_network_data();
+    std::pair<std::vector<knp::core::AllPopulationsVariant>, std::vector<knp::core::AllProjectionsVariant>> res;
+    for (auto &iter = *data_ranges.population_range.first; iter != *data_ranges.population_range.second; ++iter)
+        res.first.push_back(*iter);
+    for (auto &iter = *data_ranges.projection_range.first; iter != *data_ranges.projection_range.second; ++iter)
+        res.second.push_back(*iter);
+    return res;
+}
+
 // Drive a network for 20 steps: input spikes on steps 0, 5, 10, 15; record steps with output spikes.
 static std::vector<knp::core::Step> run_smallest(knp::core::Backend &backend, const UID &input_proj,
                                                  const UID &population) {
@@ -94,7 +107,7 @@ static void run_resource_network() {
     backend._init();
     backend.start_learning();
     CHECK(run_smallest(backend, input_uid, population.get_uid()) == kExpected);
-    auto data = backend.get_network_data();
+    auto data = network_copy(backend);
     const auto &loop_after = std::get<STDPDeltaProjection>(data.second[1]);
     CHECK(std::get<knp::core::synapse_data>(loop_after[0]).weight_ != 1.0f);
 }
@@ -166,7 +179,7 @@ int main(int argc, char **argv) {
         backend.load_projections({input, loop});
         backend._init();
         CHECK(run_smallest(backend, input_uid, population.get_uid()) == kExpected);
-        auto data = backend.get_network_data();
+        auto data = network_copy(backend);
         const auto &loop_after = std::get<STDPDeltaProjection>(data.second[1]);
         CHECK(std::get<knp::core::synapse_data>(loop_after[0]).weight_ != 1.0f);
     });
@@ -205,7 +218,7 @@ int main(int argc, char **argv) {
             if (!endpoint.unload_messages<SpikeMessage>(out_channel_uid).empty()) results.push_back(step);
         }
         CHECK((results == std::vector<knp::core::Step>{2, 6, 10, 14}));
-        const auto data = backend.get_network_data();
+        const auto data = network_copy(backend);
         const auto &after = std::get<LIFPopulation>(data.first[0]);
         CHECK(after[0].refract_counter_ == 1);  // fired at step 14, one refractory step done
         CHECK(after[0].potential_ == 0.0f);
@@ -232,7 +245,7 @@ int main(int argc, char **argv) {
         std::vector<knp::core::Step> results;
         std::vector<float> potentials;
         for (knp::core::Step step = 0; step < 10; ++step) {
-            potentials.push_back(std::get<AltAIPopulation>(backend.get_network_data().first[0])[0].potential_);
+            potentials.push_back(std::get<AltAIPopulation>(network_copy(backend).first[0])[0].potential_);
             backend._step();
             endpoint.receive_all_messages();
             if (!endpoint.unload_messages<SpikeMessage>(out_channel_uid).empty()) results.push_back(step);
@@ -247,6 +260,54 @@ int main(int argc, char **argv) {
         CHECK(backend.get_supported_neurons().back() == "LIFNeuron");
         CHECK(backend.get_supported_synapses()[0] == "DeltaSynapse");
         CHECK(backend.plasticity_supported());
+        CHECK((backend.get_supported_population_indexes() == std::vector<size_t>{0, 1, 2, 3, 4}));
+        CHECK((backend.get_supported_projection_indexes() == std::vector<size_t>{0, 1, 2}));
+    });
+
+    run_case("NetworkDataThroughConstBackend", [] {
+        // get_network_data through a const Backend (as get_network_copy and the weights logger call it): the
+        // ranges yield copies with the device state, and a second call sees the state of later steps
+        Backend backend;
+        BLIFATPopulation population{[](size_t) { return knp::neuron_traits::neuron_parameters<knp::neuron_traits::BLIFATNeuron>{}; }, 3};
+        DeltaProjection input{UID{false}, population.get_uid(),
+                              [](size_t i) { return DeltaProjection::Synapse{{0.25f * float(i + 1), 1, knp::synapse_traits::OutputType::EXCITATORY}, i, i}; },
+                              3};
+        const UID input_uid = input.get_uid();
+        backend.load_populations({population});
+        backend.load_projections({input});
+        backend._init();
+        const knp::core::Backend &cb = backend;
+        auto endpoint = backend.get_message_bus().create_endpoint();
+        const UID in_channel_uid;
+        backend.subscribe<SpikeMessage>(input_uid, {in_channel_uid});
+        endpoint.send_message(SpikeMessage{{in_channel_uid, 0}, {0, 1, 2}});
+        backend._step();
+        backend._step();
+        auto net = network_copy(cb);
+        CHECK(net.first.size() == 1 && net.second.size() == 1);
+        const auto &p = std::get<BLIFATPopulation>(net.first[0]);
+        CHECK(p.size() == 3 && p.get_uid() == population.get_uid());
+        // impacts 0.25, 0.5, 0.75 arrived at step 1 below the threshold 1: the potentials hold them
+        CHECK(p[0].potential_ == 0.25 && p[1].potential_ == 0.5 && p[2].potential_ == 0.75);
+        const auto &q = std::get<DeltaProjection>(net.second[0]);
+        CHECK(q.size() == 3 && std::get<knp::core::synapse_data>(q[2]).weight_ == 0.75f);
+        backend._step();
+        auto later = network_copy(cb);
+        CHECK(std::get<BLIFATPopulation>(later.first[0])[1].potential_ == 0.0);  // memoryless: no input at step 2
+        // two ranges from one call stay independent of later steps
+        auto ranges = cb.get_network_data();
+        CHECK(*ranges.population_range.first != *ranges.population_range.second);
+    });
+
+    run_case("SelectDevice", [] {
+        Backend backend;
+        auto devices = backend.get_devices();
+        CHECK(!devices.empty());
+        const UID uid = devices[0]->get_uid();
+        backend.select_device(std::move(devices[0]));
+        CHECK(backend.get_current_devices().size() == 1 && backend.get_current_devices()[0]->get_uid() == uid);
+        backend.select_devices({uid});
+        CHECK(!backend.get_current_devices().empty());
     });
 
     run_case("BackendLoader", [&] {

@@ -28,5 +28,6 @@ def test_cpp_backend_cases():
     r = run()
     assert r.returncode == 0, r.stdout
     for name in ("SmallestNetwork", "AdditiveSTDPBLIFATNetwork", "ResourceSTDPBLIFATNetwork",
-                 "NeuronsAndSynapsesGetting", "BackendLoader", "StartWithStopPredicate"):
+                 "NeuronsAndSynapsesGetting", "NetworkDataThroughConstBackend", "SelectDevice", "BackendLoader",
+                 "StartWithStopPredicate"):
         assert f"PASS {name}" in r.stdout

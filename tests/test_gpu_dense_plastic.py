@@ -72,7 +72,7 @@ def dense_blocks_network(o, seed, flags=None):
     return pop, inp
 
 
-@pytest.mark.parametrize("flags", [0, abi.DENSE_SKIP_EMPTY])
+@pytest.mark.parametrize("flags", [0, abi.DENSE_SKIP_EMPTY, abi.DENSE_SKIP_EMPTY | abi.DENSE_SPLIT])
 def test_dense_expansion_matches_oracle(flags):
     say(f"case: projections expanded on MFMA, flags={flags}: uneven instances vs oracle")
     steps = 160
@@ -109,18 +109,23 @@ def test_dense_resource_hand_traces(flags):
     resource_traces.run_all_traces(lambda: MI355XBackend(0), dense_flags=flags)
 
 
-def test_dense_stdp_batch_weight_copy():
+@pytest.mark.parametrize("split", [False, True])
+def test_dense_stdp_batch_weight_copy(split):
     """Eight instances of the STDP configuration's network (784 -> 1,024, lateral inhibition, dopamine) with
     both dense projections on MFMA, 300 steps: after training the BF16 copy is the FP32 weights rounded to
-    nearest even; spike agreement with the oracle is reported per instance."""
-    say("case: STDP batch of 8 on MFMA, 300 steps")
+    nearest even (split: and the low part is the remainder rounded to nearest even); spike agreement with the
+    oracle is reported per instance."""
+    say(f"case: STDP batch of 8 on MFMA, 300 steps, split={split}")
     B, steps = 8, 300
-    spec = workloads.batched_stdp(batch=B, n_images=steps // 20 + 1, dense="full")
+    spec = workloads.batched_stdp(batch=B, n_images=steps // 20 + 1, dense="full", split=split)
     g = spec.load_into(MI355XBackend(0))
     g.record_spikes(0, steps)
     g.step(steps)
     w = np.asarray(g.weights(0), np.float32)
-    assert np.array_equal(g.dense_weights(0), abi.bf16_encode(w))
+    hi, lo = g.dense_weights(0, low=True)
+    assert np.array_equal(hi, abi.bf16_encode(w))
+    if split:
+        assert np.array_equal(lo, abi.bf16_encode(w - abi.bf16_decode(hi)))
     sg = g.read_spike_record(0)
     trained = int(np.sum(w != np.asarray(spec.ops[1][1][5], np.float32)))
     say(f"  weights changed by training: {trained}")
