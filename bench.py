@@ -45,24 +45,30 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10_000)
     ap.add_argument("--warmup", type=int, default=200)
-    ap.add_argument("--parity-steps", type=int, default=100)
+    ap.add_argument("--parity-steps", type=int, default=10_000,
+                    help="c1: steps from step 0 whose spike trains are compared with the oracle (the configuration "
+                         "runs 10,000); the multi-threaded oracle, equal to the single-threaded one, runs them")
     ap.add_argument("--timing-steps", type=int, default=1000)
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="target CPU-baseline sample length")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-batched", action="store_true",
+                    help="c1: leave out the batched-instances measurement (the metric's ratio half)")
     ap.add_argument("--workload", choices=["c1", "c0", "c2", "c3", "c4"], default="c1",
-                    help="c1 (default): the plain FP32 network at GPU scale, the headline line. The others print "
-                         "their own measurement line: c0 small plain network, c2 single STDP instance, c3 batched "
-                         "STDP instances (instance-steps/s and ratio to c2), c4 uneven instances with the FP8 dense "
-                         "projection on MFMA")
+                    help="c1 (default): the plain FP32 network at GPU scale, the headline line, with the batched "
+                         "STDP instances (instance-steps/s, ratio to one instance on the plain FP32 path, FP64 "
+                         "agreement) in its `batched` field. The others print their own measurement line: c0 small "
+                         "plain network, c2 single STDP instance, c3 batched STDP instances, c4 uneven instances "
+                         "with the FP8 dense projection on MFMA")
     ap.add_argument("--batch", type=int, default=256, help="c3: instances")
-    ap.add_argument("--agree-steps", type=int, default=400, help="c2/c3/c4: steps compared with the FP64 oracle")
+    ap.add_argument("--agree-steps", type=int, default=5000,
+                    help="c2/c3/c4: steps compared with the FP64 oracle (c3 runs 5,000)")
     ap.add_argument("--agree-instances", type=int, default=4, help="c3: instances compared with the FP64 oracle")
-    ap.add_argument("--dense", choices=["full", "skip", "none"], default="full",
+    ap.add_argument("--dense", choices=["full", "skip", "none"], default="skip",
                     help="c3: the plastic input projection expanded on MFMA over all inputs (full: binary spikes x "
                          "BF16 weight copies), over the groups of 8 inputs holding a spike (skip), or by the FP32 "
                          "event-driven scatter (none)")
-    ap.add_argument("--split", action="store_true",
-                    help="c3: the MFMA weight copies as two BF16 parts each (hi + lo, 16 significant bits)")
+    ap.add_argument("--no-split", dest="split", action="store_false",
+                    help="c3: one BF16 value per weight (default: two parts, hi + lo, 16 significant bits)")
     ap.add_argument("--scatter", choices=["auto", "stepwise", "atomic"], default="auto",
                     help="impact summation: tiled populations and persistent launches where eligible (auto), "
                          "tiled with one launch per step (stepwise), or atomics everywhere")
@@ -73,20 +79,27 @@ def main():
               "(sharding over several GPUs is designed, not implemented). Run with --gpus 1.", file=sys.stderr)
         sys.exit(2)
 
-    from knp_amd import workloads
-    from knp_amd.backend import MI355XBackend, device_name, KERNEL_IDS
-
     if args.workload != "c1":
         other_workload(args)
         return
+    line = run_c1(args)
+    if not args.no_batched:
+        # the metric's second half: batched BF16 instances against one instance on the plain FP32 path
+        c3 = dict(vars(args))
+        c3.update(steps=1000, warmup=200)
+        line["batched"] = measure_batched(argparse.Namespace(**c3))
+    print(json.dumps(line), flush=True)
+
+
+def run_c1(args):
+    from knp_amd import abi, workloads
+    from knp_amd.backend import MI355XBackend, device_name, KERNEL_IDS
 
     t_gen = time.perf_counter()
     spec = workloads.plain_gpu_scale()
     pop = spec.meta["pop"]
     n_neurons = spec.pop_sizes[pop]
     log(f"generated {spec.name}: {n_neurons} neurons, {spec.n_synapses} synapses in {time.perf_counter() - t_gen:.1f}s")
-
-    from knp_amd import abi
 
     gpu = MI355XBackend(0)
     gpu.set_scatter_mode({"auto": abi.SCATTER_AUTO, "stepwise": abi.SCATTER_STEPWISE,
@@ -97,14 +110,20 @@ def main():
     gpu.synchronize()
     log(f"uploaded in {time.perf_counter() - t_up:.1f}s on {device_name(0)}")
 
-    parity_steps = min(args.parity_steps, args.warmup)
-    gpu.record_spikes(pop, parity_steps)
-    gpu.step(parity_steps)
-    gpu_pot = gpu.potentials(pop)  # state after the compared prefix
-    gpu.step(args.warmup - parity_steps)
+    # the compared run: steps 0 .. P-1 recorded (untimed), then the warmup
+    P = max(1, args.parity_steps)
+    P64 = min(P, 100)  # the prefix also compared with the FP64 oracle (single-threaded)
+    gpu.record_spikes(pop, P)
+    gpu.step(P64)
+    gpu_pot64 = gpu.potentials(pop)
+    gpu.step(P - P64)
     gpu.synchronize()
+    gpu_pot = gpu.potentials(pop)
+    gpu_events_p = gpu.events
     gpu_prefix = gpu.read_spike_record(pop)
-    log(f"warmup {args.warmup} steps done")
+    gpu.step(args.warmup)
+    gpu.synchronize()
+    log(f"{P} recorded steps and {args.warmup} warmup steps done")
 
     # ---- timed region: K steps, synchronized on both sides
     gpu.synchronize()
@@ -116,9 +135,8 @@ def main():
     e1, r1 = gpu.counters()
     dt = t1 - t0
     events = e1 - e0
-    rows = r1 - r0
     value = events / dt
-    log(f"timed {args.steps} steps in {dt:.3f}s: {events} events, {rows} rows, {value:.3e} events/s")
+    log(f"timed {args.steps} steps in {dt:.3f}s: {events} events, {r1 - r0} rows, {value:.3e} events/s")
 
     # ---- per-kernel durations (HIP events on the library's stream), separate pass
     tsteps = args.timing_steps
@@ -135,6 +153,7 @@ def main():
         if n:
             kern[name] = {"avg_us": avg_ms * 1e3, "launches": n, "us_per_step": avg_ms * 1e3 * n / tsteps}
     gpu.enable_timing(False)
+    gpu.close()
     ev_step = (e3 - e2) / tsteps
     rows_step = (r3 - r2) / tsteps
     parts_step = (p3 - p2) / tsteps
@@ -175,42 +194,51 @@ def main():
         "bytes_per_launch": round(step_bytes * per_launch, 1),
         "us_per_step": round(dk["us_per_step"], 3),
     }
-    traffic_file = os.path.join(ROOT, "profiles", f"{kname}_plain_gpu_scale_traffic.json")
-    if os.path.exists(traffic_file):
-        try:
-            # HBM bytes per step measured with rocprofv3 counters, scaled to this launch length
-            roofline["traffic"] = json.load(open(traffic_file))["hbm_bytes_per_step"] * per_launch
-        except (OSError, ValueError, KeyError):
-            pass
+    roofline.update(profiled_traffic(kname, per_launch))
 
-    # ---- CPU baseline: the oracle (restated single-threaded backend), bounded sample, 1 thread
-    cpu = None
-    cpu_mt = None
-    cpu_release = None
+    # ---- parity and CPU baselines
+    cpu = cpu_mt = cpu_release = None
     parity = {"steps_compared": 0}
     if not args.no_cpu_baseline:
-        import numpy as np
-
         from oracle.pyoracle import Oracle
 
+        threads = max(1, min(16, os.cpu_count() or 1))
+        # the compared steps on the multi-threaded oracle (its spikes, potentials and events equal the
+        # single-threaded oracle's), timed: the multi-threaded CPU baseline over the same steps
         tb = time.perf_counter()
-        ora = spec.load_into(Oracle("exact"))
+        mt = spec.load_into(Oracle("exact", threads=threads) if threads > 1 else Oracle("exact"))
         log(f"oracle built in {time.perf_counter() - tb:.1f}s")
-        ora.record_spikes(pop, parity_steps)
-        ora.step(parity_steps)
-        ora_prefix = ora.read_spike_record(pop)
+        mt.record_spikes(pop, P)
+        tm = time.perf_counter()
+        mt.step(P)
+        tmt = time.perf_counter() - tm
+        ora_prefix = mt.read_spike_record(pop)
         same = bool(np.array_equal(ora_prefix, gpu_prefix))
-        parity = {"steps_compared": int(parity_steps), "spike_trains_identical": same,
-                  "spikes_compared": int(ora_prefix.sum()),
-                  "potential_rel_err_max": pot_error(gpu_pot, ora.potentials(pop))[0],
-                  "fp64": compare_f64(spec, gpu_prefix, gpu_pot, pop, parity_steps)}
-        # probe, then a sample sized to ~cpu_seconds
-        ev_a = ora.events
+        first = np.argwhere(ora_prefix != gpu_prefix)
+        parity = {"steps_compared": int(P), "spike_trains_identical": same,
+                  "first_differing_step": int(first[0][0]) if len(first) else None,
+                  "spikes_compared": int(ora_prefix.sum()), "events_equal": int(mt.events) == int(gpu_events_p),
+                  "events_compared": int(mt.events),
+                  "potential_rel_err_max": pot_error(gpu_pot, mt.potentials(pop))[0],
+                  "oracle": f"the restated single-threaded backend's step in the multi-threaded backend's "
+                            f"decomposition ({threads} threads; equal to one thread), KNP-exact float types",
+                  "fp64": compare_f64(spec, gpu_prefix[:P64], gpu_pot64, pop, P64)}
+        log(f"parity over {P} steps: identical={same}, events equal={parity['events_equal']}, oracle {tmt:.1f}s")
+        cpu_mt = {
+            "value": mt.events / tmt, "unit": "events/s", "cores": threads, "kind": "port",
+            "sample": f"steps 0..{P - 1} of the same seeded network ({mt.events} events, {tmt:.1f}s), {threads} "
+                      f"threads: the multi-threaded backend's decomposition (parallel neuron parts and projection "
+                      f"chunks; cpu-multi-threaded-backend/impl/backend.cpp:77-323) of the oracle, -O2 strict "
+                      f"IEEE, OpenMP",
+            "net_steps_per_s": P / tmt,
+        }
+        mt.close()
+        # the single-threaded oracle on a bounded sample (probe, then ~cpu_seconds of steps)
+        ora = spec.load_into(Oracle("exact"))
         tp = time.perf_counter()
         ora.step(10)
         probe = time.perf_counter() - tp
-        n_cpu = max(10, int(args.cpu_seconds / max(probe / 10, 1e-6)))
-        n_cpu = min(n_cpu, 5000)
+        n_cpu = min(5000, max(10, int(args.cpu_seconds / max(probe / 10, 1e-6))))
         ev_b = ora.events
         tc = time.perf_counter()
         ora.step(n_cpu)
@@ -221,42 +249,17 @@ def main():
             "unit": "events/s",
             "cores": 1,
             "kind": "port",
-            "sample": f"{spec.name}: steps {parity_steps + 10}..{parity_steps + 10 + n_cpu - 1} of the same seeded "
-                      f"network ({n_cpu} steps, {ev_c - ev_b} events, {tcpu:.1f}s), single thread, "
-                      f"oracle built -O2 strict IEEE",
+            "sample": f"{spec.name}: steps 10..{10 + n_cpu - 1} of the same seeded network ({n_cpu} steps, "
+                      f"{ev_c - ev_b} events, {tcpu:.1f}s), single thread, oracle built -O2 strict IEEE",
             "net_steps_per_s": n_cpu / tcpu,
             "host_cpu": cpu_model(),
             "host_threads": os.cpu_count(),
         }
-        _ = ev_a
         log(f"cpu baseline {cpu['value']:.3e} events/s over {n_cpu} steps")
         ora.close()
-        # the multi-threaded backend's decomposition of the same step on the same sample (at most 16 threads:
-        # one GPU's share of the host); its spikes equal the single thread's
-        threads = max(1, min(16, os.cpu_count() or 1))
-        if threads > 1:
-            mt = spec.load_into(Oracle("exact", threads=threads))
-            mt.step(parity_steps + 10)
-            eb = mt.events
-            tm = time.perf_counter()
-            mt.step(n_cpu)
-            tmt = time.perf_counter() - tm
-            cpu_mt = {
-                "value": (mt.events - eb) / tmt,
-                "unit": "events/s",
-                "cores": threads,
-                "kind": "port",
-                "sample": f"the same {n_cpu} steps, {threads} threads: the multi-threaded backend's decomposition "
-                          f"(parallel neuron parts and projection chunks; cpu-multi-threaded-backend/impl/"
-                          f"backend.cpp:77-323) of the oracle, -O2 strict IEEE, OpenMP",
-                "net_steps_per_s": n_cpu / tmt,
-                "events_equal_single_thread": mt.events - eb == ev_c - ev_b,
-            }
-            mt.close()
-            log(f"cpu baseline ({threads} threads) {cpu_mt['value']:.3e} events/s")
         # the same single-thread step built with Release-like flags (-Ofast, AVX2 / FMA): a speed figure only
         rl = spec.load_into(Oracle("release"))
-        rl.step(parity_steps + 10)
+        rl.step(10)
         eb = rl.events
         tr = time.perf_counter()
         rl.step(n_cpu)
@@ -270,7 +273,7 @@ def main():
         rl.close()
         log(f"cpu baseline (Release-like) {cpu_release['value']:.3e} events/s")
 
-    line = {
+    return {
         "metric": "synaptic events/s",
         "value": value,
         "unit": "events/s",
@@ -304,8 +307,27 @@ def main():
         "parity": parity,
         "device": device_name(0),
     }
-    gpu.close()
-    print(json.dumps(line), flush=True)
+
+
+def profiled_traffic(kname, per_launch):
+    """HBM traffic per launch from the committed rocprofv3 counter summary of this kernel, used only when that
+    summary was taken of the current kernel source (its recorded source digest matches); else null."""
+    import hashlib
+
+    out = {"traffic_source": None}
+    path = os.path.join(ROOT, "profiles", f"{kname}_plain_gpu_scale_traffic.json")
+    try:
+        rec = json.load(open(path))
+        digest = hashlib.sha256(open(os.path.join(ROOT, "knp_amd", "csrc", "step_kernels.h"), "rb").read()).hexdigest()
+        if rec.get("step_kernels_sha256") == digest:
+            out["traffic"] = rec["hbm_bytes_per_step"] * per_launch
+            out["traffic_source"] = (f"profiles/{os.path.basename(path)}: rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE "
+                                     f"of this kernel source, scaled by a copy kernel of known size")
+        else:
+            out["traffic_source"] = f"profiles/{os.path.basename(path)} was taken of another kernel source: not used"
+    except (OSError, ValueError, KeyError):
+        pass
+    return out
 
 
 def agreement(a, b):
@@ -391,12 +413,10 @@ def agreement_report(g_rec, o_rec, g_pot, o_pot, steps):
             "potential_rel_err_max_at_end": pmax, "potential_rel_err_mean_at_end": pmean}
 
 
-def other_workload(args):
-    """Measurement lines of the other configurations (not the headline line)."""
-    import numpy as np
-
-    from knp_amd import abi, workloads
-    from knp_amd.backend import KERNEL_IDS, MI355XBackend, device_name
+def device_loaders(args):
+    """(load(spec) -> initialized device network, kernels(g, steps) -> per-kernel HIP-event durations)."""
+    from knp_amd import abi
+    from knp_amd.backend import KERNEL_IDS, MI355XBackend
 
     def load(spec):
         g = MI355XBackend(0)
@@ -420,6 +440,100 @@ def other_workload(args):
         g.enable_timing(False)
         return out
 
+    return load, kernels
+
+
+def plain_stdp_run(spec2, load, kernels, agree, warm, steps):
+    """One STDP instance on the plain FP32 path: `agree` recorded steps, warmup to `warm`, `steps` timed."""
+    g2 = load(spec2)
+    g2.record_spikes(0, agree)
+    g2.step(agree)
+    rec2, pot2 = g2.read_spike_record(0), g2.potentials(0)
+    g2.step(warm - agree)
+    dt2, ev2 = timed_steps(g2, steps)
+    k2 = kernels(g2, min(steps, 500))
+    g2.close()
+    log(f"plain fp32 instance: {steps / dt2:.0f} steps/s")
+    return ({"net_steps_per_s": steps / dt2, "events_per_s": ev2 / dt2, "steps": steps,
+             "ms_per_step": dt2 / steps * 1e3, "kernels": k2, "path": "fp32 event-driven scatter"}, rec2, pot2)
+
+
+def measure_batched(args):
+    """The batched configuration: args.batch instances of the STDP network stepped together (the plastic input
+    projection and the lateral one expanded on MFMA per args.dense / args.split), against one instance on the
+    plain FP32 path timed over the same step range; spike agreement of args.agree_instances instances (and of
+    the plain path's instance) with the FP64 oracle over args.agree_steps steps, by horizon."""
+    from knp_amd import workloads
+
+    load, kernels = device_loaders(args)
+    steps = args.steps if args.steps != 10_000 else 1000
+    agree = args.agree_steps
+    warm = max(args.warmup, agree)  # the plain and the batched runs time the same step range
+    n_img = (warm + steps) // 20 + 1
+    src_seed = 30 * 7919 + 17
+    # the plain FP32 path, one instance (the batch's instance 0): the denominator of the batching ratio
+    spec2 = workloads.stdp_learning(seed=30, n_images=n_img, source_seed=src_seed, instance=0)
+    c2_line, rec2, pot2 = plain_stdp_run(spec2, load, kernels, agree, warm, steps)
+    B = args.batch
+    tg = time.perf_counter()
+    spec3 = workloads.batched_stdp(batch=B, n_images=n_img, dense=None if args.dense == "none" else args.dense,
+                                   split=args.split)
+    log(f"generated {B} instances ({spec3.n_synapses} synapses) in {time.perf_counter() - tg:.1f}s")
+    tl = time.perf_counter()
+    g = load(spec3)
+    log(f"loaded in {time.perf_counter() - tl:.1f}s")
+    check = list(range(min(B, args.agree_instances)))
+    g.record_spikes(0, agree)
+    g.step(agree)
+    rec = g.read_spike_record(0)
+    pot = g.potentials(0)
+    g.step(warm - agree)
+    dt3, ev3 = timed_steps(g, steps)
+    ksteps = min(steps, 500)
+    wb0, full = g.dense_weight_bytes()
+    k3 = kernels(g, ksteps)
+    wb1, _ = g.dense_weight_bytes()
+    g.close()
+    inst_steps_per_s = B * steps / dt3
+    log(f"batched: {inst_steps_per_s:.3e} instance-steps/s")
+    parts = [workloads.stdp_learning(seed=30 + b, n_images=n_img, source_seed=src_seed, instance=b) for b in check]
+    f64 = f64_runs(parts, check, agree)
+    per = [agreement_report(rec[:, b * 1024:(b + 1) * 1024], f64[i][0], pot[b * 1024:(b + 1) * 1024], f64[i][1], agree)
+           for i, b in enumerate(check)]
+    allg = np.concatenate([rec[:, b * 1024:(b + 1) * 1024] for b in check], axis=1)
+    allo = np.concatenate([x[0] for x in f64], axis=1)
+    horizons = [h for h in (100, 500, 1000, 2000, 5000) if h <= agree]
+    batch_agr = {str(h): round(agreement(allg[:h], allo[:h])[0], 6) for h in horizons}
+    plain_agr = {str(h): round(agreement(rec2[:h], f64[0][0][:h])[0], 6) for h in horizons}
+    roof = None
+    if "dense_expand" in k3:
+        d = k3["dense_expand"]
+        bytes_step = full + (wb1 - wb0) / ksteps
+        ach = bytes_step / (d["us_per_step"] * 1e-6) / 1e9
+        roof = {"kernel": "k_dense_counts + k_dense_expand (both MFMA projections)", "bound": "hbm",
+                "achieved": round(ach, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(ach / HBM_PEAK_GBS, 5), "weight_bytes_per_step": bytes_step,
+                "dense_equivalent_bytes_per_step": float(B * 784 * 1024 * (4 if args.split else 2)),
+                "useful_mfma_flops_per_step": 2.0 * B * 784 * 1024, "us_per_step": d["us_per_step"]}
+    return dict(
+        metric="instance-steps/s", value=inst_steps_per_s, unit="instance-steps/s",
+        dtype="fp32" if args.dense == "none" else
+        ("bf16 hi+lo" if args.split else "bf16") + " weights x spike counts, fp32 accumulate",
+        workload=f"batched_stdp: {B} instances of the STDP network (784 -> 1,024 synaptic-resource STDP, lateral "
+                 f"inhibition, dopamine), own seeds and image streams",
+        dense=args.dense, split=args.split, batch=B, steps=steps, ms_per_step=dt3 / steps * 1e3,
+        events_per_s=ev3 / dt3, ratio_to_plain_fp32=inst_steps_per_s / c2_line["net_steps_per_s"],
+        plain_fp32=c2_line, kernels=k3, dense_roofline=roof,
+        fp64={"instances": check, "steps": agree, "spike_agreement_by_horizon": batch_agr,
+              "plain_fp32_instance0_by_horizon": plain_agr, "per_instance": per})
+
+
+def other_workload(args):
+    """Measurement lines of the other configurations (not the headline line)."""
+    from knp_amd import workloads
+    from knp_amd.backend import device_name
+
+    load, kernels = device_loaders(args)
     line = {"workload": args.workload, "device": device_name(0), "data": "synthetic (seeded generators)"}
     if args.workload == "c0":
         from oracle.pyoracle import Oracle
@@ -467,80 +581,18 @@ def other_workload(args):
                     cpu_baseline={"value": o.events / tcpu, "unit": "events/s", "cores": 1, "kind": "port",
                                   "net_steps_per_s": steps / tcpu, "sample": f"all {steps} steps"}, **others)
         o.close()
-    elif args.workload in ("c2", "c3"):
+    elif args.workload == "c2":
         steps = args.steps if args.steps != 10_000 else 1000
         agree = args.agree_steps
-        warm = max(args.warmup, agree)  # the plain and the batched runs time the same step range
-        n_img = (warm + steps) // 20 + 1
-        src_seed = 30 * 7919 + 17
-        # the plain FP32 path, one instance (the batch's instance 0): the denominator of the batching ratio
-        spec2 = workloads.stdp_learning(seed=30, n_images=n_img, source_seed=src_seed, instance=0)
-        g2 = load(spec2)
-        g2.record_spikes(0, agree)
-        g2.step(agree)
-        rec2, pot2 = g2.read_spike_record(0), g2.potentials(0)
-        g2.step(warm - agree)
-        dt2, ev2 = timed_steps(g2, steps)
-        k2 = kernels(g2, min(steps, 500))
-        g2.close()
-        c2_line = {"net_steps_per_s": steps / dt2, "events_per_s": ev2 / dt2, "steps": steps,
-                   "ms_per_step": dt2 / steps * 1e3, "kernels": k2, "path": "fp32 event-driven scatter"}
-        log(f"c2: {steps / dt2:.0f} steps/s")
-        if args.workload == "c2":
-            f64 = f64_runs([spec2], [0], agree)[0]
-            line.update(metric="network steps/s", value=steps / dt2, unit="net-steps/s", dtype="fp32",
-                        fp64=agreement_report(rec2, f64[0], pot2, f64[1], agree), **c2_line)
-        else:
-            B = args.batch
-            tg = time.perf_counter()
-            spec3 = workloads.batched_stdp(batch=B, n_images=n_img, dense=None if args.dense == "none" else args.dense,
-                                           split=args.split)
-            log(f"generated {B} instances ({spec3.n_synapses} synapses) in {time.perf_counter() - tg:.1f}s")
-            tl = time.perf_counter()
-            g = load(spec3)
-            log(f"loaded in {time.perf_counter() - tl:.1f}s")
-            check = list(range(min(B, args.agree_instances)))
-            g.record_spikes(0, agree)
-            g.step(agree)
-            rec = g.read_spike_record(0)
-            pot = g.potentials(0)
-            g.step(warm - agree)
-            dt3, ev3 = timed_steps(g, steps)
-            wb0, full = g.dense_weight_bytes()
-            k3 = kernels(g, min(steps, 500))
-            wb1, _ = g.dense_weight_bytes()
-            g.close()
-            ksteps = min(steps, 500)
-            inst_steps_per_s = B * steps / dt3
-            parts = [workloads.stdp_learning(seed=30 + b, n_images=n_img, source_seed=src_seed, instance=b)
-                     for b in check]
-            f64 = f64_runs(parts, check, agree)
-            per = [agreement_report(rec[:, b * 1024:(b + 1) * 1024], f64[i][0], pot[b * 1024:(b + 1) * 1024],
-                                    f64[i][1], agree) for i, b in enumerate(check)]
-            allg = np.concatenate([rec[:, b * 1024:(b + 1) * 1024] for b in check], axis=1)
-            allo = np.concatenate([x[0] for x in f64], axis=1)
-            horizons = [h for h in (100, 500, 1000, 2000, 5000) if h <= agree]
-            batch_agr = {str(h): round(agreement(allg[:h], allo[:h])[0], 6) for h in horizons}
-            plain_agr = {str(h): round(agreement(rec2[:h], f64[0][0][:h])[0], 6) for h in horizons}
-            roof = None
-            if "dense_expand" in k3:
-                d = k3["dense_expand"]
-                bytes_step = full + (wb1 - wb0) / ksteps
-                ach = bytes_step / (d["us_per_step"] * 1e-6) / 1e9
-                roof = {"kernel": "k_dense_counts + k_dense_expand (both MFMA projections)", "bound": "hbm",
-                        "achieved": round(ach, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                        "frac": round(ach / HBM_PEAK_GBS, 5), "weight_bytes_per_step": bytes_step,
-                        "dense_equivalent_bytes_per_step": float(B * 784 * 1024 * (4 if args.split else 2)),
-                        "useful_mfma_flops_per_step": 2.0 * B * 784 * 1024,
-                        "us_per_step": d["us_per_step"]}
-            line.update(metric="instance-steps/s", value=inst_steps_per_s, unit="instance-steps/s",
-                        dtype="fp32" if args.dense == "none" else
-                        ("bf16 hi+lo" if args.split else "bf16") + " weights x spike counts, fp32 accumulate",
-                        dense=args.dense, split=args.split, batch=B, steps=steps, ms_per_step=dt3 / steps * 1e3,
-                        events_per_s=ev3 / dt3, ratio_to_plain_fp32=inst_steps_per_s / (steps / dt2),
-                        plain_fp32=c2_line, kernels=k3, dense_roofline=roof,
-                        fp64={"instances": check, "steps": agree, "spike_agreement_by_horizon": batch_agr,
-                              "plain_fp32_instance0_by_horizon": plain_agr, "per_instance": per})
+        warm = max(args.warmup, agree)
+        spec2 = workloads.stdp_learning(seed=30, n_images=(warm + steps) // 20 + 1, source_seed=30 * 7919 + 17,
+                                        instance=0)
+        c2_line, rec2, pot2 = plain_stdp_run(spec2, load, kernels, agree, warm, steps)
+        f64 = f64_runs([spec2], [0], agree)[0]
+        line.update(metric="network steps/s", value=c2_line["net_steps_per_s"], unit="net-steps/s", dtype="fp32",
+                    fp64=agreement_report(rec2, f64[0], pot2, f64[1], agree), **c2_line)
+    elif args.workload == "c3":
+        line.update(measure_batched(args))
     elif args.workload == "c4":
         steps = args.steps if args.steps != 10_000 else 2000
         tg = time.perf_counter()

@@ -9,8 +9,11 @@ library surface as `BackendError` (a RuntimeError) carrying the library's messag
 
 There is no fallback: if the library is missing or no HIP device is present, construction raises.
 """
+import atexit
 import ctypes as C
 import os
+import sys
+import weakref
 
 import numpy as np
 
@@ -19,6 +22,15 @@ from . import abi
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libknp_mi355x.so")
 _LIB = None
+# Networks still open when the interpreter exits are destroyed by an exit hook, while the HIP runtime is still
+# up; a finalizer that runs later (interpreter teardown) makes no HIP call.
+_OPEN = weakref.WeakSet()
+
+
+@atexit.register
+def _close_open_networks():
+    for g in list(_OPEN):
+        g.close()
 
 
 class BackendError(RuntimeError):
@@ -162,6 +174,7 @@ class MI355XBackend:
         self.lib = load_library()
         self.net = C.c_void_p()
         self._chk(self.lib.knp_mi355x_create(device, C.byref(self.net)))
+        _OPEN.add(self)
         self.pop_sizes = []
         self.proj_sizes = []
         self._rec_cap = {}
@@ -171,8 +184,11 @@ class MI355XBackend:
         if self.net:
             self.lib.knp_mi355x_destroy(self.net)
             self.net = C.c_void_p()
+        _OPEN.discard(self)
 
     def __del__(self):
+        if sys.is_finalizing():
+            return  # the exit hook has closed it; no HIP call after the runtime's teardown
         try:
             self.close()
         except Exception:

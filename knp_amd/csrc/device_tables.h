@@ -191,6 +191,7 @@ struct ProjDev {
     const uint32_t *dn_tile;      // [dn_tiles] instance << 16 | first neuron of the tile / 32
     uint16_t *dn_w;              // BF16 weights; instance k from base: [octet][neuron (npad)][8 inputs]
     uint16_t *dn_w_lo;           // kDenseSplit: BF16 of w - dn_w (same layout), else null
+    unsigned long long *dn_part; // [kDenseCountBlocks][4] impacts, rows, weight bytes (skip) per counts workgroup
     uint32_t *dn_cnt;            // [2][dn_oct][8] spikes of each presynaptic index (one byte each) in the
                                  // step, by step parity; instance k from octet cbase
 };
@@ -201,6 +202,7 @@ struct alignas(8) DenseInst {
     uint32_t pre_begin, post_begin, post_count, npad, n_in, n_oct, cbase, pad_;
     uint64_t base;  // first BF16 weight of the instance in dn_w
 };
+constexpr uint32_t kDenseCountBlocks = 1024;  // workgroups of k_dense_counts at most
 constexpr uint32_t kDenseSkipEmpty = 1;  // flag: multiply only the octets of presynaptic indexes that spiked
 constexpr uint32_t kDenseSplit = 2;      // flag: each weight as two BF16 values (hi + lo), two products
 

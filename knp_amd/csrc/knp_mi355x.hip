@@ -118,6 +118,9 @@ struct knp_net {
     bool fused = false;     // every population tiled: phase 1 of step t + 1 runs inside the tile kernel of t
     bool persistent = false;  // fused, and every tile fits on the device at once (cooperative launch)
     bool wide_tiles = false;  // tiles outnumber the workgroups resident at once: the 6-waves tile kernel
+    // the MFMA expansions' spike counts were last staged in the device scratch (knp_mi355x_step_noalloc): the
+    // per-projection buffers must be cleared before the next knp_mi355x_step uses them
+    bool counts_in_scratch = false;
     uint32_t test_hold_block = 0xFFFFFFFFu, test_spin_limit = kSpinLimit;  // knp_mi355x_set_wait_test
     uint64_t test_hold_until = UINT64_MAX;
     uint32_t run_kind = POP_LEAN;  // population kind of the persistent launch
@@ -1136,6 +1139,7 @@ int init_network(knp_net *net) {
             if ((rc = upload(net, &tl, tiles.data(), tiles.size()))) return rc;
             if ((rc = upload(net, &wd, wb.data(), wb.size()))) return rc;
             if ((rc = dev_alloc(net, (void **)&Q.dn_cnt, (size_t)2 * B.n_oct * 8))) return rc;
+            if ((rc = dev_alloc(net, (void **)&Q.dn_part, (size_t)kDenseCountBlocks * 4 * 8))) return rc;
             Q.dense = 1 + H.dense_flags;
             Q.dn_oct = B.n_oct;
             Q.dn_inst = (uint32_t)B.inst.size();
@@ -1533,13 +1537,53 @@ int deliver_pushes(knp_net *net) {
     return KNP_OK;
 }
 
-int run_steps(knp_net *net, uint64_t n_steps) {
+void fill_u32(knp_net *net, uint32_t *p, uint64_t n, uint32_t v) {
+    const uint32_t g = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(1024, (n + kBlock - 1) / kBlock));
+    k_fill_u32<<<g, kBlock, 0, net->stream>>>(p, n, v);
+}
+
+// Words of spike-count staging the MFMA expansions of a network use per call (both parities).
+uint64_t dense_count_words(const knp_net *net) {
+    uint64_t w = 0;
+    for (uint32_t q = 0; q < net->projs.size(); ++q)
+        if (net->table.projs[q].dense) w += (uint64_t)net->table.projs[q].dn_oct * 4;
+    return w;
+}
+
+// The step loop. counts: null (the MFMA expansions stage their spike counts in their own buffers) or a device
+// region of dense_count_words(net) words that holds them instead (the caller clears it first); timing: HIP
+// events may be created for kernel timing.
+int run_steps(knp_net *net, uint64_t n_steps, uint32_t *counts = nullptr, bool timing = true) {
     if (!net->initialized) {
         int rc = init_network(net);
         if (rc) return rc;
     }
     HIP_TRY(hipSetDevice(net->device));
     NetDev *D = net->dev_table;
+    const bool timing_was = net->timing;
+    net->timing = net->timing && timing;
+    struct Restore {
+        knp_net *n;
+        bool t;
+        ~Restore() { n->timing = t; }
+    } restore{net, timing_was};
+    if (!counts && net->counts_in_scratch) {
+        // the previous call staged in the scratch: the buffers' current parity holds stale counts
+        for (uint32_t q = 0; q < net->projs.size(); ++q)
+            if (net->table.projs[q].dense) fill_u32(net, net->table.projs[q].dn_cnt, (uint64_t)net->table.projs[q].dn_oct * 4, 0);
+        HIP_TRY(hipGetLastError());
+    }
+    net->counts_in_scratch = counts != nullptr;
+    std::vector<uint32_t *> cnt(net->projs.size(), nullptr);
+    {
+        uint64_t off = 0;
+        for (uint32_t q = 0; q < net->projs.size(); ++q) {
+            const ProjDev &Qd = net->table.projs[q];
+            if (!Qd.dense) continue;
+            cnt[q] = counts ? counts + off : Qd.dn_cnt;
+            off += (uint64_t)Qd.dn_oct * 4;
+        }
+    }
     auto launch_dense_due = [&](uint64_t t) -> int {
         for (uint32_t d = 0; d < net->dense.size(); ++d) {
             const uint64_t t_first = net->pops[net->dense[d].post_pop].t_first;
@@ -1553,7 +1597,8 @@ int run_steps(knp_net *net, uint64_t n_steps) {
         // persistent launches, split where a dense projection starts a new time block (its product must be
         // ready before phase 1 of that step). The abort flag is cleared once per call: a launch that starts
         // with it raised returns at once, and the host reads it after the last launch.
-        HIP_TRY(hipMemsetAsync(net->table.grid_bar, 0, kBarWords * 4, net->stream));
+        fill_u32(net, net->table.grid_bar, kBarWords, 0);
+        HIP_TRY(hipGetLastError());
         uint64_t done = 0;
         while (done < n_steps) {
             const uint64_t t = net->step;
@@ -1624,15 +1669,16 @@ int run_steps(knp_net *net, uint64_t n_steps) {
             if (!Qd.dense) continue;
             const uint64_t words = (uint64_t)Qd.dn_oct * 2;
             const uint32_t gc = (uint32_t)std::max<uint64_t>(
-                1, std::min<uint64_t>(1024, std::max<uint64_t>(Qd.dn_v1 - Qd.dn_v0, (words + kBlock - 1) / kBlock)));
+                1, std::min<uint64_t>(kDenseCountBlocks, std::max<uint64_t>(Qd.dn_v1 - Qd.dn_v0, (words + kBlock - 1) / kBlock)));
             const uint32_t ge = (Qd.dn_tiles + 3) / 4;
             const bool skip = (Qd.dense - 1) & kDenseSkipEmpty, split = Qd.dn_w_lo != nullptr;
             if ((rc = timed_launch(net, KNP_KERNEL_DENSE_EXPAND, [&] {
-                     k_dense_counts<<<gc, kBlock, 0, net->stream>>>(D, q, t);
-                     if (skip && split) k_dense_expand<true, true><<<ge, kBlock, 0, net->stream>>>(D, q, t);
-                     else if (skip) k_dense_expand<true, false><<<ge, kBlock, 0, net->stream>>>(D, q, t);
-                     else if (split) k_dense_expand<false, true><<<ge, kBlock, 0, net->stream>>>(D, q, t);
-                     else k_dense_expand<false, false><<<ge, kBlock, 0, net->stream>>>(D, q, t);
+                     uint32_t *c = cnt[q];
+                     k_dense_counts<<<gc, kBlock, 0, net->stream>>>(D, q, t, c);
+                     if (skip && split) k_dense_expand<true, true><<<ge, kBlock, 0, net->stream>>>(D, q, t, c);
+                     else if (skip) k_dense_expand<true, false><<<ge, kBlock, 0, net->stream>>>(D, q, t, c);
+                     else if (split) k_dense_expand<false, true><<<ge, kBlock, 0, net->stream>>>(D, q, t, c);
+                     else k_dense_expand<false, false><<<ge, kBlock, 0, net->stream>>>(D, q, t, c);
                  })))
                 return rc;
         }
@@ -2512,6 +2558,16 @@ static int read_counters(knp_net *net, uint64_t *events, uint64_t *rows) {
         *events += part[2 * b];
         *rows += part[2 * b + 1];
     }
+    for (uint32_t q = 0; q < net->projs.size(); ++q) {
+        const ProjDev &Q = net->table.projs[q];
+        if (!Q.dense) continue;
+        std::vector<unsigned long long> dp(4 * (size_t)kDenseCountBlocks);
+        if ((rc = download(net, dp.data(), Q.dn_part, dp.size()))) return rc;
+        for (size_t b = 0; b < kDenseCountBlocks; ++b) {
+            *events += dp[4 * b];
+            *rows += dp[4 * b + 1];
+        }
+    }
     if (net->table.n_tiles) {
         std::vector<unsigned long long> tp(4 * (size_t)net->table.n_tiles);
         if ((rc = download(net, tp.data(), net->table.tile_part, tp.size()))) return rc;
@@ -2568,18 +2624,22 @@ int knp_mi355x_get_dense_weight_bytes(knp_net *net, uint64_t *skip_bytes, uint64
     if (skip_bytes) *skip_bytes = 0;
     if (full_bytes_per_step) *full_bytes_per_step = 0;
     if (!net->initialized) return KNP_OK;
-    unsigned long long c[4];
-    int rc = download(net, c, net->table.events, 4);
-    if (rc) return rc;
-    uint64_t full = 0;
+    int rc;
+    uint64_t full = 0, skip = 0;
     for (uint32_t q = 0; q < net->projs.size(); ++q) {
         const ProjDev &Q = net->table.projs[q];
-        if (!Q.dense || ((Q.dense - 1) & kDenseSkipEmpty)) continue;
+        if (!Q.dense) continue;
+        if ((Q.dense - 1) & kDenseSkipEmpty) {
+            std::vector<unsigned long long> dp(4 * (size_t)kDenseCountBlocks);
+            if ((rc = download(net, dp.data(), Q.dn_part, dp.size()))) return rc;
+            for (size_t b = 0; b < kDenseCountBlocks; ++b) skip += dp[4 * b + 2];
+            continue;
+        }
         std::vector<DenseInst> inst(Q.dn_inst);
         if ((rc = download(net, inst.data(), Q.dn_inst_tab, inst.size()))) return rc;
         for (const auto &I : inst) full += (uint64_t)I.n_oct * I.npad * (Q.dn_w_lo ? 32 : 16);
     }
-    if (skip_bytes) *skip_bytes = c[2];
+    if (skip_bytes) *skip_bytes = skip;
     if (full_bytes_per_step) *full_bytes_per_step = full;
     return KNP_OK;
 }

@@ -1871,11 +1871,14 @@ __global__ __launch_bounds__(kBlock) void k_dense_mfma(const NetDev *__restrict_
 // segments adds 1 to its byte (parity t & 1); the counts of parity (t + 1) & 1, read by the previous step's
 // product, are cleared for the next step. Synaptic-resource projections note the step each row spiked
 // (init_synapse, synaptic_resource_stdp.h:36-62). Counts the impacts (the instance's neurons per spike) and rows.
-__global__ __launch_bounds__(kBlock) void k_dense_counts(const NetDev *__restrict__ net, uint32_t q, uint64_t t) {
+// cnt: the projection's count buffer ([2][dn_oct][8] bytes: Q.dn_cnt, or a region of the library's device
+// scratch for knp_mi355x_step_noalloc).
+__global__ __launch_bounds__(kBlock) void k_dense_counts(const NetDev *__restrict__ net, uint32_t q, uint64_t t,
+                                                         uint32_t *cnt) {
     const ProjDev &Q = net->projs[q];
     const size_t words = (size_t)Q.dn_oct * 2;
-    uint32_t *cur = Q.dn_cnt + (size_t)(t & 1) * words;
-    uint32_t *nxt = Q.dn_cnt + (size_t)((t + 1) & 1) * words;
+    uint32_t *cur = cnt + (size_t)(t & 1) * words;
+    uint32_t *nxt = cnt + (size_t)((t + 1) & 1) * words;
     for (size_t k = (size_t)blockIdx.x * kBlock + threadIdx.x; k < words; k += (size_t)gridDim.x * kBlock) gp(nxt)[k] = 0;
     const uint32_t *vc = net->vcount + (size_t)(t & 1) * net->n_vseg;
     const size_t par = (size_t)(t & 1) * net->list_pstride;
@@ -1900,16 +1903,30 @@ __global__ __launch_bounds__(kBlock) void k_dense_counts(const NetDev *__restric
             rows += 1;
         }
     }
+    // the workgroup's totals into its own counters (no atomics on shared addresses: every workgroup would
+    // queue on them)
+    __shared__ unsigned long long red[3][kBlock / 64];
     for (int off = 32; off > 0; off >>= 1) {
         ev += __shfl_xor(ev, off);
         rows += __shfl_xor(rows, off);
         oct_bytes += __shfl_xor(oct_bytes, off);
     }
-    if (lane_id() == 0 && rows) {
-        atomicAdd(&net->events[0], ev);
-        atomicAdd(&net->events[1], rows);
-        if (Q.dense - 1 & kDenseSkipEmpty) atomicAdd(&net->events[2], oct_bytes);
+    if (lane_id() == 0) {
+        red[0][threadIdx.x / 64] = ev;
+        red[1][threadIdx.x / 64] = rows;
+        red[2][threadIdx.x / 64] = oct_bytes;
     }
+    __syncthreads();
+    if (threadIdx.x < 3) {
+        unsigned long long x = 0;
+        for (int w = 0; w < kBlock / 64; ++w) x += red[threadIdx.x][w];
+        if (x) gp(Q.dn_part)[4 * blockIdx.x + threadIdx.x] += x;
+    }
+}
+
+// p[0 .. n) = v (a kernel, not a memset node, so that captured sequences replay it correctly).
+__global__ __launch_bounds__(kBlock) void k_fill_u32(uint32_t *p, uint64_t n, uint32_t v) {
+    for (uint64_t k = (uint64_t)blockIdx.x * kBlock + threadIdx.x; k < n; k += (uint64_t)gridDim.x * kBlock) gp(p)[k] = v;
 }
 
 // 8 spike counts (one byte each, below 256) to 8 bf16 values (exact).
@@ -1930,7 +1947,8 @@ __device__ __forceinline__ v8s counts_to_bf16(uint64_t c) {
 // 32-63 the second), four K-steps per round with their loads issued first. SKIP: only octets holding a spike.
 // Every column of C is the neuron's sum; lanes 0-31 add register 0 (row 0) to the ring as fixed point.
 template <bool SKIP, bool SPLIT>
-__global__ __launch_bounds__(kBlock) void k_dense_expand(const NetDev *__restrict__ net, uint32_t q, uint64_t t) {
+__global__ __launch_bounds__(kBlock) void k_dense_expand(const NetDev *__restrict__ net, uint32_t q, uint64_t t,
+                                                         const uint32_t *cnt_base) {
     const ProjDev &Q = net->projs[q];
     const uint32_t tile = blockIdx.x * (kBlock / 64) + threadIdx.x / 64;
     if (tile >= Q.dn_tiles) return;
@@ -1939,7 +1957,7 @@ __global__ __launch_bounds__(kBlock) void k_dense_expand(const NetDev *__restric
     const uint32_t tv = gp(Q.dn_tile)[tile], k = tv >> 16, j0 = (tv & 0xFFFFu) * 32;
     const DenseInst I = Q.dn_inst_tab[k];
     const uint32_t n_oct = I.n_oct;
-    const KNP_GLOBAL uint64_t *cnt = (const KNP_GLOBAL uint64_t *)gp(Q.dn_cnt) + (size_t)(t & 1) * Q.dn_oct + I.cbase;
+    const KNP_GLOBAL uint64_t *cnt = (const KNP_GLOBAL uint64_t *)gp(cnt_base) + (size_t)(t & 1) * Q.dn_oct + I.cbase;
     const KNP_GLOBAL uint16_t *wb = gp(Q.dn_w) + I.base + (size_t)(j0 + col) * 8;
     const KNP_GLOBAL uint16_t *wl = SPLIT ? gp(Q.dn_w_lo) + I.base + (size_t)(j0 + col) * 8 : nullptr;
     const size_t ostride = (size_t)I.npad * 8;
