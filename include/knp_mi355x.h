@@ -331,6 +331,15 @@ int knp_mi355x_read_dense_weights(knp_net *net, uint32_t proj, uint16_t *out, ui
 
 /* ---- stepping (Backend::_step, backend.cpp:89-141) ------------------------------------------------ */
 int knp_mi355x_step(knp_net *net, uint64_t n_steps);
+/* No counterpart in the reference: knp_mi355x_step for callers that cannot allocate device memory (code under
+ * HIP stream capture, serving loops, harnesses that forbid hipMalloc). The network must be initialized already
+ * (knp_mi355x_init allocates its state; calling this first is KNP_ERR_STATE). The spike counts that the MFMA
+ * expansions (knp_mi355x_set_projection_dense) stage between their two kernels go to a file-scope device
+ * scratch of 1 MiB instead of the projections' own buffers: 16 bytes per group of 8 presynaptic indexes of
+ * every such projection, at most 1 MiB in all (more is KNP_ERR_RANGE). No kernel timing events are created.
+ * Every call on one GPU shares that GPU's scratch: calls that overlap on one GPU (threads, streams) race.
+ * Results equal knp_mi355x_step's. */
+int knp_mi355x_step_noalloc(knp_net *net, uint64_t n_steps);
 /* Backend::get_step (backend.h:290). */
 int knp_mi355x_get_step(knp_net *net, uint64_t *step);
 int knp_mi355x_synchronize(knp_net *net);

@@ -76,6 +76,7 @@ def _declare(lib):
         "knp_mi355x_read_dense_weights": (i32, [P, u32, C.POINTER(C.c_uint16), C.POINTER(C.c_uint16), u64]),
         "knp_mi355x_get_dense_weight_bytes": (i32, [P, C.POINTER(u64), C.POINTER(u64)]),
         "knp_mi355x_step": (i32, [P, u64]),
+        "knp_mi355x_step_noalloc": (i32, [P, u64]),
         "knp_mi355x_get_step": (i32, [P, C.POINTER(u64)]),
         "knp_mi355x_synchronize": (i32, [P]),
         "knp_mi355x_record_spikes": (i32, [P, u32, u64]),
@@ -340,6 +341,10 @@ class MI355XBackend:
         t = C.c_uint32()
         self._chk(self.lib.knp_mi355x_get_dense_time_block(self.net, dense, C.byref(t)))
         return t.value
+
+    def step_noalloc(self, n=1):
+        """knp_mi355x_step_noalloc: steps an initialized network without allocating device memory."""
+        self._chk(self.lib.knp_mi355x_step_noalloc(self.net, n))
 
     def step(self, n=1):
         self._chk(self.lib.knp_mi355x_step(self.net, n))

@@ -14,6 +14,7 @@
 #include <cstring>
 #include <fstream>
 #include <limits>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -1790,6 +1791,31 @@ int add_projection_common(knp_net *net, uint32_t kind, uint32_t pre_pop, uint32_
 
 }  // namespace
 
+// ---- device scratch of knp_mi355x_step_noalloc ----------------------------------------------------------
+// Staging for the spike counts of the MFMA expansions (knp_mi355x_set_projection_dense) in a file-scope device
+// array, so that the no-allocation step needs no buffer of its own: every GPU holds one copy until the process
+// ends. In host code `scratch` names a host-side stand-in, not device memory; its device address is taken once
+// per device with hipGetSymbolAddress and only that address is used.
+namespace {
+constexpr uint64_t kScratchBytes = 1u << 20;
+__device__ __align__(256) uint32_t scratch[kScratchBytes / 4];
+std::mutex g_scratch_mu;
+std::vector<uint32_t *> g_scratch_ptr;  // per device, null until first use
+
+int scratch_pointer(int device, uint32_t **out) {
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if ((int)g_scratch_ptr.size() <= device) g_scratch_ptr.resize(device + 1, nullptr);
+    if (!g_scratch_ptr[device]) {
+        HIP_TRY(hipSetDevice(device));  // the address is the current device's copy
+        void *p = nullptr;
+        HIP_TRY(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        g_scratch_ptr[device] = static_cast<uint32_t *>(p);
+    }
+    *out = g_scratch_ptr[device];
+    return KNP_OK;
+}
+}  // namespace
+
 // =====================================================================================================
 // C ABI
 // =====================================================================================================
@@ -2202,6 +2228,27 @@ int knp_mi355x_get_dense_time_block(knp_net *net, uint32_t dense, uint32_t *time
 int knp_mi355x_step(knp_net *net, uint64_t n_steps) {
     if (!net) return fail(KNP_ERR_INVALID_ARGUMENT, "null handle");
     return run_steps(net, n_steps);
+}
+
+int knp_mi355x_step_noalloc(knp_net *net, uint64_t n_steps) {
+    if (!net) return fail(KNP_ERR_INVALID_ARGUMENT, "null handle");
+    if (!net->initialized)
+        return fail(KNP_ERR_STATE, "knp_mi355x_step_noalloc needs an initialized network (knp_mi355x_init allocates)");
+    const uint64_t words = dense_count_words(net);
+    if (words * 4 > kScratchBytes)
+        return fail(KNP_ERR_RANGE, "knp_mi355x_step_noalloc: the MFMA expansions stage " + std::to_string(words * 4) +
+                                       " bytes of spike counts, above the " + std::to_string(kScratchBytes) +
+                                       "-byte device scratch");
+    HIP_TRY(hipSetDevice(net->device));
+    uint32_t *sp = nullptr;
+    int rc = scratch_pointer(net->device, &sp);
+    if (rc) return rc;
+    // the scratch keeps whatever the last call on this device left: the counts start at zero
+    if (words) {
+        fill_u32(net, sp, words, 0);
+        HIP_TRY(hipGetLastError());
+    }
+    return run_steps(net, n_steps, sp, false);
 }
 
 int knp_mi355x_get_step(knp_net *net, uint64_t *step) {
