@@ -340,6 +340,15 @@ int knp_mi355x_step(knp_net *net, uint64_t n_steps);
  * Every call on one GPU shares that GPU's scratch: calls that overlap on one GPU (threads, streams) race.
  * Results equal knp_mi355x_step's. */
 int knp_mi355x_step_noalloc(knp_net *net, uint64_t n_steps);
+/* Captured step graphs (no reference counterpart: an execution choice with identical results). Where a call
+ * runs one launch per step (populations that are not all tiled, or tiles that do not all fit on the device at
+ * once), the launches of `steps` consecutive steps are captured once into a HIP graph, whose kernels read the
+ * step from device memory, and replayed for the middle of every longer call (the first and last steps of a
+ * call run as plain launches; pushed channel spikes are delivered at the first). With dense input projections
+ * a graph holds one time block. 0 disables graphs; the default is 32. Kernel timing turns them off. */
+int knp_mi355x_set_step_graph(knp_net *net, uint32_t steps);
+/* The steps of the captured graph (0: none yet) and the number of replays so far. */
+int knp_mi355x_get_step_graph(knp_net *net, uint32_t *steps, uint64_t *replays);
 /* Backend::get_step (backend.h:290). */
 int knp_mi355x_get_step(knp_net *net, uint64_t *step);
 int knp_mi355x_synchronize(knp_net *net);

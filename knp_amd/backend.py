@@ -77,6 +77,8 @@ def _declare(lib):
         "knp_mi355x_get_dense_weight_bytes": (i32, [P, C.POINTER(u64), C.POINTER(u64)]),
         "knp_mi355x_step": (i32, [P, u64]),
         "knp_mi355x_step_noalloc": (i32, [P, u64]),
+        "knp_mi355x_set_step_graph": (i32, [P, u32]),
+        "knp_mi355x_get_step_graph": (i32, [P, pu32, C.POINTER(u64)]),
         "knp_mi355x_get_step": (i32, [P, C.POINTER(u64)]),
         "knp_mi355x_synchronize": (i32, [P]),
         "knp_mi355x_record_spikes": (i32, [P, u32, u64]),
@@ -341,6 +343,16 @@ class MI355XBackend:
         t = C.c_uint32()
         self._chk(self.lib.knp_mi355x_get_dense_time_block(self.net, dense, C.byref(t)))
         return t.value
+
+    def set_step_graph(self, steps):
+        """Steps per captured HIP graph of one-launch-per-step runs (0: no graphs)."""
+        self._chk(self.lib.knp_mi355x_set_step_graph(self.net, steps))
+
+    def step_graph(self):
+        """(steps of the captured graph or 0, replays so far)."""
+        g, r = C.c_uint32(), C.c_uint64()
+        self._chk(self.lib.knp_mi355x_get_step_graph(self.net, C.byref(g), C.byref(r)))
+        return g.value, r.value
 
     def step_noalloc(self, n=1):
         """knp_mi355x_step_noalloc: steps an initialized network without allocating device memory."""

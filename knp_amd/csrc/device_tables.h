@@ -307,6 +307,7 @@ struct NetDev {
     uint32_t n_dense;
     DenseDev dense[kMaxDense];
     unsigned long long *events;  // [0] impacts, [1] rows expanded by the dense path (scatter: part)
+    unsigned long long *step_ptr;  // the current step on the device (kernels launched from a captured graph read it)
 };
 
 }  // namespace knp_mi355x

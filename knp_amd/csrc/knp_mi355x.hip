@@ -122,6 +122,13 @@ struct knp_net {
     // the MFMA expansions' spike counts were last staged in the device scratch (knp_mi355x_step_noalloc): the
     // per-projection buffers must be cleared before the next knp_mi355x_step uses them
     bool counts_in_scratch = false;
+    // captured step graph (knp_mi355x_set_step_graph): the instantiated graph, its steps, whether its MFMA
+    // expansions stage in the device scratch, replays so far, and the steps per graph without dense projections
+    hipGraphExec_t graph_exec = nullptr;
+    uint32_t graph_steps = 0;
+    bool graph_scratch = false;
+    uint64_t graph_replays = 0;
+    uint32_t graph_default = 32;
     uint32_t test_hold_block = 0xFFFFFFFFu, test_spin_limit = kSpinLimit;  // knp_mi355x_set_wait_test
     uint64_t test_hold_until = UINT64_MAX;
     uint32_t run_kind = POP_LEAN;  // population kind of the persistent launch
@@ -1414,6 +1421,7 @@ int init_network(knp_net *net) {
     int rc;
     if ((rc = dev_alloc(net, (void **)&T.ctl, 2 * sizeof(Ctl)))) return rc;
     if ((rc = dev_alloc(net, (void **)&T.events, 32))) return rc;
+    if ((rc = dev_alloc(net, (void **)&T.step_ptr, 8))) return rc;
 
     // ---- dense input projections
     T.n_dense = (uint32_t)net->dense.size();
@@ -1543,6 +1551,25 @@ void fill_u32(knp_net *net, uint32_t *p, uint64_t n, uint32_t v) {
     k_fill_u32<<<g, kBlock, 0, net->stream>>>(p, n, v);
 }
 
+// Steps per captured graph of a network's stepwise loop: the dense input projections' time block (they must all
+// share it), else the chosen default; 0 = no graph.
+uint32_t graph_steps(const knp_net *net) {
+    if (net->graph_default == 0) return 0;
+    uint32_t T = 0;
+    for (const auto &Dh : net->dense) {
+        if (T && Dh.T != T) return 0;
+        T = Dh.T;
+    }
+    return T ? T : net->graph_default;
+}
+
+// Phase of step t in the dense time blocks (0 = a block starts at t; every dense projection's population
+// started with the network, so they share it); 0 without dense input projections.
+uint64_t graph_phase(const knp_net *net, uint64_t t, uint32_t G) {
+    if (net->dense.empty()) return 0;
+    return (t - net->pops[net->dense[0].post_pop].t_first) % G;
+}
+
 // Words of spike-count staging the MFMA expansions of a network use per call (both parities).
 uint64_t dense_count_words(const knp_net *net) {
     uint64_t w = 0;
@@ -1585,11 +1612,12 @@ int run_steps(knp_net *net, uint64_t n_steps, uint32_t *counts = nullptr, bool t
             off += (uint64_t)Qd.dn_oct * 4;
         }
     }
-    auto launch_dense_due = [&](uint64_t t) -> int {
+    // t: the step (host decisions); targ: the kernels' argument for it (t, or kStepDev + offset in a graph)
+    auto launch_dense_due = [&](uint64_t t, uint64_t targ) -> int {
         for (uint32_t d = 0; d < net->dense.size(); ++d) {
             const uint64_t t_first = net->pops[net->dense[d].post_pop].t_first;
             if ((t - t_first) % net->dense[d].T != 0) continue;
-            int rc = timed_launch(net, KNP_KERNEL_DENSE, [&] { launch_dense(net, d, t); });
+            int rc = timed_launch(net, KNP_KERNEL_DENSE, [&] { launch_dense(net, d, targ); });
             if (rc) return rc;
         }
         return KNP_OK;
@@ -1605,7 +1633,7 @@ int run_steps(knp_net *net, uint64_t n_steps, uint32_t *counts = nullptr, bool t
             const uint64_t t = net->step;
             int rc = deliver_pushes(net);
             if (rc) return rc;
-            if ((rc = launch_dense_due(t))) return rc;
+            if ((rc = launch_dense_due(t, t))) return rc;
             // 2^16 steps per launch at most: the launch's per-thread impact counters are 32-bit
             uint64_t len = std::min<uint64_t>(n_steps - done, uint64_t(1) << 16);
             for (const auto &Dh : net->dense) {
@@ -1637,17 +1665,16 @@ int run_steps(knp_net *net, uint64_t n_steps, uint32_t *counts = nullptr, bool t
         }
         return KNP_OK;
     }
-    // phase 1 of the current step already ran inside the previous step's tile kernel
-    bool phase1_done = false;
-    for (uint64_t it = 0; it < n_steps; ++it) {
-        const uint64_t t = net->step;
-        int rc = deliver_pushes(net);
-        if (rc) return rc;
+    // The launches of one step. t: the step (host decisions); targ: the kernels' step argument (t, or kStepDev
+    // in a captured graph); phase1_done: phase 1 of step t already ran (fused into the previous tile kernel);
+    // p1: the tile kernel of step t also runs phase 1 of step t + 1.
+    auto launch_step = [&](uint64_t t, uint64_t targ, bool phase1_done, bool p1) -> int {
+        int rc;
         if (!phase1_done) {
-            if ((rc = launch_dense_due(t))) return rc;
+            if ((rc = launch_dense_due(t, targ))) return rc;
             const uint32_t blocks = net->table.total_blocks;
             if ((rc = timed_launch(net, KNP_KERNEL_NEURON, [&] {
-                     k_phase1<<<blocks, kBlock, 0, net->stream>>>(D, t);
+                     k_phase1<<<blocks, kBlock, 0, net->stream>>>(D, targ);
                  })))
                 return rc;
         }
@@ -1655,13 +1682,13 @@ int run_steps(knp_net *net, uint64_t n_steps, uint32_t *counts = nullptr, bool t
             if (!net->pops[p].resource) continue;
             const uint32_t g = net->train_grid;
             if ((rc = timed_launch(net, KNP_KERNEL_TRAIN, [&] {
-                     k_train<<<g, kBlock, 0, net->stream>>>(D, p, t);
+                     k_train<<<g, kBlock, 0, net->stream>>>(D, p, targ);
                  })))
                 return rc;
         }
         for (uint32_t q = 0; q < net->projs.size(); ++q) {
             if (net->projs[q].kind != PROJ_ADDITIVE || net->projs[q].stdp_pops.empty()) continue;
-            k_additive_append<<<net->scatter_grid, kBlock, 0, net->stream>>>(D, q, t);
+            k_additive_append<<<net->scatter_grid, kBlock, 0, net->stream>>>(D, q, targ);
             HIP_TRY(hipGetLastError());
         }
         // projections expanded on MFMA (after training: they use the step's trained weights)
@@ -1675,41 +1702,99 @@ int run_steps(knp_net *net, uint64_t n_steps, uint32_t *counts = nullptr, bool t
             const bool skip = (Qd.dense - 1) & kDenseSkipEmpty, split = Qd.dn_w_lo != nullptr;
             if ((rc = timed_launch(net, KNP_KERNEL_DENSE_EXPAND, [&] {
                      uint32_t *c = cnt[q];
-                     k_dense_counts<<<gc, kBlock, 0, net->stream>>>(D, q, t, c);
-                     if (skip && split) k_dense_expand<true, true><<<ge, kBlock, 0, net->stream>>>(D, q, t, c);
-                     else if (skip) k_dense_expand<true, false><<<ge, kBlock, 0, net->stream>>>(D, q, t, c);
-                     else if (split) k_dense_expand<false, true><<<ge, kBlock, 0, net->stream>>>(D, q, t, c);
-                     else k_dense_expand<false, false><<<ge, kBlock, 0, net->stream>>>(D, q, t, c);
+                     k_dense_counts<<<gc, kBlock, 0, net->stream>>>(D, q, targ, c);
+                     if (skip && split) k_dense_expand<true, true><<<ge, kBlock, 0, net->stream>>>(D, q, targ, c);
+                     else if (skip) k_dense_expand<true, false><<<ge, kBlock, 0, net->stream>>>(D, q, targ, c);
+                     else if (split) k_dense_expand<false, true><<<ge, kBlock, 0, net->stream>>>(D, q, targ, c);
+                     else k_dense_expand<false, false><<<ge, kBlock, 0, net->stream>>>(D, q, targ, c);
                  })))
                 return rc;
         }
         const uint32_t sg = net->scatter_grid;
         if (net->table.n_vseg_atomic &&
             (rc = timed_launch(net, KNP_KERNEL_SCATTER, [&] {
-                 k_scatter<<<sg, kBlock, (net->table.n_vseg_atomic + 1) * 4, net->stream>>>(D, t);
+                 k_scatter<<<sg, kBlock, (net->table.n_vseg_atomic + 1) * 4, net->stream>>>(D, targ);
              })))
             return rc;
-        // fused: the tile kernel of step t also runs phase 1 of step t + 1, except on the last step of the call
-        // (the state a caller reads between calls is the state after whole steps)
-        const bool p1 = net->fused && it + 1 < n_steps;
-        if (p1 && (rc = launch_dense_due(t + 1))) return rc;
+        if (p1 && (rc = launch_dense_due(t + 1, targ + 1))) return rc;
         const uint32_t nt = net->table.n_tiles;
         if (nt && (rc = timed_launch(net, KNP_KERNEL_TILE_SCATTER, [&] {
                        const uint32_t lds = net->tile_lds;
-                       if (p1 && net->wide_tiles) k_tile_scatter<true, true><<<nt, kTileThreads, lds, net->stream>>>(D, t);
-                       else if (p1) k_tile_scatter<true, false><<<nt, kTileThreads, lds, net->stream>>>(D, t);
-                       else if (net->wide_tiles) k_tile_scatter<false, true><<<nt, kTileThreads, lds, net->stream>>>(D, t);
-                       else k_tile_scatter<false, false><<<nt, kTileThreads, lds, net->stream>>>(D, t);
+                       if (p1 && net->wide_tiles) k_tile_scatter<true, true><<<nt, kTileThreads, lds, net->stream>>>(D, targ);
+                       else if (p1) k_tile_scatter<true, false><<<nt, kTileThreads, lds, net->stream>>>(D, targ);
+                       else if (net->wide_tiles) k_tile_scatter<false, true><<<nt, kTileThreads, lds, net->stream>>>(D, targ);
+                       else k_tile_scatter<false, false><<<nt, kTileThreads, lds, net->stream>>>(D, targ);
                    })))
             return rc;
-        phase1_done = p1;
         for (uint32_t q = 0; q < net->projs.size(); ++q) {
             if (net->projs[q].kind != PROJ_ADDITIVE) continue;
             const uint32_t g = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(4096, (net->projs[q].n_syn + kBlock - 1) / kBlock));
             k_additive_modify<<<g, kBlock, 0, net->stream>>>(D, q);
             HIP_TRY(hipGetLastError());
         }
+        return KNP_OK;
+    };
+    // Graph replays: G steps captured once (every step's launches with the device's step, then its advance)
+    // and replayed while more than G steps remain after the call's first step, from steps whose dense time
+    // blocks are in the captured phase (none without dense projections).
+    const uint32_t G = net->timing ? 0 : graph_steps(net);
+    const bool staged_in_scratch = counts != nullptr;
+    // phase 1 of the current step already ran inside the previous step's tile kernel
+    bool phase1_done = false;
+    for (uint64_t it = 0; it < n_steps;) {
+        const uint64_t t = net->step;
+        if (G && it >= 1 && n_steps - it > G && graph_phase(net, t, G) == 0 && phase1_done == net->fused) {
+            if (net->graph_exec && (net->graph_steps != G || net->graph_scratch != staged_in_scratch)) {
+                (void)hipGraphExecDestroy(net->graph_exec);
+                net->graph_exec = nullptr;
+            }
+            if (!net->graph_exec) {
+                hipGraph_t graph = nullptr;
+                HIP_TRY(hipStreamBeginCapture(net->stream, hipStreamCaptureModeThreadLocal));
+                int rc = KNP_OK;
+                for (uint32_t i = 0; i < G && !rc; ++i) {
+                    rc = launch_step(t + i, kStepDev, net->fused, net->fused);
+                    if (!rc) {
+                        k_step_advance<<<1, 64, 0, net->stream>>>(D);
+                        if (hipGetLastError() != hipSuccess) rc = fail(KNP_ERR_HIP, "graph capture: launch failed");
+                    }
+                }
+                const hipError_t ce = hipStreamEndCapture(net->stream, &graph);
+                if (rc) {
+                    if (graph) (void)hipGraphDestroy(graph);
+                    return rc;
+                }
+                if (ce != hipSuccess) return fail(KNP_ERR_HIP, std::string("graph capture: ") + hipGetErrorString(ce));
+                const hipError_t ie = hipGraphInstantiate(&net->graph_exec, graph, nullptr, nullptr, 0);
+                (void)hipGraphDestroy(graph);
+                if (ie != hipSuccess) {
+                    net->graph_exec = nullptr;
+                    return fail(KNP_ERR_HIP, std::string("graph instantiation: ") + hipGetErrorString(ie));
+                }
+                net->graph_steps = G;
+                net->graph_scratch = staged_in_scratch;
+            }
+            k_set_step<<<1, 64, 0, net->stream>>>(D, t);
+            HIP_TRY(hipGetLastError());
+            while (n_steps - it > G) {
+                HIP_TRY(hipGraphLaunch(net->graph_exec, net->stream));
+                net->graph_replays++;
+                net->step += G;
+                it += G;
+            }
+            for (auto &P : net->pops) P.ran = true;
+            phase1_done = net->fused;
+            continue;
+        }
+        int rc = deliver_pushes(net);
+        if (rc) return rc;
+        // fused: the tile kernel of step t also runs phase 1 of step t + 1, except on the last step of the call
+        // (the state a caller reads between calls is the state after whole steps)
+        const bool p1 = net->fused && it + 1 < n_steps;
+        if ((rc = launch_step(t, t, phase1_done, p1))) return rc;
+        phase1_done = p1;
         net->step++;
+        ++it;
         for (auto &P : net->pops) P.ran = true;
         if (net->timing && net->ev_used > 8192) {
             if ((rc = collect_timing(net))) return rc;
@@ -1922,6 +2007,7 @@ int knp_mi355x_destroy(knp_net *net) {
     if (!net) return KNP_OK;
     (void)hipSetDevice(net->device);
     if (net->stream) (void)hipStreamSynchronize(net->stream);
+    if (net->graph_exec) (void)hipGraphExecDestroy(net->graph_exec);
     for (void *p : net->allocations) (void)hipFree(p);
     for (hipEvent_t e : net->ev_pool) (void)hipEventDestroy(e);
     if (net->stream) (void)hipStreamDestroy(net->stream);
@@ -2717,6 +2803,20 @@ int knp_mi355x_read_dense_weights(knp_net *net, uint32_t proj, uint16_t *out, ui
         out[s] = wb[pos];
         if (out_lo) out_lo[s] = wlo.empty() ? 0 : wlo[pos];
     }
+    return KNP_OK;
+}
+
+int knp_mi355x_set_step_graph(knp_net *net, uint32_t steps) {
+    if (!net) return fail(KNP_ERR_INVALID_ARGUMENT, "null handle");
+    if (steps > 4096) return fail(KNP_ERR_INVALID_ARGUMENT, "at most 4096 steps per graph");
+    net->graph_default = steps;
+    return KNP_OK;
+}
+
+int knp_mi355x_get_step_graph(knp_net *net, uint32_t *steps, uint64_t *replays) {
+    if (!net) return fail(KNP_ERR_INVALID_ARGUMENT, "null handle");
+    if (steps) *steps = net->graph_exec ? net->graph_steps : 0;
+    if (replays) *replays = net->graph_replays;
     return KNP_OK;
 }
 

@@ -42,6 +42,13 @@ __host__ __device__ inline uint32_t bernoulli_hash(uint64_t seed, uint64_t step,
     return (uint32_t)(b >> 32);
 }
 
+// A kernel's step argument: an absolute step, or kStepDev + d for "the device's current step + d" (the form the
+// launches of a captured graph take, so that one graph serves every replay; k_step_advance moves the step).
+constexpr uint64_t kStepDev = 1ull << 63;
+__device__ __forceinline__ uint64_t resolve_step(const NetDev *__restrict__ net, uint64_t t) {
+    return t >= kStepDev ? (uint64_t)*gp(net->step_ptr) + (t - kStepDev) : t;
+}
+
 __device__ __forceinline__ float pf(const ParamF &p, uint32_t i) { return p.a ? gp(p.a)[i] : p.v; }
 __device__ __forceinline__ uint32_t pu(const ParamU &p, uint32_t i) { return p.a ? gp(p.a)[i] : p.v; }
 
@@ -485,6 +492,7 @@ __device__ __forceinline__ uint32_t source_step(const NetDev *__restrict__ net, 
 }
 
 __global__ __launch_bounds__(kBlock) void k_phase1(const NetDev *__restrict__ net, uint64_t t) {
+    t = resolve_step(net, t);
     Ctl *ctl = &net->ctl[t & 1];
     uint32_t *seg_count = net->seg_count + (size_t)(t & 1) * net->n_seg;
     if (blockIdx.x == 0) {
@@ -567,6 +575,7 @@ __device__ __forceinline__ void impact(const PopDev &P, const ProjDev &Q, const 
 }
 
 __global__ __launch_bounds__(kBlock) void k_scatter(const NetDev *__restrict__ net, uint64_t t) {
+    t = resolve_step(net, t);
     extern __shared__ uint32_t pfx[];  // [n_vseg + 1]
     __shared__ uint32_t wsum[kBlock / 64];
     const uint32_t nv = net->n_vseg_atomic;
@@ -1220,6 +1229,7 @@ __device__ __forceinline__ void tile_flush_counts(const NetDev *__restrict__ net
 template <bool PHASE1, bool WIDE>
 __global__ __launch_bounds__(kTileThreads, WIDE ? 6 : 1) void k_tile_scatter(const NetDev *__restrict__ net,
                                                                               uint64_t t) {
+    t = resolve_step(net, t);
     extern __shared__ unsigned long long tacc[];
     __shared__ TileShared sh;
     if (blockIdx.x == 0) clear_push_counts<false>(net, t + 1);
@@ -1444,6 +1454,7 @@ __device__ __forceinline__ float serial_free(float freer, uint64_t active, uint6
 constexpr uint32_t kTrainChunks = 1;
 
 __global__ __launch_bounds__(kBlock) void k_train(const NetDev *__restrict__ net, uint32_t p, uint64_t t) {
+    t = resolve_step(net, t);
     const PopDev &P = net->pops[p];
     const ResDev &R = P.res;
     const Ctl *ctl = &net->ctl[t & 1];
@@ -1657,6 +1668,7 @@ __global__ __launch_bounds__(kBlock) void k_train(const NetDev *__restrict__ net
 // init_projection: spikes of the projection's stdp_populations_ append the step to the spike histories of
 // every synapse whose POSTsynaptic index equals the spiking neuron's index (additive_stdp.h:149-160).
 __global__ __launch_bounds__(kBlock) void k_additive_append(const NetDev *__restrict__ net, uint32_t q, uint64_t t) {
+    t = resolve_step(net, t);
     const ProjDev &Q = net->projs[q];
     const uint32_t *cnt = net->seg_count + (size_t)(t & 1) * net->n_seg;
     const uint32_t lane = lane_id();
@@ -1725,6 +1737,7 @@ typedef short v8s __attribute__((ext_vector_type(8)));
 // up to the row pitch are 0). Counts the dense synaptic events and rows of the block (one atomic pair per
 // wave).
 __global__ __launch_bounds__(kBlock) void k_dense_spikes(const NetDev *__restrict__ net, uint32_t d, uint64_t t0) {
+    t0 = resolve_step(net, t0);
     const DenseDev &Dn = net->dense[d];
     const uint32_t pitch = Dn.pitch, used = 2u * Dn.ksteps;
     const uint64_t total = (uint64_t)Dn.T * Dn.n_inst * pitch;
@@ -1783,6 +1796,7 @@ __device__ __forceinline__ v8s spread_bf16_ones(uint32_t byte) {
 // column n) of a 32x32 accumulator sits in lane n + 32 * ((m / 4) % 2), register 4 * (m / 8) + m % 4.
 template <int FMT>
 __global__ __launch_bounds__(kBlock) void k_dense_mfma(const NetDev *__restrict__ net, uint32_t d, uint64_t t0) {
+    t0 = resolve_step(net, t0);
     const DenseDev &Dn = net->dense[d];
     const PopDev &P = net->pops[Dn.post_pop];
     constexpr uint32_t FB = FMT == 0 ? 8 : 16;  // fragment bytes per lane per K-step
@@ -1875,6 +1889,7 @@ __global__ __launch_bounds__(kBlock) void k_dense_mfma(const NetDev *__restrict_
 // scratch for knp_mi355x_step_noalloc).
 __global__ __launch_bounds__(kBlock) void k_dense_counts(const NetDev *__restrict__ net, uint32_t q, uint64_t t,
                                                          uint32_t *cnt) {
+    t = resolve_step(net, t);
     const ProjDev &Q = net->projs[q];
     const size_t words = (size_t)Q.dn_oct * 2;
     uint32_t *cur = cnt + (size_t)(t & 1) * words;
@@ -1924,6 +1939,14 @@ __global__ __launch_bounds__(kBlock) void k_dense_counts(const NetDev *__restric
     }
 }
 
+// The device's current step: set by the host before graph replays, advanced at the end of every captured step.
+__global__ void k_set_step(const NetDev *__restrict__ net, uint64_t t) {
+    if (threadIdx.x == 0) *gp(net->step_ptr) = t;
+}
+__global__ void k_step_advance(const NetDev *__restrict__ net) {
+    if (threadIdx.x == 0) *gp(net->step_ptr) += 1;
+}
+
 // p[0 .. n) = v (a kernel, not a memset node, so that captured sequences replay it correctly).
 __global__ __launch_bounds__(kBlock) void k_fill_u32(uint32_t *p, uint64_t n, uint32_t v) {
     for (uint64_t k = (uint64_t)blockIdx.x * kBlock + threadIdx.x; k < n; k += (uint64_t)gridDim.x * kBlock) gp(p)[k] = v;
@@ -1949,6 +1972,7 @@ __device__ __forceinline__ v8s counts_to_bf16(uint64_t c) {
 template <bool SKIP, bool SPLIT>
 __global__ __launch_bounds__(kBlock) void k_dense_expand(const NetDev *__restrict__ net, uint32_t q, uint64_t t,
                                                          const uint32_t *cnt_base) {
+    t = resolve_step(net, t);
     const ProjDev &Q = net->projs[q];
     const uint32_t tile = blockIdx.x * (kBlock / 64) + threadIdx.x / 64;
     if (tile >= Q.dn_tiles) return;
