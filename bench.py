@@ -69,6 +69,9 @@ def main():
                          "event-driven scatter (none)")
     ap.add_argument("--no-split", dest="split", action="store_false",
                     help="c3: one BF16 value per weight (default: two parts, hi + lo, 16 significant bits)")
+    ap.add_argument("--train-wg-limit", type=int, default=None,
+                    help="synaptic-resource training: population size up to which each candidate gets a workgroup "
+                         "(beyond it, a wave); default: the library's")
     ap.add_argument("--scatter", choices=["auto", "stepwise", "atomic"], default="auto",
                     help="impact summation: tiled populations and persistent launches where eligible (auto), "
                          "tiled with one launch per step (stepwise), or atomics everywhere")
@@ -109,6 +112,7 @@ def run_c1(args):
     gpu._init()
     gpu.synchronize()
     log(f"uploaded in {time.perf_counter() - t_up:.1f}s on {device_name(0)}")
+    exactness = {"accumulator_frac_bits": gpu.accumulator_bits(pop), "inexact_weights": gpu.inexact_weights(pop)}
 
     # the compared run: steps 0 .. P-1 recorded (untimed), then the warmup
     P = max(1, args.parity_steps)
@@ -305,6 +309,7 @@ def run_c1(args):
         "cpu_baseline_mt": cpu_mt,
         "cpu_baseline_release": cpu_release,
         "parity": parity,
+        "exactness": exactness,
         "device": device_name(0),
     }
 
@@ -422,6 +427,8 @@ def device_loaders(args):
         g = MI355XBackend(0)
         g.set_scatter_mode({"auto": abi.SCATTER_AUTO, "stepwise": abi.SCATTER_STEPWISE,
                             "atomic": abi.SCATTER_ATOMIC}[args.scatter])
+        if args.train_wg_limit is not None:
+            g.set_train_workgroup_limit(args.train_wg_limit)
         spec.load_into(g)
         g._init()
         g.synchronize()
@@ -614,17 +621,27 @@ def other_workload(args):
             mfma = {"kernel": "k_dense_mfma (+ spike bits)", "bound": "mfma", "achieved": round(ach, 2),
                     "peak": 5000.0, "unit": "TFLOP/s", "frac": round(ach / 5000.0, 5), "time_block": T,
                     "flops_per_launch": flops}
-        # agreement on a reduced batch (the FP64 oracle expands every dense weight into a synapse)
-        small = workloads.uneven_inference(n_instances=16, steps=args.agree_steps)
+        # agreement on a reduced batch (the FP64 oracle expands every dense weight into a synapse): against the
+        # FP64 oracle with the same FP8 weights (the kernels' exactness), and with the weights before
+        # quantization (what FP8 costs), by horizon
+        agree = min(args.agree_steps, 2000)
+        small = workloads.uneven_inference(n_instances=16, steps=agree)
         gs = load(small)
-        gs.record_spikes(0, args.agree_steps)
-        gs.step(args.agree_steps)
-        fp64 = compare_f64(small, gs.read_spike_record(0), gs.potentials(0), 0, args.agree_steps)
-        fp64["instances"] = 16
+        gs.record_spikes(0, agree)
+        gs.step(agree)
+        rec, pot = gs.read_spike_record(0), gs.potentials(0)
         gs.close()
+        runs = f64_runs([small, workloads.unquantized_dense(small)], [0, 1], agree)
+        horizons = [h for h in (100, 500, 2000) if h <= agree]
+        fp64 = agreement_report(rec, runs[0][0], pot, runs[0][1], agree)
+        fp64["instances"] = 16
+        unq = agreement_report(rec, runs[1][0], pot, runs[1][1], agree)
+        unq["by_horizon"] = {str(h): round(agreement(rec[:h], runs[1][0][:h])[0], 6) for h in horizons}
+        unq["instances"] = 16
         line.update(metric="synaptic events/s", value=ev / dt, unit="events/s", dtype="fp32 sparse + fp8 dense",
                     neurons=N, instances=512, steps=steps, ms_per_step=dt / steps * 1e3, net_steps_per_s=steps / dt,
-                    instance_steps_per_s=512 * steps / dt, dense_roofline=mfma, kernels=k4, fp64=fp64)
+                    instance_steps_per_s=512 * steps / dt, dense_roofline=mfma, kernels=k4, fp64=fp64,
+                    fp64_unquantized_weights=unq)
     print(json.dumps(line), flush=True)
 
 

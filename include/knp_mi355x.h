@@ -212,6 +212,11 @@ int knp_mi355x_set_scatter_mode(knp_net *net, int mode);
  * later launches of the same call return at once, and the call returns KNP_ERR_HIP instead of hanging, which
  * is what the hook lets a test check. The network state is undefined after such an error. */
 int knp_mi355x_set_wait_test(knp_net *net, uint32_t hold_block, uint32_t spin_limit, uint64_t hold_until);
+/* Execution choice of synaptic-resource training (no reference counterpart, identical results): populations
+ * of at most max_neurons neurons spread each training candidate's synapses over a workgroup (latency: the few
+ * candidates of a step cannot fill the device otherwise); larger ones give each candidate a wave (throughput:
+ * batches of instances, whose candidates fill it). Default 16384. Takes effect at the next step call. */
+int knp_mi355x_set_train_workgroup_limit(knp_net *net, uint32_t max_neurons);
 /* 1 if the population's impacts are summed by tiles (valid after the first step or an explicit init). */
 int knp_mi355x_get_tiled(knp_net *net, uint32_t pop, int *tiled);
 
@@ -389,6 +394,10 @@ int knp_mi355x_get_event_count(knp_net *net, uint64_t *events);
 int knp_mi355x_get_counters(knp_net *net, uint64_t *events, uint64_t *rows);
 /* Fixed-point fraction bits of the exact impact accumulator of a population (chosen at init). */
 int knp_mi355x_get_accumulator_bits(knp_net *net, uint32_t pop, int32_t *frac_bits);
+/* Loaded synapse weights into a population that are not multiples of 2^-frac_bits (float weights below
+ * 2^(23 - frac_bits) with low mantissa bits): each of their impacts is rounded to the accumulator's grid, so
+ * sums that include them are exact only to 2^-(frac_bits + 1) per impact. 0 means every sum is exact. */
+int knp_mi355x_get_inexact_weights(knp_net *net, uint32_t pop, uint64_t *count);
 /* Work of the tiled path so far: row parts read (a spiking row's synapses in one tile) and ring cells
  * updated in HBM (non-zero tile sums). */
 int knp_mi355x_get_tile_counters(knp_net *net, uint64_t *row_parts, uint64_t *ring_cells);

@@ -51,6 +51,7 @@ def _declare(lib):
         "knp_mi355x_set_memory_cap": (i32, [P, u64]),
         "knp_mi355x_set_scatter_mode": (i32, [P, i32]),
         "knp_mi355x_set_wait_test": (i32, [P, u32, u32, u64]),
+        "knp_mi355x_set_train_workgroup_limit": (i32, [P, u32]),
         "knp_mi355x_get_tiled": (i32, [P, u32, C.POINTER(i32)]),
         "knp_mi355x_get_tile_counters": (i32, [P, C.POINTER(u64), C.POINTER(u64)]),
         "knp_mi355x_add_blifat_population": (i32, [P, C.POINTER(abi.BLIFATNeuron), u32, pu32]),
@@ -78,6 +79,7 @@ def _declare(lib):
         "knp_mi355x_step": (i32, [P, u64]),
         "knp_mi355x_step_noalloc": (i32, [P, u64]),
         "knp_mi355x_set_step_graph": (i32, [P, u32]),
+        "knp_mi355x_get_inexact_weights": (i32, [P, u32, C.POINTER(u64)]),
         "knp_mi355x_get_step_graph": (i32, [P, pu32, C.POINTER(u64)]),
         "knp_mi355x_get_step": (i32, [P, C.POINTER(u64)]),
         "knp_mi355x_synchronize": (i32, [P]),
@@ -390,6 +392,11 @@ class MI355XBackend:
         before the first step."""
         self._chk(self.lib.knp_mi355x_set_scatter_mode(self.net, mode))
 
+    def set_train_workgroup_limit(self, max_neurons):
+        """Synaptic-resource populations of at most max_neurons neurons train with a workgroup per candidate,
+        larger ones with a wave per candidate (identical results; see knp_mi355x_set_train_workgroup_limit)."""
+        self._chk(self.lib.knp_mi355x_set_train_workgroup_limit(self.net, max_neurons))
+
     def set_wait_test(self, hold_block, spin_limit, hold_until=2**64 - 1):
         """Testing hook: a persistent workgroup that withholds its spike counts at steps before hold_until, and
         a small bound on every wait, so that a step call must end with an error (see knp_mi355x_set_wait_test)."""
@@ -405,6 +412,12 @@ class MI355XBackend:
         a, b = C.c_uint64(), C.c_uint64()
         self._chk(self.lib.knp_mi355x_get_tile_counters(self.net, C.byref(a), C.byref(b)))
         return a.value, b.value
+
+    def inexact_weights(self, pop):
+        """Loaded weights into `pop` off its accumulator's 2^-frac_bits grid (rounded as impacts)."""
+        n = C.c_uint64()
+        self._chk(self.lib.knp_mi355x_get_inexact_weights(self.net, pop, C.byref(n)))
+        return n.value
 
     def accumulator_bits(self, pop):
         b = C.c_int32()

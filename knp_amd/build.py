@@ -64,8 +64,19 @@ def build_cpp_tests():
     return out
 
 
+def build_float_repeat_test():
+    """bin/float_repeat_test: the closed-form repeated float addition (csrc/float_repeat.h, used by the training
+    kernels) against additions one by one, run on the host."""
+    os.makedirs(os.path.join(HERE, "bin"), exist_ok=True)
+    out = os.path.join(HERE, "bin", "float_repeat_test")
+    _run([HIPCC, "-O2", "-std=c++17", "-ffp-contract=off", os.path.join(ROOT, "tests", "cpp", "float_repeat_test.hip"),
+          "-o", out + ".tmp"])
+    os.replace(out + ".tmp", out)
+    return out
+
+
 def build_all():
-    return [build_library(), build_backend_plugin(), build_cpp_tests()]
+    return [build_library(), build_backend_plugin(), build_cpp_tests(), build_float_repeat_test()]
 
 
 if __name__ == "__main__":

@@ -148,6 +148,7 @@ struct ProjDev {
     const uint8_t *delay_syn;    // [n_syn] delay in synapse-index order
     const uint64_t *col_ptr;     // [n_post + 1] CSC by postsynaptic neuron
     const uint32_t *col_syn;     // [n_syn] synapse indexes, ascending within a column
+    uint32_t csc_identity;       // col_syn[k] == k (synapses stored by postsynaptic neuron): not loaded
     // additive STDP
     float *tau_plus, *tau_minus;
     uint32_t *lim;               // spike-history cap per synapse

@@ -51,6 +51,7 @@ struct HostPop {
     uint32_t D = 1;
     bool has_cond = false, has_dop = false, has_block = false, has_forcing = false;
     int32_t frac_bits = 48;
+    uint64_t inexact = 0;  // loaded synapse weights into it that are not multiples of 2^-frac_bits
     bool tiled = false;  // impacts summed by the tile-owner scatter
     uint64_t t_first = 0;
     bool ran = false;
@@ -114,7 +115,7 @@ struct knp_net {
     NetDev *dev_table = nullptr;
     std::vector<void *> allocations;
     uint64_t allocated = 0, cap = 0;
-    uint32_t scatter_grid = 1, train_grid = 1;
+    uint32_t scatter_grid = 1;
     int scatter_mode = KNP_SCATTER_AUTO;
     bool fused = false;     // every population tiled: phase 1 of step t + 1 runs inside the tile kernel of t
     bool persistent = false;  // fused, and every tile fits on the device at once (cooperative launch)
@@ -131,6 +132,7 @@ struct knp_net {
     uint32_t graph_default = 32;
     uint32_t test_hold_block = 0xFFFFFFFFu, test_spin_limit = kSpinLimit;  // knp_mi355x_set_wait_test
     uint64_t test_hold_until = UINT64_MAX;
+    uint32_t train_wg_limit = 16384;  // knp_mi355x_set_train_workgroup_limit
     uint32_t run_kind = POP_LEAN;  // population kind of the persistent launch
     uint32_t tile_lds = 0;  // dynamic LDS bytes of the tile scatter (largest tiled ring)
     std::vector<uint32_t *> seg_list_host;
@@ -672,6 +674,16 @@ int init_network(knp_net *net) {
         if (H.frac_bits < 16)
             return fail(KNP_ERR_RANGE, "population " + std::to_string(p) +
                                            ": incoming |weight| sums too large for the exact accumulator");
+        // weights off the accumulator's grid (magnitudes below 2^(23 - frac_bits) with low mantissa bits) are
+        // rounded when they become impacts: counted, so that a caller can see when the sums are not exact
+        H.inexact = 0;
+        for (const auto &Q : net->projs) {
+            if (Q.post_pop != p) continue;
+            for (uint64_t s2 = 0; s2 < Q.n_syn; ++s2) {
+                const double v = std::ldexp((double)Q.w[s2], H.frac_bits);
+                if (v != std::nearbyint(v)) ++H.inexact;
+            }
+        }
         H.kind = H.resource ? POP_RESOURCE : (lean_eligible(H) ? POP_LEAN : POP_FULL);
         P.n = n;
         P.kind = H.kind;
@@ -1061,6 +1073,9 @@ int init_network(knp_net *net) {
             uint32_t *cy;
             if ((rc = upload(net, &cy, col_syn.data(), S))) return rc;
             Q.col_syn = cy;
+            bool identity = true;
+            for (uint64_t k = 0; k < S && identity; ++k) identity = col_syn[k] == (uint32_t)k;
+            Q.csc_identity = identity ? 1u : 0u;
             std::vector<uint8_t> ds(S);
             for (uint64_t s = 0; s < S; ++s) ds[s] = (uint8_t)H.delay[s];
             uint8_t *dsd;
@@ -1429,10 +1444,6 @@ int init_network(knp_net *net) {
         if ((rc = init_dense(net, d))) return rc;
 
     // launch shapes
-    uint32_t max_res = 1;
-    for (const auto &P : net->pops)
-        if (P.resource) max_res = std::max(max_res, P.n);
-    net->train_grid = std::max(1u, std::min(1024u, (max_res + 3) / 4));
 
     if ((rc = upload_working_sets(net))) return rc;
     net->initialized = true;
@@ -1680,9 +1691,14 @@ int run_steps(knp_net *net, uint64_t n_steps, uint32_t *counts = nullptr, bool t
         }
         for (uint32_t p = 0; p < net->pops.size(); ++p) {
             if (!net->pops[p].resource) continue;
-            const uint32_t g = net->train_grid;
+            // a workgroup per candidate up to the limit's population size, a wave per candidate beyond it
+            const uint32_t n = net->pops[p].n;
+            const bool per_wave = n > net->train_wg_limit;
             if ((rc = timed_launch(net, KNP_KERNEL_TRAIN, [&] {
-                     k_train<<<g, kBlock, 0, net->stream>>>(D, p, targ);
+                     if (per_wave)
+                         k_train_wave<<<std::min(1024u, (n + 3) / 4), kBlock, 0, net->stream>>>(D, p, targ);
+                     else
+                         k_train<<<std::max(1u, std::min(4096u, n)), kBlock, 0, net->stream>>>(D, p, targ);
                  })))
                 return rc;
         }
@@ -2739,6 +2755,16 @@ int knp_mi355x_set_wait_test(knp_net *net, uint32_t hold_block, uint32_t spin_li
     return upload_table(net);
 }
 
+int knp_mi355x_set_train_workgroup_limit(knp_net *net, uint32_t max_neurons) {
+    if (!net) return fail(KNP_ERR_INVALID_ARGUMENT, "null handle");
+    net->train_wg_limit = max_neurons;
+    if (net->graph_exec) {  // a captured step graph holds the previous choice of training kernel
+        (void)hipGraphExecDestroy(net->graph_exec);
+        net->graph_exec = nullptr;
+    }
+    return KNP_OK;
+}
+
 int knp_mi355x_set_projection_dense(knp_net *net, uint32_t proj, int format, uint32_t flags) {
     int rc = check_proj(net, proj);
     if (rc) return rc;
@@ -2851,6 +2877,15 @@ int knp_mi355x_get_counters(knp_net *net, uint64_t *events, uint64_t *rows) {
     if (events) *events = e;
     if (rows) *rows = r;
     return rc;
+}
+
+int knp_mi355x_get_inexact_weights(knp_net *net, uint32_t pop, uint64_t *count) {
+    int rc = check_pop(net, pop);
+    if (rc) return rc;
+    if (!count) return fail(KNP_ERR_INVALID_ARGUMENT, "null pointer");
+    if ((rc = ensure_init(net))) return rc;
+    *count = net->pops[pop].inexact;
+    return KNP_OK;
 }
 
 int knp_mi355x_get_accumulator_bits(knp_net *net, uint32_t pop, int32_t *frac_bits) {

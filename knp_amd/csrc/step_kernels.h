@@ -16,6 +16,7 @@
 #include <type_traits>
 
 #include "device_tables.h"
+#include "float_repeat.h"
 
 namespace knp_mi355x {
 
@@ -1447,220 +1448,380 @@ __device__ __forceinline__ float serial_free(float freer, uint64_t active, uint6
     return freer;
 }
 
-// 64-synapse chunks of one neuron whose loads k_train issues before their stores. Larger groups keep more
-// loads in flight per wave but cost registers (97 VGPRs at 2, 129 at 4, 76 at 1); measured on one instance
-// and on 256 batched instances, one chunk was the fastest in both: the batch needs the waves more than the
-// single instance needs the loads.
-constexpr uint32_t kTrainChunks = 1;
+// The per-neuron state of a training candidate (SynapticResourceSTDPNeuron's fields, blifat_stdp.h).
+struct TrainNeuron {
+    uint32_t status, isi_max;
+    uint64_t last_step, lss, fis;
+    float stab, freer;
+    int64_t dop_fix;
+    double dop;
+    bool forced, spiked;
+};
 
+// Load candidate n's state; a spiking candidate's ISI status moves on (process_spiking_neurons_impl,
+// blifat_stdp.h:64-103: status machine, last spike step, stability change at the start of an ISI period).
+__device__ __forceinline__ TrainNeuron train_begin(const PopDev &P, uint32_t n, uint64_t t) {
+    const ResDev &R = P.res;
+    TrainNeuron s;
+    s.spiked = (P.bitmap[n >> 5] >> (n & 31)) & 1u;
+    // dopamine_value_ (a double sum of float impacts in the reference) from the exact accumulator sum
+    s.dop_fix = gp(R.dop_fix)[n];
+    s.dop = (double)s.dop_fix * (double)P.inv_scale;
+    s.forced = R.forced_now[n] != 0;
+    s.status = R.isi_status[n];
+    s.last_step = R.last_step[n];
+    s.lss = R.last_spike_step[n];
+    s.fis = R.first_isi_spike[n];
+    s.stab = R.stability[n];
+    s.freer = R.free_res[n];
+    s.isi_max = pu(R.isi_max, n);
+    if (s.spiked) {
+        s.lss = t;
+        if (s.forced) {
+            s.status = 0;  // is_forced
+        } else {
+            switch (s.status) {
+                case 3:  // not_in_period
+                case 0:  // is_forced
+                    s.status = 1;
+                    s.fis = t;
+                    break;
+                case 1:  // period_started
+                    if (s.last_step - t < (uint64_t)s.isi_max) s.status = 2;
+                    break;
+                case 2:  // period_continued
+                    if (s.last_step - t >= (uint64_t)s.isi_max || s.dop_fix != 0) {
+                        s.status = 1;
+                        s.fis = t;
+                    }
+                    break;
+                default: break;
+            }
+            s.last_step = t;
+        }
+        if (s.status == 1) s.stab -= pf(R.change_at_isi, n);
+    }
+    return s;
+}
+
+// Does the dopamine pass run (do_dopamine_plasticity_impl's condition), and its float resource_change
+// (a double product, rounded once).
+__device__ __forceinline__ bool dopamine_runs(const TrainNeuron &s) {
+    return s.dop > 0.0 || (s.dop < 0.0 && s.status != 0);
+}
+
+// stability_ after the dopamine pass, in the reference's double promotions (blifat_stdp.h:135-150), rounded once.
+__device__ __forceinline__ float dopamine_stability(const ResDev &R, uint32_t n, const TrainNeuron &s, uint64_t t) {
+    float stab = s.stab;
+    if (s.forced || s.dop < 0.0) {
+        stab = (float)((double)stab - s.dop * (double)pf(R.change_param, n));
+        stab = stab < 0.f ? 0.f : stab;  // std::max(stability_, 0.0F)
+    } else {
+        const double difference = (double)(t - s.fis - (uint64_t)s.isi_max);  // u64 arithmetic, as fabs sees it
+        double f = 2.0 - difference / (double)s.isi_max;
+        f = f < -1.0 ? -1.0 : f;  // std::max(..., -1.0)
+        stab = (float)((double)stab + (double)pf(R.change_param, n) * s.dop * f);
+    }
+    return stab;
+}
+
+// Does renormalize_resource run after the passes (outside an ISI period, |free resource| at the threshold)?
+__device__ __forceinline__ bool renorm_runs(const ResDev &R, uint32_t n, const TrainNeuron &s, uint64_t t) {
+    const bool in_isi = (t - s.last_step <= (uint64_t)s.isi_max) && s.status != 0;
+    return !in_isi && fabsf(s.freer) >= pf(R.res_thr, n);
+}
+
+__device__ __forceinline__ void train_end(const ResDev &R, uint32_t n, const TrainNeuron &s) {
+    R.isi_status[n] = (uint8_t)s.status;
+    R.last_step[n] = s.last_step;
+    R.last_spike_step[n] = s.lss;
+    R.first_isi_spike[n] = s.fis;
+    R.stability[n] = s.stab;
+    R.free_res[n] = s.freer;
+}
+
+// Spiking pass of synapse k (CSC position) of post neuron n (blifat_stdp.h:105-131): had_contribution /
+// had_hebbian_update flags, resource -= d_u (and += d_h on a new contribution), weight recalculated. Adds the
+// old weight's fixed-point value to sum when positive; returns d_u in du (0 when the neuron is forced) and
+// whether the synapse contributed.
+__device__ __forceinline__ bool spiking_synapse(const ProjDev &Q, uint32_t n, uint64_t k, uint64_t t,
+                                                uint32_t status, float d_h, float scale, long long &sum, float &du) {
+    const uint32_t sy = Q.csc_identity ? (uint32_t)k : Q.col_syn[k];
+    const float w = Q.W[sy];
+    const uint8_t fl = Q.flags[sy];
+    const uint64_t per = Q.period[sy];
+    const uint64_t rs = Q.row_spike[Q.col_pre[k]];
+    const uint64_t spk = (rs != UINT64_MAX ? rs : Q.last_spike_step[sy]) + Q.delay_syn[sy] - 1;
+    const float r = Q.r[sy];
+    du = status != 0 ? Q.d_u[sy] : 0.f;
+    const float wmin = Q.w_min[sy], wmax = Q.w_max[sy];
+    if (w > 0.f) sum += __float2ll_rn(w * scale);
+    bool hc = fl & 2, hh = fl & 1;
+    const bool had_spike = in_interval(t - per, t, spk);
+    if (status != 2 || had_spike) hc = had_spike;
+    if (status != 2) hh = false;
+    float rr = r;
+    bool contributing = false;
+    if (status != 0) {
+        rr -= du;
+        if (hc && !hh) {
+            rr += d_h;
+            hh = true;
+            contributing = true;
+        }
+        Q.r[sy] = rr;
+    }
+    const float wn = recalc_weight(rr, wmin, wmax);
+    Q.W[sy] = wn;
+    if (Q.dense) dense_store_weight(Q, n, gp(Q.col_pre)[k], wn);
+    Q.flags[sy] = (uint8_t)((hc ? 2 : 0) | (hh ? 1 : 0));
+    return contributing;
+}
+
+// Dopamine pass of synapse k (blifat_stdp.h:133-156): resource += change where the synapse contributed and the
+// neuron spiked recently; the weight is recalculated in every case. Returns whether the resource changed.
+__device__ __forceinline__ bool dopamine_synapse(const ProjDev &Q, uint32_t n, uint64_t k, bool time_ok, float rc) {
+    const uint32_t sy = Q.csc_identity ? (uint32_t)k : Q.col_syn[k];
+    const float r = Q.r[sy];
+    const uint8_t fl = Q.flags[sy];
+    const float wmin = Q.w_min[sy], wmax = Q.w_max[sy];
+    float rr = r;
+    bool contributing = false;
+    if (time_ok && (fl & 2)) {
+        rr += rc;
+        Q.r[sy] = rr;
+        contributing = true;
+    }
+    const float wn = recalc_weight(rr, wmin, wmax);
+    Q.W[sy] = wn;
+    if (Q.dense) dense_store_weight(Q, n, gp(Q.col_pre)[k], wn);
+    return contributing;
+}
+
+// renormalize_resource for synapse k (blifat_stdp.h:174-190): resource += add, weight recalculated.
+__device__ __forceinline__ void renorm_synapse(const ProjDev &Q, uint32_t n, uint64_t k, float add) {
+    const uint32_t s = Q.csc_identity ? (uint32_t)k : Q.col_syn[k];
+    const float r = Q.r[s] + add;
+    Q.r[s] = r;
+    const float wn = recalc_weight(r, Q.w_min[s], Q.w_max[s]);
+    Q.W[s] = wn;
+    if (Q.dense) dense_store_weight(Q, n, gp(Q.col_pre)[k], wn);
+}
+
+// One training candidate on one wave: the synapses of each working projection (load order, then CSC order)
+// 64 at a time, the order-dependent free-resource sum replayed after every chunk.
+__device__ __forceinline__ void train_candidate_wave(const NetDev *__restrict__ net, const PopDev &P, uint32_t n,
+                                                     uint64_t t, uint32_t lane) {
+    const ResDev &R = P.res;
+    TrainNeuron s = train_begin(P, n, t);
+    if (s.spiked) {
+        const float d_h = pf(R.d_h, n) * stability_factor(s.stab);
+        long long sum = 0;
+        for (uint32_t j = 0; j < R.n_work; ++j) {
+            const ProjDev &Q = net->projs[R.work[j]];
+            const uint64_t cb = Q.col_ptr[n], ce = Q.col_ptr[n + 1];
+            for (uint64_t base = cb; base < ce; base += 64) {
+                const uint64_t k = base + lane;
+                const bool act = k < ce;
+                float du = 0.f;
+                const bool contributing = act && spiking_synapse(Q, n, k, t, s.status, d_h, P.scale, sum, du);
+                const uint64_t am = __ballot(act), nz = __ballot(act && du != 0.f), cm = __ballot(contributing);
+                if (s.status != 0) {
+                    // lane 0 is active in every chunk; one d_u over the chunk takes the closed form
+                    const float du0 = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(du)));
+                    if (du0 != 0.f && __ballot(act && __float_as_int(du) != __float_as_int(du0)) == 0)
+                        s.freer = replay_uniform(s.freer, am, cm, du0, -d_h);
+                    else
+                        s.freer = serial_free(s.freer, am, nz, cm, du, -d_h, true);
+                }
+            }
+        }
+        sum = wave_sum_i64(sum);
+        if (lane == 0) R.add_thr[n] = fixed_to_float(sum, P.inv_scale) * pf(R.coeff, n);
+    }
+    if (dopamine_runs(s)) {
+        const bool time_ok = (t - s.lss) <= (uint64_t)pu(R.dpt, n);
+        const float rc = (float)(s.dop * (double)stability_factor(s.stab));
+        for (uint32_t j = 0; j < R.n_work; ++j) {
+            const ProjDev &Q = net->projs[R.work[j]];
+            const uint64_t cb = Q.col_ptr[n], ce = Q.col_ptr[n + 1];
+            for (uint64_t base = cb; base < ce; base += 64) {
+                const uint64_t k = base + lane;
+                const uint64_t cm = __ballot(k < ce && dopamine_synapse(Q, n, k, time_ok, rc));
+                // free -= change once per contributing synapse: the same float operation each time
+                s.freer = repeat_add(s.freer, -rc, (uint64_t)__popcll(cm));
+            }
+        }
+        s.stab = dopamine_stability(R, n, s, t);
+    }
+    if (renorm_runs(R, n, s, t)) {
+        uint64_t k_total = 0;
+        for (uint32_t j = 0; j < R.n_work; ++j) {
+            const ProjDev &Q = net->projs[R.work[j]];
+            k_total += Q.col_ptr[n + 1] - Q.col_ptr[n];
+        }
+        const float add = s.freer / (float)(k_total + (uint64_t)pu(R.drain, n));
+        for (uint32_t j = 0; j < R.n_work; ++j) {
+            const ProjDev &Q = net->projs[R.work[j]];
+            const uint64_t cb = Q.col_ptr[n], ce = Q.col_ptr[n + 1];
+            for (uint64_t k = cb + lane; k < ce; k += 64) renorm_synapse(Q, n, k, add);
+        }
+        s.freer = 0.f;
+    }
+    if (lane == 0) train_end(R, n, s);
+}
+
+// Synaptic-resource STDP of one population's training candidates (train_population_impl, blifat_stdp.h:158-172).
+// Every per-synapse quantity depends only on the neuron and the synapse, and candidates own disjoint synapses;
+// only the free-resource sum of a candidate depends on the order of its float additions.
+//
+// k_train_wave: a wave per candidate, which replays its own sum between chunks, so no wave waits for another
+// (populations large enough that a step's candidates fill the device: batches of instances).
+__global__ __launch_bounds__(kBlock) void k_train_wave(const NetDev *__restrict__ net, uint32_t p, uint64_t t) {
+    t = resolve_step(net, t);
+    const PopDev &P = net->pops[p];
+    const uint32_t count = net->ctl[t & 1].cand_count[p];
+    const uint32_t lane = lane_id();
+    constexpr uint32_t W = kBlock / 64;
+    for (uint32_t c = blockIdx.x * W + threadIdx.x / 64; c < count; c += gridDim.x * W)
+        train_candidate_wave(net, P, P.res.cand[c], t, lane);
+}
+
+// k_train: a workgroup per candidate (smaller populations, whose few candidates a step cannot fill the device
+// with): the candidate's synapses (working projections in load order, then CSC order) are cut into 64-synapse
+// chunks, the four waves update four consecutive chunks at a time, and wave 0 then replays the free-resource
+// sum chunk by chunk in synapse order from the d_u values and masks the waves left in LDS.
 __global__ __launch_bounds__(kBlock) void k_train(const NetDev *__restrict__ net, uint32_t p, uint64_t t) {
     t = resolve_step(net, t);
     const PopDev &P = net->pops[p];
     const ResDev &R = P.res;
-    const Ctl *ctl = &net->ctl[t & 1];
-    const uint32_t count = ctl->cand_count[p];
+    const uint32_t count = net->ctl[t & 1].cand_count[p];
     const uint32_t lane = lane_id();
     const uint32_t wv = threadIdx.x / 64;
-    const uint32_t waves = gridDim.x * (kBlock / 64);
-    for (uint32_t c = blockIdx.x * (kBlock / 64) + wv; c < count; c += waves) {
-        const uint32_t n = R.cand[c];
-        const bool spiked = (P.bitmap[n >> 5] >> (n & 31)) & 1u;
-        // dopamine_value_ (a double sum of float impacts in the reference) from the exact accumulator sum
-        const int64_t dop_fix = gp(R.dop_fix)[n];
-        const double dop = (double)dop_fix * (double)P.inv_scale;
-        const bool forced = R.forced_now[n] != 0;
-        uint32_t status = R.isi_status[n];
-        uint64_t last_step = R.last_step[n], lss = R.last_spike_step[n], fis = R.first_isi_spike[n];
-        float stab = R.stability[n];
-        float freer = R.free_res[n];
-        const uint32_t isi_max = pu(R.isi_max, n);
-        // ---- process_spiking_neurons_impl
-        if (spiked) {
-            lss = t;
-            if (forced) {
-                status = 0;  // is_forced
-            } else {
-                switch (status) {
-                    case 3:  // not_in_period
-                    case 0:  // is_forced
-                        status = 1;
-                        fis = t;
-                        break;
-                    case 1:  // period_started
-                        if (last_step - t < (uint64_t)isi_max) status = 2;
-                        break;
-                    case 2:  // period_continued
-                        if (last_step - t >= (uint64_t)isi_max || dop_fix != 0) {
-                            status = 1;
-                            fis = t;
-                        }
-                        break;
-                    default: break;
-                }
-                last_step = t;
+    constexpr uint32_t W = kBlock / 64;
+    __shared__ float s_du[W][64];
+    __shared__ unsigned long long s_mask[W][3];  // active, nonzero d_u, contributing
+    __shared__ float s_uni[W];                   // the chunk's d_u when all its synapses carry it (else 0)
+    __shared__ long long s_sum[W];
+    __shared__ uint32_t s_cnt[W];
+    __shared__ uint64_t s_cb[kMaxWorking], s_ce[kMaxWorking];
+    __shared__ uint32_t s_nch[kMaxWorking];
+    __shared__ float s_free;
+    // chunk g of the candidate: its working projection, CSC position of its first synapse and the column's end
+    auto chunk = [&](uint32_t g, uint32_t &j, uint64_t &base, uint64_t &end) -> bool {
+        for (j = 0; j < R.n_work; ++j) {
+            if (g < s_nch[j]) {
+                base = s_cb[j] + 64ull * g;
+                end = s_ce[j];
+                return true;
             }
-            if (status == 1) stab -= pf(R.change_at_isi, n);
-            const float d_h = pf(R.d_h, n) * stability_factor(stab);
+            g -= s_nch[j];
+        }
+        return false;
+    };
+    for (uint32_t c = blockIdx.x; c < count; c += gridDim.x) {
+        const uint32_t n = R.cand[c];
+        __syncthreads();  // the previous candidate's LDS reads are done
+        if (threadIdx.x < R.n_work) {
+            const ProjDev &Q = net->projs[R.work[threadIdx.x]];
+            s_cb[threadIdx.x] = Q.col_ptr[n];
+            s_ce[threadIdx.x] = Q.col_ptr[n + 1];
+            s_nch[threadIdx.x] = (uint32_t)((s_ce[threadIdx.x] - s_cb[threadIdx.x] + 63) / 64);
+        }
+        __syncthreads();
+        uint32_t G = 0;
+        for (uint32_t j = 0; j < R.n_work; ++j) G += s_nch[j];
+        TrainNeuron s = train_begin(P, n, t);
+        if (s.spiked) {
+            const float d_h = pf(R.d_h, n) * stability_factor(s.stab);
             long long sum = 0;
-            for (uint32_t j = 0; j < R.n_work; ++j) {
-                const ProjDev &Q = net->projs[R.work[j]];
-                const uint64_t cb = Q.col_ptr[n], ce = Q.col_ptr[n + 1];
-                // kTrainChunks chunks of 64 synapses at a time: every load of the group is issued before any
-                // store (stores through synapse indexes would otherwise hold back the next chunk's loads);
-                // the order-dependent free-resource sum then replays the chunks in synapse order
-                for (uint64_t base = cb; base < ce; base += 64 * kTrainChunks) {
-                    uint32_t sy[kTrainChunks];
-                    float w[kTrainChunks], r[kTrainChunks], du[kTrainChunks], wmin[kTrainChunks], wmax[kTrainChunks];
-                    uint8_t fl[kTrainChunks];
-                    uint64_t spk[kTrainChunks], per[kTrainChunks];
-#pragma unroll
-                    for (uint32_t u = 0; u < kTrainChunks; ++u) {
-                        const uint64_t k = base + 64 * u + lane;
-                        if (k < ce) {
-                            const uint32_t sn = Q.col_syn[k];
-                            sy[u] = sn;
-                            w[u] = Q.W[sn];
-                            fl[u] = Q.flags[sn];
-                            per[u] = Q.period[sn];
-                            const uint64_t rs = Q.row_spike[Q.col_pre[k]];
-                            spk[u] = (rs != UINT64_MAX ? rs : Q.last_spike_step[sn]) + Q.delay_syn[sn] - 1;
-                            r[u] = Q.r[sn];
-                            du[u] = status != 0 ? Q.d_u[sn] : 0.f;
-                            wmin[u] = Q.w_min[sn];
-                            wmax[u] = Q.w_max[sn];
-                        }
-                    }
-                    uint64_t am[kTrainChunks], nz[kTrainChunks], cm[kTrainChunks];
-#pragma unroll
-                    for (uint32_t u = 0; u < kTrainChunks; ++u) {
-                        const uint64_t k = base + 64 * u + lane;
-                        const bool act = k < ce;
-                        bool contributing = false;
-                        if (act) {
-                            if (w[u] > 0.f) sum += __float2ll_rn(w[u] * P.scale);
-                            bool hc = fl[u] & 2, hh = fl[u] & 1;
-                            const bool had_spike = in_interval(t - per[u], t, spk[u]);
-                            if (status != 2 || had_spike) hc = had_spike;
-                            if (status != 2) hh = false;
-                            float rr = r[u];
-                            if (status != 0) {
-                                rr -= du[u];
-                                if (hc && !hh) {
-                                    rr += d_h;
-                                    hh = true;
-                                    contributing = true;
-                                }
-                                Q.r[sy[u]] = rr;
-                            }
-                            const float wn = recalc_weight(rr, wmin[u], wmax[u]);
-                            Q.W[sy[u]] = wn;
-                            if (Q.dense) dense_store_weight(Q, n, gp(Q.col_pre)[k], wn);
-                            Q.flags[sy[u]] = (uint8_t)((hc ? 2 : 0) | (hh ? 1 : 0));
-                        } else {
-                            du[u] = 0.f;
-                        }
-                        am[u] = __ballot(act);
-                        nz[u] = __ballot(act && du[u] != 0.f);
-                        cm[u] = __ballot(contributing);
-                    }
-                    if (status != 0) {
-#pragma unroll
-                        for (uint32_t u = 0; u < kTrainChunks; ++u)
-                            freer = serial_free(freer, am[u], nz[u], cm[u], du[u], -d_h, true);
-                    }
+            for (uint32_t r0 = 0; r0 < G; r0 += W) {
+                uint64_t am = 0, nz = 0, cm = 0;
+                float du = 0.f;
+                uint32_t j;
+                uint64_t base, end;
+                if (r0 + wv < G && chunk(r0 + wv, j, base, end)) {
+                    const uint64_t k = base + lane;
+                    const bool act = k < end;
+                    const bool contributing =
+                        act && spiking_synapse(net->projs[R.work[j]], n, k, t, s.status, d_h, P.scale, sum, du);
+                    am = __ballot(act);
+                    nz = __ballot(act && du != 0.f);
+                    cm = __ballot(contributing);
                 }
+                // lane 0 is active in every chunk; one d_u over the chunk takes the closed form
+                const float du0 = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(du)));
+                const bool uni = du0 != 0.f && __ballot(am != 0 && ((am >> lane) & 1) &&
+                                                        __float_as_int(du) != __float_as_int(du0)) == 0;
+                s_du[wv][lane] = du;
+                if (lane == 0) {
+                    s_mask[wv][0] = am;
+                    s_mask[wv][1] = nz;
+                    s_mask[wv][2] = cm;
+                    s_uni[wv] = uni ? du0 : 0.f;
+                }
+                __syncthreads();
+                if (wv == 0 && s.status != 0)
+                    for (uint32_t w = 0; w < W && r0 + w < G; ++w) {
+                        const float u = s_uni[w];
+                        if (u != 0.f)
+                            s.freer = replay_uniform(s.freer, s_mask[w][0], s_mask[w][2], u, -d_h);
+                        else
+                            s.freer = serial_free(s.freer, s_mask[w][0], s_mask[w][1], s_mask[w][2], s_du[w][lane],
+                                                  -d_h, true);
+                    }
+                __syncthreads();
             }
             sum = wave_sum_i64(sum);
-            if (lane == 0) R.add_thr[n] = fixed_to_float(sum, P.inv_scale) * pf(R.coeff, n);
-        }
-        // ---- do_dopamine_plasticity_impl
-        if (dop > 0.0 || (dop < 0.0 && status != 0)) {
-            const bool time_ok = (t - lss) <= (uint64_t)pu(R.dpt, n);
-            // float resource_change = dopamine_value_ * factor (double product, rounded once)
-            const float rc = (float)(dop * (double)stability_factor(stab));
-            for (uint32_t j = 0; j < R.n_work; ++j) {
-                const ProjDev &Q = net->projs[R.work[j]];
-                const uint64_t cb = Q.col_ptr[n], ce = Q.col_ptr[n + 1];
-                for (uint64_t base = cb; base < ce; base += 64 * kTrainChunks) {
-                    uint32_t sy[kTrainChunks];
-                    float r[kTrainChunks], wmin[kTrainChunks], wmax[kTrainChunks];
-                    uint8_t fl[kTrainChunks];
-#pragma unroll
-                    for (uint32_t u = 0; u < kTrainChunks; ++u) {
-                        const uint64_t k = base + 64 * u + lane;
-                        if (k < ce) {
-                            const uint32_t sn = Q.col_syn[k];
-                            sy[u] = sn;
-                            r[u] = Q.r[sn];
-                            fl[u] = Q.flags[sn];
-                            wmin[u] = Q.w_min[sn];
-                            wmax[u] = Q.w_max[sn];
-                        }
-                    }
-                    uint64_t cm[kTrainChunks];
-#pragma unroll
-                    for (uint32_t u = 0; u < kTrainChunks; ++u) {
-                        const uint64_t k = base + 64 * u + lane;
-                        bool contributing = false;
-                        if (k < ce) {
-                            float rr = r[u];
-                            if (time_ok && (fl[u] & 2)) {
-                                rr += rc;
-                                Q.r[sy[u]] = rr;
-                                contributing = true;
-                            }
-                            const float wn = recalc_weight(rr, wmin[u], wmax[u]);
-                            Q.W[sy[u]] = wn;
-                            if (Q.dense) dense_store_weight(Q, n, gp(Q.col_pre)[k], wn);
-                        }
-                        cm[u] = __ballot(contributing);
-                    }
-#pragma unroll
-                    for (uint32_t u = 0; u < kTrainChunks; ++u) freer = serial_free(freer, cm[u], 0, cm[u], 0.f, -rc, false);
-                }
-            }
-            // stability_ updates in the reference's double promotions (blifat_stdp.h:135-150), rounded once
-            if (forced || dop < 0.0) {
-                stab = (float)((double)stab - dop * (double)pf(R.change_param, n));
-                stab = stab < 0.f ? 0.f : stab;  // std::max(stability_, 0.0F)
-            } else {
-                const double difference = (double)(t - fis - (uint64_t)isi_max);  // u64 arithmetic, as fabs sees it
-                double f = 2.0 - difference / (double)isi_max;
-                f = f < -1.0 ? -1.0 : f;  // std::max(..., -1.0)
-                stab = (float)((double)stab + (double)pf(R.change_param, n) * dop * f);
+            if (lane == 0) s_sum[wv] = sum;
+            if (threadIdx.x == 0) s_free = s.freer;
+            __syncthreads();
+            s.freer = s_free;  // the replay's result (wave 0) for every thread
+            if (threadIdx.x == 0) {
+                long long total = 0;
+                for (uint32_t w = 0; w < W; ++w) total += s_sum[w];
+                R.add_thr[n] = fixed_to_float(total, P.inv_scale) * pf(R.coeff, n);
             }
         }
-        // ---- renormalize_resource
-        const bool in_isi = (t - last_step <= (uint64_t)isi_max) && status != 0;
-        if (!in_isi && fabsf(freer) >= pf(R.res_thr, n)) {
+        if (dopamine_runs(s)) {
+            const bool time_ok = (t - s.lss) <= (uint64_t)pu(R.dpt, n);
+            const float rc = (float)(s.dop * (double)stability_factor(s.stab));
+            uint32_t contrib = 0;
+            for (uint32_t g = wv; g < G; g += W) {
+                uint32_t j;
+                uint64_t base, end;
+                if (!chunk(g, j, base, end)) break;
+                const uint64_t k = base + lane;
+                contrib += (uint32_t)__popcll(
+                    __ballot(k < end && dopamine_synapse(net->projs[R.work[j]], n, k, time_ok, rc)));
+            }
+            if (lane == 0) s_cnt[wv] = contrib;
+            __syncthreads();
+            // free -= change once per contributing synapse: the same float operation each time, in any order
+            uint32_t total = 0;
+            for (uint32_t w = 0; w < W; ++w) total += s_cnt[w];
+            if (threadIdx.x == 0) {
+                s.freer = repeat_add(s.freer, -rc, total);
+                s_free = s.freer;
+            }
+            __syncthreads();
+            s.freer = s_free;
+            s.stab = dopamine_stability(R, n, s, t);
+        }
+        if (renorm_runs(R, n, s, t)) {
             uint64_t k_total = 0;
-            for (uint32_t j = 0; j < R.n_work; ++j) {
-                const ProjDev &Q = net->projs[R.work[j]];
-                k_total += Q.col_ptr[n + 1] - Q.col_ptr[n];
+            for (uint32_t j = 0; j < R.n_work; ++j) k_total += s_ce[j] - s_cb[j];
+            const float add = s.freer / (float)(k_total + (uint64_t)pu(R.drain, n));
+            for (uint32_t g = wv; g < G; g += W) {
+                uint32_t j;
+                uint64_t base, end;
+                if (!chunk(g, j, base, end)) break;
+                const uint64_t k = base + lane;
+                if (k < end) renorm_synapse(net->projs[R.work[j]], n, k, add);
             }
-            const float add = freer / (float)(k_total + (uint64_t)pu(R.drain, n));
-            for (uint32_t j = 0; j < R.n_work; ++j) {
-                const ProjDev &Q = net->projs[R.work[j]];
-                const uint64_t cb = Q.col_ptr[n], ce = Q.col_ptr[n + 1];
-                for (uint64_t k = cb + lane; k < ce; k += 64) {
-                    const uint32_t s = Q.col_syn[k];
-                    const float r = Q.r[s] + add;
-                    Q.r[s] = r;
-                    const float wn = recalc_weight(r, Q.w_min[s], Q.w_max[s]);
-                    Q.W[s] = wn;
-                    if (Q.dense) dense_store_weight(Q, n, gp(Q.col_pre)[k], wn);
-                }
-            }
-            freer = 0.f;
+            s.freer = 0.f;
         }
-        if (lane == 0) {
-            R.isi_status[n] = (uint8_t)status;
-            R.last_step[n] = last_step;
-            R.last_spike_step[n] = lss;
-            R.first_isi_spike[n] = fis;
-            R.stability[n] = stab;
-            R.free_res[n] = freer;
-        }
+        if (threadIdx.x == 0) train_end(R, n, s);
     }
 }
 

@@ -409,3 +409,19 @@ def uneven_inference(seed=4, n_instances=512, n_min=200, n_max=5000, n_in=784, p
     spec.meta.update(pop=pop, recurrent_proj=rec, steps=steps, sizes=sizes, begin=begin, n_in=n_in,
                      dense_weights_f32=wff)
     return spec
+
+
+def unquantized_dense(spec):
+    """The same network with the dense input projection's weights before quantization (float32, no row scale):
+    a recipe for the FP64 oracle only (the device takes FP8 or BF16 codes), to measure what quantization costs."""
+    out = NetworkSpec(spec.name + "_unquantized")
+    out.pop_sizes, out.proj_sizes, out.learning = list(spec.pop_sizes), list(spec.proj_sizes), spec.learning
+    out.meta = dict(spec.meta)
+    for name, args, kw in spec.ops:
+        if name == "add_dense_input_projection":
+            a = list(args)
+            a[4], a[5], a[6] = "f32", spec.meta["dense_weights_f32"], None
+            args = tuple(a)
+        out.ops.append((name, args, kw))
+    return out
+

@@ -222,7 +222,9 @@ class Oracle:
         pc = np.asarray(post_count, dtype=np.int64)
         K = len(pb)
         rows = int(pc.sum())
-        if fmt == abi.DENSE_FP8_E4M3:
+        if fmt == "f32":  # the unquantized weights (rows x n_in float32), for agreement measurements
+            w = np.asarray(weights, dtype=np.float64).reshape(rows, n_in)
+        elif fmt == abi.DENSE_FP8_E4M3:
             w = abi.e4m3_decode(np.asarray(weights).reshape(rows, n_in))
         else:
             w = abi.bf16_decode(np.asarray(weights).reshape(rows, n_in)).astype(np.float64)

@@ -31,8 +31,20 @@ def need_device():
     assert device_count() >= 1, "no HIP device visible"
 
 
-def both(spec):
-    return spec.load_into(MI355XBackend(0)), spec.load_into(Oracle("exact"))
+# Synaptic-resource training runs a workgroup per candidate in populations up to a size (the default limit
+# covers every network here) and a wave per candidate in larger ones; limit 0 forces the wave form.
+TRAIN_FORMS = {"workgroup": None, "wave": 0}
+
+
+def device(train="workgroup"):
+    g = MI355XBackend(0)
+    if TRAIN_FORMS[train] is not None:
+        g.set_train_workgroup_limit(TRAIN_FORMS[train])
+    return g
+
+
+def both(spec, train="workgroup"):
+    return spec.load_into(device(train)), spec.load_into(Oracle("exact"))
 
 
 def rel_err(a, b):
@@ -123,22 +135,24 @@ def test_resource_stdp_network(model):
     assert g.weights(gl)[0] != 1.0
 
 
-def test_resource_stdp_hand_traces_on_device():
+@pytest.mark.parametrize("train", list(TRAIN_FORMS))
+def test_resource_stdp_hand_traces_on_device(train):
     """The hand-computed synaptic-resource STDP traces (tests/resource_traces.py) on the device: every
     expected resource, weight, free resource, stability, ISI field and flag, bit for bit."""
     import resource_traces
 
-    say("case: synaptic-resource STDP hand traces on the device")
-    resource_traces.run_all_traces(lambda: MI355XBackend(0))
+    say(f"case: synaptic-resource STDP hand traces on the device, a {train} per candidate")
+    resource_traces.run_all_traces(lambda: device(train))
 
 
-def test_resource_free_negative_zero():
+@pytest.mark.parametrize("train", list(TRAIN_FORMS))
+def test_resource_free_negative_zero(train):
     """The free-resource replay skips zero-d_u additions except when free is -0 (-0 + +0 = +0 in the
     reference's float order): a neuron loaded with free = -0.0 and d_u = 0 must follow the oracle bit for bit."""
     import struct
     say("case: free synaptic resource starting at -0.0 with d_u = 0")
     runs = []
-    for o in (MI355XBackend(0), Oracle("exact")):
+    for o in (device(train), Oracle("exact")):
         cell = abi.resource_blifat_defaults(free_synaptic_resource=-0.0, isi_max=3, d_h=0.25)
         pop = o.add_resource_blifat_population(abi.neuron_array(cell, 1))
         rules = abi.rule_array(abi.resource_rule_defaults(synaptic_resource=0.0, w_min=1.0, w_max=2.0, d_u=0.0,
@@ -184,13 +198,14 @@ def resource_altai_network(o, seed):
     return pop, inp, dop
 
 
-def test_resource_altai_training_matches_oracle():
+@pytest.mark.parametrize("train", list(TRAIN_FORMS))
+def test_resource_altai_training_matches_oracle(train):
     """Synaptic-resource AltAI-LIF training (altai_stdp.h:35-149) over 300 steps with dopamine every 10
     steps: spike trains identical, weights within 1e-6."""
-    say("case: resource AltAI-LIF training, 300 steps")
+    say(f"case: resource AltAI-LIF training, 300 steps, a {train} per candidate")
     runs = {}
     for name in ("gpu", "oracle"):
-        o = Oracle("exact") if name == "oracle" else MI355XBackend(0)
+        o = Oracle("exact") if name == "oracle" else device(train)
         pop, inp, dop = resource_altai_network(o, 5)
         o.record_spikes(pop, 300)
         for t in range(0, 300, 10):
@@ -605,12 +620,13 @@ def agreement(a, b):
     return 1.0 - (union - inter) / max(1, union)
 
 
-def test_stdp_learning_short():
+@pytest.mark.parametrize("train", list(TRAIN_FORMS))
+def test_stdp_learning_short(train):
     """Config 2 shape, 100 images (2,000 steps): FP32 device training against the KNP-exact oracle, spikes
     and weights identical."""
-    say("case: stdp_learning 100 images")
+    say(f"case: stdp_learning 100 images, a {train} per candidate")
     spec = workloads.stdp_learning(n_images=100)
-    g, o = both(spec)
+    g, o = both(spec, train)
     steps = 2000
     g.record_spikes(0, steps)
     o.record_spikes(0, steps)
@@ -926,3 +942,45 @@ def test_dense_input_projection(fmt, time_block):
         o.step((-steps) % time_block)
         assert g.events == o.events
     assert g.dense_time_block(0) % 32 == 0
+
+
+def test_dense_input_projection_at_shape():
+    """The uneven configuration's shape on a reduced batch: 16 instances of 200..5,000 neurons (log-uniform),
+    784 inputs (seven chunks of 8 K-steps in the MFMA kernel), the autotuned time block, FP8 weights: spikes and
+    potentials against the oracle bit for bit, impact counts equal at a time-block boundary."""
+    say("case: dense input projection at the configuration's shape, 16 instances")
+    spec = workloads.uneven_inference(n_instances=16, steps=150)
+    assert spec.meta["n_in"] == 784
+    g, o = both(spec)
+    steps = 150
+    g.record_spikes(0, steps)
+    o.record_spikes(0, steps)
+    g.step(steps)
+    o.step(steps)
+    sg, so = g.read_spike_record(0), o.read_spike_record(0)
+    say(f"  neurons {spec.pop_sizes[0]}, time block {g.dense_time_block(0)}, spikes {int(so.sum())}")
+    assert so.sum() > 1000
+    assert np.array_equal(sg, so), np.argwhere(sg != so)[:10]
+    assert rel_err(g.potentials(0), o.potentials(0)) <= POT_RTOL
+    T = g.dense_time_block(0)
+    g.step((-steps) % T)
+    o.step((-steps) % T)
+    assert g.events == o.events
+
+
+def test_inexact_weights_are_counted():
+    """Weights off the accumulator's grid are counted at load: a float weight of 1e-12 has mantissa bits below
+    2^-52 (the fraction bits of a population whose incoming sums stay below 1); weights on a 2^-8 grid do not."""
+    say("case: weights off the exact accumulator's grid")
+    g = MI355XBackend(0)
+    pop = g.add_blifat_population(abi.neuron_array(abi.blifat_defaults(), 4))
+    g.add_delta_projection(abi.KNP_NO_POPULATION, 4, pop, [0, 1, 2, 3], [0, 1, 2, 3],
+                           np.array([0.25, 1e-12, 0.5, 3 / 256], np.float32), [1, 1, 1, 1])
+    assert g.accumulator_bits(pop) == 52
+    assert g.inexact_weights(pop) == 1
+    g.close()
+    spec = workloads.plain_small(seed=11, n=300, n_syn=20_000)
+    g = spec.load_into(MI355XBackend(0))
+    say(f"  plain_small: {g.inexact_weights(0)} inexact of {spec.n_synapses} (frac bits {g.accumulator_bits(0)})")
+    g.close()
+
