@@ -69,6 +69,9 @@ def main():
                          "event-driven scatter (none)")
     ap.add_argument("--no-split", dest="split", action="store_false",
                     help="c3: one BF16 value per weight (default: two parts, hi + lo, 16 significant bits)")
+    ap.add_argument("--graph-steps", type=int, default=None,
+                    help="steps per captured step graph in long one-launch-per-step calls (0: no graphs); default: "
+                         "the library's")
     ap.add_argument("--train-wg-limit", type=int, default=None,
                     help="synaptic-resource training: population size up to which each candidate gets a workgroup "
                          "(beyond it, a wave); default: the library's")
@@ -429,6 +432,8 @@ def device_loaders(args):
                             "atomic": abi.SCATTER_ATOMIC}[args.scatter])
         if args.train_wg_limit is not None:
             g.set_train_workgroup_limit(args.train_wg_limit)
+        if args.graph_steps is not None:
+            g.set_step_graph(args.graph_steps)
         spec.load_into(g)
         g._init()
         g.synchronize()

@@ -16,6 +16,8 @@ constexpr int kMaxRowSources = 128;
 constexpr int kMaxVSeg = 16384;   // virtual segments the scatter scans in LDS
 constexpr int kMaxTasks = kMaxPops + kMaxSources;
 constexpr int kMaxWorking = 8;
+// ProjDev::rule_uniform bits: w_min, w_max, d_u, dopamine_plasticity_period, delay the same on every synapse
+constexpr uint32_t kUniWmin = 1, kUniWmax = 2, kUniDu = 4, kUniPeriod = 8, kUniDelay = 16;
 constexpr int kMaxDense = 4;
 constexpr int kBlock = 256;  // 4 wave64 per workgroup
 // A spike segment: word 0 = count, words 1.. = ascending spike indexes. Population and source segments hold
@@ -149,6 +151,10 @@ struct ProjDev {
     const uint64_t *col_ptr;     // [n_post + 1] CSC by postsynaptic neuron
     const uint32_t *col_syn;     // [n_syn] synapse indexes, ascending within a column
     uint32_t csc_identity;       // col_syn[k] == k (synapses stored by postsynaptic neuron): not loaded
+    // synaptic-resource rule parameters carried by every synapse alike (kUni* bits): the one value
+    uint32_t rule_uniform;
+    float u_wmin, u_wmax, u_du;
+    uint32_t u_period, u_delay;
     // additive STDP
     float *tau_plus, *tau_minus;
     uint32_t *lim;               // spike-history cap per synapse

@@ -1102,6 +1102,30 @@ int init_network(knp_net *net) {
             if ((rc = upload(net, &Q.d_u, f.data(), S))) return rc;
             for (uint64_t s = 0; s < S; ++s) u[s] = H.rules[s].dopamine_plasticity_period;
             if ((rc = upload(net, &Q.period, u.data(), S))) return rc;
+            // parameters every synapse carries alike are read from the projection entry instead
+            auto same = [&](auto get) {
+                for (uint64_t s = 1; s < S; ++s)
+                    if (!(get(s) == get(0))) return false;
+                return S > 0;
+            };
+            auto fbits = [](float x) {
+                uint32_t b;
+                std::memcpy(&b, &x, 4);
+                return b;
+            };
+            Q.rule_uniform = 0;
+            if (same([&](uint64_t s) { return fbits(H.rules[s].w_min); })) Q.rule_uniform |= kUniWmin;
+            if (same([&](uint64_t s) { return fbits(H.rules[s].w_max); })) Q.rule_uniform |= kUniWmax;
+            if (same([&](uint64_t s) { return fbits(H.rules[s].d_u); })) Q.rule_uniform |= kUniDu;
+            if (same([&](uint64_t s) { return H.rules[s].dopamine_plasticity_period; })) Q.rule_uniform |= kUniPeriod;
+            if (same([&](uint64_t s) { return (uint8_t)H.delay[s]; })) Q.rule_uniform |= kUniDelay;  // as delay_syn
+            if (S > 0) {
+                Q.u_wmin = H.rules[0].w_min;
+                Q.u_wmax = H.rules[0].w_max;
+                Q.u_du = H.rules[0].d_u;
+                Q.u_period = H.rules[0].dopamine_plasticity_period;
+                Q.u_delay = (uint8_t)H.delay[0];
+            }
             for (uint64_t s = 0; s < S; ++s) u64[s] = H.rules[s].last_spike_step;
             if ((rc = upload(net, &Q.last_spike_step, u64.data(), S))) return rc;
             std::vector<uint64_t> never((size_t)std::max(1u, H.n_pre), UINT64_MAX);
@@ -1698,7 +1722,7 @@ int run_steps(knp_net *net, uint64_t n_steps, uint32_t *counts = nullptr, bool t
                      if (per_wave)
                          k_train_wave<<<std::min(1024u, (n + 3) / 4), kBlock, 0, net->stream>>>(D, p, targ);
                      else
-                         k_train<<<std::max(1u, std::min(4096u, n)), kBlock, 0, net->stream>>>(D, p, targ);
+                         k_train<<<std::max(1u, std::min(512u, n)), kTrainBlock, 0, net->stream>>>(D, p, targ);
                  })))
                 return rc;
         }

@@ -1399,6 +1399,15 @@ __device__ __forceinline__ void dense_store_weight(const ProjDev &Q, uint32_t po
     if (Q.dn_w_lo) gp(Q.dn_w_lo)[pos] = bf16_lo(w, hi);
 }
 
+// A synaptic-resource rule parameter: the projection's one value when every synapse carries the same one (no
+// per-synapse load), else the synapse's.
+__device__ __forceinline__ float rule_wmin(const ProjDev &Q, uint32_t sy) {
+    return (Q.rule_uniform & kUniWmin) ? Q.u_wmin : Q.w_min[sy];
+}
+__device__ __forceinline__ float rule_wmax(const ProjDev &Q, uint32_t sy) {
+    return (Q.rule_uniform & kUniWmax) ? Q.u_wmax : Q.w_max[sy];
+}
+
 __device__ __forceinline__ float recalc_weight(float r, float w_min, float w_max) {
     // recalculate_synapse_weights (training/stdp.h:40-53)
     const float syn_w = r > 0.f ? r : 0.f;
@@ -1549,12 +1558,14 @@ __device__ __forceinline__ bool spiking_synapse(const ProjDev &Q, uint32_t n, ui
     const uint32_t sy = Q.csc_identity ? (uint32_t)k : Q.col_syn[k];
     const float w = Q.W[sy];
     const uint8_t fl = Q.flags[sy];
-    const uint64_t per = Q.period[sy];
-    const uint64_t rs = Q.row_spike[Q.col_pre[k]];
-    const uint64_t spk = (rs != UINT64_MAX ? rs : Q.last_spike_step[sy]) + Q.delay_syn[sy] - 1;
+    const uint64_t per = (Q.rule_uniform & kUniPeriod) ? Q.u_period : Q.period[sy];
+    const uint32_t pre = Q.col_pre[k];
+    const uint64_t rs = Q.row_spike[pre];
+    const uint32_t delay = (Q.rule_uniform & kUniDelay) ? Q.u_delay : Q.delay_syn[sy];
+    const uint64_t spk = (rs != UINT64_MAX ? rs : Q.last_spike_step[sy]) + delay - 1;
     const float r = Q.r[sy];
-    du = status != 0 ? Q.d_u[sy] : 0.f;
-    const float wmin = Q.w_min[sy], wmax = Q.w_max[sy];
+    du = status != 0 ? ((Q.rule_uniform & kUniDu) ? Q.u_du : Q.d_u[sy]) : 0.f;
+    const float wmin = rule_wmin(Q, sy), wmax = rule_wmax(Q, sy);
     if (w > 0.f) sum += __float2ll_rn(w * scale);
     bool hc = fl & 2, hh = fl & 1;
     const bool had_spike = in_interval(t - per, t, spk);
@@ -1569,12 +1580,16 @@ __device__ __forceinline__ bool spiking_synapse(const ProjDev &Q, uint32_t n, ui
             hh = true;
             contributing = true;
         }
-        Q.r[sy] = rr;
+        if (__float_as_uint(rr) != __float_as_uint(r)) Q.r[sy] = rr;
     }
+    // stores of unchanged values are left out (the BF16 copy of an unchanged weight is current too)
     const float wn = recalc_weight(rr, wmin, wmax);
-    Q.W[sy] = wn;
-    if (Q.dense) dense_store_weight(Q, n, gp(Q.col_pre)[k], wn);
-    Q.flags[sy] = (uint8_t)((hc ? 2 : 0) | (hh ? 1 : 0));
+    if (__float_as_uint(wn) != __float_as_uint(w)) {
+        Q.W[sy] = wn;
+        if (Q.dense) dense_store_weight(Q, n, pre, wn);
+    }
+    const uint8_t nf = (uint8_t)((hc ? 2 : 0) | (hh ? 1 : 0));
+    if (nf != fl) Q.flags[sy] = nf;
     return contributing;
 }
 
@@ -1583,29 +1598,35 @@ __device__ __forceinline__ bool spiking_synapse(const ProjDev &Q, uint32_t n, ui
 __device__ __forceinline__ bool dopamine_synapse(const ProjDev &Q, uint32_t n, uint64_t k, bool time_ok, float rc) {
     const uint32_t sy = Q.csc_identity ? (uint32_t)k : Q.col_syn[k];
     const float r = Q.r[sy];
+    const float w = Q.W[sy];
     const uint8_t fl = Q.flags[sy];
-    const float wmin = Q.w_min[sy], wmax = Q.w_max[sy];
+    const float wmin = rule_wmin(Q, sy), wmax = rule_wmax(Q, sy);
     float rr = r;
     bool contributing = false;
     if (time_ok && (fl & 2)) {
         rr += rc;
-        Q.r[sy] = rr;
+        if (__float_as_uint(rr) != __float_as_uint(r)) Q.r[sy] = rr;
         contributing = true;
     }
     const float wn = recalc_weight(rr, wmin, wmax);
-    Q.W[sy] = wn;
-    if (Q.dense) dense_store_weight(Q, n, gp(Q.col_pre)[k], wn);
+    if (__float_as_uint(wn) != __float_as_uint(w)) {
+        Q.W[sy] = wn;
+        if (Q.dense) dense_store_weight(Q, n, gp(Q.col_pre)[k], wn);
+    }
     return contributing;
 }
 
 // renormalize_resource for synapse k (blifat_stdp.h:174-190): resource += add, weight recalculated.
 __device__ __forceinline__ void renorm_synapse(const ProjDev &Q, uint32_t n, uint64_t k, float add) {
     const uint32_t s = Q.csc_identity ? (uint32_t)k : Q.col_syn[k];
-    const float r = Q.r[s] + add;
-    Q.r[s] = r;
-    const float wn = recalc_weight(r, Q.w_min[s], Q.w_max[s]);
-    Q.W[s] = wn;
-    if (Q.dense) dense_store_weight(Q, n, gp(Q.col_pre)[k], wn);
+    const float r0 = Q.r[s], w = Q.W[s];
+    const float r = r0 + add;
+    if (__float_as_uint(r) != __float_as_uint(r0)) Q.r[s] = r;
+    const float wn = recalc_weight(r, rule_wmin(Q, s), rule_wmax(Q, s));
+    if (__float_as_uint(wn) != __float_as_uint(w)) {
+        Q.W[s] = wn;
+        if (Q.dense) dense_store_weight(Q, n, gp(Q.col_pre)[k], wn);
+    }
 }
 
 // One training candidate on one wave: the synapses of each working projection (load order, then CSC order)
@@ -1689,16 +1710,18 @@ __global__ __launch_bounds__(kBlock) void k_train_wave(const NetDev *__restrict_
 
 // k_train: a workgroup per candidate (smaller populations, whose few candidates a step cannot fill the device
 // with): the candidate's synapses (working projections in load order, then CSC order) are cut into 64-synapse
-// chunks, the four waves update four consecutive chunks at a time, and wave 0 then replays the free-resource
-// sum chunk by chunk in synapse order from the d_u values and masks the waves left in LDS.
-__global__ __launch_bounds__(kBlock) void k_train(const NetDev *__restrict__ net, uint32_t p, uint64_t t) {
+// chunks, the sixteen waves update sixteen consecutive chunks at a time, and wave 0 then replays the
+// free-resource sum chunk by chunk in synapse order from the d_u values and masks the waves left in LDS.
+constexpr uint32_t kTrainBlock = 1024;  // 16 waves: a 784-input neuron's 13 chunks in one round
+
+__global__ __launch_bounds__(kTrainBlock) void k_train(const NetDev *__restrict__ net, uint32_t p, uint64_t t) {
     t = resolve_step(net, t);
     const PopDev &P = net->pops[p];
     const ResDev &R = P.res;
     const uint32_t count = net->ctl[t & 1].cand_count[p];
     const uint32_t lane = lane_id();
     const uint32_t wv = threadIdx.x / 64;
-    constexpr uint32_t W = kBlock / 64;
+    constexpr uint32_t W = kTrainBlock / 64;
     __shared__ float s_du[W][64];
     __shared__ unsigned long long s_mask[W][3];  // active, nonzero d_u, contributing
     __shared__ float s_uni[W];                   // the chunk's d_u when all its synapses carry it (else 0)

@@ -171,9 +171,11 @@ def test_resource_free_negative_zero(train):
     assert runs[1][0] == struct.pack("<f", -0.0) and runs[1][-1] != struct.pack("<f", -0.0)
 
 
-def resource_altai_network(o, seed):
+def resource_altai_network(o, seed, varied=False):
     """Synaptic-resource AltAI-LIF population (256 neurons, leak -1, threshold 4) trained by a plastic
-    channel projection (64 channels, Bernoulli source), with lateral inhibition and a dopamine channel."""
+    channel projection (64 channels, Bernoulli source), with lateral inhibition and a dopamine channel.
+    varied: w_min, w_max, d_u, the dopamine plasticity period and the delay differ from synapse to synapse
+    (else every synapse carries the same ones)."""
     rng = np.random.default_rng(seed)
     n, nch, S = 256, 64, 3000
     base = abi.altai_lif_defaults(activation_threshold=4, potential_leak=-1)
@@ -183,10 +185,18 @@ def resource_altai_network(o, seed):
     pop = o.add_resource_altai_lif_population(abi.neuron_array(cell, n))
     rules = abi.rule_array(abi.resource_rule_defaults(synaptic_resource=0.0, w_min=0.5, w_max=3.0, d_u=0.0625,
                                                       dopamine_plasticity_period=5), S)
+    delays = np.ones(S, np.uint32)
+    if varied:
+        for i in range(S):
+            rules[i].w_min = 0.5 - 0.125 * (i % 3)
+            rules[i].w_max = 3.0 - 0.25 * (i % 5)
+            rules[i].d_u = 0.0625 * (1 + i % 2) if i % 7 else 0.0
+            rules[i].dopamine_plasticity_period = 4 + i % 3
+        delays = (1 + np.arange(S) % 2).astype(np.uint32)
     pre = rng.integers(0, nch, S).astype(np.uint32)
     post = rng.integers(0, n, S).astype(np.uint32)
     inp = o.add_resource_stdp_projection(abi.KNP_NO_POPULATION, nch, pop, pre, post, np.full(S, 0.5, np.float32),
-                                         np.ones(S, np.uint32), rules)
+                                         delays, rules)
     o.add_bernoulli_source(inp, np.full(nch, workloads.bernoulli_threshold(0.15), np.uint32), seed=21)
     L = 4000
     o.add_delta_projection(pop, n, pop, rng.integers(0, n, L).astype(np.uint32), rng.integers(0, n, L).astype(np.uint32),
@@ -198,15 +208,16 @@ def resource_altai_network(o, seed):
     return pop, inp, dop
 
 
+@pytest.mark.parametrize("varied", [False, True], ids=["shared-rule", "varied-rules"])
 @pytest.mark.parametrize("train", list(TRAIN_FORMS))
-def test_resource_altai_training_matches_oracle(train):
+def test_resource_altai_training_matches_oracle(train, varied):
     """Synaptic-resource AltAI-LIF training (altai_stdp.h:35-149) over 300 steps with dopamine every 10
-    steps: spike trains identical, weights within 1e-6."""
-    say(f"case: resource AltAI-LIF training, 300 steps, a {train} per candidate")
+    steps: spike trains, weights and the neurons' training state identical."""
+    say(f"case: resource AltAI-LIF training, 300 steps, a {train} per candidate, varied rules {varied}")
     runs = {}
     for name in ("gpu", "oracle"):
         o = Oracle("exact") if name == "oracle" else device(train)
-        pop, inp, dop = resource_altai_network(o, 5)
+        pop, inp, dop = resource_altai_network(o, 5, varied)
         o.record_spikes(pop, 300)
         for t in range(0, 300, 10):
             o.step(9)
@@ -221,9 +232,11 @@ def test_resource_altai_training_matches_oracle(train):
     assert so.sum() > 300
     assert np.any(wo != 0.5)
     assert np.array_equal(sg, so), np.argwhere(sg != so)[:5].tolist()
-    assert np.max(np.abs(wg - wo)) <= 1e-6
+    # training replays the reference's float operations in their order: weights and neuron state bit for bit
+    assert np.array_equal(np.asarray(wg, np.float32), np.asarray(wo, np.float32))
     ng, no = runs["gpu"][2], runs["oracle"][2]
-    assert [x.isi_status for x in ng] == [x.isi_status for x in no]
+    for f in ("isi_status", "free_synaptic_resource", "stability", "last_spike_step", "first_isi_spike"):
+        assert [getattr(x, f) for x in ng] == [getattr(x, f) for x in no], f
 
 
 def test_plain_small_bit_exact():
