@@ -66,6 +66,8 @@ struct ResDev {
     ParamF coeff, res_thr, change_param, change_at_isi, d_h;
     ParamU dpt, drain, isi_max;
     uint32_t *cand;      // training candidates of this step (neuron indexes)
+    uint8_t *w_current;  // 1: every working synapse's weight is the recalculation of its resource (0 at load
+                         // and whenever the working projections change)
     uint32_t n_work;     // working projections: unlocked resource-STDP projections targeting the population
     uint32_t work[kMaxWorking];
 };

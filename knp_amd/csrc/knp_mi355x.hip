@@ -830,6 +830,7 @@ int init_network(knp_net *net) {
             if ((rc = dev_alloc(net, (void **)&R.dop_fix, (size_t)n * 8))) return rc;
             if ((rc = dev_alloc(net, (void **)&R.forced_now, n))) return rc;
             if ((rc = dev_alloc(net, (void **)&R.cand, (size_t)n * 4))) return rc;
+            if ((rc = dev_alloc(net, (void **)&R.w_current, n))) return rc;  // zero: not known current
             if ((rc = make_param_f(net, n, [&](uint32_t i) { return nr[i].synapse_sum_threshold_coefficient; }, &R.coeff))) return rc;
             if ((rc = make_param_f(net, n, [&](uint32_t i) { return nr[i].synaptic_resource_threshold; }, &R.res_thr))) return rc;
             if ((rc = make_param_f(net, n, [&](uint32_t i) { return nr[i].stability_change_parameter; }, &R.change_param))) return rc;
@@ -1492,6 +1493,9 @@ int upload_working_sets(knp_net *net) {
     for (uint32_t p = 0; p < net->pops.size(); ++p) {
         if (!net->pops[p].resource) continue;
         ResDev &R = net->table.pops[p].res;
+        const uint32_t before = R.n_work;
+        uint32_t work_before[kMaxWorking];
+        std::memcpy(work_before, R.work, sizeof(work_before));
         R.n_work = 0;
         for (uint32_t q = 0; q < net->projs.size(); ++q) {
             const HostProj &H = net->projs[q];
@@ -1499,6 +1503,12 @@ int upload_working_sets(knp_net *net) {
             if (R.n_work >= (uint32_t)kMaxWorking)
                 return fail(KNP_ERR_UNSUPPORTED, "more than 8 trainable projections into one population");
             R.work[R.n_work++] = q;
+        }
+        // a projection that joins the working set may hold weights that are not its resources' recalculation
+        const bool changed = R.n_work != before || std::memcmp(work_before, R.work, R.n_work * 4) != 0;
+        if (changed && R.w_current) {
+            const uint32_t n = net->table.pops[p].n;
+            HIP_TRY(hipMemsetAsync(R.w_current, 0, n, net->stream));
         }
     }
     return upload_table(net);

@@ -1553,19 +1553,36 @@ __device__ __forceinline__ void train_end(const ResDev &R, uint32_t n, const Tra
 // had_hebbian_update flags, resource -= d_u (and += d_h on a new contribution), weight recalculated. Adds the
 // old weight's fixed-point value to sum when positive; returns d_u in du (0 when the neuron is forced) and
 // whether the synapse contributed.
-__device__ __forceinline__ bool spiking_synapse(const ProjDev &Q, uint32_t n, uint64_t k, uint64_t t,
-                                                uint32_t status, float d_h, float scale, long long &sum, float &du) {
-    const uint32_t sy = Q.csc_identity ? (uint32_t)k : Q.col_syn[k];
-    const float w = Q.W[sy];
-    const uint8_t fl = Q.flags[sy];
-    const uint64_t per = (Q.rule_uniform & kUniPeriod) ? Q.u_period : Q.period[sy];
-    const uint32_t pre = Q.col_pre[k];
-    const uint64_t rs = Q.row_spike[pre];
-    const uint32_t delay = (Q.rule_uniform & kUniDelay) ? Q.u_delay : Q.delay_syn[sy];
-    const uint64_t spk = (rs != UINT64_MAX ? rs : Q.last_spike_step[sy]) + delay - 1;
-    const float r = Q.r[sy];
-    du = status != 0 ? ((Q.rule_uniform & kUniDu) ? Q.u_du : Q.d_u[sy]) : 0.f;
-    const float wmin = rule_wmin(Q, sy), wmax = rule_wmax(Q, sy);
+struct SpikingSyn {  // what the spiking pass reads of one synapse
+    uint32_t sy, pre;
+    float w, r, du, wmin, wmax;
+    uint64_t per, spk;
+    uint8_t fl;
+};
+
+__device__ __forceinline__ SpikingSyn spiking_load(const ProjDev &Q, uint64_t k, uint32_t status) {
+    SpikingSyn x;
+    x.sy = Q.csc_identity ? (uint32_t)k : Q.col_syn[k];
+    x.w = Q.W[x.sy];
+    x.fl = Q.flags[x.sy];
+    x.per = (Q.rule_uniform & kUniPeriod) ? Q.u_period : Q.period[x.sy];
+    x.pre = Q.col_pre[k];
+    const uint64_t rs = Q.row_spike[x.pre];
+    const uint32_t delay = (Q.rule_uniform & kUniDelay) ? Q.u_delay : Q.delay_syn[x.sy];
+    x.spk = (rs != UINT64_MAX ? rs : Q.last_spike_step[x.sy]) + delay - 1;
+    x.r = Q.r[x.sy];
+    x.du = status != 0 ? ((Q.rule_uniform & kUniDu) ? Q.u_du : Q.d_u[x.sy]) : 0.f;
+    x.wmin = rule_wmin(Q, x.sy);
+    x.wmax = rule_wmax(Q, x.sy);
+    return x;
+}
+
+__device__ __forceinline__ bool spiking_update(const ProjDev &Q, uint32_t n, const SpikingSyn &x, uint64_t t,
+                                               uint32_t status, float d_h, float scale, long long &sum) {
+    const uint32_t sy = x.sy, pre = x.pre;
+    const float w = x.w, r = x.r, du = x.du, wmin = x.wmin, wmax = x.wmax;
+    const uint64_t per = x.per, spk = x.spk;
+    const uint8_t fl = x.fl;
     if (w > 0.f) sum += __float2ll_rn(w * scale);
     bool hc = fl & 2, hh = fl & 1;
     const bool had_spike = in_interval(t - per, t, spk);
@@ -1591,6 +1608,13 @@ __device__ __forceinline__ bool spiking_synapse(const ProjDev &Q, uint32_t n, ui
     const uint8_t nf = (uint8_t)((hc ? 2 : 0) | (hh ? 1 : 0));
     if (nf != fl) Q.flags[sy] = nf;
     return contributing;
+}
+
+__device__ __forceinline__ bool spiking_synapse(const ProjDev &Q, uint32_t n, uint64_t k, uint64_t t,
+                                                uint32_t status, float d_h, float scale, long long &sum, float &du) {
+    const SpikingSyn x = spiking_load(Q, k, status);
+    du = x.du;
+    return spiking_update(Q, n, x, t, status, d_h, scale, sum);
 }
 
 // Dopamine pass of synapse k (blifat_stdp.h:133-156): resource += change where the synapse contributed and the
@@ -1641,29 +1665,46 @@ __device__ __forceinline__ void train_candidate_wave(const NetDev *__restrict__ 
         for (uint32_t j = 0; j < R.n_work; ++j) {
             const ProjDev &Q = net->projs[R.work[j]];
             const uint64_t cb = Q.col_ptr[n], ce = Q.col_ptr[n + 1];
-            for (uint64_t base = cb; base < ce; base += 64) {
-                const uint64_t k = base + lane;
-                const bool act = k < ce;
-                float du = 0.f;
-                const bool contributing = act && spiking_synapse(Q, n, k, t, s.status, d_h, P.scale, sum, du);
-                const uint64_t am = __ballot(act), nz = __ballot(act && du != 0.f), cm = __ballot(contributing);
-                if (s.status != 0) {
-                    // lane 0 is active in every chunk; one d_u over the chunk takes the closed form
-                    const float du0 = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(du)));
-                    if (du0 != 0.f && __ballot(act && __float_as_int(du) != __float_as_int(du0)) == 0)
-                        s.freer = replay_uniform(s.freer, am, cm, du0, -d_h);
-                    else
-                        s.freer = serial_free(s.freer, am, nz, cm, du, -d_h, true);
+            // two chunks at a time: both chunks' loads are issued before the first chunk's stores (the chunks
+            // hold different synapses), then each chunk is updated and its free-resource sum replayed in order
+            for (uint64_t base = cb; base < ce; base += 128) {
+                SpikingSyn x[2];
+                bool act[2];
+#pragma unroll
+                for (int u = 0; u < 2; ++u) {
+                    act[u] = base + 64 * u + lane < ce;
+                    if (act[u]) x[u] = spiking_load(Q, base + 64 * u + lane, s.status);
+                }
+#pragma unroll
+                for (int u = 0; u < 2; ++u) {
+                    if (base + 64 * u >= ce) break;
+                    const bool contributing = act[u] && spiking_update(Q, n, x[u], t, s.status, d_h, P.scale, sum);
+                    const float du = act[u] ? x[u].du : 0.f;
+                    const uint64_t am = __ballot(act[u]), nz = __ballot(act[u] && du != 0.f);
+                    const uint64_t cm = __ballot(contributing);
+                    if (s.status != 0) {
+                        // lane 0 is active in every chunk; one d_u over the chunk takes the closed form
+                        const float du0 = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(du)));
+                        if (du0 != 0.f && __ballot(act[u] && __float_as_int(du) != __float_as_int(du0)) == 0)
+                            s.freer = replay_uniform(s.freer, am, cm, du0, -d_h);
+                        else
+                            s.freer = serial_free(s.freer, am, nz, cm, du, -d_h, true);
+                    }
                 }
             }
         }
         sum = wave_sum_i64(sum);
         if (lane == 0) R.add_thr[n] = fixed_to_float(sum, P.inv_scale) * pf(R.coeff, n);
     }
+    bool passed = s.spiked;  // a pass over every working synapse recalculated the weights from the resources
     if (dopamine_runs(s)) {
         const bool time_ok = (t - s.lss) <= (uint64_t)pu(R.dpt, n);
         const float rc = (float)(s.dop * (double)stability_factor(s.stab));
-        for (uint32_t j = 0; j < R.n_work; ++j) {
+        // without time_ok no resource changes; the pass then only recalculates weights, which changes none
+        // once a pass has made every weight its recalculation
+        const bool needed = time_ok || !R.w_current[n];
+        passed |= needed;
+        for (uint32_t j = 0; j < R.n_work && needed; ++j) {
             const ProjDev &Q = net->projs[R.work[j]];
             const uint64_t cb = Q.col_ptr[n], ce = Q.col_ptr[n + 1];
             for (uint64_t base = cb; base < ce; base += 64) {
@@ -1688,8 +1729,12 @@ __device__ __forceinline__ void train_candidate_wave(const NetDev *__restrict__ 
             for (uint64_t k = cb + lane; k < ce; k += 64) renorm_synapse(Q, n, k, add);
         }
         s.freer = 0.f;
+        passed = true;
     }
-    if (lane == 0) train_end(R, n, s);
+    if (lane == 0) {
+        train_end(R, n, s);
+        if (passed) R.w_current[n] = 1;
+    }
 }
 
 // Synaptic-resource STDP of one population's training candidates (train_population_impl, blifat_stdp.h:158-172).
@@ -1806,11 +1851,15 @@ __global__ __launch_bounds__(kTrainBlock) void k_train(const NetDev *__restrict_
                 R.add_thr[n] = fixed_to_float(total, P.inv_scale) * pf(R.coeff, n);
             }
         }
+        bool passed = s.spiked;  // a pass over every working synapse recalculated the weights
         if (dopamine_runs(s)) {
             const bool time_ok = (t - s.lss) <= (uint64_t)pu(R.dpt, n);
             const float rc = (float)(s.dop * (double)stability_factor(s.stab));
+            // without time_ok the pass only recalculates weights, which changes none once they are current
+            const bool needed = time_ok || !R.w_current[n];
+            passed |= needed;
             uint32_t contrib = 0;
-            for (uint32_t g = wv; g < G; g += W) {
+            for (uint32_t g = wv; g < G && needed; g += W) {
                 uint32_t j;
                 uint64_t base, end;
                 if (!chunk(g, j, base, end)) break;
@@ -1843,8 +1892,12 @@ __global__ __launch_bounds__(kTrainBlock) void k_train(const NetDev *__restrict_
                 if (k < end) renorm_synapse(net->projs[R.work[j]], n, k, add);
             }
             s.freer = 0.f;
+            passed = true;
         }
-        if (threadIdx.x == 0) train_end(R, n, s);
+        if (threadIdx.x == 0) {
+            train_end(R, n, s);
+            if (passed) R.w_current[n] = 1;
+        }
     }
 }
 
