@@ -1641,9 +1641,10 @@ __device__ __forceinline__ bool dopamine_synapse(const ProjDev &Q, uint32_t n, u
 }
 
 // renormalize_resource for synapse k (blifat_stdp.h:174-190): resource += add, weight recalculated.
-__device__ __forceinline__ void renorm_synapse(const ProjDev &Q, uint32_t n, uint64_t k, float add) {
+// r0 and w: the synapse's resource and weight as loaded.
+__device__ __forceinline__ void renorm_update(const ProjDev &Q, uint32_t n, uint64_t k, float r0, float w,
+                                              float add) {
     const uint32_t s = Q.csc_identity ? (uint32_t)k : Q.col_syn[k];
-    const float r0 = Q.r[s], w = Q.W[s];
     const float r = r0 + add;
     if (__float_as_uint(r) != __float_as_uint(r0)) Q.r[s] = r;
     const float wn = recalc_weight(r, rule_wmin(Q, s), rule_wmax(Q, s));
@@ -1652,6 +1653,18 @@ __device__ __forceinline__ void renorm_synapse(const ProjDev &Q, uint32_t n, uin
         if (Q.dense) dense_store_weight(Q, n, gp(Q.col_pre)[k], wn);
     }
 }
+
+__device__ __forceinline__ void renorm_synapse(const ProjDev &Q, uint32_t n, uint64_t k, float add) {
+    const uint32_t s = Q.csc_identity ? (uint32_t)k : Q.col_syn[k];
+    renorm_update(Q, n, k, Q.r[s], Q.W[s], add);
+}
+
+#ifndef KNP_TRAIN_UNROLL
+#define KNP_TRAIN_UNROLL 2
+#endif
+// Chunks whose loads the wave form issues together (latency: a batch's candidates fill the device with waves
+// that each walk one neuron's chunks).
+constexpr uint32_t kTrainUnroll = KNP_TRAIN_UNROLL;
 
 // One training candidate on one wave: the synapses of each working projection (load order, then CSC order)
 // 64 at a time, the order-dependent free-resource sum replayed after every chunk.
@@ -1665,18 +1678,19 @@ __device__ __forceinline__ void train_candidate_wave(const NetDev *__restrict__ 
         for (uint32_t j = 0; j < R.n_work; ++j) {
             const ProjDev &Q = net->projs[R.work[j]];
             const uint64_t cb = Q.col_ptr[n], ce = Q.col_ptr[n + 1];
-            // two chunks at a time: both chunks' loads are issued before the first chunk's stores (the chunks
-            // hold different synapses), then each chunk is updated and its free-resource sum replayed in order
-            for (uint64_t base = cb; base < ce; base += 128) {
-                SpikingSyn x[2];
-                bool act[2];
+            // kTrainUnroll chunks at a time: all their loads are issued before the first chunk's stores (the
+            // chunks hold different synapses), then each chunk is updated and its free-resource sum replayed
+            // in order
+            for (uint64_t base = cb; base < ce; base += 64 * kTrainUnroll) {
+                SpikingSyn x[kTrainUnroll];
+                bool act[kTrainUnroll];
 #pragma unroll
-                for (int u = 0; u < 2; ++u) {
+                for (uint32_t u = 0; u < kTrainUnroll; ++u) {
                     act[u] = base + 64 * u + lane < ce;
                     if (act[u]) x[u] = spiking_load(Q, base + 64 * u + lane, s.status);
                 }
 #pragma unroll
-                for (int u = 0; u < 2; ++u) {
+                for (uint32_t u = 0; u < kTrainUnroll; ++u) {
                     if (base + 64 * u >= ce) break;
                     const bool contributing = act[u] && spiking_update(Q, n, x[u], t, s.status, d_h, P.scale, sum);
                     const float du = act[u] ? x[u].du : 0.f;
@@ -1726,7 +1740,19 @@ __device__ __forceinline__ void train_candidate_wave(const NetDev *__restrict__ 
         for (uint32_t j = 0; j < R.n_work; ++j) {
             const ProjDev &Q = net->projs[R.work[j]];
             const uint64_t cb = Q.col_ptr[n], ce = Q.col_ptr[n + 1];
-            for (uint64_t k = cb + lane; k < ce; k += 64) renorm_synapse(Q, n, k, add);
+            for (uint64_t k = cb + lane; k < ce; k += 64 * kTrainUnroll) {
+                float r0[kTrainUnroll], w0[kTrainUnroll];
+#pragma unroll
+                for (uint32_t u = 0; u < kTrainUnroll; ++u)
+                    if (k + 64 * u < ce) {
+                        const uint32_t sy = Q.csc_identity ? (uint32_t)(k + 64 * u) : Q.col_syn[k + 64 * u];
+                        r0[u] = Q.r[sy];
+                        w0[u] = Q.W[sy];
+                    }
+#pragma unroll
+                for (uint32_t u = 0; u < kTrainUnroll; ++u)
+                    if (k + 64 * u < ce) renorm_update(Q, n, k + 64 * u, r0[u], w0[u], add);
+            }
         }
         s.freer = 0.f;
         passed = true;
