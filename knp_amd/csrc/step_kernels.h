@@ -103,18 +103,6 @@ __device__ __forceinline__ float fixed_to_float(int64_t a, float inv_scale) {
     return __ll2float_rn((long long)a) * inv_scale;
 }
 
-// Wave-level compaction: lanes with `pred` append `value` to list[*counter ...] with one atomic per wave.
-__device__ __forceinline__ void wave_append(bool pred, uint32_t value, uint32_t *counter, uint32_t *list) {
-    uint64_t mask = __ballot(pred);
-    if (mask == 0) return;
-    uint32_t lane = lane_id();
-    int leader = __ffsll((unsigned long long)mask) - 1;
-    uint32_t base = 0;
-    if ((int)lane == leader) base = atomicAdd(counter, (uint32_t)__popcll(mask));
-    base = __shfl(base, leader);
-    if (pred) list[base + __popcll(mask & ((1ull << lane) - 1ull))] = value;
-}
-
 // Stores and loads of data that another workgroup reads or wrote within the same persistent launch:
 // device-scope relaxed atomics, i.e. written through and read past this XCD's L2 (the L2s of the XCDs are
 // not coherent with each other inside a kernel). COHERENT = false: plain accesses (kernel boundaries order
@@ -185,6 +173,29 @@ __device__ __forceinline__ uint32_t block_segment_write(const NetDev *__restrict
     }
     __syncthreads();
     return rows;
+}
+
+// Append the workgroup's values with pred set to a list: one atomic on the list's counter per workgroup (a
+// counter shared by every workgroup of a large population is otherwise hit by each of its waves). The
+// entries of a workgroup are contiguous; the order of workgroups is whatever the atomics give. Every thread
+// of the workgroup calls it.
+template <uint32_t NT>
+__device__ __forceinline__ void block_append(bool pred, uint32_t value, uint32_t *counter, uint32_t *list) {
+    __shared__ uint32_t wcnt[NT / 64];
+    __shared__ uint32_t base;
+    const uint64_t mask = __ballot(pred);
+    const uint32_t lane = lane_id(), w = threadIdx.x / 64;
+    if (lane == 0) wcnt[w] = (uint32_t)__popcll(mask);
+    __syncthreads();
+    uint32_t off = 0, total = 0;
+    for (uint32_t k = 0; k < NT / 64; ++k) {
+        if (k < w) off += wcnt[k];
+        total += wcnt[k];
+    }
+    if (threadIdx.x == 0 && total) base = atomicAdd(counter, total);
+    __syncthreads();
+    if (pred) list[base + off + __popcll(mask & ((1ull << lane) - 1ull))] = value;
+    __syncthreads();  // wcnt and base are free for the next call
 }
 
 // Write the wave's 64-neuron spike mask into the bitmap words (and the record), bit i = neuron i.
@@ -463,7 +474,7 @@ __device__ __forceinline__ uint32_t population_step(const NetDev *__restrict__ n
         P.seg_base + local_block, vp_pre);
     const uint64_t mask = __ballot(spike);
     if (threadIdx.x < (uint32_t)kBlock) wave_bitmap(mask, local_block * kBlock + (threadIdx.x & ~63u), P, t);
-    if (KIND == POP_RESOURCE) wave_append(cand, i, &ctl->cand_count[p], P.res.cand);
+    if (KIND == POP_RESOURCE) block_append<NT>(cand, i, &ctl->cand_count[p], P.res.cand);
     return rows;
 }
 
