@@ -198,6 +198,9 @@ struct ProjDev {
     const uint32_t *dn_pre_inst;  // [n_pre] instance of each presynaptic index (UINT32_MAX: none)
     const uint32_t *dn_post_inst; // [n_post] instance of each postsynaptic neuron (UINT32_MAX: none)
     const uint32_t *dn_tile;      // [dn_tiles] instance << 16 | first neuron of the tile / 32
+    const struct DenseTile *dn_tile_info;  // [dn_tiles] what the expansion of a tile reads, in one entry
+    const uint64_t *dn_pre_info;  // [n_pre] count word (octet) | byte << 32 | instance post_count << 35
+                                  // (UINT64_MAX: the index is in no instance)
     uint16_t *dn_w;              // BF16 weights; instance k from base: [octet][neuron (npad)][8 inputs]
     uint16_t *dn_w_lo;           // kDenseSplit: BF16 of w - dn_w (same layout), else null
     unsigned long long *dn_part; // [kDenseCountBlocks][4] impacts, rows, weight bytes (skip) per counts workgroup
@@ -210,6 +213,22 @@ struct ProjDev {
 struct alignas(8) DenseInst {
     uint32_t pre_begin, post_begin, post_count, npad, n_in, n_oct, cbase, pad_;
     uint64_t base;  // first BF16 weight of the instance in dn_w
+};
+
+// A 32-neuron tile of an instance: the instance's octets (cbase, n_oct), weights (base, npad), neurons
+// (post_begin, post_count) and the tile's first neuron j0 within them.
+struct alignas(16) DenseTile {
+    uint32_t cbase, n_oct, npad, post_begin, post_count, j0;
+    uint64_t base;
+};
+
+// The MFMA-expanded projections whose spike counts one launch of k_dense_counts takes (their count
+// buffers: the projection's own, or a region of the device scratch).
+constexpr uint32_t kDenseSetMax = 8;
+struct DenseCountSet {
+    uint32_t n;
+    uint32_t proj[kDenseSetMax];
+    uint32_t *cnt[kDenseSetMax];
 };
 constexpr uint32_t kDenseCountBlocks = 1024;  // workgroups of k_dense_counts at most
 constexpr uint32_t kDenseSkipEmpty = 1;  // flag: multiply only the octets of presynaptic indexes that spiked

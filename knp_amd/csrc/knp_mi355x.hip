@@ -1185,6 +1185,25 @@ int init_network(knp_net *net) {
             if ((rc = upload(net, &pi, B.pre_inst.data(), B.pre_inst.size()))) return rc;
             if ((rc = upload(net, &po, B.post_inst.data(), B.post_inst.size()))) return rc;
             if ((rc = upload(net, &tl, tiles.data(), tiles.size()))) return rc;
+            // one-load tables of the kernels: per tile its instance's fields, per input its count byte
+            std::vector<DenseTile> tinfo(tiles.size());
+            for (size_t x = 0; x < tiles.size(); ++x) {
+                const DenseInst &I = B.inst[tiles[x] >> 16];
+                tinfo[x] = DenseTile{I.cbase, I.n_oct, I.npad, I.post_begin, I.post_count, (tiles[x] & 0xFFFFu) * 32, I.base};
+            }
+            std::vector<uint64_t> pinfo(B.pre_inst.size(), UINT64_MAX);
+            for (size_t i = 0; i < B.pre_inst.size(); ++i) {
+                if (B.pre_inst[i] == 0xFFFFFFFFu) continue;
+                const DenseInst &I = B.inst[B.pre_inst[i]];
+                const uint32_t x = (uint32_t)i - I.pre_begin;
+                pinfo[i] = (uint64_t)(I.cbase + (x >> 3)) | (uint64_t)(x & 7) << 32 | (uint64_t)I.post_count << 35;
+            }
+            DenseTile *ti;
+            uint64_t *pf;
+            if ((rc = upload(net, &ti, tinfo.data(), tinfo.size()))) return rc;
+            if ((rc = upload(net, &pf, pinfo.data(), pinfo.size()))) return rc;
+            Q.dn_tile_info = ti;
+            Q.dn_pre_info = pf;
             if ((rc = upload(net, &wd, wb.data(), wb.size()))) return rc;
             if ((rc = dev_alloc(net, (void **)&Q.dn_cnt, (size_t)2 * B.n_oct * 8))) return rc;
             if ((rc = dev_alloc(net, (void **)&Q.dn_part, (size_t)kDenseCountBlocks * 4 * 8))) return rc;
@@ -1648,6 +1667,9 @@ int run_steps(knp_net *net, uint64_t n_steps, uint32_t *counts = nullptr, bool t
     }
     net->counts_in_scratch = counts != nullptr;
     std::vector<uint32_t *> cnt(net->projs.size(), nullptr);
+    std::vector<uint32_t> dense_projs;  // projections expanded on MFMA, in load order
+    for (uint32_t q = 0; q < net->projs.size(); ++q)
+        if (net->table.projs[q].dense) dense_projs.push_back(q);
     {
         uint64_t off = 0;
         for (uint32_t q = 0; q < net->projs.size(); ++q) {
@@ -1741,18 +1763,32 @@ int run_steps(knp_net *net, uint64_t n_steps, uint32_t *counts = nullptr, bool t
             k_additive_append<<<net->scatter_grid, kBlock, 0, net->stream>>>(D, q, targ);
             HIP_TRY(hipGetLastError());
         }
-        // projections expanded on MFMA (after training: they use the step's trained weights)
-        for (uint32_t q = 0; q < net->projs.size(); ++q) {
+        // projections expanded on MFMA (after training: they use the step's trained weights): the spike counts
+        // of all of them in one launch (per set of kDenseSetMax), then each product
+        for (size_t s0 = 0; s0 < dense_projs.size(); s0 += kDenseSetMax) {
+            DenseCountSet set{};
+            uint64_t gc = 1;
+            for (size_t m = s0; m < dense_projs.size() && set.n < kDenseSetMax; ++m) {
+                const uint32_t q = dense_projs[m];
+                const ProjDev &Qd = net->table.projs[q];
+                const uint64_t words = (uint64_t)Qd.dn_oct * 2;
+                gc = std::max<uint64_t>(gc, std::max<uint64_t>(Qd.dn_v1 - Qd.dn_v0, (words + kBlock - 1) / kBlock));
+                set.proj[set.n] = q;
+                set.cnt[set.n] = cnt[q];
+                ++set.n;
+            }
+            const uint32_t g = (uint32_t)std::min<uint64_t>(kDenseCountBlocks, gc);
+            if ((rc = timed_launch(net, KNP_KERNEL_DENSE_EXPAND, [&] {
+                     k_dense_counts<<<g, kBlock, 0, net->stream>>>(D, set, targ);
+                 })))
+                return rc;
+        }
+        for (uint32_t q : dense_projs) {
             const ProjDev &Qd = net->table.projs[q];
-            if (!Qd.dense) continue;
-            const uint64_t words = (uint64_t)Qd.dn_oct * 2;
-            const uint32_t gc = (uint32_t)std::max<uint64_t>(
-                1, std::min<uint64_t>(kDenseCountBlocks, std::max<uint64_t>(Qd.dn_v1 - Qd.dn_v0, (words + kBlock - 1) / kBlock)));
             const uint32_t ge = (Qd.dn_tiles + 3) / 4;
             const bool skip = (Qd.dense - 1) & kDenseSkipEmpty, split = Qd.dn_w_lo != nullptr;
             if ((rc = timed_launch(net, KNP_KERNEL_DENSE_EXPAND, [&] {
                      uint32_t *c = cnt[q];
-                     k_dense_counts<<<gc, kBlock, 0, net->stream>>>(D, q, targ, c);
                      if (skip && split) k_dense_expand<true, true><<<ge, kBlock, 0, net->stream>>>(D, q, targ, c);
                      else if (skip) k_dense_expand<true, false><<<ge, kBlock, 0, net->stream>>>(D, q, targ, c);
                      else if (split) k_dense_expand<false, true><<<ge, kBlock, 0, net->stream>>>(D, q, targ, c);

@@ -2155,61 +2155,54 @@ __global__ __launch_bounds__(kBlock) void k_dense_mfma(const NetDev *__restrict_
 // instance's spike counts (K = presynaptic indexes) and a BF16 copy of its weights (N = postsynaptic
 // neurons) on the matrix cores, with FP32 accumulation; training keeps the FP32 weights and refreshes the copy.
 
-// Spike counts of a dense projection for step t: every spiking presynaptic index in the projection's virtual
-// segments adds 1 to its byte (parity t & 1); the counts of parity (t + 1) & 1, read by the previous step's
-// product, are cleared for the next step. Synaptic-resource projections note the step each row spiked
-// (init_synapse, synaptic_resource_stdp.h:36-62). Counts the impacts (the instance's neurons per spike) and rows.
-// cnt: the projection's count buffer ([2][dn_oct][8] bytes: Q.dn_cnt, or a region of the library's device
-// scratch for knp_mi355x_step_noalloc).
-__global__ __launch_bounds__(kBlock) void k_dense_counts(const NetDev *__restrict__ net, uint32_t q, uint64_t t,
-                                                         uint32_t *cnt) {
+// Spike counts of step t for a set of MFMA-expanded projections, one launch for all of them: every spiking
+// row of a projection's virtual segments adds 1 to its index's byte in the octet's 64-bit count word (a
+// non-returning atomic; a channel message repeating an index adds more), synaptic-resource projections note
+// the step each row spiked (init_synapse, synaptic_resource_stdp.h:36-62), and the next step's words (read by
+// the previous step's product) are cleared. set.cnt: each projection's count buffer ([2][dn_oct][8] bytes:
+// its own, or a region of the library's device scratch for knp_mi355x_step_noalloc). The workgroup's impact and row totals go to its own
+// counter slot (no shared addresses: every workgroup would queue on them).
+__global__ __launch_bounds__(kBlock) void k_dense_counts(const NetDev *__restrict__ net, DenseCountSet set, uint64_t t) {
     t = resolve_step(net, t);
-    const ProjDev &Q = net->projs[q];
-    const size_t words = (size_t)Q.dn_oct * 2;
-    uint32_t *cur = cnt + (size_t)(t & 1) * words;
-    uint32_t *nxt = cnt + (size_t)((t + 1) & 1) * words;
-    for (size_t k = (size_t)blockIdx.x * kBlock + threadIdx.x; k < words; k += (size_t)gridDim.x * kBlock) gp(nxt)[k] = 0;
     const uint32_t *vc = net->vcount + (size_t)(t & 1) * net->n_vseg;
     const size_t par = (size_t)(t & 1) * net->list_pstride;
-    unsigned long long ev = 0, rows = 0, oct_bytes = 0;
-    for (uint32_t v = Q.dn_v0 + blockIdx.x; v < Q.dn_v1; v += gridDim.x) {
-        const uint32_t cnt = gp(vc)[v];
-        const uint32_t *list = net->seg_list[net->vseg[v]] + par + 1;
-        for (uint32_t e = threadIdx.x; e < cnt; e += kBlock) {
-            const uint32_t pre = gp(list)[e];
-            if (pre >= Q.n_pre) continue;
-            const uint32_t k = gp(Q.dn_pre_inst)[pre];
-            if (k == 0xFFFFFFFFu) continue;
-            const DenseInst &I = Q.dn_inst_tab[k];
-            const uint32_t i = pre - I.pre_begin;
-            // the octet's 8 counts as one 64-bit word: the spike that makes it non-zero adds the weight bytes
-            // a KNP_DENSE_SKIP_EMPTY product reads for it
-            const unsigned long long old =
-                atomicAdd((unsigned long long *)&cur[((size_t)I.cbase + (i >> 3)) * 2], 1ull << (8u * (i & 7u)));
-            if (old == 0) oct_bytes += (unsigned long long)I.npad * (Q.dn_w_lo ? 32 : 16);
-            if (Q.kind == PROJ_RESOURCE) gp(Q.row_spike)[pre] = t;
-            ev += I.post_count;
-            rows += 1;
+    unsigned long long ev = 0, rows = 0;
+    for (uint32_t m = 0; m < set.n; ++m) {
+        const ProjDev &Q = net->projs[set.proj[m]];
+        const size_t words = (size_t)Q.dn_oct * 2;
+        uint32_t *cur = set.cnt[m] + (size_t)(t & 1) * words;
+        uint32_t *nxt = set.cnt[m] + (size_t)((t + 1) & 1) * words;
+        for (size_t k = (size_t)blockIdx.x * kBlock + threadIdx.x; k < words; k += (size_t)gridDim.x * kBlock)
+            gp(nxt)[k] = 0;
+        for (uint32_t v = Q.dn_v0 + blockIdx.x; v < Q.dn_v1; v += gridDim.x) {
+            const uint32_t cnt = gp(vc)[v];
+            const uint32_t *list = net->seg_list[net->vseg[v]] + par + 1;
+            for (uint32_t e = threadIdx.x; e < cnt; e += kBlock) {
+                const uint32_t pre = gp(list)[e];
+                if (pre >= Q.n_pre) continue;
+                const uint64_t info = gp(Q.dn_pre_info)[pre];
+                if (info == UINT64_MAX) continue;
+                atomicAdd((unsigned long long *)&cur[(size_t)(uint32_t)info * 2], 1ull << (8u * ((info >> 32) & 7u)));
+                if (Q.kind == PROJ_RESOURCE) gp(Q.row_spike)[pre] = t;
+                ev += info >> 35;
+                rows += 1;
+            }
         }
     }
-    // the workgroup's totals into its own counters (no atomics on shared addresses: every workgroup would
-    // queue on them)
-    __shared__ unsigned long long red[3][kBlock / 64];
+    __shared__ unsigned long long red[2][kBlock / 64];
     for (int off = 32; off > 0; off >>= 1) {
         ev += __shfl_xor(ev, off);
         rows += __shfl_xor(rows, off);
-        oct_bytes += __shfl_xor(oct_bytes, off);
     }
     if (lane_id() == 0) {
         red[0][threadIdx.x / 64] = ev;
         red[1][threadIdx.x / 64] = rows;
-        red[2][threadIdx.x / 64] = oct_bytes;
     }
     __syncthreads();
-    if (threadIdx.x < 3) {
+    if (threadIdx.x < 2) {
         unsigned long long x = 0;
         for (int w = 0; w < kBlock / 64; ++w) x += red[threadIdx.x][w];
-        if (x) gp(Q.dn_part)[4 * blockIdx.x + threadIdx.x] += x;
+        if (x) gp(net->projs[set.proj[0]].dn_part)[4 * blockIdx.x + threadIdx.x] += x;
     }
 }
 
@@ -2252,14 +2245,19 @@ __global__ __launch_bounds__(kBlock) void k_dense_expand(const NetDev *__restric
     if (tile >= Q.dn_tiles) return;
     const PopDev &P = net->pops[Q.post_pop];
     const uint32_t lane = lane_id(), h = lane >> 5, col = lane & 31;
-    const uint32_t tv = gp(Q.dn_tile)[tile], k = tv >> 16, j0 = (tv & 0xFFFFu) * 32;
-    const DenseInst I = Q.dn_inst_tab[k];
-    const uint32_t n_oct = I.n_oct;
-    const KNP_GLOBAL uint64_t *cnt = (const KNP_GLOBAL uint64_t *)gp(cnt_base) + (size_t)(t & 1) * Q.dn_oct + I.cbase;
-    const KNP_GLOBAL uint16_t *wb = gp(Q.dn_w) + I.base + (size_t)(j0 + col) * 8;
-    const KNP_GLOBAL uint16_t *wl = SPLIT ? gp(Q.dn_w_lo) + I.base + (size_t)(j0 + col) * 8 : nullptr;
-    const size_t ostride = (size_t)I.npad * 8;
+    const DenseTile T = Q.dn_tile_info[tile];
+    const uint32_t n_oct = T.n_oct, j0 = T.j0;
+    // the ring cell this lane adds to, loaded now (no other launch runs meanwhile, and each neuron of the tile
+    // belongs to one lane), so that the sum is stored without a further round trip
+    const bool writer = h == 0 && j0 + col < T.post_count;
+    const size_t ai = (size_t)((t + Q.dn_delay) % P.D) * P.n + (T.post_begin + j0 + col);
+    const int64_t old = writer ? gp(P.acc_v)[ai] : 0;
+    const KNP_GLOBAL uint64_t *cnt = (const KNP_GLOBAL uint64_t *)gp(cnt_base) + (size_t)(t & 1) * Q.dn_oct + T.cbase;
+    const KNP_GLOBAL uint16_t *wb = gp(Q.dn_w) + T.base + (size_t)(j0 + col) * 8;
+    const KNP_GLOBAL uint16_t *wl = SPLIT ? gp(Q.dn_w_lo) + T.base + (size_t)(j0 + col) * 8 : nullptr;
+    const size_t ostride = (size_t)T.npad * 8;
     constexpr uint32_t kNone = 0xFFFFFFFFu;
+    uint32_t read_oct = 0;  // octets whose weights this wave reads
     v16f acc, acc_lo;
 #pragma unroll
     for (int v = 0; v < 16; ++v) acc[v] = acc_lo[v] = 0.f;
@@ -2268,6 +2266,7 @@ __global__ __launch_bounds__(kBlock) void k_dense_expand(const NetDev *__restric
         const uint32_t o = c0 + lane;
         const uint64_t mine = o < n_oct ? cnt[o] : 0ull;
         uint64_t mask = SKIP ? __ballot(mine != 0ull) : __ballot(o < n_oct);
+        read_oct += (uint32_t)__popcll(mask);
         while (mask) {
             uint32_t oct[4];
 #pragma unroll
@@ -2301,12 +2300,15 @@ __global__ __launch_bounds__(kBlock) void k_dense_expand(const NetDev *__restric
             }
         }
     }
-    if (h == 0 && j0 + col < I.post_count) {
-        const uint32_t n = I.post_begin + j0 + col;
-        const size_t ai = (size_t)((t + Q.dn_delay) % P.D) * P.n + n;
+    if (writer) {
         const long long v = __float2ll_rn((SPLIT ? acc[0] + acc_lo[0] : acc[0]) * P.scale);
-        gp(P.acc_v)[ai] += Q.dn_sign < 0.f ? -v : v;
+        gp(P.acc_v)[ai] = old + (Q.dn_sign < 0.f ? -v : v);
     }
+    // weight bytes read (32 neurons x 8 inputs x 2 bytes per octet, twice for two-part weights): the traffic
+    // of the skipping schedule, counted in the slots the counting workgroups use
+    if (SKIP && lane == 0 && read_oct)
+        atomicAdd(&Q.dn_part[4 * (blockIdx.x % kDenseCountBlocks) + 2],
+                  (unsigned long long)read_oct * (SPLIT ? 1024ull : 512ull));
 }
 
 }  // namespace knp_mi355x
