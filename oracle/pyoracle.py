@@ -22,14 +22,18 @@ def build():
     subprocess.run(["make", "-C", _HERE, "-s"], check=True)
 
 
-def _load(policy):
-    if policy in _LIBS:
-        return _LIBS[policy]
-    name = {"exact": "liboracle_exact.so", "f64": "liboracle_f64.so", "mt": "liboracle_mt.so",
-            "release": "liboracle_release.so"}[policy]
-    path = os.path.join(_HERE, "lib", name)
-    if not os.path.exists(path):
-        build()
+def _load(policy, path=None):
+    """The oracle library of a precision policy; path: a build of the oracle elsewhere (the tests that plant
+    mistakes in a copy of the oracle source load their builds this way)."""
+    key = path or policy
+    if key in _LIBS:
+        return _LIBS[key]
+    if path is None:
+        name = {"exact": "liboracle_exact.so", "f64": "liboracle_f64.so", "mt": "liboracle_mt.so",
+                "release": "liboracle_release.so"}[policy]
+        path = os.path.join(_HERE, "lib", name)
+        if not os.path.exists(path):
+            build()
     lib = C.CDLL(path)
     P = C.c_void_p
     u32, u64, i32 = C.c_uint32, C.c_uint64, C.c_int
@@ -75,7 +79,7 @@ def _load(policy):
         f = getattr(lib, name_)
         f.restype = res
         f.argtypes = args
-    _LIBS[policy] = lib
+    _LIBS[key] = lib
     return lib
 
 
@@ -87,12 +91,12 @@ class Oracle:
     """One network instance of the restated single-threaded CPU backend. threads > 1: the multi-threaded
     backend's decomposition of the same step (exact policy only; same results as one thread)."""
 
-    def __init__(self, policy="exact", threads=1):
+    def __init__(self, policy="exact", threads=1, lib_path=None):
         if threads > 1:
             if policy != "exact":
                 raise ValueError("the multi-threaded step is built for the exact policy only")
             policy = "mt"
-        self.lib = _load(policy)
+        self.lib = _load(policy, lib_path)
         self.net = self.lib.ko_create()
         if threads > 1:
             self.lib.ko_set_threads(self.net, int(threads))

@@ -239,6 +239,39 @@ RENORM_AFTER_4 = {"free": [0.0, 0.0, 0.0], "r": [0.5] * 3 + [0.25] * 3 + [-0.5] 
 
 
 # ---------------------------------------------------------------------------------------------------------
+# U: a non-zero d_u, and dopamine at the edge of dopamine_plasticity_time.
+# One neuron: d_h 0.25, isi_max 4, dopamine_plasticity_time 5, stability 0, stability_change_parameter 0
+# (the stability stays 0, factor 1), free 1. Trainable synapses ch0, ch1: r 1, w 0.5, period 2, d_u 0.125.
+# "dop": plain Delta DOPAMINE +0.5 from one channel. All delay 1.
+#   step 5 (ch0, ch1 at 4): spike; not_in_period -> period_started, first_isi 5; window [3, 5] holds 4: both
+#     contributed. Per synapse in order: r -= d_u, free += d_u, then r += d_h, free -= d_h:
+#     syn 0: r 1 - 0.125 + 0.25 = 1.125, free 1 + 0.125 - 0.25 = 0.875;
+#     syn 1: r 1.125, free 0.875 + 0.125 - 0.25 = 0.75.  w = 1.125 / 2.125 = 9/17.
+#   step 10 (dop at 9): 10 - 5 = 5 <= 5: change (float)(0.5 * 1) = 0.5 for both: r 1.625,
+#     free 0.75 - 0.5 - 0.5 = -0.25, w = 1.625 / 2.625 = 13/21; stability += 0 * ... = 0.
+#   step 11 (dop at 10): 11 - 5 = 6 > 5: no resource change; weights recalculated unchanged.
+def build_du(o):
+    cell = abi.resource_blifat_defaults(d_h=0.25, isi_max=4, dopamine_plasticity_time=5,
+                                        stability_change_parameter=0.0, free_synaptic_resource=1.0)
+    pop = resource_pop(o, [cell])
+    q = o.add_resource_stdp_projection(abi.KNP_NO_POPULATION, 2, pop, [0, 1], [0, 0], [0.5, 0.5], [1, 1],
+                                       rules(2, synaptic_resource=1.0, w_min=0.0, w_max=1.0, d_u=0.125,
+                                             dopamine_plasticity_period=2))
+    dop = o.add_delta_projection(abi.KNP_NO_POPULATION, 1, pop, [0], [0], [0.5], [1], [abi.DOPAMINE])
+    o.set_locked(q, False)
+    return pop, q, dop
+
+
+DU_PLAN = lambda q, dop: {4: [(q, [0, 1])], 9: [(dop, [0])], 10: [(dop, [0])]}  # noqa: E731
+DU_AFTER_6 = {"r": [1.125, 1.125], "free": [0.75], "w": [f32(9 / 17)] * 2, "stability": [0.0],
+              "isi_status": [abi.ISI_PERIOD_STARTED], "first_isi_spike": [5], "last_spike_step": [5],
+              "has_contributed": [1, 1], "had_hebbian": [1, 1]}
+DU_AFTER_11 = {"r": [1.625, 1.625], "free": [-0.25], "w": [f32(13 / 21)] * 2, "stability": [0.0]}
+DU_AFTER_12 = DU_AFTER_11
+DU_SPIKES = {0: [5]}
+
+
+# ---------------------------------------------------------------------------------------------------------
 def state(o, pop, q):
     n = o.resource_neurons(pop)
     rl = o.resource_rules(q)
@@ -268,7 +301,7 @@ def spike_steps(o, pop, n_neurons, steps):
 
 
 def run_all_traces(make, dense_flags=None):
-    """Runs the four traces on fresh networks from make() (an oracle or a device backend) and checks every
+    """Runs the five traces on fresh networks from make() (an oracle or a device backend) and checks every
     expected value. dense_flags (device only): expand the trained projection on MFMA with these flags; the
     window trace (its synapses have two delays, which the expansion does not take) is then left out."""
     def dense(o, q):
@@ -308,6 +341,19 @@ def run_all_traces(make, dense_flags=None):
         done = upto
         check(o, pop, q, exp, f"dopamine after {upto} steps")
     assert spike_steps(o, pop, 1, 21) == DOPAMINE_SPIKES
+    # U
+    o = make()
+    pop, q, dop = build_du(o)
+    dense(o, q)
+    o.record_spikes(pop, 12)
+    plan = DU_PLAN(q, dop)
+    run(o, plan, 6)
+    check(o, pop, q, DU_AFTER_6, "d_u after 6 steps")
+    run(o, {t - 6: v for t, v in plan.items() if t >= 6}, 5)
+    check(o, pop, q, DU_AFTER_11, "d_u: dopamine at the edge of the window")
+    run(o, {}, 1)
+    check(o, pop, q, DU_AFTER_12, "d_u: dopamine past the window")
+    assert spike_steps(o, pop, 1, 12) == DU_SPIKES
     # R
     o = make()
     pop, q = build_renorm(o)
