@@ -230,6 +230,15 @@ struct DenseCountSet {
     uint32_t proj[kDenseSetMax];
     uint32_t *cnt[kDenseSetMax];
 };
+
+// MFMA-expanded projections that one launch of k_dense_expand takes together: they reach the same neurons
+// through instances with the same postsynaptic ranges, with the same delay (one wave per tile sums them all).
+constexpr uint32_t kDenseGroupMax = 2;
+struct DenseGroup {
+    uint32_t n;
+    uint32_t proj[kDenseGroupMax];
+    const uint32_t *cnt[kDenseGroupMax];
+};
 constexpr uint32_t kDenseCountBlocks = 1024;  // workgroups of k_dense_counts at most
 constexpr uint32_t kDenseSkipEmpty = 1;  // flag: multiply only the octets of presynaptic indexes that spiked
 constexpr uint32_t kDenseSplit = 2;      // flag: each weight as two BF16 values (hi + lo), two products

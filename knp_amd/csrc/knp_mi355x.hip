@@ -76,6 +76,7 @@ struct HostProj {
     uint32_t gen_channels = 0;                   // channels of device sources feeding it
     int dense_format = -1;                       // knp_mi355x_set_projection_dense (-1: scatter)
     uint32_t dense_flags = 0;
+    std::vector<uint64_t> dense_sig;  // MFMA expansion: per tile, first neuron << 32 | neurons (grouping)
 };
 
 // The dense blocks (instances) of a projection expanded on MFMA.
@@ -1204,6 +1205,10 @@ int init_network(knp_net *net) {
             if ((rc = upload(net, &pf, pinfo.data(), pinfo.size()))) return rc;
             Q.dn_tile_info = ti;
             Q.dn_pre_info = pf;
+            H.dense_sig.resize(tinfo.size());
+            for (size_t x = 0; x < tinfo.size(); ++x)
+                H.dense_sig[x] = (uint64_t)(tinfo[x].post_begin + tinfo[x].j0) << 32 |
+                                 std::min<uint32_t>(32u, tinfo[x].post_count - tinfo[x].j0);
             if ((rc = upload(net, &wd, wb.data(), wb.size()))) return rc;
             if ((rc = dev_alloc(net, (void **)&Q.dn_cnt, (size_t)2 * B.n_oct * 8))) return rc;
             if ((rc = dev_alloc(net, (void **)&Q.dn_part, (size_t)kDenseCountBlocks * 4 * 8))) return rc;
@@ -1670,6 +1675,22 @@ int run_steps(knp_net *net, uint64_t n_steps, uint32_t *counts = nullptr, bool t
     std::vector<uint32_t> dense_projs;  // projections expanded on MFMA, in load order
     for (uint32_t q = 0; q < net->projs.size(); ++q)
         if (net->table.projs[q].dense) dense_projs.push_back(q);
+    // products summed by one launch: projections into the same neurons (same tiles) with the same delay
+    std::vector<std::vector<uint32_t>> dense_groups;
+    for (uint32_t q : dense_projs) {
+        const ProjDev &Qd = net->table.projs[q];
+        bool placed = false;
+        for (auto &g : dense_groups) {
+            const ProjDev &G0 = net->table.projs[g[0]];
+            if (g.size() < kDenseGroupMax && G0.post_pop == Qd.post_pop && G0.dn_delay == Qd.dn_delay &&
+                net->projs[g[0]].dense_sig == net->projs[q].dense_sig) {
+                g.push_back(q);
+                placed = true;
+                break;
+            }
+        }
+        if (!placed) dense_groups.push_back({q});
+    }
     {
         uint64_t off = 0;
         for (uint32_t q = 0; q < net->projs.size(); ++q) {
@@ -1783,16 +1804,16 @@ int run_steps(knp_net *net, uint64_t n_steps, uint32_t *counts = nullptr, bool t
                  })))
                 return rc;
         }
-        for (uint32_t q : dense_projs) {
-            const ProjDev &Qd = net->table.projs[q];
-            const uint32_t ge = (Qd.dn_tiles + 3) / 4;
-            const bool skip = (Qd.dense - 1) & kDenseSkipEmpty, split = Qd.dn_w_lo != nullptr;
+        for (const auto &members : dense_groups) {
+            DenseGroup grp{};
+            for (uint32_t q : members) {
+                grp.proj[grp.n] = q;
+                grp.cnt[grp.n] = cnt[q];
+                ++grp.n;
+            }
+            const uint32_t ge = (net->table.projs[members[0]].dn_tiles + 3) / 4;
             if ((rc = timed_launch(net, KNP_KERNEL_DENSE_EXPAND, [&] {
-                     uint32_t *c = cnt[q];
-                     if (skip && split) k_dense_expand<true, true><<<ge, kBlock, 0, net->stream>>>(D, q, targ, c);
-                     else if (skip) k_dense_expand<true, false><<<ge, kBlock, 0, net->stream>>>(D, q, targ, c);
-                     else if (split) k_dense_expand<false, true><<<ge, kBlock, 0, net->stream>>>(D, q, targ, c);
-                     else k_dense_expand<false, false><<<ge, kBlock, 0, net->stream>>>(D, q, targ, c);
+                     k_dense_expand<<<ge, kBlock, 0, net->stream>>>(D, grp, targ);
                  })))
                 return rc;
         }

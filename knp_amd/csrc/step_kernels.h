@@ -2232,39 +2232,26 @@ __device__ __forceinline__ v8s counts_to_bf16(uint64_t c) {
     return __builtin_bit_cast(v8s, w);
 }
 
-// One wave per 32-neuron tile of an instance: C (32 x 32) += A (16 K: spike counts, the same in every row) x
-// B (16 K x 32 neurons: BF16 weights) over the instance's octets, two octets per K-step (lanes 0-31 the first,
-// 32-63 the second), four K-steps per round with their loads issued first. SKIP: only octets holding a spike.
-// Every column of C is the neuron's sum; lanes 0-31 add register 0 (row 0) to the ring as fixed point.
+// The sum over one projection of a wave's 32-neuron tile: C (32 x 32) += A (16 K: spike counts, the same in
+// every row) x B (16 K x 32 neurons: BF16 weights) over the instance's octets, two octets per K-step (lanes
+// 0-31 the first, 32-63 the second), four K-steps per round with their loads issued first. SKIP: only octets
+// holding a spike. Every column of C is the neuron's sum: the lane's column of row 0 is returned (hi + lo for
+// two-part weights). mine0: this lane's count word of the first 64 octets, loaded by the caller.
 template <bool SKIP, bool SPLIT>
-__global__ __launch_bounds__(kBlock) void k_dense_expand(const NetDev *__restrict__ net, uint32_t q, uint64_t t,
-                                                         const uint32_t *cnt_base) {
-    t = resolve_step(net, t);
-    const ProjDev &Q = net->projs[q];
-    const uint32_t tile = blockIdx.x * (kBlock / 64) + threadIdx.x / 64;
-    if (tile >= Q.dn_tiles) return;
-    const PopDev &P = net->pops[Q.post_pop];
-    const uint32_t lane = lane_id(), h = lane >> 5, col = lane & 31;
-    const DenseTile T = Q.dn_tile_info[tile];
-    const uint32_t n_oct = T.n_oct, j0 = T.j0;
-    // the ring cell this lane adds to, loaded now (no other launch runs meanwhile, and each neuron of the tile
-    // belongs to one lane), so that the sum is stored without a further round trip
-    const bool writer = h == 0 && j0 + col < T.post_count;
-    const size_t ai = (size_t)((t + Q.dn_delay) % P.D) * P.n + (T.post_begin + j0 + col);
-    const int64_t old = writer ? gp(P.acc_v)[ai] : 0;
-    const KNP_GLOBAL uint64_t *cnt = (const KNP_GLOBAL uint64_t *)gp(cnt_base) + (size_t)(t & 1) * Q.dn_oct + T.cbase;
-    const KNP_GLOBAL uint16_t *wb = gp(Q.dn_w) + T.base + (size_t)(j0 + col) * 8;
-    const KNP_GLOBAL uint16_t *wl = SPLIT ? gp(Q.dn_w_lo) + T.base + (size_t)(j0 + col) * 8 : nullptr;
+__device__ __forceinline__ float dense_tile_sum(const ProjDev &Q, const DenseTile &T, const KNP_GLOBAL uint64_t *cnt,
+                                                uint64_t mine0, uint32_t lane, uint32_t &read_oct) {
+    const uint32_t h = lane >> 5, col = lane & 31, n_oct = T.n_oct;
+    const KNP_GLOBAL uint16_t *wb = gp(Q.dn_w) + T.base + (size_t)(T.j0 + col) * 8;
+    const KNP_GLOBAL uint16_t *wl = SPLIT ? gp(Q.dn_w_lo) + T.base + (size_t)(T.j0 + col) * 8 : nullptr;
     const size_t ostride = (size_t)T.npad * 8;
     constexpr uint32_t kNone = 0xFFFFFFFFu;
-    uint32_t read_oct = 0;  // octets whose weights this wave reads
     v16f acc, acc_lo;
 #pragma unroll
     for (int v = 0; v < 16; ++v) acc[v] = acc_lo[v] = 0.f;
     const v8s zero = {0, 0, 0, 0, 0, 0, 0, 0};
     for (uint32_t c0 = 0; c0 < n_oct; c0 += 64) {
         const uint32_t o = c0 + lane;
-        const uint64_t mine = o < n_oct ? cnt[o] : 0ull;
+        const uint64_t mine = c0 == 0 ? mine0 : (o < n_oct ? cnt[o] : 0ull);
         uint64_t mask = SKIP ? __ballot(mine != 0ull) : __ballot(o < n_oct);
         read_oct += (uint32_t)__popcll(mask);
         while (mask) {
@@ -2300,15 +2287,60 @@ __global__ __launch_bounds__(kBlock) void k_dense_expand(const NetDev *__restric
             }
         }
     }
-    if (writer) {
-        const long long v = __float2ll_rn((SPLIT ? acc[0] + acc_lo[0] : acc[0]) * P.scale);
-        gp(P.acc_v)[ai] = old + (Q.dn_sign < 0.f ? -v : v);
+    return SPLIT ? acc[0] + acc_lo[0] : acc[0];
+}
+
+// One wave per 32-neuron tile for a group of MFMA-expanded projections into the same neurons with the same
+// delay (their instances cover the same postsynaptic ranges): every tile entry and first count word of the
+// group is loaded at once, then each projection's sum is taken in turn, converted to fixed point on its own
+// (as it would be alone) and added to the ring cell, which the lane owns (no other launch runs meanwhile) and
+// loaded at the start, so that the total is stored without a further round trip.
+__global__ __launch_bounds__(kBlock) void k_dense_expand(const NetDev *__restrict__ net, DenseGroup grp, uint64_t t) {
+    t = resolve_step(net, t);
+    const ProjDev &Q0 = net->projs[grp.proj[0]];
+    const uint32_t tile = blockIdx.x * (kBlock / 64) + threadIdx.x / 64;
+    if (tile >= Q0.dn_tiles) return;
+    const PopDev &P = net->pops[Q0.post_pop];
+    const uint32_t lane = lane_id(), h = lane >> 5, col = lane & 31;
+    DenseTile T[kDenseGroupMax];
+    uint64_t mine0[kDenseGroupMax];
+#pragma unroll
+    for (uint32_t m = 0; m < kDenseGroupMax; ++m)
+        if (m < grp.n) T[m] = net->projs[grp.proj[m]].dn_tile_info[tile];
+#pragma unroll
+    for (uint32_t m = 0; m < kDenseGroupMax; ++m)
+        if (m < grp.n) {
+            const ProjDev &Q = net->projs[grp.proj[m]];
+            const KNP_GLOBAL uint64_t *cnt =
+                (const KNP_GLOBAL uint64_t *)gp(grp.cnt[m]) + (size_t)(t & 1) * Q.dn_oct + T[m].cbase;
+            mine0[m] = lane < T[m].n_oct ? cnt[lane] : 0ull;
+        }
+    const uint32_t j0 = T[0].j0;
+    const bool writer = h == 0 && j0 + col < T[0].post_count;
+    const size_t ai = (size_t)((t + Q0.dn_delay) % P.D) * P.n + (T[0].post_begin + j0 + col);
+    int64_t total = writer ? gp(P.acc_v)[ai] : 0;
+#pragma unroll
+    for (uint32_t m = 0; m < kDenseGroupMax; ++m) {
+        if (m >= grp.n) break;
+        const ProjDev &Q = net->projs[grp.proj[m]];
+        const KNP_GLOBAL uint64_t *cnt =
+            (const KNP_GLOBAL uint64_t *)gp(grp.cnt[m]) + (size_t)(t & 1) * Q.dn_oct + T[m].cbase;
+        const bool skip = (Q.dense - 1) & kDenseSkipEmpty, split = Q.dn_w_lo != nullptr;
+        uint32_t read_oct = 0;  // octets whose weights this wave reads
+        float sum;
+        if (skip && split) sum = dense_tile_sum<true, true>(Q, T[m], cnt, mine0[m], lane, read_oct);
+        else if (skip) sum = dense_tile_sum<true, false>(Q, T[m], cnt, mine0[m], lane, read_oct);
+        else if (split) sum = dense_tile_sum<false, true>(Q, T[m], cnt, mine0[m], lane, read_oct);
+        else sum = dense_tile_sum<false, false>(Q, T[m], cnt, mine0[m], lane, read_oct);
+        const long long v = __float2ll_rn(sum * P.scale);
+        total += Q.dn_sign < 0.f ? -v : v;
+        // weight bytes read (32 neurons x 8 inputs x 2 bytes per octet, twice for two-part weights): the
+        // traffic of the skipping schedule, counted in the slots the counting workgroups use
+        if (skip && lane == 0 && read_oct)
+            atomicAdd(&Q.dn_part[4 * (blockIdx.x % kDenseCountBlocks) + 2],
+                      (unsigned long long)read_oct * (split ? 1024ull : 512ull));
     }
-    // weight bytes read (32 neurons x 8 inputs x 2 bytes per octet, twice for two-part weights): the traffic
-    // of the skipping schedule, counted in the slots the counting workgroups use
-    if (SKIP && lane == 0 && read_oct)
-        atomicAdd(&Q.dn_part[4 * (blockIdx.x % kDenseCountBlocks) + 2],
-                  (unsigned long long)read_oct * (SPLIT ? 1024ull : 512ull));
+    if (writer) gp(P.acc_v)[ai] = total;
 }
 
 }  // namespace knp_mi355x
