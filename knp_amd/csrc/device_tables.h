@@ -148,6 +148,8 @@ struct ProjDev {
     uint64_t *last_spike_step;   // [n_syn] loaded values, synapse-index order
     uint64_t *row_spike;         // [n_pre] last step each presynaptic row spiked (UINT64_MAX: not since load)
     const uint32_t *col_pre;     // [n_syn] CSC order: presynaptic neuron of col_syn[k]
+    const uint32_t *col_pre0;    // [n_post] first presynaptic index of each column when every column holds
+                                 // consecutive indexes (col_pre[k] = col_pre0[post] + k - col_ptr[post]), else null
     uint8_t *flags;              // bit0 had_hebbian_update_, bit1 has_contributed_
     const uint8_t *delay_syn;    // [n_syn] delay in synapse-index order
     const uint64_t *col_ptr;     // [n_post + 1] CSC by postsynaptic neuron

@@ -1088,6 +1088,18 @@ int init_network(knp_net *net) {
             uint32_t *cpd;
             if ((rc = upload(net, &cpd, cpre.data(), S))) return rc;
             Q.col_pre = cpd;
+            // columns of consecutive presynaptic indexes (all-to-all blocks): the index follows from the position
+            bool consecutive = true;
+            for (uint32_t j = 0; j < n_post && consecutive; ++j)
+                for (uint64_t k = col_ptr[j] + 1; k < col_ptr[j + 1] && consecutive; ++k)
+                    consecutive = cpre[k] == cpre[k - 1] + 1;
+            if (consecutive) {
+                std::vector<uint32_t> c0(std::max(1u, n_post), 0);
+                for (uint32_t j = 0; j < n_post; ++j) c0[j] = col_ptr[j] < col_ptr[j + 1] ? cpre[col_ptr[j]] : 0;
+                uint32_t *c0d;
+                if ((rc = upload(net, &c0d, c0.data(), c0.size()))) return rc;
+                Q.col_pre0 = c0d;
+            }
         }
         if (H.kind == PROJ_RESOURCE) {
             std::vector<float> f(S);

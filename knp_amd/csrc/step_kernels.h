@@ -1571,13 +1571,15 @@ struct SpikingSyn {  // what the spiking pass reads of one synapse
     uint8_t fl;
 };
 
-__device__ __forceinline__ SpikingSyn spiking_load(const ProjDev &Q, uint64_t k, uint32_t status) {
+// pre_off: col_pre0[post] - col_ptr[post] of the synapse's column when the projection's columns hold
+// consecutive presynaptic indexes (the index is then k + pre_off, with no load in front of row_spike).
+__device__ __forceinline__ SpikingSyn spiking_load(const ProjDev &Q, uint64_t k, uint32_t status, int64_t pre_off) {
     SpikingSyn x;
     x.sy = Q.csc_identity ? (uint32_t)k : Q.col_syn[k];
     x.w = Q.W[x.sy];
     x.fl = Q.flags[x.sy];
     x.per = (Q.rule_uniform & kUniPeriod) ? Q.u_period : Q.period[x.sy];
-    x.pre = Q.col_pre[k];
+    x.pre = Q.col_pre0 ? (uint32_t)((int64_t)k + pre_off) : Q.col_pre[k];
     const uint64_t rs = Q.row_spike[x.pre];
     const uint32_t delay = (Q.rule_uniform & kUniDelay) ? Q.u_delay : Q.delay_syn[x.sy];
     x.spk = (rs != UINT64_MAX ? rs : Q.last_spike_step[x.sy]) + delay - 1;
@@ -1622,8 +1624,9 @@ __device__ __forceinline__ bool spiking_update(const ProjDev &Q, uint32_t n, con
 }
 
 __device__ __forceinline__ bool spiking_synapse(const ProjDev &Q, uint32_t n, uint64_t k, uint64_t t,
-                                                uint32_t status, float d_h, float scale, long long &sum, float &du) {
-    const SpikingSyn x = spiking_load(Q, k, status);
+                                                uint32_t status, float d_h, float scale, long long &sum, float &du,
+                                                int64_t pre_off) {
+    const SpikingSyn x = spiking_load(Q, k, status, pre_off);
     du = x.du;
     return spiking_update(Q, n, x, t, status, d_h, scale, sum);
 }
@@ -1689,6 +1692,7 @@ __device__ __forceinline__ void train_candidate_wave(const NetDev *__restrict__ 
         for (uint32_t j = 0; j < R.n_work; ++j) {
             const ProjDev &Q = net->projs[R.work[j]];
             const uint64_t cb = Q.col_ptr[n], ce = Q.col_ptr[n + 1];
+            const int64_t pre_off = Q.col_pre0 ? (int64_t)Q.col_pre0[n] - (int64_t)cb : 0;
             // kTrainUnroll chunks at a time: all their loads are issued before the first chunk's stores (the
             // chunks hold different synapses), then each chunk is updated and its free-resource sum replayed
             // in order
@@ -1698,7 +1702,7 @@ __device__ __forceinline__ void train_candidate_wave(const NetDev *__restrict__ 
 #pragma unroll
                 for (uint32_t u = 0; u < kTrainUnroll; ++u) {
                     act[u] = base + 64 * u + lane < ce;
-                    if (act[u]) x[u] = spiking_load(Q, base + 64 * u + lane, s.status);
+                    if (act[u]) x[u] = spiking_load(Q, base + 64 * u + lane, s.status, pre_off);
                 }
 #pragma unroll
                 for (uint32_t u = 0; u < kTrainUnroll; ++u) {
@@ -1810,6 +1814,7 @@ __global__ __launch_bounds__(kTrainBlock) void k_train(const NetDev *__restrict_
     __shared__ long long s_sum[W];
     __shared__ uint32_t s_cnt[W];
     __shared__ uint64_t s_cb[kMaxWorking], s_ce[kMaxWorking];
+    __shared__ int64_t s_po[kMaxWorking];
     __shared__ uint32_t s_nch[kMaxWorking];
     __shared__ float s_free;
     // chunk g of the candidate: its working projection, CSC position of its first synapse and the column's end
@@ -1832,6 +1837,7 @@ __global__ __launch_bounds__(kTrainBlock) void k_train(const NetDev *__restrict_
             s_cb[threadIdx.x] = Q.col_ptr[n];
             s_ce[threadIdx.x] = Q.col_ptr[n + 1];
             s_nch[threadIdx.x] = (uint32_t)((s_ce[threadIdx.x] - s_cb[threadIdx.x] + 63) / 64);
+            s_po[threadIdx.x] = Q.col_pre0 ? (int64_t)Q.col_pre0[n] - (int64_t)s_cb[threadIdx.x] : 0;
         }
         __syncthreads();
         uint32_t G = 0;
@@ -1849,7 +1855,7 @@ __global__ __launch_bounds__(kTrainBlock) void k_train(const NetDev *__restrict_
                     const uint64_t k = base + lane;
                     const bool act = k < end;
                     const bool contributing =
-                        act && spiking_synapse(net->projs[R.work[j]], n, k, t, s.status, d_h, P.scale, sum, du);
+                        act && spiking_synapse(net->projs[R.work[j]], n, k, t, s.status, d_h, P.scale, sum, du, s_po[j]);
                     am = __ballot(act);
                     nz = __ballot(act && du != 0.f);
                     cm = __ballot(contributing);
