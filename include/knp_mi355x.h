@@ -203,8 +203,11 @@ int knp_mi355x_set_memory_cap(knp_net *net, uint64_t bytes);
  * per-impact 64-bit atomics. When every population is tiled and all tiles fit on the device at once, a
  * multi-step call runs as one persistent launch (ring kept in LDS; each step's spike lists carry a step
  * stamp that the readers wait for).
- * STEPWISE: as AUTO but one launch per step. ATOMIC: atomics everywhere. Set before the first step. */
-enum knp_scatter_mode { KNP_SCATTER_AUTO = 0, KNP_SCATTER_ATOMIC = 1, KNP_SCATTER_STEPWISE = 2 };
+ * A network whose neurons all fit on one workgroup's threads then runs its multi-step calls in one workgroup
+ * instead (rings and the step's spiking rows in LDS: no device-scope stores or polls).
+ * STEPWISE: as AUTO but one launch per step. ATOMIC: atomics everywhere. TILES: as AUTO without the
+ * one-workgroup schedule. Set before the first step. */
+enum knp_scatter_mode { KNP_SCATTER_AUTO = 0, KNP_SCATTER_ATOMIC = 1, KNP_SCATTER_STEPWISE = 2, KNP_SCATTER_TILES = 3 };
 int knp_mi355x_set_scatter_mode(knp_net *net, int mode);
 /* Testing hook (no reference counterpart): in persistent launches, workgroup hold_block (~0u: none) does
  * not publish its spike counts at steps before hold_until (UINT64_MAX: every step), and every wait gives up
@@ -219,6 +222,13 @@ int knp_mi355x_set_wait_test(knp_net *net, uint32_t hold_block, uint32_t spin_li
 int knp_mi355x_set_train_workgroup_limit(knp_net *net, uint32_t max_neurons);
 /* 1 if the population's impacts are summed by tiles (valid after the first step or an explicit init). */
 int knp_mi355x_get_tiled(knp_net *net, uint32_t pop, int *tiled);
+/* How multi-step calls run (no reference counterpart; initializes the network): ATOMIC one launch per step with
+ * atomic impact sums, STEPWISE one or more launches per step (tile kernels and/or atomics; captured step graphs
+ * where enabled), PERSISTENT one cooperative launch of every tile for many steps, SINGLE_WORKGROUP the whole
+ * network in one workgroup for many steps (every neuron on one workgroup's threads, rings in LDS). */
+enum knp_schedule { KNP_SCHEDULE_ATOMIC = 0, KNP_SCHEDULE_STEPWISE = 1, KNP_SCHEDULE_PERSISTENT = 2,
+                    KNP_SCHEDULE_SINGLE_WORKGROUP = 3 };
+int knp_mi355x_get_schedule(knp_net *net, int *schedule);
 
 /* ---- loading (Backend::load_all_populations / load_all_projections, backend.h:137,146;
  *      SingleThreadedCPUBackend::load_populations / load_projections, backend.cpp:144-170) ---------- */

@@ -50,6 +50,8 @@ PROJ_ADDITIVE = 2
 SCATTER_AUTO = 0
 SCATTER_ATOMIC = 1
 SCATTER_STEPWISE = 2
+SCATTER_TILES = 3
+SCHEDULE_ATOMIC, SCHEDULE_STEPWISE, SCHEDULE_PERSISTENT, SCHEDULE_SINGLE_WORKGROUP = 0, 1, 2, 3
 
 # knp_dense_format
 DENSE_FP8_E4M3 = 0

@@ -53,6 +53,7 @@ def _declare(lib):
         "knp_mi355x_set_wait_test": (i32, [P, u32, u32, u64]),
         "knp_mi355x_set_train_workgroup_limit": (i32, [P, u32]),
         "knp_mi355x_get_tiled": (i32, [P, u32, C.POINTER(i32)]),
+        "knp_mi355x_get_schedule": (i32, [P, C.POINTER(i32)]),
         "knp_mi355x_get_tile_counters": (i32, [P, C.POINTER(u64), C.POINTER(u64)]),
         "knp_mi355x_add_blifat_population": (i32, [P, C.POINTER(abi.BLIFATNeuron), u32, pu32]),
         "knp_mi355x_add_resource_blifat_population": (i32, [P, C.POINTER(abi.ResourceBLIFATNeuron), u32, pu32]),
@@ -401,6 +402,12 @@ class MI355XBackend:
         """Testing hook: a persistent workgroup that withholds its spike counts at steps before hold_until, and
         a small bound on every wait, so that a step call must end with an error (see knp_mi355x_set_wait_test)."""
         self._chk(self.lib.knp_mi355x_set_wait_test(self.net, hold_block, spin_limit, hold_until))
+
+    def schedule(self):
+        """How multi-step calls run: abi.SCHEDULE_ATOMIC, _STEPWISE, _PERSISTENT or _SINGLE_WORKGROUP."""
+        x = C.c_int()
+        self._chk(self.lib.knp_mi355x_get_schedule(self.net, C.byref(x)))
+        return x.value
 
     def tiled(self, pop):
         t = C.c_int()

@@ -233,6 +233,19 @@ struct DenseCountSet {
     uint32_t *cnt[kDenseSetMax];
 };
 
+// Small networks (every population tiled, all neurons on the threads of one workgroup): k_small_run runs
+// whole steps in one workgroup with the rings and the step's spiking rows in LDS, so a step needs no
+// device-scope store or poll. Population p's neurons sit on threads [pop_off[p], pop_off[p] + n) (pop_off a
+// multiple of 64, so each wave holds one population's neurons); its ring at LDS cells [ring_off[p], + D * n).
+constexpr uint32_t kSmallThreads = 1024, kSmallMaxPops = 4, kSmallRows = 4096;
+constexpr uint32_t kSmallRingCells = 12 * 1024;  // 96 KiB of 8-byte ring cells
+struct SmallRun {
+    uint32_t n_pops;
+    uint32_t pop_off[kSmallMaxPops];
+    uint32_t ring_off[kSmallMaxPops];
+    uint64_t proj_of_pop[kSmallMaxPops];  // bit q: projection q's presynaptic population is p
+};
+
 // MFMA-expanded projections that one launch of k_dense_expand takes together: they reach the same neurons
 // through instances with the same postsynaptic ranges, with the same delay (one wave per tile sums them all).
 constexpr uint32_t kDenseGroupMax = 2;

@@ -120,6 +120,9 @@ struct knp_net {
     int scatter_mode = KNP_SCATTER_AUTO;
     bool fused = false;     // every population tiled: phase 1 of step t + 1 runs inside the tile kernel of t
     bool persistent = false;  // fused, and every tile fits on the device at once (cooperative launch)
+    bool small = false;       // ... and the whole network fits one workgroup: k_small_run (small_run, small_lds)
+    SmallRun small_run{};
+    uint32_t small_lds = 0;
     bool wide_tiles = false;  // tiles outnumber the workgroups resident at once: the 6-waves tile kernel
     // the MFMA expansions' spike counts were last staged in the device scratch (knp_mi355x_step_noalloc): the
     // per-projection buffers must be cleared before the next knp_mi355x_step uses them
@@ -1465,7 +1468,7 @@ int init_network(knp_net *net) {
     // at most kCachedSrcBlocks source blocks per tile
     net->persistent = false;
     net->run_kind = POP_LEAN;
-    if (net->fused && net->scatter_mode == KNP_SCATTER_AUTO) {
+    if (net->fused && (net->scatter_mode == KNP_SCATTER_AUTO || net->scatter_mode == KNP_SCATTER_TILES)) {
         bool one_kind = true;
         for (uint32_t p = 0; p < n_pops; ++p) one_kind = one_kind && T.pops[p].kind == T.pops[0].kind;
         net->run_kind = one_kind ? T.pops[0].kind : kMixedKinds;
@@ -1476,6 +1479,44 @@ int init_network(knp_net *net) {
         HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, kTileThreads, net->tile_lds));
         const bool src_fit = T.n_src_blocks <= (uint64_t)kCachedSrcBlocks * T.n_tiles;
         net->persistent = coop && src_fit && (uint64_t)per_cu * cus >= T.n_tiles;
+        // a network whose neurons all fit on one workgroup's threads runs its steps inside that workgroup
+        net->small = false;
+        if (net->persistent && net->scatter_mode == KNP_SCATTER_AUTO && one_kind &&
+            (T.pops[0].kind == POP_LEAN || T.pops[0].kind == POP_FULL) &&
+            n_pops <= kSmallMaxPops && net->dense.empty() && n_projs <= (uint32_t)kMaxProjs &&
+            net->test_hold_block == 0xFFFFFFFFu) {
+            SmallRun L{};
+            L.n_pops = n_pops;
+            uint64_t threads = 0, cells = 0, rows = 0;
+            bool ok = true;
+            for (uint32_t p = 0; p < n_pops; ++p) {
+                L.pop_off[p] = (uint32_t)threads;
+                L.ring_off[p] = (uint32_t)cells;
+                threads += (net->pops[p].n + 63) / 64 * 64;
+                cells += (uint64_t)T.pops[p].D * net->pops[p].n;
+            }
+            for (uint32_t q = 0; q < n_projs; ++q) {
+                const HostProj &H = net->projs[q];
+                ok = ok && H.dense_format < 0 && T.projs[q].syn8 && T.projs[q].row_ptr && H.n_pre < (1u << 24);
+                if (H.pre_pop != KNP_NO_POPULATION) {
+                    L.proj_of_pop[H.pre_pop] |= 1ull << q;
+                    rows += net->pops[H.pre_pop].n;
+                }
+            }
+            for (uint32_t s2 = 0; s2 < T.n_srcs; ++s2) rows += net->srcs[s2].n_ch;
+            const uint32_t lds = (uint32_t)(cells * 8);
+            int per_cu1 = 0;
+            if (ok && threads <= kSmallThreads && cells <= kSmallRingCells && rows <= kSmallRows) {
+                const void *fs = T.pops[0].kind == POP_LEAN ? reinterpret_cast<const void *>(&k_small_run<POP_LEAN>)
+                                                            : reinterpret_cast<const void *>(&k_small_run<POP_FULL>);
+                HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu1, fs, kSmallThreads, lds));
+            }
+            if (per_cu1 > 0) {
+                net->small = true;
+                net->small_run = L;
+                net->small_lds = lds;
+            }
+        }
     }
     net->wide_tiles = false;
     if (T.n_tiles) {
@@ -1745,9 +1786,15 @@ int run_steps(knp_net *net, uint64_t n_steps, uint32_t *counts = nullptr, bool t
             const uint32_t nt = net->table.n_tiles, steps = (uint32_t)len;
             uint64_t t0 = t;
             void *args[] = {(void *)&D, (void *)&t0, (void *)&steps};
+            const bool small = net->small && net->test_hold_block == 0xFFFFFFFFu;
             if ((rc = timed_launch(net, KNP_KERNEL_TILE_RUN, [&] {
-                     le = hipLaunchCooperativeKernel(tile_run_kernel(net->run_kind), dim3(nt), dim3(kTileThreads),
-                                                     args, net->tile_lds, net->stream);
+                     if (small && net->run_kind == POP_LEAN)
+                         k_small_run<POP_LEAN><<<1, kSmallThreads, net->small_lds, net->stream>>>(D, net->small_run, t0, steps);
+                     else if (small)
+                         k_small_run<POP_FULL><<<1, kSmallThreads, net->small_lds, net->stream>>>(D, net->small_run, t0, steps);
+                     else
+                         le = hipLaunchCooperativeKernel(tile_run_kernel(net->run_kind), dim3(nt), dim3(kTileThreads),
+                                                         args, net->tile_lds, net->stream);
                  })))
                 return rc;
             if (le != hipSuccess) return fail(KNP_ERR_HIP, std::string("persistent launch: ") + hipGetErrorString(le));
@@ -2951,7 +2998,7 @@ int knp_mi355x_get_step_graph(knp_net *net, uint32_t *steps, uint64_t *replays) 
 
 int knp_mi355x_set_scatter_mode(knp_net *net, int mode) {
     if (!net) return fail(KNP_ERR_INVALID_ARGUMENT, "null handle");
-    if (mode != KNP_SCATTER_AUTO && mode != KNP_SCATTER_ATOMIC && mode != KNP_SCATTER_STEPWISE)
+    if (mode != KNP_SCATTER_AUTO && mode != KNP_SCATTER_ATOMIC && mode != KNP_SCATTER_STEPWISE && mode != KNP_SCATTER_TILES)
         return fail(KNP_ERR_INVALID_ARGUMENT, "unknown scatter mode");
     if (net->initialized) return fail(KNP_ERR_STATE, "the scatter mode must be set before the first step");
     net->scatter_mode = mode;
@@ -2964,6 +3011,17 @@ int knp_mi355x_get_tiled(knp_net *net, uint32_t pop, int *tiled) {
     if (!tiled) return fail(KNP_ERR_INVALID_ARGUMENT, "null pointer");
     if ((rc = ensure_init(net))) return rc;
     *tiled = net->pops[pop].tiled ? 1 : 0;
+    return KNP_OK;
+}
+
+int knp_mi355x_get_schedule(knp_net *net, int *schedule) {
+    if (!net || !schedule) return fail(KNP_ERR_INVALID_ARGUMENT, "null pointer");
+    int rc;
+    if ((rc = ensure_init(net))) return rc;
+    if (net->persistent && net->small && net->test_hold_block == 0xFFFFFFFFu) *schedule = KNP_SCHEDULE_SINGLE_WORKGROUP;
+    else if (net->persistent) *schedule = KNP_SCHEDULE_PERSISTENT;
+    else if (net->table.n_tiles || net->table.n_vseg_atomic == 0) *schedule = KNP_SCHEDULE_STEPWISE;
+    else *schedule = KNP_SCHEDULE_ATOMIC;
     return KNP_OK;
 }
 

@@ -1016,6 +1016,21 @@ __device__ __forceinline__ bool tile_gather(const NetDev *__restrict__ net, cons
     return true;
 }
 
+// The channel counts of step parity t1 cleared by one thread (k_small_run).
+__device__ __forceinline__ void clear_push_counts_one(const NetDev *__restrict__ net, uint64_t t1) {
+    uint32_t *nc = net->seg_count + (size_t)(t1 & 1) * net->n_seg;
+    uint32_t *nvc = net->vcount + (size_t)(t1 & 1) * net->n_vseg;
+    for (uint32_t k = 0; k < net->n_projs; ++k) {
+        const uint32_t ps = net->projs[k].push_seg;
+        nc[ps] = 0u;
+        net->projs[k].in_list[(size_t)(t1 & 1) * net->list_pstride] = 0u;
+        for (uint32_t j = 0; j < 4; ++j) {
+            const uint32_t vp = gp(net->seg_vpos)[4 * ps + j];
+            if (vp != 0xFFFFFFFFu) nvc[vp] = 0u;
+        }
+    }
+}
+
 // Channel segments of step t + 1 start empty (the host writes the counts of pushed messages before that
 // step); run by one workgroup.
 template <bool COHERENT>
@@ -1380,6 +1395,184 @@ __global__ __launch_bounds__(kTileThreads) void k_tile_run(const NetDev *__restr
     tile_count(sh, my_ev, my_parts, my_cells);
     if (my_rows) atomicAdd(&sh.cnt[3], my_rows);
     tile_flush_counts(net, sh);
+}
+
+// Whole steps t0 .. t0 + n - 1 of a small network in one workgroup (SmallRun). Per step: phase 1 of every
+// neuron (its ring slot in LDS) and every source channel; the spiking rows of every projection into an LDS
+// list; then the rows' records (row_ptr, then the packed syn8 records in CSR order) added with LDS atomics
+// into the rings. Channel messages exist only for t0 (pushed before the launch). The sums are exact integers,
+// so the order of the additions does not matter and the results equal the tile kernels'.
+__device__ __forceinline__ void small_append(uint32_t *rows, uint32_t *n_rows, bool pred, uint32_t value) {
+    const uint64_t mask = __ballot(pred);
+    if (mask == 0ull) return;
+    const uint32_t lane = lane_id();
+    const int leader = __ffsll((unsigned long long)mask) - 1;
+    uint32_t base = 0;
+    if ((int)lane == leader) base = atomicAdd(n_rows, (uint32_t)__popcll(mask));
+    base = __shfl(base, leader);
+    if (pred) rows[base + __popcll(mask & ((1ull << lane) - 1ull))] = value;
+}
+
+struct SmallProj {  // what the record pass needs of a projection, staged in LDS
+    const uint64_t *row_ptr;
+    const uint64_t *syn8;
+    uint32_t n_pre, ring_off, D, n_post;
+};
+
+// Rows (projection << 24 | presynaptic index) rows[0 .. count) of step t into the rings; adds the impacts and
+// rows to ev and n_rows_done. rows: LDS, or a channel list in global memory (then q is fixed).
+template <bool GLOBAL_LIST>
+__device__ __forceinline__ void small_expand(const SmallProj *sp, const uint32_t *list, uint32_t count, uint32_t q_fixed,
+                                             uint64_t t, unsigned long long *ring, uint32_t *rq, uint32_t *rpfx,
+                                             uint64_t *rstart, uint32_t *wsum, uint32_t &ev) {
+    const uint32_t tid = threadIdx.x;
+    for (uint32_t rb = 0; rb < count; rb += kSmallThreads) {
+        const uint32_t row = rb + tid;
+        uint32_t len = 0, q = 0;
+        uint64_t b = 0;
+        if (row < count) {
+            const uint32_t e = GLOBAL_LIST ? gp(list)[row] : list[row];
+            q = GLOBAL_LIST ? q_fixed : e >> 24;
+            const uint32_t r = GLOBAL_LIST ? e : (e & 0xFFFFFFu);
+            if (r < sp[q].n_pre) {
+                b = gp(sp[q].row_ptr)[r];
+                len = (uint32_t)(gp(sp[q].row_ptr)[r + 1] - b);
+            }
+        }
+        uint32_t total;
+        const uint32_t ex = block_excl_scan<kSmallThreads>(len, wsum, &total);
+        rq[tid] = q;
+        rpfx[tid] = ex;
+        rstart[tid] = b;
+        __syncthreads();
+        for (uint32_t k = tid; k < total; k += kSmallThreads) {
+            uint32_t lo = 0, hi = kSmallThreads;  // largest row of the chunk whose prefix is <= k
+            while (hi - lo > 1) {
+                const uint32_t mid = (lo + hi) >> 1;
+                if (rpfx[mid] <= k) lo = mid;
+                else hi = mid;
+            }
+            const SmallProj &P = sp[rq[lo]];
+            const uint64_t rec = gp(P.syn8)[rstart[lo] + (k - rpfx[lo])];
+            const uint32_t post = (uint32_t)(rec >> 41);
+            const uint32_t slot = (uint32_t)((t + ((uint32_t)(rec >> 32) & 0xFFu)) % P.D);
+            const int32_t m = (int32_t)((uint32_t)rec << 7) >> 7;
+            const unsigned long long v = (unsigned long long)(long long)m << (((uint32_t)rec >> 25) & 63u);
+            atomicAdd(&ring[P.ring_off + slot * P.n_post + post], v);
+            ++ev;
+        }
+        __syncthreads();  // rq, rpfx, rstart are free again
+    }
+}
+
+template <uint32_t KIND>
+__global__ __launch_bounds__(kSmallThreads) void k_small_run(const NetDev *__restrict__ net, SmallRun L, uint64_t t0,
+                                                             uint32_t n) {
+    extern __shared__ unsigned long long ring[];
+    __shared__ uint32_t rows[kSmallRows];
+    __shared__ uint32_t rq[kSmallThreads], rpfx[kSmallThreads];
+    __shared__ uint64_t rstart[kSmallThreads];
+    __shared__ uint32_t wsum[kSmallThreads / 64];
+    __shared__ uint32_t n_rows;
+    __shared__ PopDev pops[kSmallMaxPops];
+    __shared__ SmallProj sp[kMaxProjs];
+    const uint32_t tid = threadIdx.x;
+    for (uint32_t p = 0; p < L.n_pops; ++p) lds_copy(&pops[p], &net->pops[p]);
+    if (tid < net->n_projs) {
+        const ProjDev &Q = net->projs[tid];
+        const PopDev &Pp = net->pops[Q.post_pop];
+        sp[tid] = SmallProj{Q.row_ptr, Q.syn8, Q.n_pre, L.ring_off[Q.post_pop], Pp.D, Pp.n};
+    }
+    if (tid == 0) n_rows = 0;
+    __syncthreads();
+    // the rings from HBM
+    for (uint32_t p = 0; p < L.n_pops; ++p) {
+        const PopDev &P = pops[p];
+        for (uint32_t c = tid; c < P.D * P.n; c += kSmallThreads) ring[L.ring_off[p] + c] = (unsigned long long)gp(P.acc_v)[c];
+    }
+    // this thread's neuron: its population (the last one starting at or before the thread) and index
+    uint32_t mp = 0;
+    for (uint32_t p = 1; p < L.n_pops; ++p)
+        if (tid >= L.pop_off[p]) mp = p;
+    const PopDev &MP = pops[mp];
+    const uint32_t mi = tid - L.pop_off[mp];
+    const bool mine = mi < MP.n;
+    float V_keep = 0.f;
+    uint32_t ls_keep = 0;
+    if (KIND == POP_LEAN && mine) {
+        V_keep = gp(MP.V)[mi];
+        ls_keep = gp(MP.last_spike)[mi];
+    }
+    __syncthreads();
+    uint32_t ev = 0, rows_done = 0;
+    for (uint32_t k = 0; k < n; ++k) {
+        const uint64_t t = t0 + k;
+        // phase 1 (blifat_impl.h:31-168): the neuron's exact sum of the impacts arriving at t
+        bool spike = false;
+        {
+            int64_t a = 0;
+            if (mine) {
+                unsigned long long &cell = ring[L.ring_off[mp] + (uint32_t)(t % MP.D) * MP.n + mi];
+                a = (int64_t)cell;
+                cell = 0ull;
+            }
+            bool cand;
+            if (KIND == POP_LEAN)
+                spike = population_update<POP_LEAN, true, true>(net, MP, mi, mine, t, a, cand, &V_keep, &ls_keep);
+            else
+                spike = population_update<POP_FULL, true>(net, MP, mi, mine, t, a, cand);
+            const uint64_t mask = __ballot(spike);
+            if (mi - lane_id() < MP.n) wave_bitmap(mask, mi - lane_id(), MP, t);  // a wave holding neurons
+            uint64_t qs = L.proj_of_pop[mp];
+            while (qs) {  // uniform over the wave: every lane of it belongs to population mp
+                const uint32_t q = (uint32_t)__ffsll((unsigned long long)qs) - 1u;
+                qs &= qs - 1ull;
+                small_append(rows, &n_rows, spike, q << 24 | mi);
+            }
+        }
+        // sources (their channels' spikes are rows of their projections)
+        for (uint32_t s = 0; s < net->n_srcs; ++s) {
+            const SourceDev &S = net->srcs[s];
+            for (uint32_t c0 = 0; c0 < S.n_ch; c0 += kSmallThreads) {
+                const uint32_t c = c0 + tid;
+                const bool sp_ = source_spike(S, c, true, t, 0xFFFFFFFFu);
+                small_append(rows, &n_rows, sp_, S.proj << 24 | (S.first + c));
+            }
+        }
+        __syncthreads();
+        const uint32_t count = n_rows;
+        small_expand<false>(sp, rows, count, 0, t, ring, rq, rpfx, rstart, wsum, ev);
+        rows_done += count;
+        if (k == 0) {
+            // channel messages pushed for t0, then their counts cleared for the steps after the launch
+            for (uint32_t q = 0; q < net->n_projs; ++q) {
+                const uint32_t *lst = net->projs[q].in_list + (size_t)(t0 & 1) * net->list_pstride;
+                const uint32_t cnt = gp(lst)[0];
+                if (cnt == 0) continue;
+                small_expand<true>(sp, lst + 1, cnt, q, t, ring, rq, rpfx, rstart, wsum, ev);
+                rows_done += cnt;
+            }
+            __syncthreads();
+            if (tid == 0) {
+                clear_push_counts_one(net, t0);
+                clear_push_counts_one(net, t0 + 1);
+            }
+        }
+        if (tid == 0) n_rows = 0;
+        __syncthreads();
+    }
+    // the rings and the kept neuron state back to HBM; the counters into the first tile's slots
+    for (uint32_t p = 0; p < L.n_pops; ++p) {
+        const PopDev &P = pops[p];
+        for (uint32_t c = tid; c < P.D * P.n; c += kSmallThreads) gp(P.acc_v)[c] = (int64_t)ring[L.ring_off[p] + c];
+    }
+    if (KIND == POP_LEAN && mine) {
+        gp(MP.V)[mi] = V_keep;
+        gp(MP.last_spike)[mi] = ls_keep;
+    }
+    ev = wave_sum_u32(ev);
+    if (lane_id() == 0 && ev) atomicAdd(&net->tile_part[0], (unsigned long long)ev);
+    if (tid == 0) net->tile_part[1] += rows_done;
 }
 
 // ---- synaptic-resource STDP (blifat_stdp.h:35-172, training/stdp.h:40-186) ---------------------------
