@@ -239,11 +239,18 @@ struct DenseCountSet {
 // multiple of 64, so each wave holds one population's neurons); its ring at LDS cells [ring_off[p], + D * n).
 constexpr uint32_t kSmallThreads = 1024, kSmallMaxPops = 4, kSmallRows = 4096;
 constexpr uint32_t kSmallRingCells = 12 * 1024;  // 96 KiB of 8-byte ring cells
+// Dynamic LDS: the rings (8-byte cells), then every projection's row offsets (row_ptr as 32-bit words from
+// row_base[q], n_pre + 1 each), then the thresholds of single-pattern sources (from thr_base[s]).
+constexpr uint32_t kSmallMaxSrcs = 8;
 struct SmallRun {
     uint32_t n_pops;
     uint32_t pop_off[kSmallMaxPops];
     uint32_t ring_off[kSmallMaxPops];
     uint64_t proj_of_pop[kSmallMaxPops];  // bit q: projection q's presynaptic population is p
+    uint32_t ring_cells;                  // words of the rings (8 bytes each)
+    uint32_t row_words;                   // words of the row offsets (4 bytes each)
+    uint32_t row_base[kMaxProjs];
+    uint32_t thr_base[kSmallMaxSrcs];     // UINT32_MAX: several patterns, thresholds read per step
 };
 
 // MFMA-expanded projections that one launch of k_dense_expand takes together: they reach the same neurons

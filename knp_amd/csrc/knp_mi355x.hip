@@ -1495,16 +1495,32 @@ int init_network(knp_net *net) {
                 threads += (net->pops[p].n + 63) / 64 * 64;
                 cells += (uint64_t)T.pops[p].D * net->pops[p].n;
             }
+            uint64_t row_words = 0, thr_words = 0;
             for (uint32_t q = 0; q < n_projs; ++q) {
                 const HostProj &H = net->projs[q];
-                ok = ok && H.dense_format < 0 && T.projs[q].syn8 && T.projs[q].row_ptr && H.n_pre < (1u << 24);
+                ok = ok && H.dense_format < 0 && T.projs[q].syn8 && T.projs[q].row_ptr && H.n_pre < (1u << 24) &&
+                     T.projs[q].n_syn < (uint64_t(1) << 32);
+                L.row_base[q] = (uint32_t)row_words;
+                row_words += (uint64_t)H.n_pre + 1;
                 if (H.pre_pop != KNP_NO_POPULATION) {
                     L.proj_of_pop[H.pre_pop] |= 1ull << q;
                     rows += net->pops[H.pre_pop].n;
                 }
             }
-            for (uint32_t s2 = 0; s2 < T.n_srcs; ++s2) rows += net->srcs[s2].n_ch;
-            const uint32_t lds = (uint32_t)(cells * 8);
+            ok = ok && T.n_srcs <= kSmallMaxSrcs;
+            for (uint32_t s2 = 0; s2 < T.n_srcs && s2 < kSmallMaxSrcs; ++s2) {
+                rows += net->srcs[s2].n_ch;
+                L.thr_base[s2] = 0xFFFFFFFFu;
+                if (T.srcs[s2].n_patterns == 1) {
+                    L.thr_base[s2] = (uint32_t)thr_words;
+                    thr_words += net->srcs[s2].n_ch;
+                }
+            }
+            L.ring_cells = (uint32_t)cells;
+            L.row_words = (uint32_t)row_words;
+            const uint64_t lds64 = cells * 8 + row_words * 4 + thr_words * 4;
+            ok = ok && lds64 <= 128 * 1024;
+            const uint32_t lds = (uint32_t)lds64;
             int per_cu1 = 0;
             if (ok && threads <= kSmallThreads && cells <= kSmallRingCells && rows <= kSmallRows) {
                 const void *fs = T.pops[0].kind == POP_LEAN ? reinterpret_cast<const void *>(&k_small_run<POP_LEAN>)

@@ -1414,9 +1414,9 @@ __device__ __forceinline__ void small_append(uint32_t *rows, uint32_t *n_rows, b
 }
 
 struct SmallProj {  // what the record pass needs of a projection, staged in LDS
-    const uint64_t *row_ptr;
+    const uint32_t *rowp;  // the projection's row offsets in LDS
     const uint64_t *syn8;
-    uint32_t n_pre, ring_off, D, n_post;
+    uint32_t n_pre, ring_off, D, n_post, slot0;  // slot0: the step's ring slot of its postsynaptic population
 };
 
 // Rows (projection << 24 | presynaptic index) rows[0 .. count) of step t into the rings; adds the impacts and
@@ -1435,8 +1435,8 @@ __device__ __forceinline__ void small_expand(const SmallProj *sp, const uint32_t
             q = GLOBAL_LIST ? q_fixed : e >> 24;
             const uint32_t r = GLOBAL_LIST ? e : (e & 0xFFFFFFu);
             if (r < sp[q].n_pre) {
-                b = gp(sp[q].row_ptr)[r];
-                len = (uint32_t)(gp(sp[q].row_ptr)[r + 1] - b);
+                b = sp[q].rowp[r];
+                len = (uint32_t)(sp[q].rowp[r + 1] - b);
             }
         }
         uint32_t total;
@@ -1445,8 +1445,9 @@ __device__ __forceinline__ void small_expand(const SmallProj *sp, const uint32_t
         rpfx[tid] = ex;
         rstart[tid] = b;
         __syncthreads();
+        const uint32_t rows_here = min(count - rb, kSmallThreads);
         for (uint32_t k = tid; k < total; k += kSmallThreads) {
-            uint32_t lo = 0, hi = kSmallThreads;  // largest row of the chunk whose prefix is <= k
+            uint32_t lo = 0, hi = rows_here;  // largest row of the chunk whose prefix is <= k
             while (hi - lo > 1) {
                 const uint32_t mid = (lo + hi) >> 1;
                 if (rpfx[mid] <= k) lo = mid;
@@ -1455,7 +1456,8 @@ __device__ __forceinline__ void small_expand(const SmallProj *sp, const uint32_t
             const SmallProj &P = sp[rq[lo]];
             const uint64_t rec = gp(P.syn8)[rstart[lo] + (k - rpfx[lo])];
             const uint32_t post = (uint32_t)(rec >> 41);
-            const uint32_t slot = (uint32_t)((t + ((uint32_t)(rec >> 32) & 0xFFu)) % P.D);
+            uint32_t slot = P.slot0 + ((uint32_t)(rec >> 32) & 0xFFu);
+            if (slot >= P.D) slot -= P.D;
             const int32_t m = (int32_t)((uint32_t)rec << 7) >> 7;
             const unsigned long long v = (unsigned long long)(long long)m << (((uint32_t)rec >> 25) & 63u);
             atomicAdd(&ring[P.ring_off + slot * P.n_post + post], v);
@@ -1469,6 +1471,8 @@ template <uint32_t KIND>
 __global__ __launch_bounds__(kSmallThreads) void k_small_run(const NetDev *__restrict__ net, SmallRun L, uint64_t t0,
                                                              uint32_t n) {
     extern __shared__ unsigned long long ring[];
+    uint32_t *const rowp = reinterpret_cast<uint32_t *>(ring + L.ring_cells);
+    uint32_t *const thr_lds = rowp + L.row_words;
     __shared__ uint32_t rows[kSmallRows];
     __shared__ uint32_t rq[kSmallThreads], rpfx[kSmallThreads];
     __shared__ uint64_t rstart[kSmallThreads];
@@ -1481,7 +1485,17 @@ __global__ __launch_bounds__(kSmallThreads) void k_small_run(const NetDev *__res
     if (tid < net->n_projs) {
         const ProjDev &Q = net->projs[tid];
         const PopDev &Pp = net->pops[Q.post_pop];
-        sp[tid] = SmallProj{Q.row_ptr, Q.syn8, Q.n_pre, L.ring_off[Q.post_pop], Pp.D, Pp.n};
+        sp[tid] = SmallProj{rowp + L.row_base[tid], Q.syn8, Q.n_pre, L.ring_off[Q.post_pop], Pp.D, Pp.n, 0u};
+    }
+    // the row offsets and single-pattern thresholds into LDS
+    for (uint32_t q = 0; q < net->n_projs; ++q) {
+        const ProjDev &Q = net->projs[q];
+        for (uint32_t r = tid; r <= Q.n_pre; r += kSmallThreads) rowp[L.row_base[q] + r] = (uint32_t)gp(Q.row_ptr)[r];
+    }
+    for (uint32_t s = 0; s < net->n_srcs; ++s) {
+        if (L.thr_base[s] == 0xFFFFFFFFu) continue;
+        const SourceDev &S = net->srcs[s];
+        for (uint32_t c = tid; c < S.n_ch; c += kSmallThreads) thr_lds[L.thr_base[s] + c] = gp(S.thr)[c];
     }
     if (tid == 0) n_rows = 0;
     __syncthreads();
@@ -1507,6 +1521,7 @@ __global__ __launch_bounds__(kSmallThreads) void k_small_run(const NetDev *__res
     uint32_t ev = 0, rows_done = 0;
     for (uint32_t k = 0; k < n; ++k) {
         const uint64_t t = t0 + k;
+        if (tid < net->n_projs) sp[tid].slot0 = (uint32_t)(t % sp[tid].D);  // read after the step's first barrier
         // phase 1 (blifat_impl.h:31-168): the neuron's exact sum of the impacts arriving at t
         bool spike = false;
         {
@@ -1535,7 +1550,8 @@ __global__ __launch_bounds__(kSmallThreads) void k_small_run(const NetDev *__res
             const SourceDev &S = net->srcs[s];
             for (uint32_t c0 = 0; c0 < S.n_ch; c0 += kSmallThreads) {
                 const uint32_t c = c0 + tid;
-                const bool sp_ = source_spike(S, c, true, t, 0xFFFFFFFFu);
+                const uint32_t thr = (L.thr_base[s] != 0xFFFFFFFFu && c < S.n_ch) ? thr_lds[L.thr_base[s] + c] : 0xFFFFFFFFu;
+                const bool sp_ = source_spike(S, c, true, t, thr);
                 small_append(rows, &n_rows, sp_, S.proj << 24 | (S.first + c));
             }
         }
