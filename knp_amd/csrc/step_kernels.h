@@ -481,13 +481,19 @@ __device__ __forceinline__ uint32_t population_step(const NetDev *__restrict__ n
 // Does channel c of source S fire at step t (counter-based Bernoulli draw against the channel's threshold
 // for the current pattern)? thr_pre: the threshold if the caller holds it (single-pattern sources), else
 // UINT32_MAX and it is loaded.
-__device__ __forceinline__ bool source_spike(const SourceDev &S, uint32_t c, bool active, uint64_t t,
-                                             uint32_t thr_pre) {
-    const uint32_t phase = (uint32_t)(t % S.steps_per_pattern);
+// The same with the step's phase (t mod steps_per_pattern) and pattern ((t / steps_per_pattern) mod
+// n_patterns) given by a caller that keeps them from step to step.
+__device__ __forceinline__ bool source_spike_at(const SourceDev &S, uint32_t c, bool active, uint64_t t,
+                                                uint32_t phase, uint32_t pattern, uint32_t thr_pre) {
     if (!active || c >= S.n_ch || phase < S.lo || phase > S.hi) return false;
-    const uint32_t pattern = (uint32_t)((t / S.steps_per_pattern) % S.n_patterns);
     const uint32_t thr = thr_pre != 0xFFFFFFFFu ? thr_pre : gp(S.thr)[(size_t)pattern * S.n_ch + c];
     return bernoulli_hash(S.seed, t, S.hash_offset + c) < thr;
+}
+
+__device__ __forceinline__ bool source_spike(const SourceDev &S, uint32_t c, bool active, uint64_t t,
+                                             uint32_t thr_pre) {
+    return source_spike_at(S, c, active, t, (uint32_t)(t % S.steps_per_pattern),
+                           (uint32_t)((t / S.steps_per_pattern) % S.n_patterns), thr_pre);
 }
 
 // thr_pre as in source_spike; vp_pre as in block_segment_write.
@@ -1419,51 +1425,40 @@ struct SmallProj {  // what the record pass needs of a projection, staged in LDS
     uint32_t n_pre, ring_off, D, n_post, slot0;  // slot0: the step's ring slot of its postsynaptic population
 };
 
-// Rows (projection << 24 | presynaptic index) rows[0 .. count) of step t into the rings; adds the impacts and
-// rows to ev and n_rows_done. rows: LDS, or a channel list in global memory (then q is fixed).
+// Rows (projection << 24 | presynaptic index) rows[0 .. count) of step t into the rings, a wave per row (the
+// waves take rows in turn): the row's records are contiguous, so its lanes load 64 of them at a time, two
+// such loads in flight, and add them with LDS atomics. Adds the impacts to ev. rows: LDS, or a channel list
+// in global memory (then q is q_fixed). No barrier inside: the caller synchronizes.
 template <bool GLOBAL_LIST>
 __device__ __forceinline__ void small_expand(const SmallProj *sp, const uint32_t *list, uint32_t count, uint32_t q_fixed,
-                                             uint64_t t, unsigned long long *ring, uint32_t *rq, uint32_t *rpfx,
-                                             uint64_t *rstart, uint32_t *wsum, uint32_t &ev) {
-    const uint32_t tid = threadIdx.x;
-    for (uint32_t rb = 0; rb < count; rb += kSmallThreads) {
-        const uint32_t row = rb + tid;
-        uint32_t len = 0, q = 0;
-        uint64_t b = 0;
-        if (row < count) {
-            const uint32_t e = GLOBAL_LIST ? gp(list)[row] : list[row];
-            q = GLOBAL_LIST ? q_fixed : e >> 24;
-            const uint32_t r = GLOBAL_LIST ? e : (e & 0xFFFFFFu);
-            if (r < sp[q].n_pre) {
-                b = sp[q].rowp[r];
-                len = (uint32_t)(sp[q].rowp[r + 1] - b);
+                                             unsigned long long *ring, uint32_t &ev) {
+    const uint32_t lane = lane_id(), w = threadIdx.x / 64;
+    constexpr uint32_t W = kSmallThreads / 64;
+    for (uint32_t row = w; row < count; row += W) {
+        const uint32_t e = GLOBAL_LIST ? gp(list)[row] : list[row];
+        const uint32_t q = GLOBAL_LIST ? q_fixed : e >> 24;
+        const uint32_t r = GLOBAL_LIST ? e : (e & 0xFFFFFFu);
+        const SmallProj &P = sp[q];
+        if (r >= P.n_pre) continue;
+        const uint32_t b = P.rowp[r], end = P.rowp[r + 1];
+        const KNP_GLOBAL uint64_t *syn8 = gp(P.syn8);
+        for (uint32_t k0 = b + lane; k0 < end; k0 += 128) {
+            const uint32_t k1 = k0 + 64;
+            const uint64_t rec0 = syn8[k0];
+            const uint64_t rec1 = k1 < end ? syn8[k1] : 0ull;
+#pragma unroll
+            for (uint32_t u = 0; u < 2; ++u) {
+                if (u == 1 && k1 >= end) break;
+                const uint64_t rec = u ? rec1 : rec0;
+                const uint32_t post = (uint32_t)(rec >> 41);
+                uint32_t slot = P.slot0 + ((uint32_t)(rec >> 32) & 0xFFu);
+                if (slot >= P.D) slot -= P.D;
+                const int32_t m = (int32_t)((uint32_t)rec << 7) >> 7;
+                const unsigned long long v = (unsigned long long)(long long)m << (((uint32_t)rec >> 25) & 63u);
+                atomicAdd(&ring[P.ring_off + slot * P.n_post + post], v);
+                ++ev;
             }
         }
-        uint32_t total;
-        const uint32_t ex = block_excl_scan<kSmallThreads>(len, wsum, &total);
-        rq[tid] = q;
-        rpfx[tid] = ex;
-        rstart[tid] = b;
-        __syncthreads();
-        const uint32_t rows_here = min(count - rb, kSmallThreads);
-        for (uint32_t k = tid; k < total; k += kSmallThreads) {
-            uint32_t lo = 0, hi = rows_here;  // largest row of the chunk whose prefix is <= k
-            while (hi - lo > 1) {
-                const uint32_t mid = (lo + hi) >> 1;
-                if (rpfx[mid] <= k) lo = mid;
-                else hi = mid;
-            }
-            const SmallProj &P = sp[rq[lo]];
-            const uint64_t rec = gp(P.syn8)[rstart[lo] + (k - rpfx[lo])];
-            const uint32_t post = (uint32_t)(rec >> 41);
-            uint32_t slot = P.slot0 + ((uint32_t)(rec >> 32) & 0xFFu);
-            if (slot >= P.D) slot -= P.D;
-            const int32_t m = (int32_t)((uint32_t)rec << 7) >> 7;
-            const unsigned long long v = (unsigned long long)(long long)m << (((uint32_t)rec >> 25) & 63u);
-            atomicAdd(&ring[P.ring_off + slot * P.n_post + post], v);
-            ++ev;
-        }
-        __syncthreads();  // rq, rpfx, rstart are free again
     }
 }
 
@@ -1474,10 +1469,7 @@ __global__ __launch_bounds__(kSmallThreads) void k_small_run(const NetDev *__res
     uint32_t *const rowp = reinterpret_cast<uint32_t *>(ring + L.ring_cells);
     uint32_t *const thr_lds = rowp + L.row_words;
     __shared__ uint32_t rows[kSmallRows];
-    __shared__ uint32_t rq[kSmallThreads], rpfx[kSmallThreads];
-    __shared__ uint64_t rstart[kSmallThreads];
-    __shared__ uint32_t wsum[kSmallThreads / 64];
-    __shared__ uint32_t n_rows;
+    __shared__ uint32_t n_rows_buf[2];  // the rows of step k are counted in n_rows_buf[k & 1]
     __shared__ PopDev pops[kSmallMaxPops];
     __shared__ SmallProj sp[kMaxProjs];
     const uint32_t tid = threadIdx.x;
@@ -1497,7 +1489,7 @@ __global__ __launch_bounds__(kSmallThreads) void k_small_run(const NetDev *__res
         const SourceDev &S = net->srcs[s];
         for (uint32_t c = tid; c < S.n_ch; c += kSmallThreads) thr_lds[L.thr_base[s] + c] = gp(S.thr)[c];
     }
-    if (tid == 0) n_rows = 0;
+    if (tid == 0) n_rows_buf[0] = n_rows_buf[1] = 0;
     __syncthreads();
     // the rings from HBM
     for (uint32_t p = 0; p < L.n_pops; ++p) {
@@ -1517,17 +1509,31 @@ __global__ __launch_bounds__(kSmallThreads) void k_small_run(const NetDev *__res
         V_keep = gp(MP.V)[mi];
         ls_keep = gp(MP.last_spike)[mi];
     }
+    __shared__ SourceDev srcs[kSmallMaxSrcs];
+    const uint32_t n_srcs = net->n_srcs;
+    for (uint32_t s = 0; s < n_srcs; ++s) lds_copy(&srcs[s], &net->srcs[s]);
     __syncthreads();
+    // the step's ring slots and source phases, kept from step to step (no 64-bit division per step)
+    uint32_t my_slot = (uint32_t)(t0 % MP.D);
+    uint32_t proj_slot = tid < net->n_projs ? (uint32_t)(t0 % sp[tid].D) : 0u;
+    uint32_t phase[kSmallMaxSrcs], pattern[kSmallMaxSrcs];
+#pragma unroll
+    for (uint32_t s = 0; s < kSmallMaxSrcs; ++s)
+        if (s < n_srcs) {
+            phase[s] = (uint32_t)(t0 % srcs[s].steps_per_pattern);
+            pattern[s] = (uint32_t)((t0 / srcs[s].steps_per_pattern) % srcs[s].n_patterns);
+        }
     uint32_t ev = 0, rows_done = 0;
     for (uint32_t k = 0; k < n; ++k) {
         const uint64_t t = t0 + k;
-        if (tid < net->n_projs) sp[tid].slot0 = (uint32_t)(t % sp[tid].D);  // read after the step's first barrier
+        uint32_t *n_rows = &n_rows_buf[k & 1];
+        if (tid < net->n_projs) sp[tid].slot0 = proj_slot;  // read after the step's first barrier
         // phase 1 (blifat_impl.h:31-168): the neuron's exact sum of the impacts arriving at t
         bool spike = false;
         {
             int64_t a = 0;
             if (mine) {
-                unsigned long long &cell = ring[L.ring_off[mp] + (uint32_t)(t % MP.D) * MP.n + mi];
+                unsigned long long &cell = ring[L.ring_off[mp] + my_slot * MP.n + mi];
                 a = (int64_t)cell;
                 cell = 0ull;
             }
@@ -1542,22 +1548,24 @@ __global__ __launch_bounds__(kSmallThreads) void k_small_run(const NetDev *__res
             while (qs) {  // uniform over the wave: every lane of it belongs to population mp
                 const uint32_t q = (uint32_t)__ffsll((unsigned long long)qs) - 1u;
                 qs &= qs - 1ull;
-                small_append(rows, &n_rows, spike, q << 24 | mi);
+                small_append(rows, n_rows, spike, q << 24 | mi);
             }
         }
         // sources (their channels' spikes are rows of their projections)
-        for (uint32_t s = 0; s < net->n_srcs; ++s) {
-            const SourceDev &S = net->srcs[s];
+#pragma unroll
+        for (uint32_t s = 0; s < kSmallMaxSrcs; ++s) {
+            if (s >= n_srcs) break;
+            const SourceDev &S = srcs[s];
             for (uint32_t c0 = 0; c0 < S.n_ch; c0 += kSmallThreads) {
                 const uint32_t c = c0 + tid;
                 const uint32_t thr = (L.thr_base[s] != 0xFFFFFFFFu && c < S.n_ch) ? thr_lds[L.thr_base[s] + c] : 0xFFFFFFFFu;
-                const bool sp_ = source_spike(S, c, true, t, thr);
-                small_append(rows, &n_rows, sp_, S.proj << 24 | (S.first + c));
+                const bool sp_ = source_spike_at(S, c, true, t, phase[s], pattern[s], thr);
+                small_append(rows, n_rows, sp_, S.proj << 24 | (S.first + c));
             }
         }
         __syncthreads();
-        const uint32_t count = n_rows;
-        small_expand<false>(sp, rows, count, 0, t, ring, rq, rpfx, rstart, wsum, ev);
+        const uint32_t count = *n_rows;
+        small_expand<false>(sp, rows, count, 0, ring, ev);
         rows_done += count;
         if (k == 0) {
             // channel messages pushed for t0, then their counts cleared for the steps after the launch
@@ -1565,7 +1573,7 @@ __global__ __launch_bounds__(kSmallThreads) void k_small_run(const NetDev *__res
                 const uint32_t *lst = net->projs[q].in_list + (size_t)(t0 & 1) * net->list_pstride;
                 const uint32_t cnt = gp(lst)[0];
                 if (cnt == 0) continue;
-                small_expand<true>(sp, lst + 1, cnt, q, t, ring, rq, rpfx, rstart, wsum, ev);
+                small_expand<true>(sp, lst + 1, cnt, q, ring, ev);
                 rows_done += cnt;
             }
             __syncthreads();
@@ -1574,8 +1582,18 @@ __global__ __launch_bounds__(kSmallThreads) void k_small_run(const NetDev *__res
                 clear_push_counts_one(net, t0 + 1);
             }
         }
-        if (tid == 0) n_rows = 0;
-        __syncthreads();
+        // the next step's slots and source phases
+        if (++my_slot == MP.D) my_slot = 0;
+        if (tid < net->n_projs && ++proj_slot == sp[tid].D) proj_slot = 0;
+#pragma unroll
+        for (uint32_t s = 0; s < kSmallMaxSrcs; ++s)
+            if (s < n_srcs && ++phase[s] == srcs[s].steps_per_pattern) {
+                phase[s] = 0;
+                if (++pattern[s] == srcs[s].n_patterns) pattern[s] = 0;
+            }
+        // the next step's row count starts at 0; its last readers (step k - 1) are past two barriers
+        if (tid == 0) n_rows_buf[(k + 1) & 1] = 0;
+        __syncthreads();  // the step's sums are in the rings before phase 1 of the next step reads them
     }
     // the rings and the kept neuron state back to HBM; the counters into the first tile's slots
     for (uint32_t p = 0; p < L.n_pops; ++p) {
