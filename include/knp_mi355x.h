@@ -209,6 +209,23 @@ int knp_mi355x_set_memory_cap(knp_net *net, uint64_t bytes);
  * one-workgroup schedule. Set before the first step. */
 enum knp_scatter_mode { KNP_SCATTER_AUTO = 0, KNP_SCATTER_ATOMIC = 1, KNP_SCATTER_STEPWISE = 2, KNP_SCATTER_TILES = 3 };
 int knp_mi355x_set_scatter_mode(knp_net *net, int mode);
+/* Which reference backend's projection messages the network follows. SINGLE_THREADED (default):
+ * SingleThreadedCPUBackend, whose projections expand every SpikeMessage of the step, each occurrence of an
+ * index once (delta_impl.h:54-114). MULTI_THREADED: MultiThreadedCPUBackend::calculate_projections
+ * (knp/backends/cpu/cpu-multi-threaded-backend/impl/backend.cpp:250-295, delta_impl.h:117-175): a projection
+ * expands only the first SpikeMessage of the step. Its bus routes the population messages of the step and the
+ * channel messages sent before it in one routing, the backend's endpoint first (message_bus_cpu_impl.cpp:41-88),
+ * so the first message is its presynaptic population's if that population spiked, else the first one pushed
+ * with knp_mi355x_push_spikes, else that of its first device source; later pushes of the step are dropped. An
+ * index occurring c times in that message gives one impact of weight float(weight * c) per synapse. The
+ * network may hold only the multi-threaded backend's types (BLIFAT, synaptic-resource BLIFAT and LIF
+ * populations; Delta and synaptic-resource STDP Delta projections, presynaptic ranges below 2^24), no BLOCKING
+ * synapses (that backend's last-impact order follows its threads' completion order) and no MFMA expansions;
+ * impacts are summed on the atomic path. The swapped presynaptic / postsynaptic UIDs of that backend's
+ * SynapticImpactMessages (delta_impl.h:164-169) concern message headers only, which this library does not
+ * build. Set before the first step. */
+enum knp_message_semantics { KNP_SEMANTICS_SINGLE_THREADED = 0, KNP_SEMANTICS_MULTI_THREADED = 1 };
+int knp_mi355x_set_message_semantics(knp_net *net, int semantics);
 /* Testing hook (no reference counterpart): in persistent launches, workgroup hold_block (~0u: none) does
  * not publish its spike counts at steps before hold_until (UINT64_MAX: every step), and every wait gives up
  * after spin_limit polls (at least 64; default 2^20). A launch whose workgroups cannot all proceed then ends,

@@ -51,6 +51,9 @@ SCATTER_AUTO = 0
 SCATTER_ATOMIC = 1
 SCATTER_STEPWISE = 2
 SCATTER_TILES = 3
+# enum knp_message_semantics
+SEMANTICS_SINGLE_THREADED = 0
+SEMANTICS_MULTI_THREADED = 1
 SCHEDULE_ATOMIC, SCHEDULE_STEPWISE, SCHEDULE_PERSISTENT, SCHEDULE_SINGLE_WORKGROUP = 0, 1, 2, 3
 
 # knp_dense_format

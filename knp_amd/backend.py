@@ -50,6 +50,7 @@ def _declare(lib):
         "knp_mi355x_destroy": (i32, [P]),
         "knp_mi355x_set_memory_cap": (i32, [P, u64]),
         "knp_mi355x_set_scatter_mode": (i32, [P, i32]),
+        "knp_mi355x_set_message_semantics": (i32, [P, i32]),
         "knp_mi355x_set_wait_test": (i32, [P, u32, u32, u64]),
         "knp_mi355x_set_train_workgroup_limit": (i32, [P, u32]),
         "knp_mi355x_get_tiled": (i32, [P, u32, C.POINTER(i32)]),
@@ -392,6 +393,12 @@ class MI355XBackend:
         population is tiled), abi.SCATTER_STEPWISE (tiled, one launch per step) or abi.SCATTER_ATOMIC;
         before the first step."""
         self._chk(self.lib.knp_mi355x_set_scatter_mode(self.net, mode))
+
+    def set_message_semantics(self, semantics):
+        """abi.SEMANTICS_SINGLE_THREADED (default) or abi.SEMANTICS_MULTI_THREADED: the multi-threaded CPU
+        backend's projections (only the first SpikeMessage of a step, repeated indexes as weight multiples);
+        before the first step."""
+        self._chk(self.lib.knp_mi355x_set_message_semantics(self.net, semantics))
 
     def set_train_workgroup_limit(self, max_neurons):
         """Synaptic-resource populations of at most max_neurons neurons train with a workgroup per candidate,

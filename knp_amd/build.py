@@ -44,11 +44,13 @@ CXX_FLAGS = ["-std=c++17", "-O2", "-Wall", "-I" + os.path.join(HERE, "host", "in
              "-I" + os.path.join(ROOT, "include")]
 
 
-def build_backend_plugin():
-    """libknp-mi355x-backend.so: the knp::core::Backend implementation exporting create_knp_backend."""
-    out = os.path.join(HERE, "libknp-mi355x-backend.so")
+def build_backend_plugin(name="libknp-mi355x-backend.so", factory="plugin_single_threaded.cpp"):
+    """The knp::core::Backend implementation exporting create_knp_backend: libknp-mi355x-backend.so (beside the
+    single-threaded CPU backend) or libknp-mi355x-mt-backend.so (beside the multi-threaded one)."""
+    out = os.path.join(HERE, name)
     _run([CXX, *CXX_FLAGS, "-fPIC", "-shared", os.path.join(HERE, "host", "src", "mi355x_backend.cpp"),
-          "-L" + HERE, "-lknp_mi355x", "-Wl,-rpath,$ORIGIN", "-o", out + ".tmp"])
+          os.path.join(HERE, "host", "src", factory), "-L" + HERE, "-lknp_mi355x", "-Wl,-rpath,$ORIGIN",
+          "-o", out + ".tmp"])
     os.replace(out + ".tmp", out)
     return out
 
@@ -76,7 +78,9 @@ def build_float_repeat_test():
 
 
 def build_all():
-    return [build_library(), build_backend_plugin(), build_cpp_tests(), build_float_repeat_test()]
+    return [build_library(), build_backend_plugin(),
+            build_backend_plugin("libknp-mi355x-mt-backend.so", "plugin_multi_threaded.cpp"), build_cpp_tests(),
+            build_float_repeat_test()]
 
 
 if __name__ == "__main__":

@@ -119,6 +119,12 @@ struct PopDev {
 // no global atomics. Host-pushed channel messages of a projection form one more segment. The scatter walks a
 // "virtual" segment list: for every projection (load order), its channel segments, then the segments of every
 // population whose spike messages it receives, with the message rank of each.
+// Message order of a projection's virtual segments under the multi-threaded backend's semantics: the
+// population's message, then host pushes, then device source si (kMsgKeySource + si); the smallest key with
+// spikes is the step's only message. Pushed entries then carry the index's count above kMsgCountShift.
+constexpr uint32_t kMsgKeyPopulation = 0, kMsgKeyPush = 1, kMsgKeySource = 2;
+constexpr uint32_t kMsgCountShift = 24, kMsgIndexMask = (1u << kMsgCountShift) - 1u;
+
 struct RowSource {
     uint32_t proj;
     uint32_t rank;      // 0 = channel messages, 1 + population index for population messages
@@ -347,6 +353,7 @@ struct NetDev {
     const uint32_t *list_pool; // base of the spike-list pool (both parities)
     const uint32_t *vseg;      // [n_vseg] segment of every virtual segment
     const uint32_t *vrs;       // [n_vseg] row source of every virtual segment
+    const uint32_t *vkey;      // [n_vseg] message key (multi-threaded semantics only, else null)
     const uint32_t *seg_vpos;  // [n_seg][4] virtual positions of every segment (UINT32_MAX = none)
     uint32_t *vcount;          // [2][n_vseg] spikes per virtual segment, indexed by step parity
     unsigned long long *part;  // [2 * scatter grid] per-workgroup impact and row counters

@@ -118,6 +118,7 @@ struct knp_net {
     uint64_t allocated = 0, cap = 0;
     uint32_t scatter_grid = 1;
     int scatter_mode = KNP_SCATTER_AUTO;
+    int semantics = KNP_SEMANTICS_SINGLE_THREADED;  // knp_mi355x_set_message_semantics
     bool fused = false;     // every population tiled: phase 1 of step t + 1 runs inside the tile kernel of t
     bool persistent = false;  // fused, and every tile fits on the device at once (cooperative launch)
     bool small = false;       // ... and the whole network fits one workgroup: k_small_run (small_run, small_lds)
@@ -592,9 +593,35 @@ int dense_blocks(const knp_net *net, const HostProj &H, DenseBlocks &B) {
     return KNP_OK;
 }
 
+// The multi-threaded backend's semantics cover that backend's types only
+// (cpu-multi-threaded-backend/include/knp/backends/cpu-multi-threaded/backend.h:80-88), and no BLOCKING
+// synapses: that backend delivers one projection's impacts in the completion order of its threads' parts.
+int check_multi_threaded(const knp_net *net) {
+    for (const auto &P : net->pops)
+        if (P.model == MODEL_ALTAI) return fail(KNP_ERR_UNSUPPORTED, "Type is not supported.");
+    for (const auto &Q : net->projs) {
+        if (Q.kind == PROJ_ADDITIVE) return fail(KNP_ERR_UNSUPPORTED, "Type is not supported.");
+        if (Q.has_blocking)
+            return fail(KNP_ERR_UNSUPPORTED, "BLOCKING synapses under the multi-threaded message semantics");
+        if (Q.dense_format >= 0)
+            return fail(KNP_ERR_UNSUPPORTED, "MFMA expansions under the multi-threaded message semantics");
+        if (Q.gen_channels && !Q.pending.empty())
+            return fail(KNP_ERR_UNSUPPORTED, "under the multi-threaded message semantics a projection fed by device "
+                                             "sources takes no pushed messages");
+        if (Q.n_pre > kMsgIndexMask + 1u)
+            return fail(KNP_ERR_UNSUPPORTED, "presynaptic ranges above 2^24 under the multi-threaded message semantics");
+    }
+    return KNP_OK;
+}
+
 int init_network(knp_net *net) {
     HIP_TRY(hipSetDevice(net->device));
     const uint32_t n_pops = (uint32_t)net->pops.size(), n_projs = (uint32_t)net->projs.size();
+    const bool mt = net->semantics == KNP_SEMANTICS_MULTI_THREADED;
+    if (mt) {
+        int rc = check_multi_threaded(net);
+        if (rc) return rc;
+    }
     if (n_pops > (uint32_t)kMaxPops) return fail(KNP_ERR_UNSUPPORTED, "too many populations");
     if (n_projs > (uint32_t)kMaxProjs) return fail(KNP_ERR_UNSUPPORTED, "too many projections");
     if (net->srcs.size() > (size_t)kMaxSources) return fail(KNP_ERR_UNSUPPORTED, "too many sources");
@@ -879,7 +906,7 @@ int init_network(knp_net *net) {
     }
     for (uint32_t p = 0; p < n_pops; ++p) {
         HostPop &H = net->pops[p];
-        H.tiled = net->scatter_mode != KNP_SCATTER_ATOMIC && !H.resource && !H.has_cond && !H.has_dop &&
+        H.tiled = net->scatter_mode != KNP_SCATTER_ATOMIC && !mt && !H.resource && !H.has_cond && !H.has_dop &&
                   !H.has_block && H.D <= tile_max_d && H.n > 0 && H.n < (1u << 23);
     }
     for (uint32_t q = 0; q < n_projs; ++q)
@@ -1283,7 +1310,7 @@ int init_network(knp_net *net) {
         // segment can hold: for every projection (load order) its channel segments, then the segments of every
         // population whose spike messages it receives
         struct VDesc {
-            uint32_t seg, rs, q, row_lo, row_hi;
+            uint32_t seg, rs, q, row_lo, row_hi, key;  // key: message order under the multi-threaded semantics
         };
         std::vector<VDesc> vd;
         uint32_t nrs = 0;
@@ -1295,7 +1322,7 @@ int init_network(knp_net *net) {
             seg_list.push_back(Q.in_list);
             if (nrs >= (uint32_t)kMaxRowSources) return fail(KNP_ERR_UNSUPPORTED, "too many row sources");
             T.rows[nrs] = RowSource{q, 0u};
-            vd.push_back(VDesc{Q.push_seg, nrs, q, 0u, H.n_pre});
+            vd.push_back(VDesc{Q.push_seg, nrs, q, 0u, H.n_pre, kMsgKeyPush});
             for (uint32_t si = 0; si < T.n_srcs; ++si) {
                 SourceDev &S = T.srcs[si];
                 if (S.proj != q) continue;
@@ -1303,7 +1330,7 @@ int init_network(knp_net *net) {
                 S.seg_base = (uint32_t)seg_list.size();
                 for (uint32_t c = 0; c < chunks; ++c) {
                     const uint32_t r0 = S.first + c * kBlock, r1 = S.first + std::min((c + 1) * kBlock, S.n_ch);
-                    vd.push_back(VDesc{(uint32_t)seg_list.size(), nrs, q, r0, r1});
+                    vd.push_back(VDesc{(uint32_t)seg_list.size(), nrs, q, r0, r1, kMsgKeySource + si});
                     seg_list.push_back(S.list + (size_t)c * kSegStride);
                 }
             }
@@ -1324,7 +1351,7 @@ int init_network(knp_net *net) {
                 T.rows[nrs] = RowSource{q, 1u + p};
                 for (uint32_t c = 0; c < pop_chunks[p]; ++c)
                     vd.push_back(VDesc{T.pops[p].seg_base + c, nrs, q, c * kBlock,
-                                       std::min((c + 1) * kBlock, T.pops[p].n)});
+                                       std::min((c + 1) * kBlock, T.pops[p].n), kMsgKeyPopulation});
                 ++nrs;
             }
         }
@@ -1361,11 +1388,12 @@ int init_network(knp_net *net) {
             if (net->projs[vd[k].q].dense_format >= 0) return 1;
             return net->pops[net->projs[vd[k].q].post_pop].tiled ? 2 : 0;
         };
-        std::vector<uint32_t> vseg, vrs, vidx(vd.size());
+        std::vector<uint32_t> vseg, vrs, vkey, vidx(vd.size());
         for (int pass = 0; pass < 3; ++pass)
             for (size_t k = 0; k < vd.size(); ++k) {
                 if (vclass(k) != pass) continue;
                 vidx[k] = (uint32_t)vseg.size();
+                vkey.push_back(vd[k].key);
                 ProjDev &Qv = T.projs[vd[k].q];
                 if (pass == 1) {
                     if (Qv.dn_v1 == Qv.dn_v0) Qv.dn_v0 = (uint32_t)vseg.size();
@@ -1425,6 +1453,12 @@ int init_network(knp_net *net) {
         if ((rc = upload(net, &vr, vrs.data(), vrs.size()))) return rc;
         T.vseg = vs;
         T.vrs = vr;
+        T.vkey = nullptr;
+        if (mt) {
+            uint32_t *vk;
+            if ((rc = upload(net, &vk, vkey.data(), vkey.size()))) return rc;
+            T.vkey = vk;
+        }
         if ((rc = dev_alloc(net, (void **)&T.seg_count, (size_t)2 * T.n_seg * 4))) return rc;
         if ((rc = dev_alloc(net, (void **)&T.vcount, (size_t)2 * std::max(1u, T.n_vseg) * 4))) return rc;
         std::vector<uint32_t> vpos((size_t)4 * T.n_seg, 0xFFFFFFFFu);
@@ -1669,6 +1703,18 @@ int deliver_pushes(knp_net *net) {
         auto &L = lists[q];
         for (auto &m : H.pending) L.insert(L.end(), m.begin(), m.end());
         H.pending.clear();
+        if (net->semantics == KNP_SEMANTICS_MULTI_THREADED) {
+            // one entry per index: its count (at most 4, push_spikes checks) above the index bits
+            std::sort(L.begin(), L.end());
+            size_t o = 0;
+            for (size_t i = 0; i < L.size();) {
+                size_t j = i;
+                while (j < L.size() && L[j] == L[i]) ++j;
+                L[o++] = (uint32_t)(j - i) << kMsgCountShift | L[i];
+                i = j;
+            }
+            L.resize(o);
+        }
         ProjDev &Q = net->table.projs[q];
         if (L.size() > Q.in_cap) return fail(KNP_ERR_RANGE, "more channel spikes in one step than the list holds");
         const uint32_t cnt = (uint32_t)L.size();
@@ -2382,6 +2428,14 @@ int knp_mi355x_push_spikes(knp_net *net, uint32_t proj, const uint32_t *idx, uin
     HostProj &H = net->projs[proj];
     for (uint32_t i = 0; i < n; ++i)
         if (idx[i] >= H.n_pre) return fail(KNP_ERR_INVALID_ARGUMENT, "spike index out of the presynaptic range");
+    if (net->semantics == KNP_SEMANTICS_MULTI_THREADED) {
+        // the first channel message of the step would be either this or a device source's, by send order
+        if (H.gen_channels)
+            return fail(KNP_ERR_UNSUPPORTED, "under the multi-threaded message semantics a projection fed by device "
+                                             "sources takes no pushed messages");
+        // the multi-threaded backend expands the first message of a step only (an empty one included)
+        if (!H.pending.empty()) return KNP_OK;
+    }
     // The exact accumulators keep a factor 4 of headroom over one impact per synapse and step: an index may
     // repeat at most 4 times over the messages delivered in one step.
     std::vector<uint32_t> all(idx, idx + n);
@@ -3018,6 +3072,15 @@ int knp_mi355x_set_scatter_mode(knp_net *net, int mode) {
         return fail(KNP_ERR_INVALID_ARGUMENT, "unknown scatter mode");
     if (net->initialized) return fail(KNP_ERR_STATE, "the scatter mode must be set before the first step");
     net->scatter_mode = mode;
+    return KNP_OK;
+}
+
+int knp_mi355x_set_message_semantics(knp_net *net, int semantics) {
+    if (!net) return fail(KNP_ERR_INVALID_ARGUMENT, "null handle");
+    if (semantics != KNP_SEMANTICS_SINGLE_THREADED && semantics != KNP_SEMANTICS_MULTI_THREADED)
+        return fail(KNP_ERR_INVALID_ARGUMENT, "unknown message semantics");
+    if (net->initialized) return fail(KNP_ERR_STATE, "the message semantics must be set before the first step");
+    net->semantics = semantics;
     return KNP_OK;
 }
 

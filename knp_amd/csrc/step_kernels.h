@@ -600,6 +600,19 @@ __global__ __launch_bounds__(kBlock) void k_scatter(const NetDev *__restrict__ n
     const uint32_t *vc = net->vcount + (size_t)(t & 1) * net->n_vseg;
     const uint32_t lane = lane_id(), w = threadIdx.x / 64;
     for (uint32_t v = threadIdx.x; v < nv; v += kBlock) pfx[v] = vc[v];
+    const uint32_t *vkey = net->vkey;
+    if (vkey) {
+        // the multi-threaded backend's semantics: each projection expands only its first message of the step,
+        // the nonempty one of smallest key (every workgroup finds the same)
+        __shared__ uint32_t first_key[kMaxProjs];
+        for (uint32_t q = threadIdx.x; q < net->n_projs; q += kBlock) first_key[q] = 0xFFFFFFFFu;
+        __syncthreads();
+        for (uint32_t v = threadIdx.x; v < nv; v += kBlock)
+            if (pfx[v]) atomicMin(&first_key[net->rows[net->vrs[v]].proj], vkey[v]);
+        __syncthreads();
+        for (uint32_t v = threadIdx.x; v < nv; v += kBlock)
+            if (vkey[v] != first_key[net->rows[net->vrs[v]].proj]) pfx[v] = 0;
+    }
     __syncthreads();
     uint32_t carry = 0;
     for (uint32_t base = 0; base < nv; base += kBlock) {
@@ -634,7 +647,14 @@ __global__ __launch_bounds__(kBlock) void k_scatter(const NetDev *__restrict__ n
         }
         const uint32_t seg = net->vseg[lo];
         const RowSource &R = net->rows[net->vrs[lo]];
-        const uint32_t pre = net->seg_list[seg][(size_t)(t & 1) * net->list_pstride + 1 + row - pfx[lo]];
+        uint32_t pre = net->seg_list[seg][(size_t)(t & 1) * net->list_pstride + 1 + row - pfx[lo]];
+        // pushed entries under the multi-threaded semantics: the index's count in the message, which multiplies
+        // the weight in float (delta_impl.h:144)
+        float mult = 1.f;
+        if (vkey && vkey[lo] == kMsgKeyPush) {
+            mult = (float)(pre >> kMsgCountShift);
+            pre &= kMsgIndexMask;
+        }
         const ProjDev &Q = net->projs[R.proj];
         if (pre >= Q.n_pre) continue;
         const uint64_t b = Q.row_ptr[pre], e = Q.row_ptr[pre + 1];
@@ -663,7 +683,8 @@ __global__ __launch_bounds__(kBlock) void k_scatter(const NetDev *__restrict__ n
                 }
             }
             for (int j = 0; j < 4; ++j)
-                if (ok[j]) impact(P, Q, R, slot0, pre, base + threadIdx.x + (uint64_t)j * kBlock - b, pt[j], d[j], wv[j]);
+                if (ok[j])
+                    impact(P, Q, R, slot0, pre, base + threadIdx.x + (uint64_t)j * kBlock - b, pt[j], d[j], wv[j] * mult);
         }
     }
     if (threadIdx.x == 0) {

@@ -3,7 +3,8 @@
 // same create(), load_populations/load_projections, start/stop_learning, _init and _step, same
 // SpikeMessage traffic on the message bus. The network state lives on the GPU; population spikes are sent
 // on the bus only for populations some other endpoint subscribes to. The library exports the
-// create_knp_backend alias that BackendLoader resolves (backend.cpp:340 of the reference ST backend).
+// create_knp_backend alias that BackendLoader resolves (backend.cpp:340 of the reference ST backend);
+// libknp-mi355x-mt-backend.so exports one that creates the backend with the multi-threaded semantics.
 #pragma once
 #include <memory>
 #include <set>
@@ -22,8 +23,13 @@ public:
     using PopulationVariants = core::AllPopulationsVariant;
     using ProjectionVariants = core::AllProjectionsVariant;
 
+    // Whose projection messages the device network follows (knp_mi355x_set_message_semantics):
+    // SingleThreadedCPUBackend's, or MultiThreadedCPUBackend's (cpu-multi-threaded-backend/impl/backend.cpp:
+    // 250-295), whose supported types the backend then reports (its backend.h:80-88).
+    enum class Semantics { single_threaded, multi_threaded };
+
     static std::shared_ptr<MI355XBackend> create();
-    explicit MI355XBackend(int device = 0);
+    explicit MI355XBackend(int device = 0, Semantics semantics = Semantics::single_threaded);
     ~MI355XBackend() override;
     MI355XBackend(const MI355XBackend &) = delete;
     MI355XBackend &operator=(const MI355XBackend &) = delete;
@@ -65,6 +71,7 @@ private:
     void check(int rc, const char *what) const;
     void use_device(const core::UID &uid);
     int device_ = 0;
+    Semantics semantics_ = Semantics::single_threaded;
     knp_net *net_ = nullptr;
     bool learning_ = false;
     std::vector<PopulationVariants> populations_;

@@ -155,28 +155,35 @@ void MI355XBackend::check(int rc, const char *what) const {
 
 std::shared_ptr<MI355XBackend> MI355XBackend::create() { return std::make_shared<MI355XBackend>(); }
 
-MI355XBackend::MI355XBackend(int device) : device_(device) {}
+MI355XBackend::MI355XBackend(int device, Semantics semantics) : device_(device), semantics_(semantics) {}
 
 MI355XBackend::~MI355XBackend() {
     if (net_) knp_mi355x_destroy(net_);
 }
 
+// The multi-threaded backend runs BLIFATNeuron, SynapticResourceSTDPBLIFATNeuron and LIFNeuron with
+// DeltaSynapse and SynapticResourceSTDPDeltaSynapse (cpu-multi-threaded-backend/include/knp/backends/
+// cpu-multi-threaded/backend.h:80-88): indexes 0, 1, 4 and 0, 2 of the variants.
 std::vector<std::string> MI355XBackend::get_supported_neurons() const {
+    if (semantics_ == Semantics::multi_threaded) return {"BLIFATNeuron", "SynapticResourceSTDPBLIFATNeuron", "LIFNeuron"};
     return {"BLIFATNeuron", "SynapticResourceSTDPBLIFATNeuron", "AltAILIF", "SynapticResourceSTDPAltAILIFNeuron",
             "LIFNeuron"};
 }
 
 std::vector<std::string> MI355XBackend::get_supported_synapses() const {
+    if (semantics_ == Semantics::multi_threaded) return {"DeltaSynapse", "SynapticResourceSTDPDeltaSynapse"};
     return {"DeltaSynapse", "AdditiveSTDPDeltaSynapse", "SynapticResourceSTDPDeltaSynapse"};
 }
 
 std::vector<size_t> MI355XBackend::get_supported_population_indexes() const {
+    if (semantics_ == Semantics::multi_threaded) return {0, 1, 4};
     std::vector<size_t> out(std::variant_size_v<core::AllPopulationsVariant>);
     for (size_t i = 0; i < out.size(); ++i) out[i] = i;
     return out;
 }
 
 std::vector<size_t> MI355XBackend::get_supported_projection_indexes() const {
+    if (semantics_ == Semantics::multi_threaded) return {0, 2};
     std::vector<size_t> out(std::variant_size_v<core::AllProjectionsVariant>);
     for (size_t i = 0; i < out.size(); ++i) out[i] = i;
     return out;
@@ -298,6 +305,8 @@ void MI355XBackend::stop_learning() {
 void MI355XBackend::_init() {
     if (net_) return;
     check(knp_mi355x_create(device_, &net_), "create");
+    if (semantics_ == Semantics::multi_threaded)
+        check(knp_mi355x_set_message_semantics(net_, KNP_SEMANTICS_MULTI_THREADED), "message semantics");
     std::unordered_map<core::UID, uint32_t, core::uid_hash> pop_index;
     for (const auto &v : populations_) {
         uint32_t idx = 0;
@@ -511,15 +520,3 @@ MI355XBackend::network_copy() const {
 }
 
 }  // namespace knp::backends::gpu_mi355x
-
-namespace {
-std::shared_ptr<knp::core::Backend> create_backend() {
-    return std::make_shared<knp::backends::gpu_mi355x::MI355XBackend>();
-}
-}  // namespace
-
-// The factory alias BackendLoader resolves (BOOST_DLL_ALIAS(..., create_knp_backend) in the reference,
-// knp/backends/cpu/cpu-single-threaded-backend/impl/backend.cpp:340).
-extern "C" {
-__attribute__((visibility("default"))) std::shared_ptr<knp::core::Backend> (*create_knp_backend)() = &create_backend;
-}

@@ -175,6 +175,7 @@ typedef struct ko_net {
     uint64_t events;
     int initialized;
     int threads;  /* > 1: the multi-threaded step (OpenMP builds) */
+    int semantics; /* enum knp_message_semantics: whose projection messages the step follows */
     char err[256];
 } ko_net;
 
@@ -250,6 +251,36 @@ const char *ko_last_error(ko_net *net) { return net->err; }
  * OpenMP builds). */
 int ko_set_threads(ko_net *net, int threads) {
     net->threads = threads > 1 ? threads : 1;
+    return 0;
+}
+
+int ko_set_message_semantics(ko_net *net, int semantics) {
+    if (semantics != KNP_SEMANTICS_SINGLE_THREADED && semantics != KNP_SEMANTICS_MULTI_THREADED) return -1;
+    net->semantics = semantics;
+    return 0;
+}
+
+/* Under the multi-threaded backend's semantics the network holds that backend's types only
+ * (cpu-multi-threaded-backend/include/knp/backends/cpu-multi-threaded/backend.h:80-88) and no BLOCKING
+ * synapses (their last-impact order follows that backend's thread completion order). */
+static int check_multi_threaded(ko_net *net) {
+    for (uint32_t k = 0; k < net->n_pops; ++k)
+        if (net->pops[k].altai) {
+            snprintf(net->err, sizeof net->err, "Type is not supported.");
+            return -1;
+        }
+    for (uint32_t q = 0; q < net->n_projs; ++q) {
+        const ko_proj *p = &net->projs[q];
+        if (p->kind == KO_PROJ_ADDITIVE) {
+            snprintf(net->err, sizeof net->err, "Type is not supported.");
+            return -1;
+        }
+        for (uint64_t s = 0; s < p->n; ++s)
+            if (p->type[s] == KNP_BLOCKING) {
+                snprintf(net->err, sizeof net->err, "BLOCKING synapses under the multi-threaded semantics");
+                return -1;
+            }
+    }
     return 0;
 }
 
@@ -956,6 +987,8 @@ static void calculate_projection_mt(ko_net *net, ko_proj *p, ko_spike_msg *msgs,
     net->events += ev;
 }
 
+static void send_due(ko_proj *p, uint64_t step);
+
 static void calculate_projection(ko_net *net, ko_proj *p, ko_spike_msg *msgs, uint32_t n_msgs, uint64_t step) {
     if (p->kind == KO_PROJ_ADDITIVE) additive_init_projection(net, p, msgs, n_msgs);
     if (net->threads > 1 && p->kind == KO_PROJ_DELTA) {
@@ -977,6 +1010,34 @@ static void calculate_projection(ko_net *net, ko_proj *p, ko_spike_msg *msgs, ui
         }
     }
     if (p->kind == KO_PROJ_ADDITIVE) additive_modify_weights(p);
+    send_due(p, step);
+}
+
+/* MultiThreadedCPUBackend::calculate_projections (cpu-multi-threaded-backend/impl/backend.cpp:250-295) over
+ * calculate_projection_multithreaded_impl (delta_impl.h:117-175): only message m, its indexes counted
+ * (convert_spikes, backend.cpp:235-247); every synapse whose presynaptic index occurs c times gives one impact
+ * of weight weight * c, synapses in index order (that backend merges its parts in thread completion order,
+ * which with exact sums and no BLOCKING changes no result). */
+static void calculate_projection_first_message(ko_net *net, ko_proj *p, const ko_spike_msg *m, uint64_t step) {
+    uint32_t *count = (uint32_t *)xcalloc(p->n_pre ? p->n_pre : 1, 4);
+    for (uint32_t i = 0; i < m->n; ++i)
+        if (m->idx[i] < p->n_pre) count[m->idx[i]]++;
+    for (uint64_t s = 0; s < p->n; ++s) {
+        const uint32_t c = count[p->pre[s]];
+        if (!c) continue;
+        if (p->kind == KO_PROJ_RESOURCE) p->rule[s].last_spike_step = step; /* delta_impl.h:137-140 */
+        const uint64_t future_step = (uint64_t)p->delay[s] + step - 1;
+        const ko_impact imp = {s, (wreal)(p->w[s] * (wreal)c), p->type[s], p->pre[s], p->post[s]};
+        msg_push(&p->queue[future_step % p->max_delay], &imp);
+        net->events++;
+    }
+    free(count);
+    send_due(p, step);
+}
+
+/* The projection's message of this step from its delay queue, sent (cpu-library projections.h:76-84;
+ * cpu-multi-threaded-backend/impl/backend.cpp:284-294). */
+static void send_due(ko_proj *p, uint64_t step) {
     ko_msg *out = &p->queue[step % p->max_delay];
     p->has_outbox = 0;
     if (out->exists) {
@@ -1079,6 +1140,7 @@ static void generate_sources(ko_net *net, uint64_t step) {
 }
 
 int ko_step(ko_net *net, uint64_t n_steps) {
+    if (net->semantics == KNP_SEMANTICS_MULTI_THREADED && check_multi_threaded(net)) return -1;
     for (uint64_t it = 0; it < n_steps; ++it) {
         uint64_t step = net->step;
         generate_sources(net, step);
@@ -1103,8 +1165,17 @@ int ko_step(ko_net *net, uint64_t n_steps) {
             ko_spike_msg *all = (ko_spike_msg *)xcalloc(total ? total : 1, sizeof(ko_spike_msg));
             for (uint32_t m = 0; m < p->n_inbox; ++m) all[m] = p->inbox[m];
             for (uint32_t m = 0; m < n_pop_msgs; ++m) all[p->n_inbox + m] = pm[m];
-            /* `all` holds copies of the message headers: additive STDPOnly empties only this view */
-            calculate_projection(net, p, all, total, step);
+            if (net->semantics == KNP_SEMANTICS_MULTI_THREADED) {
+                /* one routing carries the step's population messages and the channel messages sent before
+                 * the step, the backend's endpoint first (message_bus_cpu_impl.cpp:41-88); the projection
+                 * expands only the first (cpu-multi-threaded-backend/impl/backend.cpp:267) */
+                const ko_spike_msg *first = n_pop_msgs ? &pm[0] : p->n_inbox ? &p->inbox[0] : NULL;
+                if (first) calculate_projection_first_message(net, p, first, step);
+                else send_due(p, step);
+            } else {
+                /* `all` holds copies of the message headers: additive STDPOnly empties only this view */
+                calculate_projection(net, p, all, total, step);
+            }
             for (uint32_t m = 0; m < p->n_inbox; ++m) free(p->inbox[m].idx);
             p->n_inbox = 0;
             free(all);

@@ -43,6 +43,7 @@ def _load(policy, path=None):
         "ko_destroy": (None, [P]),
         "ko_last_error": (C.c_char_p, [P]),
         "ko_set_threads": (i32, [P, i32]),
+        "ko_set_message_semantics": (i32, [P, i32]),
         "ko_add_blifat_population": (i32, [P, C.POINTER(abi.BLIFATNeuron), u32]),
         "ko_add_resource_blifat_population": (i32, [P, C.POINTER(abi.ResourceBLIFATNeuron), u32]),
         "ko_add_lif_population": (i32, [P, C.POINTER(abi.LIFNeuron), u32]),
@@ -203,6 +204,11 @@ class Oracle:
     def start_learning(self):
         for q in range(len(self.proj_sizes)):
             self.set_locked(q, False)
+
+    def set_message_semantics(self, semantics):
+        """abi.SEMANTICS_SINGLE_THREADED or abi.SEMANTICS_MULTI_THREADED (the multi-threaded backend's projections:
+        the first message of a step only, repeated indexes as weight multiples)."""
+        self._chk(self.lib.ko_set_message_semantics(self.net, int(semantics)))
 
     # ---- inputs / stepping --------------------------------------------------------------------
     def push_spikes(self, proj, idx):

@@ -5,6 +5,9 @@
 // a LIF neuron trace (lif_impl.h:31-84), an AltAI-LIF known answer (knp/tests/framework/altai_lif_test.cpp),
 // the supported-type names (:362-381), BackendLoader (knp/tests/framework/backend_loader_test.cpp:29-70)
 // and the Backend::start run loop with a stop predicate.
+// The multi-threaded cases load libknp-mi355x-mt-backend.so from the same folder and run the reference's
+// MultiThreadCpuSuite cases (knp/tests/backend/multi_thread_cpu_test.cpp:54-160) and a repeated-index message,
+// which that backend expands as one weight multiple (cpu-library delta_impl.h:144).
 // Usage: mi355x_backend_test <path to libknp-mi355x-backend.so>. Prints one PASS/FAIL line per case.
 #include <algorithm>
 #include <cstdio>
@@ -112,6 +115,38 @@ static void run_resource_network() {
     CHECK(std::get<knp::core::synapse_data>(loop_after[0]).weight_ != 1.0f);
 }
 
+// One message {0, 1, 0, 0} into two default BLIFAT neurons (weight float(0.1), delay 1, threshold 0.300000008):
+// three impacts of float(0.1) sum to 40265319 * 2^-27 < threshold in the single-threaded backend; the
+// multi-threaded one sends float(3 * float(0.1)) = 10066330 * 2^-25 >= threshold. Returns the neurons that
+// fired at step 1.
+static std::vector<uint32_t> repeated_index_spikes(knp::core::Backend &backend) {
+    BLIFATPopulation population{[](size_t) {
+                                    knp::neuron_traits::neuron_parameters<knp::neuron_traits::BLIFATNeuron> n;
+                                    n.activation_threshold_ = 0.3000000080;
+                                    return n;
+                                },
+                                2};
+    DeltaProjection input{UID{false}, population.get_uid(),
+                          [](size_t i) { return DeltaProjection::Synapse{{0.1f, 1, knp::synapse_traits::OutputType::EXCITATORY}, i, i}; },
+                          2};
+    const UID input_uid = input.get_uid();
+    backend.load_all_populations({population});
+    backend.load_all_projections({input});
+    auto endpoint = backend.get_message_bus().create_endpoint();
+    const UID in_channel_uid, out_channel_uid;
+    backend.subscribe<SpikeMessage>(input_uid, {in_channel_uid});
+    endpoint.subscribe<SpikeMessage>(out_channel_uid, {population.get_uid()});
+    endpoint.send_message(SpikeMessage{{in_channel_uid, 0}, {0, 1, 0, 0}});
+    std::vector<uint32_t> fired;
+    for (int step = 0; step < 2; ++step) {
+        backend._step();
+        endpoint.receive_all_messages();
+        for (const auto &m : endpoint.unload_messages<SpikeMessage>(out_channel_uid))
+            if (step == 1) fired.insert(fired.end(), m.neuron_indexes_.begin(), m.neuron_indexes_.end());
+    }
+    return fired;
+}
+
 int main(int argc, char **argv) {
     if (argc < 2) {
         std::fprintf(stderr, "usage: %s <libknp-mi355x-backend.so>\n", argv[0]);
@@ -120,6 +155,19 @@ int main(int argc, char **argv) {
     const std::filesystem::path lib = argv[1];
     // --cpu-only: the cases that need no HIP device (type names, loader resolution)
     const bool cpu_only = argc > 2 && std::string(argv[2]) == "--cpu-only";
+    const std::filesystem::path mt_lib = lib.parent_path() / "libknp-mi355x-mt-backend.so";
+    // NeuronsGettingTest / SynapsesGettingTest (multi_thread_cpu_test.cpp:141-160) through the loader
+    run_case("MultiThreadedTypesGetting", [&] {
+        knp::framework::BackendLoader loader;
+        CHECK(knp::framework::BackendLoader::is_backend(mt_lib));
+        auto backend = loader.load(mt_lib);
+        const auto neurons = backend->get_supported_neurons();
+        const auto synapses = backend->get_supported_synapses();
+        CHECK(neurons.size() <= std::variant_size_v<knp::core::AllPopulationsVariant> && neurons[0] == "BLIFATNeuron");
+        CHECK(synapses.size() <= std::variant_size_v<knp::core::AllProjectionsVariant> && synapses[0] == "DeltaSynapse");
+        CHECK((backend->get_supported_population_indexes() == std::vector<size_t>{0, 1, 4}));
+        CHECK((backend->get_supported_projection_indexes() == std::vector<size_t>{0, 2}));
+    });
     if (cpu_only) {
         run_case("NeuronsAndSynapsesGetting", [] {
             Backend backend;
@@ -331,6 +379,59 @@ int main(int argc, char **argv) {
             threw = true;
         }
         CHECK(threw);
+    });
+
+    // MultiThreadCpuSuite SmallestNetwork and SmallestResourceNetwork (multi_thread_cpu_test.cpp:54-138) through
+    // the multi-threaded plugin
+    run_case("MultiThreadedSmallestNetwork", [&] {
+        knp::framework::BackendLoader loader;
+        auto backend = loader.load(mt_lib);
+        BLIFATPopulation population{[](size_t) { return knp::neuron_traits::neuron_parameters<knp::neuron_traits::BLIFATNeuron>{}; }, 1};
+        DeltaProjection loop{population.get_uid(), population.get_uid(),
+                             [](size_t) { return DeltaProjection::Synapse{{1.1, 6, knp::synapse_traits::OutputType::EXCITATORY}, 0, 0}; },
+                             1};
+        DeltaProjection input{UID{false}, population.get_uid(),
+                              [](size_t) { return DeltaProjection::Synapse{{1.0, 1, knp::synapse_traits::OutputType::EXCITATORY}, 0, 0}; },
+                              1};
+        const UID input_uid = input.get_uid();
+        backend->load_all_populations({population});
+        backend->load_all_projections({input, loop});
+        CHECK(run_smallest(*backend, input_uid, population.get_uid()) == kExpected);
+    });
+
+    run_case("MultiThreadedSmallestResourceNetwork", [&] {
+        using STDPDeltaProjection = knp::core::Projection<knp::synapse_traits::SynapticResourceSTDPDeltaSynapse>;
+        using ResourcePopulation = knp::core::Population<knp::neuron_traits::SynapticResourceSTDPBLIFATNeuron>;
+        knp::framework::BackendLoader loader;
+        auto backend = loader.load(mt_lib);
+        ResourcePopulation population{[](size_t) { return knp::neuron_traits::neuron_parameters<knp::neuron_traits::SynapticResourceSTDPBLIFATNeuron>{}; }, 1};
+        auto synapse = [](uint32_t delay) {
+            return [delay](size_t) {
+                knp::synapse_traits::synapse_parameters<knp::synapse_traits::SynapticResourceSTDPDeltaSynapse> syn;
+                syn.rule_.w_max_ = 2.0;
+                syn.rule_.w_min_ = 1.0;
+                syn.rule_.synaptic_resource_ = 1.0;
+                syn.rule_.dopamine_plasticity_period_ = 5;
+                syn.weight_ = 1.5;
+                syn.delay_ = delay;
+                syn.output_type_ = knp::synapse_traits::OutputType::EXCITATORY;
+                return STDPDeltaProjection::Synapse{syn, 0, 0};
+            };
+        };
+        STDPDeltaProjection loop{population.get_uid(), population.get_uid(), synapse(6), 1};
+        STDPDeltaProjection input{UID{false}, population.get_uid(), synapse(1), 1};
+        const UID input_uid = input.get_uid();
+        backend->load_all_populations({population});
+        backend->load_all_projections({input, loop});
+        CHECK(run_smallest(*backend, input_uid, population.get_uid()) == kExpected);
+    });
+
+    run_case("MultiThreadedRepeatedIndex", [&] {
+        knp::framework::BackendLoader loader;
+        auto mt = loader.load(mt_lib);
+        CHECK((repeated_index_spikes(*mt) == std::vector<uint32_t>{0}));
+        auto st = loader.load(lib);
+        CHECK(repeated_index_spikes(*st).empty());
     });
 
     run_case("StartWithStopPredicate", [&] {

@@ -20,7 +20,8 @@ def run(*extra, timeout=120):
 def test_cpp_cpu_cases():
     r = run("--cpu-only")
     assert r.returncode == 0, r.stdout
-    assert r.stdout.count("PASS") == 2
+    assert r.stdout.count("PASS") == 3
+    assert "PASS MultiThreadedTypesGetting" in r.stdout
 
 
 @pytest.mark.gpu
@@ -29,5 +30,6 @@ def test_cpp_backend_cases():
     assert r.returncode == 0, r.stdout
     for name in ("SmallestNetwork", "AdditiveSTDPBLIFATNetwork", "ResourceSTDPBLIFATNetwork",
                  "NeuronsAndSynapsesGetting", "NetworkDataThroughConstBackend", "SelectDevice", "BackendLoader",
-                 "StartWithStopPredicate"):
+                 "StartWithStopPredicate", "MultiThreadedTypesGetting", "MultiThreadedSmallestNetwork",
+                 "MultiThreadedSmallestResourceNetwork", "MultiThreadedRepeatedIndex"):
         assert f"PASS {name}" in r.stdout
