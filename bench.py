@@ -75,6 +75,10 @@ def main():
     ap.add_argument("--train-wg-limit", type=int, default=None,
                     help="synaptic-resource training: population size up to which each candidate gets a workgroup "
                          "(beyond it, a wave); default: the library's")
+    ap.add_argument("--no-autotune", dest="autotune", action="store_false",
+                    help="keep the library's default execution choices (default: the step tuner times the "
+                         "candidates from the loaded state, restores it, and keeps the fastest)")
+    ap.add_argument("--tune-steps", type=int, default=256, help="steps per step-tuner trial")
     ap.add_argument("--scatter", choices=["auto", "stepwise", "atomic"], default="auto",
                     help="impact summation: tiled populations and persistent launches where eligible (auto), "
                          "tiled with one launch per step (stepwise), or atomics everywhere")
@@ -115,6 +119,8 @@ def run_c1(args):
     gpu._init()
     gpu.synchronize()
     log(f"uploaded in {time.perf_counter() - t_up:.1f}s on {device_name(0)}")
+    tuning = gpu.autotune(args.tune_steps) if args.autotune else gpu.tuning()
+    log(f"execution choices: {tuning}")
     exactness = {"accumulator_frac_bits": gpu.accumulator_bits(pop), "inexact_weights": gpu.inexact_weights(pop)}
 
     # the compared run: steps 0 .. P-1 recorded (untimed), then the warmup
@@ -313,6 +319,7 @@ def run_c1(args):
         "cpu_baseline_release": cpu_release,
         "parity": parity,
         "exactness": exactness,
+        "tuning": dict(tuning, tuned=bool(args.autotune)),
         "device": device_name(0),
     }
 
@@ -437,6 +444,8 @@ def device_loaders(args):
         spec.load_into(g)
         g._init()
         g.synchronize()
+        g.tuned = g.autotune(args.tune_steps) if args.autotune else g.tuning()
+        log(f"{spec.name}: execution choices {g.tuned}")
         return g
 
     def kernels(g, steps):
@@ -467,7 +476,8 @@ def plain_stdp_run(spec2, load, kernels, agree, warm, steps):
     g2.close()
     log(f"plain fp32 instance: {steps / dt2:.0f} steps/s")
     return ({"net_steps_per_s": steps / dt2, "events_per_s": ev2 / dt2, "steps": steps,
-             "ms_per_step": dt2 / steps * 1e3, "kernels": k2, "path": "fp32 event-driven scatter"}, rec2, pot2)
+             "ms_per_step": dt2 / steps * 1e3, "kernels": k2, "path": "fp32 event-driven scatter",
+             "tuning": g2.tuned}, rec2, pot2)
 
 
 def measure_batched(args):
@@ -494,6 +504,7 @@ def measure_batched(args):
     tl = time.perf_counter()
     g = load(spec3)
     log(f"loaded in {time.perf_counter() - tl:.1f}s")
+    tuning3 = g.tuned
     check = list(range(min(B, args.agree_instances)))
     g.record_spikes(0, agree)
     g.step(agree)
@@ -535,7 +546,7 @@ def measure_batched(args):
                  f"inhibition, dopamine), own seeds and image streams",
         dense=args.dense, split=args.split, batch=B, steps=steps, ms_per_step=dt3 / steps * 1e3,
         events_per_s=ev3 / dt3, ratio_to_plain_fp32=inst_steps_per_s / c2_line["net_steps_per_s"],
-        plain_fp32=c2_line, kernels=k3, dense_roofline=roof,
+        plain_fp32=c2_line, kernels=k3, dense_roofline=roof, tuning=tuning3,
         fp64={"instances": check, "steps": agree, "spike_agreement_by_horizon": batch_agr,
               "plain_fp32_instance0_by_horizon": plain_agr, "per_instance": per})
 
@@ -585,7 +596,7 @@ def other_workload(args):
                            "net_steps_per_s": steps / tx, "sample": f"all {steps} steps, {what}"}
             x.close()
         line.update(metric="synaptic events/s", value=ev / dt, net_steps_per_s=steps / dt, steps=steps,
-                    events=ev, dtype="fp32",
+                    events=ev, dtype="fp32", tuning=g.tuned,
                     parity_knp_exact={"spike_trains_identical": bool(np.array_equal(rec, o.read_spike_record(0))),
                                       "events_equal": ev == o.events,
                                       "potential_rel_err_max": pot_error(pot, o.potentials(0))[0]},
@@ -612,6 +623,7 @@ def other_workload(args):
         log(f"generated c4 ({spec.n_synapses} sparse synapses) in {time.perf_counter() - tg:.1f}s")
         N = spec.pop_sizes[0]
         g = load(spec)
+        tuning4 = g.tuned
         g.step(args.warmup)
         dt, ev = timed_steps(g, steps)
         k4 = kernels(g, min(steps, 1024))
@@ -645,7 +657,7 @@ def other_workload(args):
         unq["instances"] = 16
         line.update(metric="synaptic events/s", value=ev / dt, unit="events/s", dtype="fp32 sparse + fp8 dense",
                     neurons=N, instances=512, steps=steps, ms_per_step=dt / steps * 1e3, net_steps_per_s=steps / dt,
-                    instance_steps_per_s=512 * steps / dt, dense_roofline=mfma, kernels=k4, fp64=fp64,
+                    instance_steps_per_s=512 * steps / dt, dense_roofline=mfma, kernels=k4, fp64=fp64, tuning=tuning4,
                     fp64_unquantized_weights=unq)
     print(json.dumps(line), flush=True)
 

@@ -246,6 +246,19 @@ int knp_mi355x_get_tiled(knp_net *net, uint32_t pop, int *tiled);
 enum knp_schedule { KNP_SCHEDULE_ATOMIC = 0, KNP_SCHEDULE_STEPWISE = 1, KNP_SCHEDULE_PERSISTENT = 2,
                     KNP_SCHEDULE_SINGLE_WORKGROUP = 3 };
 int knp_mi355x_get_schedule(knp_net *net, int *schedule);
+/* Step tuner (no reference counterpart: every choice gives identical results). Picks the execution choices of
+ * multi-step calls by timing trial_steps steps (16..65536) of each candidate from the network's current state,
+ * restoring that state (every device allocation, the step, pending pushes) after each: the schedule among
+ * those the network allows (one workgroup, persistent, stepwise launches), then for stepwise launches the tile
+ * kernel's occupancy (6 or 2 waves per SIMD), the steps per captured step graph (0, 8, 32, 128; for networks
+ * with dense input projections, off or their time block) and the training form (a workgroup or a wave per
+ * candidate). The choice is cached per network shape in the tune cache (KNP_MI355X_TUNE_CACHE, else
+ * ~/.knp_mi355x_tune.txt) and reused without trials. Needs device memory for one copy of the network's device
+ * state. Initializes the network; pending pushes stay pending. */
+int knp_mi355x_autotune(knp_net *net, uint32_t trial_steps);
+/* The execution choices in force: schedule (enum knp_schedule), the 6-waves tile kernel (0/1), steps per
+ * captured step graph, and the population size up to which training runs a workgroup per candidate. */
+int knp_mi355x_get_tuning(knp_net *net, int *schedule, int *wide_tiles, uint32_t *graph_steps, uint32_t *train_limit);
 
 /* ---- loading (Backend::load_all_populations / load_all_projections, backend.h:137,146;
  *      SingleThreadedCPUBackend::load_populations / load_projections, backend.cpp:144-170) ---------- */

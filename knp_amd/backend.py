@@ -51,6 +51,8 @@ def _declare(lib):
         "knp_mi355x_set_memory_cap": (i32, [P, u64]),
         "knp_mi355x_set_scatter_mode": (i32, [P, i32]),
         "knp_mi355x_set_message_semantics": (i32, [P, i32]),
+        "knp_mi355x_autotune": (i32, [P, u32]),
+        "knp_mi355x_get_tuning": (i32, [P, C.POINTER(i32), C.POINTER(i32), pu32, pu32]),
         "knp_mi355x_set_wait_test": (i32, [P, u32, u32, u64]),
         "knp_mi355x_set_train_workgroup_limit": (i32, [P, u32]),
         "knp_mi355x_get_tiled": (i32, [P, u32, C.POINTER(i32)]),
@@ -393,6 +395,18 @@ class MI355XBackend:
         population is tiled), abi.SCATTER_STEPWISE (tiled, one launch per step) or abi.SCATTER_ATOMIC;
         before the first step."""
         self._chk(self.lib.knp_mi355x_set_scatter_mode(self.net, mode))
+
+    def autotune(self, trial_steps=256):
+        """Times the execution choices of multi-step calls from the current state (restored after each trial) and
+        keeps the fastest; returns tuning()."""
+        self._chk(self.lib.knp_mi355x_autotune(self.net, int(trial_steps)))
+        return self.tuning()
+
+    def tuning(self):
+        """{"schedule", "wide_tiles", "graph_steps", "train_workgroup_limit"}: the execution choices in force."""
+        s, w, g, t = C.c_int(), C.c_int(), C.c_uint32(), C.c_uint32()
+        self._chk(self.lib.knp_mi355x_get_tuning(self.net, C.byref(s), C.byref(w), C.byref(g), C.byref(t)))
+        return {"schedule": s.value, "wide_tiles": w.value, "graph_steps": g.value, "train_workgroup_limit": t.value}
 
     def set_message_semantics(self, semantics):
         """abi.SEMANTICS_SINGLE_THREADED (default) or abi.SEMANTICS_MULTI_THREADED: the multi-threaded CPU

@@ -12,6 +12,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <chrono>
 #include <fstream>
 #include <limits>
 #include <mutex>
@@ -115,6 +116,7 @@ struct knp_net {
     NetDev table{};
     NetDev *dev_table = nullptr;
     std::vector<void *> allocations;
+    std::vector<size_t> alloc_bytes;  // size of each allocation (state snapshots of the step tuner)
     uint64_t allocated = 0, cap = 0;
     uint32_t scatter_grid = 1;
     int scatter_mode = KNP_SCATTER_AUTO;
@@ -125,6 +127,8 @@ struct knp_net {
     SmallRun small_run{};
     uint32_t small_lds = 0;
     bool wide_tiles = false;  // tiles outnumber the workgroups resident at once: the 6-waves tile kernel
+    // what init found possible (the step tuner may run a network persistent-eligible stepwise, etc.)
+    bool can_persistent = false, can_small = false;
     // the MFMA expansions' spike counts were last staged in the device scratch (knp_mi355x_step_noalloc): the
     // per-projection buffers must be cleared before the next knp_mi355x_step uses them
     bool counts_in_scratch = false;
@@ -164,6 +168,7 @@ int dev_alloc(knp_net *net, void **out, size_t bytes, bool zero = true) {
         return fail(KNP_ERR_OUT_OF_MEMORY, std::string("hipMalloc: ") + hipGetErrorString(e));
     }
     net->allocations.push_back(*out);
+    net->alloc_bytes.push_back(bytes);
     net->allocated += bytes;
     if (zero) HIP_TRY(hipMemsetAsync(*out, 0, bytes, net->stream));
     return KNP_OK;
@@ -1568,6 +1573,8 @@ int init_network(knp_net *net) {
             }
         }
     }
+    net->can_persistent = net->persistent;
+    net->can_small = net->small;
     net->wide_tiles = false;
     if (T.n_tiles) {
         int cus = 0, per_cu = 0;
@@ -1644,7 +1651,12 @@ int upload_working_sets(knp_net *net) {
 void dev_free(knp_net *net, void *p) {
     if (!p) return;
     auto it = std::find(net->allocations.begin(), net->allocations.end(), p);
-    if (it != net->allocations.end()) net->allocations.erase(it);
+    if (it != net->allocations.end()) {
+        const size_t i = (size_t)(it - net->allocations.begin());
+        net->allocated -= net->alloc_bytes[i];
+        net->alloc_bytes.erase(net->alloc_bytes.begin() + (ptrdiff_t)i);
+        net->allocations.erase(it);
+    }
     (void)hipFree(p);
 }
 
@@ -2027,6 +2039,186 @@ int run_steps(knp_net *net, uint64_t n_steps, uint32_t *counts = nullptr, bool t
         if (net->timing && net->ev_used > 8192) {
             if ((rc = collect_timing(net))) return rc;
         }
+    }
+    return KNP_OK;
+}
+
+// ---- step tuner -------------------------------------------------------------------------------------
+// The execution choices of multi-step calls, all with identical results: the schedule (one workgroup,
+// persistent, or stepwise launches), the stepwise tile kernel's occupancy, steps per captured step graph and
+// the training form.
+struct StepChoice {
+    int schedule;
+    bool wide;
+    uint32_t graph, train_limit;
+};
+
+StepChoice current_choice(const knp_net *net) {
+    StepChoice c{KNP_SCHEDULE_STEPWISE, net->wide_tiles, net->graph_default, net->train_wg_limit};
+    if (net->persistent) c.schedule = net->small ? KNP_SCHEDULE_SINGLE_WORKGROUP : KNP_SCHEDULE_PERSISTENT;
+    return c;
+}
+
+void apply_choice(knp_net *net, const StepChoice &c) {
+    net->persistent = net->can_persistent &&
+                      (c.schedule == KNP_SCHEDULE_PERSISTENT || c.schedule == KNP_SCHEDULE_SINGLE_WORKGROUP);
+    net->small = net->can_small && c.schedule == KNP_SCHEDULE_SINGLE_WORKGROUP;
+    net->wide_tiles = c.wide;
+    net->graph_default = c.graph;
+    net->train_wg_limit = c.train_limit;
+    if (net->graph_exec) (void)hipGraphExecDestroy(net->graph_exec);
+    net->graph_exec = nullptr;
+}
+
+// A copy of every device allocation and of the host state a step call changes.
+struct StateSnapshot {
+    std::vector<void *> copies;
+    uint64_t step = 0;
+    std::vector<bool> ran;
+    bool counts_in_scratch = false;
+    std::vector<std::vector<std::vector<uint32_t>>> pending;
+    ~StateSnapshot() {
+        for (void *p : copies) (void)hipFree(p);
+    }
+};
+
+int take_snapshot(knp_net *net, StateSnapshot &S) {
+    if (net->cap && 2 * net->allocated > net->cap)
+        return fail(KNP_ERR_OUT_OF_MEMORY, "the step tuner needs room for a copy of the network's device state");
+    for (size_t i = 0; i < net->allocations.size(); ++i) {
+        void *c = nullptr;
+        hipError_t e = hipMalloc(&c, net->alloc_bytes[i]);
+        if (e != hipSuccess) return fail(KNP_ERR_OUT_OF_MEMORY, std::string("step tuner snapshot: ") + hipGetErrorString(e));
+        S.copies.push_back(c);
+        HIP_TRY(hipMemcpyAsync(c, net->allocations[i], net->alloc_bytes[i], hipMemcpyDeviceToDevice, net->stream));
+    }
+    HIP_TRY(hipStreamSynchronize(net->stream));
+    S.step = net->step;
+    for (const auto &P : net->pops) S.ran.push_back(P.ran);
+    S.counts_in_scratch = net->counts_in_scratch;
+    for (const auto &Q : net->projs) S.pending.push_back(Q.pending);
+    return KNP_OK;
+}
+
+int restore_snapshot(knp_net *net, const StateSnapshot &S) {
+    for (size_t i = 0; i < S.copies.size(); ++i)
+        HIP_TRY(hipMemcpyAsync(net->allocations[i], S.copies[i], net->alloc_bytes[i], hipMemcpyDeviceToDevice,
+                               net->stream));
+    HIP_TRY(hipStreamSynchronize(net->stream));
+    net->step = S.step;
+    for (size_t p = 0; p < net->pops.size(); ++p) net->pops[p].ran = S.ran[p];
+    net->counts_in_scratch = S.counts_in_scratch;
+    for (size_t q = 0; q < net->projs.size(); ++q) net->projs[q].pending = S.pending[q];
+    return KNP_OK;
+}
+
+std::string tune_key(const knp_net *net) {
+    uint64_t neurons = 0, syn = 0;
+    uint32_t res = 0, mfma = 0;
+    for (const auto &P : net->pops) {
+        neurons += P.n;
+        res += P.resource ? 1u : 0u;
+    }
+    for (const auto &Q : net->projs) {
+        syn += Q.n_syn;
+        mfma += Q.dense_format >= 0 ? 1u : 0u;
+    }
+    char key[256];
+    std::snprintf(key, sizeof key, "step pops=%zu neurons=%llu projs=%zu syn=%llu tiles=%u resource=%u mfma=%u dense=%zu "
+                  "persistent=%d small=%d", net->pops.size(), (unsigned long long)neurons, net->projs.size(),
+                  (unsigned long long)syn, net->table.n_tiles, res, mfma, net->dense.size(), (int)net->can_persistent,
+                  (int)net->can_small);
+    return key;
+}
+
+// Seconds of `steps` steps under choice c from snapshot S (a few untimed steps first: graph capture and first
+// launches), the state restored afterwards.
+int time_choice(knp_net *net, const StateSnapshot &S, const StepChoice &c, uint32_t steps, double *sec) {
+    int rc;
+    if ((rc = restore_snapshot(net, S))) return rc;
+    apply_choice(net, c);
+    if ((rc = run_steps(net, std::max<uint32_t>(2, steps / 8), nullptr, false))) return rc;
+    HIP_TRY(hipStreamSynchronize(net->stream));
+    const auto t0 = std::chrono::steady_clock::now();
+    if ((rc = run_steps(net, steps, nullptr, false))) return rc;
+    HIP_TRY(hipStreamSynchronize(net->stream));
+    *sec = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    return KNP_OK;
+}
+
+int autotune_steps(knp_net *net, uint32_t steps) {
+    const std::string key = tune_key(net), path = tune_cache_path();
+    if (!path.empty()) {
+        std::ifstream in(path);
+        std::string line;
+        while (std::getline(in, line)) {
+            if (line.compare(0, key.size(), key) != 0 || line.size() <= key.size() || line[key.size()] != ' ') continue;
+            int sched = 0, wide = 0;
+            unsigned graph = 0, train = 0;
+            if (std::sscanf(line.c_str() + key.size(), " %d %d %u %u", &sched, &wide, &graph, &train) == 4) {
+                apply_choice(net, StepChoice{sched, wide != 0, graph, train});
+                return KNP_OK;
+            }
+        }
+    }
+    StateSnapshot S;
+    int rc;
+    if ((rc = take_snapshot(net, S))) return rc;
+    const bool timing_was = net->timing;
+    net->timing = false;
+    StepChoice best = current_choice(net);
+    double best_s = 1e30;
+    auto consider = [&](const StepChoice &c) -> int {
+        double sec = 0;
+        int r = time_choice(net, S, c, steps, &sec);
+        if (r) return r;
+        if (sec < best_s) {
+            best_s = sec;
+            best = c;
+        }
+        return KNP_OK;
+    };
+    // 1. the schedule, with the default options
+    std::vector<int> schedules{KNP_SCHEDULE_STEPWISE};
+    if (net->can_persistent) schedules.push_back(KNP_SCHEDULE_PERSISTENT);
+    if (net->can_small) schedules.push_back(KNP_SCHEDULE_SINGLE_WORKGROUP);
+    const StepChoice base = best;
+    for (int sc : schedules) {
+        StepChoice c = base;
+        c.schedule = sc;
+        if ((rc = consider(c))) break;
+    }
+    // 2. the stepwise options, one at a time from the best so far
+    if (!rc && best.schedule == KNP_SCHEDULE_STEPWISE) {
+        if (net->table.n_tiles) {
+            StepChoice c = best;
+            c.wide = !c.wide;
+            rc = consider(c);
+        }
+        const uint32_t graphs[] = {0u, 8u, 32u, 128u};
+        const StepChoice g0 = best;
+        for (uint32_t g : graphs)
+            if (!rc && g != g0.graph && (net->dense.empty() || g <= 1u)) {
+                StepChoice c = g0;
+                c.graph = g;
+                rc = consider(c);
+            }
+        bool resource = false;
+        for (const auto &P : net->pops) resource = resource || (P.resource && P.n <= best.train_limit);
+        if (!rc && resource) {
+            StepChoice c = best;
+            c.train_limit = 0;  // a wave per candidate
+            rc = consider(c);
+        }
+    }
+    net->timing = timing_was;
+    const int rr = restore_snapshot(net, S);
+    apply_choice(net, best);
+    if (rc) return rc;
+    if (rr) return rr;
+    if (!path.empty()) {
+        std::ofstream out(path, std::ios::app);
+        if (out) out << key << " " << best.schedule << " " << (int)best.wide << " " << best.graph << " " << best.train_limit << "\n";
     }
     return KNP_OK;
 }
@@ -3101,6 +3293,27 @@ int knp_mi355x_get_schedule(knp_net *net, int *schedule) {
     else if (net->persistent) *schedule = KNP_SCHEDULE_PERSISTENT;
     else if (net->table.n_tiles || net->table.n_vseg_atomic == 0) *schedule = KNP_SCHEDULE_STEPWISE;
     else *schedule = KNP_SCHEDULE_ATOMIC;
+    return KNP_OK;
+}
+
+int knp_mi355x_autotune(knp_net *net, uint32_t trial_steps) {
+    if (!net) return fail(KNP_ERR_INVALID_ARGUMENT, "null handle");
+    if (trial_steps < 16 || trial_steps > 65536) return fail(KNP_ERR_INVALID_ARGUMENT, "trial_steps must be 16..65536");
+    int rc;
+    if ((rc = ensure_init(net))) return rc;
+    HIP_TRY(hipSetDevice(net->device));
+    return autotune_steps(net, trial_steps);
+}
+
+int knp_mi355x_get_tuning(knp_net *net, int *schedule, int *wide_tiles, uint32_t *graph_steps, uint32_t *train_limit) {
+    if (!net) return fail(KNP_ERR_INVALID_ARGUMENT, "null handle");
+    int rc;
+    if ((rc = ensure_init(net))) return rc;
+    const StepChoice c = current_choice(net);
+    if (schedule) *schedule = c.schedule;
+    if (wide_tiles) *wide_tiles = c.wide ? 1 : 0;
+    if (graph_steps) *graph_steps = c.graph;
+    if (train_limit) *train_limit = c.train_limit;
     return KNP_OK;
 }
 
