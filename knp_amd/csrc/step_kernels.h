@@ -492,8 +492,11 @@ __device__ __forceinline__ bool source_spike_at(const SourceDev &S, uint32_t c, 
 
 __device__ __forceinline__ bool source_spike(const SourceDev &S, uint32_t c, bool active, uint64_t t,
                                              uint32_t thr_pre) {
-    return source_spike_at(S, c, active, t, (uint32_t)(t % S.steps_per_pattern),
-                           (uint32_t)((t / S.steps_per_pattern) % S.n_patterns), thr_pre);
+    // one pattern of one step (a plain Bernoulli channel group) needs no 64-bit division
+    const uint64_t q = S.steps_per_pattern == 1 ? t : t / S.steps_per_pattern;
+    const uint32_t phase = (uint32_t)(t - q * S.steps_per_pattern);
+    const uint32_t pattern = S.n_patterns == 1 ? 0u : (uint32_t)(q % S.n_patterns);
+    return source_spike_at(S, c, active, t, phase, pattern, thr_pre);
 }
 
 // thr_pre as in source_spike; vp_pre as in block_segment_write.
@@ -942,7 +945,7 @@ __device__ __forceinline__ void part_of(const TileProj &Q, uint32_t pre, uint32_
 template <bool COHERENT, bool SPLIT = false>
 __device__ __forceinline__ bool tile_gather(const NetDev *__restrict__ net, const TileCtx &c, TileShared &sh,
                                             uint64_t t, uint32_t &my_ev, uint32_t &my_parts, bool channels = true,
-                                            TilePart *part = nullptr) {
+                                            TilePart *part = nullptr, uint32_t slot_t = 0xFFFFFFFFu) {
     if (SPLIT) *part = TilePart{0u, 0u, 0u};
     const uint32_t tid = threadIdx.x;
     const size_t par_off = (size_t)(t & 1) * net->list_pstride;
@@ -1005,7 +1008,7 @@ __device__ __forceinline__ bool tile_gather(const NetDev *__restrict__ net, cons
     } else {
         __syncthreads();
     }
-    const uint32_t slot0 = (uint32_t)(t % c.D);
+    const uint32_t slot0 = slot_t != 0xFFFFFFFFu ? slot_t : (uint32_t)(t % c.D);  // ring slot of step t
     for (uint32_t rb = 0; rb < total; rb += kGatherRows) {
         // 2. one thread per row: its presynaptic index and projection, and where its part for this tile lies
         const uint32_t row = rb + tid;
@@ -1112,6 +1115,10 @@ struct PhaseCache {
     PopDev pop;
     SourceDev srcd[kCachedSrcBlocks];
     uint32_t thr[kCachedSrcBlocks][kBlock];  // UINT32_MAX: load the threshold (several patterns)
+    // the table fields the publish reads (LDS reads instead of dependent scalar loads from the table)
+    uint32_t *seg_count, *vcount;
+    uint32_t n_seg, n_vseg, hold_block;
+    uint64_t hold_until;
 };
 
 // Word-wise copy of a table entry into LDS by the workgroup (the caller synchronizes).
@@ -1147,7 +1154,15 @@ __device__ __forceinline__ void phase_cache(const NetDev *__restrict__ net, cons
         }
         b0 += nb;
     }
-    if (threadIdx.x == 0) pc.n_src = n < kCachedSrcBlocks ? n : kCachedSrcBlocks;
+    if (threadIdx.x == 0) {
+        pc.n_src = n < kCachedSrcBlocks ? n : kCachedSrcBlocks;
+        pc.seg_count = net->seg_count;
+        pc.vcount = net->vcount;
+        pc.n_seg = net->n_seg;
+        pc.n_vseg = net->n_vseg;
+        pc.hold_block = net->hold_block;
+        pc.hold_until = net->hold_until;
+    }
     __syncthreads();
 }
 
@@ -1159,7 +1174,7 @@ constexpr uint32_t kMixedKinds = 0xFF;
 // 0-255 into seg_a, threads 256-511 into seg_b (null: that half has no candidates), each in ascending order,
 // with one pair of barriers. Returns the rows published (threads 0 and 256).
 template <bool COHERENT>
-__device__ __forceinline__ uint32_t dual_segment_write(const NetDev *__restrict__ net, uint64_t t, bool pred,
+__device__ __forceinline__ uint32_t dual_segment_write(const PhaseCache &pc, uint64_t t, bool pred,
                                                        uint32_t value, uint32_t *seg_a, uint32_t sa,
                                                        const uint32_t *vp_a, uint32_t *seg_b, uint32_t sb,
                                                        const uint32_t *vp_b) {
@@ -1188,12 +1203,12 @@ __device__ __forceinline__ uint32_t dual_segment_write(const NetDev *__restrict_
         const uint32_t *vp = upper ? vp_b : vp_a;
         const size_t par = (size_t)(t & 1);
         if (h == 0) {
-            if (!COHERENT || blockIdx.x != net->hold_block || t >= net->hold_until)  // (a test may have one workgroup withhold)
+            if (!COHERENT || blockIdx.x != pc.hold_block || t >= pc.hold_until)  // (a test may have one workgroup withhold)
                 st_u32<COHERENT>(&seg[0], COHERENT ? total | list_stamp(t) << kStampShift : total);
-            st_u32<COHERENT>(&net->seg_count[par * net->n_seg + sg], total);
+            st_u32<COHERENT>(&pc.seg_count[par * pc.n_seg + sg], total);
             for (uint32_t k = 0; k < 4; ++k) rows += vp[k] != 0xFFFFFFFFu ? total : 0u;
         }
-        if (vp[h] != 0xFFFFFFFFu) st_u32<COHERENT>(&net->vcount[par * net->n_vseg + vp[h]], total);
+        if (vp[h] != 0xFFFFFFFFu) st_u32<COHERENT>(&pc.vcount[par * pc.n_vseg + vp[h]], total);
     }
     // wcnt is next written at the following call, after the caller's barriers in between
     return rows;
@@ -1229,7 +1244,7 @@ __device__ __forceinline__ uint32_t tile_phase1_dual(const NetDev *__restrict__ 
     uint32_t *seg_a = pc.pop.spike_list + par_off + (size_t)c.td.tile * kSegStride;
     uint32_t *seg_b = n_src > 0 ? pc.srcd[0].list + par_off + (size_t)pc.blk[0] * kSegStride : nullptr;
     const uint32_t sb = n_src > 0 ? pc.srcd[0].seg_base + pc.blk[0] : 0u;
-    uint32_t rows = dual_segment_write<true>(net, t1, pred, value, seg_a, pc.pop.seg_base + c.td.tile, pc.pop_vp,
+    uint32_t rows = dual_segment_write<true>(pc, t1, pred, value, seg_a, pc.pop.seg_base + c.td.tile, pc.pop_vp,
                                              seg_b, sb, pc.vp[0]);
     const uint64_t mask = __ballot(pred);
     if (tid < (uint32_t)kBlock) wave_bitmap(mask, c.tile_lo + (tid & ~63u), pc.pop, t1);
@@ -1390,15 +1405,16 @@ __global__ __launch_bounds__(kTileThreads) void k_tile_run(const NetDev *__restr
         }
         my_rows += tile_phase1_dual<KIND>(net, c, pc, t0, a, V_keep, ls_keep);
     }
+    uint32_t slot_t = (uint32_t)(t0 % c.D);  // ring slot of step t, advanced without a division per step
     for (uint32_t k = 0; k < n; ++k) {
         const uint64_t t = t0 + k;
+        const uint32_t s1 = slot_t + 1 == c.D ? 0u : slot_t + 1;  // slot of t + 1
         // the impacts of step t that phase 1 of t + 1 needs (delay 1), then phase 1, then the others of step
         // t (delays 2 and up: later slots), while the other workgroups finish phase 1
         TilePart part;
-        if (!tile_gather<true, true>(net, c, sh, t, my_ev, my_parts, k == 0, &part)) break;
+        if (!tile_gather<true, true>(net, c, sh, t, my_ev, my_parts, k == 0, &part, slot_t)) break;
         if (k == 1 && blockIdx.x == 0) clear_push_counts<true>(net, t0);
         if (k + 1 < n) {
-            const uint32_t s1 = (uint32_t)((t + 1) % c.D);
             int64_t a = 0;
             if (tid < (uint32_t)kBlock) {
                 a = (int64_t)c.ring[s1 * kBlock + tid];
@@ -1406,7 +1422,8 @@ __global__ __launch_bounds__(kTileThreads) void k_tile_run(const NetDev *__restr
             }
             my_rows += tile_phase1_dual<KIND>(net, c, pc, t + 1, a, V_keep, ls_keep);
         }
-        tile_add_parts<false>(c, sh, (uint32_t)(t % c.D), part.b, part.len, part.q, my_ev);
+        tile_add_parts<false>(c, sh, slot_t, part.b, part.len, part.q, my_ev);
+        slot_t = s1;
     }
     // the ring and the kept neuron state back to HBM
     if (keeper) {
