@@ -251,14 +251,16 @@ int knp_mi355x_get_schedule(knp_net *net, int *schedule);
  * restoring that state (every device allocation, the step, pending pushes) after each: the schedule among
  * those the network allows (one workgroup, persistent, stepwise launches), then for stepwise launches the tile
  * kernel's occupancy (6 or 2 waves per SIMD), the steps per captured step graph (0, 8, 32, 128; for networks
- * with dense input projections, off or their time block) and the training form (a workgroup or a wave per
- * candidate). The choice is cached per network shape in the tune cache (KNP_MI355X_TUNE_CACHE, else
+ * with dense input projections, off or their time block), the training form (a workgroup or a wave per
+ * candidate) and the atomic scatter's workgroups (its default, a quarter, a sixteenth, a sixty-fourth). The choice is cached per network shape in the tune cache (KNP_MI355X_TUNE_CACHE, else
  * ~/.knp_mi355x_tune.txt) and reused without trials. Needs device memory for one copy of the network's device
  * state. Initializes the network; pending pushes stay pending. */
 int knp_mi355x_autotune(knp_net *net, uint32_t trial_steps);
 /* The execution choices in force: schedule (enum knp_schedule), the 6-waves tile kernel (0/1), steps per
- * captured step graph, and the population size up to which training runs a workgroup per candidate. */
-int knp_mi355x_get_tuning(knp_net *net, int *schedule, int *wide_tiles, uint32_t *graph_steps, uint32_t *train_limit);
+ * captured step graph, the population size up to which training runs a workgroup per candidate, and the
+ * atomic scatter's workgroups. */
+int knp_mi355x_get_tuning(knp_net *net, int *schedule, int *wide_tiles, uint32_t *graph_steps, uint32_t *train_limit,
+                          uint32_t *scatter_grid);
 
 /* ---- loading (Backend::load_all_populations / load_all_projections, backend.h:137,146;
  *      SingleThreadedCPUBackend::load_populations / load_projections, backend.cpp:144-170) ---------- */

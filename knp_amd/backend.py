@@ -52,7 +52,7 @@ def _declare(lib):
         "knp_mi355x_set_scatter_mode": (i32, [P, i32]),
         "knp_mi355x_set_message_semantics": (i32, [P, i32]),
         "knp_mi355x_autotune": (i32, [P, u32]),
-        "knp_mi355x_get_tuning": (i32, [P, C.POINTER(i32), C.POINTER(i32), pu32, pu32]),
+        "knp_mi355x_get_tuning": (i32, [P, C.POINTER(i32), C.POINTER(i32), pu32, pu32, pu32]),
         "knp_mi355x_set_wait_test": (i32, [P, u32, u32, u64]),
         "knp_mi355x_set_train_workgroup_limit": (i32, [P, u32]),
         "knp_mi355x_get_tiled": (i32, [P, u32, C.POINTER(i32)]),
@@ -403,10 +403,12 @@ class MI355XBackend:
         return self.tuning()
 
     def tuning(self):
-        """{"schedule", "wide_tiles", "graph_steps", "train_workgroup_limit"}: the execution choices in force."""
-        s, w, g, t = C.c_int(), C.c_int(), C.c_uint32(), C.c_uint32()
-        self._chk(self.lib.knp_mi355x_get_tuning(self.net, C.byref(s), C.byref(w), C.byref(g), C.byref(t)))
-        return {"schedule": s.value, "wide_tiles": w.value, "graph_steps": g.value, "train_workgroup_limit": t.value}
+        """{"schedule", "wide_tiles", "graph_steps", "train_workgroup_limit", "scatter_grid"}: the execution
+        choices in force."""
+        s, w, g, t, sg = C.c_int(), C.c_int(), C.c_uint32(), C.c_uint32(), C.c_uint32()
+        self._chk(self.lib.knp_mi355x_get_tuning(self.net, C.byref(s), C.byref(w), C.byref(g), C.byref(t), C.byref(sg)))
+        return {"schedule": s.value, "wide_tiles": w.value, "graph_steps": g.value, "train_workgroup_limit": t.value,
+                "scatter_grid": sg.value}
 
     def set_message_semantics(self, semantics):
         """abi.SEMANTICS_SINGLE_THREADED (default) or abi.SEMANTICS_MULTI_THREADED: the multi-threaded CPU

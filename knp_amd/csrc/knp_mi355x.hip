@@ -118,7 +118,8 @@ struct knp_net {
     std::vector<void *> allocations;
     std::vector<size_t> alloc_bytes;  // size of each allocation (state snapshots of the step tuner)
     uint64_t allocated = 0, cap = 0;
-    uint32_t scatter_grid = 1;
+    uint32_t scatter_grid = 1;        // workgroups of the atomic scatter (the step tuner may lower it)
+    uint32_t scatter_grid_alloc = 1;  // ... its largest value: the per-workgroup counters allocated
     int scatter_mode = KNP_SCATTER_AUTO;
     int semantics = KNP_SEMANTICS_SINGLE_THREADED;  // knp_mi355x_set_message_semantics
     bool fused = false;     // every population tiled: phase 1 of step t + 1 runs inside the tile kernel of t
@@ -1482,6 +1483,7 @@ int init_network(knp_net *net) {
         // segments hold at most 256 rows (pushes more, handled by the grid-stride loop)
         const uint64_t max_rows = (uint64_t)n_atomic * kBlock;
         net->scatter_grid = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(2048, max_rows));
+        net->scatter_grid_alloc = net->scatter_grid;
         if ((rc = dev_alloc(net, (void **)&T.part, (size_t)2 * net->scatter_grid * 8))) return rc;
     }
 
@@ -2050,11 +2052,11 @@ int run_steps(knp_net *net, uint64_t n_steps, uint32_t *counts = nullptr, bool t
 struct StepChoice {
     int schedule;
     bool wide;
-    uint32_t graph, train_limit;
+    uint32_t graph, train_limit, scatter_grid;
 };
 
 StepChoice current_choice(const knp_net *net) {
-    StepChoice c{KNP_SCHEDULE_STEPWISE, net->wide_tiles, net->graph_default, net->train_wg_limit};
+    StepChoice c{KNP_SCHEDULE_STEPWISE, net->wide_tiles, net->graph_default, net->train_wg_limit, net->scatter_grid};
     if (net->persistent) c.schedule = net->small ? KNP_SCHEDULE_SINGLE_WORKGROUP : KNP_SCHEDULE_PERSISTENT;
     return c;
 }
@@ -2066,6 +2068,7 @@ void apply_choice(knp_net *net, const StepChoice &c) {
     net->wide_tiles = c.wide;
     net->graph_default = c.graph;
     net->train_wg_limit = c.train_limit;
+    net->scatter_grid = std::max(1u, std::min(c.scatter_grid, net->scatter_grid_alloc));
     if (net->graph_exec) (void)hipGraphExecDestroy(net->graph_exec);
     net->graph_exec = nullptr;
 }
@@ -2154,9 +2157,9 @@ int autotune_steps(knp_net *net, uint32_t steps) {
         while (std::getline(in, line)) {
             if (line.compare(0, key.size(), key) != 0 || line.size() <= key.size() || line[key.size()] != ' ') continue;
             int sched = 0, wide = 0;
-            unsigned graph = 0, train = 0;
-            if (std::sscanf(line.c_str() + key.size(), " %d %d %u %u", &sched, &wide, &graph, &train) == 4) {
-                apply_choice(net, StepChoice{sched, wide != 0, graph, train});
+            unsigned graph = 0, train = 0, grid = 0;
+            if (std::sscanf(line.c_str() + key.size(), " %d %d %u %u %u", &sched, &wide, &graph, &train, &grid) == 5) {
+                apply_choice(net, StepChoice{sched, wide != 0, graph, train, grid});
                 return KNP_OK;
             }
         }
@@ -2210,6 +2213,18 @@ int autotune_steps(knp_net *net, uint32_t steps) {
             c.train_limit = 0;  // a wave per candidate
             rc = consider(c);
         }
+        // the atomic scatter's grid: each workgroup scans every atomic segment's count, so a network whose
+        // steps have few atomic rows runs faster with fewer workgroups
+        if (net->table.n_vseg_atomic) {
+            const StepChoice g1 = best;
+            for (uint32_t div : {4u, 16u, 64u}) {
+                const uint32_t g = std::max(1u, net->scatter_grid_alloc / div);
+                if (rc || g == g1.scatter_grid || g < 16u) continue;
+                StepChoice c = g1;
+                c.scatter_grid = g;
+                rc = consider(c);
+            }
+        }
     }
     net->timing = timing_was;
     const int rr = restore_snapshot(net, S);
@@ -2218,7 +2233,9 @@ int autotune_steps(knp_net *net, uint32_t steps) {
     if (rr) return rr;
     if (!path.empty()) {
         std::ofstream out(path, std::ios::app);
-        if (out) out << key << " " << best.schedule << " " << (int)best.wide << " " << best.graph << " " << best.train_limit << "\n";
+        if (out)
+            out << key << " " << best.schedule << " " << (int)best.wide << " " << best.graph << " " << best.train_limit
+                << " " << best.scatter_grid << "\n";
     }
     return KNP_OK;
 }
@@ -3111,11 +3128,11 @@ static int read_counters(knp_net *net, uint64_t *events, uint64_t *rows) {
     if (!net->initialized) return KNP_OK;
     int rc = download(net, c, net->table.events, 2);
     if (rc) return rc;
-    std::vector<unsigned long long> part(2 * (size_t)net->scatter_grid);
+    std::vector<unsigned long long> part(2 * (size_t)net->scatter_grid_alloc);
     if ((rc = download(net, part.data(), net->table.part, part.size()))) return rc;
     *events = c[0];
     *rows = c[1];
-    for (size_t b = 0; b < net->scatter_grid; ++b) {
+    for (size_t b = 0; b < net->scatter_grid_alloc; ++b) {
         *events += part[2 * b];
         *rows += part[2 * b + 1];
     }
@@ -3305,7 +3322,8 @@ int knp_mi355x_autotune(knp_net *net, uint32_t trial_steps) {
     return autotune_steps(net, trial_steps);
 }
 
-int knp_mi355x_get_tuning(knp_net *net, int *schedule, int *wide_tiles, uint32_t *graph_steps, uint32_t *train_limit) {
+int knp_mi355x_get_tuning(knp_net *net, int *schedule, int *wide_tiles, uint32_t *graph_steps, uint32_t *train_limit,
+                          uint32_t *scatter_grid) {
     if (!net) return fail(KNP_ERR_INVALID_ARGUMENT, "null handle");
     int rc;
     if ((rc = ensure_init(net))) return rc;
@@ -3314,6 +3332,7 @@ int knp_mi355x_get_tuning(knp_net *net, int *schedule, int *wide_tiles, uint32_t
     if (wide_tiles) *wide_tiles = c.wide ? 1 : 0;
     if (graph_steps) *graph_steps = c.graph;
     if (train_limit) *train_limit = c.train_limit;
+    if (scatter_grid) *scatter_grid = c.scatter_grid;
     return KNP_OK;
 }
 
