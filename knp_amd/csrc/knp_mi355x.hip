@@ -17,6 +17,7 @@
 #include <limits>
 #include <mutex>
 #include <string>
+#include <unordered_map>
 #include <vector>
 
 #include "../../include/knp_mi355x.h"
@@ -128,7 +129,8 @@ struct knp_net {
     SmallRun small_run{};
     uint32_t small_lds = 0;
     bool wide_tiles = false;  // tiles outnumber the workgroups resident at once: the 6-waves tile kernel
-    // what init found possible (the step tuner may run a network persistent-eligible stepwise, etc.)
+    // what init found possible (the step tuner may run a network persistent-eligible stepwise, etc.): the
+    // persistent launch of every tile, the one-workgroup launch
     bool can_persistent = false, can_small = false;
     // the MFMA expansions' spike counts were last staged in the device scratch (knp_mi355x_step_noalloc): the
     // per-projection buffers must be cleared before the next knp_mi355x_step uses them
@@ -618,6 +620,45 @@ int check_multi_threaded(const knp_net *net) {
             return fail(KNP_ERR_UNSUPPORTED, "presynaptic ranges above 2^24 under the multi-threaded message semantics");
     }
     return KNP_OK;
+}
+
+// In a persistent launch a workgroup waits, at each step, for the stamps of the segments its tile reads, and
+// overwrites its own segments of step t at step t + 2. That is safe when every workgroup whose segment another
+// workgroup reads in turn reads a segment of that reader: it then cannot publish step t + 2 before the reader
+// has published step t + 1, which the reader does only after reading step t. Segments and their writers:
+// population segment -> the workgroup of its tile (tile_of_block), source block j -> workgroup j mod tiles;
+// channel segments are written by the host before the launch. Networks where the relation is not symmetric
+// run stepwise.
+bool waits_cover_readers(const knp_net *net, const std::vector<std::vector<uint32_t>> &tile_reads,
+                         const std::unordered_map<uint32_t, uint32_t> &seg_tile) {
+    const NetDev &T = net->table;
+    const uint32_t n = T.n_tiles;
+    std::vector<uint32_t> wg_of_tile(n);
+    for (uint32_t b = 0; b < n; ++b) {
+        const uint32_t q = n / 8, r = n % 8, x = b % 8;  // tile_of_block
+        wg_of_tile[x * q + std::min(x, r) + b / 8] = b;
+    }
+    std::unordered_map<uint32_t, uint32_t> writer;  // segment -> workgroup
+    for (const auto &kv : seg_tile) writer[kv.first] = wg_of_tile[kv.second];
+    uint32_t j = 0;
+    for (uint32_t s2 = 0; s2 < T.n_srcs; ++s2) {
+        const uint32_t nb = std::max(1u, (net->srcs[s2].n_ch + kBlock - 1) / kBlock);
+        for (uint32_t c = 0; c < nb; ++c, ++j) writer[T.srcs[s2].seg_base + c] = j % n;
+    }
+    std::vector<std::vector<uint32_t>> reads(n);  // workgroup -> workgroups whose segments it reads
+    for (uint32_t i = 0; i < n; ++i) {
+        auto &rd = reads[wg_of_tile[i]];
+        for (uint32_t sg : tile_reads[i]) {
+            auto it = writer.find(sg);
+            if (it != writer.end()) rd.push_back(it->second);
+        }
+        std::sort(rd.begin(), rd.end());
+        rd.erase(std::unique(rd.begin(), rd.end()), rd.end());
+    }
+    for (uint32_t y = 0; y < n; ++y)
+        for (uint32_t x : reads[y])
+            if (x != y && !std::binary_search(reads[x].begin(), reads[x].end(), y)) return false;
+    return true;
 }
 
 int init_network(knp_net *net) {
@@ -1302,6 +1343,9 @@ int init_network(knp_net *net) {
     }
 
     // ---- spike segments and the scatter's virtual segment list
+    std::vector<std::vector<uint32_t>> tile_reads;  // per tile: the population and source segments it reads
+    std::unordered_map<uint32_t, uint32_t> seg_tile;  // population segment -> the tile that writes it
+    bool multi_ok = false;
     {
         int rc;
         std::vector<uint32_t *> seg_list;
@@ -1426,11 +1470,14 @@ int init_network(knp_net *net) {
                 if (net->projs[vd[k].q].post_pop == p && vd_tiles[k].first <= vd_tiles[k].second)
                     for (uint32_t x = vd_tiles[k].first; x <= vd_tiles[k].second; ++x) reach[x].push_back((uint32_t)k);
             for (uint32_t x = 0; x < nt; ++x) {
+                seg_tile[T.pops[p].seg_base + x] = (uint32_t)tiles.size();
                 tiles.push_back(TileDev{p, x, (uint32_t)tile_seg.size(), (uint32_t)(tile_seg.size() + reach[x].size())});
                 maxL = std::max(maxL, (uint32_t)reach[x].size());
+                tile_reads.emplace_back();
                 for (uint32_t k : reach[x]) {
                     const bool chan = vd[k].seg == T.projs[vd[k].q].push_seg;
                     tile_seg.push_back(TileSeg{seg_list[vd[k].seg], vd[k].q | (chan ? kTileSegChannel : 0u)});
+                    if (!chan) tile_reads.back().push_back(vd[k].seg);
                 }
             }
         }
@@ -1520,6 +1567,7 @@ int init_network(knp_net *net) {
         HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, kTileThreads, net->tile_lds));
         const bool src_fit = T.n_src_blocks <= (uint64_t)kCachedSrcBlocks * T.n_tiles;
         net->persistent = coop && src_fit && (uint64_t)per_cu * cus >= T.n_tiles;
+        multi_ok = net->persistent && waits_cover_readers(net, tile_reads, seg_tile);
         // a network whose neurons all fit on one workgroup's threads runs its steps inside that workgroup
         net->small = false;
         if (net->persistent && net->scatter_mode == KNP_SCATTER_AUTO && one_kind &&
@@ -1575,7 +1623,10 @@ int init_network(knp_net *net) {
             }
         }
     }
-    net->can_persistent = net->persistent;
+    // the persistent launch of every tile needs waits that keep each segment's readers within a step of its
+    // writer (waits_cover_readers); the one-workgroup launch runs every step in order anyway
+    if (!multi_ok && !net->small) net->persistent = false;
+    net->can_persistent = multi_ok;
     net->can_small = net->small;
     net->wide_tiles = false;
     if (T.n_tiles) {
@@ -1839,7 +1890,8 @@ int run_steps(knp_net *net, uint64_t n_steps, uint32_t *counts = nullptr, bool t
         }
         return KNP_OK;
     };
-    if (net->persistent && n_steps >= 2) {
+    const bool small_run = net->small && net->test_hold_block == 0xFFFFFFFFu;
+    if (net->persistent && n_steps >= 2 && (small_run || net->can_persistent)) {
         // persistent launches, split where a dense projection starts a new time block (its product must be
         // ready before phase 1 of that step). The abort flag is cleared once per call: a launch that starts
         // with it raised returns at once, and the host reads it after the last launch.
@@ -1862,7 +1914,7 @@ int run_steps(knp_net *net, uint64_t n_steps, uint32_t *counts = nullptr, bool t
             const uint32_t nt = net->table.n_tiles, steps = (uint32_t)len;
             uint64_t t0 = t;
             void *args[] = {(void *)&D, (void *)&t0, (void *)&steps};
-            const bool small = net->small && net->test_hold_block == 0xFFFFFFFFu;
+            const bool small = small_run;
             if ((rc = timed_launch(net, KNP_KERNEL_TILE_RUN, [&] {
                      if (small && net->run_kind == POP_LEAN)
                          k_small_run<POP_LEAN><<<1, kSmallThreads, net->small_lds, net->stream>>>(D, net->small_run, t0, steps);
@@ -2062,9 +2114,8 @@ StepChoice current_choice(const knp_net *net) {
 }
 
 void apply_choice(knp_net *net, const StepChoice &c) {
-    net->persistent = net->can_persistent &&
-                      (c.schedule == KNP_SCHEDULE_PERSISTENT || c.schedule == KNP_SCHEDULE_SINGLE_WORKGROUP);
     net->small = net->can_small && c.schedule == KNP_SCHEDULE_SINGLE_WORKGROUP;
+    net->persistent = net->small || (net->can_persistent && c.schedule == KNP_SCHEDULE_PERSISTENT);
     net->wide_tiles = c.wide;
     net->graph_default = c.graph;
     net->train_wg_limit = c.train_limit;
@@ -3307,7 +3358,7 @@ int knp_mi355x_get_schedule(knp_net *net, int *schedule) {
     int rc;
     if ((rc = ensure_init(net))) return rc;
     if (net->persistent && net->small && net->test_hold_block == 0xFFFFFFFFu) *schedule = KNP_SCHEDULE_SINGLE_WORKGROUP;
-    else if (net->persistent) *schedule = KNP_SCHEDULE_PERSISTENT;
+    else if (net->persistent && net->can_persistent) *schedule = KNP_SCHEDULE_PERSISTENT;
     else if (net->table.n_tiles || net->table.n_vseg_atomic == 0) *schedule = KNP_SCHEDULE_STEPWISE;
     else *schedule = KNP_SCHEDULE_ATOMIC;
     return KNP_OK;

@@ -551,8 +551,8 @@ def heavy_parts_network(o, seed):
     tile and step (a 2,048-channel source at p = 0.3 read whole by every tile: several gather passes), row
     parts of 300 records (200 of delay 1) and of 270 records of delay 1 (more than 255: the whole part is
     added before phase 1), long parts on both sides of phase 1, a ranged directory (rows of A spanning one tile of the
-    16-tile population B), pushed channel messages, and source blocks run by the workgroups of both
-    populations' tiles. Weights on a 2^-8 grid keep FP32 potentials exact."""
+    16-tile population B), pushed channel messages, source blocks run by the workgroups of both
+    populations' tiles, and a sparse B -> A projection. Weights on a 2^-8 grid keep FP32 potentials exact."""
     rng = np.random.default_rng(seed)
     na, nb = 1024, 4096
     a = o.add_blifat_population(abi.neuron_array(abi.blifat_defaults(), na))
@@ -586,6 +586,12 @@ def heavy_parts_network(o, seed):
     o.add_delta_projection(a, na, b, pre, post, (rng.integers(128, 320, S) / 256.0).astype(np.float32),
                            rng.integers(1, 4, S).astype(np.uint32),
                            rng.choice([abi.EXCITATORY, abi.INHIBITORY_CURRENT], S, p=[0.8, 0.2]).astype(np.uint8))
+    # B -> A: 4 per row, so that every workgroup of A also waits for every workgroup of B (the persistent
+    # launch needs each segment's readers to be waited for by its writer)
+    S = nb * 4
+    o.add_delta_projection(b, nb, a, np.repeat(np.arange(nb, dtype=np.uint32), 4), rng.integers(0, na, S).astype(np.uint32),
+                           (rng.integers(1, 4, S) / 256.0).astype(np.float32), rng.integers(1, 6, S).astype(np.uint32),
+                           rng.choice([abi.EXCITATORY, abi.INHIBITORY_CURRENT], S).astype(np.uint8))
     return (a, b), push
 
 
@@ -625,6 +631,49 @@ def test_persistent_rare_paths_match_oracle():
             for name in ("persistent", "stepwise", "atomic"):
                 assert rel_err(runs[name][1][j][k], runs["oracle"][1][j][k]) <= POT_RTOL, (name, j, k)
     assert runs["persistent"][2] == runs["stepwise"][2] == runs["atomic"][2] == runs["oracle"][2]
+
+
+def feed_forward_network(o, back):
+    """Population A (512 default BLIFAT) driven by a Bernoulli source, A -> B (512) with delays 1..3 and, with
+    `back`, B -> A. Without it no workgroup of A reads a segment that B's workgroups write (weights on a 2^-8
+    grid: every sum exact)."""
+    rng = np.random.default_rng(5)
+    n, S = 512, 20_000
+    a = o.add_blifat_population(abi.neuron_array(abi.blifat_defaults(), n))
+    b = o.add_blifat_population(abi.neuron_array(abi.blifat_defaults(), n))
+    ident = np.arange(n, dtype=np.uint32)
+    src = o.add_delta_projection(abi.KNP_NO_POPULATION, n, a, ident, ident, np.full(n, 1.0, np.float32),
+                                 np.ones(n, np.uint32))
+    o.add_bernoulli_source(src, np.full(n, workloads.bernoulli_threshold(0.05), np.uint32), seed=3)
+    for pre, post, lo, hi in ((a, b, 64, 192),) + (((b, a, 16, 64),) if back else ()):
+        o.add_delta_projection(pre, n, post, rng.integers(0, n, S).astype(np.uint32),
+                               rng.integers(0, n, S).astype(np.uint32), (rng.integers(lo, hi, S) / 256.0).astype(np.float32),
+                               rng.integers(1, 4, S).astype(np.uint32))
+    return a, b
+
+
+@pytest.mark.parametrize("back", [False, True])
+def test_persistent_launch_needs_mutual_waits(back):
+    """A persistent launch only runs networks where every workgroup whose segment another reads also reads a
+    segment of that reader (else a writer could overwrite step t's segment before a reader has taken it): the
+    feed-forward network runs stepwise, the one with a back projection persistent; both equal the oracle."""
+    say(f"case: persistent eligibility, back projection {back}")
+    steps = 300
+    g, o = MI355XBackend(0), Oracle("exact")
+    g.set_scatter_mode(abi.SCATTER_TILES)  # (not the one-workgroup schedule, which runs every step in order)
+    for x in (g, o):
+        a, b = feed_forward_network(x, back)
+        x.record_spikes(a, steps)
+        x.record_spikes(b, steps)
+    assert g.schedule() == (abi.SCHEDULE_PERSISTENT if back else abi.SCHEDULE_STEPWISE)
+    g.step(steps)
+    o.step(steps)
+    for p in (a, b):
+        so = o.read_spike_record(p)
+        assert so.sum() > 200, p
+        assert np.array_equal(g.read_spike_record(p), so), p
+    assert g.events == o.events
+    g.close()
 
 
 def agreement(a, b):

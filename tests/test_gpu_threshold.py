@@ -72,6 +72,15 @@ def threshold_network(o, n=768):
         q = o.add_delta_projection(abi.KNP_NO_POPULATION, n, p, pre, post, w, np.ones(2 * n, np.uint32))
         o.add_bernoulli_source(q, np.full(n, workloads.bernoulli_threshold(0.5), np.uint32), seed=41 + k)
         pops.append(p)
+    # zero-weight synapses from the first neuron of every 256-neuron segment to every segment of every
+    # population: they change no potential, but every tile then reads every population segment, which the
+    # persistent launch needs (each segment's writer waits for the segments of its readers)
+    starts = np.arange(0, n, 256, dtype=np.uint32)
+    for p in pops:
+        for q2 in pops:
+            pre = np.repeat(starts, len(starts))
+            post = np.tile(starts + 1, len(starts))
+            o.add_delta_projection(p, n, q2, pre, post, np.zeros(len(pre), np.float32), np.ones(len(pre), np.uint32))
     return pops
 
 
