@@ -1926,6 +1926,10 @@ int run_steps(knp_net *net, uint64_t n_steps, uint32_t *counts = nullptr, bool t
                  })))
                 return rc;
             if (le != hipSuccess) return fail(KNP_ERR_HIP, std::string("persistent launch: ") + hipGetErrorString(le));
+            if (!small) {
+                k_clear_push_counts<<<1, kTileThreads, 0, net->stream>>>(D, t0);
+                HIP_TRY(hipGetLastError());
+            }
             net->step += len;
             done += len;
             for (auto &P : net->pops) P.ran = true;

@@ -1385,9 +1385,8 @@ __global__ __launch_bounds__(kTileThreads) void k_tile_run(const NetDev *__restr
     unsigned long long my_rows = 0;  // thread 0: rows published by this workgroup's phase 1
     if (blockIdx.x == 0) {
         // Channel messages exist only for t0 (pushed by the host before the launch): their rows are counted
-        // here, and the channel counts of t0 + 1 are cleared now, those of t0 (parity of t0 + 2) once the
-        // scatter of t0 is done everywhere (after this workgroup has seen every segment of t0 + 1); the
-        // launch reads no channel segment after t0, so the cleared counts are for the steps that follow it.
+        // here, and the channel counts of t0 + 1 (which the launch never reads) are cleared now; those of t0
+        // are cleared by k_clear_push_counts after the launch, when no workgroup can still be reading them.
         if (threadIdx.x == 0)
             for (uint32_t q = 0; q < net->n_projs; ++q) {
                 const uint32_t vp = gp(net->seg_vpos)[4 * net->projs[q].push_seg];
@@ -1413,7 +1412,6 @@ __global__ __launch_bounds__(kTileThreads) void k_tile_run(const NetDev *__restr
         // t (delays 2 and up: later slots), while the other workgroups finish phase 1
         TilePart part;
         if (!tile_gather<true, true>(net, c, sh, t, my_ev, my_parts, k == 0, &part, slot_t)) break;
-        if (k == 1 && blockIdx.x == 0) clear_push_counts<true>(net, t0);
         if (k + 1 < n) {
             int64_t a = 0;
             if (tid < (uint32_t)kBlock) {
@@ -2475,6 +2473,12 @@ __global__ __launch_bounds__(kBlock) void k_dense_counts(const NetDev *__restric
         for (int w = 0; w < kBlock / 64; ++w) x += red[threadIdx.x][w];
         if (x) gp(net->projs[set.proj[0]].dn_part)[4 * blockIdx.x + threadIdx.x] += x;
     }
+}
+
+// The channel counts of step parity t1 cleared (one workgroup, after a persistent launch whose first step read
+// them).
+__global__ __launch_bounds__(kTileThreads) void k_clear_push_counts(const NetDev *__restrict__ net, uint64_t t1) {
+    clear_push_counts<false>(net, t1);
 }
 
 // The device's current step: set by the host before graph replays, advanced at the end of every captured step.
