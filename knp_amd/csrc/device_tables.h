@@ -18,6 +18,7 @@ constexpr int kMaxTasks = kMaxPops + kMaxSources;
 constexpr int kMaxWorking = 8;
 // ProjDev::rule_uniform bits: w_min, w_max, d_u, dopamine_plasticity_period, delay the same on every synapse
 constexpr uint32_t kUniWmin = 1, kUniWmax = 2, kUniDu = 4, kUniPeriod = 8, kUniDelay = 16;
+constexpr uint32_t kUniAll = kUniWmin | kUniWmax | kUniDu | kUniPeriod | kUniDelay;
 constexpr int kMaxDense = 4;
 constexpr int kBlock = 256;  // 4 wave64 per workgroup
 // A spike segment: word 0 = count, words 1.. = ascending spike indexes. Population and source segments hold
@@ -152,7 +153,10 @@ struct ProjDev {
     // last_spike_step_ (init_synapse, synaptic_resource_stdp.h:36-62, gives every synapse of a spiking row
     // the step): the row's last spike step once the row has spiked since load, else the loaded value
     uint64_t *last_spike_step;   // [n_syn] loaded values, synapse-index order
-    uint64_t *row_spike;         // [n_pre] last step each presynaptic row spiked (UINT64_MAX: not since load)
+    uint64_t *row_spike;         // [n_pre] last step each presynaptic row spiked; before its first spike since
+                                 // load, the loaded last_spike_step that all the row's synapses carry, or
+                                 // UINT64_MAX where they differ (each synapse's own loaded value then counts)
+    uint32_t lss_by_row;         // no row's synapses differ: row_spike alone gives every last_spike_step_
     const uint32_t *col_pre;     // [n_syn] CSC order: presynaptic neuron of col_syn[k]
     const uint32_t *col_pre0;    // [n_post] first presynaptic index of each column when every column holds
                                  // consecutive indexes (col_pre[k] = col_pre0[post] + k - col_ptr[post]), else null

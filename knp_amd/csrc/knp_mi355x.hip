@@ -1219,8 +1219,25 @@ int init_network(knp_net *net) {
             }
             for (uint64_t s = 0; s < S; ++s) u64[s] = H.rules[s].last_spike_step;
             if ((rc = upload(net, &Q.last_spike_step, u64.data(), S))) return rc;
-            std::vector<uint64_t> never((size_t)std::max(1u, H.n_pre), UINT64_MAX);
-            if ((rc = upload(net, &Q.row_spike, never.data(), never.size()))) return rc;
+            // each row's last_spike_step until it spikes: the value its synapses share, else UINT64_MAX
+            std::vector<uint64_t> row0((size_t)std::max(1u, H.n_pre), UINT64_MAX);
+            std::vector<uint8_t> seen(row0.size(), 0), mixed(row0.size(), 0);
+            for (uint64_t s = 0; s < S; ++s) {
+                const uint32_t r = H.pre[s];
+                if (!seen[r]) {
+                    seen[r] = 1;
+                    row0[r] = u64[s];
+                } else if (row0[r] != u64[s]) {
+                    mixed[r] = 1;
+                }
+            }
+            Q.lss_by_row = 1;
+            for (size_t r = 0; r < row0.size(); ++r)
+                if (mixed[r]) {
+                    row0[r] = UINT64_MAX;
+                    Q.lss_by_row = 0;
+                }
+            if ((rc = upload(net, &Q.row_spike, row0.data(), row0.size()))) return rc;
             for (uint64_t s = 0; s < S; ++s)
                 fl[s] = (uint8_t)((H.rules[s].had_hebbian_update ? 1 : 0) | (H.rules[s].has_contributed ? 2 : 0));
             if ((rc = upload(net, &Q.flags, fl.data(), S))) return rc;

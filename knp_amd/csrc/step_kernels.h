@@ -1659,29 +1659,6 @@ __host__ __device__ inline uint16_t bf16_lo(float f, uint16_t hi) {
 }
 
 
-// Position of the weight of synapse (pre -> post) in a dense projection's BF16 copy.
-__device__ __forceinline__ size_t dense_weight_pos(const ProjDev &Q, uint32_t post, uint32_t pre) {
-    const DenseInst &I = Q.dn_inst_tab[gp(Q.dn_post_inst)[post]];
-    const uint32_t i = pre - I.pre_begin, j = post - I.post_begin;
-    return I.base + ((size_t)(i >> 3) * I.npad + j) * 8 + (i & 7);
-}
-
-__device__ __forceinline__ void dense_store_weight(const ProjDev &Q, uint32_t post, uint32_t pre, float w) {
-    const size_t pos = dense_weight_pos(Q, post, pre);
-    const uint16_t hi = bf16_rne(w);
-    gp(Q.dn_w)[pos] = hi;
-    if (Q.dn_w_lo) gp(Q.dn_w_lo)[pos] = bf16_lo(w, hi);
-}
-
-// A synaptic-resource rule parameter: the projection's one value when every synapse carries the same one (no
-// per-synapse load), else the synapse's.
-__device__ __forceinline__ float rule_wmin(const ProjDev &Q, uint32_t sy) {
-    return (Q.rule_uniform & kUniWmin) ? Q.u_wmin : Q.w_min[sy];
-}
-__device__ __forceinline__ float rule_wmax(const ProjDev &Q, uint32_t sy) {
-    return (Q.rule_uniform & kUniWmax) ? Q.u_wmax : Q.w_max[sy];
-}
-
 __device__ __forceinline__ float recalc_weight(float r, float w_min, float w_max) {
     // recalculate_synapse_weights (training/stdp.h:40-53)
     const float syn_w = r > 0.f ? r : 0.f;
@@ -1746,17 +1723,17 @@ struct TrainNeuron {
 __device__ __forceinline__ TrainNeuron train_begin(const PopDev &P, uint32_t n, uint64_t t) {
     const ResDev &R = P.res;
     TrainNeuron s;
-    s.spiked = (P.bitmap[n >> 5] >> (n & 31)) & 1u;
+    s.spiked = (gp(P.bitmap)[n >> 5] >> (n & 31)) & 1u;
     // dopamine_value_ (a double sum of float impacts in the reference) from the exact accumulator sum
     s.dop_fix = gp(R.dop_fix)[n];
     s.dop = (double)s.dop_fix * (double)P.inv_scale;
-    s.forced = R.forced_now[n] != 0;
-    s.status = R.isi_status[n];
-    s.last_step = R.last_step[n];
-    s.lss = R.last_spike_step[n];
-    s.fis = R.first_isi_spike[n];
-    s.stab = R.stability[n];
-    s.freer = R.free_res[n];
+    s.forced = gp(R.forced_now)[n] != 0;
+    s.status = gp(R.isi_status)[n];
+    s.last_step = gp(R.last_step)[n];
+    s.lss = gp(R.last_spike_step)[n];
+    s.fis = gp(R.first_isi_spike)[n];
+    s.stab = gp(R.stability)[n];
+    s.freer = gp(R.free_res)[n];
     s.isi_max = pu(R.isi_max, n);
     if (s.spiked) {
         s.lss = t;
@@ -1815,15 +1792,111 @@ __device__ __forceinline__ bool renorm_runs(const ResDev &R, uint32_t n, const T
 }
 
 __device__ __forceinline__ void train_end(const ResDev &R, uint32_t n, const TrainNeuron &s) {
-    R.isi_status[n] = (uint8_t)s.status;
-    R.last_step[n] = s.last_step;
-    R.last_spike_step[n] = s.lss;
-    R.first_isi_spike[n] = s.fis;
-    R.stability[n] = s.stab;
-    R.free_res[n] = s.freer;
+    gp(R.isi_status)[n] = (uint8_t)s.status;
+    gp(R.last_step)[n] = s.last_step;
+    gp(R.last_spike_step)[n] = s.lss;
+    gp(R.first_isi_spike)[n] = s.fis;
+    gp(R.stability)[n] = s.stab;
+    gp(R.free_res)[n] = s.freer;
 }
 
-// Spiking pass of synapse k (CSC position) of post neuron n (blifat_stdp.h:105-131): had_contribution /
+// One working projection's column of training candidate n as the passes read it: the table's pointers as
+// global-memory pointers and the rule's shared values, copied out of the table once per column (otherwise every
+// store into the synapse arrays makes the compiler reload them from the table, one dependent round trip each),
+// the column's CSC range and its place in the projection's BF16 copy.
+struct TrainCol {
+    KNP_GLOBAL float *W, *r;
+    KNP_GLOBAL uint8_t *flags;
+    const KNP_GLOBAL float *w_min, *w_max, *d_u;
+    const KNP_GLOBAL uint32_t *period, *col_syn, *col_pre;
+    const KNP_GLOBAL uint64_t *lss, *row_spike;
+    const KNP_GLOBAL uint8_t *delay_syn;
+    KNP_GLOBAL uint16_t *dn_w, *dn_w_lo;  // the BF16 copy and its low parts (null: none)
+    uint64_t cb, ce;                      // the column's CSC positions
+    int64_t pre_off;                      // presynaptic index - CSC position (columns of consecutive indexes)
+    uint64_t dn_col;                      // copy position of the column's weight from the instance's first input
+    uint32_t dn_stride;                   // npad * 8: from one octet of inputs to the next
+    uint32_t dn_pre0;                     // the instance's first presynaptic index
+    uint32_t uni;                         // kUni* bits
+    bool ident, pre0, by_row;
+    float u_wmin, u_wmax, u_du;
+    uint32_t u_period, u_delay;
+
+    // synapses stored by postsynaptic neuron, consecutive presynaptic indexes, every rule parameter shared and
+    // the last spike steps given by the rows (the layout of all-to-all projections): a synapse's own loads are
+    // its weight, flags and resource only
+    __device__ __forceinline__ bool fast() const { return ident && pre0 && by_row && (uni & kUniAll) == kUniAll; }
+    __device__ __forceinline__ uint32_t syn(uint64_t k) const { return ident ? (uint32_t)k : col_syn[k]; }
+    __device__ __forceinline__ uint32_t pre(uint64_t k) const {
+        return pre0 ? (uint32_t)((int64_t)k + pre_off) : col_pre[k];
+    }
+    // a rule parameter: the projection's one value when every synapse carries the same one (no per-synapse
+    // load), else the synapse's
+    __device__ __forceinline__ float wmin(uint32_t s) const { return (uni & kUniWmin) ? u_wmin : w_min[s]; }
+    __device__ __forceinline__ float wmax(uint32_t s) const { return (uni & kUniWmax) ? u_wmax : w_max[s]; }
+    // a new weight of synapse s (presynaptic index pre) into W and into the BF16 copy
+    __device__ __forceinline__ void store_weight(uint32_t s, uint32_t pre, float w) const {
+        W[s] = w;
+        if (dn_w) {
+            const uint32_t i = pre - dn_pre0;
+            const uint64_t pos = dn_col + (uint64_t)(i >> 3) * dn_stride + (i & 7);
+            const uint16_t hi = bf16_rne(w);
+            dn_w[pos] = hi;
+            if (dn_w_lo) dn_w_lo[pos] = bf16_lo(w, hi);
+        }
+    }
+};
+
+// A value every lane of the wave holds alike, as a scalar (loop bounds and offsets then live in scalar registers).
+__device__ __forceinline__ uint32_t wave_uniform(uint32_t x) { return __builtin_amdgcn_readfirstlane(x); }
+__device__ __forceinline__ uint64_t wave_uniform(uint64_t x) {
+    return (uint64_t)wave_uniform((uint32_t)x) | (uint64_t)wave_uniform((uint32_t)(x >> 32)) << 32;
+}
+
+// n: the same neuron in every lane of the wave.
+__device__ __forceinline__ TrainCol train_col(const ProjDev &Q, uint32_t n) {
+    TrainCol c;
+    c.W = gp(Q.W);
+    c.r = gp(Q.r);
+    c.flags = gp(Q.flags);
+    c.w_min = gp((const float *)Q.w_min);
+    c.w_max = gp((const float *)Q.w_max);
+    c.d_u = gp((const float *)Q.d_u);
+    c.period = gp((const uint32_t *)Q.period);
+    c.col_syn = gp(Q.col_syn);
+    c.col_pre = gp(Q.col_pre);
+    c.lss = gp((const uint64_t *)Q.last_spike_step);
+    c.row_spike = gp((const uint64_t *)Q.row_spike);
+    c.delay_syn = gp(Q.delay_syn);
+    c.cb = wave_uniform(gp(Q.col_ptr)[n]);
+    c.ce = wave_uniform(gp(Q.col_ptr)[n + 1]);
+    c.pre0 = Q.col_pre0 != nullptr;
+    c.pre_off = c.pre0 ? (int64_t)wave_uniform(gp(Q.col_pre0)[n]) - (int64_t)c.cb : 0;
+    c.uni = Q.rule_uniform;
+    c.ident = Q.csc_identity != 0;
+    c.by_row = Q.lss_by_row != 0;
+    c.u_wmin = Q.u_wmin;
+    c.u_wmax = Q.u_wmax;
+    c.u_du = Q.u_du;
+    c.u_period = Q.u_period;
+    c.u_delay = Q.u_delay;
+    c.dn_w = nullptr;
+    c.dn_w_lo = nullptr;
+    c.dn_col = 0;
+    c.dn_stride = 0;
+    c.dn_pre0 = 0;
+    if (Q.dense && c.cb < c.ce) {  // a neuron with synapses in a dense projection lies in one of its instances
+        const KNP_GLOBAL DenseInst *I = gp(Q.dn_inst_tab) + gp(Q.dn_post_inst)[n];
+        c.dn_w = gp(Q.dn_w);
+        c.dn_w_lo = gp(Q.dn_w_lo);
+        c.dn_col = wave_uniform(I->base + (uint64_t)(n - I->post_begin) * 8);
+        c.dn_stride = wave_uniform(I->npad * 8);
+        c.dn_pre0 = wave_uniform(I->pre_begin);
+    }
+    return c;
+}
+
+// Spiking pass of synapse k (CSC position) of the candidate (blifat_stdp.h:105-131): had_contribution /
 // had_hebbian_update flags, resource -= d_u (and += d_h on a new contribution), weight recalculated. Adds the
 // old weight's fixed-point value to sum when positive; returns d_u in du (0 when the neuron is forced) and
 // whether the synapse contributed.
@@ -1834,34 +1907,39 @@ struct SpikingSyn {  // what the spiking pass reads of one synapse
     uint8_t fl;
 };
 
-// pre_off: col_pre0[post] - col_ptr[post] of the synapse's column when the projection's columns hold
-// consecutive presynaptic indexes (the index is then k + pre_off, with no load in front of row_spike).
-__device__ __forceinline__ SpikingSyn spiking_load(const ProjDev &Q, uint64_t k, uint32_t status, int64_t pre_off) {
+// The presynaptic index of a column of consecutive indexes is k + pre_off, with no load in front of row_spike.
+__device__ __forceinline__ SpikingSyn spiking_load(const TrainCol &C, uint64_t k, uint32_t status) {
     SpikingSyn x;
-    x.sy = Q.csc_identity ? (uint32_t)k : Q.col_syn[k];
-    x.w = Q.W[x.sy];
-    x.fl = Q.flags[x.sy];
-    x.per = (Q.rule_uniform & kUniPeriod) ? Q.u_period : Q.period[x.sy];
-    x.pre = Q.col_pre0 ? (uint32_t)((int64_t)k + pre_off) : Q.col_pre[k];
-    const uint64_t rs = Q.row_spike[x.pre];
-    const uint32_t delay = (Q.rule_uniform & kUniDelay) ? Q.u_delay : Q.delay_syn[x.sy];
-    x.spk = (rs != UINT64_MAX ? rs : Q.last_spike_step[x.sy]) + delay - 1;
-    x.r = Q.r[x.sy];
-    x.du = status != 0 ? ((Q.rule_uniform & kUniDu) ? Q.u_du : Q.d_u[x.sy]) : 0.f;
-    x.wmin = rule_wmin(Q, x.sy);
-    x.wmax = rule_wmax(Q, x.sy);
+    x.sy = C.syn(k);
+    x.w = C.W[x.sy];
+    x.fl = C.flags[x.sy];
+    x.per = (C.uni & kUniPeriod) ? C.u_period : C.period[x.sy];
+    x.pre = C.pre(k);
+    const uint64_t rs = C.row_spike[x.pre];
+    const uint32_t delay = (C.uni & kUniDelay) ? C.u_delay : C.delay_syn[x.sy];
+    x.spk = (rs != UINT64_MAX ? rs : C.lss[x.sy]) + delay - 1;
+    x.r = C.r[x.sy];
+    x.du = status != 0 ? ((C.uni & kUniDu) ? C.u_du : C.d_u[x.sy]) : 0.f;
+    x.wmin = C.wmin(x.sy);
+    x.wmax = C.wmax(x.sy);
     return x;
 }
 
-__device__ __forceinline__ bool spiking_update(const ProjDev &Q, uint32_t n, const SpikingSyn &x, uint64_t t,
-                                               uint32_t status, float d_h, float scale, long long &sum) {
-    const uint32_t sy = x.sy, pre = x.pre;
-    const float w = x.w, r = x.r, du = x.du, wmin = x.wmin, wmax = x.wmax;
-    const uint64_t per = x.per, spk = x.spk;
+// What the spiking pass writes back for one synapse (stores of unchanged values are left out; the BF16 copy of
+// an unchanged weight is current too).
+struct SpikingOut {
+    float rr, wn;
+    uint8_t nf;
+    bool st_r, st_w, st_f;
+};
+
+__device__ __forceinline__ bool spiking_compute(const SpikingSyn &x, uint64_t t, uint32_t status, float d_h,
+                                                float scale, long long &sum, SpikingOut &o) {
+    const float w = x.w, r = x.r, du = x.du;
     const uint8_t fl = x.fl;
     if (w > 0.f) sum += __float2ll_rn(w * scale);
     bool hc = fl & 2, hh = fl & 1;
-    const bool had_spike = in_interval(t - per, t, spk);
+    const bool had_spike = in_interval(t - x.per, t, x.spk);
     if (status != 2 || had_spike) hc = had_spike;
     if (status != 2) hh = false;
     float rr = r;
@@ -1873,68 +1951,147 @@ __device__ __forceinline__ bool spiking_update(const ProjDev &Q, uint32_t n, con
             hh = true;
             contributing = true;
         }
-        if (__float_as_uint(rr) != __float_as_uint(r)) Q.r[sy] = rr;
     }
-    // stores of unchanged values are left out (the BF16 copy of an unchanged weight is current too)
-    const float wn = recalc_weight(rr, wmin, wmax);
-    if (__float_as_uint(wn) != __float_as_uint(w)) {
-        Q.W[sy] = wn;
-        if (Q.dense) dense_store_weight(Q, n, pre, wn);
-    }
-    const uint8_t nf = (uint8_t)((hc ? 2 : 0) | (hh ? 1 : 0));
-    if (nf != fl) Q.flags[sy] = nf;
+    o.rr = rr;
+    o.st_r = __float_as_uint(rr) != __float_as_uint(r);
+    o.wn = recalc_weight(rr, x.wmin, x.wmax);
+    o.st_w = __float_as_uint(o.wn) != __float_as_uint(w);
+    o.nf = (uint8_t)((hc ? 2 : 0) | (hh ? 1 : 0));
+    o.st_f = o.nf != fl;
     return contributing;
 }
 
-__device__ __forceinline__ bool spiking_synapse(const ProjDev &Q, uint32_t n, uint64_t k, uint64_t t,
-                                                uint32_t status, float d_h, float scale, long long &sum, float &du,
-                                                int64_t pre_off) {
-    const SpikingSyn x = spiking_load(Q, k, status, pre_off);
-    du = x.du;
-    return spiking_update(Q, n, x, t, status, d_h, scale, sum);
+__device__ __forceinline__ void spiking_store(const TrainCol &C, uint32_t sy, uint32_t pre, const SpikingOut &o) {
+    if (o.st_r) C.r[sy] = o.rr;
+    if (o.st_w) C.store_weight(sy, pre, o.wn);
+    if (o.st_f) C.flags[sy] = o.nf;
+}
+
+__device__ __forceinline__ bool spiking_update(const TrainCol &C, const SpikingSyn &x, uint64_t t, uint32_t status,
+                                               float d_h, float scale, long long &sum) {
+    SpikingOut o;
+    const bool contributing = spiking_compute(x, t, status, d_h, scale, sum, o);
+    spiking_store(C, x.sy, x.pre, o);
+    return contributing;
 }
 
 // Dopamine pass of synapse k (blifat_stdp.h:133-156): resource += change where the synapse contributed and the
 // neuron spiked recently; the weight is recalculated in every case. Returns whether the resource changed.
-__device__ __forceinline__ bool dopamine_synapse(const ProjDev &Q, uint32_t n, uint64_t k, bool time_ok, float rc) {
-    const uint32_t sy = Q.csc_identity ? (uint32_t)k : Q.col_syn[k];
-    const float r = Q.r[sy];
-    const float w = Q.W[sy];
-    const uint8_t fl = Q.flags[sy];
-    const float wmin = rule_wmin(Q, sy), wmax = rule_wmax(Q, sy);
+__device__ __forceinline__ bool dopamine_synapse(const TrainCol &C, uint64_t k, bool time_ok, float rc) {
+    const uint32_t sy = C.syn(k);
+    const float r = C.r[sy];
+    const float w = C.W[sy];
+    const uint8_t fl = C.flags[sy];
+    const float wmin = C.wmin(sy), wmax = C.wmax(sy);
     float rr = r;
     bool contributing = false;
     if (time_ok && (fl & 2)) {
         rr += rc;
-        if (__float_as_uint(rr) != __float_as_uint(r)) Q.r[sy] = rr;
+        if (__float_as_uint(rr) != __float_as_uint(r)) C.r[sy] = rr;
         contributing = true;
     }
     const float wn = recalc_weight(rr, wmin, wmax);
-    if (__float_as_uint(wn) != __float_as_uint(w)) {
-        Q.W[sy] = wn;
-        if (Q.dense) dense_store_weight(Q, n, gp(Q.col_pre)[k], wn);
-    }
+    if (__float_as_uint(wn) != __float_as_uint(w)) C.store_weight(sy, C.pre(k), wn);
     return contributing;
 }
 
 // renormalize_resource for synapse k (blifat_stdp.h:174-190): resource += add, weight recalculated.
 // r0 and w: the synapse's resource and weight as loaded.
-__device__ __forceinline__ void renorm_update(const ProjDev &Q, uint32_t n, uint64_t k, float r0, float w,
-                                              float add) {
-    const uint32_t s = Q.csc_identity ? (uint32_t)k : Q.col_syn[k];
+__device__ __forceinline__ void renorm_update(const TrainCol &C, uint64_t k, float r0, float w, float add) {
+    const uint32_t s = C.syn(k);
     const float r = r0 + add;
-    if (__float_as_uint(r) != __float_as_uint(r0)) Q.r[s] = r;
-    const float wn = recalc_weight(r, rule_wmin(Q, s), rule_wmax(Q, s));
-    if (__float_as_uint(wn) != __float_as_uint(w)) {
-        Q.W[s] = wn;
-        if (Q.dense) dense_store_weight(Q, n, gp(Q.col_pre)[k], wn);
+    if (__float_as_uint(r) != __float_as_uint(r0)) C.r[s] = r;
+    const float wn = recalc_weight(r, C.wmin(s), C.wmax(s));
+    if (__float_as_uint(wn) != __float_as_uint(w)) C.store_weight(s, C.pre(k), wn);
+}
+
+__device__ __forceinline__ void renorm_synapse(const TrainCol &C, uint64_t k, float add) {
+    const uint32_t s = C.syn(k);
+    renorm_update(C, k, C.r[s], C.W[s], add);
+}
+
+// The free-resource replay of one chunk of the spiking pass (serial_free; lane 0 is active in every chunk, and
+// one d_u over the chunk takes the closed form). Every lane of the wave calls it.
+__device__ __forceinline__ float replay_chunk(float freer, bool act, float du, bool contributing, float d_h) {
+    const uint64_t am = __ballot(act), nz = __ballot(act && du != 0.f), cm = __ballot(contributing);
+    const float du0 = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(du)));
+    if (du0 != 0.f && __ballot(act && __float_as_int(du) != __float_as_int(du0)) == 0)
+        return replay_uniform(freer, am, cm, du0, -d_h);
+    return serial_free(freer, am, nz, cm, du, -d_h, true);
+}
+
+// What the spiking pass loads of U chunks of a fast column (TrainCol::fast).
+template <uint32_t U>
+struct FastChunks {
+    float w[U], r[U];
+    uint64_t rs[U];  // the row's last spike step (the synapse's last_spike_step_)
+    uint8_t fl[U];
+};
+
+template <uint32_t U>
+__device__ __forceinline__ void fast_load(const TrainCol &C, uint64_t base, uint32_t lane, FastChunks<U> &g) {
+#pragma unroll
+    for (uint32_t u = 0; u < U; ++u) {
+        uint64_t k = base + 64 * u + lane;
+        k = k < C.ce ? k : C.ce - 1;  // lanes past the column's end load its last synapse and store nothing
+        g.w[u] = C.W[k];
+        g.fl[u] = C.flags[k];
+        g.r[u] = C.r[k];
+        g.rs[u] = C.row_spike[(uint32_t)((int64_t)k + C.pre_off)];
     }
 }
 
-__device__ __forceinline__ void renorm_synapse(const ProjDev &Q, uint32_t n, uint64_t k, float add) {
-    const uint32_t s = Q.csc_identity ? (uint32_t)k : Q.col_syn[k];
-    renorm_update(Q, n, k, Q.r[s], Q.W[s], add);
+// The spiking pass over a fast column, U chunks at a time, software-pipelined: the next chunks' loads are issued,
+// the current chunks are updated in registers and their free-resource sums replayed in synapse order, then the
+// wave waits for the next chunks' loads and only after that issues the current chunks' stores. (Stores of
+// unchanged values are skipped, so the number of stores in flight is unknown to the compiler; a load issued
+// before them could only be waited for together with them.)
+template <uint32_t U>
+__device__ __forceinline__ void spiking_column_fast(const TrainCol &C, TrainNeuron &s, uint64_t t, float d_h,
+                                                    float scale, long long &sum, uint32_t lane) {
+    constexpr uint64_t kStep = 64 * U;
+    FastChunks<U> cur, nxt;
+    fast_load<U>(C, C.cb, lane, cur);
+    nxt = cur;
+    for (uint64_t base = C.cb; base < C.ce; base += kStep) {
+        const bool more = base + kStep < C.ce;
+        if (more) fast_load<U>(C, base + kStep, lane, nxt);
+        SpikingOut o[U];
+#pragma unroll
+        for (uint32_t u = 0; u < U; ++u) {
+            if (base + 64 * u >= C.ce) break;
+            const uint64_t k = base + 64 * u + lane;
+            const bool act = k < C.ce;
+            SpikingSyn x;
+            x.w = cur.w[u];
+            x.r = cur.r[u];
+            x.fl = cur.fl[u];
+            x.per = C.u_period;
+            x.spk = cur.rs[u] + C.u_delay - 1;
+            x.du = s.status != 0 ? C.u_du : 0.f;
+            x.wmin = C.u_wmin;
+            x.wmax = C.u_wmax;
+            const bool contributing = spiking_compute(x, t, s.status, d_h, scale, sum, o[u]) && act;
+            if (!act) o[u].st_r = o[u].st_w = o[u].st_f = false;
+            if (s.status != 0) s.freer = replay_chunk(s.freer, act, act ? x.du : 0.f, contributing, d_h);
+        }
+        // s_waitcnt vmcnt(0): every vector memory access issued so far (the next chunks' loads) complete
+        __builtin_amdgcn_s_waitcnt(0x0F70);
+#pragma unroll
+        for (uint32_t u = 0; u < U; ++u) {
+            if (base + 64 * u >= C.ce) break;
+            const uint64_t k = base + 64 * u + lane;
+            spiking_store(C, (uint32_t)k, (uint32_t)((int64_t)k + C.pre_off), o[u]);
+        }
+        cur = nxt;
+    }
 }
+
+#ifndef KNP_TRAIN_FAST_UNROLL
+#define KNP_TRAIN_FAST_UNROLL 2
+#endif
+// Chunks of a fast column loaded together (the pipeline holds twice as many).
+constexpr uint32_t kTrainFastUnroll = KNP_TRAIN_FAST_UNROLL;
 
 #ifndef KNP_TRAIN_UNROLL
 #define KNP_TRAIN_UNROLL 2
@@ -1953,40 +2110,33 @@ __device__ __forceinline__ void train_candidate_wave(const NetDev *__restrict__ 
         const float d_h = pf(R.d_h, n) * stability_factor(s.stab);
         long long sum = 0;
         for (uint32_t j = 0; j < R.n_work; ++j) {
-            const ProjDev &Q = net->projs[R.work[j]];
-            const uint64_t cb = Q.col_ptr[n], ce = Q.col_ptr[n + 1];
-            const int64_t pre_off = Q.col_pre0 ? (int64_t)Q.col_pre0[n] - (int64_t)cb : 0;
+            const TrainCol C = train_col(net->projs[R.work[j]], n);
+            if (C.fast()) {
+                spiking_column_fast<kTrainFastUnroll>(C, s, t, d_h, P.scale, sum, lane);
+                continue;
+            }
             // kTrainUnroll chunks at a time: all their loads are issued before the first chunk's stores (the
             // chunks hold different synapses), then each chunk is updated and its free-resource sum replayed
             // in order
-            for (uint64_t base = cb; base < ce; base += 64 * kTrainUnroll) {
+            for (uint64_t base = C.cb; base < C.ce; base += 64 * kTrainUnroll) {
                 SpikingSyn x[kTrainUnroll];
                 bool act[kTrainUnroll];
 #pragma unroll
                 for (uint32_t u = 0; u < kTrainUnroll; ++u) {
-                    act[u] = base + 64 * u + lane < ce;
-                    if (act[u]) x[u] = spiking_load(Q, base + 64 * u + lane, s.status, pre_off);
+                    act[u] = base + 64 * u + lane < C.ce;
+                    if (act[u]) x[u] = spiking_load(C, base + 64 * u + lane, s.status);
                 }
 #pragma unroll
                 for (uint32_t u = 0; u < kTrainUnroll; ++u) {
-                    if (base + 64 * u >= ce) break;
-                    const bool contributing = act[u] && spiking_update(Q, n, x[u], t, s.status, d_h, P.scale, sum);
-                    const float du = act[u] ? x[u].du : 0.f;
-                    const uint64_t am = __ballot(act[u]), nz = __ballot(act[u] && du != 0.f);
-                    const uint64_t cm = __ballot(contributing);
-                    if (s.status != 0) {
-                        // lane 0 is active in every chunk; one d_u over the chunk takes the closed form
-                        const float du0 = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(du)));
-                        if (du0 != 0.f && __ballot(act[u] && __float_as_int(du) != __float_as_int(du0)) == 0)
-                            s.freer = replay_uniform(s.freer, am, cm, du0, -d_h);
-                        else
-                            s.freer = serial_free(s.freer, am, nz, cm, du, -d_h, true);
-                    }
+                    if (base + 64 * u >= C.ce) break;
+                    const bool contributing = act[u] && spiking_update(C, x[u], t, s.status, d_h, P.scale, sum);
+                    if (s.status != 0)
+                        s.freer = replay_chunk(s.freer, act[u], act[u] ? x[u].du : 0.f, contributing, d_h);
                 }
             }
         }
         sum = wave_sum_i64(sum);
-        if (lane == 0) R.add_thr[n] = fixed_to_float(sum, P.inv_scale) * pf(R.coeff, n);
+        if (lane == 0) gp(R.add_thr)[n] = fixed_to_float(sum, P.inv_scale) * pf(R.coeff, n);
     }
     bool passed = s.spiked;  // a pass over every working synapse recalculated the weights from the resources
     if (dopamine_runs(s)) {
@@ -1994,14 +2144,13 @@ __device__ __forceinline__ void train_candidate_wave(const NetDev *__restrict__ 
         const float rc = (float)(s.dop * (double)stability_factor(s.stab));
         // without time_ok no resource changes; the pass then only recalculates weights, which changes none
         // once a pass has made every weight its recalculation
-        const bool needed = time_ok || !R.w_current[n];
+        const bool needed = time_ok || !gp(R.w_current)[n];
         passed |= needed;
         for (uint32_t j = 0; j < R.n_work && needed; ++j) {
-            const ProjDev &Q = net->projs[R.work[j]];
-            const uint64_t cb = Q.col_ptr[n], ce = Q.col_ptr[n + 1];
-            for (uint64_t base = cb; base < ce; base += 64) {
+            const TrainCol C = train_col(net->projs[R.work[j]], n);
+            for (uint64_t base = C.cb; base < C.ce; base += 64) {
                 const uint64_t k = base + lane;
-                const uint64_t cm = __ballot(k < ce && dopamine_synapse(Q, n, k, time_ok, rc));
+                const uint64_t cm = __ballot(k < C.ce && dopamine_synapse(C, k, time_ok, rc));
                 // free -= change once per contributing synapse: the same float operation each time
                 s.freer = repeat_add(s.freer, -rc, (uint64_t)__popcll(cm));
             }
@@ -2011,25 +2160,24 @@ __device__ __forceinline__ void train_candidate_wave(const NetDev *__restrict__ 
     if (renorm_runs(R, n, s, t)) {
         uint64_t k_total = 0;
         for (uint32_t j = 0; j < R.n_work; ++j) {
-            const ProjDev &Q = net->projs[R.work[j]];
-            k_total += Q.col_ptr[n + 1] - Q.col_ptr[n];
+            const KNP_GLOBAL uint64_t *cp = gp(net->projs[R.work[j]].col_ptr);
+            k_total += cp[n + 1] - cp[n];
         }
         const float add = s.freer / (float)(k_total + (uint64_t)pu(R.drain, n));
         for (uint32_t j = 0; j < R.n_work; ++j) {
-            const ProjDev &Q = net->projs[R.work[j]];
-            const uint64_t cb = Q.col_ptr[n], ce = Q.col_ptr[n + 1];
-            for (uint64_t k = cb + lane; k < ce; k += 64 * kTrainUnroll) {
+            const TrainCol C = train_col(net->projs[R.work[j]], n);
+            for (uint64_t k = C.cb + lane; k < C.ce; k += 64 * kTrainUnroll) {
                 float r0[kTrainUnroll], w0[kTrainUnroll];
 #pragma unroll
                 for (uint32_t u = 0; u < kTrainUnroll; ++u)
-                    if (k + 64 * u < ce) {
-                        const uint32_t sy = Q.csc_identity ? (uint32_t)(k + 64 * u) : Q.col_syn[k + 64 * u];
-                        r0[u] = Q.r[sy];
-                        w0[u] = Q.W[sy];
+                    if (k + 64 * u < C.ce) {
+                        const uint32_t sy = C.syn(k + 64 * u);
+                        r0[u] = C.r[sy];
+                        w0[u] = C.W[sy];
                     }
 #pragma unroll
                 for (uint32_t u = 0; u < kTrainUnroll; ++u)
-                    if (k + 64 * u < ce) renorm_update(Q, n, k + 64 * u, r0[u], w0[u], add);
+                    if (k + 64 * u < C.ce) renorm_update(C, k + 64 * u, r0[u], w0[u], add);
             }
         }
         s.freer = 0.f;
@@ -2037,7 +2185,7 @@ __device__ __forceinline__ void train_candidate_wave(const NetDev *__restrict__ 
     }
     if (lane == 0) {
         train_end(R, n, s);
-        if (passed) R.w_current[n] = 1;
+        if (passed) gp(R.w_current)[n] = 1;
     }
 }
 
@@ -2053,8 +2201,10 @@ __global__ __launch_bounds__(kBlock) void k_train_wave(const NetDev *__restrict_
     const uint32_t count = net->ctl[t & 1].cand_count[p];
     const uint32_t lane = lane_id();
     constexpr uint32_t W = kBlock / 64;
-    for (uint32_t c = blockIdx.x * W + threadIdx.x / 64; c < count; c += gridDim.x * W)
-        train_candidate_wave(net, P, P.res.cand[c], t, lane);
+    // the candidate is the wave's (scalar loads of its state and its columns' table entries)
+    const uint32_t c0 = __builtin_amdgcn_readfirstlane(blockIdx.x * W + threadIdx.x / 64);
+    for (uint32_t c = c0; c < count; c += gridDim.x * W)
+        train_candidate_wave(net, P, wave_uniform(gp(P.res.cand)[c]), t, lane);
 }
 
 // k_train: a workgroup per candidate (smaller populations, whose few candidates a step cannot fill the device
@@ -2077,7 +2227,6 @@ __global__ __launch_bounds__(kTrainBlock) void k_train(const NetDev *__restrict_
     __shared__ long long s_sum[W];
     __shared__ uint32_t s_cnt[W];
     __shared__ uint64_t s_cb[kMaxWorking], s_ce[kMaxWorking];
-    __shared__ int64_t s_po[kMaxWorking];
     __shared__ uint32_t s_nch[kMaxWorking];
     __shared__ float s_free;
     // chunk g of the candidate: its working projection, CSC position of its first synapse and the column's end
@@ -2093,14 +2242,13 @@ __global__ __launch_bounds__(kTrainBlock) void k_train(const NetDev *__restrict_
         return false;
     };
     for (uint32_t c = blockIdx.x; c < count; c += gridDim.x) {
-        const uint32_t n = R.cand[c];
+        const uint32_t n = wave_uniform(gp(R.cand)[c]);
         __syncthreads();  // the previous candidate's LDS reads are done
         if (threadIdx.x < R.n_work) {
             const ProjDev &Q = net->projs[R.work[threadIdx.x]];
-            s_cb[threadIdx.x] = Q.col_ptr[n];
-            s_ce[threadIdx.x] = Q.col_ptr[n + 1];
+            s_cb[threadIdx.x] = gp(Q.col_ptr)[n];
+            s_ce[threadIdx.x] = gp(Q.col_ptr)[n + 1];
             s_nch[threadIdx.x] = (uint32_t)((s_ce[threadIdx.x] - s_cb[threadIdx.x] + 63) / 64);
-            s_po[threadIdx.x] = Q.col_pre0 ? (int64_t)Q.col_pre0[n] - (int64_t)s_cb[threadIdx.x] : 0;
         }
         __syncthreads();
         uint32_t G = 0;
@@ -2115,10 +2263,15 @@ __global__ __launch_bounds__(kTrainBlock) void k_train(const NetDev *__restrict_
                 uint32_t j;
                 uint64_t base, end;
                 if (r0 + wv < G && chunk(r0 + wv, j, base, end)) {
+                    const TrainCol C = train_col(net->projs[R.work[j]], n);
                     const uint64_t k = base + lane;
                     const bool act = k < end;
-                    const bool contributing =
-                        act && spiking_synapse(net->projs[R.work[j]], n, k, t, s.status, d_h, P.scale, sum, du, s_po[j]);
+                    bool contributing = false;
+                    if (act) {
+                        const SpikingSyn x = spiking_load(C, k, s.status);
+                        du = x.du;
+                        contributing = spiking_update(C, x, t, s.status, d_h, P.scale, sum);
+                    }
                     am = __ballot(act);
                     nz = __ballot(act && du != 0.f);
                     cm = __ballot(contributing);
@@ -2154,7 +2307,7 @@ __global__ __launch_bounds__(kTrainBlock) void k_train(const NetDev *__restrict_
             if (threadIdx.x == 0) {
                 long long total = 0;
                 for (uint32_t w = 0; w < W; ++w) total += s_sum[w];
-                R.add_thr[n] = fixed_to_float(total, P.inv_scale) * pf(R.coeff, n);
+                gp(R.add_thr)[n] = fixed_to_float(total, P.inv_scale) * pf(R.coeff, n);
             }
         }
         bool passed = s.spiked;  // a pass over every working synapse recalculated the weights
@@ -2162,16 +2315,16 @@ __global__ __launch_bounds__(kTrainBlock) void k_train(const NetDev *__restrict_
             const bool time_ok = (t - s.lss) <= (uint64_t)pu(R.dpt, n);
             const float rc = (float)(s.dop * (double)stability_factor(s.stab));
             // without time_ok the pass only recalculates weights, which changes none once they are current
-            const bool needed = time_ok || !R.w_current[n];
+            const bool needed = time_ok || !gp(R.w_current)[n];
             passed |= needed;
             uint32_t contrib = 0;
             for (uint32_t g = wv; g < G && needed; g += W) {
                 uint32_t j;
                 uint64_t base, end;
                 if (!chunk(g, j, base, end)) break;
+                const TrainCol C = train_col(net->projs[R.work[j]], n);
                 const uint64_t k = base + lane;
-                contrib += (uint32_t)__popcll(
-                    __ballot(k < end && dopamine_synapse(net->projs[R.work[j]], n, k, time_ok, rc)));
+                contrib += (uint32_t)__popcll(__ballot(k < end && dopamine_synapse(C, k, time_ok, rc)));
             }
             if (lane == 0) s_cnt[wv] = contrib;
             __syncthreads();
@@ -2194,15 +2347,16 @@ __global__ __launch_bounds__(kTrainBlock) void k_train(const NetDev *__restrict_
                 uint32_t j;
                 uint64_t base, end;
                 if (!chunk(g, j, base, end)) break;
+                const TrainCol C = train_col(net->projs[R.work[j]], n);
                 const uint64_t k = base + lane;
-                if (k < end) renorm_synapse(net->projs[R.work[j]], n, k, add);
+                if (k < end) renorm_synapse(C, k, add);
             }
             s.freer = 0.f;
             passed = true;
         }
         if (threadIdx.x == 0) {
             train_end(R, n, s);
-            if (passed) R.w_current[n] = 1;
+            if (passed) gp(R.w_current)[n] = 1;
         }
     }
 }

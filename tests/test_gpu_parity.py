@@ -706,6 +706,45 @@ def test_stdp_learning_short(train):
     assert np.array_equal(np.asarray(wg, np.float32), np.asarray(wo, np.float32))
 
 
+@pytest.mark.parametrize("train", list(TRAIN_FORMS))
+@pytest.mark.parametrize("rows", ["shared", "mixed"])
+def test_stdp_loaded_last_spike_steps(train, rows):
+    """Synapses loaded with last_spike_step_ values that fall inside the first steps' contribution windows:
+    shared by every synapse of a presynaptic row (the row record starts from them), or differing inside some
+    rows (each synapse keeps its own value until its row spikes). Config-2 shape at 200 inputs x 256 neurons,
+    600 steps: spikes, weights and every rule's state identical to the oracle."""
+    import ctypes
+
+    say(f"case: stdp_learning with loaded last spike steps ({rows} by row), a {train} per candidate")
+    spec = workloads.stdp_learning(n_images=30, n_in=200, n_out=256)
+    op = next(x for x in spec.ops if x[0] == "add_resource_stdp_projection")
+    pre, rules = op[1][3], op[1][7]
+    raw = np.frombuffer(rules, dtype=np.uint8).reshape(len(pre), ctypes.sizeof(abi.ResourceRule))
+    lss = 40 + (pre.astype(np.uint64) * 7) % 29
+    if rows == "mixed":
+        lss[::5] += 3
+    off = abi.ResourceRule.last_spike_step.offset
+    raw[:, off:off + 8] = lss.view(np.uint8).reshape(-1, 8)
+    g, o = both(spec, train)
+    steps = 600
+    for x in (g, o):
+        x.record_spikes(0, steps)
+        x.step(steps)
+    sg, so = g.read_spike_record(0), o.read_spike_record(0)
+    say(f"  spikes gpu={sg.sum()} oracle={so.sum()}")
+    assert so.sum() > 100
+    assert np.array_equal(sg, so)
+    R = abi.ResourceRule
+    dt = np.dtype({"names": ["r", "lss", "hh", "hc"], "formats": [np.float32, np.uint64, np.uint8, np.uint8],
+                   "offsets": [R.synaptic_resource.offset, R.last_spike_step.offset, R.had_hebbian_update.offset,
+                               R.has_contributed.offset], "itemsize": ctypes.sizeof(R)})
+    rg, ro = (np.frombuffer(x.resource_rules(0), dtype=dt) for x in (g, o))
+    for f in dt.names:
+        assert np.array_equal(rg[f], ro[f]), f
+    assert np.array_equal(np.asarray(g.weights(0), np.float32), np.asarray(o.weights(0), np.float32))
+    g.close()
+
+
 def test_plain_gpu_scale_prefix():
     """Config 1 at full size (100,000 neurons, 1e8 synapses), first 40 steps: spikes bit-exact."""
     say("case: plain_gpu_scale 40 steps (full size)")
