@@ -56,7 +56,7 @@ struct ParamI64 {
 
 // Synaptic-resource neuron state and parameters (neuron_parameters<SynapticResourceSTDPNeuron<...>>).
 struct ResDev {
-    float *add_thr;      // additional_threshold_ (state written by training)
+    double *add_thr;     // additional_threshold_ (written by training)
     float *free_res;     // free_synaptic_resource_
     float *stability;    // stability_
     uint8_t *isi_status; // ISIPeriodType

@@ -889,9 +889,10 @@ int init_network(knp_net *net) {
             std::vector<float> f(n);
             std::vector<uint64_t> u64(n);
             std::vector<uint8_t> u8(n);
+            std::vector<double> add(n);
             for (uint32_t i = 0; i < n; ++i)
-                f[i] = (float)(H.model == MODEL_ALTAI ? H.na[i].additional_threshold : H.nb[i].additional_threshold);
-            if ((rc = upload(net, &R.add_thr, f.data(), n))) return rc;
+                add[i] = H.model == MODEL_ALTAI ? H.na[i].additional_threshold : H.nb[i].additional_threshold;
+            if ((rc = upload(net, &R.add_thr, add.data(), n))) return rc;
             for (uint32_t i = 0; i < n; ++i) f[i] = nr[i].free_synaptic_resource;
             if ((rc = upload(net, &R.free_res, f.data(), n))) return rc;
             for (uint32_t i = 0; i < n; ++i) f[i] = nr[i].stability;
@@ -2999,7 +3000,7 @@ int knp_mi355x_read_blifat_neurons(knp_net *net, uint32_t pop, knp_blifat_neuron
     std::vector<knp_blifat_neuron> v;
     if ((rc = export_blifat(net, pop, v))) return rc;
     if (net->pops[pop].resource) {
-        std::vector<float> add(net->pops[pop].n);
+        std::vector<double> add(net->pops[pop].n);
         if ((rc = download(net, add.data(), net->table.pops[pop].res.add_thr, add.size()))) return rc;
         for (uint32_t i = 0; i < v.size(); ++i) v[i].additional_threshold = add[i];
     }
@@ -3039,7 +3040,7 @@ int knp_mi355x_read_altai_lif_neurons(knp_net *net, uint32_t pop, knp_altai_lif_
     if ((rc = download(net, Vpre.data(), P.Vpre, n))) return rc;
     if ((rc = download(net, dop.data(), P.dop, n))) return rc;
     if ((rc = download(net, at.data(), P.tbp, n))) return rc;
-    std::vector<float> add;
+    std::vector<double> add;
     if (net->pops[pop].resource) {
         add.resize(n);
         if ((rc = download(net, add.data(), P.res.add_thr, n))) return rc;

@@ -70,12 +70,18 @@ __device__ __forceinline__ void two_sum(float x, float y, float &s, float &e) {
     e = (x - (s - bb)) + (y - bb);
 }
 
-// A threshold with a dynamic part, hi + lo + x1 + x2, as a float-float pair s + l (about 48 bits).
-__device__ __forceinline__ void threshold_ff(float hi, float lo, float x1, float x2, float &s, float &l) {
+// A threshold with a dynamic part, hi + lo + x1 + x2 + x2_lo, as a float-float pair s + l (about 48 bits).
+__device__ __forceinline__ void threshold_ff(float hi, float lo, float x1, float x2, float x2_lo, float &s, float &l) {
     float s1, e1, e2;
     two_sum(hi, x1, s1, e1);
     two_sum(s1, x2, s, e2);
-    l = lo + (e1 + e2);
+    l = lo + (e1 + e2) + x2_lo;
+}
+
+// d as a float pair hi + lo (about 48 bits).
+__device__ __forceinline__ void split_double(double d, float &hi, float &lo) {
+    hi = (float)d;
+    lo = (float)(d - (double)hi);
 }
 
 // Does an exact accumulator sum a (on the 2^-frac_bits grid, |a| < 2^60) reach the threshold s + l?
@@ -320,7 +326,12 @@ __device__ __forceinline__ bool population_update(const NetDev *__restrict__ net
             // plain: the smallest float not below activation + additional threshold (exact for V >= T);
             // resource: activation_threshold_ + the trained threshold, as a float-float sum
             float ts = pf(P.act, i), tl = 0.f;
-            if (KIND == POP_RESOURCE) two_sum(ts, P.res.add_thr[i], ts, tl);
+            if (KIND == POP_RESOURCE) {
+                float add, add_lo;
+                split_double(gp(P.res.add_thr)[i], add, add_lo);
+                two_sum(ts, add, ts, tl);
+                tl += add_lo;
+            }
             if (KIND == POP_RESOURCE ? float_ge_ff(V, ts, tl) : V >= ts) {
                 spike = at > 0;
                 if (fl & kAltDiff) {
@@ -427,9 +438,9 @@ __device__ __forceinline__ bool population_update(const NetDev *__restrict__ net
             if (KIND != POP_RESOURCE && dyn == 0.f) {
                 above = exact ? a >= pi64(P.thr_fix, i) : V >= pf(P.thr_up, i);
             } else {
-                float ts, tl;
-                threshold_ff(pf(P.thr_hi, i), pf(P.thr_lo, i), dyn, KIND == POP_RESOURCE ? P.res.add_thr[i] : 0.f,
-                             ts, tl);
+                float ts, tl, add = 0.f, add_lo = 0.f;
+                if (KIND == POP_RESOURCE) split_double(gp(P.res.add_thr)[i], add, add_lo);
+                threshold_ff(pf(P.thr_hi, i), pf(P.thr_lo, i), dyn, add, add_lo, ts, tl);
                 above = exact ? fixed_ge_ff(a, ts, tl, P.scale) : float_ge_ff(V, ts, tl);
             }
             spike = (n_ts > pu(P.refr, i)) && above;
@@ -1791,6 +1802,12 @@ __device__ __forceinline__ bool renorm_runs(const ResDev &R, uint32_t n, const T
     return !in_isi && fabsf(s.freer) >= pf(R.res_thr, n);
 }
 
+// additional_threshold_ = (the sum of the positive weights, in double) * synapse_sum_threshold_coefficient_
+// (blifat_stdp.h:53-67): the exact sum times the coefficient, rounded once in double as the reference rounds it.
+__device__ __forceinline__ void store_add_thr(const PopDev &P, uint32_t n, long long sum) {
+    gp(P.res.add_thr)[n] = (double)sum * (double)P.inv_scale * (double)pf(P.res.coeff, n);
+}
+
 __device__ __forceinline__ void train_end(const ResDev &R, uint32_t n, const TrainNeuron &s) {
     gp(R.isi_status)[n] = (uint8_t)s.status;
     gp(R.last_step)[n] = s.last_step;
@@ -2063,7 +2080,7 @@ __device__ __forceinline__ void spiking_column_fast(const TrainCol &C, TrainNeur
             const uint64_t k = base + 64 * u + lane;
             const bool act = k < C.ce;
             SpikingSyn x;
-            x.w = cur.w[u];
+            x.w = act ? cur.w[u] : 0.f;  // (a lane past the end adds nothing to the weight sum)
             x.r = cur.r[u];
             x.fl = cur.fl[u];
             x.per = C.u_period;
@@ -2082,6 +2099,59 @@ __device__ __forceinline__ void spiking_column_fast(const TrainCol &C, TrainNeur
             if (base + 64 * u >= C.ce) break;
             const uint64_t k = base + 64 * u + lane;
             spiking_store(C, (uint32_t)k, (uint32_t)((int64_t)k + C.pre_off), o[u]);
+        }
+        cur = nxt;
+    }
+}
+
+// What the dopamine and renormalization passes load of U chunks of a fast column.
+template <uint32_t U>
+struct FastWRF {
+    float w[U], r[U];
+    uint8_t fl[U];
+};
+
+template <uint32_t U>
+__device__ __forceinline__ void wrf_load(const TrainCol &C, uint64_t base, uint32_t lane, FastWRF<U> &g) {
+#pragma unroll
+    for (uint32_t u = 0; u < U; ++u) {
+        uint64_t k = base + 64 * u + lane;
+        k = k < C.ce ? k : C.ce - 1;  // lanes past the column's end load its last synapse and store nothing
+        g.w[u] = C.W[k];
+        g.r[u] = C.r[k];
+        g.fl[u] = C.flags[k];
+    }
+}
+
+// A pass over a fast column that changes resources and weights (dopamine, renormalization), pipelined as the
+// spiking pass: update(act, w, r, fl, &r_new) gives the synapse's new resource; the weight is its recalculation.
+// Every lane calls update for every chunk, in synapse order.
+template <uint32_t U, typename F>
+__device__ __forceinline__ void resource_column_fast(const TrainCol &C, uint32_t lane, F update) {
+    constexpr uint64_t kStep = 64 * U;
+    FastWRF<U> cur, nxt;
+    wrf_load<U>(C, C.cb, lane, cur);
+    nxt = cur;
+    for (uint64_t base = C.cb; base < C.ce; base += kStep) {
+        if (base + kStep < C.ce) wrf_load<U>(C, base + kStep, lane, nxt);
+        float rn[U], wn[U];
+        bool st_r[U], st_w[U];
+#pragma unroll
+        for (uint32_t u = 0; u < U; ++u) {
+            if (base + 64 * u >= C.ce) break;
+            const bool act = base + 64 * u + lane < C.ce;
+            rn[u] = update(act, cur.w[u], cur.r[u], cur.fl[u]);
+            wn[u] = recalc_weight(rn[u], C.u_wmin, C.u_wmax);
+            st_r[u] = act && __float_as_uint(rn[u]) != __float_as_uint(cur.r[u]);
+            st_w[u] = act && __float_as_uint(wn[u]) != __float_as_uint(cur.w[u]);
+        }
+        __builtin_amdgcn_s_waitcnt(0x0F70);  // the next chunks' loads complete before this group's stores
+#pragma unroll
+        for (uint32_t u = 0; u < U; ++u) {
+            if (base + 64 * u >= C.ce) break;
+            const uint64_t k = base + 64 * u + lane;
+            if (st_r[u]) C.r[k] = rn[u];
+            if (st_w[u]) C.store_weight((uint32_t)k, (uint32_t)((int64_t)k + C.pre_off), wn[u]);
         }
         cur = nxt;
     }
@@ -2136,7 +2206,7 @@ __device__ __forceinline__ void train_candidate_wave(const NetDev *__restrict__ 
             }
         }
         sum = wave_sum_i64(sum);
-        if (lane == 0) gp(R.add_thr)[n] = fixed_to_float(sum, P.inv_scale) * pf(R.coeff, n);
+        if (lane == 0) store_add_thr(P, n, sum);
     }
     bool passed = s.spiked;  // a pass over every working synapse recalculated the weights from the resources
     if (dopamine_runs(s)) {
@@ -2148,10 +2218,20 @@ __device__ __forceinline__ void train_candidate_wave(const NetDev *__restrict__ 
         passed |= needed;
         for (uint32_t j = 0; j < R.n_work && needed; ++j) {
             const TrainCol C = train_col(net->projs[R.work[j]], n);
+            if (C.fast()) {
+                uint64_t contrib = 0;
+                resource_column_fast<kTrainFastUnroll>(C, lane, [&](bool act, float, float r, uint8_t fl) {
+                    const bool c = time_ok && (fl & 2);
+                    contrib += (uint64_t)__popcll(__ballot(act && c));
+                    return c ? r + rc : r;
+                });
+                // free -= change once per contributing synapse: the same float operation each time
+                s.freer = repeat_add(s.freer, -rc, contrib);
+                continue;
+            }
             for (uint64_t base = C.cb; base < C.ce; base += 64) {
                 const uint64_t k = base + lane;
                 const uint64_t cm = __ballot(k < C.ce && dopamine_synapse(C, k, time_ok, rc));
-                // free -= change once per contributing synapse: the same float operation each time
                 s.freer = repeat_add(s.freer, -rc, (uint64_t)__popcll(cm));
             }
         }
@@ -2166,6 +2246,11 @@ __device__ __forceinline__ void train_candidate_wave(const NetDev *__restrict__ 
         const float add = s.freer / (float)(k_total + (uint64_t)pu(R.drain, n));
         for (uint32_t j = 0; j < R.n_work; ++j) {
             const TrainCol C = train_col(net->projs[R.work[j]], n);
+            if (C.fast()) {
+                resource_column_fast<kTrainFastUnroll>(C, lane,
+                                                       [&](bool, float, float r, uint8_t) { return r + add; });
+                continue;
+            }
             for (uint64_t k = C.cb + lane; k < C.ce; k += 64 * kTrainUnroll) {
                 float r0[kTrainUnroll], w0[kTrainUnroll];
 #pragma unroll
@@ -2307,7 +2392,7 @@ __global__ __launch_bounds__(kTrainBlock) void k_train(const NetDev *__restrict_
             if (threadIdx.x == 0) {
                 long long total = 0;
                 for (uint32_t w = 0; w < W; ++w) total += s_sum[w];
-                gp(R.add_thr)[n] = fixed_to_float(total, P.inv_scale) * pf(R.coeff, n);
+                store_add_thr(P, n, total);
             }
         }
         bool passed = s.spiked;  // a pass over every working synapse recalculated the weights

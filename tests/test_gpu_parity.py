@@ -745,6 +745,47 @@ def test_stdp_loaded_last_spike_steps(train, rows):
     g.close()
 
 
+@pytest.mark.parametrize("train", list(TRAIN_FORMS))
+def test_stdp_thresholds_renormalization_stability(train):
+    """Config-2 shape at 200 inputs x 256 neurons with the neuron rules the configuration leaves at their
+    defaults switched on: the weight sum raises the threshold (synapse_sum_threshold_coefficient_ 0.02), the
+    free resource is renormalized (threshold 1.5, drain coefficient 3) and stability changes (0.05 by dopamine,
+    0.02 at ISI starts). 600 steps: spikes, weights, rule states and neuron states identical to the oracle."""
+    import ctypes
+
+    say(f"case: stdp_learning with threshold coefficient, renormalization and stability, a {train} per candidate")
+    spec = workloads.stdp_learning(n_images=30, n_in=200, n_out=256)
+    neurons = next(x for x in spec.ops if x[0] == "add_resource_blifat_population")[1][0]
+    for nr in neurons:
+        nr.synapse_sum_threshold_coefficient = 0.02
+        nr.synaptic_resource_threshold = 1.5
+        nr.resource_drain_coefficient = 3
+        nr.stability_change_parameter = 0.05
+        nr.stability_change_at_isi = 0.02
+    g, o = both(spec, train)
+    steps = 600
+    for x in (g, o):
+        x.record_spikes(0, steps)
+        x.step(steps)
+    sg, so = g.read_spike_record(0), o.read_spike_record(0)
+    say(f"  spikes gpu={sg.sum()} oracle={so.sum()}")
+    assert so.sum() > 100
+    assert np.array_equal(sg, so)
+    assert np.array_equal(np.asarray(g.weights(0), np.float32), np.asarray(o.weights(0), np.float32))
+    R = abi.ResourceRule
+    dt = np.dtype({"names": ["r", "hh", "hc"], "formats": [np.float32, np.uint8, np.uint8],
+                   "offsets": [R.synaptic_resource.offset, R.had_hebbian_update.offset, R.has_contributed.offset],
+                   "itemsize": ctypes.sizeof(R)})
+    rg, ro = (np.frombuffer(x.resource_rules(0), dtype=dt) for x in (g, o))
+    for f in dt.names:
+        assert np.array_equal(rg[f], ro[f]), f
+    ng, no = g.resource_neurons(0), o.resource_neurons(0)
+    for f in ("free_synaptic_resource", "stability", "isi_status", "last_step", "last_spike_step", "first_isi_spike"):
+        assert [getattr(x, f) for x in ng] == [getattr(x, f) for x in no], f
+    assert [x.base.additional_threshold for x in ng] == [np.float32(x.base.additional_threshold) for x in no]
+    g.close()
+
+
 def test_plain_gpu_scale_prefix():
     """Config 1 at full size (100,000 neurons, 1e8 synapses), first 40 steps: spikes bit-exact."""
     say("case: plain_gpu_scale 40 steps (full size)")
