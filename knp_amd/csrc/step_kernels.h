@@ -198,9 +198,9 @@ __device__ __forceinline__ void block_append(bool pred, uint32_t value, uint32_t
         if (k < w) off += wcnt[k];
         total += wcnt[k];
     }
-    if (threadIdx.x == 0 && total) base = atomicAdd(counter, total);
+    if (threadIdx.x == 0 && total) base = atomicAdd((uint32_t *)gp(counter), total);
     __syncthreads();
-    if (pred) list[base + off + __popcll(mask & ((1ull << lane) - 1ull))] = value;
+    if (pred) gp(list)[base + off + __popcll(mask & ((1ull << lane) - 1ull))] = value;
     __syncthreads();  // wcnt and base are free for the next call
 }
 
@@ -528,17 +528,17 @@ __global__ __launch_bounds__(kBlock) void k_phase1(const NetDev *__restrict__ ne
     Ctl *ctl = &net->ctl[t & 1];
     uint32_t *seg_count = net->seg_count + (size_t)(t & 1) * net->n_seg;
     if (blockIdx.x == 0) {
-        uint32_t *nxt = (uint32_t *)&net->ctl[(t + 1) & 1];
+        KNP_GLOBAL uint32_t *nxt = gp((uint32_t *)&net->ctl[(t + 1) & 1]);
         for (uint32_t k = threadIdx.x; k < sizeof(Ctl) / 4; k += blockDim.x) nxt[k] = 0;
         // next step's counts: host-pushed segments must start empty (every other segment is rewritten)
-        uint32_t *nc = net->seg_count + (size_t)((t + 1) & 1) * net->n_seg;
-        uint32_t *nv = net->vcount + (size_t)((t + 1) & 1) * net->n_vseg;
+        KNP_GLOBAL uint32_t *nc = gp(net->seg_count) + (size_t)((t + 1) & 1) * net->n_seg;
+        KNP_GLOBAL uint32_t *nv = gp(net->vcount) + (size_t)((t + 1) & 1) * net->n_vseg;
         for (uint32_t k = threadIdx.x; k < net->n_projs; k += blockDim.x) {
             const uint32_t ps = net->projs[k].push_seg;
             nc[ps] = 0;
-            net->projs[k].in_list[(size_t)((t + 1) & 1) * net->list_pstride] = 0;  // segment count word
+            gp(net->projs[k].in_list)[(size_t)((t + 1) & 1) * net->list_pstride] = 0;  // segment count word
             for (uint32_t j = 0; j < 4; ++j) {
-                const uint32_t vp = net->seg_vpos[4 * ps + j];
+                const uint32_t vp = gp(net->seg_vpos)[4 * ps + j];
                 if (vp != 0xFFFFFFFFu) nv[vp] = 0;
             }
         }
@@ -571,36 +571,63 @@ __global__ __launch_bounds__(kBlock) void k_phase1(const NetDev *__restrict__ ne
 // Exact accumulation: a 64-bit integer atomic per impact (non-returning). Splitting it into two 32-bit
 // atomics on the same 8 bytes measured slower: device-wide atomics are executed past the L2, one transaction
 // each, so the count of atomics, not their width, sets the cost.
-__device__ __forceinline__ void atomic_add_i64(int64_t *p, long long v) {
-    atomicAdd(reinterpret_cast<unsigned long long *>(p), (unsigned long long)v);
+__device__ __forceinline__ void atomic_add_i64(KNP_GLOBAL int64_t *p, long long v) {
+    atomicAdd((unsigned long long *)p, (unsigned long long)v);
 }
 
-__device__ __forceinline__ void impact(const PopDev &P, const ProjDev &Q, const RowSource &R, uint32_t slot0,
-                                       uint32_t pre, uint64_t pos, uint32_t pt, uint32_t d, float w) {
+// Where one row's impacts go, copied out of the tables once per row (global-memory pointers: no flat atomics, and
+// no reload of the tables after every atomic).
+struct ImpactDst {
+    KNP_GLOBAL int64_t *acc_v, *acc_g, *acc_d;
+    KNP_GLOBAL uint32_t *acc_f;
+    KNP_GLOBAL uint64_t *acc_b;
+    uint32_t D, n, slot0;
+    float scale;
+    bool forcing;
+    uint64_t key_hi;  // BLOCKING key bits of the projection and message rank
+};
+
+__device__ __forceinline__ ImpactDst impact_dst(const PopDev &P, const ProjDev &Q, uint32_t rank, uint64_t t) {
+    ImpactDst x;
+    x.acc_v = gp(P.acc_v);
+    x.acc_g = gp(P.acc_g);
+    x.acc_d = gp(P.acc_d);
+    x.acc_f = gp(P.acc_f);
+    x.acc_b = gp(P.acc_b);
+    x.D = P.D;
+    x.n = P.n;
+    x.slot0 = (uint32_t)(t % P.D);
+    x.scale = P.scale;
+    x.forcing = Q.forcing && P.acc_f;
+    x.key_hi = ((uint64_t)Q.blocking_id << 57) | ((uint64_t)rank << 46);
+    return x;
+}
+
+__device__ __forceinline__ void impact(const ImpactDst &X, uint32_t pre, uint64_t pos, uint32_t pt, uint32_t d,
+                                       float w) {
     const uint32_t post = pt & 0x1FFFFFFFu;
     const uint32_t type = pt >> 29;
-    uint32_t slot = slot0 + d;
-    if (slot >= P.D) slot -= P.D;
-    const size_t ai = (size_t)slot * P.n + post;
-    const long long v = __float2ll_rn(w * P.scale);
+    uint32_t slot = X.slot0 + d;
+    if (slot >= X.D) slot -= X.D;
+    const size_t ai = (size_t)slot * X.n + post;
+    const long long v = __float2ll_rn(w * X.scale);
     switch (type) {
         case 0:  // EXCITATORY
-            atomic_add_i64(&P.acc_v[ai], v);
-            if (Q.forcing && P.acc_f) atomicOr(&P.acc_f[ai], 1u);
+            atomic_add_i64(X.acc_v + ai, v);
+            if (X.forcing) atomicOr((uint32_t *)(X.acc_f + ai), 1u);
             break;
         case 1:  // INHIBITORY_CURRENT
-            atomic_add_i64(&P.acc_v[ai], -v);
+            atomic_add_i64(X.acc_v + ai, -v);
             break;
         case 2:  // INHIBITORY_CONDUCTANCE
-            atomic_add_i64(&P.acc_g[ai], v);
+            atomic_add_i64(X.acc_g + ai, v);
             break;
         case 3:  // DOPAMINE
-            atomic_add_i64(&P.acc_d[ai], v);
+            atomic_add_i64(X.acc_d + ai, v);
             break;
         default: {  // BLOCKING
-            const uint64_t key = 1ull + (((uint64_t)Q.blocking_id << 57) | ((uint64_t)(64u - d) << 51) |
-                                         ((uint64_t)R.rank << 46) | ((uint64_t)pre << 17) | pos);
-            atomicMax((unsigned long long *)&P.acc_b[ai], (unsigned long long)key);
+            const uint64_t key = 1ull + (X.key_hi | ((uint64_t)(64u - d) << 51) | ((uint64_t)pre << 17) | pos);
+            atomicMax((unsigned long long *)(X.acc_b + ai), (unsigned long long)key);
             break;
         }
     }
@@ -611,10 +638,10 @@ __global__ __launch_bounds__(kBlock) void k_scatter(const NetDev *__restrict__ n
     extern __shared__ uint32_t pfx[];  // [n_vseg + 1]
     __shared__ uint32_t wsum[kBlock / 64];
     const uint32_t nv = net->n_vseg_atomic;
-    const uint32_t *vc = net->vcount + (size_t)(t & 1) * net->n_vseg;
+    const KNP_GLOBAL uint32_t *vc = gp(net->vcount) + (size_t)(t & 1) * net->n_vseg;
     const uint32_t lane = lane_id(), w = threadIdx.x / 64;
     for (uint32_t v = threadIdx.x; v < nv; v += kBlock) pfx[v] = vc[v];
-    const uint32_t *vkey = net->vkey;
+    const KNP_GLOBAL uint32_t *vkey = gp(net->vkey);
     if (vkey) {
         // the multi-threaded backend's semantics: each projection expands only its first message of the step,
         // the nonempty one of smallest key (every workgroup finds the same)
@@ -659,9 +686,9 @@ __global__ __launch_bounds__(kBlock) void k_scatter(const NetDev *__restrict__ n
             if (pfx[mid] <= row) lo = mid;
             else hi = mid;
         }
-        const uint32_t seg = net->vseg[lo];
-        const RowSource &R = net->rows[net->vrs[lo]];
-        uint32_t pre = net->seg_list[seg][(size_t)(t & 1) * net->list_pstride + 1 + row - pfx[lo]];
+        const uint32_t seg = gp(net->vseg)[lo];
+        const KNP_GLOBAL RowSource *R = gp(net->rows) + gp(net->vrs)[lo];
+        uint32_t pre = gp(net->seg_list[seg])[(size_t)(t & 1) * net->list_pstride + 1 + row - pfx[lo]];
         // pushed entries under the multi-threaded semantics: the index's count in the message, which multiplies
         // the weight in float (delta_impl.h:144)
         float mult = 1.f;
@@ -669,14 +696,18 @@ __global__ __launch_bounds__(kBlock) void k_scatter(const NetDev *__restrict__ n
             mult = (float)(pre >> kMsgCountShift);
             pre &= kMsgIndexMask;
         }
-        const ProjDev &Q = net->projs[R.proj];
+        const ProjDev &Q = net->projs[R->proj];
         if (pre >= Q.n_pre) continue;
-        const uint64_t b = Q.row_ptr[pre], e = Q.row_ptr[pre + 1];
+        const uint64_t b = gp(Q.row_ptr)[pre], e = gp(Q.row_ptr)[pre + 1];
         my_ev += e - b;
         my_rows += 1;
-        const PopDev &P = net->pops[Q.post_pop];
-        const uint32_t slot0 = (uint32_t)(t % P.D);
-        if (Q.kind == PROJ_RESOURCE && threadIdx.x == 0) Q.row_spike[pre] = t;  // init_synapse of the row
+        const ImpactDst X = impact_dst(net->pops[Q.post_pop], Q, R->rank, t);
+        const KNP_GLOBAL uint32_t *post_type = gp(Q.post_type);
+        const KNP_GLOBAL uint8_t *delay = gp(Q.delay);
+        const KNP_GLOBAL float *w_csr = gp(Q.w_csr), *W = gp((const float *)Q.W);
+        const KNP_GLOBAL uint32_t *csr_syn = gp(Q.csr_syn);
+        const bool delta = Q.kind == PROJ_DELTA;
+        if (Q.kind == PROJ_RESOURCE && threadIdx.x == 0) gp(Q.row_spike)[pre] = t;  // init_synapse of the row
         for (uint64_t base = b; base < e; base += 4 * kBlock) {
             uint32_t pt[4], d[4];
             float wv[4];
@@ -685,25 +716,18 @@ __global__ __launch_bounds__(kBlock) void k_scatter(const NetDev *__restrict__ n
                 const uint64_t k = base + threadIdx.x + (uint64_t)j * kBlock;
                 ok[j] = k < e;
                 if (ok[j]) {
-                    pt[j] = Q.post_type[k];
-                    d[j] = Q.delay[k];
-                    if (Q.kind == PROJ_DELTA) {
-                        wv[j] = Q.w_csr[k];
-                    } else {
-                        const uint32_t s = Q.csr_syn[k];
-                        wv[j] = Q.W[s];
-
-                    }
+                    pt[j] = post_type[k];
+                    d[j] = delay[k];
+                    wv[j] = delta ? w_csr[k] : W[csr_syn[k]];
                 }
             }
             for (int j = 0; j < 4; ++j)
-                if (ok[j])
-                    impact(P, Q, R, slot0, pre, base + threadIdx.x + (uint64_t)j * kBlock - b, pt[j], d[j], wv[j] * mult);
+                if (ok[j]) impact(X, pre, base + threadIdx.x + (uint64_t)j * kBlock - b, pt[j], d[j], wv[j] * mult);
         }
     }
     if (threadIdx.x == 0) {
-        net->part[2 * blockIdx.x] += my_ev;
-        net->part[2 * blockIdx.x + 1] += my_rows;
+        gp(net->part)[2 * blockIdx.x] += my_ev;
+        gp(net->part)[2 * blockIdx.x + 1] += my_rows;
     }
 }
 
@@ -2812,7 +2836,7 @@ __device__ __forceinline__ float dense_tile_sum(const ProjDev &Q, const DenseTil
 __global__ __launch_bounds__(kBlock) void k_dense_expand(const NetDev *__restrict__ net, DenseGroup grp, uint64_t t) {
     t = resolve_step(net, t);
     const ProjDev &Q0 = net->projs[grp.proj[0]];
-    const uint32_t tile = blockIdx.x * (kBlock / 64) + threadIdx.x / 64;
+    const uint32_t tile = wave_uniform(blockIdx.x * (kBlock / 64) + threadIdx.x / 64);
     if (tile >= Q0.dn_tiles) return;
     const PopDev &P = net->pops[Q0.post_pop];
     const uint32_t lane = lane_id(), h = lane >> 5, col = lane & 31;
