@@ -203,6 +203,8 @@ struct ProjDev {
     uint32_t dense;              // 0: expanded by the scatter; else 1 + kDense* flags
     uint32_t dn_oct;             // octets (groups of 8 presynaptic indexes) over all instances
     uint32_t dn_inst, dn_tiles;  // instances, 32-neuron tiles over all instances
+    uint32_t dn_max_oct;         // octets of the largest instance
+    uint32_t dn_max_npad;        // the largest instance's padded neuron count
     uint32_t dn_delay;           // the projection's delay
     float dn_sign;               // +1 EXCITATORY, -1 INHIBITORY_CURRENT
     uint32_t dn_v0, dn_v1;       // the projection's virtual segments [dn_v0, dn_v1)

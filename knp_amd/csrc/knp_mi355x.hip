@@ -1321,6 +1321,11 @@ int init_network(knp_net *net) {
             if ((rc = dev_alloc(net, (void **)&Q.dn_part, (size_t)kDenseCountBlocks * 4 * 8))) return rc;
             Q.dense = 1 + H.dense_flags;
             Q.dn_oct = B.n_oct;
+            Q.dn_max_oct = Q.dn_max_npad = 0;
+            for (const DenseInst &I : B.inst) {
+                Q.dn_max_oct = std::max(Q.dn_max_oct, I.n_oct);
+                Q.dn_max_npad = std::max(Q.dn_max_npad, I.npad);
+            }
             Q.dn_inst = (uint32_t)B.inst.size();
             Q.dn_tiles = (uint32_t)tiles.size();
             Q.dn_delay = B.delay;
@@ -2021,8 +2026,13 @@ int run_steps(knp_net *net, uint64_t n_steps, uint32_t *counts = nullptr, bool t
                 ++grp.n;
             }
             const uint32_t ge = (net->table.projs[members[0]].dn_tiles + 3) / 4;
+            // every instance's count words fit the wave's LDS buffer, and its weights 32-bit offsets
+            bool staged = true;
+            for (uint32_t q : members) staged = staged && net->table.projs[q].dn_max_oct <= kDenseStagedOct &&
+                                                net->table.projs[q].dn_max_npad <= (1u << 22);
             if ((rc = timed_launch(net, KNP_KERNEL_DENSE_EXPAND, [&] {
-                     k_dense_expand<<<ge, kBlock, 0, net->stream>>>(D, grp, targ);
+                     if (staged) k_dense_expand<true><<<ge, kBlock, 0, net->stream>>>(D, grp, targ);
+                     else k_dense_expand<false><<<ge, kBlock, 0, net->stream>>>(D, grp, targ);
                  })))
                 return rc;
         }

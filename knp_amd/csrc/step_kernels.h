@@ -2828,35 +2828,121 @@ __device__ __forceinline__ float dense_tile_sum(const ProjDev &Q, const DenseTil
     return SPLIT ? acc[0] + acc_lo[0] : acc[0];
 }
 
+// dense_tile_sum for an instance of at most kDenseStagedOct octets whose count words the wave has staged in LDS
+// (cw): the octets holding a spike are taken eight to a round across the whole instance (no round is cut short
+// at a 64-octet boundary, and no count word is loaded in the loop).
+constexpr uint32_t kDenseStagedOct = 128;
+
+template <bool SKIP, bool SPLIT>
+__device__ __forceinline__ float dense_tile_sum_staged(const ProjDev &Q, const DenseTile &T, const uint64_t *cw,
+                                                       uint8_t *list, uint32_t lane, uint32_t &read_oct) {
+    const uint32_t h = lane >> 5, col = lane & 31, n_oct = T.n_oct;
+    // the instance's weights from one base (the instance holds fewer than 2^32 of them): 32-bit lane offsets
+    const KNP_GLOBAL uint16_t *wb = gp(Q.dn_w) + T.base;
+    const KNP_GLOBAL uint16_t *wl = SPLIT ? gp(Q.dn_w_lo) + T.base : nullptr;
+    const uint32_t off0 = (T.j0 + col) * 8, ostride = T.npad * 8;
+    // the octets to multiply, in ascending order, listed in LDS (each lane places its own)
+    const bool s0 = SKIP ? cw[lane] != 0ull : lane < n_oct;
+    const bool s1 = SKIP ? cw[64 + lane] != 0ull : 64 + lane < n_oct;
+    const uint64_t m0 = __ballot(s0), m1 = __ballot(s1), below = (1ull << lane) - 1ull;
+    const uint32_t n0 = (uint32_t)__popcll(m0), total = n0 + (uint32_t)__popcll(m1);
+    if (s0) list[__popcll(m0 & below)] = (uint8_t)lane;
+    if (s1) list[n0 + __popcll(m1 & below)] = (uint8_t)(64 + lane);
+    __builtin_amdgcn_wave_barrier();
+    read_oct += total;
+    v16f acc, acc_lo;
+#pragma unroll
+    for (int v = 0; v < 16; ++v) acc[v] = acc_lo[v] = 0.f;
+    // rounds of eight octets: lanes 0-31 take the even places of the round, lanes 32-63 the odd ones; a lane
+    // without an octet loads octet 0 and multiplies it by zero counts (no branch around the loads)
+    for (uint32_t r = 0; r < total; r += 8) {
+        v8s frag[4], frag_lo[4];
+        uint64_t cv[4];
+        uint32_t my[4];
+#pragma unroll
+        for (int g = 0; g < 4; ++g) {
+            const uint32_t idx = r + 2 * g + h;
+            my[g] = list[idx < total ? idx : 0u];  // (total > 0 inside the loop)
+            if (idx >= total) my[g] = 0u;
+        }
+#pragma unroll
+        for (int g = 0; g < 4; ++g) {
+            const uint32_t off = off0 + my[g] * ostride;
+            frag[g] = *(const KNP_GLOBAL v8s *)(wb + off);
+            if (SPLIT) frag_lo[g] = *(const KNP_GLOBAL v8s *)(wl + off);
+        }
+#pragma unroll
+        for (int g = 0; g < 4; ++g) cv[g] = r + 2 * g + h < total ? cw[my[g]] : 0ull;
+#pragma unroll
+        for (int g = 0; g < 4; ++g) {
+            const v8s a = counts_to_bf16(cv[g]);
+            acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, frag[g], acc, 0, 0, 0);
+            if (SPLIT) acc_lo = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, frag_lo[g], acc_lo, 0, 0, 0);
+        }
+    }
+    return SPLIT ? acc[0] + acc_lo[0] : acc[0];
+}
+
 // One wave per 32-neuron tile for a group of MFMA-expanded projections into the same neurons with the same
 // delay (their instances cover the same postsynaptic ranges): every tile entry and first count word of the
 // group is loaded at once, then each projection's sum is taken in turn, converted to fixed point on its own
 // (as it would be alone) and added to the ring cell, which the lane owns (no other launch runs meanwhile) and
 // loaded at the start, so that the total is stored without a further round trip.
-__global__ __launch_bounds__(kBlock) void k_dense_expand(const NetDev *__restrict__ net, DenseGroup grp, uint64_t t) {
+// STAGED: every instance of the group has at most kDenseStagedOct octets (count words staged in LDS).
+template <bool STAGED>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 4))) void k_dense_expand(const NetDev *__restrict__ net, DenseGroup grp, uint64_t t) {
     t = resolve_step(net, t);
     const ProjDev &Q0 = net->projs[grp.proj[0]];
     const uint32_t tile = wave_uniform(blockIdx.x * (kBlock / 64) + threadIdx.x / 64);
     if (tile >= Q0.dn_tiles) return;
     const PopDev &P = net->pops[Q0.post_pop];
     const uint32_t lane = lane_id(), h = lane >> 5, col = lane & 31;
+    // the count words of the wave's instances (STAGED)
+    __shared__ uint64_t s_cw[STAGED ? kBlock / 64 : 1][kDenseGroupMax][kDenseStagedOct];
+    __shared__ uint8_t s_list[STAGED ? kBlock / 64 : 1][kDenseGroupMax][kDenseStagedOct];
+    uint64_t *cw_wave = &s_cw[STAGED ? threadIdx.x / 64 : 0][0][0];
+    uint8_t *list_wave = &s_list[STAGED ? threadIdx.x / 64 : 0][0][0];
     DenseTile T[kDenseGroupMax];
     uint64_t mine0[kDenseGroupMax];
 #pragma unroll
     for (uint32_t m = 0; m < kDenseGroupMax; ++m)
         if (m < grp.n) T[m] = net->projs[grp.proj[m]].dn_tile_info[tile];
+    if (STAGED) {
+        // every count word of the group loaded at once (lanes past an instance's octets load its last one),
+        // then staged in LDS
+        uint64_t w[kDenseGroupMax][2];
 #pragma unroll
-    for (uint32_t m = 0; m < kDenseGroupMax; ++m)
-        if (m < grp.n) {
-            const ProjDev &Q = net->projs[grp.proj[m]];
-            const KNP_GLOBAL uint64_t *cnt =
-                (const KNP_GLOBAL uint64_t *)gp(grp.cnt[m]) + (size_t)(t & 1) * Q.dn_oct + T[m].cbase;
-            mine0[m] = lane < T[m].n_oct ? cnt[lane] : 0ull;
-        }
+        for (uint32_t m = 0; m < kDenseGroupMax; ++m)
+            if (m < grp.n) {
+                const ProjDev &Q = net->projs[grp.proj[m]];
+                const KNP_GLOBAL uint64_t *cnt =
+                    (const KNP_GLOBAL uint64_t *)gp(grp.cnt[m]) + (size_t)(t & 1) * Q.dn_oct + T[m].cbase;
+                const uint32_t last = T[m].n_oct - 1;  // an instance has at least one octet
+                w[m][0] = cnt[lane < last ? lane : last];
+                w[m][1] = cnt[64 + lane < last ? 64 + lane : last];
+            }
+#pragma unroll
+        for (uint32_t m = 0; m < kDenseGroupMax; ++m)
+            if (m < grp.n) {
+                cw_wave[m * kDenseStagedOct + lane] = lane < T[m].n_oct ? w[m][0] : 0ull;
+                cw_wave[m * kDenseStagedOct + 64 + lane] = 64 + lane < T[m].n_oct ? w[m][1] : 0ull;
+            }
+        __builtin_amdgcn_wave_barrier();  // the wave's count words are in LDS before any lane reads another's
+    } else {
+#pragma unroll
+        for (uint32_t m = 0; m < kDenseGroupMax; ++m)
+            if (m < grp.n) {
+                const ProjDev &Q = net->projs[grp.proj[m]];
+                const KNP_GLOBAL uint64_t *cnt =
+                    (const KNP_GLOBAL uint64_t *)gp(grp.cnt[m]) + (size_t)(t & 1) * Q.dn_oct + T[m].cbase;
+                mine0[m] = lane < T[m].n_oct ? cnt[lane] : 0ull;
+            }
+    }
     const uint32_t j0 = T[0].j0;
     const bool writer = h == 0 && j0 + col < T[0].post_count;
-    const size_t ai = (size_t)((t + Q0.dn_delay) % P.D) * P.n + (T[0].post_begin + j0 + col);
-    int64_t total = writer ? gp(P.acc_v)[ai] : 0;
+    // the ring cell (lanes that write none load the tile's first one)
+    const size_t ai = (size_t)((t + Q0.dn_delay) % P.D) * P.n + (T[0].post_begin + j0 + (writer ? col : 0));
+    int64_t total = gp(P.acc_v)[ai];
 #pragma unroll
     for (uint32_t m = 0; m < kDenseGroupMax; ++m) {
         if (m >= grp.n) break;
@@ -2866,7 +2952,14 @@ __global__ __launch_bounds__(kBlock) void k_dense_expand(const NetDev *__restric
         const bool skip = (Q.dense - 1) & kDenseSkipEmpty, split = Q.dn_w_lo != nullptr;
         uint32_t read_oct = 0;  // octets whose weights this wave reads
         float sum;
-        if (skip && split) sum = dense_tile_sum<true, true>(Q, T[m], cnt, mine0[m], lane, read_oct);
+        const uint64_t *cw = cw_wave + m * kDenseStagedOct;
+        uint8_t *list = list_wave + m * kDenseStagedOct;
+        if (STAGED) {
+            if (skip && split) sum = dense_tile_sum_staged<true, true>(Q, T[m], cw, list, lane, read_oct);
+            else if (skip) sum = dense_tile_sum_staged<true, false>(Q, T[m], cw, list, lane, read_oct);
+            else if (split) sum = dense_tile_sum_staged<false, true>(Q, T[m], cw, list, lane, read_oct);
+            else sum = dense_tile_sum_staged<false, false>(Q, T[m], cw, list, lane, read_oct);
+        } else if (skip && split) sum = dense_tile_sum<true, true>(Q, T[m], cnt, mine0[m], lane, read_oct);
         else if (skip) sum = dense_tile_sum<true, false>(Q, T[m], cnt, mine0[m], lane, read_oct);
         else if (split) sum = dense_tile_sum<false, true>(Q, T[m], cnt, mine0[m], lane, read_oct);
         else sum = dense_tile_sum<false, false>(Q, T[m], cnt, mine0[m], lane, read_oct);
