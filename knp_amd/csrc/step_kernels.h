@@ -2561,18 +2561,18 @@ __global__ __launch_bounds__(kBlock) void k_dense_spikes(const NetDev *__restric
                 if (i < Dn.n_in && bernoulli_hash(Dn.seed, t, Dn.hash_offset + inst * Dn.n_in + i) < Dn.threshold)
                     v |= 1u << b;
             }
-        Dn.bits[idx] = (uint8_t)v;
+        gp(Dn.bits)[idx] = (uint8_t)v;
         const uint32_t c = __popc(v);
         rows += c;
-        ev += (unsigned long long)c * Dn.inst_post_count[inst];
+        ev += (unsigned long long)c * gp(Dn.inst_post_count)[inst];
     }
     for (int off = 32; off > 0; off >>= 1) {
         ev += __shfl_xor(ev, off);
         rows += __shfl_xor(rows, off);
     }
     if (lane_id() == 0 && rows) {
-        atomicAdd(&net->events[0], ev);
-        atomicAdd(&net->events[1], rows);
+        atomicAdd((unsigned long long *)gp(net->events), ev);
+        atomicAdd((unsigned long long *)gp(net->events) + 1, rows);
     }
 }
 
@@ -2615,7 +2615,7 @@ __global__ __launch_bounds__(kBlock) void k_dense_mfma(const NetDev *__restrict_
     for (uint64_t u = (uint64_t)blockIdx.x * (kBlock / 64) + threadIdx.x / 64; u < units; u += waves) {
         const uint32_t tile = (uint32_t)(u / n_grp), rb0 = (uint32_t)(u % n_grp) * G;
         const uint32_t nb = min(G, n_rb - rb0);
-        const uint32_t inst = Dn.tile_inst[tile], j0 = Dn.tile_j0[tile], ncol = Dn.tile_ncol[tile];
+        const uint32_t inst = gp(Dn.tile_inst)[tile], j0 = gp(Dn.tile_j0)[tile], ncol = gp(Dn.tile_ncol)[tile];
         const KNP_GLOBAL uint8_t *fr = gp(Dn.frag) + (size_t)tile * Dn.ksteps * 64 * FB + (size_t)lane * FB;
         const KNP_GLOBAL uint8_t *brow[G];
 #pragma unroll
@@ -2666,7 +2666,7 @@ __global__ __launch_bounds__(kBlock) void k_dense_mfma(const NetDev *__restrict_
             if (ck + 1 < nck) multiply(ck + 1, c1);
         }
         if (col < ncol) {
-            const float sc = Dn.row_scale[j0 + col];
+            const float sc = gp(Dn.row_scale)[j0 + col];
 #pragma unroll
             for (uint32_t g = 0; g < G; ++g) {
                 if (g >= nb) break;
@@ -2708,7 +2708,7 @@ __global__ __launch_bounds__(kBlock) void k_dense_counts(const NetDev *__restric
             gp(nxt)[k] = 0;
         for (uint32_t v = Q.dn_v0 + blockIdx.x; v < Q.dn_v1; v += gridDim.x) {
             const uint32_t cnt = gp(vc)[v];
-            const uint32_t *list = net->seg_list[net->vseg[v]] + par + 1;
+            const uint32_t *list = net->seg_list[gp(net->vseg)[v]] + par + 1;
             for (uint32_t e = threadIdx.x; e < cnt; e += kBlock) {
                 const uint32_t pre = gp(list)[e];
                 if (pre >= Q.n_pre) continue;
@@ -2906,7 +2906,12 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 4))) 
     uint64_t mine0[kDenseGroupMax];
 #pragma unroll
     for (uint32_t m = 0; m < kDenseGroupMax; ++m)
-        if (m < grp.n) T[m] = net->projs[grp.proj[m]].dn_tile_info[tile];
+        if (m < grp.n) {
+            // one tile entry in two 16-byte loads (cbase, n_oct, npad, post_begin | post_count, j0, base)
+            const KNP_GLOBAL v4u *e = (const KNP_GLOBAL v4u *)gp(net->projs[grp.proj[m]].dn_tile_info) + 2 * tile;
+            const v4u a = e[0], b = e[1];
+            T[m] = DenseTile{a[0], a[1], a[2], a[3], b[0], b[1], (uint64_t)b[2] | (uint64_t)b[3] << 32};
+        }
     if (STAGED) {
         // every count word of the group loaded at once (lanes past an instance's octets load its last one),
         // then staged in LDS
@@ -2968,7 +2973,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 4))) 
         // weight bytes read (32 neurons x 8 inputs x 2 bytes per octet, twice for two-part weights): the
         // traffic of the skipping schedule, counted in the slots the counting workgroups use
         if (skip && lane == 0 && read_oct)
-            atomicAdd(&Q.dn_part[4 * (blockIdx.x % kDenseCountBlocks) + 2],
+            atomicAdd((unsigned long long *)gp(Q.dn_part) + 4 * (blockIdx.x % kDenseCountBlocks) + 2,
                       (unsigned long long)read_oct * (split ? 1024ull : 512ull));
     }
     if (writer) gp(P.acc_v)[ai] = total;
