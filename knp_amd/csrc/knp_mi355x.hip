@@ -145,6 +145,7 @@ struct knp_net {
     uint32_t test_hold_block = 0xFFFFFFFFu, test_spin_limit = kSpinLimit;  // knp_mi355x_set_wait_test
     uint64_t test_hold_until = UINT64_MAX;
     uint32_t train_wg_limit = 16384;  // knp_mi355x_set_train_workgroup_limit
+    bool tiles_shallow = false;       // every tiled population's ring is at most kTileShallowD deep
     uint32_t run_kind = POP_LEAN;  // population kind of the persistent launch
     uint32_t tile_lds = 0;  // dynamic LDS bytes of the tile scatter (largest tiled ring)
     std::vector<uint32_t *> seg_list_host;
@@ -948,7 +949,7 @@ int init_network(knp_net *net) {
         int lds_limit = 0;
         HIP_TRY(hipDeviceGetAttribute(&lds_limit, hipDeviceAttributeMaxSharedMemoryPerBlock, net->device));
         hipFuncAttributes fa{};
-        HIP_TRY(hipFuncGetAttributes(&fa, reinterpret_cast<const void *>(&k_tile_scatter<true, false>)));
+        HIP_TRY(hipFuncGetAttributes(&fa, reinterpret_cast<const void *>(&k_tile_scatter<true, false, false>)));
         const int64_t room = (int64_t)lds_limit - (int64_t)fa.sharedSizeBytes - (int64_t)tile_lds_bytes(0, kTileMaxList);
         tile_max_d = (uint32_t)std::max<int64_t>(0, std::min<int64_t>(kTileMaxD, room / (kBlock * 8)));
     }
@@ -1652,11 +1653,14 @@ int init_network(knp_net *net) {
     net->can_persistent = multi_ok;
     net->can_small = net->small;
     net->wide_tiles = false;
+    net->tiles_shallow = true;
+    for (const auto &H : net->pops)
+        if (H.tiled && H.D > kTileShallowD) net->tiles_shallow = false;
     if (T.n_tiles) {
         int cus = 0, per_cu = 0;
         HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, net->device));
         HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(
-            &per_cu, reinterpret_cast<const void *>(&k_tile_scatter<true, false>), kTileThreads, net->tile_lds));
+            &per_cu, reinterpret_cast<const void *>(&k_tile_scatter<true, false, false>), kTileThreads, net->tile_lds));
         net->wide_tiles = (uint64_t)per_cu * cus < T.n_tiles;
     }
     {
@@ -2046,10 +2050,18 @@ int run_steps(knp_net *net, uint64_t n_steps, uint32_t *counts = nullptr, bool t
         const uint32_t nt = net->table.n_tiles;
         if (nt && (rc = timed_launch(net, KNP_KERNEL_TILE_SCATTER, [&] {
                        const uint32_t lds = net->tile_lds;
-                       if (p1 && net->wide_tiles) k_tile_scatter<true, true><<<nt, kTileThreads, lds, net->stream>>>(D, targ);
-                       else if (p1) k_tile_scatter<true, false><<<nt, kTileThreads, lds, net->stream>>>(D, targ);
-                       else if (net->wide_tiles) k_tile_scatter<false, true><<<nt, kTileThreads, lds, net->stream>>>(D, targ);
-                       else k_tile_scatter<false, false><<<nt, kTileThreads, lds, net->stream>>>(D, targ);
+                       const bool w = net->wide_tiles;
+                       if (net->tiles_shallow) {
+                           if (p1 && w) k_tile_scatter<true, true, true><<<nt, kTileThreads, lds, net->stream>>>(D, targ);
+                           else if (p1) k_tile_scatter<true, false, true><<<nt, kTileThreads, lds, net->stream>>>(D, targ);
+                           else if (w) k_tile_scatter<false, true, true><<<nt, kTileThreads, lds, net->stream>>>(D, targ);
+                           else k_tile_scatter<false, false, true><<<nt, kTileThreads, lds, net->stream>>>(D, targ);
+                       } else {
+                           if (p1 && w) k_tile_scatter<true, true, false><<<nt, kTileThreads, lds, net->stream>>>(D, targ);
+                           else if (p1) k_tile_scatter<true, false, false><<<nt, kTileThreads, lds, net->stream>>>(D, targ);
+                           else if (w) k_tile_scatter<false, true, false><<<nt, kTileThreads, lds, net->stream>>>(D, targ);
+                           else k_tile_scatter<false, false, false><<<nt, kTileThreads, lds, net->stream>>>(D, targ);
+                       }
                    })))
             return rc;
         for (uint32_t q = 0; q < net->projs.size(); ++q) {

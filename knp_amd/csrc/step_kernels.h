@@ -1330,7 +1330,10 @@ __device__ __forceinline__ void tile_flush_counts(const NetDev *__restrict__ net
 // instead of two, at the cost of a few spilled registers; the host picks it when the tiles do not all fit
 // on the device at once (many-instance networks: 129 -> 126 us a step; with one round of tiles it is
 // slower).
-template <bool PHASE1, bool WIDE>
+// SHALLOW: every tiled population's ring is at most kTileShallowD deep (half the registers for the ring cells'
+// old values, which otherwise spill in the 6-wave build).
+constexpr uint32_t kTileShallowD = 16;
+template <bool PHASE1, bool WIDE, bool SHALLOW>
 __global__ __launch_bounds__(kTileThreads, WIDE ? 6 : 1) void k_tile_scatter(const NetDev *__restrict__ net,
                                                                               uint64_t t) {
     t = resolve_step(net, t);
@@ -1342,7 +1345,7 @@ __global__ __launch_bounds__(kTileThreads, WIDE ? 6 : 1) void k_tile_scatter(con
     const uint32_t s1 = PHASE1 ? (uint32_t)((t + 1) % c.D) : 0xFFFFFFFFu;  // slot consumed by phase 1 of t + 1
     // The ring cells of the tile are written by no one else during this kernel: their values are loaded now,
     // overlapping the rest, so that adding this step's sums at the end needs no further round trip.
-    constexpr uint32_t kCells = kTileMaxD * kBlock / kTileThreads;
+    constexpr uint32_t kCells = (SHALLOW ? kTileShallowD : kTileMaxD) * kBlock / kTileThreads;
     int64_t old[kCells];
 #pragma unroll
     for (uint32_t u = 0; u < kCells; ++u) {
