@@ -1494,7 +1494,7 @@ __device__ __forceinline__ void small_append(uint32_t *rows, uint32_t *n_rows, b
 }
 
 struct SmallProj {  // what the record pass needs of a projection, staged in LDS
-    const uint32_t *rowp;  // the projection's row offsets in LDS
+    uint32_t row_base;     // the projection's row offsets in the launch's LDS offset table from here
     const uint64_t *syn8;
     uint32_t n_pre, ring_off, D, n_post, slot0;  // slot0: the step's ring slot of its postsynaptic population
 };
@@ -1504,8 +1504,8 @@ struct SmallProj {  // what the record pass needs of a projection, staged in LDS
 // such loads in flight, and add them with LDS atomics. Adds the impacts to ev. rows: LDS, or a channel list
 // in global memory (then q is q_fixed). No barrier inside: the caller synchronizes.
 template <bool GLOBAL_LIST>
-__device__ __forceinline__ void small_expand(const SmallProj *sp, const uint32_t *list, uint32_t count, uint32_t q_fixed,
-                                             unsigned long long *ring, uint32_t &ev) {
+__device__ __forceinline__ void small_expand(const SmallProj *sp, const uint32_t *rowp, const uint32_t *list,
+                                             uint32_t count, uint32_t q_fixed, unsigned long long *ring, uint32_t &ev) {
     const uint32_t lane = lane_id(), w = threadIdx.x / 64;
     constexpr uint32_t W = kSmallThreads / 64;
     for (uint32_t row = w; row < count; row += W) {
@@ -1514,7 +1514,7 @@ __device__ __forceinline__ void small_expand(const SmallProj *sp, const uint32_t
         const uint32_t r = GLOBAL_LIST ? e : (e & 0xFFFFFFu);
         const SmallProj &P = sp[q];
         if (r >= P.n_pre) continue;
-        const uint32_t b = P.rowp[r], end = P.rowp[r + 1];
+        const uint32_t b = rowp[P.row_base + r], end = rowp[P.row_base + r + 1];
         const KNP_GLOBAL uint64_t *syn8 = gp(P.syn8);
         for (uint32_t k0 = b + lane; k0 < end; k0 += 128) {
             const uint32_t k1 = k0 + 64;
@@ -1551,7 +1551,7 @@ __global__ __launch_bounds__(kSmallThreads) void k_small_run(const NetDev *__res
     if (tid < net->n_projs) {
         const ProjDev &Q = net->projs[tid];
         const PopDev &Pp = net->pops[Q.post_pop];
-        sp[tid] = SmallProj{rowp + L.row_base[tid], Q.syn8, Q.n_pre, L.ring_off[Q.post_pop], Pp.D, Pp.n, 0u};
+        sp[tid] = SmallProj{L.row_base[tid], Q.syn8, Q.n_pre, L.ring_off[Q.post_pop], Pp.D, Pp.n, 0u};
     }
     // the row offsets and single-pattern thresholds into LDS
     for (uint32_t q = 0; q < net->n_projs; ++q) {
@@ -1577,6 +1577,9 @@ __global__ __launch_bounds__(kSmallThreads) void k_small_run(const NetDev *__res
     const PopDev &MP = pops[mp];
     const uint32_t mi = tid - L.pop_off[mp];
     const bool mine = mi < MP.n;
+    // the thread's ring and its population's projections, read from the launch's arguments once
+    const uint32_t my_ring = L.ring_off[mp];
+    const uint64_t my_projs = L.proj_of_pop[mp];
     float V_keep = 0.f;
     uint32_t ls_keep = 0;
     if (KIND == POP_LEAN && mine) {
@@ -1607,7 +1610,7 @@ __global__ __launch_bounds__(kSmallThreads) void k_small_run(const NetDev *__res
         {
             int64_t a = 0;
             if (mine) {
-                unsigned long long &cell = ring[L.ring_off[mp] + my_slot * MP.n + mi];
+                unsigned long long &cell = ring[my_ring + my_slot * MP.n + mi];
                 a = (int64_t)cell;
                 cell = 0ull;
             }
@@ -1618,7 +1621,7 @@ __global__ __launch_bounds__(kSmallThreads) void k_small_run(const NetDev *__res
                 spike = population_update<POP_FULL, true>(net, MP, mi, mine, t, a, cand);
             const uint64_t mask = __ballot(spike);
             if (mi - lane_id() < MP.n) wave_bitmap(mask, mi - lane_id(), MP, t);  // a wave holding neurons
-            uint64_t qs = L.proj_of_pop[mp];
+            uint64_t qs = my_projs;
             while (qs) {  // uniform over the wave: every lane of it belongs to population mp
                 const uint32_t q = (uint32_t)__ffsll((unsigned long long)qs) - 1u;
                 qs &= qs - 1ull;
@@ -1639,7 +1642,7 @@ __global__ __launch_bounds__(kSmallThreads) void k_small_run(const NetDev *__res
         }
         __syncthreads();
         const uint32_t count = *n_rows;
-        small_expand<false>(sp, rows, count, 0, ring, ev);
+        small_expand<false>(sp, rowp, rows, count, 0, ring, ev);
         rows_done += count;
         if (k == 0) {
             // channel messages pushed for t0, then their counts cleared for the steps after the launch
@@ -1647,7 +1650,7 @@ __global__ __launch_bounds__(kSmallThreads) void k_small_run(const NetDev *__res
                 const uint32_t *lst = net->projs[q].in_list + (size_t)(t0 & 1) * net->list_pstride;
                 const uint32_t cnt = gp(lst)[0];
                 if (cnt == 0) continue;
-                small_expand<true>(sp, lst + 1, cnt, q, ring, ev);
+                small_expand<true>(sp, rowp, lst + 1, cnt, q, ring, ev);
                 rows_done += cnt;
             }
             __syncthreads();
