@@ -36,13 +36,14 @@ def need_device():
     assert device_count() >= 1, "no HIP device visible"
 
 
-def dense_blocks_network(o, seed, flags=None):
+def dense_blocks_network(o, seed, flags=None, n_in=N_IN):
     """One default BLIFAT population holding len(SIZES) instances back to back. Per instance: an all-to-all
-    EXCITATORY projection from its own N_IN channels (weights k/256, k in 1..64: exact in BF16) and an
-    all-to-all INHIBITORY_CURRENT lateral projection (weight 1/8), both delay 1; a Bernoulli source (p = 0.3)
-    over all channels, and a pushed channel projection onto instance 0 (to repeat indexes in a message).
+    EXCITATORY projection from its own n_in channels (weights k/256, k in 1..64: exact in BF16) and an
+    all-to-all INHIBITORY_CURRENT lateral projection (weight 1/8), both delay 1; a Bernoulli source (p = 0.3
+    at 72 inputs per instance, proportionally less for more) over all channels, and a pushed channel
+    projection onto instance 0 (to repeat indexes in a message).
     flags: None (the oracle, or the device scatter) or the MFMA expansion flags of the input projection (the
-    lateral one is expanded with KNP_DENSE_SKIP_EMPTY)."""
+    lateral one is expanded with KNP_DENSE_SKIP_EMPTY). n_in: inputs per instance."""
     rng = np.random.default_rng(seed)
     sizes = np.array(SIZES)
     begin = np.concatenate([[0], np.cumsum(sizes)[:-1]])
@@ -50,12 +51,12 @@ def dense_blocks_network(o, seed, flags=None):
     pop = o.add_blifat_population(abi.neuron_array(abi.blifat_defaults(), N))
     pre, post, w = [], [], []
     for k, (b, n) in enumerate(zip(begin, sizes)):
-        jj, ii = np.meshgrid(np.arange(n), np.arange(N_IN), indexing="ij")
+        jj, ii = np.meshgrid(np.arange(n), np.arange(n_in), indexing="ij")
         post.append(b + jj.ravel())
-        pre.append(k * N_IN + ii.ravel())
-        w.append(rng.integers(1, 65, n * N_IN) / 256.0)
+        pre.append(k * n_in + ii.ravel())
+        w.append(rng.integers(1, 65, n * n_in) / 256.0)
     pre, post, w = (np.concatenate(x) for x in (pre, post, w))
-    inp = o.add_delta_projection(abi.KNP_NO_POPULATION, len(SIZES) * N_IN, pop, pre, post, w.astype(np.float32),
+    inp = o.add_delta_projection(abi.KNP_NO_POPULATION, len(SIZES) * n_in, pop, pre, post, w.astype(np.float32),
                                  np.ones(len(pre), np.uint32))
     lp, lq = [], []
     for b, n in zip(begin, sizes):
@@ -65,23 +66,30 @@ def dense_blocks_network(o, seed, flags=None):
     lp, lq = np.concatenate(lp), np.concatenate(lq)
     lat = o.add_delta_projection(pop, N, pop, lp, lq, np.full(len(lp), 0.125, np.float32), np.ones(len(lp), np.uint32),
                                  np.full(len(lp), abi.INHIBITORY_CURRENT, np.uint8))
-    o.add_bernoulli_source(inp, np.full(len(SIZES) * N_IN, workloads.bernoulli_threshold(0.3), np.uint32), seed=seed)
+    # p = 0.3 at N_IN inputs, scaled down for more inputs (the same expected drive)
+    p_in = 0.3 * min(1.0, N_IN / n_in)
+    o.add_bernoulli_source(inp, np.full(len(SIZES) * n_in, workloads.bernoulli_threshold(p_in), np.uint32), seed=seed)
     if flags is not None:
         o.set_projection_dense(inp, abi.DENSE_BF16, flags)
         o.set_projection_dense(lat, abi.DENSE_BF16, abi.DENSE_SKIP_EMPTY)
     return pop, inp
 
 
-@pytest.mark.parametrize("flags", [0, abi.DENSE_SKIP_EMPTY, abi.DENSE_SKIP_EMPTY | abi.DENSE_SPLIT])
-def test_dense_expansion_matches_oracle(flags):
-    say(f"case: projections expanded on MFMA, flags={flags}: uneven instances vs oracle")
+# inputs per instance: 9 octets (the LDS-staged form, one 64-octet half), 75 (staged, both halves), 138 (more
+# octets than the staged form holds: count words loaded chunk by chunk)
+@pytest.mark.parametrize("flags,n_in", [(0, N_IN), (abi.DENSE_SKIP_EMPTY, N_IN),
+                                        (abi.DENSE_SKIP_EMPTY | abi.DENSE_SPLIT, N_IN), (0, 600),
+                                        (abi.DENSE_SKIP_EMPTY | abi.DENSE_SPLIT, 600), (0, 1100),
+                                        (abi.DENSE_SKIP_EMPTY, 1100), (abi.DENSE_SKIP_EMPTY | abi.DENSE_SPLIT, 1100)])
+def test_dense_expansion_matches_oracle(flags, n_in):
+    say(f"case: projections expanded on MFMA, flags={flags}, {n_in} inputs per instance: uneven instances vs oracle")
     steps = 160
     runs = {}
     for name in ("oracle", "gpu"):
         o = Oracle("exact") if name == "oracle" else MI355XBackend(0)
         if name == "gpu":
             o.enable_timing(True)
-        pop, inp = dense_blocks_network(o, 12, flags if name == "gpu" else None)
+        pop, inp = dense_blocks_network(o, 12, flags if name == "gpu" else None, n_in)
         o.record_spikes(pop, steps)
         for t in range(0, steps, 40):
             # a channel message repeating index 3 (its impacts count twice) before some steps
