@@ -595,8 +595,8 @@ def other_workload(args):
                            "cores": threads if key == "cpu_baseline_mt" else 1, "kind": "port",
                            "net_steps_per_s": steps / tx, "sample": f"all {steps} steps, {what}"}
             x.close()
-        line.update(metric="synaptic events/s", value=ev / dt, net_steps_per_s=steps / dt, steps=steps,
-                    events=ev, dtype="fp32", tuning=g.tuned,
+        line.update(metric="synaptic events/s", value=ev / dt, unit="events/s", net_steps_per_s=steps / dt,
+                    steps=steps, ms_per_step=dt / steps * 1e3, events=ev, dtype="fp32", tuning=g.tuned,
                     parity_knp_exact={"spike_trains_identical": bool(np.array_equal(rec, o.read_spike_record(0))),
                                       "events_equal": ev == o.events,
                                       "potential_rel_err_max": pot_error(pot, o.potentials(0))[0]},
